@@ -1,0 +1,123 @@
+/*
+ * ovx_fdg.h -- C ABI of the MI355X (gfx950) flexible-dual-grid library
+ * (libovx_fdg.so).  Plain pointers, sizes and a hipStream_t; no torch types.
+ *
+ * Conventions
+ *   - Every function returns an ovx_status; 0 is success.  On failure
+ *     ovx_last_error() returns a message for the calling thread.
+ *   - Device pointers (d_*) must be on the context's device.  Input buffers
+ *     are borrowed read-only; output buffers are owned by the caller.
+ *   - A context owns a grow-only device workspace.  Calls on one context are
+ *     serialised by an internal lock; use one context per concurrent caller
+ *     to run conversions in parallel.
+ *   - A conversion is two calls: *_begin computes the occupied voxel set and
+ *     returns its size N (one blocking readback); the caller allocates the
+ *     [N,3] outputs and calls *_finish, which enqueues the rest on `stream`
+ *     and returns without waiting.  The input buffers must stay valid until
+ *     *_finish's work has completed on the stream.
+ *
+ * Drop-in boundary: these entry points are what the reference's extension
+ * module binds for this path (src/ext.cpp:18-21), split into begin/finish so
+ * that the caller's allocator owns the outputs.
+ */
+#ifndef OVX_FDG_H
+#define OVX_FDG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int ovx_status;
+#define OVX_OK 0
+#define OVX_ERR_INVALID_ARGUMENT 1 /* bad shape / size / parameter          */
+#define OVX_ERR_INPUT 2            /* face index out of range, non-finite   */
+#define OVX_ERR_CAPACITY 3         /* a count exceeds a supported limit     */
+#define OVX_ERR_HIP 4              /* a HIP runtime call failed             */
+#define OVX_ERR_STATE 5            /* finish without a matching begin, ...  */
+
+typedef struct ovx_fdg_ctx ovx_fdg_ctx;
+
+/* Grid description shared by every entry point.  Mirrors the reference's
+ * `voxel_size` (3 floats) and `grid_range` ([min_x, min_y, min_z, max_x,
+ * max_y, max_z]) arguments (src/convert/api.h:31-39). */
+typedef struct ovx_grid
+{
+    float voxel_size[3];
+    int64_t grid_range[6];
+} ovx_grid;
+
+/* Library version string. */
+const char *ovx_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *ovx_last_error(void);
+
+/* Create / destroy a context bound to HIP device `device`. */
+ovx_status ovx_fdg_create(int device, ovx_fdg_ctx **ctx);
+ovx_status ovx_fdg_destroy(ovx_fdg_ctx *ctx);
+
+/* Bytes of device workspace currently held by the context. */
+int64_t ovx_fdg_workspace_bytes(const ovx_fdg_ctx *ctx);
+
+/* Release the context's workspace (it is re-grown on the next call). */
+ovx_status ovx_fdg_trim(ovx_fdg_ctx *ctx);
+
+/*
+ * mesh_to_flexible_dual_grid, device path.
+ * Replaces o_voxel::fdg::mesh_to_flexible_dual_grid_cuda
+ * (src/convert/mesh_to_flexible_dual_grid_gpu/mesh_to_flexible_dual_grid.cu:797-895;
+ *  declared src/convert/api.h:164-172, bound src/ext.cpp:19) and, run on host
+ * copies, the semantics of mesh_to_flexible_dual_grid_cpu
+ * (src/convert/flexible_dual_grid.cpp:660-992, bound src/ext.cpp:18).
+ *
+ *   d_vertices  float32 [n_vertices, 3], grid-local (already shifted by aabb[0])
+ *   d_faces     int32   [n_faces, 3]
+ *   weights     face / boundary / regularization weights (reference semantics:
+ *               face_weight > 0 enables the unscaled face term, boundary
+ *               terms are scaled by boundary_weight, regularization by
+ *               regularization_weight * count)
+ * begin returns N (number of occupied voxels) in *n_voxels.
+ * finish writes
+ *   d_coords       int32   [N,3]  voxel coordinates, ascending x + Gx*(y + Gy*z)
+ *   d_dual         float32 [N,3]  dual vertices in grid-local units
+ *   d_intersected  uint8   [N,3]  0/1 intersected-edge flags (torch.bool layout)
+ */
+ovx_status ovx_fdg_begin(ovx_fdg_ctx *ctx, const float *d_vertices, int64_t n_vertices, const int32_t *d_faces,
+                         int64_t n_faces, const ovx_grid *grid, float face_weight, float boundary_weight,
+                         float regularization_weight, void *stream, int64_t *n_voxels);
+ovx_status ovx_fdg_finish(ovx_fdg_ctx *ctx, int32_t *d_coords, float *d_dual, uint8_t *d_intersected, void *stream);
+
+/*
+ * intersect_occ, device path: occupancy of a triangle soup.
+ * Replaces o_voxel::fdg::intersect_occ_cuda
+ * (src/convert/mesh_to_flexible_dual_grid_gpu/intersect_qef.cu:896-920;
+ *  declared src/convert/api.h:63-66, bound src/ext.cpp:21).
+ *   d_triangles float32 [n_triangles, 3, 3], grid-local.
+ * finish writes d_coords int32 [N,3] in ascending x + Gx*(y + Gy*z) order.
+ */
+ovx_status ovx_occ_begin(ovx_fdg_ctx *ctx, const float *d_triangles, int64_t n_triangles, const ovx_grid *grid,
+                         void *stream, int64_t *n_voxels);
+ovx_status ovx_occ_finish(ovx_fdg_ctx *ctx, int32_t *d_coords, void *stream);
+
+/*
+ * Per-stage statistics of the last conversion on this context:
+ *   out[0] N voxels          out[1] scanline events     out[2] QEF records
+ *   out[3] face records      out[4] boundary edges      out[5] boundary records
+ *   out[6] workspace bytes   out[7] voxels solved on the slow (giant) path
+ * Reading them synchronises the context's last stream.
+ */
+ovx_status ovx_fdg_stats(ovx_fdg_ctx *ctx, int64_t *out8);
+
+/*
+ * Per-stage device times (ms) of the last conversion, when enabled with
+ * ovx_fdg_set_timing(ctx, 1): [prep+mark, compact, count, emit, fold+solve].
+ */
+ovx_status ovx_fdg_set_timing(ovx_fdg_ctx *ctx, int enable);
+ovx_status ovx_fdg_stage_times(ovx_fdg_ctx *ctx, float *ms5);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OVX_FDG_H */

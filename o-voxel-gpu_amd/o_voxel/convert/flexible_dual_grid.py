@@ -1,0 +1,177 @@
+"""Public flexible-dual-grid conversion API (MI355X build).
+
+Same signatures, grid set-up and error behaviour as the reference module
+``o_voxel/convert/flexible_dual_grid.py``; the work is done by the HIP
+library behind ``o_voxel._C``.
+"""
+
+from typing import Any, List, Optional, Tuple, Union
+
+import numpy as np
+import torch
+
+from .. import _C
+
+FloatArrayLike = Union[float, List[Any], Tuple[Any, ...], np.ndarray, torch.Tensor]
+IntArrayLike = Union[int, List[Any], Tuple[Any, ...], np.ndarray, torch.Tensor]
+ArrayLike = Union[List[Any], Tuple[Any, ...], np.ndarray, torch.Tensor]
+
+__all__ = [
+    "mesh_to_flexible_dual_grid",
+    "intersect_occ",
+]
+
+
+def _host_vector(value, name: str, dtype: torch.dtype, kind: str) -> torch.Tensor:
+    """Scalar / sequence / ndarray / CPU tensor -> CPU tensor of 3 entries
+    (reference: flexible_dual_grid.py:95-127)."""
+    scalar_type = float if dtype == torch.float32 else int
+    if isinstance(value, scalar_type):
+        value = [value, value, value]
+    if isinstance(value, (list, tuple)):
+        value = np.array(value)
+    if isinstance(value, np.ndarray):
+        t = torch.tensor(value, dtype=dtype)
+    else:
+        assert isinstance(value, torch.Tensor), (
+            f"{name} must be a {kind}, list, tuple, np.ndarray, or torch.Tensor, but got {type(value)}"
+        )
+        assert not value.is_cuda, (
+            f"{name} Tensor metadata must be on CPU; pass Python values to avoid implicit CUDA sync"
+        )
+        t = value.to(dtype=dtype).contiguous()
+    assert t.dim() == 1, f"{name} must be a 1D tensor, but got {t.shape}"
+    assert t.size(0) == 3, f"{name} must have 3 elements, but got {t.size(0)}"
+    return t
+
+
+def _grid_setup(vertices: torch.Tensor, voxel_size, grid_size, aabb):
+    """Derive (voxel_size list, grid_range list, aabb_min tensor on device) exactly as
+    the reference wrapper does (flexible_dual_grid.py:88-178)."""
+    assert voxel_size is not None or grid_size is not None, "Either voxel_size or grid_size must be provided"
+    device = vertices.device
+    vs_cpu = gs_cpu = aabb_cpu = None
+    vs_dev = gs_dev = None
+    if voxel_size is not None:
+        vs_cpu = _host_vector(voxel_size, "voxel_size", torch.float32, "float")
+        vs_dev = vs_cpu.to(device=device)
+    if grid_size is not None:
+        gs_cpu = _host_vector(grid_size, "grid_size", torch.int32, "int")
+        gs_dev = gs_cpu.to(device=device)
+    aabb_dev = None
+    if aabb is not None:
+        if isinstance(aabb, (list, tuple)):
+            aabb = np.array(aabb)
+        if isinstance(aabb, np.ndarray):
+            aabb_cpu = torch.tensor(aabb, dtype=torch.float32)
+        else:
+            assert isinstance(aabb, torch.Tensor), (
+                f"aabb must be a list, tuple, np.ndarray, or torch.Tensor, but got {type(aabb)}"
+            )
+            assert not aabb.is_cuda, "aabb Tensor metadata must be on CPU; pass Python values to avoid implicit CUDA sync"
+            aabb_cpu = aabb.to(dtype=torch.float32).contiguous()
+        aabb_dev = aabb_cpu.to(device=device).contiguous()
+        assert aabb_dev.dim() == 2, f"aabb must be a 2D tensor, but got {aabb_dev.shape}"
+        assert aabb_dev.size(0) == 2, f"aabb must have 2 rows, but got {aabb_dev.size(0)}"
+        assert aabb_dev.size(1) == 3, f"aabb must have 3 columns, but got {aabb_dev.size(1)}"
+
+    if aabb_dev is None:
+        # automatic aabb, padded to whole voxels (flexible_dual_grid.py:146-159)
+        lo = vertices.min(dim=0).values
+        hi = vertices.max(dim=0).values
+        if vs_dev is not None:
+            pad = torch.ceil((hi - lo) / vs_dev) * vs_dev - (hi - lo)
+            lo = lo - pad * 0.5
+            hi = hi + pad * 0.5
+        if gs_dev is not None:
+            pad = (hi - lo) / (gs_dev - 1)
+            lo = lo - pad * 0.5
+            hi = hi + pad * 0.5
+        aabb_dev = torch.stack([lo, hi], dim=0).float().contiguous()
+
+    if vs_cpu is None:
+        if aabb_cpu is None:
+            assert not aabb_dev.is_cuda, "CUDA inputs require CPU aabb or explicit voxel_size to avoid implicit CUDA sync"
+            aabb_cpu = aabb_dev.detach()
+        vs_cpu = ((aabb_cpu[1] - aabb_cpu[0]) / gs_cpu).to(dtype=torch.float32).contiguous()
+    if gs_cpu is None:
+        if aabb_cpu is None:
+            assert not aabb_dev.is_cuda, "CUDA inputs require CPU aabb or explicit grid_size to avoid implicit CUDA sync"
+            aabb_cpu = aabb_dev.detach()
+        gs_cpu = ((aabb_cpu[1] - aabb_cpu[0]) / vs_cpu).round().to(dtype=torch.int32).contiguous()
+    voxel_size_arg = [float(x) for x in vs_cpu.tolist()]
+    grid_range_arg = [0, 0, 0] + [int(x) for x in gs_cpu.tolist()]
+    return voxel_size_arg, grid_range_arg, aabb_dev
+
+
+def _check_mesh(vertices, faces):
+    assert isinstance(vertices, torch.Tensor), f"vertices must be a torch.Tensor, but got {type(vertices)}"
+    assert isinstance(faces, torch.Tensor), f"faces must be a torch.Tensor, but got {type(faces)}"
+    assert vertices.dim() == 2, f"vertices must be a 2D tensor, but got {vertices.shape}"
+    assert vertices.size(1) == 3, f"vertices must have 3 columns, but got {vertices.size(1)}"
+    assert faces.dim() == 2, f"faces must be a 2D tensor, but got {faces.shape}"
+    assert faces.size(1) == 3, f"faces must have 3 columns, but got {faces.size(1)}"
+    assert vertices.device == faces.device, "vertices and faces must be on the same device"
+
+
+@torch.no_grad()
+def mesh_to_flexible_dual_grid(
+    vertices: torch.Tensor,
+    faces: torch.Tensor,
+    voxel_size: Optional[FloatArrayLike] = None,
+    grid_size: Optional[IntArrayLike] = None,
+    aabb: Optional[ArrayLike] = None,
+    face_weight: float = 1.0,
+    boundary_weight: float = 1.0,
+    regularization_weight: float = 0.1,
+    timing: bool = False,
+) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+    """Convert a triangle mesh into a sparse flexible dual grid.
+
+    Reference: o_voxel/convert/flexible_dual_grid.py:38-205.  Intersection,
+    face and boundary QEF terms are accumulated for the occupied voxels and
+    each voxel's dual vertex is the constrained QEF minimiser inside it.
+
+    Returns (coords int32 [N,3], dual_vertices float32 [N,3] in grid-local
+    world units, intersected bool [N,3]).  Voxels come in ascending
+    x + Gx*(y + Gy*z) order (the reference CPU path returns first-insertion
+    order and its CUDA path an unspecified order; the sets and the per-voxel
+    values are the same).
+    """
+    _check_mesh(vertices, faces)
+    device = vertices.device
+    vertices = vertices.to(device=device, dtype=torch.float32).contiguous()
+    faces = faces.to(device=device, dtype=torch.int32).contiguous()
+    voxel_size_arg, grid_range_arg, aabb_dev = _grid_setup(vertices, voxel_size, grid_size, aabb)
+    vertices = vertices - aabb_dev[0].reshape(1, 3)
+    if vertices.is_cuda:
+        return _C.mesh_to_flexible_dual_grid_cuda(vertices, faces, voxel_size_arg, grid_range_arg, face_weight,
+                                                  boundary_weight, regularization_weight)
+    return _C.mesh_to_flexible_dual_grid_cpu(vertices, faces, voxel_size_arg, grid_range_arg, face_weight,
+                                             boundary_weight, regularization_weight, timing)
+
+
+@torch.no_grad()
+def intersect_occ(
+    vertices: torch.Tensor,
+    faces: torch.Tensor,
+    voxel_size: Optional[FloatArrayLike] = None,
+    grid_size: Optional[IntArrayLike] = None,
+    aabb: Optional[ArrayLike] = None,
+) -> torch.Tensor:
+    """Voxel coordinates intersected by a triangle mesh (occupancy only).
+
+    Reference: o_voxel/convert/flexible_dual_grid.py:208-329.  Same grid set-up
+    as :func:`mesh_to_flexible_dual_grid`; returns coords int32 [N,3] in
+    ascending x + Gx*(y + Gy*z) order.
+    """
+    _check_mesh(vertices, faces)
+    device = vertices.device
+    vertices = vertices.to(device=device, dtype=torch.float32).contiguous()
+    faces = faces.to(device=device, dtype=torch.int32).contiguous()
+    voxel_size_arg, grid_range_arg, aabb_dev = _grid_setup(vertices, voxel_size, grid_size, aabb)
+    vertices = vertices - aabb_dev[0].reshape(1, 3)
+    triangles = vertices[faces.to(dtype=torch.long)].contiguous()
+    if vertices.is_cuda:
+        return _C.intersect_occ_cuda(triangles, voxel_size_arg, grid_range_arg)
+    return _C.intersect_occ_cpu(triangles, voxel_size_arg, grid_range_arg)

@@ -1,0 +1,167 @@
+"""GPU parity: the HIP path against the CPU oracle on the same inputs.
+
+Bar (per the workload definition): occupancy and intersected flags
+bit-exact; dual vertices RMS <= 2e-4 voxel and max <= 1e-3 voxel against the
+oracle float path after canonical voxel ordering.  The kernels replay the
+oracle's operation order, so the duals are expected to be bit-identical; the
+tests also report the bit-identical fraction.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import meshgen
+
+pytestmark = pytest.mark.gpu
+
+DUAL_RMS_TOL = 2e-4  # voxel units
+DUAL_MAX_TOL = 1e-3  # voxel units
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch.device("cuda", 0)
+
+
+def _C():
+    from o_voxel import _C
+
+    return _C
+
+
+def run_gpu(v, f, vs, gr, fw, bw, rw):
+    dev = _dev()
+    vt = torch.from_numpy(np.ascontiguousarray(v, np.float32)).to(dev)
+    ft = torch.from_numpy(np.ascontiguousarray(f, np.int32)).to(dev)
+    c, d, i = _C().mesh_to_flexible_dual_grid_cuda(vt, ft, vs, gr, fw, bw, rw)
+    torch.cuda.synchronize()
+    return c.cpu().numpy(), d.cpu().numpy(), i.cpu().numpy()
+
+
+def compare(oracle, v, f, vs, gr, fw=1.0, bw=0.2, rw=1e-2, label=""):
+    print(f"[case] {label} F={len(f)} grid={gr}", flush=True)
+    ref = oracle.flexible_dual_grid(v, f, vs, gr, fw, bw, rw)
+    c, d, i = run_gpu(v, f, vs, gr, fw, bw, rw)
+    perm = oracle.canonical_order(ref.coords, gr)
+    assert c.shape == ref.coords.shape, f"{label}: N {c.shape[0]} vs oracle {ref.coords.shape[0]}"
+    assert np.array_equal(c, ref.coords[perm]), f"{label}: occupancy mismatch"
+    assert np.array_equal(i, ref.flags[perm]), f"{label}: intersected flags mismatch"
+    if len(c) == 0:
+        return 0.0, 0.0, 1.0
+    vox = np.asarray(vs, np.float64)[None, :]
+    diff = (d.astype(np.float64) - ref.dual[perm].astype(np.float64)) / vox
+    rms = float(np.sqrt((diff ** 2).sum(1).mean()))
+    mx = float(np.abs(diff).max())
+    exact = float(np.mean(np.all(d == ref.dual[perm], axis=1)))
+    print(f"[case] {label}: N={len(c)} dual rms={rms:.3e} max={mx:.3e} voxel, bit-identical {exact:.6f}", flush=True)
+    assert rms <= DUAL_RMS_TOL and mx <= DUAL_MAX_TOL, (label, rms, mx)
+    return rms, mx, exact
+
+
+def test_ico_cap_grid64(oracle):
+    v, f = meshgen.ico_cap()
+    vs, gr, a0 = meshgen.grid_args(64)
+    rms, mx, exact = compare(oracle, (v - a0).astype(np.float32), f, vs, gr, label="ico_cap/64")
+    assert exact == 1.0
+
+
+@pytest.mark.parametrize("grid", [16, 37, 128])
+def test_ico_cap_other_grids(oracle, grid):
+    v, f = meshgen.ico_cap()
+    vs, gr, a0 = meshgen.grid_args(grid)
+    compare(oracle, (v - a0).astype(np.float32), f, vs, gr, label=f"ico_cap/{grid}")
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_small_scene(oracle, seed):
+    v, f = meshgen.scene(seed=seed, target_faces=20000)
+    vs, gr, a0 = meshgen.grid_args(128)
+    compare(oracle, (v - a0).astype(np.float32), f, vs, gr, label=f"scene20k/s{seed}")
+
+
+def test_weights_variants(oracle):
+    v, f = meshgen.scene(seed=7, target_faces=8000)
+    vs, gr, a0 = meshgen.grid_args(96)
+    vl = (v - a0).astype(np.float32)
+    for fw, bw, rw in [(0.0, 0.0, 0.0), (1.0, 0.0, 0.1), (0.0, 1.0, 0.1), (2.5, 0.7, 0.0), (1.0, 1.0, 1.0)]:
+        compare(oracle, vl, f, vs, gr, fw, bw, rw, label=f"weights fw={fw} bw={bw} rw={rw}")
+
+
+def test_anisotropic_grid_and_offset(oracle):
+    v, f = meshgen.scene(seed=11, target_faces=6000)
+    vl = (v + 0.5).astype(np.float32)
+    vs = [1.0 / 70, 1.0 / 45, 1.0 / 90]
+    gr = [0, 0, 0, 70, 45, 90]
+    compare(oracle, vl, f, vs, gr, label="anisotropic")
+    gr2 = [5, -3, 10, 60, 40, 80]
+    compare(oracle, vl, f, vs, gr2, label="offset grid_min")
+
+
+def test_empty_and_outside(oracle):
+    vs, gr, a0 = meshgen.grid_args(32)
+    v = np.zeros((0, 3), np.float32)
+    f = np.zeros((0, 3), np.int32)
+    c, d, i = run_gpu(v, f, vs, gr, 1.0, 0.2, 1e-2)
+    assert c.shape == (0, 3) and d.shape == (0, 3) and i.shape == (0, 3)
+    # all faces outside the grid
+    v = np.array([[5, 5, 5], [6, 5, 5], [5, 6, 5]], np.float32)
+    f = np.array([[0, 1, 2]], np.int32)
+    c, d, i = run_gpu(v, f, vs, gr, 1.0, 0.2, 1e-2)
+    assert c.shape == (0, 3)
+
+
+def test_degenerate_triangles(oracle):
+    rng = np.random.default_rng(5)
+    pts = rng.uniform(0.1, 0.9, size=(60, 3)).astype(np.float32)
+    faces = []
+    for k in range(20):
+        a, b, c = 3 * k, 3 * k + 1, 3 * k + 2
+        faces.append([a, b, c])
+    # zero-area (collinear / repeated vertex) triangles and a repeated index
+    pts[3:6] = np.array([[0.2, 0.2, 0.2], [0.4, 0.4, 0.4], [0.6, 0.6, 0.6]], np.float32)
+    faces.append([0, 0, 1])
+    faces.append([7, 7, 7])
+    f = np.array(faces, np.int32)
+    vs = [1.0 / 40] * 3
+    gr = [0, 0, 0, 40, 40, 40]
+    compare(oracle, pts, f, vs, gr, label="degenerate")
+
+
+def test_intersect_occ(oracle):
+    v, f = meshgen.scene(seed=3, target_faces=10000)
+    vs, gr, a0 = meshgen.grid_args(100)
+    tri = (v - a0).astype(np.float32)[f]
+    ref = oracle.intersect_occ(tri, vs, gr)
+    dev = _dev()
+    c = _C().intersect_occ_cuda(torch.from_numpy(tri).to(dev), vs, gr).cpu().numpy()
+    perm = oracle.canonical_order(ref, gr)
+    assert np.array_equal(c, ref[perm])
+
+
+def test_public_api_matches_c_boundary(oracle):
+    import o_voxel
+
+    v, f = meshgen.ico_cap()
+    dev = _dev()
+    aabb = [[-0.5] * 3, [0.5] * 3]
+    c, d, i = o_voxel.convert.mesh_to_flexible_dual_grid(torch.from_numpy(v).to(dev), torch.from_numpy(f).to(dev),
+                                                         grid_size=64, aabb=aabb, face_weight=1.0,
+                                                         boundary_weight=0.2, regularization_weight=1e-2)
+    vs, gr, a0 = meshgen.grid_args(64)
+    ref = oracle.flexible_dual_grid((v - a0).astype(np.float32), f.astype(np.int32), vs, gr, 1.0, 0.2, 1e-2)
+    perm = oracle.canonical_order(ref.coords, gr)
+    assert np.array_equal(c.cpu().numpy(), ref.coords[perm])
+    assert np.array_equal(d.cpu().numpy(), ref.dual[perm])
+    assert np.array_equal(i.cpu().numpy(), ref.flags[perm])
+
+
+def test_determinism(oracle):
+    v, f = meshgen.scene(seed=4, target_faces=15000)
+    vs, gr, a0 = meshgen.grid_args(128)
+    vl = (v - a0).astype(np.float32)
+    a = run_gpu(vl, f, vs, gr, 1.0, 0.2, 1e-2)
+    b = run_gpu(vl, f, vs, gr, 1.0, 0.2, 1e-2)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
