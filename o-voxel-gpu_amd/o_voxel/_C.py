@@ -38,7 +38,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.environ.get("OVX_FDG_LIB", os.path.join(_HERE, "_lib", "libovx_fdg.so"))
 
 _lib = None
-_lib_lock = threading.Lock()
+_lib_lock = threading.Lock()   # guards loading the shared library
+_ctx_lock = threading.Lock()   # guards creating per-device contexts
 _ctxs = {}
 
 
@@ -115,7 +116,8 @@ def _ctx(device: torch.device) -> _Ctx:
     idx = device.index if device.index is not None else torch.cuda.current_device()
     c = _ctxs.get(idx)
     if c is None:
-        with _lib_lock:
+        _load()  # takes _lib_lock itself; must not run while _ctx_lock is held by a loader
+        with _ctx_lock:
             c = _ctxs.get(idx)
             if c is None:
                 c = _Ctx(idx)
