@@ -1,0 +1,272 @@
+#!/usr/bin/env python3
+"""Benchmark of mesh_to_flexible_dual_grid on one MI355X.
+
+One step = one call of the public drop-in API
+``o_voxel.convert.mesh_to_flexible_dual_grid`` on a synthetic mesh already
+resident in HBM (float32 vertices, int64 faces, as a caller passes them),
+including the wrapper's dtype cast and aabb shift.  Prints ONE JSON line.
+
+  python bench.py [--steps K] [--warmup W] [--workload scene|scene_large|ico_cap|scene_rankdef] [--grid G]
+
+Default workload: the ~280k-face seeded scene (closed icospheres/tori, open
+patches, 1% slivers) at grid 512, weights face 1.0 / boundary 0.2 /
+regularization 1e-2.
+
+Reported next to the throughput:
+  roofline      the dominant kernel's algorithmic bytes per launch over its
+                average launch time (HIP events on the conversion's stream,
+                recorded around every launch of the timed steps) vs 8 TB/s
+  cpu_baseline  the CPU oracle (float path, single thread) on the same
+                workload (a bounded face sample for the 10M-face mesh)
+  parity        occupancy / flag mismatches and dual error against the oracle
+"""
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (os.path.join(ROOT, "o-voxel-gpu_amd"), ROOT):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+METRIC = "M faces/s + ms/mesh at grid 512; batched/FP32-comp vs plain at dual RMS≤2e-4"
+HBM_PEAK_GBS = 8000.0
+WEIGHTS = (1.0, 0.2, 1e-2)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(name, grid):
+    import meshgen
+
+    if name == "ico_cap":
+        v, f = meshgen.ico_cap()
+        g = grid or 64
+        seed = None
+    elif name == "scene":
+        v, f = meshgen.scene(seed=2)
+        g = grid or 512
+        seed = 2
+    elif name == "scene_large":
+        v, f = meshgen.scene_large(seed=3)
+        g = grid or 512
+        seed = 3
+    elif name == "scene_rankdef":
+        v, f = meshgen.scene_rankdef(seed=4)
+        g = grid or 1024
+        seed = 4
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return v, f, g, seed
+
+
+def algorithmic_bytes(slot, st, F, V, n_words):
+    """Compulsory HBM bytes of one launch of a kernel slot."""
+    N, R, E = st["voxels"], st["records"], st["boundary_edges"]
+    table = {
+        # faces (12 B) + vertices (12 B) read once, 16 B plane per face written
+        "prep": 12 * F + 12 * V + 16 * F,
+        # faces + vertices read, occupancy bitset written once
+        "mark": 12 * F + 12 * V + 8 * n_words,
+        # bitset read, word offsets written
+        "word_scan": 8 * n_words + 4 * n_words,
+        # bitset + offsets read, coords written
+        "coords": 12 * n_words + 12 * N,
+        "count_events": 12 * F + 12 * V + 12 * n_words + 4 * N,
+        "count_face": 12 * F + 12 * V + 12 * n_words + 4 * N,
+        "rec_scan": 8 * N,
+        # records written once (8 B key + 4 B z)
+        "emit_events": 12 * F + 12 * V + 12 * n_words + 4 * N + 12 * R,
+        "emit_face": 12 * F + 12 * V + 12 * n_words,
+        # records + segment offsets + coords + planes read, dual + flags written
+        "fold_solve": 12 * R + 4 * (N + 1) + 12 * N + 16 * F + 56 * E + 12 * N + 3 * N,
+    }
+    return table.get(slot)
+
+
+def cpu_baseline(v, f, grid, workload, max_faces=None):
+    """Oracle float path (single thread) on the workload or a bounded face sample."""
+    import meshgen
+    from oracle import oracle as O
+
+    O.build()
+    vs, gr, a0 = meshgen.grid_args(grid)
+    vl = (v - a0).astype(np.float32)
+    fi = f.astype(np.int32)
+    sample = f"full {workload} mesh ({len(f)} faces) at grid {grid}, 1 run"
+    if max_faces is not None and len(fi) > max_faces:
+        rng = np.random.default_rng(0)
+        idx = np.sort(rng.choice(len(fi), size=max_faces, replace=False))
+        fi = fi[idx]
+        sample = f"{max_faces} of {len(f)} faces (seeded sample) of {workload} at grid {grid}, 1 run"
+    t0 = time.perf_counter()
+    ref = O.flexible_dual_grid(vl, fi, vs, gr, *WEIGHTS)
+    dt = time.perf_counter() - t0
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return ref, dt, len(fi), sample, cpu, vs, gr
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="scene")
+    ap.add_argument("--grid", type=int, default=None)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build measures one GPU only (the conversion of one mesh runs on one device); "
+              f"--gpus {args.gpus} / WORLD_SIZE {world} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import o_voxel
+    from o_voxel import _C
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+
+    v, f, grid, seed = make_workload(args.workload, args.grid)
+    F, V = int(f.shape[0]), int(v.shape[0])
+    log(f"[bench] workload={args.workload} F={F} V={V} grid={grid}")
+    vt = torch.from_numpy(v).to(dev)
+    ft = torch.from_numpy(f).to(dev)
+    aabb = [[-0.5, -0.5, -0.5], [0.5, 0.5, 0.5]]
+
+    def step():
+        return o_voxel.convert.mesh_to_flexible_dual_grid(vt, ft, grid_size=grid, aabb=aabb, face_weight=WEIGHTS[0],
+                                                          boundary_weight=WEIGHTS[1],
+                                                          regularization_weight=WEIGHTS[2])
+
+    # cold call (first in the process)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = step()
+    torch.cuda.synchronize()
+    cold_ms = (time.perf_counter() - t0) * 1e3
+    for _ in range(args.warmup):
+        out = step()
+    torch.cuda.synchronize()
+    _C.set_timing(dev, True)
+    torch.cuda.reset_peak_memory_stats(dev)
+
+    # timed region: K steps; per-kernel HIP event times collected after each step
+    per_kernel = {}
+    step_ms = []
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        out = step()
+        kt = _C.kernel_times(dev)  # waits for this step's last kernel
+        step_ms.append((time.perf_counter() - ts) * 1e3)
+        for k, ms in kt.items():
+            per_kernel.setdefault(k, []).append(ms)
+    torch.cuda.synchronize()
+    t_end = time.perf_counter()
+    ms_per_step = (t_end - t_start) * 1e3 / args.steps
+    value = F / (ms_per_step * 1e-3) / 1e6
+    st = _C.last_stats(dev)
+    peak_mb = torch.cuda.max_memory_allocated(dev) / 2 ** 20 + st["workspace_bytes"] / 2 ** 20
+    coords, dual, inter = (t.cpu().numpy() for t in out)
+
+    G = grid
+    n_words = G * G * ((G + 63) // 64)
+    kern_avg = {k: float(np.mean(x)) for k, x in per_kernel.items()}
+    dom = max(kern_avg, key=kern_avg.get)
+    alg = algorithmic_bytes(dom, st, F, V, n_words)
+    achieved = alg / (kern_avg[dom] * 1e-3) / 1e9 if alg else None
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"{dom}_{args.workload}_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    roofline = {
+        "kernel": dom,
+        "bound": "hbm",
+        "achieved": round(achieved, 3) if achieved else None,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 6) if achieved else None,
+        "traffic": traffic,
+        "alg_bytes": int(alg) if alg else None,
+        "avg_ms": round(kern_avg[dom], 4),
+    }
+    # whole-path algorithmic bytes 12V + 12F + 27N over the step time
+    path_alg = 12 * V + 12 * F + 27 * st["voxels"]
+
+    cpu = None
+    parity = None
+    if not args.no_cpu_baseline:
+        max_faces = 300_000 if F > 1_000_000 else None
+        ref, dt, nf, sample, cpu_model, vs, gr = cpu_baseline(v, f, grid, args.workload, max_faces)
+        cpu = {"value": round(nf / dt / 1e6, 4), "unit": "M faces/s", "cores": 1, "kind": "port",
+               "sample": sample, "seconds": round(dt, 3), "cpu": cpu_model, "nproc": os.cpu_count()}
+        if nf == F:
+            from oracle import oracle as O
+
+            perm = O.canonical_order(ref.coords, gr)
+            occ_ok = coords.shape == ref.coords.shape and bool(np.array_equal(coords, ref.coords[perm]))
+            parity = {"occupancy_match": occ_ok}
+            if occ_ok:
+                d = (dual.astype(np.float64) - ref.dual[perm].astype(np.float64)) / np.float64(vs[0])
+                parity.update({
+                    "intersected_mismatch": int((inter != ref.flags[perm]).sum()),
+                    "dual_rms_voxel": float(np.sqrt((d ** 2).sum(1).mean())) if len(d) else 0.0,
+                    "dual_max_voxel": float(np.abs(d).max()) if len(d) else 0.0,
+                    "dual_bit_identical": float(np.mean(np.all(dual == ref.dual[perm], axis=1))) if len(d) else 1.0,
+                    "vs": "oracle float path",
+                })
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "M faces/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}-{F}f@{grid}", "faces": F, "vertices": V, "grid": grid,
+                   "voxels": st["voxels"], "records": st["records"], "seed": seed,
+                   "weights": {"face": WEIGHTS[0], "boundary": WEIGHTS[1], "regularization": WEIGHTS[2]}},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "parity": parity,
+        "cold_ms": round(cold_ms, 3),
+        "p50_ms": round(float(np.percentile(step_ms, 50)), 4),
+        "p99_ms": round(float(np.percentile(step_ms, 99)), 4),
+        "path_alg_GBps": round(path_alg / (ms_per_step * 1e-3) / 1e9, 3),
+        "kernels_ms": {k: round(x, 4) for k, x in sorted(kern_avg.items(), key=lambda kv: -kv[1])},
+        "peak_mem_mb": round(peak_mb, 1),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

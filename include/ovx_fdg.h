@@ -111,10 +111,17 @@ ovx_status ovx_occ_finish(ovx_fdg_ctx *ctx, int32_t *d_coords, void *stream);
 ovx_status ovx_fdg_stats(ovx_fdg_ctx *ctx, int64_t *out8);
 
 /*
- * Per-stage device times (ms) of the last conversion, when enabled with
- * ovx_fdg_set_timing(ctx, 1): [prep+mark, compact, count, emit, fold+solve].
+ * Device timing of the last conversion, recorded with HIP events around each
+ * kernel launch on the caller's stream once enabled with
+ * ovx_fdg_set_timing(ctx, 1).
+ *   ovx_fdg_kernel_times: per timing slot (ovx_fdg_kernel_count() slots, named
+ *     by ovx_fdg_kernel_name), -1 for kernels that did not run.
+ *   ovx_fdg_stage_times: [prep+mark, compact, count, emit, fold+solve].
  */
 ovx_status ovx_fdg_set_timing(ovx_fdg_ctx *ctx, int enable);
+int ovx_fdg_kernel_count(void);
+const char *ovx_fdg_kernel_name(int slot);
+ovx_status ovx_fdg_kernel_times(ovx_fdg_ctx *ctx, float *ms, int n);
 ovx_status ovx_fdg_stage_times(ovx_fdg_ctx *ctx, float *ms5);
 
 #ifdef __cplusplus

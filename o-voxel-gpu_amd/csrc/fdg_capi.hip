@@ -84,10 +84,10 @@ struct ovx_fdg_ctx
     // statistics of the last conversion
     int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     hipStream_t last_stream = nullptr;
-    // timing
+    // per-kernel timing (events around each launch on the caller's stream)
     bool timing = false;
-    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    float stage_ms[5] = {0, 0, 0, 0, 0};
+    hipEvent_t kev[2 * 16] = {};
+    bool kran[16] = {};
 };
 
 namespace
@@ -183,10 +183,40 @@ ovx_status input_error(unsigned long long flags)
     return fail(OVX_ERR_INPUT, "invalid input (flags %llx)", flags);
 }
 
-void record_event(ovx_fdg_ctx *c, int i, hipStream_t s)
+// timing slots, one per kernel (or kernel group) of the pipeline
+enum KSlot
 {
-    if (c->timing && c->ev[i])
-        (void)hipEventRecord(c->ev[i], s);
+    KS_PREP,
+    KS_MARK,
+    KS_WORDSCAN,
+    KS_COORDS,
+    KS_COUNT_SCAN,
+    KS_COUNT_FACE,
+    KS_EDGES,
+    KS_COUNT_BDRY,
+    KS_RECSCAN,
+    KS_EMIT_SCAN,
+    KS_EMIT_FACE,
+    KS_EMIT_BDRY,
+    KS_FOLD,
+    KS_N
+};
+const char *const kslot_names[KS_N] = {"prep",       "mark",       "word_scan", "coords",    "count_events",
+                                       "count_face", "edges",      "count_bdry", "rec_scan", "emit_events",
+                                       "emit_face",  "emit_bdry",  "fold_solve"};
+
+void tic(ovx_fdg_ctx *c, int k, hipStream_t s)
+{
+    if (c->timing)
+        (void)hipEventRecord(c->kev[2 * k], s);
+}
+void toc(ovx_fdg_ctx *c, int k, hipStream_t s)
+{
+    if (c->timing)
+    {
+        (void)hipEventRecord(c->kev[2 * k + 1], s);
+        c->kran[k] = true;
+    }
 }
 
 // common begin: validation, (planes), mark, compaction count
@@ -231,14 +261,20 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int64_t V, const ovx_g
     g.woff = (const uint32_t *)c->woff.p;
     auto *ctr = (unsigned long long *)c->counters.p;
 
-    record_event(c, 0, s);
+    for (bool &r : c->kran)
+        r = false;
+    tic(c, KS_PREP, s);
     HIP_TRY(hipMemsetAsync(c->bits.p, 0, n_words * sizeof(uint64_t), s));
     HIP_TRY(launch_prep(m, V, g, planes ? (float4 *)c->planes.p : nullptr, ctr, s));
+    toc(c, KS_PREP, s);
     RecArgs none{};
+    tic(c, KS_MARK, s);
     HIP_TRY(launch_scan_events(MODE_MARK, m, g, none, ctr, s));
-    record_event(c, 1, s);
+    toc(c, KS_MARK, s);
+    tic(c, KS_WORDSCAN, s);
     HIP_TRY(launch_scan_popc(g.bits, n_words, (uint32_t *)c->woff.p, (uint64_t *)c->tiles.p, ctr + CTR_WORDS_TOTAL,
                              s));
+    toc(c, KS_WORDSCAN, s);
     if ((st = read_counters(c, s)))
         return st;
     if (c->h_ctr[CTR_ERRORS])
@@ -263,8 +299,9 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
     const MeshDev &m = c->mesh;
     GridDev g = c->grid;
     auto *ctr = (unsigned long long *)c->counters.p;
-    record_event(c, 2, s);
+    tic(c, KS_COORDS, s);
     HIP_TRY(launch_emit_coords(g, c->n_words, coords, s));
+    toc(c, KS_COORDS, s);
 
     // ---- per-voxel record counts
     if ((st = ensure(c->cnt, (size_t)(N + 1) * sizeof(uint32_t))) ||
@@ -278,9 +315,15 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
     r.cnt = (uint32_t *)c->cnt.p;
     r.rec_off = (const uint32_t *)c->rec_off.p;
     r.err = ctr + CTR_ERRORS;
+    tic(c, KS_COUNT_SCAN, s);
     HIP_TRY(launch_scan_events(MODE_COUNT, m, g, r, ctr, s));
+    toc(c, KS_COUNT_SCAN, s);
     if (c->fw > 0.0f)
+    {
+        tic(c, KS_COUNT_FACE, s);
         HIP_TRY(launch_face(MODE_COUNT, m, g, r, ctr, s));
+        toc(c, KS_COUNT_FACE, s);
+    }
 
     // ---- boundary edges (edge multiset of FDG.cpp:728-739, kept in std::map order)
     int64_t E = 0;
@@ -298,6 +341,7 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
         auto *nbr = (uint32_t *)c->nbr.p;
         auto *bcnt = (uint32_t *)c->bcnt.p;
         auto *boff = (uint32_t *)c->boff.p;
+        tic(c, KS_EDGES, s);
         HIP_TRY(hipMemsetAsync(vcnt, 0, (size_t)(V + 1) * 4, s));
         HIP_TRY(launch_edge_count(m, vcnt, voff, nbr, 0, s));
         HIP_TRY(launch_scan_u32(vcnt, (uint64_t)V + 1, voff, (uint64_t *)c->tiles.p, ctr + CTR_EDGE_TOTAL, s));
@@ -305,6 +349,7 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
         HIP_TRY(hipMemsetAsync(bcnt, 0, (size_t)(V + 1) * 4, s));
         HIP_TRY(launch_edge_singletons(V, voff, nbr, bcnt, boff, nullptr, 0, s));
         HIP_TRY(launch_scan_u32(bcnt, (uint64_t)V + 1, boff, (uint64_t *)c->tiles.p, ctr + CTR_BEDGE_TOTAL, s));
+        toc(c, KS_EDGES, s);
         if ((st = read_counters(c, s)))
             return st;
         E = (int64_t)c->h_ctr[CTR_BEDGE_TOTAL];
@@ -313,17 +358,20 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
             if ((st = ensure(c->bedge_v, (size_t)E * sizeof(int2))) ||
                 (st = ensure(c->bedges, (size_t)E * sizeof(BEdge))))
                 return st;
+            tic(c, KS_COUNT_BDRY, s);
             HIP_TRY(launch_edge_singletons(V, voff, nbr, bcnt, boff, (int2 *)c->bedge_v.p, 1, s));
             HIP_TRY(launch_bedge_prep(m.v, E, (const int2 *)c->bedge_v.p, (BEdge *)c->bedges.p, s));
             HIP_TRY(launch_bdry(MODE_COUNT, E, (const BEdge *)c->bedges.p, g, r, ctr, s));
+            toc(c, KS_COUNT_BDRY, s);
         }
     }
     c->stats[4] = E;
 
     // ---- record offsets
+    tic(c, KS_RECSCAN, s);
     HIP_TRY(launch_scan_u32(r.cnt, (uint64_t)N + 1, (uint32_t *)c->rec_off.p, (uint64_t *)c->tiles.p,
                             ctr + CTR_REC_TOTAL, s));
-    record_event(c, 3, s);
+    toc(c, KS_RECSCAN, s);
     if ((st = read_counters(c, s)))
         return st;
     if (c->h_ctr[CTR_ERRORS])
@@ -341,12 +389,21 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
     r.zs = (float *)c->zs.p;
 
     // ---- emit records (the counters run back down to zero)
+    tic(c, KS_EMIT_SCAN, s);
     HIP_TRY(launch_scan_events(MODE_EMIT, m, g, r, nullptr, s));
+    toc(c, KS_EMIT_SCAN, s);
     if (c->fw > 0.0f)
+    {
+        tic(c, KS_EMIT_FACE, s);
         HIP_TRY(launch_face(MODE_EMIT, m, g, r, nullptr, s));
+        toc(c, KS_EMIT_FACE, s);
+    }
     if (E > 0)
+    {
+        tic(c, KS_EMIT_BDRY, s);
         HIP_TRY(launch_bdry(MODE_EMIT, E, (const BEdge *)c->bedges.p, g, r, nullptr, s));
-    record_event(c, 4, s);
+        toc(c, KS_EMIT_BDRY, s);
+    }
 
     // ---- ordered fold + solve
     FoldArgs a{};
@@ -368,8 +425,9 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
     a.bw = c->bw;
     a.rw = c->rw;
     a.counters = ctr;
+    tic(c, KS_FOLD, s);
     HIP_TRY(launch_fold_solve(a, s));
-    record_event(c, 5, s);
+    toc(c, KS_FOLD, s);
     return OVX_OK;
 }
 
@@ -418,7 +476,7 @@ ovx_status ovx_fdg_destroy(ovx_fdg_ctx *c)
             release(*b);
         if (c->h_ctr)
             (void)hipHostFree(c->h_ctr);
-        for (auto &e : c->ev)
+        for (auto &e : c->kev)
             if (e)
                 (void)hipEventDestroy(e);
     }
@@ -533,23 +591,47 @@ ovx_status ovx_fdg_set_timing(ovx_fdg_ctx *c, int enable)
     std::lock_guard<std::mutex> lk(c->mu);
     c->timing = enable != 0;
     if (c->timing)
-        for (auto &e : c->ev)
+        for (auto &e : c->kev)
             if (!e)
                 HIP_TRY(hipEventCreate(&e));
     return OVX_OK;
 }
 
-ovx_status ovx_fdg_stage_times(ovx_fdg_ctx *c, float *ms5)
+int ovx_fdg_kernel_count(void) { return KS_N; }
+
+const char *ovx_fdg_kernel_name(int slot) { return (slot >= 0 && slot < KS_N) ? kslot_names[slot] : ""; }
+
+ovx_status ovx_fdg_kernel_times(ovx_fdg_ctx *c, float *ms, int n)
 {
-    if (!c || !ms5)
+    if (!c || !ms)
         return fail(OVX_ERR_INVALID_ARGUMENT, "null argument");
     DeviceGuard dg(c->device);
     std::lock_guard<std::mutex> lk(c->mu);
     if (!c->timing)
         return fail(OVX_ERR_STATE, "timing is not enabled");
-    HIP_TRY(hipEventSynchronize(c->ev[5]));
-    for (int i = 0; i < 5; ++i)
-        HIP_TRY(hipEventElapsedTime(&ms5[i], c->ev[i], c->ev[i + 1]));
+    for (int k = 0; k < n && k < KS_N; ++k)
+    {
+        ms[k] = -1.0f;
+        if (!c->kran[k])
+            continue;
+        HIP_TRY(hipEventSynchronize(c->kev[2 * k + 1]));
+        HIP_TRY(hipEventElapsedTime(&ms[k], c->kev[2 * k], c->kev[2 * k + 1]));
+    }
+    return OVX_OK;
+}
+
+ovx_status ovx_fdg_stage_times(ovx_fdg_ctx *c, float *ms5)
+{
+    float k[KS_N];
+    ovx_status st = ovx_fdg_kernel_times(c, k, KS_N);
+    if (st)
+        return st;
+    auto v = [&](int i) { return k[i] > 0.0f ? k[i] : 0.0f; };
+    ms5[0] = v(KS_PREP) + v(KS_MARK);
+    ms5[1] = v(KS_WORDSCAN) + v(KS_COORDS);
+    ms5[2] = v(KS_COUNT_SCAN) + v(KS_COUNT_FACE) + v(KS_EDGES) + v(KS_COUNT_BDRY) + v(KS_RECSCAN);
+    ms5[3] = v(KS_EMIT_SCAN) + v(KS_EMIT_FACE) + v(KS_EMIT_BDRY);
+    ms5[4] = v(KS_FOLD);
     return OVX_OK;
 }
 

@@ -31,6 +31,7 @@ __all__ = [
     "last_stats",
     "stage_times",
     "set_timing",
+    "kernel_times",
     "workspace_bytes",
 ]
 
@@ -80,9 +81,13 @@ def _load():
         L.ovx_fdg_stats.argtypes = [vp, ctypes.POINTER(i64)]
         L.ovx_fdg_set_timing.argtypes = [vp, ctypes.c_int]
         L.ovx_fdg_stage_times.argtypes = [vp, ctypes.POINTER(f32)]
+        L.ovx_fdg_kernel_count.restype = ctypes.c_int
+        L.ovx_fdg_kernel_name.argtypes = [ctypes.c_int]
+        L.ovx_fdg_kernel_name.restype = ctypes.c_char_p
+        L.ovx_fdg_kernel_times.argtypes = [vp, ctypes.POINTER(f32), ctypes.c_int]
         for name in ("ovx_fdg_create", "ovx_fdg_destroy", "ovx_fdg_trim", "ovx_fdg_begin", "ovx_fdg_finish",
                      "ovx_occ_begin", "ovx_occ_finish", "ovx_fdg_stats", "ovx_fdg_set_timing",
-                     "ovx_fdg_stage_times"):
+                     "ovx_fdg_stage_times", "ovx_fdg_kernel_times"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -295,6 +300,18 @@ def stage_times(device=None) -> List[float]:
     with torch.cuda.device(dev), ctx.lock:
         _check(_load().ovx_fdg_stage_times(ctx.h, out), "ovx_fdg_stage_times")
     return [float(x) for x in out]
+
+
+def kernel_times(device=None) -> dict:
+    """Device ms per kernel slot of the last conversion (timing must be enabled)."""
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    ctx = _ctx(dev)
+    L = _load()
+    n = L.ovx_fdg_kernel_count()
+    out = (ctypes.c_float * n)()
+    with torch.cuda.device(dev), ctx.lock:
+        _check(L.ovx_fdg_kernel_times(ctx.h, out, n), "ovx_fdg_kernel_times")
+    return {L.ovx_fdg_kernel_name(k).decode(): float(out[k]) for k in range(n) if out[k] >= 0.0}
 
 
 def workspace_bytes(device=None) -> int:
