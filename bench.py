@@ -70,24 +70,25 @@ def make_workload(name, grid):
 
 def algorithmic_bytes(slot, st, F, V, n_words):
     """Compulsory HBM bytes of one launch of a kernel slot."""
-    N, R, E = st["voxels"], st["records"], st["boundary_edges"]
+    N, R, E, ev = st["voxels"], st["records"], st["boundary_edges"], st["events"]
+    FR = st["face_records"]
     table = {
         # faces (12 B) + vertices (12 B) read once, 16 B plane per face written
         "prep": 12 * F + 12 * V + 16 * F,
-        # faces + vertices read, occupancy bitset written once
-        "mark": 12 * F + 12 * V + 8 * n_words,
+        # faces + vertices read, occupancy bitset written once, 16 B per crossing recorded
+        "mark": 12 * F + 12 * V + 8 * n_words + 16 * ev,
         # bitset read, word offsets written
         "word_scan": 8 * n_words + 4 * n_words,
+        # half-edges bucketed by min vertex: faces read, 3F neighbour slots written + read
+        "edges": 12 * F + 24 * F + 16 * V,
         # bitset + offsets read, coords written
         "coords": 12 * n_words + 12 * N,
-        "count_events": 12 * F + 12 * V + 12 * n_words + 4 * N,
-        "count_face": 12 * F + 12 * V + 12 * n_words + 4 * N,
-        "rec_scan": 8 * N,
-        # records written once (8 B key + 4 B z)
-        "emit_events": 12 * F + 12 * V + 12 * n_words + 4 * N + 12 * R,
-        "emit_face": 12 * F + 12 * V + 12 * n_words,
-        # records + segment offsets + coords + planes read, dual + flags written
-        "fold_solve": 12 * R + 4 * (N + 1) + 12 * N + 16 * F + 56 * E + 12 * N + 3 * N,
+        # crossings read, 4 records of 16 B written per crossing
+        "gen_events": 16 * ev + 64 * ev,
+        # faces + vertices + bitset read, one 16 B record per face hit written
+        "gen_face": 12 * F + 12 * V + 8 * n_words + 16 * FR,
+        # records + planes + boundary quadrics read, voxel ids looked up, dual + flags written
+        "fold_solve": 16 * R + 16 * F + 56 * E + 12 * N + 15 * N,
     }
     return table.get(slot)
 

@@ -111,6 +111,14 @@ ovx_status ovx_occ_finish(ovx_fdg_ctx *ctx, int32_t *d_coords, void *stream);
 ovx_status ovx_fdg_stats(ovx_fdg_ctx *ctx, int64_t *out8);
 
 /*
+ * Capacities (in items) for the next conversion on this context: the
+ * scanline-crossing buffer and the QEF-record buffer (0 keeps the current
+ * sizing).  A conversion that needs more re-runs the affected pass with exact
+ * sizes, so any value is correct; it only trades memory against a second pass.
+ */
+ovx_status ovx_fdg_set_capacity(ovx_fdg_ctx *ctx, int64_t events, int64_t hits);
+
+/*
  * Device timing of the last conversion, recorded with HIP events around each
  * kernel launch on the caller's stream once enabled with
  * ovx_fdg_set_timing(ctx, 1).

@@ -2,6 +2,7 @@
 // pipeline on one HIP device.  See include/ovx_fdg.h for the contract.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -70,8 +71,14 @@ struct ovx_fdg_ctx
     int device = 0;
     std::mutex mu;
     // workspace
-    DevBuf planes, bits, woff, tiles, counters, cnt, rec_off, keys, zs, gperm;
+    DevBuf planes, bits, woff, tiles, counters;
     DevBuf vcnt, voff, nbr, bcnt, boff, bedge_v, bedges;
+    DevBuf events, recs, chunks, heads, choff, bchunks, active, bhits, fhits, shard_ctr;
+    uint64_t per[SB_N] = {0, 0, 0, 0, 0}; // per-shard capacities of the sharded buffers (grow-only)
+    uint64_t force_per[SB_N] = {0, 0, 0, 0, 0}; // one-shot overrides (tests, tuning)
+    double face_per_face = 8.0;           // face records per face seen last time (capacity estimate)
+    uint64_t n_events = 0;                // crossings recorded by the last mark pass
+    int64_t E = 0;           // boundary edges of the last begin
     unsigned long long *h_ctr = nullptr; // pinned mirror of the counters
     // state between begin and finish
     int pending = 0; // 0 none, 1 fdg, 2 occ
@@ -122,9 +129,9 @@ void release(DevBuf &b)
 
 int64_t held_bytes(const ovx_fdg_ctx *c)
 {
-    const DevBuf *all[] = {&c->planes, &c->bits, &c->woff, &c->tiles, &c->counters, &c->cnt, &c->rec_off, &c->keys,
-                           &c->zs, &c->gperm, &c->vcnt, &c->voff, &c->nbr, &c->bcnt, &c->boff, &c->bedge_v,
-                           &c->bedges};
+    const DevBuf *all[] = {&c->planes, &c->bits, &c->woff,  &c->tiles, &c->counters, &c->vcnt,   &c->voff,
+                           &c->nbr,    &c->bcnt, &c->boff,  &c->bedge_v, &c->bedges, &c->events, &c->recs,
+                           &c->chunks, &c->heads, &c->choff, &c->bchunks, &c->active, &c->bhits, &c->fhits, &c->shard_ctr};
     int64_t s = 0;
     for (const DevBuf *b : all)
         s += (int64_t)b->bytes;
@@ -178,8 +185,6 @@ ovx_status input_error(unsigned long long flags)
         return fail(OVX_ERR_INPUT, "a vertex lies more than 2^30 voxels from the grid origin");
     if (flags & ERR_DDA)
         return fail(OVX_ERR_INPUT, "boundary traversal exceeded its step bound (non-finite geometry?)");
-    if (flags & ERR_RECORDS)
-        return fail(OVX_ERR_STATE, "record count and emit passes disagree (internal error)");
     return fail(OVX_ERR_INPUT, "invalid input (flags %llx)", flags);
 }
 
@@ -189,21 +194,17 @@ enum KSlot
     KS_PREP,
     KS_MARK,
     KS_WORDSCAN,
-    KS_COORDS,
-    KS_COUNT_SCAN,
-    KS_COUNT_FACE,
     KS_EDGES,
-    KS_COUNT_BDRY,
-    KS_RECSCAN,
-    KS_EMIT_SCAN,
-    KS_EMIT_FACE,
-    KS_EMIT_BDRY,
+    KS_COORDS,
+    KS_GEN_EVENTS,
+    KS_GEN_FACE,
+    KS_GEN_BDRY,
+    KS_ACTIVE,
     KS_FOLD,
     KS_N
 };
-const char *const kslot_names[KS_N] = {"prep",       "mark",       "word_scan", "coords",    "count_events",
-                                       "count_face", "edges",      "count_bdry", "rec_scan", "emit_events",
-                                       "emit_face",  "emit_bdry",  "fold_solve"};
+const char *const kslot_names[KS_N] = {"prep",      "mark",     "word_scan", "edges",  "coords",
+                                       "gen_events", "gen_face", "gen_bdry",  "active", "fold_solve"};
 
 void tic(ovx_fdg_ctx *c, int k, hipStream_t s)
 {
@@ -217,6 +218,37 @@ void toc(ovx_fdg_ctx *c, int k, hipStream_t s)
         (void)hipEventRecord(c->kev[2 * k + 1], s);
         c->kran[k] = true;
     }
+}
+
+
+// ShardBuf view of a buffer of `per` items per shard
+template <typename T>
+ShardBuf<T> shard_view(ovx_fdg_ctx *c, DevBuf &b, int sb)
+{
+    return ShardBuf<T>{(T *)b.p, c->per[sb], (unsigned long long *)c->shard_ctr.p + (uint64_t)sb * NSHARD * SHARD_STRIDE};
+}
+
+// per-shard capacity for `total` items spread over the shards, with headroom
+uint64_t per_for(uint64_t total) { return (total + total / 4) / NSHARD + 256; }
+
+ovx_status ensure_shard(ovx_fdg_ctx *c, DevBuf &b, int sb, uint64_t want_per, size_t item)
+{
+    if (c->force_per[sb])
+    {
+        want_per = c->force_per[sb];
+        c->force_per[sb] = 0;
+        c->per[sb] = want_per;
+    }
+    else if (want_per > c->per[sb])
+        c->per[sb] = want_per;
+    return ensure(b, (size_t)c->per[sb] * NSHARD * item);
+}
+
+ovx_status clear_shards(ovx_fdg_ctx *c, int sb0, int nsb, hipStream_t s)
+{
+    HIP_TRY(hipMemsetAsync((unsigned long long *)c->shard_ctr.p + (uint64_t)sb0 * NSHARD * SHARD_STRIDE, 0,
+                           (size_t)nsb * NSHARD * SHARD_STRIDE * sizeof(unsigned long long), s));
+    return OVX_OK;
 }
 
 // common begin: validation, (planes), mark, compaction count
@@ -267,14 +299,51 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int64_t V, const ovx_g
     HIP_TRY(hipMemsetAsync(c->bits.p, 0, n_words * sizeof(uint64_t), s));
     HIP_TRY(launch_prep(m, V, g, planes ? (float4 *)c->planes.p : nullptr, ctr, s));
     toc(c, KS_PREP, s);
-    RecArgs none{};
+    // crossings are recorded once, here, when the buffer can hold them
+    if ((st = ensure(c->shard_ctr, (size_t)SB_N * NSHARD * SHARD_STRIDE * sizeof(unsigned long long))))
+        return st;
+    ShardBuf<Event> eb{};
+    if (planes)
+    {
+        if ((st = ensure_shard(c, c->events, SB_EVENTS, per_for(std::max<uint64_t>(1ull << 20, 2ull * (uint64_t)m.F)),
+                               sizeof(Event))) ||
+            (st = clear_shards(c, SB_EVENTS, 1, s)))
+            return st;
+        eb = shard_view<Event>(c, c->events, SB_EVENTS);
+    }
     tic(c, KS_MARK, s);
-    HIP_TRY(launch_scan_events(MODE_MARK, m, g, none, ctr, s));
+    HIP_TRY(launch_scan_events(m, g, ctr, eb, s));
     toc(c, KS_MARK, s);
     tic(c, KS_WORDSCAN, s);
     HIP_TRY(launch_scan_popc(g.bits, n_words, (uint32_t *)c->woff.p, (uint64_t *)c->tiles.p, ctr + CTR_WORDS_TOTAL,
                              s));
     toc(c, KS_WORDSCAN, s);
+    c->E = 0;
+    if (planes && c->bw > 0.0f)
+    {
+        // boundary edges (edge multiset of FDG.cpp:728-739, kept in std::map order):
+        // half-edges bucketed by their smaller vertex, lists sorted, singletons counted
+        if ((uint64_t)m.F * 3ull >= (1ull << 32))
+            return fail(OVX_ERR_CAPACITY, "more than 2^32 half-edges");
+        if ((st = ensure(c->vcnt, (size_t)(V + 1) * 4)) || (st = ensure(c->voff, (size_t)(V + 1) * 4)) ||
+            (st = ensure(c->nbr, (size_t)m.F * 3 * 4)) || (st = ensure(c->bcnt, (size_t)(V + 1) * 4)) ||
+            (st = ensure(c->boff, (size_t)(V + 1) * 4)) ||
+            (st = ensure(c->tiles, (scan_tile_count(n_words) + 2 * scan_tile_count((uint64_t)V + 1) + 64) * 8)))
+            return st;
+        auto *vcnt = (uint32_t *)c->vcnt.p;
+        auto *voff = (uint32_t *)c->voff.p;
+        tic(c, KS_EDGES, s);
+        HIP_TRY(hipMemsetAsync(vcnt, 0, (size_t)(V + 1) * 4, s));
+        HIP_TRY(launch_edge_count(m, vcnt, voff, (uint32_t *)c->nbr.p, 0, s));
+        HIP_TRY(launch_scan_u32(vcnt, (uint64_t)V + 1, voff, (uint64_t *)c->tiles.p, ctr + CTR_EDGE_TOTAL, s));
+        HIP_TRY(launch_edge_count(m, vcnt, voff, (uint32_t *)c->nbr.p, 1, s));
+        HIP_TRY(hipMemsetAsync(c->bcnt.p, 0, (size_t)(V + 1) * 4, s));
+        HIP_TRY(launch_edge_singletons(V, voff, (uint32_t *)c->nbr.p, (uint32_t *)c->bcnt.p, nullptr, nullptr, 0, s));
+        HIP_TRY(launch_scan_u32((uint32_t *)c->bcnt.p, (uint64_t)V + 1, (uint32_t *)c->boff.p,
+                                (uint64_t *)c->tiles.p + scan_tile_count(n_words) + 16, ctr + CTR_BEDGE_TOTAL, s));
+        toc(c, KS_EDGES, s);
+    }
+    HIP_TRY(launch_shard_reduce((unsigned long long *)c->shard_ctr.p, ctr, s));
     if ((st = read_counters(c, s)))
         return st;
     if (c->h_ctr[CTR_ERRORS])
@@ -282,11 +351,40 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int64_t V, const ovx_g
     const unsigned long long N = c->h_ctr[CTR_WORDS_TOTAL];
     if (N >= (1ull << 32) - 1)
         return fail(OVX_ERR_CAPACITY, "%llu occupied voxels exceed the 32-bit voxel index", N);
+    c->n_events = c->h_ctr[CTR_EVENTS];
+    if (planes)
+    {
+        if (c->h_ctr[CTR_MAX + SB_EVENTS] > c->per[SB_EVENTS])
+        {
+            // a shard was too small: grow and record the crossings again
+            // (re-marking the bitset is idempotent)
+            const uint64_t want = c->h_ctr[CTR_MAX + SB_EVENTS] + c->h_ctr[CTR_MAX + SB_EVENTS] / 4 + 256;
+            if ((st = ensure_shard(c, c->events, SB_EVENTS, want, sizeof(Event))) ||
+                (st = clear_shards(c, SB_EVENTS, 1, s)))
+                return st;
+            ShardBuf<Event> eb2 = shard_view<Event>(c, c->events, SB_EVENTS);
+            tic(c, KS_MARK, s);
+            HIP_TRY(launch_scan_events(m, g, nullptr, eb2, s));
+            toc(c, KS_MARK, s);
+        }
+        c->E = (int64_t)c->h_ctr[CTR_BEDGE_TOTAL];
+        if (c->E > 0)
+        {
+            if ((st = ensure(c->bedge_v, (size_t)c->E * sizeof(int2))) ||
+                (st = ensure(c->bedges, (size_t)c->E * sizeof(BEdge))))
+                return st;
+            HIP_TRY(launch_edge_singletons(V, (const uint32_t *)c->voff.p, (uint32_t *)c->nbr.p,
+                                           (uint32_t *)c->bcnt.p, (const uint32_t *)c->boff.p,
+                                           (int2 *)c->bedge_v.p, 1, s));
+            HIP_TRY(launch_bedge_prep(m.v, c->E, (const int2 *)c->bedge_v.p, (BEdge *)c->bedges.p, s));
+        }
+    }
     c->grid = g;
     c->n_words = n_words;
     c->N = (int64_t)N;
     c->stats[0] = (int64_t)N;
-    c->stats[1] = (int64_t)c->h_ctr[CTR_EVENTS];
+    c->stats[1] = (int64_t)c->n_events;
+    c->stats[4] = c->E;
     c->pending = planes ? 1 : 2;
     *n_voxels = (int64_t)N;
     return OVX_OK;
@@ -295,7 +393,6 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int64_t V, const ovx_g
 ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *flags, hipStream_t s)
 {
     ovx_status st;
-    const int64_t N = c->N;
     const MeshDev &m = c->mesh;
     GridDev g = c->grid;
     auto *ctr = (unsigned long long *)c->counters.p;
@@ -303,130 +400,127 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
     HIP_TRY(launch_emit_coords(g, c->n_words, coords, s));
     toc(c, KS_COORDS, s);
 
-    // ---- per-voxel record counts
-    if ((st = ensure(c->cnt, (size_t)(N + 1) * sizeof(uint32_t))) ||
-        (st = ensure(c->rec_off, (size_t)(N + 1) * sizeof(uint32_t))) ||
-        (st = ensure(c->tiles, (scan_tile_count((uint64_t)N + 1) + scan_tile_count(c->n_words) +
-                                scan_tile_count((uint64_t)c->V + 1) + 64) *
-                                   8)))
-        return st;
-    HIP_TRY(hipMemsetAsync(c->cnt.p, 0, (size_t)(N + 1) * sizeof(uint32_t), s));
-    RecArgs r{};
-    r.cnt = (uint32_t *)c->cnt.p;
-    r.rec_off = (const uint32_t *)c->rec_off.p;
-    r.err = ctr + CTR_ERRORS;
-    tic(c, KS_COUNT_SCAN, s);
-    HIP_TRY(launch_scan_events(MODE_COUNT, m, g, r, ctr, s));
-    toc(c, KS_COUNT_SCAN, s);
-    if (c->fw > 0.0f)
+    // ---- records grouped by 8x8x8 brick
+    int nb[3];
+    for (int a = 0; a < 3; ++a)
+        nb[a] = (g.G[a] + 7) / 8;
+    const uint64_t n_bricks = (uint64_t)nb[0] * (uint64_t)nb[1] * (uint64_t)nb[2];
+    if (n_bricks >= (1ull << 32))
+        return fail(OVX_ERR_CAPACITY, "brick grid exceeds 2^32 bricks");
+    // phase-0 records are exactly 4 per crossing; face and boundary records are
+    // estimated from the previous conversion (an overflow is detected and the
+    // generation re-run with exact sizes)
+    const uint64_t face_est = (uint64_t)(c->face_per_face * 1.25 * (double)m.F) + 4096;
+    uint64_t want_rec = per_for(4ull * c->n_events + face_est + 16ull * (uint64_t)c->E);
+    uint64_t want_fhit = per_for(face_est);
+    uint64_t want_bhit = per_for(16ull * (uint64_t)c->E + 4096);
+    const uint64_t lim = 0xffffffffull / NSHARD; // record offsets and chunk indices are 32-bit
+    for (int attempt = 0; attempt < 2; ++attempt)
     {
-        tic(c, KS_COUNT_FACE, s);
-        HIP_TRY(launch_face(MODE_COUNT, m, g, r, ctr, s));
-        toc(c, KS_COUNT_FACE, s);
-    }
-
-    // ---- boundary edges (edge multiset of FDG.cpp:728-739, kept in std::map order)
-    int64_t E = 0;
-    if (c->bw > 0.0f)
-    {
-        const int64_t V = c->V;
-        if ((uint64_t)m.F * 3ull >= (1ull << 32))
-            return fail(OVX_ERR_CAPACITY, "more than 2^32 half-edges");
-        if ((st = ensure(c->vcnt, (size_t)(V + 1) * 4)) || (st = ensure(c->voff, (size_t)(V + 1) * 4)) ||
-            (st = ensure(c->nbr, (size_t)m.F * 3 * 4)) || (st = ensure(c->bcnt, (size_t)(V + 1) * 4)) ||
-            (st = ensure(c->boff, (size_t)(V + 1) * 4)))
+        want_rec = std::min(want_rec, lim);
+        if ((st = ensure_shard(c, c->recs, SB_RECS, want_rec, sizeof(Rec))) ||
+            (st = ensure_shard(c, c->chunks, SB_CHUNKS, std::min(c->per[SB_RECS] / 2 + 256, lim), sizeof(Chunk))) ||
+            (st = ensure_shard(c, c->fhits, SB_FHITS, c->fw > 0.0f ? want_fhit : 1, sizeof(FHit))) ||
+            (st = ensure_shard(c, c->bhits, SB_BHITS, c->E > 0 ? want_bhit : 1, sizeof(BHit))) ||
+            (st = ensure(c->heads, (n_bricks + 1) * 4)))
             return st;
-        auto *vcnt = (uint32_t *)c->vcnt.p;
-        auto *voff = (uint32_t *)c->voff.p;
-        auto *nbr = (uint32_t *)c->nbr.p;
-        auto *bcnt = (uint32_t *)c->bcnt.p;
-        auto *boff = (uint32_t *)c->boff.p;
-        tic(c, KS_EDGES, s);
-        HIP_TRY(hipMemsetAsync(vcnt, 0, (size_t)(V + 1) * 4, s));
-        HIP_TRY(launch_edge_count(m, vcnt, voff, nbr, 0, s));
-        HIP_TRY(launch_scan_u32(vcnt, (uint64_t)V + 1, voff, (uint64_t *)c->tiles.p, ctr + CTR_EDGE_TOTAL, s));
-        HIP_TRY(launch_edge_count(m, vcnt, voff, nbr, 1, s));
-        HIP_TRY(hipMemsetAsync(bcnt, 0, (size_t)(V + 1) * 4, s));
-        HIP_TRY(launch_edge_singletons(V, voff, nbr, bcnt, boff, nullptr, 0, s));
-        HIP_TRY(launch_scan_u32(bcnt, (uint64_t)V + 1, boff, (uint64_t *)c->tiles.p, ctr + CTR_BEDGE_TOTAL, s));
-        toc(c, KS_EDGES, s);
+        if ((st = clear_shards(c, SB_FHITS, SB_N - SB_FHITS, s)))
+            return st;
+        HIP_TRY(hipMemsetAsync(c->heads.p, 0, (n_bricks + 1) * 4, s));
+        HIP_TRY(hipMemsetAsync(ctr + CTR_REC_FACE, 0, 2 * sizeof(unsigned long long), s)); // FACE, BOUNDARY
+        HIP_TRY(hipMemsetAsync(ctr + CTR_ACTIVE, 0, sizeof(unsigned long long), s));
+        BucketArgs ba{};
+        ba.recs = shard_view<Rec>(c, c->recs, SB_RECS);
+        ba.chunks = shard_view<Chunk>(c, c->chunks, SB_CHUNKS);
+        ba.brick_nch = (uint32_t *)c->heads.p;
+        for (int a = 0; a < 3; ++a)
+            ba.nb[a] = nb[a];
+        ba.ctr = ctr;
+        tic(c, KS_GEN_EVENTS, s);
+        HIP_TRY(launch_gen_events(shard_view<Event>(c, c->events, SB_EVENTS), g, ba, s));
+        toc(c, KS_GEN_EVENTS, s);
+        if (c->fw > 0.0f)
+        {
+            tic(c, KS_GEN_FACE, s);
+            const ShardBuf<FHit> fb = shard_view<FHit>(c, c->fhits, SB_FHITS);
+            HIP_TRY(launch_face(m, g, fb, ctr, s));
+            HIP_TRY(launch_gen_fhits(fb, g, ba, s));
+            toc(c, KS_GEN_FACE, s);
+        }
+        if (c->E > 0)
+        {
+            tic(c, KS_GEN_BDRY, s);
+            const ShardBuf<BHit> hb = shard_view<BHit>(c, c->bhits, SB_BHITS);
+            HIP_TRY(launch_bdry(c->E, (const BEdge *)c->bedges.p, g, hb, ctr, s));
+            HIP_TRY(launch_gen_bhits(hb, g, ba, s));
+            toc(c, KS_GEN_BDRY, s);
+        }
+        HIP_TRY(launch_shard_reduce((unsigned long long *)c->shard_ctr.p, ctr, s));
         if ((st = read_counters(c, s)))
             return st;
-        E = (int64_t)c->h_ctr[CTR_BEDGE_TOTAL];
-        if (E > 0)
+        if (c->h_ctr[CTR_ERRORS])
+            return input_error(c->h_ctr[CTR_ERRORS]);
+        const unsigned long long *mx = c->h_ctr + CTR_MAX, *sum = c->h_ctr + CTR_SUM;
+        const bool hits_over = mx[SB_FHITS] > c->per[SB_FHITS] || mx[SB_BHITS] > c->per[SB_BHITS];
+        const bool over = hits_over || mx[SB_RECS] > c->per[SB_RECS] || mx[SB_CHUNKS] > c->per[SB_CHUNKS];
+        c->face_per_face = m.F ? (double)sum[SB_FHITS] / (double)m.F : 8.0;
+        if (!over)
         {
-            if ((st = ensure(c->bedge_v, (size_t)E * sizeof(int2))) ||
-                (st = ensure(c->bedges, (size_t)E * sizeof(BEdge))))
-                return st;
-            tic(c, KS_COUNT_BDRY, s);
-            HIP_TRY(launch_edge_singletons(V, voff, nbr, bcnt, boff, (int2 *)c->bedge_v.p, 1, s));
-            HIP_TRY(launch_bedge_prep(m.v, E, (const int2 *)c->bedge_v.p, (BEdge *)c->bedges.p, s));
-            HIP_TRY(launch_bdry(MODE_COUNT, E, (const BEdge *)c->bedges.p, g, r, ctr, s));
-            toc(c, KS_COUNT_BDRY, s);
+            c->stats[2] = (int64_t)sum[SB_RECS];
+            c->stats[3] = (int64_t)c->h_ctr[CTR_REC_FACE];
+            c->stats[5] = (int64_t)c->h_ctr[CTR_REC_BOUNDARY];
+            break;
         }
+        if (sum[SB_RECS] >= 0xffffffffull || sum[SB_CHUNKS] >= 0xffffffffull)
+            return fail(OVX_ERR_CAPACITY, "%llu QEF records / %llu chunks exceed the 32-bit record index",
+                        (unsigned long long)sum[SB_RECS], (unsigned long long)sum[SB_CHUNKS]);
+        if (attempt == 1)
+            return fail(OVX_ERR_STATE, "record buffers overflowed twice (internal error)");
+        // sizes are known now: grow and generate again
+        want_fhit = mx[SB_FHITS] + mx[SB_FHITS] / 4 + 256;
+        want_bhit = mx[SB_BHITS] + mx[SB_BHITS] / 4 + 256;
+        want_rec = mx[SB_RECS] + mx[SB_RECS] / 4 + 256;
+        if (hits_over) // records of dropped hits were never requested: size from all the hits
+            want_rec = std::max(want_rec, per_for(4ull * c->n_events + sum[SB_FHITS] + sum[SB_BHITS]) * 2);
+        // a region's chunks never outnumber its records (both are claimed by the same block)
+        if (hits_over)
+            c->per[SB_CHUNKS] = std::max<uint64_t>(c->per[SB_CHUNKS], std::min(want_rec, lim));
+        else if (mx[SB_CHUNKS] > c->per[SB_CHUNKS])
+            c->per[SB_CHUNKS] = std::min<uint64_t>(mx[SB_CHUNKS] + mx[SB_CHUNKS] / 4 + 256, lim);
     }
-    c->stats[4] = E;
 
-    // ---- record offsets
-    tic(c, KS_RECSCAN, s);
-    HIP_TRY(launch_scan_u32(r.cnt, (uint64_t)N + 1, (uint32_t *)c->rec_off.p, (uint64_t *)c->tiles.p,
-                            ctr + CTR_REC_TOTAL, s));
-    toc(c, KS_RECSCAN, s);
-    if ((st = read_counters(c, s)))
+    // ---- chunks grouped by brick, bricks with records, then the per-brick ordered fold + solve
+    const uint64_t n_chunks = c->h_ctr[CTR_SUM + SB_CHUNKS];
+    if ((st = ensure(c->choff, (n_bricks + 1) * 4)) || (st = ensure(c->bchunks, (n_chunks + 1) * 4)) ||
+        (st = ensure(c->active, n_bricks * 4)) ||
+        (st = ensure(c->tiles, (scan_tile_count(n_bricks + 1) + 64) * 8)))
         return st;
-    if (c->h_ctr[CTR_ERRORS])
-        return input_error(c->h_ctr[CTR_ERRORS]);
-    const unsigned long long R = c->h_ctr[CTR_REC_TOTAL];
-    if (R >= (1ull << 32))
-        return fail(OVX_ERR_CAPACITY, "%llu QEF records exceed the 32-bit record index", R);
-    c->stats[2] = (int64_t)R;
-    c->stats[3] = (int64_t)c->h_ctr[CTR_REC_FACE];
-    c->stats[5] = (int64_t)c->h_ctr[CTR_REC_BOUNDARY];
-    if ((st = ensure(c->keys, (size_t)R * 8)) || (st = ensure(c->zs, (size_t)R * 4)) ||
-        (st = ensure(c->gperm, (size_t)R * 4)))
-        return st;
-    r.keys = (uint64_t *)c->keys.p;
-    r.zs = (float *)c->zs.p;
-
-    // ---- emit records (the counters run back down to zero)
-    tic(c, KS_EMIT_SCAN, s);
-    HIP_TRY(launch_scan_events(MODE_EMIT, m, g, r, nullptr, s));
-    toc(c, KS_EMIT_SCAN, s);
-    if (c->fw > 0.0f)
-    {
-        tic(c, KS_EMIT_FACE, s);
-        HIP_TRY(launch_face(MODE_EMIT, m, g, r, nullptr, s));
-        toc(c, KS_EMIT_FACE, s);
-    }
-    if (E > 0)
-    {
-        tic(c, KS_EMIT_BDRY, s);
-        HIP_TRY(launch_bdry(MODE_EMIT, E, (const BEdge *)c->bedges.p, g, r, nullptr, s));
-        toc(c, KS_EMIT_BDRY, s);
-    }
-
-    // ---- ordered fold + solve
+    tic(c, KS_ACTIVE, s);
+    HIP_TRY(launch_scan_u32((const uint32_t *)c->heads.p, n_bricks + 1, (uint32_t *)c->choff.p, (uint64_t *)c->tiles.p,
+                            ctr + CTR_CHUNK_SCAN, s));
+    HIP_TRY(launch_place_chunks(shard_view<Chunk>(c, c->chunks, SB_CHUNKS), (const uint32_t *)c->choff.p,
+                                (uint32_t *)c->bchunks.p, s));
+    HIP_TRY(launch_active_bricks((const uint32_t *)c->heads.p, n_bricks, (uint32_t *)c->active.p, ctr + CTR_ACTIVE, s));
+    toc(c, KS_ACTIVE, s);
     FoldArgs a{};
-    a.N = (uint64_t)N;
-    a.rec_off = (const uint32_t *)c->rec_off.p;
-    a.keys = (const uint64_t *)c->keys.p;
-    a.zs = (const float *)c->zs.p;
-    a.gperm = (uint32_t *)c->gperm.p;
+    a.choff = (const uint32_t *)c->choff.p;
+    a.bchunks = (const uint32_t *)c->bchunks.p;
+    a.chunks = (const Chunk *)c->chunks.p;
+    a.recs = (const Rec *)c->recs.p;
+    a.active = (const uint32_t *)c->active.p;
+    a.n_active = ctr + CTR_ACTIVE;
+    for (int i = 0; i < 3; ++i)
+        a.nb[i] = nb[i];
+    a.g = g;
     a.planes = (const float4 *)c->planes.p;
     a.bedges = (const BEdge *)c->bedges.p;
-    a.coords = coords;
     a.dual = dual;
     a.flags = flags;
-    for (int i = 0; i < 3; ++i)
-    {
-        a.vs[i] = g.vs[i];
-        a.gmin[i] = g.gmin[i];
-    }
     a.bw = c->bw;
     a.rw = c->rw;
-    a.counters = ctr;
+    a.ctr = ctr;
     tic(c, KS_FOLD, s);
-    HIP_TRY(launch_fold_solve(a, s));
+    HIP_TRY(launch_fold_bricks(a, n_bricks, s));
     toc(c, KS_FOLD, s);
     return OVX_OK;
 }
@@ -469,9 +563,9 @@ ovx_status ovx_fdg_destroy(ovx_fdg_ctx *c)
         std::lock_guard<std::mutex> lk(c->mu);
         if (c->last_stream)
             (void)hipStreamSynchronize(c->last_stream);
-        DevBuf *all[] = {&c->planes, &c->bits, &c->woff, &c->tiles, &c->counters, &c->cnt, &c->rec_off, &c->keys,
-                         &c->zs, &c->gperm, &c->vcnt, &c->voff, &c->nbr, &c->bcnt, &c->boff, &c->bedge_v,
-                         &c->bedges};
+        DevBuf *all[] = {&c->planes, &c->bits, &c->woff,  &c->tiles, &c->counters, &c->vcnt,   &c->voff,
+                         &c->nbr,    &c->bcnt, &c->boff,  &c->bedge_v, &c->bedges, &c->events, &c->recs,
+                         &c->chunks, &c->heads, &c->choff, &c->bchunks, &c->active, &c->bhits, &c->fhits, &c->shard_ctr};
         for (DevBuf *b : all)
             release(*b);
         if (c->h_ctr)
@@ -494,10 +588,13 @@ ovx_status ovx_fdg_trim(ovx_fdg_ctx *c)
     std::lock_guard<std::mutex> lk(c->mu);
     if (c->last_stream)
         HIP_TRY(hipStreamSynchronize(c->last_stream));
-    DevBuf *all[] = {&c->planes, &c->bits, &c->woff, &c->tiles, &c->cnt, &c->rec_off, &c->keys, &c->zs, &c->gperm,
-                     &c->vcnt, &c->voff, &c->nbr, &c->bcnt, &c->boff, &c->bedge_v, &c->bedges};
+    DevBuf *all[] = {&c->planes, &c->bits, &c->woff,  &c->tiles,   &c->vcnt,   &c->voff,   &c->nbr,
+                     &c->bcnt,   &c->boff, &c->bedge_v, &c->bedges, &c->events, &c->recs, &c->chunks,
+                     &c->heads,  &c->choff, &c->bchunks, &c->active, &c->bhits, &c->fhits, &c->shard_ctr};
     for (DevBuf *b : all)
         release(*b);
+    for (int i = 0; i < SB_N; ++i)
+        c->per[i] = 0;
     c->pending = 0;
     return OVX_OK;
 }
@@ -597,6 +694,35 @@ ovx_status ovx_fdg_set_timing(ovx_fdg_ctx *c, int enable)
     return OVX_OK;
 }
 
+ovx_status ovx_fdg_set_capacity(ovx_fdg_ctx *c, int64_t events, int64_t hits)
+{
+    if (!c || events < 0 || hits < 0)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "invalid capacity arguments");
+    DeviceGuard dg(c->device);
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (c->last_stream)
+        HIP_TRY(hipStreamSynchronize(c->last_stream));
+    // capacities are per shard; at least one item per shard
+    if (events)
+    {
+        release(c->events);
+        c->per[SB_EVENTS] = 0;
+        c->force_per[SB_EVENTS] = std::max<uint64_t>(1, (uint64_t)events / NSHARD);
+    }
+    if (hits)
+    {
+        for (int sb : {SB_RECS, SB_CHUNKS, SB_FHITS})
+        {
+            c->per[sb] = 0;
+            c->force_per[sb] = std::max<uint64_t>(1, (uint64_t)hits / NSHARD);
+        }
+        release(c->recs);
+        release(c->chunks);
+        release(c->fhits);
+    }
+    return OVX_OK;
+}
+
 int ovx_fdg_kernel_count(void) { return KS_N; }
 
 const char *ovx_fdg_kernel_name(int slot) { return (slot >= 0 && slot < KS_N) ? kslot_names[slot] : ""; }
@@ -629,9 +755,9 @@ ovx_status ovx_fdg_stage_times(ovx_fdg_ctx *c, float *ms5)
     auto v = [&](int i) { return k[i] > 0.0f ? k[i] : 0.0f; };
     ms5[0] = v(KS_PREP) + v(KS_MARK);
     ms5[1] = v(KS_WORDSCAN) + v(KS_COORDS);
-    ms5[2] = v(KS_COUNT_SCAN) + v(KS_COUNT_FACE) + v(KS_EDGES) + v(KS_COUNT_BDRY) + v(KS_RECSCAN);
-    ms5[3] = v(KS_EMIT_SCAN) + v(KS_EMIT_FACE) + v(KS_EMIT_BDRY);
-    ms5[4] = v(KS_FOLD);
+    ms5[2] = v(KS_EDGES);
+    ms5[3] = v(KS_GEN_EVENTS) + v(KS_GEN_FACE) + v(KS_GEN_BDRY);
+    ms5[4] = v(KS_ACTIVE) + v(KS_FOLD);
     return OVX_OK;
 }
 

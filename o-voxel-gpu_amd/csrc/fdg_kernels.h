@@ -9,27 +9,48 @@
 namespace ovx
 {
 
-enum
-{
-    MODE_MARK = 0,
-    MODE_COUNT = 1,
-    MODE_EMIT = 2
-};
-
 // device counters (unsigned long long each)
 enum
 {
     CTR_EVENTS = 0,
     CTR_ERRORS = 1,
     CTR_WORDS_TOTAL = 2, // total of the bitset popcount scan (= N)
-    CTR_REC_TOTAL = 3,   // total of the record-count scan (= R)
-    CTR_REC_INTERSECT = 4,
-    CTR_REC_FACE = 5,
-    CTR_REC_BOUNDARY = 6,
-    CTR_GIANT = 7,
-    CTR_EDGE_TOTAL = 8, // half-edge count scan total
-    CTR_BEDGE_TOTAL = 9, // boundary edges
-    CTR_COUNT = 16
+    CTR_REC_FACE = 3,    // face-term records
+    CTR_REC_BOUNDARY = 4,
+    CTR_GIANT = 5,
+    CTR_EDGE_TOTAL = 6,  // half-edge count scan total
+    CTR_BEDGE_TOTAL = 7, // boundary edges
+    CTR_ACTIVE = 8,      // bricks with records
+    CTR_CHUNK_SCAN = 9,  // total of the per-brick chunk-count scan
+    // sum and largest shard of each sharded buffer (filled by k_shard_reduce)
+    CTR_SUM = 16,        // CTR_SUM + SB_*  : total items requested
+    CTR_MAX = 24,        // CTR_MAX + SB_*  : largest per-shard request
+    CTR_COUNT = 32
+};
+
+// sharded buffers
+enum
+{
+    SB_EVENTS = 0,
+    SB_FHITS = 1,
+    SB_BHITS = 2,
+    SB_RECS = 3,
+    SB_CHUNKS = 4,
+    SB_N = 5
+};
+
+// Append / allocation counters are split over NSHARD counters 128 bytes
+// apart: a single global counter serialises every wave of the GPU on one
+// address.  A buffer is NSHARD equal regions, region i filled through counter i.
+constexpr int NSHARD = 256;
+constexpr int SHARD_STRIDE = 16;
+
+template <typename T>
+struct ShardBuf
+{
+    T *items;
+    uint64_t per;            // capacity of one region
+    unsigned long long *cnt; // [NSHARD * SHARD_STRIDE] region fill counters
 };
 
 // error flags (CTR_ERRORS)
@@ -38,19 +59,61 @@ enum
     ERR_FACE_INDEX = 1,
     ERR_NONFINITE = 2,
     ERR_RANGE = 4,
-    ERR_DDA = 8,
-    ERR_RECORDS = 16
+    ERR_DDA = 8
 };
 
-struct RecArgs
+// One scanline crossing, recorded once by the mark pass:
+//   key  = (face*3 + axis) << 28 | ly << 15 | lx << 2  (the phase-0 key without dx, dy)
+//   z    = FP32 crossing coordinate along the axis, zi = its voxel index
+struct Event
 {
-    uint32_t *cnt;           // [N] per-voxel record counters
-    const uint32_t *rec_off; // [N+1] record segment offsets
-    uint64_t *keys;          // [R] order keys
-    float *zs;               // [R] intersection z (phase 0 records)
-    unsigned long long *err; // error flags word
+    uint64_t key;
+    float z;
+    int zi;
 };
 
+// one QEF contribution: order key, crossing z (phase 0), voxel index in its brick
+struct Rec
+{
+    uint64_t key;
+    float z;
+    uint32_t lvox;
+};
+
+// contiguous run of one brick's records written by one block; `rank` is the
+// chunk's position among its brick's chunks
+struct Chunk
+{
+    uint32_t start;
+    uint32_t count;
+    uint32_t brick;
+    uint32_t rank;
+};
+
+// face-test hit: face index and voxel (grid-local coordinates)
+struct FHit
+{
+    uint32_t face;
+    uint16_t lx, ly, lz, pad;
+};
+
+// boundary DDA hit: boundary-edge rank and voxel
+struct BHit
+{
+    uint32_t edge;
+    int x, y, z;
+};
+
+struct BucketArgs
+{
+    ShardBuf<Rec> recs;
+    ShardBuf<Chunk> chunks;
+    uint32_t *brick_nch;     // [bricks] chunks per brick
+    int nb[3];               // brick grid (8x8x8 voxels per brick)
+    unsigned long long *ctr; // device counters
+};
+
+// boundary segment: quadric (FDG.cpp:433-453) and DDA set-up
 struct BEdge
 {
     double dir[3];
@@ -62,34 +125,43 @@ struct BEdge
 
 struct FoldArgs
 {
-    uint64_t N;
-    const uint32_t *rec_off;
-    const uint64_t *keys;
-    const float *zs;
-    uint32_t *gperm;
+    const uint32_t *choff;              // [bricks+1] first chunk of each brick in `bchunks`
+    const uint32_t *bchunks;            // chunk indices grouped by brick
+    const Chunk *chunks;
+    const Rec *recs;
+    const uint32_t *active;             // bricks with records
+    const unsigned long long *n_active; // their count (device)
+    int nb[3];
+    GridDev g; // bitset + voxel ids, grid
     const float4 *planes;
     const BEdge *bedges;
-    const int32_t *coords;
     float *dual;
     uint8_t *flags;
-    float vs[3];
-    int gmin[3];
     float bw, rw;
-    unsigned long long *counters;
+    unsigned long long *ctr;
 };
 
 hipError_t launch_prep(const MeshDev &m, int64_t V, const GridDev &g, float4 *planes, unsigned long long *ctr,
                        hipStream_t s);
-hipError_t launch_scan_events(int mode, const MeshDev &m, const GridDev &g, const RecArgs &r,
-                              unsigned long long *ctr, hipStream_t s);
-hipError_t launch_face(int mode, const MeshDev &m, const GridDev &g, const RecArgs &r, unsigned long long *ctr,
+hipError_t launch_scan_events(const MeshDev &m, const GridDev &g, unsigned long long *ctr, const ShardBuf<Event> &eb,
+                              hipStream_t s);
+hipError_t launch_face(const MeshDev &m, const GridDev &g, const ShardBuf<FHit> &hb, unsigned long long *ctr,
                        hipStream_t s);
+hipError_t launch_gen_events(const ShardBuf<Event> &ev, const GridDev &g, const BucketArgs &ba, hipStream_t s);
+hipError_t launch_gen_fhits(const ShardBuf<FHit> &h, const GridDev &g, const BucketArgs &ba, hipStream_t s);
+hipError_t launch_gen_bhits(const ShardBuf<BHit> &h, const GridDev &g, const BucketArgs &ba, hipStream_t s);
+hipError_t launch_shard_reduce(unsigned long long *shard_ctr, unsigned long long *ctr, hipStream_t s);
+hipError_t launch_place_chunks(const ShardBuf<Chunk> &chunks, const uint32_t *choff, uint32_t *bchunks,
+                               hipStream_t s);
+hipError_t launch_active_bricks(const uint32_t *nch, uint64_t n_bricks, uint32_t *list, unsigned long long *count,
+                                hipStream_t s);
+hipError_t launch_fold_bricks(const FoldArgs &a, uint64_t max_bricks, hipStream_t s);
 hipError_t launch_edge_count(const MeshDev &m, uint32_t *vcnt, uint32_t *voff, uint32_t *nbr, int pass,
                              hipStream_t s);
 hipError_t launch_edge_singletons(int64_t V, const uint32_t *voff, uint32_t *nbr, uint32_t *bcnt, const uint32_t *boff,
                                   int2 *bedge_v, int pass, hipStream_t s);
 hipError_t launch_bedge_prep(const float *verts, int64_t E, const int2 *bedge_v, BEdge *be, hipStream_t s);
-hipError_t launch_bdry(int mode, int64_t E, const BEdge *be, const GridDev &g, const RecArgs &r,
+hipError_t launch_bdry(int64_t E, const BEdge *be, const GridDev &g, const ShardBuf<BHit> &hb,
                        unsigned long long *ctr, hipStream_t s);
 uint64_t scan_tile_count(uint64_t n);
 hipError_t launch_scan_popc(const uint64_t *bits, uint64_t n, uint32_t *out, uint64_t *tile_sums,
@@ -97,6 +169,5 @@ hipError_t launch_scan_popc(const uint64_t *bits, uint64_t n, uint32_t *out, uin
 hipError_t launch_scan_u32(const uint32_t *in, uint64_t n, uint32_t *out, uint64_t *tile_sums,
                            unsigned long long *total, hipStream_t s);
 hipError_t launch_emit_coords(const GridDev &g, uint64_t n_words, int32_t *coords, hipStream_t s);
-hipError_t launch_fold_solve(const FoldArgs &a, hipStream_t s);
 
 } // namespace ovx

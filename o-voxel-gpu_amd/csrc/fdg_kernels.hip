@@ -14,6 +14,7 @@
 // large and small triangles balance across the wave64 lanes.
 #include "fdg_device.h"
 #include "fdg_kernels.h"
+#include "fdg_solve.h"
 
 namespace ovx
 {
@@ -105,29 +106,162 @@ __device__ __forceinline__ uint32_t voxel_id(const GridDev &g, uint64_t w, int b
     return g.woff[w] + (uint32_t)__popcll(m & ((1ull << bit) - 1ull));
 }
 
-// count / emit one record for voxel `vid`
-template <int MODE>
-__device__ __forceinline__ void record(uint32_t vid, uint64_t key, float z, const RecArgs &r)
+// brick of an absolute voxel coordinate: id in the dense brick grid and the
+// voxel's index inside its 8x8x8 brick
+__device__ __forceinline__ void brick_of(const GridDev &g, const int nb[3], int c0, int c1, int c2, uint32_t &brick,
+                                         uint32_t &lvox)
 {
-    if (MODE == MODE_COUNT)
+    const int lx = c0 - g.gmin[0], ly = c1 - g.gmin[1], lz = c2 - g.gmin[2];
+    brick = (uint32_t)(lx >> 3) + (uint32_t)nb[0] * ((uint32_t)(ly >> 3) + (uint32_t)nb[1] * (uint32_t)(lz >> 3));
+    lvox = (uint32_t)((lx & 7) | ((ly & 7) << 3) | ((lz & 7) << 6));
+}
+
+// Wave-aggregated append: one atomic per wave.  All lanes of the wave must call it.
+__device__ __forceinline__ uint64_t wave_append(bool pred, unsigned long long *counter)
+{
+    const uint64_t m = __ballot(pred);
+    const int lane = threadIdx.x & 63;
+    const uint32_t below = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    unsigned long long base = 0;
+    if (m)
     {
-        atomicAdd(r.cnt + vid, 1u);
+        const int leader = __ffsll((long long)m) - 1;
+        if (lane == leader)
+            base = atomicAdd(counter, (unsigned long long)__popcll(m));
+        base = __shfl(base, leader, 64);
     }
-    else
+    return (uint64_t)base + below;
+}
+
+// Claims n items of shard `sh` of b; returns the global index of the first.
+// ok = false when the region is full (the counter still advances, so the
+// totals stay exact for re-sizing).
+template <typename T>
+__device__ __forceinline__ uint64_t shard_claim(const ShardBuf<T> &b, uint32_t sh, uint64_t n, bool &ok)
+{
+    const unsigned long long off = atomicAdd(b.cnt + (uint64_t)sh * SHARD_STRIDE, (unsigned long long)n);
+    ok = off + n <= b.per;
+    return (uint64_t)sh * b.per + off;
+}
+
+// Whether global slot i of a sharded buffer holds an item.  A region whose
+// counter passed its capacity had a claim dropped and is skipped whole (its
+// slots below the capacity may be unwritten); the host re-runs the producer
+// with larger regions in that case.
+template <typename T>
+__device__ __forceinline__ bool shard_valid(const ShardBuf<T> &b, uint64_t i)
+{
+    const uint64_t sh = i / b.per;
+    const uint64_t local = i - sh * b.per;
+    if (sh >= (uint64_t)NSHARD)
+        return false;
+    const unsigned long long n = b.cnt[sh * SHARD_STRIDE];
+    return n <= b.per && local < n;
+}
+
+// LDS table of one block's records grouped by brick
+template <int LOG2>
+struct BucketLds
+{
+    static constexpr int N = 1 << LOG2;
+    uint32_t key[N];
+    uint32_t cnt[N];
+    uint32_t base[N];
+    unsigned long long sh[NWAVE];
+    unsigned long long rbase, cbase;
+    bool fits;
+};
+
+// Block-level bucketed emission.  Each thread passes K records with their
+// brick ids and validity flags.  Records of one brick are counted in LDS; the
+// block reserves space for all of its records and chunks with one atomic each
+// (on the counters of its shard), gives every brick it touched one contiguous
+// chunk, and writes the records contiguously.  Capacity misses are counted
+// (the totals stay exact) and nothing is written out of bounds.  Contains
+// barriers: every thread of the block must call it.
+template <int K, int LOG2>
+__device__ __forceinline__ void bucket_emit(const uint32_t (&bk)[K], const Rec (&rc)[K], const bool (&ok)[K],
+                                            const BucketArgs &a, BucketLds<LOG2> &L)
+{
+    constexpr int N = BucketLds<LOG2>::N;
+    constexpr int PER = N / BT; // table entries per thread
+    static_assert(N > BT * K, "bucket table too small");
+    static_assert(N % BT == 0, "table size must be a multiple of the block size");
+    for (int s = threadIdx.x; s < N; s += BT)
     {
-        // cnt[vid] runs back down from the COUNT pass; a zero here would mean
-        // the two passes disagree -- never write outside the segment then.
-        const uint32_t old = atomicSub(r.cnt + vid, 1u);
-        if (old == 0u)
+        L.key[s] = 0xffffffffu;
+        L.cnt[s] = 0u;
+    }
+    __syncthreads();
+    int slot[K];
+    uint32_t rank[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+    {
+        slot[k] = 0;
+        rank[k] = 0;
+        if (ok[k])
         {
-            atomicAdd(r.cnt + vid, 1u);
-            atomicOr(r.err, (unsigned long long)ERR_RECORDS);
-            return;
+            uint32_t h = (bk[k] * 2654435761u) >> (32 - LOG2);
+            while (true)
+            {
+                const uint32_t prev = atomicCAS(&L.key[h], 0xffffffffu, bk[k]);
+                if (prev == 0xffffffffu || prev == bk[k])
+                    break;
+                h = (h + 1u) & (uint32_t)(N - 1);
+            }
+            slot[k] = (int)h;
+            rank[k] = atomicAdd(&L.cnt[h], 1u);
         }
-        const uint32_t slot = r.rec_off[vid] + old - 1u;
-        r.keys[slot] = key;
-        r.zs[slot] = z;
     }
+    __syncthreads();
+    // this thread's entries: PER consecutive table slots
+    const int s0 = threadIdx.x * PER;
+    unsigned long long my_n = 0, my_c = 0;
+#pragma unroll
+    for (int e = 0; e < PER; ++e)
+    {
+        const uint32_t n = L.cnt[s0 + e];
+        my_n += n;
+        my_c += n ? 1ull : 0ull;
+    }
+    unsigned long long tot_n, tot_c;
+    const unsigned long long ex_n = block_excl_scan(my_n, L.sh, tot_n);
+    const unsigned long long ex_c = block_excl_scan(my_c, L.sh, tot_c);
+    if (threadIdx.x == 0)
+    {
+        const uint32_t sh = blockIdx.x % NSHARD;
+        bool ok_r = true, ok_c = true;
+        L.rbase = tot_n ? shard_claim(a.recs, sh, tot_n, ok_r) : 0ull;
+        L.cbase = tot_c ? shard_claim(a.chunks, sh, tot_c, ok_c) : 0ull;
+        L.fits = ok_r && ok_c;
+    }
+    __syncthreads();
+    const bool fits = L.fits;
+    unsigned long long rpos = L.rbase + ex_n, cpos = L.cbase + ex_c;
+#pragma unroll
+    for (int e = 0; e < PER; ++e)
+    {
+        const uint32_t n = L.cnt[s0 + e];
+        if (n && fits)
+        {
+            const uint32_t b = L.key[s0 + e];
+            const uint32_t r = atomicAdd(a.brick_nch + b, 1u);
+            a.chunks.items[cpos] = Chunk{(uint32_t)rpos, n, b, r};
+            L.base[s0 + e] = (uint32_t)rpos;
+        }
+        rpos += n;
+        cpos += n ? 1ull : 0ull;
+    }
+    __syncthreads();
+    if (fits)
+    {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (ok[k])
+                a.recs.items[(uint64_t)L.base[slot[k]] + rank[k]] = rc[k];
+    }
+    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -196,11 +330,9 @@ struct ScanRow
     int item;
 };
 
-template <int MODE>
-__device__ __forceinline__ void event_action(const GridDev &g, int ax0, int ax1, int ax2, int x_idx, int y_idx,
-                                             int z_idx, float zf, uint64_t keybase, const RecArgs &r)
+// the 4 voxels around a crossing, in the reference's (dx, dy) order
+__device__ __forceinline__ void event_voxels(int ax0, int ax1, int x_idx, int y_idx, int z_idx, int (&c)[4][3])
 {
-    int c[4][3];
 #pragma unroll
     for (int dx = 0; dx < 2; ++dx)
 #pragma unroll
@@ -212,51 +344,46 @@ __device__ __forceinline__ void event_action(const GridDev &g, int ax0, int ax1,
             c[k][1] = ax0 == 1 ? cx : (ax1 == 1 ? cy : z_idx);
             c[k][2] = ax0 == 2 ? cx : (ax1 == 2 ? cy : z_idx);
         }
-    if (MODE == MODE_MARK)
-    {
-        uint64_t w[4], msk[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-        {
-            int b;
-            w[k] = word_of(g, c[k][0], c[k][1], c[k][2], b);
-            msk[k] = 1ull << b;
-        }
-#pragma unroll
-        for (int k = 1; k < 4; ++k)
-        {
-            bool merged = false;
-#pragma unroll
-            for (int j = 0; j < k; ++j)
-                if (!merged && msk[j] != 0 && w[j] == w[k])
-                {
-                    msk[j] |= msk[k];
-                    merged = true;
-                }
-            if (merged)
-                msk[k] = 0;
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (msk[k])
-                atomicOr((unsigned long long *)(g.bits + w[k]), (unsigned long long)msk[k]);
-    }
-    else
-    {
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-        {
-            int b;
-            const uint64_t w = word_of(g, c[k][0], c[k][1], c[k][2], b);
-            const uint32_t vid = voxel_id(g, w, b);
-            record<MODE>(vid, keybase | (uint64_t)k, zf, r);
-        }
-    }
 }
 
-template <int MODE>
-__global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, GridDev g, RecArgs r, unsigned long long *counters)
+// set the occupancy bits of the 4 voxels of a crossing (one atomicOr per word)
+__device__ __forceinline__ void mark_event(const GridDev &g, int ax0, int ax1, int x_idx, int y_idx, int z_idx)
 {
+    int c[4][3];
+    event_voxels(ax0, ax1, x_idx, y_idx, z_idx, c);
+    uint64_t w[4], msk[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+        int b;
+        w[k] = word_of(g, c[k][0], c[k][1], c[k][2], b);
+        msk[k] = 1ull << b;
+    }
+#pragma unroll
+    for (int k = 1; k < 4; ++k)
+    {
+        bool merged = false;
+#pragma unroll
+        for (int j = 0; j < k; ++j)
+            if (!merged && msk[j] != 0 && w[j] == w[k])
+            {
+                msk[j] |= msk[k];
+                merged = true;
+            }
+        if (merged)
+            msk[k] = 0;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (msk[k])
+            atomicOr((unsigned long long *)(g.bits + w[k]), (unsigned long long)msk[k]);
+}
+
+__global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, GridDev g, unsigned long long *counters,
+                                                    ShardBuf<Event> eb)
+{
+    __shared__ unsigned long long sbase;
+    __shared__ bool sfit;
     __shared__ ScanItem items[BT];
     __shared__ ScanRow rows[BT];
     __shared__ uint32_t incl_items[BT];
@@ -382,6 +509,8 @@ __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, GridDev g, RecArg
             for (uint32_t cb = 0; cb < tot_c; cb += BT)
             {
                 const uint32_t h = cb + tid;
+                Event evr;
+                uint32_t has = 0;
                 if (h < tot_c)
                 {
                     const int rr = upper_find(incl_rows, BT, h);
@@ -407,7 +536,29 @@ __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, GridDev g, RecArg
                         const uint64_t fa = (uint64_t)it.face * 3u + (uint64_t)ax2;
                         const uint64_t keybase = (fa << KEY_FA_SHIFT) | ((uint64_t)(rw.y_idx - ylo) << KEY_Y_SHIFT) |
                                                  ((uint64_t)(x_idx - xlo) << KEY_X_SHIFT);
-                        event_action<MODE>(g, ax0, ax1, ax2, x_idx, rw.y_idx, z_idx, (float)z, keybase, r);
+                        mark_event(g, ax0, ax1, x_idx, rw.y_idx, z_idx);
+                        evr.key = keybase;
+                        evr.z = (float)z;
+                        evr.zi = z_idx;
+                        has = 1;
+                    }
+                }
+                if (eb.items)
+                {
+                    // block-aggregated append: one atomic per block iteration, consecutive slots
+                    uint32_t tot_e;
+                    const uint32_t off = block_excl_scan(has, sh, tot_e);
+                    if (tot_e)
+                    {
+                        if (tid == 0)
+                        {
+                            bool okk;
+                            sbase = shard_claim(eb, blockIdx.x % NSHARD, tot_e, okk);
+                            sfit = okk;
+                        }
+                        __syncthreads();
+                        if (has && sfit)
+                            eb.items[sbase + off] = evr;
                     }
                 }
             }
@@ -420,8 +571,7 @@ __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, GridDev g, RecArg
         __shared__ uint32_t shr[NWAVE];
         const uint32_t tot = block_reduce_sum(n_ev, shr);
         if (tid == 0 && tot)
-            atomicAdd(counters + (MODE == MODE_MARK ? CTR_EVENTS : CTR_REC_INTERSECT),
-                      (unsigned long long)tot * (MODE == MODE_MARK ? 1ull : 4ull));
+            atomicAdd(counters + CTR_EVENTS, (unsigned long long)tot);
     }
 }
 
@@ -470,8 +620,7 @@ __device__ __forceinline__ uint64_t row_mask(int lo_bit, int hi_bit)
     return hi & ~lo;
 }
 
-template <int MODE>
-__global__ __launch_bounds__(BT) void k_face(MeshDev m, GridDev g, RecArgs r, unsigned long long *counters)
+__global__ __launch_bounds__(BT) void k_face(MeshDev m, GridDev g, ShardBuf<FHit> hb, unsigned long long *ctr)
 {
     __shared__ FaceItem items[BT];
     __shared__ FaceRow rows[BT];
@@ -589,6 +738,8 @@ __global__ __launch_bounds__(BT) void k_face(MeshDev m, GridDev g, RecArgs r, un
             for (uint32_t ab = 0; ab < tot_a; ab += BT)
             {
                 const uint32_t h = ab + tid;
+                FHit hit;
+                bool has = false;
                 if (h < tot_a)
                 {
                     const int rr = upper_find(incl_rows, BT, h);
@@ -602,8 +753,7 @@ __global__ __launch_bounds__(BT) void k_face(MeshDev m, GridDev g, RecArgs r, un
                     {
                         const int blo = (w == (lx0 >> 6)) ? (lx0 & 63) : 0;
                         const int bhi = (w == ((lx1 - 1) >> 6)) ? (((lx1 - 1) & 63) + 1) : 64;
-                        const uint64_t word = g.bits[rowbase + w];
-                        uint64_t mm = word & row_mask(blo, bhi);
+                        uint64_t mm = g.bits[rowbase + w] & row_mask(blo, bhi);
                         const uint32_t pc = (uint32_t)__popcll(mm);
                         if (k >= pc)
                         {
@@ -617,14 +767,32 @@ __global__ __launch_bounds__(BT) void k_face(MeshDev m, GridDev g, RecArgs r, un
                         if (face_pass(it, g.vs, x, row.y, row.z))
                         {
                             ++n_hit;
-                            if (MODE != MODE_MARK)
-                            {
-                                const uint32_t vid =
-                                    g.woff[rowbase + w] + (uint32_t)__popcll(word & ((1ull << b) - 1ull));
-                                record<MODE>(vid, (1ull << KEY_PHASE_SHIFT) | (uint64_t)it.face, 0.0f, r);
-                            }
+                            hit = FHit{it.face, (uint16_t)(x - g.gmin[0]), (uint16_t)(row.y - g.gmin[1]),
+                                       (uint16_t)(row.z - g.gmin[2]), 0};
+                            has = true;
                         }
                         break;
+                    }
+                }
+                // wave-aggregated append into this wave's shard (every lane of
+                // the block runs this iteration)
+                {
+                    const uint64_t m = __ballot(has);
+                    const int lane = tid & 63;
+                    if (m)
+                    {
+                        const int leader = __ffsll((long long)m) - 1;
+                        unsigned long long base = 0;
+                        bool okk = false;
+                        if (lane == leader)
+                        {
+                            base = shard_claim(hb, (blockIdx.x * NWAVE + (tid >> 6)) % NSHARD,
+                                               (uint64_t)__popcll(m), okk);
+                            base = okk ? base : ~0ull;
+                        }
+                        base = __shfl(base, leader, 64);
+                        if (has && base != ~0ull)
+                            hb.items[base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull))] = hit;
                     }
                 }
             }
@@ -632,12 +800,11 @@ __global__ __launch_bounds__(BT) void k_face(MeshDev m, GridDev g, RecArgs r, un
         }
         __syncthreads();
     }
-    if (counters && MODE == MODE_COUNT)
     {
         __shared__ uint32_t shr[NWAVE];
         const uint32_t tot = block_reduce_sum(n_hit, shr);
         if (tid == 0 && tot)
-            atomicAdd(counters + CTR_REC_FACE, (unsigned long long)tot);
+            atomicAdd(ctr + CTR_REC_FACE, (unsigned long long)tot);
     }
 }
 
@@ -771,9 +938,9 @@ __global__ __launch_bounds__(BT) void k_bedge_prep(const float *verts, int64_t E
     be[i] = out;
 }
 
-// DDA over a boundary segment (FDG.cpp:455-522); one lane per segment.
-template <int MODE>
-__global__ __launch_bounds__(BT) void k_bdry(int64_t E, const BEdge *be, GridDev g, RecArgs r,
+// DDA over a boundary segment (FDG.cpp:455-522); one lane per segment.  Every
+// active voxel it passes is appended to `hb` as (edge rank, voxel).
+__global__ __launch_bounds__(BT) void k_bdry(int64_t E, const BEdge *be, GridDev g, ShardBuf<BHit> hb,
                                              unsigned long long *counters)
 {
     const int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
@@ -803,7 +970,6 @@ __global__ __launch_bounds__(BT) void k_bdry(int64_t E, const BEdge *be, GridDev
                 cap += ceil(e.len * fabs(e.dir[a]) / (double)g.vs[a]) + 2.0;
             }
         }
-        const uint64_t key = (2ull << KEY_PHASE_SHIFT) | (uint64_t)i;
         int64_t steps = 0;
         const int64_t max_steps = (int64_t)(cap < 4.0e9 ? cap : 4.0e9);
         while (true)
@@ -817,8 +983,10 @@ __global__ __launch_bounds__(BT) void k_bdry(int64_t E, const BEdge *be, GridDev
                 if ((word >> b) & 1ull)
                 {
                     ++n_hit;
-                    if (MODE != MODE_MARK)
-                        record<MODE>(g.woff[w] + (uint32_t)__popcll(word & ((1ull << b) - 1ull)), key, 0.0f, r);
+                    bool okk;
+                    const uint64_t slot = shard_claim(hb, (uint32_t)(i % NSHARD), 1, okk);
+                    if (okk)
+                        hb.items[slot] = BHit{(uint32_t)i, cur[0], cur[1], cur[2]};
                 }
             }
             int axis;
@@ -855,10 +1023,161 @@ __global__ __launch_bounds__(BT) void k_bdry(int64_t E, const BEdge *be, GridDev
     {
         __shared__ uint32_t shr[NWAVE];
         const uint32_t tot = block_reduce_sum(n_hit, shr);
-        if (threadIdx.x == 0 && tot && MODE == MODE_COUNT)
+        if (threadIdx.x == 0 && tot)
             atomicAdd(counters + CTR_REC_BOUNDARY, (unsigned long long)tot);
         if (err)
             atomicOr(counters + CTR_ERRORS, (unsigned long long)err);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// record generation from the recorded crossings and boundary hits
+// ---------------------------------------------------------------------------
+// 4 phase-0 records per crossing, keys (face*3+axis, y, x, dx, dy)
+__global__ __launch_bounds__(BT) void k_gen_events(ShardBuf<Event> ev, GridDev g, BucketArgs ba)
+{
+    __shared__ BucketLds<11> L;
+    const uint64_t n = (uint64_t)NSHARD * ev.per;
+    for (uint64_t base = (uint64_t)blockIdx.x * BT; base < n; base += (uint64_t)gridDim.x * BT)
+    {
+        const uint64_t i = base + threadIdx.x;
+        uint32_t bk[4] = {0, 0, 0, 0};
+        Rec rc[4];
+        bool ok[4] = {false, false, false, false};
+        if (i < n && shard_valid(ev, i))
+        {
+            const Event e = ev.items[i];
+            const uint64_t fa = (e.key >> KEY_FA_SHIFT) & KEY_FA_MASK;
+            const uint64_t face = fa / 3u;
+            const int ax2 = (int)(fa - 3u * face);
+            const int ax0 = (ax2 + 1) % 3;
+            const int ax1 = (ax2 + 2) % 3;
+            const int y_idx =
+                (int)((e.key >> KEY_Y_SHIFT) & KEY_IDX_MASK) + sel3i(ax1, g.gmin[0], g.gmin[1], g.gmin[2]);
+            const int x_idx =
+                (int)((e.key >> KEY_X_SHIFT) & KEY_IDX_MASK) + sel3i(ax0, g.gmin[0], g.gmin[1], g.gmin[2]);
+            int c[4][3];
+            event_voxels(ax0, ax1, x_idx, y_idx, e.zi, c);
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+            {
+                uint32_t lvox;
+                brick_of(g, ba.nb, c[k][0], c[k][1], c[k][2], bk[k], lvox);
+                rc[k] = Rec{e.key | (uint64_t)k, e.z, lvox};
+                ok[k] = true;
+            }
+        }
+        bucket_emit<4, 11>(bk, rc, ok, ba, L);
+    }
+}
+
+// phase-1 records from the face-test hits, key = face index
+__global__ __launch_bounds__(BT) void k_gen_fhits(ShardBuf<FHit> h, GridDev g, BucketArgs ba)
+{
+    __shared__ BucketLds<11> L;
+    const uint64_t n = (uint64_t)NSHARD * h.per;
+    for (uint64_t base = (uint64_t)blockIdx.x * (4 * BT); base < n; base += (uint64_t)gridDim.x * (4 * BT))
+    {
+        uint32_t bk[4] = {0, 0, 0, 0};
+        Rec rc[4];
+        bool ok[4] = {false, false, false, false};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+        {
+            const uint64_t i = base + (uint64_t)k * BT + threadIdx.x;
+            if (i < n && shard_valid(h, i))
+            {
+                const FHit x = h.items[i];
+                uint32_t lvox;
+                brick_of(g, ba.nb, g.gmin[0] + (int)x.lx, g.gmin[1] + (int)x.ly, g.gmin[2] + (int)x.lz, bk[k], lvox);
+                rc[k] = Rec{(1ull << KEY_PHASE_SHIFT) | (uint64_t)x.face, 0.0f, lvox};
+                ok[k] = true;
+            }
+        }
+        bucket_emit<4, 11>(bk, rc, ok, ba, L);
+    }
+}
+
+// phase-2 records from the boundary DDA hits, key = boundary-edge rank
+__global__ __launch_bounds__(BT) void k_gen_bhits(ShardBuf<BHit> h, GridDev g, BucketArgs ba)
+{
+    __shared__ BucketLds<9> L;
+    const uint64_t n = (uint64_t)NSHARD * h.per;
+    for (uint64_t base = (uint64_t)blockIdx.x * BT; base < n; base += (uint64_t)gridDim.x * BT)
+    {
+        const uint64_t i = base + threadIdx.x;
+        uint32_t bk[1] = {0};
+        Rec rc[1];
+        bool ok[1] = {false};
+        if (i < n && shard_valid(h, i))
+        {
+            const BHit x = h.items[i];
+            uint32_t lvox;
+            brick_of(g, ba.nb, x.x, x.y, x.z, bk[0], lvox);
+            rc[0] = Rec{(2ull << KEY_PHASE_SHIFT) | (uint64_t)x.edge, 0.0f, lvox};
+            ok[0] = true;
+        }
+        bucket_emit<1, 9>(bk, rc, ok, ba, L);
+    }
+}
+
+// sum and largest shard of each sharded buffer's requests
+__global__ __launch_bounds__(BT) void k_shard_reduce(const unsigned long long *shard_ctr, unsigned long long *ctr)
+{
+    __shared__ unsigned long long sh[NWAVE];
+    const int b = blockIdx.x; // one block per buffer
+    const unsigned long long v = shard_ctr[((uint64_t)b * NSHARD + threadIdx.x) * SHARD_STRIDE];
+    unsigned long long tot;
+    block_excl_scan(v, sh, tot);
+    // max via the wave / LDS reduction of the scan's building blocks
+    unsigned long long m = v;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
+    {
+        const unsigned long long o = __shfl_xor(m, d, 64);
+        m = o > m ? o : m;
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+        sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        unsigned long long mx = sh[0];
+        for (int w = 1; w < NWAVE; ++w)
+            mx = sh[w] > mx ? sh[w] : mx;
+        ctr[CTR_SUM + b] = tot;
+        ctr[CTR_MAX + b] = mx;
+    }
+}
+
+// chunk indices grouped by brick: position = first chunk of the brick + rank
+__global__ __launch_bounds__(BT) void k_place_chunks(ShardBuf<Chunk> chunks, const uint32_t *choff, uint32_t *bchunks)
+{
+    const uint64_t n = (uint64_t)NSHARD * chunks.per;
+    const uint64_t stride = (uint64_t)gridDim.x * BT;
+    for (uint64_t c = (uint64_t)blockIdx.x * BT + threadIdx.x; c < n; c += stride)
+    {
+        if (!shard_valid(chunks, c))
+            continue;
+        const Chunk ch = chunks.items[c];
+        bchunks[choff[ch.brick] + ch.rank] = (uint32_t)c;
+    }
+}
+
+// list of bricks that received records
+__global__ __launch_bounds__(BT) void k_active_bricks(const uint32_t *nch, uint64_t n_bricks, uint32_t *list,
+                                                      unsigned long long *count)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * BT;
+    // every lane of a wave runs the same number of iterations (wave_append is wave-wide)
+    const uint64_t n_round = (n_bricks + 63ull) & ~63ull;
+    for (uint64_t b = (uint64_t)blockIdx.x * BT + threadIdx.x; b < n_round; b += stride)
+    {
+        const bool act = b < n_bricks && nch[b] != 0u;
+        const uint64_t slot = wave_append(act, count);
+        if (act)
+            list[slot] = (uint32_t)b;
     }
 }
 
@@ -960,359 +1279,26 @@ __global__ __launch_bounds__(BT) void k_emit_coords(GridDev g, uint64_t n_words,
     }
 }
 
-// ---------------------------------------------------------------------------
-// QEF solve (FDG.cpp:763-983): Eigen ColPivHouseholderQR restated for n = 2, 3
-// ---------------------------------------------------------------------------
-template <int n>
-__device__ __forceinline__ void cpqr_solve(const float (&A)[n][n], const float (&b)[n], float (&x)[n])
-{
-    // one spare row/column keeps the unrolled (dead) tail indices in bounds
-    float qr[n + 1][n + 1];
-#pragma unroll
-    for (int i = 0; i <= n; ++i)
-#pragma unroll
-        for (int j = 0; j <= n; ++j)
-            qr[i][j] = (i < n && j < n) ? A[i][j] : 0.0f;
-    float hco[n];
-    int transp[n];
-    float nu[n], nd[n];
-#pragma unroll
-    for (int k = 0; k < n; ++k)
-    {
-        float s;
-        if constexpr (n == 3)
-            s = qr[0][k] * qr[0][k] + (qr[1][k] * qr[1][k] + qr[2][k] * qr[2][k]);
-        else
-            s = qr[0][k] * qr[0][k] + qr[1][k] * qr[1][k];
-        nd[k] = sqrtf(s);
-        nu[k] = nd[k];
-    }
-    float maxn = nu[0];
-#pragma unroll
-    for (int k = 1; k < n; ++k)
-        if (nu[k] > maxn)
-            maxn = nu[k];
-    const float me = maxn * FLT_EPSILON;
-    const float th = (me * me) / (float)n;
-    const float ndt = sqrtf(FLT_EPSILON);
-    int nzp = n;
-#pragma unroll
-    for (int k = 0; k < n; ++k)
-    {
-        int big = k;
-        float bigv = nu[k];
-#pragma unroll
-        for (int j = k + 1; j < n; ++j)
-            if (nu[j] > bigv)
-            {
-                bigv = nu[j];
-                big = j;
-            }
-        if (nzp == n && bigv * bigv < th * (float)(n - k))
-            nzp = k;
-        transp[k] = big;
-#pragma unroll
-        for (int j = k + 1; j < n; ++j)
-            if (j == big)
-            {
-#pragma unroll
-                for (int rr = 0; rr < n; ++rr)
-                {
-                    const float t = qr[rr][k];
-                    qr[rr][k] = qr[rr][j];
-                    qr[rr][j] = t;
-                }
-                float t = nu[k];
-                nu[k] = nu[j];
-                nu[j] = t;
-                t = nd[k];
-                nd[k] = nd[j];
-                nd[j] = t;
-            }
-        const int tl = n - k - 1;
-        float tail_sq = 0.0f;
-        if (tl > 0)
-        {
-            tail_sq = qr[k + 1][k] * qr[k + 1][k];
-#pragma unroll
-            for (int i = 1; i < n - k - 1; ++i)
-                tail_sq = tail_sq + qr[k + 1 + i][k] * qr[k + 1 + i][k];
-        }
-        const float c0 = qr[k][k];
-        float beta, tau;
-        if (tail_sq <= FLT_MIN)
-        {
-            tau = 0.0f;
-            beta = c0;
-#pragma unroll
-            for (int i = 0; i < n - k - 1; ++i)
-                qr[k + 1 + i][k] = 0.0f;
-        }
-        else
-        {
-            beta = sqrtf(c0 * c0 + tail_sq);
-            if (c0 >= 0.0f)
-                beta = -beta;
-            const float den = c0 - beta;
-#pragma unroll
-            for (int i = 0; i < n - k - 1; ++i)
-                qr[k + 1 + i][k] = qr[k + 1 + i][k] / den;
-            tau = (beta - c0) / beta;
-        }
-        hco[k] = tau;
-        qr[k][k] = beta;
-        if (n - k > 1 && tau != 0.0f)
-        {
-#pragma unroll
-            for (int j = k + 1; j < n; ++j)
-            {
-                float tmp = qr[k + 1][k] * qr[k + 1][j];
-#pragma unroll
-                for (int i = 1; i < n - k - 1; ++i)
-                    tmp = tmp + qr[k + 1 + i][k] * qr[k + 1 + i][j];
-                tmp = tmp + qr[k][j];
-                qr[k][j] = qr[k][j] - tau * tmp;
-#pragma unroll
-                for (int i = 0; i < n - k - 1; ++i)
-                {
-                    const float te = tau * qr[k + 1 + i][k];
-                    qr[k + 1 + i][j] = qr[k + 1 + i][j] - tmp * te;
-                }
-            }
-        }
-#pragma unroll
-        for (int j = k + 1; j < n; ++j)
-        {
-            if (nu[j] != 0.0f)
-            {
-                const float a = fabsf(qr[k][j]);
-                float temp = a / nu[j];
-                temp = (1.0f + temp) * (1.0f - temp);
-                temp = temp < 0.0f ? 0.0f : temp;
-                const float ratio = nu[j] / nd[j];
-                const float temp2 = temp * (ratio * ratio);
-                if (temp2 <= ndt)
-                {
-                    float s = 0.0f;
-                    if (k + 1 < n)
-                    {
-                        s = qr[k + 1][j] * qr[k + 1][j];
-#pragma unroll
-                        for (int i = k + 2; i < n; ++i)
-                            s = s + qr[i][j] * qr[i][j];
-                        s = sqrtf(s);
-                    }
-                    nd[j] = s;
-                    nu[j] = s;
-                }
-                else
-                {
-                    nu[j] = nu[j] * sqrtf(temp);
-                }
-            }
-        }
-    }
-    int perm[n];
-#pragma unroll
-    for (int i = 0; i < n; ++i)
-        perm[i] = i;
-#pragma unroll
-    for (int k = 0; k < n; ++k)
-    {
-#pragma unroll
-        for (int j = k + 1; j < n; ++j)
-            if (transp[k] == j)
-            {
-                const int t = perm[k];
-                perm[k] = perm[j];
-                perm[j] = t;
-            }
-    }
-    const int rk = nzp;
-    float c[n + 1];
-#pragma unroll
-    for (int i = 0; i < n; ++i)
-        c[i] = b[i];
-    c[n] = 0.0f;
-    if (rk == 0)
-    {
-#pragma unroll
-        for (int i = 0; i < n; ++i)
-            x[i] = 0.0f;
-        return;
-    }
-#pragma unroll
-    for (int k = 0; k < n; ++k)
-    {
-        if (k < rk)
-        {
-            const float tau = hco[k];
-            if (n - k == 1)
-            {
-                c[k] = c[k] * (1.0f - tau);
-            }
-            else if (tau != 0.0f)
-            {
-                float tmp = qr[k + 1][k] * c[k + 1];
-#pragma unroll
-                for (int i = 1; i < n - k - 1; ++i)
-                    tmp = tmp + qr[k + 1 + i][k] * c[k + 1 + i];
-                tmp = tmp + c[k];
-                c[k] = c[k] - tau * tmp;
-#pragma unroll
-                for (int i = 0; i < n - k - 1; ++i)
-                {
-                    const float te = tau * qr[k + 1 + i][k];
-                    c[k + 1 + i] = c[k + 1 + i] - tmp * te;
-                }
-            }
-        }
-    }
-#pragma unroll
-    for (int i = n - 1; i >= 0; --i)
-    {
-        if (i < rk && c[i] != 0.0f)
-        {
-            c[i] = c[i] / qr[i][i];
-#pragma unroll
-            for (int s = 0; s < i; ++s)
-                c[s] = c[s] - c[i] * qr[s][i];
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < n; ++j)
-    {
-        float v = 0.0f;
-#pragma unroll
-        for (int i = 0; i < n; ++i)
-            if (perm[i] == j && i < rk)
-                v = c[i];
-        x[j] = v;
-    }
-}
-
-// p^T Q p in the Packet4f predux order (x0 + x2) + (x1 + x3)
-__device__ __forceinline__ float quad_form(const float p[4], const float Q[4][4])
-{
-    float t[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-        t[j] = (p[0] * Q[0][j] + p[2] * Q[2][j]) + (p[1] * Q[1][j] + p[3] * Q[3][j]);
-    return (t[0] * p[0] + t[2] * p[2]) + (t[1] * p[1] + t[3] * p[3]);
-}
-
-__device__ __forceinline__ void solve_voxel(const float q[10], const int coord[3], const float vs[3], float out[3])
-{
-    float Q[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            Q[i][j] = q[qidx(i, j)];
-    float mn[3], mx[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-    {
-        mn[a] = (float)coord[a] * vs[a];
-        mx[a] = (float)(coord[a] + 1) * vs[a];
-    }
-    const float A[3][3] = {{Q[0][0], Q[0][1], Q[0][2]}, {Q[1][0], Q[1][1], Q[1][2]}, {Q[2][0], Q[2][1], Q[2][2]}};
-    const float bv[3] = {-Q[0][3], -Q[1][3], -Q[2][3]};
-    float v[3];
-    cpqr_solve<3>(A, bv, v);
-    if (!(v[0] >= mn[0] && v[0] <= mx[0] && v[1] >= mn[1] && v[1] <= mx[1] && v[2] >= mn[2] && v[2] <= mx[2]))
-    {
-        float best = __builtin_huge_valf();
-        auto consider = [&](const float p[4]) {
-            const float err = quad_form(p, Q);
-            if (err < best)
-            {
-                best = err;
-                v[0] = p[0];
-                v[1] = p[1];
-                v[2] = p[2];
-            }
-        };
-#pragma unroll
-        for (int fixed = 0; fixed < 3; ++fixed)
-        {
-            const int a1 = (fixed + 1) % 3, a2 = (fixed + 2) % 3;
-            const float A2[2][2] = {{Q[a1][a1], Q[a1][a2]}, {Q[a2][a1], Q[a2][a2]}};
-            const float B[2][2] = {{Q[a1][fixed], Q[a1][3]}, {Q[a2][fixed], Q[a2][3]}};
-#pragma unroll
-            for (int side = 0; side < 2; ++side)
-            {
-                const float q0 = side ? mx[fixed] : mn[fixed];
-                const float rhs[2] = {(-B[0][0]) * q0 + (-B[0][1]) * 1.0f, (-B[1][0]) * q0 + (-B[1][1]) * 1.0f};
-                float x2[2];
-                cpqr_solve<2>(A2, rhs, x2);
-                if (x2[0] >= mn[a1] && x2[0] <= mx[a1] && x2[1] >= mn[a2] && x2[1] <= mx[a2])
-                {
-                    float p[4];
-                    p[fixed] = q0;
-                    p[a1] = x2[0];
-                    p[a2] = x2[1];
-                    p[3] = 1.0f;
-                    consider(p);
-                }
-            }
-        }
-#pragma unroll
-        for (int fr = 0; fr < 3; ++fr)
-        {
-            const int a1 = (fr + 1) % 3, a2 = (fr + 2) % 3;
-            const float a = Q[fr][fr];
-            const float b0 = Q[fr][a1], b1 = Q[fr][a2], b2 = Q[fr][3];
-#pragma unroll
-            for (int s1 = 0; s1 < 2; ++s1)
-#pragma unroll
-                for (int s2 = 0; s2 < 2; ++s2)
-                {
-                    const float q1 = s1 ? mx[a1] : mn[a1];
-                    const float q2 = s2 ? mx[a2] : mn[a2];
-                    const float xv = -(dot3f(b0, b1, b2, q1, q2, 1.0f)) / a;
-                    if (xv >= mn[fr] && xv <= mx[fr])
-                    {
-                        float p[4];
-                        p[fr] = xv;
-                        p[a1] = q1;
-                        p[a2] = q2;
-                        p[3] = 1.0f;
-                        consider(p);
-                    }
-                }
-        }
-#pragma unroll
-        for (int cx = 0; cx < 2; ++cx)
-#pragma unroll
-            for (int cy = 0; cy < 2; ++cy)
-#pragma unroll
-                for (int cz = 0; cz < 2; ++cz)
-                {
-                    const float p[4] = {cx ? mn[0] : mx[0], cy ? mn[1] : mx[1], cz ? mn[2] : mx[2], 1.0f};
-                    consider(p);
-                }
-    }
-    out[0] = v[0];
-    out[1] = v[1];
-    out[2] = v[2];
-}
-
-// Replay one voxel's contributions in key order and solve.
-template <class Get>
-__device__ __forceinline__ void fold_solve(uint32_t vid, int n, Get get, const FoldArgs &a)
+// Running replay of one voxel's contributions in the reference's serial order
+// (FDG.cpp:254-271 intersection, 409 face, 521 boundary), then regularization
+// and solve (FDG.cpp:780-982).
+struct FoldState
 {
     float q[10];
-#pragma unroll
-    for (int k = 0; k < 10; ++k)
-        q[k] = 0.0f;
-    float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, cnt = 0.0f;
-    int fl = 0;
-    for (int i = 0; i < n; ++i)
+    float m0, m1, m2, cnt;
+    int fl;
+
+    __device__ __forceinline__ void init()
     {
-        uint64_t key;
-        float z;
-        get(i, key, z);
+#pragma unroll
+        for (int k = 0; k < 10; ++k)
+            q[k] = 0.0f;
+        m0 = m1 = m2 = cnt = 0.0f;
+        fl = 0;
+    }
+
+    __device__ __forceinline__ void add(uint64_t key, float z, const FoldArgs &a)
+    {
         const int phase = (int)(key >> KEY_PHASE_SHIFT);
         if (phase == 0)
         {
@@ -1321,10 +1307,12 @@ __device__ __forceinline__ void fold_solve(uint32_t vid, int n, Get get, const F
             const int ax2 = (int)(fa - 3ull * f);
             const int ax0 = (ax2 + 1) % 3;
             const int ax1 = (ax2 + 2) % 3;
-            const int y_idx = (int)((key >> KEY_Y_SHIFT) & KEY_IDX_MASK) + sel3i(ax1, a.gmin[0], a.gmin[1], a.gmin[2]);
-            const int x_idx = (int)((key >> KEY_X_SHIFT) & KEY_IDX_MASK) + sel3i(ax0, a.gmin[0], a.gmin[1], a.gmin[2]);
-            const float px = (float)(x_idx + 1) * sel3f(ax0, a.vs[0], a.vs[1], a.vs[2]);
-            const float py = (float)(y_idx + 1) * sel3f(ax1, a.vs[0], a.vs[1], a.vs[2]);
+            const int y_idx =
+                (int)((key >> KEY_Y_SHIFT) & KEY_IDX_MASK) + sel3i(ax1, a.g.gmin[0], a.g.gmin[1], a.g.gmin[2]);
+            const int x_idx =
+                (int)((key >> KEY_X_SHIFT) & KEY_IDX_MASK) + sel3i(ax0, a.g.gmin[0], a.g.gmin[1], a.g.gmin[2]);
+            const float px = (float)(x_idx + 1) * sel3f(ax0, a.g.vs[0], a.g.vs[1], a.g.vs[2]);
+            const float py = (float)(y_idx + 1) * sel3f(ax1, a.g.vs[0], a.g.vs[1], a.g.vs[2]);
             const float p0 = ax0 == 0 ? px : (ax1 == 0 ? py : z);
             const float p1 = ax0 == 1 ? px : (ax1 == 1 ? py : z);
             const float p2 = ax0 == 2 ? px : (ax1 == 2 ? py : z);
@@ -1332,8 +1320,9 @@ __device__ __forceinline__ void fold_solve(uint32_t vid, int n, Get get, const F
             const float pl[4] = {pl4.x, pl4.y, pl4.z, pl4.w};
             float qf[10];
             plane_quadric(pl, qf);
-            if (i == 0)
+            if (cnt == 0.0f)
             {
+                // first contribution: the reference pushes the point and Q
                 m0 = p0;
                 m1 = p1;
                 m2 = p2;
@@ -1375,134 +1364,327 @@ __device__ __forceinline__ void fold_solve(uint32_t vid, int n, Get get, const F
                 q[k] = q[k] + a.bw * qe[k];
         }
     }
-    const int coord[3] = {a.coords[3 * (uint64_t)vid + 0], a.coords[3 * (uint64_t)vid + 1],
-                          a.coords[3 * (uint64_t)vid + 2]};
-    if (a.rw > 0.0f && n > 0)
+
+    __device__ __forceinline__ void finish(int x, int y, int z, const FoldArgs &a)
     {
-        const float p0 = m0 / cnt, p1 = m1 / cnt, p2 = m2 / cnt;
-        const float s = a.rw * cnt;
-        const float R[10] = {1.0f, 0.0f, 0.0f, -p0, 1.0f, 0.0f, -p1, 1.0f, -p2, dot3f(p0, p1, p2, p0, p1, p2)};
+        if (a.rw > 0.0f && cnt > 0.0f)
+        {
+            const float p0 = m0 / cnt, p1 = m1 / cnt, p2 = m2 / cnt;
+            const float s = a.rw * cnt;
+            const float R[10] = {1.0f, 0.0f, 0.0f, -p0, 1.0f, 0.0f, -p1, 1.0f, -p2, dot3f(p0, p1, p2, p0, p1, p2)};
 #pragma unroll
-        for (int k = 0; k < 10; ++k)
-            q[k] = q[k] + s * R[k];
+            for (int k = 0; k < 10; ++k)
+                q[k] = q[k] + s * R[k];
+        }
+        const int coord[3] = {x, y, z};
+        float d[3];
+        solve_voxel(q, coord, a.g.vs, d);
+        int b;
+        const uint64_t w = word_of(a.g, x, y, z, b);
+        const uint64_t vid = (uint64_t)voxel_id(a.g, w, b);
+        float *o = a.dual + 3 * vid;
+        o[0] = d[0];
+        o[1] = d[1];
+        o[2] = d[2];
+        uint8_t *fo = a.flags + 3 * vid;
+        fo[0] = (uint8_t)(fl & 1);
+        fo[1] = (uint8_t)((fl >> 1) & 1);
+        fo[2] = (uint8_t)((fl >> 2) & 1);
     }
-    float d[3];
-    solve_voxel(q, coord, a.vs, d);
-    float *o = a.dual + 3 * (uint64_t)vid;
-    o[0] = d[0];
-    o[1] = d[1];
-    o[2] = d[2];
-    uint8_t *fo = a.flags + 3 * (uint64_t)vid;
-    fo[0] = (uint8_t)(fl & 1);
-    fo[1] = (uint8_t)((fl >> 1) & 1);
-    fo[2] = (uint8_t)((fl >> 2) & 1);
+};
+
+constexpr int FOLD_CAP = 2560; // records held in LDS at a time
+
+// LDS view of a batch of up to BT chunks of one brick
+struct ChunkBatch
+{
+    uint32_t start[BT];
+    uint32_t incl[BT];
+    uint32_t sh[NWAVE];
+};
+
+// Calls f(rec) for every record of `brick`, all threads cooperating.
+// Contains barriers: every thread of the block must call it.
+template <class Fn>
+__device__ __forceinline__ void for_each_rec(const FoldArgs &a, uint32_t brick, ChunkBatch &cb, Fn &&f)
+{
+    const uint32_t c0 = a.choff[brick], c1 = a.choff[brick + 1];
+    for (uint32_t cbeg = c0; cbeg < c1; cbeg += BT)
+    {
+        const uint32_t ci = cbeg + threadIdx.x;
+        uint32_t cnt = 0;
+        if (ci < c1)
+        {
+            const Chunk ch = a.chunks[a.bchunks[ci]];
+            cb.start[threadIdx.x] = ch.start;
+            cnt = ch.count;
+        }
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan(cnt, cb.sh, tot);
+        cb.incl[threadIdx.x] = ex + cnt;
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < tot; j += BT)
+        {
+            const int k = upper_find(cb.incl, BT, j);
+            const uint32_t before = k ? cb.incl[k - 1] : 0u;
+            f(a.recs[(uint64_t)cb.start[k] + (j - before)]);
+        }
+        __syncthreads();
+    }
 }
 
-constexpr int FOLD_CAP = 3072;
-
-__global__ __launch_bounds__(BT) void k_fold_solve(FoldArgs a)
+// One workgroup per brick that received records: group the brick's records
+// by voxel in LDS (histogram + counting sort), rank-sort each voxel's keys,
+// replay them in order, solve.  Voxel ranges are sized to the LDS tile; a
+// voxel with more records than the tile is replayed in key-ordered batches
+// selected by a binary search over the key value.
+__global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs a)
 {
     __shared__ uint64_t skey[FOLD_CAP];
     __shared__ float sz[FOLD_CAP];
     __shared__ uint16_t sperm[FOLD_CAP];
-    __shared__ uint32_t soff[BT + 1];
+    __shared__ uint16_t slv[FOLD_CAP];
+    __shared__ uint16_t perm1[FOLD_CAP];
+    __shared__ uint32_t s_n;
+    __shared__ uint32_t off[513];
+    __shared__ uint32_t cur[512];
+    __shared__ uint32_t sh[NWAVE];
+    __shared__ ChunkBatch cb;
+    __shared__ uint32_t s_m;
     const int tid = threadIdx.x;
-    const uint64_t v0 = (uint64_t)blockIdx.x * BT;
-    if (v0 >= a.N)
-        return;
-    const int nv = (int)((a.N - v0) < (uint64_t)BT ? (a.N - v0) : (uint64_t)BT);
-    for (int i = tid; i <= nv; i += BT)
-        soff[i] = a.rec_off[v0 + i];
-    __syncthreads();
+    const uint64_t n_active = *a.n_active;
 
-    int cv = 0;
-    while (cv < nv)
+    for (uint64_t bi = blockIdx.x; bi < n_active; bi += gridDim.x)
     {
-        const uint32_t base = soff[cv];
-        if (soff[cv + 1] - base > (uint32_t)FOLD_CAP)
-        {
-            // one voxel with more records than the LDS tile: rank-sort it in
-            // global memory, then fold it on one lane
-            const uint32_t nrec = soff[cv + 1] - base;
-            for (uint32_t i = tid; i < nrec; i += BT)
+        const uint32_t brick = a.active[bi];
+        const int bx = (int)(brick % (uint32_t)a.nb[0]);
+        const int byz = (int)(brick / (uint32_t)a.nb[0]);
+        const int by = byz % a.nb[1];
+        const int bz = byz / a.nb[1];
+        const int x0 = a.g.gmin[0] + 8 * bx, y0 = a.g.gmin[1] + 8 * by, z0 = a.g.gmin[2] + 8 * bz;
+
+        // one pass over the brick's records: per-voxel histogram, and the
+        // records themselves into LDS while they fit
+        for (int i = tid; i < 512; i += BT)
+            cur[i] = 0;
+        if (tid == 0)
+            s_n = 0;
+        __syncthreads();
+        for_each_rec(a, brick, cb, [&](const Rec &r) {
+            atomicAdd(&cur[r.lvox], 1u);
+            const uint32_t i = atomicAdd(&s_n, 1u);
+            if (i < (uint32_t)FOLD_CAP)
             {
-                const uint64_t k = a.keys[base + i];
-                uint32_t rank = 0;
-                for (uint32_t t = 0; t < nrec; ++t)
-                    rank += (a.keys[base + t] < k) ? 1u : 0u;
-                a.gperm[base + rank] = i;
+                skey[i] = r.key;
+                sz[i] = r.z;
+                slv[i] = (uint16_t)r.lvox;
             }
-            __syncthreads();
+        });
+        {
+            // exclusive scan of the 512 counts (2 per thread)
+            const uint32_t c0 = cur[2 * tid], c1 = cur[2 * tid + 1];
+            uint32_t tot;
+            const uint32_t ex = block_excl_scan(c0 + c1, sh, tot);
+            off[2 * tid] = ex;
+            off[2 * tid + 1] = ex + c0;
             if (tid == 0)
-            {
-                const uint64_t *K = a.keys + base;
-                const float *Z = a.zs + base;
-                const uint32_t *P = a.gperm + base;
-                fold_solve((uint32_t)(v0 + cv), (int)nrec,
-                           [&](int i, uint64_t &key, float &z) {
-                               const uint32_t j = P[i];
-                               key = K[j];
-                               z = Z[j];
-                           },
-                           a);
-                atomicAdd(a.counters + CTR_GIANT, 1ull);
-            }
-            __syncthreads();
-            cv += 1;
-            continue;
-        }
-        // largest ce with soff[ce] - base <= FOLD_CAP
-        int lo = cv + 1, hi = nv;
-        while (lo < hi)
-        {
-            const int mid = (lo + hi + 1) >> 1;
-            if (soff[mid] - base <= (uint32_t)FOLD_CAP)
-                lo = mid;
-            else
-                hi = mid - 1;
-        }
-        const int ce = lo;
-        const uint32_t nrec = soff[ce] - base;
-        for (uint32_t i = tid; i < nrec; i += BT)
-        {
-            skey[i] = a.keys[base + i];
-            sz[i] = a.zs[base + i];
+                off[512] = tot;
         }
         __syncthreads();
-        for (uint32_t i = tid; i < nrec; i += BT)
+
+        const uint32_t n_all = s_n;
+        if (n_all <= (uint32_t)FOLD_CAP)
         {
-            // voxel of record i: largest j in [cv, ce) with soff[j] - base <= i
-            int l = cv, h = ce - 1;
+            // ---- all of the brick's records are in LDS: counting sort by voxel,
+            // rank sort inside each voxel, replay
+            for (int i = tid; i < 512; i += BT)
+                cur[i] = 0;
+            __syncthreads();
+            for (uint32_t i = tid; i < n_all; i += BT)
+            {
+                const uint32_t lv = slv[i];
+                perm1[off[lv] + atomicAdd(&cur[lv], 1u)] = (uint16_t)i;
+            }
+            __syncthreads();
+            for (uint32_t p = tid; p < n_all; p += BT)
+            {
+                int vl = 0, vh = 511;
+                while (vl < vh)
+                {
+                    const int mid = (vl + vh + 1) >> 1;
+                    if (off[mid] <= p)
+                        vl = mid;
+                    else
+                        vh = mid - 1;
+                }
+                const uint32_t s0 = off[vl], s1 = off[vl + 1];
+                const uint32_t me = perm1[p];
+                const uint64_t k = skey[me];
+                uint32_t rank = 0;
+                for (uint32_t t = s0; t < s1; ++t)
+                    rank += (skey[perm1[t]] < k) ? 1u : 0u;
+                sperm[s0 + rank] = (uint16_t)me;
+            }
+            __syncthreads();
+            for (int lv = tid; lv < 512; lv += BT)
+            {
+                const uint32_t s0 = off[lv], s1 = off[lv + 1];
+                if (s1 == s0)
+                    continue;
+                FoldState st;
+                st.init();
+                for (uint32_t i = s0; i < s1; ++i)
+                {
+                    const uint32_t j = sperm[i];
+                    st.add(skey[j], sz[j], a);
+                }
+                st.finish(x0 + (lv & 7), y0 + ((lv >> 3) & 7), z0 + (lv >> 6), a);
+            }
+            __syncthreads();
+            continue;
+        }
+
+        int lo = 0;
+        while (lo < 512)
+        {
+            const uint32_t nlo = off[lo + 1] - off[lo];
+            if (nlo == 0)
+            {
+                ++lo;
+                continue;
+            }
+            if (nlo > (uint32_t)FOLD_CAP)
+            {
+                // ---- one voxel with more records than the tile: key-ordered batches
+                const uint32_t lv = (uint32_t)lo;
+                FoldState st;
+                st.init();
+                uint64_t last = 0;
+                bool have_last = false;
+                uint32_t done = 0;
+                while (done < nlo)
+                {
+                    uint64_t tlo = have_last ? last + 1ull : 0ull, thi = ~0ull;
+                    // largest T with count(last < key <= T) <= FOLD_CAP
+                    while (tlo < thi)
+                    {
+                        const uint64_t d = thi - tlo;
+                        const uint64_t mid = tlo + d / 2 + (d & 1ull);
+                        uint32_t cnt = 0;
+                        for_each_rec(a, brick, cb, [&](const Rec &r) {
+                            cnt += (r.lvox == lv && (!have_last || r.key > last) && r.key <= mid) ? 1u : 0u;
+                        });
+                        uint32_t tot;
+                        block_excl_scan(cnt, sh, tot);
+                        if (tot <= (uint32_t)FOLD_CAP)
+                            tlo = mid;
+                        else
+                            thi = mid - 1ull;
+                    }
+                    const uint64_t T = tlo;
+                    if (tid == 0)
+                        s_m = 0;
+                    __syncthreads();
+                    for_each_rec(a, brick, cb, [&](const Rec &r) {
+                        if (r.lvox == lv && (!have_last || r.key > last) && r.key <= T)
+                        {
+                            const uint32_t p = atomicAdd(&s_m, 1u);
+                            skey[p] = r.key;
+                            sz[p] = r.z;
+                        }
+                    });
+                    const uint32_t m = s_m;
+                    for (uint32_t i = tid; i < m; i += BT)
+                    {
+                        const uint64_t k = skey[i];
+                        uint32_t rank = 0;
+                        for (uint32_t t = 0; t < m; ++t)
+                            rank += (skey[t] < k) ? 1u : 0u;
+                        sperm[rank] = (uint16_t)i;
+                    }
+                    __syncthreads();
+                    if (tid == 0)
+                        for (uint32_t i = 0; i < m; ++i)
+                            st.add(skey[sperm[i]], sz[sperm[i]], a);
+                    __syncthreads();
+                    last = T;
+                    have_last = true;
+                    done += m;
+                    if (m == 0)
+                        break; // cannot happen with unique keys; keeps the loop finite
+                }
+                if (tid == 0)
+                {
+                    st.finish(x0 + (int)(lv & 7), y0 + (int)((lv >> 3) & 7), z0 + (int)(lv >> 6), a);
+                    atomicAdd(a.ctr + CTR_GIANT, 1ull);
+                }
+                __syncthreads();
+                ++lo;
+                continue;
+            }
+            // largest hi with off[hi] - off[lo] <= FOLD_CAP
+            int l = lo + 1, h = 512;
             while (l < h)
             {
                 const int mid = (l + h + 1) >> 1;
-                if (soff[mid] - base <= i)
+                if (off[mid] - off[lo] <= (uint32_t)FOLD_CAP)
                     l = mid;
                 else
                     h = mid - 1;
             }
-            const uint32_t s0 = soff[l] - base, s1 = soff[l + 1] - base;
-            const uint64_t k = skey[i];
-            uint32_t rank = 0;
-            for (uint32_t t = s0; t < s1; ++t)
-                rank += (skey[t] < k) ? 1u : 0u;
-            sperm[s0 + rank] = (uint16_t)i;
+            const int hi = l;
+            const uint32_t base = off[lo];
+            const uint32_t nrec = off[hi] - base;
+            for (int i = tid; i < 512; i += BT)
+                cur[i] = 0;
+            __syncthreads();
+            // counting sort of the range's records by voxel
+            for_each_rec(a, brick, cb, [&](const Rec &r) {
+                if (r.lvox >= (uint32_t)lo && r.lvox < (uint32_t)hi)
+                {
+                    const uint32_t p = off[r.lvox] - base + atomicAdd(&cur[r.lvox], 1u);
+                    skey[p] = r.key;
+                    sz[p] = r.z;
+                }
+            });
+            // rank of each record among its voxel's keys (keys are unique)
+            for (uint32_t i = tid; i < nrec; i += BT)
+            {
+                int vl = lo, vh = hi - 1;
+                while (vl < vh)
+                {
+                    const int mid = (vl + vh + 1) >> 1;
+                    if (off[mid] - base <= i)
+                        vl = mid;
+                    else
+                        vh = mid - 1;
+                }
+                const uint32_t s0 = off[vl] - base, s1 = off[vl + 1] - base;
+                const uint64_t k = skey[i];
+                uint32_t rank = 0;
+                for (uint32_t t = s0; t < s1; ++t)
+                    rank += (skey[t] < k) ? 1u : 0u;
+                sperm[s0 + rank] = (uint16_t)i;
+            }
+            __syncthreads();
+            for (int lv = lo + tid; lv < hi; lv += BT)
+            {
+                const uint32_t s0 = off[lv] - base, s1 = off[lv + 1] - base;
+                if (s1 == s0)
+                    continue;
+                FoldState st;
+                st.init();
+                for (uint32_t i = s0; i < s1; ++i)
+                {
+                    const uint32_t j = sperm[i];
+                    st.add(skey[j], sz[j], a);
+                }
+                st.finish(x0 + (lv & 7), y0 + ((lv >> 3) & 7), z0 + (lv >> 6), a);
+            }
+            __syncthreads();
+            lo = hi;
         }
         __syncthreads();
-        if (tid >= cv && tid < ce)
-        {
-            const uint32_t s0 = soff[tid] - base;
-            const int n = (int)(soff[tid + 1] - soff[tid]);
-            fold_solve((uint32_t)(v0 + tid), n,
-                       [&](int i, uint64_t &key, float &z) {
-                           const uint32_t j = sperm[s0 + i];
-                           key = skey[j];
-                           z = sz[j];
-                       },
-                       a);
-        }
-        __syncthreads();
-        cv = ce;
     }
 }
 
@@ -1527,27 +1709,70 @@ hipError_t launch_prep(const MeshDev &m, int64_t V, const GridDev &g, float4 *pl
     return hipGetLastError();
 }
 
-hipError_t launch_scan_events(int mode, const MeshDev &m, const GridDev &g, const RecArgs &r,
-                              unsigned long long *ctr, hipStream_t s)
+hipError_t launch_scan_events(const MeshDev &m, const GridDev &g, unsigned long long *ctr, const ShardBuf<Event> &eb,
+                              hipStream_t s)
 {
-    const unsigned nb = grid_for(3 * m.F);
-    if (mode == MODE_MARK)
-        hipLaunchKernelGGL(k_scan_events<MODE_MARK>, dim3(nb), dim3(BT), 0, s, m, g, r, ctr);
-    else if (mode == MODE_COUNT)
-        hipLaunchKernelGGL(k_scan_events<MODE_COUNT>, dim3(nb), dim3(BT), 0, s, m, g, r, ctr);
-    else
-        hipLaunchKernelGGL(k_scan_events<MODE_EMIT>, dim3(nb), dim3(BT), 0, s, m, g, r, ctr);
+    hipLaunchKernelGGL(k_scan_events, dim3(grid_for(3 * m.F)), dim3(BT), 0, s, m, g, ctr, eb);
     return hipGetLastError();
 }
 
-hipError_t launch_face(int mode, const MeshDev &m, const GridDev &g, const RecArgs &r, unsigned long long *ctr,
+hipError_t launch_face(const MeshDev &m, const GridDev &g, const ShardBuf<FHit> &hb, unsigned long long *ctr,
                        hipStream_t s)
 {
-    const unsigned nb = grid_for(m.F);
-    if (mode == MODE_COUNT)
-        hipLaunchKernelGGL(k_face<MODE_COUNT>, dim3(nb), dim3(BT), 0, s, m, g, r, ctr);
-    else
-        hipLaunchKernelGGL(k_face<MODE_EMIT>, dim3(nb), dim3(BT), 0, s, m, g, r, ctr);
+    hipLaunchKernelGGL(k_face, dim3(grid_for(m.F)), dim3(BT), 0, s, m, g, hb, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_gen_fhits(const ShardBuf<FHit> &h, const GridDev &g, const BucketArgs &ba, hipStream_t s)
+{
+    const uint64_t n = (uint64_t)NSHARD * h.per;
+    hipLaunchKernelGGL(k_gen_fhits, dim3(grid_for((int64_t)((n + 3) / 4), 16384)), dim3(BT), 0, s, h, g, ba);
+    return hipGetLastError();
+}
+
+hipError_t launch_gen_events(const ShardBuf<Event> &ev, const GridDev &g, const BucketArgs &ba, hipStream_t s)
+{
+    const uint64_t n = (uint64_t)NSHARD * ev.per;
+    hipLaunchKernelGGL(k_gen_events, dim3(grid_for((int64_t)n, 16384)), dim3(BT), 0, s, ev, g, ba);
+    return hipGetLastError();
+}
+
+hipError_t launch_gen_bhits(const ShardBuf<BHit> &h, const GridDev &g, const BucketArgs &ba, hipStream_t s)
+{
+    const uint64_t n = (uint64_t)NSHARD * h.per;
+    hipLaunchKernelGGL(k_gen_bhits, dim3(grid_for((int64_t)n, 4096)), dim3(BT), 0, s, h, g, ba);
+    return hipGetLastError();
+}
+
+hipError_t launch_shard_reduce(unsigned long long *shard_ctr, unsigned long long *ctr, hipStream_t s)
+{
+    static_assert(NSHARD == BT, "one thread per shard");
+    hipLaunchKernelGGL(k_shard_reduce, dim3(SB_N), dim3(BT), 0, s, shard_ctr, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_place_chunks(const ShardBuf<Chunk> &chunks, const uint32_t *choff, uint32_t *bchunks,
+                               hipStream_t s)
+{
+    const uint64_t n = (uint64_t)NSHARD * chunks.per;
+    hipLaunchKernelGGL(k_place_chunks, dim3(grid_for((int64_t)n, 16384)), dim3(BT), 0, s, chunks, choff, bchunks);
+    return hipGetLastError();
+}
+
+hipError_t launch_active_bricks(const uint32_t *nch, uint64_t n_bricks, uint32_t *list, unsigned long long *count,
+                                hipStream_t s)
+{
+    hipLaunchKernelGGL(k_active_bricks, dim3(grid_for((int64_t)n_bricks, 65536)), dim3(BT), 0, s, nch, n_bricks,
+                       list, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_fold_bricks(const FoldArgs &a, uint64_t max_bricks, hipStream_t s)
+{
+    uint64_t nb = max_bricks < 65536ull ? max_bricks : 65536ull;
+    if (nb == 0)
+        nb = 1;
+    hipLaunchKernelGGL(k_fold_bricks, dim3((unsigned)nb), dim3(BT), 0, s, a);
     return hipGetLastError();
 }
 
@@ -1571,14 +1796,12 @@ hipError_t launch_bedge_prep(const float *verts, int64_t E, const int2 *bedge_v,
     return hipGetLastError();
 }
 
-hipError_t launch_bdry(int mode, int64_t E, const BEdge *be, const GridDev &g, const RecArgs &r,
+hipError_t launch_bdry(int64_t E, const BEdge *be, const GridDev &g, const ShardBuf<BHit> &hb,
                        unsigned long long *ctr, hipStream_t s)
 {
-    const unsigned nb = grid_for(E);
-    if (mode == MODE_COUNT)
-        hipLaunchKernelGGL(k_bdry<MODE_COUNT>, dim3(nb), dim3(BT), 0, s, E, be, g, r, ctr);
-    else
-        hipLaunchKernelGGL(k_bdry<MODE_EMIT>, dim3(nb), dim3(BT), 0, s, E, be, g, r, ctr);
+    if (E == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_bdry, dim3(grid_for(E)), dim3(BT), 0, s, E, be, g, hb, ctr);
     return hipGetLastError();
 }
 
@@ -1623,15 +1846,6 @@ hipError_t launch_scan_u32(const uint32_t *in, uint64_t n, uint32_t *out, uint64
 hipError_t launch_emit_coords(const GridDev &g, uint64_t n_words, int32_t *coords, hipStream_t s)
 {
     hipLaunchKernelGGL(k_emit_coords, dim3(grid_for((int64_t)n_words, 65536)), dim3(BT), 0, s, g, n_words, coords);
-    return hipGetLastError();
-}
-
-hipError_t launch_fold_solve(const FoldArgs &a, hipStream_t s)
-{
-    const uint64_t nb = (a.N + BT - 1) / BT;
-    if (nb == 0)
-        return hipSuccess;
-    hipLaunchKernelGGL(k_fold_solve, dim3((unsigned)nb), dim3(BT), 0, s, a);
     return hipGetLastError();
 }
 

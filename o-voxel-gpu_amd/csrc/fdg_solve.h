@@ -1,0 +1,348 @@
+// QEF solve of one voxel (FDG.cpp:763-983), device side.
+// cpqr_solve restates Eigen's ColPivHouseholderQR (pivoting, LAWN-176 norm
+// downdate, nonzero-pivot threshold, Householder application, column-oriented
+// back substitution) for n = 2, 3; solve_voxel adds the box-constrained search.
+#pragma once
+
+#include "fdg_device.h"
+
+namespace ovx
+{
+
+// ---------------------------------------------------------------------------
+template <int n>
+__device__ __forceinline__ void cpqr_solve(const float (&A)[n][n], const float (&b)[n], float (&x)[n])
+{
+    // one spare row/column keeps the unrolled (dead) tail indices in bounds
+    float qr[n + 1][n + 1];
+#pragma unroll
+    for (int i = 0; i <= n; ++i)
+#pragma unroll
+        for (int j = 0; j <= n; ++j)
+            qr[i][j] = (i < n && j < n) ? A[i][j] : 0.0f;
+    float hco[n];
+    int transp[n];
+    float nu[n], nd[n];
+#pragma unroll
+    for (int k = 0; k < n; ++k)
+    {
+        float s;
+        if constexpr (n == 3)
+            s = qr[0][k] * qr[0][k] + (qr[1][k] * qr[1][k] + qr[2][k] * qr[2][k]);
+        else
+            s = qr[0][k] * qr[0][k] + qr[1][k] * qr[1][k];
+        nd[k] = sqrtf(s);
+        nu[k] = nd[k];
+    }
+    float maxn = nu[0];
+#pragma unroll
+    for (int k = 1; k < n; ++k)
+        if (nu[k] > maxn)
+            maxn = nu[k];
+    const float me = maxn * FLT_EPSILON;
+    const float th = (me * me) / (float)n;
+    const float ndt = sqrtf(FLT_EPSILON);
+    int nzp = n;
+#pragma unroll
+    for (int k = 0; k < n; ++k)
+    {
+        int big = k;
+        float bigv = nu[k];
+#pragma unroll
+        for (int j = k + 1; j < n; ++j)
+            if (nu[j] > bigv)
+            {
+                bigv = nu[j];
+                big = j;
+            }
+        if (nzp == n && bigv * bigv < th * (float)(n - k))
+            nzp = k;
+        transp[k] = big;
+#pragma unroll
+        for (int j = k + 1; j < n; ++j)
+            if (j == big)
+            {
+#pragma unroll
+                for (int rr = 0; rr < n; ++rr)
+                {
+                    const float t = qr[rr][k];
+                    qr[rr][k] = qr[rr][j];
+                    qr[rr][j] = t;
+                }
+                float t = nu[k];
+                nu[k] = nu[j];
+                nu[j] = t;
+                t = nd[k];
+                nd[k] = nd[j];
+                nd[j] = t;
+            }
+        const int tl = n - k - 1;
+        float tail_sq = 0.0f;
+        if (tl > 0)
+        {
+            tail_sq = qr[k + 1][k] * qr[k + 1][k];
+#pragma unroll
+            for (int i = 1; i < n - k - 1; ++i)
+                tail_sq = tail_sq + qr[k + 1 + i][k] * qr[k + 1 + i][k];
+        }
+        const float c0 = qr[k][k];
+        float beta, tau;
+        if (tail_sq <= FLT_MIN)
+        {
+            tau = 0.0f;
+            beta = c0;
+#pragma unroll
+            for (int i = 0; i < n - k - 1; ++i)
+                qr[k + 1 + i][k] = 0.0f;
+        }
+        else
+        {
+            beta = sqrtf(c0 * c0 + tail_sq);
+            if (c0 >= 0.0f)
+                beta = -beta;
+            const float den = c0 - beta;
+#pragma unroll
+            for (int i = 0; i < n - k - 1; ++i)
+                qr[k + 1 + i][k] = qr[k + 1 + i][k] / den;
+            tau = (beta - c0) / beta;
+        }
+        hco[k] = tau;
+        qr[k][k] = beta;
+        if (n - k > 1 && tau != 0.0f)
+        {
+#pragma unroll
+            for (int j = k + 1; j < n; ++j)
+            {
+                float tmp = qr[k + 1][k] * qr[k + 1][j];
+#pragma unroll
+                for (int i = 1; i < n - k - 1; ++i)
+                    tmp = tmp + qr[k + 1 + i][k] * qr[k + 1 + i][j];
+                tmp = tmp + qr[k][j];
+                qr[k][j] = qr[k][j] - tau * tmp;
+#pragma unroll
+                for (int i = 0; i < n - k - 1; ++i)
+                {
+                    const float te = tau * qr[k + 1 + i][k];
+                    qr[k + 1 + i][j] = qr[k + 1 + i][j] - tmp * te;
+                }
+            }
+        }
+#pragma unroll
+        for (int j = k + 1; j < n; ++j)
+        {
+            if (nu[j] != 0.0f)
+            {
+                const float a = fabsf(qr[k][j]);
+                float temp = a / nu[j];
+                temp = (1.0f + temp) * (1.0f - temp);
+                temp = temp < 0.0f ? 0.0f : temp;
+                const float ratio = nu[j] / nd[j];
+                const float temp2 = temp * (ratio * ratio);
+                if (temp2 <= ndt)
+                {
+                    float s = 0.0f;
+                    if (k + 1 < n)
+                    {
+                        s = qr[k + 1][j] * qr[k + 1][j];
+#pragma unroll
+                        for (int i = k + 2; i < n; ++i)
+                            s = s + qr[i][j] * qr[i][j];
+                        s = sqrtf(s);
+                    }
+                    nd[j] = s;
+                    nu[j] = s;
+                }
+                else
+                {
+                    nu[j] = nu[j] * sqrtf(temp);
+                }
+            }
+        }
+    }
+    int perm[n];
+#pragma unroll
+    for (int i = 0; i < n; ++i)
+        perm[i] = i;
+#pragma unroll
+    for (int k = 0; k < n; ++k)
+    {
+#pragma unroll
+        for (int j = k + 1; j < n; ++j)
+            if (transp[k] == j)
+            {
+                const int t = perm[k];
+                perm[k] = perm[j];
+                perm[j] = t;
+            }
+    }
+    const int rk = nzp;
+    float c[n + 1];
+#pragma unroll
+    for (int i = 0; i < n; ++i)
+        c[i] = b[i];
+    c[n] = 0.0f;
+    if (rk == 0)
+    {
+#pragma unroll
+        for (int i = 0; i < n; ++i)
+            x[i] = 0.0f;
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < n; ++k)
+    {
+        if (k < rk)
+        {
+            const float tau = hco[k];
+            if (n - k == 1)
+            {
+                c[k] = c[k] * (1.0f - tau);
+            }
+            else if (tau != 0.0f)
+            {
+                float tmp = qr[k + 1][k] * c[k + 1];
+#pragma unroll
+                for (int i = 1; i < n - k - 1; ++i)
+                    tmp = tmp + qr[k + 1 + i][k] * c[k + 1 + i];
+                tmp = tmp + c[k];
+                c[k] = c[k] - tau * tmp;
+#pragma unroll
+                for (int i = 0; i < n - k - 1; ++i)
+                {
+                    const float te = tau * qr[k + 1 + i][k];
+                    c[k + 1 + i] = c[k + 1 + i] - tmp * te;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int i = n - 1; i >= 0; --i)
+    {
+        if (i < rk && c[i] != 0.0f)
+        {
+            c[i] = c[i] / qr[i][i];
+#pragma unroll
+            for (int s = 0; s < i; ++s)
+                c[s] = c[s] - c[i] * qr[s][i];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < n; ++j)
+    {
+        float v = 0.0f;
+#pragma unroll
+        for (int i = 0; i < n; ++i)
+            if (perm[i] == j && i < rk)
+                v = c[i];
+        x[j] = v;
+    }
+}
+
+// p^T Q p in the Packet4f predux order (x0 + x2) + (x1 + x3)
+__device__ __forceinline__ float quad_form(const float p[4], const float Q[4][4])
+{
+    float t[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        t[j] = (p[0] * Q[0][j] + p[2] * Q[2][j]) + (p[1] * Q[1][j] + p[3] * Q[3][j]);
+    return (t[0] * p[0] + t[2] * p[2]) + (t[1] * p[1] + t[3] * p[3]);
+}
+
+__device__ __forceinline__ void solve_voxel(const float q[10], const int coord[3], const float vs[3], float out[3])
+{
+    float Q[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            Q[i][j] = q[qidx(i, j)];
+    float mn[3], mx[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        mn[a] = (float)coord[a] * vs[a];
+        mx[a] = (float)(coord[a] + 1) * vs[a];
+    }
+    const float A[3][3] = {{Q[0][0], Q[0][1], Q[0][2]}, {Q[1][0], Q[1][1], Q[1][2]}, {Q[2][0], Q[2][1], Q[2][2]}};
+    const float bv[3] = {-Q[0][3], -Q[1][3], -Q[2][3]};
+    float v[3];
+    cpqr_solve<3>(A, bv, v);
+    if (!(v[0] >= mn[0] && v[0] <= mx[0] && v[1] >= mn[1] && v[1] <= mx[1] && v[2] >= mn[2] && v[2] <= mx[2]))
+    {
+        float best = __builtin_huge_valf();
+        auto consider = [&](const float p[4]) {
+            const float err = quad_form(p, Q);
+            if (err < best)
+            {
+                best = err;
+                v[0] = p[0];
+                v[1] = p[1];
+                v[2] = p[2];
+            }
+        };
+#pragma unroll
+        for (int fixed = 0; fixed < 3; ++fixed)
+        {
+            const int a1 = (fixed + 1) % 3, a2 = (fixed + 2) % 3;
+            const float A2[2][2] = {{Q[a1][a1], Q[a1][a2]}, {Q[a2][a1], Q[a2][a2]}};
+            const float B[2][2] = {{Q[a1][fixed], Q[a1][3]}, {Q[a2][fixed], Q[a2][3]}};
+#pragma unroll
+            for (int side = 0; side < 2; ++side)
+            {
+                const float q0 = side ? mx[fixed] : mn[fixed];
+                const float rhs[2] = {(-B[0][0]) * q0 + (-B[0][1]) * 1.0f, (-B[1][0]) * q0 + (-B[1][1]) * 1.0f};
+                float x2[2];
+                cpqr_solve<2>(A2, rhs, x2);
+                if (x2[0] >= mn[a1] && x2[0] <= mx[a1] && x2[1] >= mn[a2] && x2[1] <= mx[a2])
+                {
+                    float p[4];
+                    p[fixed] = q0;
+                    p[a1] = x2[0];
+                    p[a2] = x2[1];
+                    p[3] = 1.0f;
+                    consider(p);
+                }
+            }
+        }
+#pragma unroll
+        for (int fr = 0; fr < 3; ++fr)
+        {
+            const int a1 = (fr + 1) % 3, a2 = (fr + 2) % 3;
+            const float a = Q[fr][fr];
+            const float b0 = Q[fr][a1], b1 = Q[fr][a2], b2 = Q[fr][3];
+#pragma unroll
+            for (int s1 = 0; s1 < 2; ++s1)
+#pragma unroll
+                for (int s2 = 0; s2 < 2; ++s2)
+                {
+                    const float q1 = s1 ? mx[a1] : mn[a1];
+                    const float q2 = s2 ? mx[a2] : mn[a2];
+                    const float xv = -(dot3f(b0, b1, b2, q1, q2, 1.0f)) / a;
+                    if (xv >= mn[fr] && xv <= mx[fr])
+                    {
+                        float p[4];
+                        p[fr] = xv;
+                        p[a1] = q1;
+                        p[a2] = q2;
+                        p[3] = 1.0f;
+                        consider(p);
+                    }
+                }
+        }
+#pragma unroll
+        for (int cx = 0; cx < 2; ++cx)
+#pragma unroll
+            for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+                for (int cz = 0; cz < 2; ++cz)
+                {
+                    const float p[4] = {cx ? mn[0] : mx[0], cy ? mn[1] : mx[1], cz ? mn[2] : mx[2], 1.0f};
+                    consider(p);
+                }
+    }
+    out[0] = v[0];
+    out[1] = v[1];
+    out[2] = v[2];
+}
+
+} // namespace ovx

@@ -32,6 +32,7 @@ __all__ = [
     "stage_times",
     "set_timing",
     "kernel_times",
+    "set_capacity",
     "workspace_bytes",
 ]
 
@@ -81,6 +82,8 @@ def _load():
         L.ovx_fdg_stats.argtypes = [vp, ctypes.POINTER(i64)]
         L.ovx_fdg_set_timing.argtypes = [vp, ctypes.c_int]
         L.ovx_fdg_stage_times.argtypes = [vp, ctypes.POINTER(f32)]
+        L.ovx_fdg_set_capacity.argtypes = [vp, i64, i64]
+        L.ovx_fdg_set_capacity.restype = ctypes.c_int
         L.ovx_fdg_kernel_count.restype = ctypes.c_int
         L.ovx_fdg_kernel_name.argtypes = [ctypes.c_int]
         L.ovx_fdg_kernel_name.restype = ctypes.c_char_p
@@ -312,6 +315,14 @@ def kernel_times(device=None) -> dict:
     with torch.cuda.device(dev), ctx.lock:
         _check(L.ovx_fdg_kernel_times(ctx.h, out, n), "ovx_fdg_kernel_times")
     return {L.ovx_fdg_kernel_name(k).decode(): float(out[k]) for k in range(n) if out[k] >= 0.0}
+
+
+def set_capacity(device=None, events: int = 0, hits: int = 0):
+    """Capacity of the crossing / face-hit buffers for the next conversion (0 keeps)."""
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    ctx = _ctx(dev)
+    with torch.cuda.device(dev), ctx.lock:
+        _check(_load().ovx_fdg_set_capacity(ctx.h, int(events), int(hits)), "ovx_fdg_set_capacity")
 
 
 def workspace_bytes(device=None) -> int:

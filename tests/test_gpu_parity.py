@@ -165,3 +165,19 @@ def test_determinism(oracle):
     b = run_gpu(vl, f, vs, gr, 1.0, 0.2, 1e-2)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+
+
+def test_buffer_overflow_fallbacks(oracle):
+    """Tiny crossing / hit buffers force the re-recording of crossings and the
+    traversal-based emit; the results must not change."""
+    v, f = meshgen.scene(seed=6, target_faces=12000)
+    vs, gr, a0 = meshgen.grid_args(128)
+    vl = (v - a0).astype(np.float32)
+    base = run_gpu(vl, f, vs, gr, 1.0, 0.2, 1e-2)
+    dev = _dev()
+    _C().set_capacity(dev, events=1000, hits=1000)
+    small = run_gpu(vl, f, vs, gr, 1.0, 0.2, 1e-2)   # both buffers overflow
+    again = run_gpu(vl, f, vs, gr, 1.0, 0.2, 1e-2)   # grown buffers
+    for x, y, z in zip(base, small, again):
+        assert np.array_equal(x, y) and np.array_equal(x, z)
+    compare(oracle, vl, f, vs, gr, label="after overflow")
