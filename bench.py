@@ -71,7 +71,12 @@ def make_workload(name, grid):
 def algorithmic_bytes(slot, st, F, V, n_words):
     """Compulsory HBM bytes of one launch of a kernel slot."""
     N, R, E, ev = st["voxels"], st["records"], st["boundary_edges"], st["events"]
-    FR = st["face_records"]
+    FR, BR = st["face_records"], st["boundary_records"]
+    # the fold reads every record once, the planes and boundary quadrics once,
+    # looks up each voxel's id (bitset word + word offset) and writes dual + flags;
+    # the two fold kernels split it in proportion to the records each folds
+    fold_all = 16 * R + 16 * F + 88 * E + 12 * N + 15 * N
+    large_share = st["large_brick_records"] / R if R else 0.0
     table = {
         # faces (12 B) + vertices (12 B) read once, 16 B plane per face written
         "prep": 12 * F + 12 * V + 16 * F,
@@ -85,10 +90,17 @@ def algorithmic_bytes(slot, st, F, V, n_words):
         "coords": 12 * n_words + 12 * N,
         # crossings read, 4 records of 16 B written per crossing
         "gen_events": 16 * ev + 64 * ev,
-        # faces + vertices + bitset read, one 16 B record per face hit written
-        "gen_face": 12 * F + 12 * V + 8 * n_words + 16 * FR,
-        # records + planes + boundary quadrics read, voxel ids looked up, dual + flags written
-        "fold_solve": 16 * R + 16 * F + 56 * E + 12 * N + 15 * N,
+        # faces + vertices + bitset read, one 12 B hit per face / voxel overlap written
+        "face": 12 * F + 12 * V + 8 * n_words + 12 * FR,
+        # hits read, one 16 B record per hit written
+        "gen_fhits": 12 * FR + 16 * FR,
+        # boundary segments (88 B) read, one 16 B hit per voxel on the segment written
+        "bdry": 88 * E + 16 * BR,
+        "gen_bhits": 16 * BR + 16 * BR,
+        "fold": (1.0 - large_share) * fold_all,
+        "fold_large": large_share * fold_all,
+        # queued voxels (72 B) read, dual (12 B) written
+        "solve_box": 84 * st["box_voxels"],
     }
     return table.get(slot)
 
@@ -265,6 +277,7 @@ def main():
         "path_alg_GBps": round(path_alg / (ms_per_step * 1e-3) / 1e9, 3),
         "kernels_ms": {k: round(x, 4) for k, x in sorted(kern_avg.items(), key=lambda kv: -kv[1])},
         "peak_mem_mb": round(peak_mb, 1),
+        "counts": {k: v for k, v in st.items() if k != "workspace_bytes"},
     }
     print(json.dumps(line), flush=True)
 

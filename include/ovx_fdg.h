@@ -102,13 +102,19 @@ ovx_status ovx_occ_begin(ovx_fdg_ctx *ctx, const float *d_triangles, int64_t n_t
 ovx_status ovx_occ_finish(ovx_fdg_ctx *ctx, int32_t *d_coords, void *stream);
 
 /*
- * Per-stage statistics of the last conversion on this context:
+ * Per-stage statistics of the last conversion on this context; the first
+ * min(n, OVX_FDG_N_STATS) are written:
  *   out[0] N voxels          out[1] scanline events     out[2] QEF records
  *   out[3] face records      out[4] boundary edges      out[5] boundary records
  *   out[6] workspace bytes   out[7] voxels solved on the slow (giant) path
+ *   out[8] bricks with records                         out[9] record chunks
+ *   out[10] bricks folded by the large-brick kernel    out[11] their records
+ *   out[12] voxels queued for the box-constrained search (the rest of the voxels
+ *           whose minimiser left the box are searched in place)
  * Reading them synchronises the context's last stream.
  */
-ovx_status ovx_fdg_stats(ovx_fdg_ctx *ctx, int64_t *out8);
+#define OVX_FDG_N_STATS 13
+ovx_status ovx_fdg_stats(ovx_fdg_ctx *ctx, int64_t *out, int n);
 
 /*
  * Capacities (in items) for the next conversion on this context: the

@@ -72,10 +72,10 @@ struct ovx_fdg_ctx
     std::mutex mu;
     // workspace
     DevBuf planes, bits, woff, tiles, counters;
-    DevBuf vcnt, voff, nbr, bcnt, boff, bedge_v, bedges;
-    DevBuf events, recs, chunks, heads, choff, bchunks, active, bhits, fhits, shard_ctr;
-    uint64_t per[SB_N] = {0, 0, 0, 0, 0}; // per-shard capacities of the sharded buffers (grow-only)
-    uint64_t force_per[SB_N] = {0, 0, 0, 0, 0}; // one-shot overrides (tests, tuning)
+    DevBuf vcnt, voff, nbr, erank, bcnt, boff, bedge_v, bedges;
+    DevBuf events, recs, chunks, heads, choff, bchunks, active, large, box, qef, bhits, fhits, shard_ctr;
+    uint64_t per[SB_N] = {};       // per-shard capacities of the sharded buffers (grow-only)
+    uint64_t force_per[SB_N] = {}; // one-shot overrides (tests, tuning)
     double face_per_face = 8.0;           // face records per face seen last time (capacity estimate)
     uint64_t n_events = 0;                // crossings recorded by the last mark pass
     int64_t E = 0;           // boundary edges of the last begin
@@ -89,12 +89,24 @@ struct ovx_fdg_ctx
     float fw = 0, bw = 0, rw = 0;
     int64_t N = 0;
     // statistics of the last conversion
-    int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    hipStream_t last_stream = nullptr;
+    int64_t stats[OVX_FDG_N_STATS] = {};
+    hipStream_t last_stream = nullptr; // stream of the last conversion (the null stream is valid)
+    bool ran = false;                 // a conversion was started on this context
     // per-kernel timing (events around each launch on the caller's stream)
     bool timing = false;
     hipEvent_t kev[2 * 16] = {};
     bool kran[16] = {};
+
+    // every workspace buffer except the counters
+    template <class Fn>
+    void for_each_buf(Fn &&fn)
+    {
+        DevBuf *all[] = {&planes, &bits,   &woff,  &tiles,   &vcnt,   &voff,  &nbr, &erank, &bcnt, &boff,
+                         &bedge_v, &bedges, &events, &recs,   &chunks, &heads, &choff, &bchunks, &active, &large,
+                         &box,     &qef,    &bhits,  &fhits,  &shard_ctr};
+        for (DevBuf *b : all)
+            fn(*b);
+    }
 };
 
 namespace
@@ -127,14 +139,10 @@ void release(DevBuf &b)
     b.bytes = 0;
 }
 
-int64_t held_bytes(const ovx_fdg_ctx *c)
+int64_t held_bytes(ovx_fdg_ctx *c)
 {
-    const DevBuf *all[] = {&c->planes, &c->bits, &c->woff,  &c->tiles, &c->counters, &c->vcnt,   &c->voff,
-                           &c->nbr,    &c->bcnt, &c->boff,  &c->bedge_v, &c->bedges, &c->events, &c->recs,
-                           &c->chunks, &c->heads, &c->choff, &c->bchunks, &c->active, &c->bhits, &c->fhits, &c->shard_ctr};
-    int64_t s = 0;
-    for (const DevBuf *b : all)
-        s += (int64_t)b->bytes;
+    int64_t s = (int64_t)c->counters.bytes;
+    c->for_each_buf([&](DevBuf &b) { s += (int64_t)b.bytes; });
     return s;
 }
 
@@ -197,14 +205,20 @@ enum KSlot
     KS_EDGES,
     KS_COORDS,
     KS_GEN_EVENTS,
-    KS_GEN_FACE,
-    KS_GEN_BDRY,
+    KS_FACE,
+    KS_GEN_FHITS,
+    KS_BDRY,
+    KS_GEN_BHITS,
     KS_ACTIVE,
     KS_FOLD,
+    KS_FOLD_LARGE,
+    KS_SOLVE,
+    KS_SOLVE_BOX,
     KS_N
 };
-const char *const kslot_names[KS_N] = {"prep",      "mark",     "word_scan", "edges",  "coords",
-                                       "gen_events", "gen_face", "gen_bdry",  "active", "fold_solve"};
+static_assert(2 * KS_N <= 32, "timing events");
+const char *const kslot_names[KS_N] = {"prep",      "mark", "word_scan", "edges",     "coords", "gen_events", "face",
+                                       "gen_fhits", "bdry", "gen_bhits", "active",    "fold",   "fold_large", "solve", "solve_box"};
 
 void tic(ovx_fdg_ctx *c, int k, hipStream_t s)
 {
@@ -257,7 +271,7 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int64_t V, const ovx_g
 {
     c->pending = 0;
     c->N = 0;
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < OVX_FDG_N_STATS; ++i)
         c->stats[i] = 0;
     GridDev g{};
     uint64_t n_words = 0;
@@ -270,12 +284,15 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int64_t V, const ovx_g
         return fail(OVX_ERR_CAPACITY, "more than 2^31-1 faces");
     if (m.F > 0 && !m.v)
         return fail(OVX_ERR_INVALID_ARGUMENT, "vertices pointer is null");
+    if (m.F > 0 && m.f && V == 0)
+        return fail(OVX_ERR_INPUT, "faces contain a vertex index outside [0, num_vertices)");
     if ((st = ensure(c->counters, CTR_COUNT * sizeof(unsigned long long))))
         return st;
     HIP_TRY(hipMemsetAsync(c->counters.p, 0, CTR_COUNT * sizeof(unsigned long long), s));
     c->mesh = m;
     c->V = V;
     c->last_stream = s;
+    c->ran = true;
     if (m.F == 0)
     {
         c->grid = g;
@@ -326,7 +343,8 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int64_t V, const ovx_g
         if ((uint64_t)m.F * 3ull >= (1ull << 32))
             return fail(OVX_ERR_CAPACITY, "more than 2^32 half-edges");
         if ((st = ensure(c->vcnt, (size_t)(V + 1) * 4)) || (st = ensure(c->voff, (size_t)(V + 1) * 4)) ||
-            (st = ensure(c->nbr, (size_t)m.F * 3 * 4)) || (st = ensure(c->bcnt, (size_t)(V + 1) * 4)) ||
+            (st = ensure(c->nbr, (size_t)m.F * 3 * 4)) || (st = ensure(c->erank, (size_t)m.F * 3 * 4)) ||
+            (st = ensure(c->bcnt, (size_t)(V + 1) * 4)) ||
             (st = ensure(c->boff, (size_t)(V + 1) * 4)) ||
             (st = ensure(c->tiles, (scan_tile_count(n_words) + 2 * scan_tile_count((uint64_t)V + 1) + 64) * 8)))
             return st;
@@ -334,9 +352,10 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int64_t V, const ovx_g
         auto *voff = (uint32_t *)c->voff.p;
         tic(c, KS_EDGES, s);
         HIP_TRY(hipMemsetAsync(vcnt, 0, (size_t)(V + 1) * 4, s));
-        HIP_TRY(launch_edge_count(m, vcnt, voff, (uint32_t *)c->nbr.p, 0, s));
+        auto *erank = (uint32_t *)c->erank.p;
+        HIP_TRY(launch_edge_count(m, vcnt, erank, voff, (uint32_t *)c->nbr.p, 0, s));
         HIP_TRY(launch_scan_u32(vcnt, (uint64_t)V + 1, voff, (uint64_t *)c->tiles.p, ctr + CTR_EDGE_TOTAL, s));
-        HIP_TRY(launch_edge_count(m, vcnt, voff, (uint32_t *)c->nbr.p, 1, s));
+        HIP_TRY(launch_edge_count(m, vcnt, erank, voff, (uint32_t *)c->nbr.p, 1, s));
         HIP_TRY(hipMemsetAsync(c->bcnt.p, 0, (size_t)(V + 1) * 4, s));
         HIP_TRY(launch_edge_singletons(V, voff, (uint32_t *)c->nbr.p, (uint32_t *)c->bcnt.p, nullptr, nullptr, 0, s));
         HIP_TRY(launch_scan_u32((uint32_t *)c->bcnt.p, (uint64_t)V + 1, (uint32_t *)c->boff.p,
@@ -351,7 +370,7 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int64_t V, const ovx_g
     const unsigned long long N = c->h_ctr[CTR_WORDS_TOTAL];
     if (N >= (1ull << 32) - 1)
         return fail(OVX_ERR_CAPACITY, "%llu occupied voxels exceed the 32-bit voxel index", N);
-    c->n_events = c->h_ctr[CTR_EVENTS];
+    c->n_events = planes ? c->h_ctr[CTR_SUM + SB_EVENTS] : 0;
     if (planes)
     {
         if (c->h_ctr[CTR_MAX + SB_EVENTS] > c->per[SB_EVENTS])
@@ -427,7 +446,6 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
         if ((st = clear_shards(c, SB_FHITS, SB_N - SB_FHITS, s)))
             return st;
         HIP_TRY(hipMemsetAsync(c->heads.p, 0, (n_bricks + 1) * 4, s));
-        HIP_TRY(hipMemsetAsync(ctr + CTR_REC_FACE, 0, 2 * sizeof(unsigned long long), s)); // FACE, BOUNDARY
         HIP_TRY(hipMemsetAsync(ctr + CTR_ACTIVE, 0, sizeof(unsigned long long), s));
         BucketArgs ba{};
         ba.recs = shard_view<Rec>(c, c->recs, SB_RECS);
@@ -441,19 +459,23 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
         toc(c, KS_GEN_EVENTS, s);
         if (c->fw > 0.0f)
         {
-            tic(c, KS_GEN_FACE, s);
             const ShardBuf<FHit> fb = shard_view<FHit>(c, c->fhits, SB_FHITS);
+            tic(c, KS_FACE, s);
             HIP_TRY(launch_face(m, g, fb, ctr, s));
+            toc(c, KS_FACE, s);
+            tic(c, KS_GEN_FHITS, s);
             HIP_TRY(launch_gen_fhits(fb, g, ba, s));
-            toc(c, KS_GEN_FACE, s);
+            toc(c, KS_GEN_FHITS, s);
         }
         if (c->E > 0)
         {
-            tic(c, KS_GEN_BDRY, s);
             const ShardBuf<BHit> hb = shard_view<BHit>(c, c->bhits, SB_BHITS);
+            tic(c, KS_BDRY, s);
             HIP_TRY(launch_bdry(c->E, (const BEdge *)c->bedges.p, g, hb, ctr, s));
+            toc(c, KS_BDRY, s);
+            tic(c, KS_GEN_BHITS, s);
             HIP_TRY(launch_gen_bhits(hb, g, ba, s));
-            toc(c, KS_GEN_BDRY, s);
+            toc(c, KS_GEN_BHITS, s);
         }
         HIP_TRY(launch_shard_reduce((unsigned long long *)c->shard_ctr.p, ctr, s));
         if ((st = read_counters(c, s)))
@@ -467,8 +489,9 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
         if (!over)
         {
             c->stats[2] = (int64_t)sum[SB_RECS];
-            c->stats[3] = (int64_t)c->h_ctr[CTR_REC_FACE];
-            c->stats[5] = (int64_t)c->h_ctr[CTR_REC_BOUNDARY];
+            c->stats[3] = (int64_t)sum[SB_FHITS];
+            c->stats[5] = (int64_t)sum[SB_BHITS];
+            c->stats[9] = (int64_t)sum[SB_CHUNKS];
             break;
         }
         if (sum[SB_RECS] >= 0xffffffffull || sum[SB_CHUNKS] >= 0xffffffffull)
@@ -492,7 +515,11 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
     // ---- chunks grouped by brick, bricks with records, then the per-brick ordered fold + solve
     const uint64_t n_chunks = c->h_ctr[CTR_SUM + SB_CHUNKS];
     if ((st = ensure(c->choff, (n_bricks + 1) * 4)) || (st = ensure(c->bchunks, (n_chunks + 1) * 4)) ||
-        (st = ensure(c->active, n_bricks * 4)) ||
+        (st = ensure(c->active, n_bricks * 4)) || (st = ensure(c->large, n_bricks * 4)) ||
+        // box-search queue: a few percent of the voxels typically; the solve
+        // kernel solves in place whatever does not fit
+        (st = ensure_shard(c, c->box, SB_BOX, per_for((uint64_t)c->N / 8 + 4096), sizeof(BoxItem))) ||
+        (st = ensure(c->qef, (size_t)c->N * 40)) ||
         (st = ensure(c->tiles, (scan_tile_count(n_bricks + 1) + 64) * 8)))
         return st;
     tic(c, KS_ACTIVE, s);
@@ -509,6 +536,8 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
     a.recs = (const Rec *)c->recs.p;
     a.active = (const uint32_t *)c->active.p;
     a.n_active = ctr + CTR_ACTIVE;
+    a.large = (uint32_t *)c->large.p;
+    a.n_large = ctr + CTR_LARGE;
     for (int i = 0; i < 3; ++i)
         a.nb[i] = nb[i];
     a.g = g;
@@ -518,10 +547,24 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
     a.flags = flags;
     a.bw = c->bw;
     a.rw = c->rw;
+    a.qef = (float *)c->qef.p;
+    a.box = shard_view<BoxItem>(c, c->box, SB_BOX);
     a.ctr = ctr;
+    HIP_TRY(hipMemsetAsync(ctr + CTR_LARGE, 0, 3 * sizeof(unsigned long long), s)); // LARGE, LARGE_RECS, BOX
     tic(c, KS_FOLD, s);
     HIP_TRY(launch_fold_bricks(a, n_bricks, s));
     toc(c, KS_FOLD, s);
+    tic(c, KS_FOLD_LARGE, s);
+    HIP_TRY(launch_fold_large(a, n_bricks, s));
+    toc(c, KS_FOLD_LARGE, s);
+    tic(c, KS_SOLVE, s);
+    HIP_TRY(launch_solve_voxels(a, coords, (uint64_t)c->N, s));
+    toc(c, KS_SOLVE, s);
+    tic(c, KS_SOLVE_BOX, s);
+    HIP_TRY(launch_solve_box(a, s));
+    toc(c, KS_SOLVE_BOX, s);
+    // totals of the queue regions, for the statistics
+    HIP_TRY(launch_shard_reduce((unsigned long long *)c->shard_ctr.p, ctr, s));
     return OVX_OK;
 }
 
@@ -561,13 +604,10 @@ ovx_status ovx_fdg_destroy(ovx_fdg_ctx *c)
     {
         DeviceGuard dg(c->device);
         std::lock_guard<std::mutex> lk(c->mu);
-        if (c->last_stream)
+        if (c->ran)
             (void)hipStreamSynchronize(c->last_stream);
-        DevBuf *all[] = {&c->planes, &c->bits, &c->woff,  &c->tiles, &c->counters, &c->vcnt,   &c->voff,
-                         &c->nbr,    &c->bcnt, &c->boff,  &c->bedge_v, &c->bedges, &c->events, &c->recs,
-                         &c->chunks, &c->heads, &c->choff, &c->bchunks, &c->active, &c->bhits, &c->fhits, &c->shard_ctr};
-        for (DevBuf *b : all)
-            release(*b);
+        c->for_each_buf([](DevBuf &b) { release(b); });
+        release(c->counters);
         if (c->h_ctr)
             (void)hipHostFree(c->h_ctr);
         for (auto &e : c->kev)
@@ -578,7 +618,7 @@ ovx_status ovx_fdg_destroy(ovx_fdg_ctx *c)
     return OVX_OK;
 }
 
-int64_t ovx_fdg_workspace_bytes(const ovx_fdg_ctx *c) { return c ? held_bytes(c) : 0; }
+int64_t ovx_fdg_workspace_bytes(const ovx_fdg_ctx *c) { return c ? held_bytes(const_cast<ovx_fdg_ctx *>(c)) : 0; }
 
 ovx_status ovx_fdg_trim(ovx_fdg_ctx *c)
 {
@@ -586,13 +626,9 @@ ovx_status ovx_fdg_trim(ovx_fdg_ctx *c)
         return fail(OVX_ERR_INVALID_ARGUMENT, "ctx is null");
     DeviceGuard dg(c->device);
     std::lock_guard<std::mutex> lk(c->mu);
-    if (c->last_stream)
+    if (c->ran)
         HIP_TRY(hipStreamSynchronize(c->last_stream));
-    DevBuf *all[] = {&c->planes, &c->bits, &c->woff,  &c->tiles,   &c->vcnt,   &c->voff,   &c->nbr,
-                     &c->bcnt,   &c->boff, &c->bedge_v, &c->bedges, &c->events, &c->recs, &c->chunks,
-                     &c->heads,  &c->choff, &c->bchunks, &c->active, &c->bhits, &c->fhits, &c->shard_ctr};
-    for (DevBuf *b : all)
-        release(*b);
+    c->for_each_buf([](DevBuf &b) { release(b); });
     for (int i = 0; i < SB_N; ++i)
         c->per[i] = 0;
     c->pending = 0;
@@ -609,7 +645,7 @@ ovx_status ovx_fdg_begin(ovx_fdg_ctx *c, const float *d_vertices, int64_t n_vert
         return fail(OVX_ERR_INVALID_ARGUMENT, "faces pointer is null");
     DeviceGuard dg(c->device);
     std::lock_guard<std::mutex> lk(c->mu);
-    MeshDev m{d_vertices, d_faces, n_faces};
+    MeshDev m{d_vertices, d_faces, n_faces, n_vertices};
     c->fw = face_weight;
     c->bw = boundary_weight;
     c->rw = regularization_weight;
@@ -630,6 +666,7 @@ ovx_status ovx_fdg_finish(ovx_fdg_ctx *c, int32_t *d_coords, float *d_dual, uint
     if (!d_coords || !d_dual || !d_intersected)
         return fail(OVX_ERR_INVALID_ARGUMENT, "output pointer is null");
     c->last_stream = (hipStream_t)stream;
+    c->ran = true;
     return finish_fdg(c, d_coords, d_dual, d_intersected, (hipStream_t)stream);
 }
 
@@ -640,7 +677,7 @@ ovx_status ovx_occ_begin(ovx_fdg_ctx *c, const float *d_triangles, int64_t n_tri
         return fail(OVX_ERR_INVALID_ARGUMENT, "ctx / n_voxels is null");
     DeviceGuard dg(c->device);
     std::lock_guard<std::mutex> lk(c->mu);
-    MeshDev m{d_triangles, nullptr, n_triangles};
+    MeshDev m{d_triangles, nullptr, n_triangles, 3 * n_triangles};
     return begin_common(c, m, 3 * n_triangles, grid, false, (hipStream_t)stream, n_voxels);
 }
 
@@ -661,22 +698,26 @@ ovx_status ovx_occ_finish(ovx_fdg_ctx *c, int32_t *d_coords, void *stream)
     return OVX_OK;
 }
 
-ovx_status ovx_fdg_stats(ovx_fdg_ctx *c, int64_t *out8)
+ovx_status ovx_fdg_stats(ovx_fdg_ctx *c, int64_t *out, int n)
 {
-    if (!c || !out8)
-        return fail(OVX_ERR_INVALID_ARGUMENT, "null argument");
+    if (!c || !out || n < 0)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "invalid stats arguments");
     DeviceGuard dg(c->device);
     std::lock_guard<std::mutex> lk(c->mu);
-    if (c->last_stream && c->counters.p)
+    if (c->ran && c->counters.p)
     {
         ovx_status st = read_counters(c, c->last_stream);
         if (st)
             return st;
         c->stats[7] = (int64_t)c->h_ctr[CTR_GIANT];
+        c->stats[8] = (int64_t)c->h_ctr[CTR_ACTIVE];
+        c->stats[10] = (int64_t)c->h_ctr[CTR_LARGE];
+        c->stats[11] = (int64_t)c->h_ctr[CTR_LARGE_RECS];
+        c->stats[12] = (int64_t)c->h_ctr[CTR_SUM + SB_BOX];
     }
     c->stats[6] = held_bytes(c);
-    for (int i = 0; i < 8; ++i)
-        out8[i] = c->stats[i];
+    for (int i = 0; i < n && i < OVX_FDG_N_STATS; ++i)
+        out[i] = c->stats[i];
     return OVX_OK;
 }
 
@@ -700,7 +741,7 @@ ovx_status ovx_fdg_set_capacity(ovx_fdg_ctx *c, int64_t events, int64_t hits)
         return fail(OVX_ERR_INVALID_ARGUMENT, "invalid capacity arguments");
     DeviceGuard dg(c->device);
     std::lock_guard<std::mutex> lk(c->mu);
-    if (c->last_stream)
+    if (c->ran)
         HIP_TRY(hipStreamSynchronize(c->last_stream));
     // capacities are per shard; at least one item per shard
     if (events)
@@ -756,8 +797,8 @@ ovx_status ovx_fdg_stage_times(ovx_fdg_ctx *c, float *ms5)
     ms5[0] = v(KS_PREP) + v(KS_MARK);
     ms5[1] = v(KS_WORDSCAN) + v(KS_COORDS);
     ms5[2] = v(KS_EDGES);
-    ms5[3] = v(KS_GEN_EVENTS) + v(KS_GEN_FACE) + v(KS_GEN_BDRY);
-    ms5[4] = v(KS_ACTIVE) + v(KS_FOLD);
+    ms5[3] = v(KS_GEN_EVENTS) + v(KS_FACE) + v(KS_GEN_FHITS) + v(KS_BDRY) + v(KS_GEN_BHITS);
+    ms5[4] = v(KS_ACTIVE) + v(KS_FOLD) + v(KS_FOLD_LARGE) + v(KS_SOLVE) + v(KS_SOLVE_BOX);
     return OVX_OK;
 }
 

@@ -44,7 +44,17 @@ struct MeshDev
     const float *v;     // [V,3]
     const int32_t *f;   // [F,3], or nullptr for a triangle soup (v is [F,3,3])
     int64_t F;
+    int64_t V;          // vertices (>= 1 when f is set and F > 0)
 };
+
+// A face with a vertex index outside [0, V) is read as vertex 0 three times
+// (a point, which no stage turns into a contribution); the prep kernel
+// flags it and the conversion reports the input error, so kernels that run
+// before that report never read outside the vertex array.
+__device__ __forceinline__ bool face_ok(const MeshDev &m, int32_t a, int32_t b, int32_t c)
+{
+    return a >= 0 && b >= 0 && c >= 0 && a < m.V && b < m.V && c < m.V;
+}
 
 __device__ __forceinline__ void load_tri(const MeshDev &m, int64_t face, float v0[3], float v1[3], float v2[3])
 {
@@ -52,9 +62,12 @@ __device__ __forceinline__ void load_tri(const MeshDev &m, int64_t face, float v
     if (m.f)
     {
         const int32_t *fi = m.f + 3 * face;
-        p0 = m.v + 3 * (int64_t)fi[0];
-        p1 = m.v + 3 * (int64_t)fi[1];
-        p2 = m.v + 3 * (int64_t)fi[2];
+        int32_t a = fi[0], b = fi[1], c = fi[2];
+        if (!face_ok(m, a, b, c))
+            a = b = c = 0;
+        p0 = m.v + 3 * (int64_t)a;
+        p1 = m.v + 3 * (int64_t)b;
+        p2 = m.v + 3 * (int64_t)c;
     }
     else
     {

@@ -12,16 +12,15 @@ namespace ovx
 // device counters (unsigned long long each)
 enum
 {
-    CTR_EVENTS = 0,
     CTR_ERRORS = 1,
     CTR_WORDS_TOTAL = 2, // total of the bitset popcount scan (= N)
-    CTR_REC_FACE = 3,    // face-term records
-    CTR_REC_BOUNDARY = 4,
     CTR_GIANT = 5,
     CTR_EDGE_TOTAL = 6,  // half-edge count scan total
     CTR_BEDGE_TOTAL = 7, // boundary edges
     CTR_ACTIVE = 8,      // bricks with records
     CTR_CHUNK_SCAN = 9,  // total of the per-brick chunk-count scan
+    CTR_LARGE = 10,      // bricks left to the fold kernel for large bricks
+    CTR_LARGE_RECS = 11, // their records
     // sum and largest shard of each sharded buffer (filled by k_shard_reduce)
     CTR_SUM = 16,        // CTR_SUM + SB_*  : total items requested
     CTR_MAX = 24,        // CTR_MAX + SB_*  : largest per-shard request
@@ -36,7 +35,8 @@ enum
     SB_BHITS = 2,
     SB_RECS = 3,
     SB_CHUNKS = 4,
-    SB_N = 5
+    SB_BOX = 5, // voxels queued for the box-constrained search
+    SB_N = 6
 };
 
 // Append / allocation counters are split over NSHARD counters 128 bytes
@@ -123,6 +123,16 @@ struct BEdge
     int valid;
 };
 
+// voxel queued for the box-constrained search: regularised quadric,
+// unconstrained minimiser, voxel coordinates, output row
+struct BoxItem
+{
+    float q[10];
+    float v[3];
+    int c[3];
+    uint64_t vid;
+};
+
 struct FoldArgs
 {
     const uint32_t *choff;              // [bricks+1] first chunk of each brick in `bchunks`
@@ -131,13 +141,17 @@ struct FoldArgs
     const Rec *recs;
     const uint32_t *active;             // bricks with records
     const unsigned long long *n_active; // their count (device)
+    uint32_t *large;                    // bricks too large for the main fold kernel
+    unsigned long long *n_large;        // their count (device, zeroed by the host)
     int nb[3];
     GridDev g; // bitset + voxel ids, grid
     const float4 *planes;
     const BEdge *bedges;
     float *dual;
     uint8_t *flags;
+    float *qef; // [N][10] folded, regularised quadric of each voxel
     float bw, rw;
+    ShardBuf<BoxItem> box; // box-search queue
     unsigned long long *ctr;
 };
 
@@ -156,8 +170,11 @@ hipError_t launch_place_chunks(const ShardBuf<Chunk> &chunks, const uint32_t *ch
 hipError_t launch_active_bricks(const uint32_t *nch, uint64_t n_bricks, uint32_t *list, unsigned long long *count,
                                 hipStream_t s);
 hipError_t launch_fold_bricks(const FoldArgs &a, uint64_t max_bricks, hipStream_t s);
-hipError_t launch_edge_count(const MeshDev &m, uint32_t *vcnt, uint32_t *voff, uint32_t *nbr, int pass,
-                             hipStream_t s);
+hipError_t launch_fold_large(const FoldArgs &a, uint64_t max_bricks, hipStream_t s);
+hipError_t launch_solve_voxels(const FoldArgs &a, const int32_t *coords, uint64_t n, hipStream_t s);
+hipError_t launch_solve_box(const FoldArgs &a, hipStream_t s);
+hipError_t launch_edge_count(const MeshDev &m, uint32_t *vcnt, uint32_t *rank, const uint32_t *voff, uint32_t *nbr,
+                             int pass, hipStream_t s);
 hipError_t launch_edge_singletons(int64_t V, const uint32_t *voff, uint32_t *nbr, uint32_t *bcnt, const uint32_t *boff,
                                   int2 *bedge_v, int pass, hipStream_t s);
 hipError_t launch_bedge_prep(const float *verts, int64_t E, const int2 *bedge_v, BEdge *be, hipStream_t s);

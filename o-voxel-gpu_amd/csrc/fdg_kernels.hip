@@ -144,6 +144,27 @@ __device__ __forceinline__ uint64_t shard_claim(const ShardBuf<T> &b, uint32_t s
     return (uint64_t)sh * b.per + off;
 }
 
+// Claims n items of shard `sh` only if they fit: the counter never passes
+// the capacity, so every slot below it holds an item (for queues that are
+// consumed without a re-run; a failed claim is handled by the caller).
+template <typename T>
+__device__ __forceinline__ bool shard_claim_fit(const ShardBuf<T> &b, uint32_t sh, uint64_t n, uint64_t &first)
+{
+    unsigned long long *cnt = b.cnt + (uint64_t)sh * SHARD_STRIDE;
+    unsigned long long cur = __hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (cur + n <= b.per)
+    {
+        const unsigned long long prev = atomicCAS(cnt, cur, cur + n);
+        if (prev == cur)
+        {
+            first = (uint64_t)sh * b.per + cur;
+            return true;
+        }
+        cur = prev;
+    }
+    return false;
+}
+
 // Whether global slot i of a sharded buffer holds an item.  A region whose
 // counter passed its capacity had a claim dropped and is skipped whole (its
 // slots below the capacity may be unwritten); the host re-runs the producer
@@ -566,13 +587,8 @@ __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, GridDev g, unsign
         }
         __syncthreads();
     }
-    if (counters)
-    {
-        __shared__ uint32_t shr[NWAVE];
-        const uint32_t tot = block_reduce_sum(n_ev, shr);
-        if (tid == 0 && tot)
-            atomicAdd(counters + CTR_EVENTS, (unsigned long long)tot);
-    }
+    (void)counters;
+    (void)n_ev;
 }
 
 // ---------------------------------------------------------------------------
@@ -800,40 +816,39 @@ __global__ __launch_bounds__(BT) void k_face(MeshDev m, GridDev g, ShardBuf<FHit
         }
         __syncthreads();
     }
-    {
-        __shared__ uint32_t shr[NWAVE];
-        const uint32_t tot = block_reduce_sum(n_hit, shr);
-        if (tid == 0 && tot)
-            atomicAdd(ctr + CTR_REC_FACE, (unsigned long long)tot);
-    }
+    (void)ctr;
+    (void)n_hit;
 }
 
 // ---------------------------------------------------------------------------
 // boundary edges (FDG.cpp:724-756): per-min-vertex adjacency lists, sorted,
 // singletons are boundary edges; ranks follow (min, max) ascending order.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BT) void k_edge_count(MeshDev m, uint32_t *vcnt, uint32_t *voff, uint32_t *nbr, int pass)
+__global__ __launch_bounds__(BT) void k_edge_count(MeshDev m, uint32_t *vcnt, uint32_t *rank, const uint32_t *voff,
+                                                   uint32_t *nbr, int pass)
 {
     const int64_t stride = (int64_t)gridDim.x * BT;
     for (int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x; i < 3 * m.F; i += stride)
     {
         const int64_t f = i / 3;
         const int k = (int)(i - 3 * f);
-        int a = m.f[3 * f + k];
-        int b = m.f[3 * f + (k + 1) % 3];
+        const int32_t *fi = m.f + 3 * f;
+        if (!face_ok(m, fi[0], fi[1], fi[2]))
+            continue; // flagged by the prep kernel
+        int a = fi[k];
+        int b = fi[(k + 1) % 3];
         if (a > b)
         {
             const int t = a;
             a = b;
             b = t;
         }
+        // pass 0: count the half-edges of each smaller vertex, keeping each
+        // one's rank; pass 1: place the larger vertex at offset + rank
         if (pass == 0)
-            atomicAdd(vcnt + a, 1u);
+            rank[i] = atomicAdd(vcnt + a, 1u);
         else
-        {
-            const uint32_t pos = voff[a] + atomicSub(vcnt + a, 1u) - 1u;
-            nbr[pos] = (uint32_t)b;
-        }
+            nbr[voff[a] + rank[i]] = (uint32_t)b;
     }
 }
 
@@ -1019,15 +1034,9 @@ __global__ __launch_bounds__(BT) void k_bdry(int64_t E, const BEdge *be, GridDev
             }
         }
     }
-    if (counters)
-    {
-        __shared__ uint32_t shr[NWAVE];
-        const uint32_t tot = block_reduce_sum(n_hit, shr);
-        if (threadIdx.x == 0 && tot)
-            atomicAdd(counters + CTR_REC_BOUNDARY, (unsigned long long)tot);
-        if (err)
-            atomicOr(counters + CTR_ERRORS, (unsigned long long)err);
-    }
+    (void)n_hit;
+    if (counters && err)
+        atomicOr(counters + CTR_ERRORS, (unsigned long long)err);
 }
 
 // ---------------------------------------------------------------------------
@@ -1165,19 +1174,33 @@ __global__ __launch_bounds__(BT) void k_place_chunks(ShardBuf<Chunk> chunks, con
     }
 }
 
-// list of bricks that received records
+// list of bricks that received records: each block takes ACT_PER
+// consecutive bricks per thread and appends them with one atomic
+constexpr int ACT_PER = 8;
 __global__ __launch_bounds__(BT) void k_active_bricks(const uint32_t *nch, uint64_t n_bricks, uint32_t *list,
                                                       unsigned long long *count)
 {
-    const uint64_t stride = (uint64_t)gridDim.x * BT;
-    // every lane of a wave runs the same number of iterations (wave_append is wave-wide)
-    const uint64_t n_round = (n_bricks + 63ull) & ~63ull;
-    for (uint64_t b = (uint64_t)blockIdx.x * BT + threadIdx.x; b < n_round; b += stride)
+    __shared__ uint32_t sh[NWAVE];
+    __shared__ unsigned long long sbase;
+    const uint64_t span = (uint64_t)BT * ACT_PER;
+    for (uint64_t b0 = (uint64_t)blockIdx.x * span; b0 < n_bricks; b0 += (uint64_t)gridDim.x * span)
     {
-        const bool act = b < n_bricks && nch[b] != 0u;
-        const uint64_t slot = wave_append(act, count);
-        if (act)
-            list[slot] = (uint32_t)b;
+        const uint64_t first = b0 + (uint64_t)threadIdx.x * ACT_PER;
+        uint32_t mine = 0;
+#pragma unroll
+        for (int e = 0; e < ACT_PER; ++e)
+            mine |= (first + e < n_bricks && nch[first + e] != 0u) ? (1u << e) : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan((uint32_t)__popc(mine), sh, tot);
+        if (threadIdx.x == 0)
+            sbase = tot ? atomicAdd(count, (unsigned long long)tot) : 0ull;
+        __syncthreads();
+        uint64_t pos = sbase + ex;
+#pragma unroll
+        for (int e = 0; e < ACT_PER; ++e)
+            if (mine & (1u << e))
+                list[pos++] = (uint32_t)(first + e);
+        __syncthreads();
     }
 }
 
@@ -1297,7 +1320,26 @@ struct FoldState
         fl = 0;
     }
 
+    // face whose plane a record uses (phases 0 and 1), NO_PLANE for boundary records
+    static constexpr uint32_t NO_PLANE = 0xffffffffu;
+    __device__ __forceinline__ static uint32_t plane_of(uint64_t key)
+    {
+        const int phase = (int)(key >> KEY_PHASE_SHIFT);
+        if (phase == 0)
+            return (uint32_t)(((key >> KEY_FA_SHIFT) & KEY_FA_MASK) / 3u);
+        if (phase == 1)
+            return (uint32_t)(key & 0xffffffffull);
+        return NO_PLANE;
+    }
+
     __device__ __forceinline__ void add(uint64_t key, float z, const FoldArgs &a)
+    {
+        const uint32_t f = plane_of(key);
+        add_pl(key, z, f != NO_PLANE ? a.planes[f] : make_float4(0.0f, 0.0f, 0.0f, 0.0f), a);
+    }
+
+    // one record; pl4 = the plane of its face (unused for boundary records)
+    __device__ __forceinline__ void add_pl(uint64_t key, float z, const float4 pl4, const FoldArgs &a)
     {
         const int phase = (int)(key >> KEY_PHASE_SHIFT);
         if (phase == 0)
@@ -1316,7 +1358,6 @@ struct FoldState
             const float p0 = ax0 == 0 ? px : (ax1 == 0 ? py : z);
             const float p1 = ax0 == 1 ? px : (ax1 == 1 ? py : z);
             const float p2 = ax0 == 2 ? px : (ax1 == 2 ? py : z);
-            const float4 pl4 = a.planes[f];
             const float pl[4] = {pl4.x, pl4.y, pl4.z, pl4.w};
             float qf[10];
             plane_quadric(pl, qf);
@@ -1346,8 +1387,6 @@ struct FoldState
         }
         else if (phase == 1)
         {
-            const uint32_t f = (uint32_t)(key & 0xffffffffull);
-            const float4 pl4 = a.planes[f];
             const float pl[4] = {pl4.x, pl4.y, pl4.z, pl4.w};
             float qf[10];
             plane_quadric(pl, qf);
@@ -1376,16 +1415,13 @@ struct FoldState
             for (int k = 0; k < 10; ++k)
                 q[k] = q[k] + s * R[k];
         }
-        const int coord[3] = {x, y, z};
-        float d[3];
-        solve_voxel(q, coord, a.g.vs, d);
         int b;
         const uint64_t w = word_of(a.g, x, y, z, b);
         const uint64_t vid = (uint64_t)voxel_id(a.g, w, b);
-        float *o = a.dual + 3 * vid;
-        o[0] = d[0];
-        o[1] = d[1];
-        o[2] = d[2];
+        float *qo = a.qef + 10 * vid;
+#pragma unroll
+        for (int i = 0; i < 10; ++i)
+            qo[i] = q[i];
         uint8_t *fo = a.flags + 3 * vid;
         fo[0] = (uint8_t)(fl & 1);
         fo[1] = (uint8_t)((fl >> 1) & 1);
@@ -1393,7 +1429,8 @@ struct FoldState
     }
 };
 
-constexpr int FOLD_CAP = 2560; // records held in LDS at a time
+constexpr int FOLD_SMALL = 2048; // records of a brick held in LDS by the main fold kernel (4 blocks per CU)
+constexpr int FOLD_CAP = 6144;   // records held in LDS at a time by the kernel for large bricks
 
 // LDS view of a batch of up to BT chunks of one brick
 struct ChunkBatch
@@ -1401,6 +1438,23 @@ struct ChunkBatch
     uint32_t start[BT];
     uint32_t incl[BT];
     uint32_t sh[NWAVE];
+};
+
+// one brick's records in LDS and the per-voxel grouping built over them
+template <int CAP>
+struct FoldLds
+{
+    static_assert(CAP <= 65536, "16-bit record indices");
+    uint64_t key[CAP];
+    float z[CAP];
+    uint16_t lv[CAP];
+    uint16_t perm[CAP]; // record indices grouped by voxel, then key-ordered inside each voxel
+    uint32_t off[513];   // first grouped slot of each voxel
+    uint32_t cur[512];   // per-voxel record counts
+    uint16_t vlist[512]; // voxels with records
+    uint32_t sh[NWAVE];
+    uint32_t n, m;
+    ChunkBatch cb;
 };
 
 // Calls f(rec) for every record of `brick`, all threads cooperating.
@@ -1412,6 +1466,7 @@ __device__ __forceinline__ void for_each_rec(const FoldArgs &a, uint32_t brick, 
     for (uint32_t cbeg = c0; cbeg < c1; cbeg += BT)
     {
         const uint32_t ci = cbeg + threadIdx.x;
+        const int nch = (int)(c1 - cbeg < (uint32_t)BT ? c1 - cbeg : (uint32_t)BT);
         uint32_t cnt = 0;
         if (ci < c1)
         {
@@ -1425,7 +1480,7 @@ __device__ __forceinline__ void for_each_rec(const FoldArgs &a, uint32_t brick, 
         __syncthreads();
         for (uint32_t j = threadIdx.x; j < tot; j += BT)
         {
-            const int k = upper_find(cb.incl, BT, j);
+            const int k = upper_find(cb.incl, nch, j);
             const uint32_t before = k ? cb.incl[k - 1] : 0u;
             f(a.recs[(uint64_t)cb.start[k] + (j - before)]);
         }
@@ -1433,121 +1488,232 @@ __device__ __forceinline__ void for_each_rec(const FoldArgs &a, uint32_t brick, 
     }
 }
 
-// One workgroup per brick that received records: group the brick's records
-// by voxel in LDS (histogram + counting sort), rank-sort each voxel's keys,
-// replay them in order, solve.  Voxel ranges are sized to the LDS tile; a
-// voxel with more records than the tile is replayed in key-ordered batches
-// selected by a binary search over the key value.
+// The brick's n records are in L.key/z/lv and L.cur holds their per-voxel
+// histogram: group them by voxel (counting sort), order each voxel's records
+// by key (rank sort, keys are unique), replay them in that order, one thread
+// per voxel with records, and solve.  Contains barriers.
+template <int CAP>
+__device__ __forceinline__ void fold_loaded(const FoldArgs &a, FoldLds<CAP> &L, uint32_t n, int x0, int y0, int z0)
+{
+    const int tid = threadIdx.x;
+    uint32_t nv;
+    {
+        const uint32_t c0 = L.cur[2 * tid], c1 = L.cur[2 * tid + 1];
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan(c0 + c1, L.sh, tot);
+        L.off[2 * tid] = ex;
+        L.off[2 * tid + 1] = ex + c0;
+        if (tid == 0)
+            L.off[512] = tot;
+        const uint32_t nz = (c0 ? 1u : 0u) + (c1 ? 1u : 0u);
+        const uint32_t exv = block_excl_scan(nz, L.sh, nv);
+        if (c0)
+            L.vlist[exv] = (uint16_t)(2 * tid);
+        if (c1)
+            L.vlist[exv + (c0 ? 1u : 0u)] = (uint16_t)(2 * tid + 1);
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < n; i += BT)
+    {
+        const uint32_t l = L.lv[i];
+        L.perm[L.off[l] + atomicSub(&L.cur[l], 1u) - 1u] = (uint16_t)i;
+    }
+    __syncthreads();
+    // rank of each record among its voxel's keys; the permutation is
+    // rewritten in place once every rank is known
+    constexpr int PER = (CAP + BT - 1) / BT;
+    uint32_t rk[PER];
+#pragma unroll
+    for (int it = 0; it < PER; ++it)
+    {
+        const uint32_t p = (uint32_t)(tid + it * BT);
+        rk[it] = 0xffffffffu;
+        if (p < n)
+        {
+            const uint32_t me = L.perm[p];
+            const uint32_t l = L.lv[me];
+            const uint32_t s0 = L.off[l], s1 = L.off[l + 1];
+            const uint64_t k = L.key[me];
+            uint32_t rank = 0;
+            for (uint32_t t = s0; t < s1; ++t)
+                rank += (L.key[L.perm[t]] < k) ? 1u : 0u;
+            rk[it] = ((s0 + rank) << 16) | me;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < PER; ++it)
+        if (rk[it] != 0xffffffffu)
+            L.perm[rk[it] >> 16] = (uint16_t)(rk[it] & 0xffffu);
+    __syncthreads();
+    for (uint32_t vi = tid; vi < nv; vi += BT)
+    {
+        const int l = L.vlist[vi];
+        const uint32_t s0 = L.off[l], s1 = L.off[l + 1];
+        FoldState st;
+        st.init();
+        for (uint32_t i = s0; i < s1; i += 4)
+        {
+            // the planes of up to 4 records are fetched together, then folded in order
+            uint64_t kk[4];
+            float zz[4];
+            float4 pp[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+            {
+                kk[u] = 0;
+                zz[u] = 0.0f;
+                pp[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (i + u < s1)
+                {
+                    const uint32_t j = L.perm[i + u];
+                    kk[u] = L.key[j];
+                    zz[u] = L.z[j];
+                    const uint32_t f = FoldState::plane_of(kk[u]);
+                    if (f != FoldState::NO_PLANE)
+                        pp[u] = a.planes[f];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (i + u < s1)
+                    st.add_pl(kk[u], zz[u], pp[u], a);
+        }
+        st.finish(x0 + (l & 7), y0 + ((l >> 3) & 7), z0 + (l >> 6), a);
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void brick_origin(const FoldArgs &a, uint32_t brick, int &x0, int &y0, int &z0)
+{
+    const int bx = (int)(brick % (uint32_t)a.nb[0]);
+    const int byz = (int)(brick / (uint32_t)a.nb[0]);
+    const int by = byz % a.nb[1];
+    const int bz = byz / a.nb[1];
+    x0 = a.g.gmin[0] + 8 * bx;
+    y0 = a.g.gmin[1] + 8 * by;
+    z0 = a.g.gmin[2] + 8 * bz;
+}
+
+// One workgroup per brick that received records, for bricks of at most
+// FOLD_SMALL records (the others are listed for k_fold_large): the records
+// are read once into LDS, grouped, replayed and solved there.
 __global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs a)
 {
-    __shared__ uint64_t skey[FOLD_CAP];
-    __shared__ float sz[FOLD_CAP];
-    __shared__ uint16_t sperm[FOLD_CAP];
-    __shared__ uint16_t slv[FOLD_CAP];
-    __shared__ uint16_t perm1[FOLD_CAP];
-    __shared__ uint32_t s_n;
-    __shared__ uint32_t off[513];
-    __shared__ uint32_t cur[512];
-    __shared__ uint32_t sh[NWAVE];
-    __shared__ ChunkBatch cb;
-    __shared__ uint32_t s_m;
+    __shared__ FoldLds<FOLD_SMALL> L;
     const int tid = threadIdx.x;
     const uint64_t n_active = *a.n_active;
-
     for (uint64_t bi = blockIdx.x; bi < n_active; bi += gridDim.x)
     {
         const uint32_t brick = a.active[bi];
-        const int bx = (int)(brick % (uint32_t)a.nb[0]);
-        const int byz = (int)(brick / (uint32_t)a.nb[0]);
-        const int by = byz % a.nb[1];
-        const int bz = byz / a.nb[1];
-        const int x0 = a.g.gmin[0] + 8 * bx, y0 = a.g.gmin[1] + 8 * by, z0 = a.g.gmin[2] + 8 * bz;
+        const uint32_t c0 = a.choff[brick], c1 = a.choff[brick + 1];
+        for (int i = tid; i < 512; i += BT)
+            L.cur[i] = 0;
+        // the brick's chunks, BT at a time: records appended to the LDS tile
+        uint32_t n = 0;
+        bool over = false;
+        for (uint32_t cbeg = c0; cbeg < c1; cbeg += BT)
+        {
+            const int nb = (int)(c1 - cbeg < (uint32_t)BT ? c1 - cbeg : (uint32_t)BT);
+            uint32_t cnt = 0;
+            if (tid < nb)
+            {
+                const Chunk ch = a.chunks[a.bchunks[cbeg + tid]];
+                L.cb.start[tid] = ch.start;
+                cnt = ch.count;
+            }
+            uint32_t tot;
+            const uint32_t ex = block_excl_scan(cnt, L.cb.sh, tot);
+            if (n + tot > (uint32_t)FOLD_SMALL)
+            {
+                over = true;
+                break;
+            }
+            L.cb.incl[tid] = ex + cnt;
+            __syncthreads();
+            for (uint32_t j = tid; j < tot; j += BT)
+            {
+                const int k = upper_find(L.cb.incl, nb, j);
+                const uint32_t before = k ? L.cb.incl[k - 1] : 0u;
+                const Rec r = a.recs[(uint64_t)L.cb.start[k] + (j - before)];
+                L.key[n + j] = r.key;
+                L.z[n + j] = r.z;
+                L.lv[n + j] = (uint16_t)r.lvox;
+                atomicAdd(&L.cur[r.lvox], 1u);
+            }
+            __syncthreads();
+            n += tot;
+        }
+        if (over)
+        {
+            if (tid == 0)
+                a.large[atomicAdd(a.n_large, 1ull)] = brick;
+            __syncthreads();
+            continue;
+        }
+        int x0, y0, z0;
+        brick_origin(a, brick, x0, y0, z0);
+        fold_loaded<FOLD_SMALL>(a, L, n, x0, y0, z0);
+    }
+}
+
+// Bricks with more records than k_fold_bricks holds.  A brick
+// whose records fit FOLD_CAP is folded like there; otherwise voxel ranges
+// are sized to the LDS tile, and a voxel with more records than the tile is
+// replayed in key-ordered batches selected by a binary search over the key.
+__global__ __launch_bounds__(BT) void k_fold_large(FoldArgs a)
+{
+    __shared__ FoldLds<FOLD_CAP> L;
+    const int tid = threadIdx.x;
+    const uint64_t n_large = *a.n_large;
+
+    for (uint64_t bi = blockIdx.x; bi < n_large; bi += gridDim.x)
+    {
+        const uint32_t brick = a.large[bi];
+        int x0, y0, z0;
+        brick_origin(a, brick, x0, y0, z0);
 
         // one pass over the brick's records: per-voxel histogram, and the
         // records themselves into LDS while they fit
         for (int i = tid; i < 512; i += BT)
-            cur[i] = 0;
+            L.cur[i] = 0;
         if (tid == 0)
-            s_n = 0;
+            L.n = 0;
         __syncthreads();
-        for_each_rec(a, brick, cb, [&](const Rec &r) {
-            atomicAdd(&cur[r.lvox], 1u);
-            const uint32_t i = atomicAdd(&s_n, 1u);
+        for_each_rec(a, brick, L.cb, [&](const Rec &r) {
+            atomicAdd(&L.cur[r.lvox], 1u);
+            const uint32_t i = atomicAdd(&L.n, 1u);
             if (i < (uint32_t)FOLD_CAP)
             {
-                skey[i] = r.key;
-                sz[i] = r.z;
-                slv[i] = (uint16_t)r.lvox;
+                L.key[i] = r.key;
+                L.z[i] = r.z;
+                L.lv[i] = (uint16_t)r.lvox;
             }
         });
-        {
-            // exclusive scan of the 512 counts (2 per thread)
-            const uint32_t c0 = cur[2 * tid], c1 = cur[2 * tid + 1];
-            uint32_t tot;
-            const uint32_t ex = block_excl_scan(c0 + c1, sh, tot);
-            off[2 * tid] = ex;
-            off[2 * tid + 1] = ex + c0;
-            if (tid == 0)
-                off[512] = tot;
-        }
-        __syncthreads();
-
-        const uint32_t n_all = s_n;
+        const uint32_t n_all = L.n;
+        if (tid == 0)
+            atomicAdd(a.ctr + CTR_LARGE_RECS, (unsigned long long)n_all);
         if (n_all <= (uint32_t)FOLD_CAP)
         {
-            // ---- all of the brick's records are in LDS: counting sort by voxel,
-            // rank sort inside each voxel, replay
-            for (int i = tid; i < 512; i += BT)
-                cur[i] = 0;
-            __syncthreads();
-            for (uint32_t i = tid; i < n_all; i += BT)
-            {
-                const uint32_t lv = slv[i];
-                perm1[off[lv] + atomicAdd(&cur[lv], 1u)] = (uint16_t)i;
-            }
-            __syncthreads();
-            for (uint32_t p = tid; p < n_all; p += BT)
-            {
-                int vl = 0, vh = 511;
-                while (vl < vh)
-                {
-                    const int mid = (vl + vh + 1) >> 1;
-                    if (off[mid] <= p)
-                        vl = mid;
-                    else
-                        vh = mid - 1;
-                }
-                const uint32_t s0 = off[vl], s1 = off[vl + 1];
-                const uint32_t me = perm1[p];
-                const uint64_t k = skey[me];
-                uint32_t rank = 0;
-                for (uint32_t t = s0; t < s1; ++t)
-                    rank += (skey[perm1[t]] < k) ? 1u : 0u;
-                sperm[s0 + rank] = (uint16_t)me;
-            }
-            __syncthreads();
-            for (int lv = tid; lv < 512; lv += BT)
-            {
-                const uint32_t s0 = off[lv], s1 = off[lv + 1];
-                if (s1 == s0)
-                    continue;
-                FoldState st;
-                st.init();
-                for (uint32_t i = s0; i < s1; ++i)
-                {
-                    const uint32_t j = sperm[i];
-                    st.add(skey[j], sz[j], a);
-                }
-                st.finish(x0 + (lv & 7), y0 + ((lv >> 3) & 7), z0 + (lv >> 6), a);
-            }
-            __syncthreads();
+            fold_loaded<FOLD_CAP>(a, L, n_all, x0, y0, z0);
             continue;
         }
+        {
+            // exclusive scan of the 512 counts (2 per thread)
+            const uint32_t c0 = L.cur[2 * tid], c1 = L.cur[2 * tid + 1];
+            uint32_t tot;
+            const uint32_t ex = block_excl_scan(c0 + c1, L.sh, tot);
+            L.off[2 * tid] = ex;
+            L.off[2 * tid + 1] = ex + c0;
+            if (tid == 0)
+                L.off[512] = tot;
+        }
+        __syncthreads();
 
         int lo = 0;
         while (lo < 512)
         {
-            const uint32_t nlo = off[lo + 1] - off[lo];
+            const uint32_t nlo = L.off[lo + 1] - L.off[lo];
             if (nlo == 0)
             {
                 ++lo;
@@ -1571,11 +1737,11 @@ __global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs a)
                         const uint64_t d = thi - tlo;
                         const uint64_t mid = tlo + d / 2 + (d & 1ull);
                         uint32_t cnt = 0;
-                        for_each_rec(a, brick, cb, [&](const Rec &r) {
+                        for_each_rec(a, brick, L.cb, [&](const Rec &r) {
                             cnt += (r.lvox == lv && (!have_last || r.key > last) && r.key <= mid) ? 1u : 0u;
                         });
                         uint32_t tot;
-                        block_excl_scan(cnt, sh, tot);
+                        block_excl_scan(cnt, L.sh, tot);
                         if (tot <= (uint32_t)FOLD_CAP)
                             tlo = mid;
                         else
@@ -1583,29 +1749,29 @@ __global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs a)
                     }
                     const uint64_t T = tlo;
                     if (tid == 0)
-                        s_m = 0;
+                        L.m = 0;
                     __syncthreads();
-                    for_each_rec(a, brick, cb, [&](const Rec &r) {
+                    for_each_rec(a, brick, L.cb, [&](const Rec &r) {
                         if (r.lvox == lv && (!have_last || r.key > last) && r.key <= T)
                         {
-                            const uint32_t p = atomicAdd(&s_m, 1u);
-                            skey[p] = r.key;
-                            sz[p] = r.z;
+                            const uint32_t p = atomicAdd(&L.m, 1u);
+                            L.key[p] = r.key;
+                            L.z[p] = r.z;
                         }
                     });
-                    const uint32_t m = s_m;
+                    const uint32_t m = L.m;
                     for (uint32_t i = tid; i < m; i += BT)
                     {
-                        const uint64_t k = skey[i];
+                        const uint64_t k = L.key[i];
                         uint32_t rank = 0;
                         for (uint32_t t = 0; t < m; ++t)
-                            rank += (skey[t] < k) ? 1u : 0u;
-                        sperm[rank] = (uint16_t)i;
+                            rank += (L.key[t] < k) ? 1u : 0u;
+                        L.perm[rank] = (uint16_t)i;
                     }
                     __syncthreads();
                     if (tid == 0)
                         for (uint32_t i = 0; i < m; ++i)
-                            st.add(skey[sperm[i]], sz[sperm[i]], a);
+                            st.add(L.key[L.perm[i]], L.z[L.perm[i]], a);
                     __syncthreads();
                     last = T;
                     have_last = true;
@@ -1627,24 +1793,24 @@ __global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs a)
             while (l < h)
             {
                 const int mid = (l + h + 1) >> 1;
-                if (off[mid] - off[lo] <= (uint32_t)FOLD_CAP)
+                if (L.off[mid] - L.off[lo] <= (uint32_t)FOLD_CAP)
                     l = mid;
                 else
                     h = mid - 1;
             }
             const int hi = l;
-            const uint32_t base = off[lo];
-            const uint32_t nrec = off[hi] - base;
+            const uint32_t base = L.off[lo];
+            const uint32_t nrec = L.off[hi] - base;
             for (int i = tid; i < 512; i += BT)
-                cur[i] = 0;
+                L.cur[i] = 0;
             __syncthreads();
             // counting sort of the range's records by voxel
-            for_each_rec(a, brick, cb, [&](const Rec &r) {
+            for_each_rec(a, brick, L.cb, [&](const Rec &r) {
                 if (r.lvox >= (uint32_t)lo && r.lvox < (uint32_t)hi)
                 {
-                    const uint32_t p = off[r.lvox] - base + atomicAdd(&cur[r.lvox], 1u);
-                    skey[p] = r.key;
-                    sz[p] = r.z;
+                    const uint32_t p = L.off[r.lvox] - base + atomicAdd(&L.cur[r.lvox], 1u);
+                    L.key[p] = r.key;
+                    L.z[p] = r.z;
                 }
             });
             // rank of each record among its voxel's keys (keys are unique)
@@ -1654,30 +1820,30 @@ __global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs a)
                 while (vl < vh)
                 {
                     const int mid = (vl + vh + 1) >> 1;
-                    if (off[mid] - base <= i)
+                    if (L.off[mid] - base <= i)
                         vl = mid;
                     else
                         vh = mid - 1;
                 }
-                const uint32_t s0 = off[vl] - base, s1 = off[vl + 1] - base;
-                const uint64_t k = skey[i];
+                const uint32_t s0 = L.off[vl] - base, s1 = L.off[vl + 1] - base;
+                const uint64_t k = L.key[i];
                 uint32_t rank = 0;
                 for (uint32_t t = s0; t < s1; ++t)
-                    rank += (skey[t] < k) ? 1u : 0u;
-                sperm[s0 + rank] = (uint16_t)i;
+                    rank += (L.key[t] < k) ? 1u : 0u;
+                L.perm[s0 + rank] = (uint16_t)i;
             }
             __syncthreads();
             for (int lv = lo + tid; lv < hi; lv += BT)
             {
-                const uint32_t s0 = off[lv] - base, s1 = off[lv + 1] - base;
+                const uint32_t s0 = L.off[lv] - base, s1 = L.off[lv + 1] - base;
                 if (s1 == s0)
                     continue;
                 FoldState st;
                 st.init();
                 for (uint32_t i = s0; i < s1; ++i)
                 {
-                    const uint32_t j = sperm[i];
-                    st.add(skey[j], sz[j], a);
+                    const uint32_t j = L.perm[i];
+                    st.add(L.key[j], L.z[j], a);
                 }
                 st.finish(x0 + (lv & 7), y0 + ((lv >> 3) & 7), z0 + (lv >> 6), a);
             }
@@ -1685,6 +1851,95 @@ __global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs a)
             lo = hi;
         }
         __syncthreads();
+    }
+}
+
+// Dual vertex of every voxel, one per lane, from its folded quadric: the
+// unconstrained minimiser when it lies in the voxel, else the voxel is
+// queued for the box-constrained search (solved in place when its queue
+// region is full: the claim is all-or-nothing, so the queue holds only
+// complete items).
+__global__ __launch_bounds__(BT) void k_solve_voxels(const float *qef, const int32_t *coords, uint64_t n, GridDev g,
+                                                     float *dual, ShardBuf<BoxItem> box)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * BT;
+    const int lane = threadIdx.x & 63;
+    const uint32_t shard = (blockIdx.x * NWAVE + (threadIdx.x >> 6)) % NSHARD;
+    // every lane of a wave runs the same number of iterations (the claim is wave-wide)
+    const uint64_t n_round = (n + 63ull) & ~63ull;
+    for (uint64_t vid = (uint64_t)blockIdx.x * BT + threadIdx.x; vid < n_round; vid += stride)
+    {
+        const bool live = vid < n;
+        float q[10];
+        int c[3] = {0, 0, 0};
+        float d[3] = {0.0f, 0.0f, 0.0f};
+        bool inside = true;
+        if (live)
+        {
+#pragma unroll
+            for (int i = 0; i < 10; ++i)
+                q[i] = qef[10 * vid + i];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                c[i] = coords[3 * vid + i];
+            inside = solve_free(q, c, g.vs, d);
+        }
+        const uint64_t m = __ballot(!inside);
+        uint64_t base = ~0ull;
+        if (m)
+        {
+            const int leader = __ffsll((long long)m) - 1;
+            if (lane == leader)
+            {
+                uint64_t first;
+                base = shard_claim_fit(box, shard, (uint64_t)__popcll(m), first) ? first : ~0ull;
+            }
+            base = __shfl(base, leader, 64);
+        }
+        if (!live)
+            continue;
+        if (!inside)
+        {
+            if (base != ~0ull)
+            {
+                BoxItem &e = box.items[base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull))];
+#pragma unroll
+                for (int i = 0; i < 10; ++i)
+                    e.q[i] = q[i];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                {
+                    e.v[i] = d[i];
+                    e.c[i] = c[i];
+                }
+                e.vid = vid;
+                continue;
+            }
+            solve_box(q, c, g.vs, d);
+        }
+        float *o = dual + 3 * vid;
+        o[0] = d[0];
+        o[1] = d[1];
+        o[2] = d[2];
+    }
+}
+
+// box-constrained search for the queued voxels, one per lane
+__global__ __launch_bounds__(BT) void k_solve_box(ShardBuf<BoxItem> box, GridDev g, float *dual)
+{
+    const uint64_t n = (uint64_t)NSHARD * box.per;
+    const uint64_t stride = (uint64_t)gridDim.x * BT;
+    for (uint64_t i = (uint64_t)blockIdx.x * BT + threadIdx.x; i < n; i += stride)
+    {
+        if (!shard_valid(box, i))
+            continue;
+        const BoxItem e = box.items[i];
+        float v[3] = {e.v[0], e.v[1], e.v[2]};
+        solve_box(e.q, e.c, g.vs, v);
+        float *o = dual + 3 * e.vid;
+        o[0] = v[0];
+        o[1] = v[1];
+        o[2] = v[2];
     }
 }
 
@@ -1762,8 +2017,8 @@ hipError_t launch_place_chunks(const ShardBuf<Chunk> &chunks, const uint32_t *ch
 hipError_t launch_active_bricks(const uint32_t *nch, uint64_t n_bricks, uint32_t *list, unsigned long long *count,
                                 hipStream_t s)
 {
-    hipLaunchKernelGGL(k_active_bricks, dim3(grid_for((int64_t)n_bricks, 65536)), dim3(BT), 0, s, nch, n_bricks,
-                       list, count);
+    hipLaunchKernelGGL(k_active_bricks, dim3(grid_for((int64_t)((n_bricks + ACT_PER - 1) / ACT_PER), 65536)), dim3(BT),
+                       0, s, nch, n_bricks, list, count);
     return hipGetLastError();
 }
 
@@ -1776,9 +2031,33 @@ hipError_t launch_fold_bricks(const FoldArgs &a, uint64_t max_bricks, hipStream_
     return hipGetLastError();
 }
 
-hipError_t launch_edge_count(const MeshDev &m, uint32_t *vcnt, uint32_t *voff, uint32_t *nbr, int pass, hipStream_t s)
+hipError_t launch_solve_voxels(const FoldArgs &a, const int32_t *coords, uint64_t n, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_edge_count, dim3(grid_for(3 * m.F, 65536)), dim3(BT), 0, s, m, vcnt, voff, nbr, pass);
+    hipLaunchKernelGGL(k_solve_voxels, dim3(grid_for((int64_t)n, 65536)), dim3(BT), 0, s, (const float *)a.qef,
+                       coords, n, a.g, a.dual, a.box);
+    return hipGetLastError();
+}
+
+hipError_t launch_solve_box(const FoldArgs &a, hipStream_t s)
+{
+    const uint64_t n = (uint64_t)NSHARD * a.box.per;
+    hipLaunchKernelGGL(k_solve_box, dim3(grid_for((int64_t)n, 16384)), dim3(BT), 0, s, a.box, a.g, a.dual);
+    return hipGetLastError();
+}
+
+hipError_t launch_fold_large(const FoldArgs &a, uint64_t max_bricks, hipStream_t s)
+{
+    uint64_t nb = max_bricks < 2048ull ? max_bricks : 2048ull;
+    if (nb == 0)
+        nb = 1;
+    hipLaunchKernelGGL(k_fold_large, dim3((unsigned)nb), dim3(BT), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_edge_count(const MeshDev &m, uint32_t *vcnt, uint32_t *rank, const uint32_t *voff, uint32_t *nbr,
+                             int pass, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_edge_count, dim3(grid_for(3 * m.F, 65536)), dim3(BT), 0, s, m, vcnt, rank, voff, nbr, pass);
     return hipGetLastError();
 }
 

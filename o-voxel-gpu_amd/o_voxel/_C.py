@@ -79,7 +79,7 @@ def _load():
         L.ovx_fdg_finish.argtypes = [vp, vp, vp, vp, vp]
         L.ovx_occ_begin.argtypes = [vp, vp, i64, ctypes.POINTER(_Grid), vp, ctypes.POINTER(i64)]
         L.ovx_occ_finish.argtypes = [vp, vp, vp]
-        L.ovx_fdg_stats.argtypes = [vp, ctypes.POINTER(i64)]
+        L.ovx_fdg_stats.argtypes = [vp, ctypes.POINTER(i64), ctypes.c_int]
         L.ovx_fdg_set_timing.argtypes = [vp, ctypes.c_int]
         L.ovx_fdg_stage_times.argtypes = [vp, ctypes.POINTER(f32)]
         L.ovx_fdg_set_capacity.argtypes = [vp, i64, i64]
@@ -280,11 +280,12 @@ def last_stats(device=None) -> dict:
     """Counts of the last conversion on ``device`` (synchronises its stream)."""
     dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
     ctx = _ctx(dev)
-    out = (ctypes.c_int64 * 8)()
-    with torch.cuda.device(dev), ctx.lock:
-        _check(_load().ovx_fdg_stats(ctx.h, out), "ovx_fdg_stats")
     keys = ("voxels", "events", "records", "face_records", "boundary_edges", "boundary_records",
-            "workspace_bytes", "giant_voxels")
+            "workspace_bytes", "giant_voxels", "active_bricks", "chunks", "large_bricks", "large_brick_records",
+            "box_voxels")
+    out = (ctypes.c_int64 * len(keys))()
+    with torch.cuda.device(dev), ctx.lock:
+        _check(_load().ovx_fdg_stats(ctx.h, out, len(keys)), "ovx_fdg_stats")
     return {k: int(v) for k, v in zip(keys, out)}
 
 

@@ -181,3 +181,24 @@ def test_buffer_overflow_fallbacks(oracle):
     for x, y, z in zip(base, small, again):
         assert np.array_equal(x, y) and np.array_equal(x, z)
     compare(oracle, vl, f, vs, gr, label="after overflow")
+
+
+def test_invalid_inputs_raise(oracle):
+    """A face index outside [0, V) or a non-finite vertex is reported as an
+    input error (the kernels that run before the report stay inside the
+    vertex array), and the context keeps working afterwards."""
+    v, f = meshgen.ico_cap()
+    vs, gr, a0 = meshgen.grid_args(64)
+    vl = (v - a0).astype(np.float32)
+    for bad in (len(vl), len(vl) + 1000, -1):
+        fb = f.astype(np.int32).copy()
+        fb[7, 1] = bad
+        print(f"[case] face index {bad}", flush=True)
+        with pytest.raises(RuntimeError, match="vertex index"):
+            run_gpu(vl, fb, vs, gr, 1.0, 0.2, 1e-2)
+    vb = vl.copy()
+    vb[3, 0] = np.nan
+    print("[case] non-finite vertex", flush=True)
+    with pytest.raises(RuntimeError, match="non-finite"):
+        run_gpu(vb, f, vs, gr, 1.0, 0.2, 1e-2)
+    compare(oracle, vl, f, vs, gr, label="after invalid inputs")
