@@ -72,10 +72,12 @@ def algorithmic_bytes(slot, st, F, V, n_words):
     """Compulsory HBM bytes of one launch of a kernel slot."""
     N, R, E, ev = st["voxels"], st["records"], st["boundary_edges"], st["events"]
     FR, BR = st["face_records"], st["boundary_records"]
-    # the fold reads every record once, the planes and boundary quadrics once,
-    # looks up each voxel's id (bitset word + word offset) and writes dual + flags;
-    # the two fold kernels split it in proportion to the records each folds
-    fold_all = 16 * R + 16 * F + 88 * E + 12 * N + 15 * N
+    # the fold reads every record once, each face plane and boundary quadric
+    # once, the (first record, count) pair of each chunk and the occupancy word
+    # + first voxel id of each brick row (64 rows of 12 B per brick), and writes
+    # the folded quadric (40 B) and flags (3 B) of every voxel; the two fold
+    # kernels split it in proportion to the records each folds
+    fold_all = (16 * R + 16 * F + 88 * E + 8 * st["chunks"] + 768 * st["active_bricks"] + 43 * N)
     large_share = st["large_brick_records"] / R if R else 0.0
     table = {
         # faces (12 B) + vertices (12 B) read once, 16 B plane per face written
@@ -99,6 +101,8 @@ def algorithmic_bytes(slot, st, F, V, n_words):
         "gen_bhits": 16 * BR + 16 * BR,
         "fold": (1.0 - large_share) * fold_all,
         "fold_large": large_share * fold_all,
+        # quadric (40 B) + coords (12 B) read, dual (12 B) written, 72 B per queued voxel
+        "solve": 64 * N + 72 * st["box_voxels"],
         # queued voxels (72 B) read, dual (12 B) written
         "solve_box": 84 * st["box_voxels"],
     }
@@ -143,6 +147,9 @@ def main():
     ap.add_argument("--workload", default="scene")
     ap.add_argument("--grid", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the timed steps, copy a 256 MiB buffer 5 times (a kernel of known traffic "
+                         "for scaling hardware byte counters)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -198,6 +205,13 @@ def main():
             per_kernel.setdefault(k, []).append(ms)
     torch.cuda.synchronize()
     t_end = time.perf_counter()
+    if args.calibrate_copy:
+        src = torch.ones(64 * 2 ** 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     ms_per_step = (t_end - t_start) * 1e3 / args.steps
     value = F / (ms_per_step * 1e-3) / 1e6
     st = _C.last_stats(dev)

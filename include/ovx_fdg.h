@@ -102,6 +102,24 @@ ovx_status ovx_occ_begin(ovx_fdg_ctx *ctx, const float *d_triangles, int64_t n_t
 ovx_status ovx_occ_finish(ovx_fdg_ctx *ctx, int32_t *d_coords, void *stream);
 
 /*
+ * Arithmetic of the QEF accumulation and solve for the next conversions on
+ * this context.  No counterpart in the reference: it is the precision
+ * variant beside the reference-faithful path.
+ *   OVX_PREC_REFERENCE (default): the reference CPU float path
+ *     (FDG.cpp:158-983) operation for operation; duals bit-identical to it.
+ *   OVX_PREC_COMPENSATED: each voxel's quadric is accumulated in the voxel's
+ *     own frame u = (p - min corner) / voxel_size with compensated FP32 sums
+ *     of error-free products; the 3x3 solve runs in FP32 and falls back to
+ *     FP64 (pivoting, rank decision, back substitution) when the system is
+ *     rank-deficient or |R00 / R22| > cond_max.  Occupancy and flags are the
+ *     same as the reference path; duals approach the FP64 solution.
+ * cond_max >= 1.  Must not be called between begin and finish.
+ */
+#define OVX_PREC_REFERENCE 0
+#define OVX_PREC_COMPENSATED 1
+ovx_status ovx_fdg_set_precision(ovx_fdg_ctx *ctx, int mode, float cond_max);
+
+/*
  * Per-stage statistics of the last conversion on this context; the first
  * min(n, OVX_FDG_N_STATS) are written:
  *   out[0] N voxels          out[1] scanline events     out[2] QEF records
@@ -111,9 +129,10 @@ ovx_status ovx_occ_finish(ovx_fdg_ctx *ctx, int32_t *d_coords, void *stream);
  *   out[10] bricks folded by the large-brick kernel    out[11] their records
  *   out[12] voxels queued for the box-constrained search (the rest of the voxels
  *           whose minimiser left the box are searched in place)
+ *   out[13] voxels solved on the FP64 path (precision variant; 0 otherwise)
  * Reading them synchronises the context's last stream.
  */
-#define OVX_FDG_N_STATS 13
+#define OVX_FDG_N_STATS 14
 ovx_status ovx_fdg_stats(ovx_fdg_ctx *ctx, int64_t *out, int n);
 
 /*
@@ -130,13 +149,15 @@ ovx_status ovx_fdg_set_capacity(ovx_fdg_ctx *ctx, int64_t events, int64_t hits);
  * ovx_fdg_set_timing(ctx, 1).
  *   ovx_fdg_kernel_times: per timing slot (ovx_fdg_kernel_count() slots, named
  *     by ovx_fdg_kernel_name), -1 for kernels that did not run.
- *   ovx_fdg_stage_times: [prep+mark, compact, count, emit, fold+solve].
+ *   ovx_fdg_stage_times: the reference's four timed stages (FDG.cpp:711-986):
+ *     [intersection QEF, face QEF, boundary QEF, dual vertices], each the sum
+ *     of the kernels that build or consume that stage's records.
  */
 ovx_status ovx_fdg_set_timing(ovx_fdg_ctx *ctx, int enable);
 int ovx_fdg_kernel_count(void);
 const char *ovx_fdg_kernel_name(int slot);
 ovx_status ovx_fdg_kernel_times(ovx_fdg_ctx *ctx, float *ms, int n);
-ovx_status ovx_fdg_stage_times(ovx_fdg_ctx *ctx, float *ms5);
+ovx_status ovx_fdg_stage_times(ovx_fdg_ctx *ctx, float *ms4);
 
 #ifdef __cplusplus
 }

@@ -87,6 +87,8 @@ struct ovx_fdg_ctx
     GridDev grid{};
     uint64_t n_words = 0;
     float fw = 0, bw = 0, rw = 0;
+    int prec = 0;           // 0 reference float path, 1 compensated voxel-frame accumulation
+    float cond_max = 1.0e6f; // precision variant: FP64 solve above this condition estimate
     int64_t N = 0;
     // statistics of the last conversion
     int64_t stats[OVX_FDG_N_STATS] = {};
@@ -171,8 +173,8 @@ ovx_status check_grid(const ovx_grid *grid, GridDev &g, uint64_t &n_words)
     }
     g.wx = (g.G[0] + 63) / 64;
     n_words = (uint64_t)g.G[2] * (uint64_t)g.G[1] * (uint64_t)g.wx;
-    if (n_words > (1ull << 32))
-        return fail(OVX_ERR_CAPACITY, "occupancy bitset of %llu words exceeds 2^32", (unsigned long long)n_words);
+    if (n_words >= (1ull << 32))
+        return fail(OVX_ERR_CAPACITY, "occupancy bitset of %llu words exceeds 2^32 - 1", (unsigned long long)n_words);
     return OVX_OK;
 }
 
@@ -328,8 +330,13 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int64_t V, const ovx_g
             return st;
         eb = shard_view<Event>(c, c->events, SB_EVENTS);
     }
+    // the LDS merging of occupancy bits pays when crossings are dense (faces
+    // spanning several voxel rows); with many more faces than voxel rows per
+    // plane the crossings are sparse and the table only costs occupancy
+    const double rows_per_plane = std::max({(double)g.G[0] * g.G[1], (double)g.G[1] * g.G[2], (double)g.G[0] * g.G[2]});
+    const bool merge = (double)m.F <= 2.0 * rows_per_plane;
     tic(c, KS_MARK, s);
-    HIP_TRY(launch_scan_events(m, g, ctr, eb, s));
+    HIP_TRY(launch_scan_events(m, g, ctr, eb, merge, s));
     toc(c, KS_MARK, s);
     tic(c, KS_WORDSCAN, s);
     HIP_TRY(launch_scan_popc(g.bits, n_words, (uint32_t *)c->woff.p, (uint64_t *)c->tiles.p, ctr + CTR_WORDS_TOTAL,
@@ -383,7 +390,7 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int64_t V, const ovx_g
                 return st;
             ShardBuf<Event> eb2 = shard_view<Event>(c, c->events, SB_EVENTS);
             tic(c, KS_MARK, s);
-            HIP_TRY(launch_scan_events(m, g, nullptr, eb2, s));
+            HIP_TRY(launch_scan_events(m, g, nullptr, eb2, merge, s));
             toc(c, KS_MARK, s);
         }
         c->E = (int64_t)c->h_ctr[CTR_BEDGE_TOTAL];
@@ -514,25 +521,24 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
 
     // ---- chunks grouped by brick, bricks with records, then the per-brick ordered fold + solve
     const uint64_t n_chunks = c->h_ctr[CTR_SUM + SB_CHUNKS];
-    if ((st = ensure(c->choff, (n_bricks + 1) * 4)) || (st = ensure(c->bchunks, (n_chunks + 1) * 4)) ||
+    if ((st = ensure(c->choff, (n_bricks + 1) * 4)) || (st = ensure(c->bchunks, (n_chunks + 1) * 8)) ||
         (st = ensure(c->active, n_bricks * 4)) || (st = ensure(c->large, n_bricks * 4)) ||
         // box-search queue: a few percent of the voxels typically; the solve
         // kernel solves in place whatever does not fit
         (st = ensure_shard(c, c->box, SB_BOX, per_for((uint64_t)c->N / 8 + 4096), sizeof(BoxItem))) ||
-        (st = ensure(c->qef, (size_t)c->N * 40)) ||
+        (st = ensure(c->qef, (size_t)c->N * (c->prec ? 80 : 40))) ||
         (st = ensure(c->tiles, (scan_tile_count(n_bricks + 1) + 64) * 8)))
         return st;
     tic(c, KS_ACTIVE, s);
     HIP_TRY(launch_scan_u32((const uint32_t *)c->heads.p, n_bricks + 1, (uint32_t *)c->choff.p, (uint64_t *)c->tiles.p,
                             ctr + CTR_CHUNK_SCAN, s));
     HIP_TRY(launch_place_chunks(shard_view<Chunk>(c, c->chunks, SB_CHUNKS), (const uint32_t *)c->choff.p,
-                                (uint32_t *)c->bchunks.p, s));
+                                (uint2 *)c->bchunks.p, s));
     HIP_TRY(launch_active_bricks((const uint32_t *)c->heads.p, n_bricks, (uint32_t *)c->active.p, ctr + CTR_ACTIVE, s));
     toc(c, KS_ACTIVE, s);
     FoldArgs a{};
     a.choff = (const uint32_t *)c->choff.p;
-    a.bchunks = (const uint32_t *)c->bchunks.p;
-    a.chunks = (const Chunk *)c->chunks.p;
+    a.gchunks = (const uint2 *)c->bchunks.p;
     a.recs = (const Rec *)c->recs.p;
     a.active = (const uint32_t *)c->active.p;
     a.n_active = ctr + CTR_ACTIVE;
@@ -549,6 +555,9 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
     a.rw = c->rw;
     a.qef = (float *)c->qef.p;
     a.box = shard_view<BoxItem>(c, c->box, SB_BOX);
+    a.prec = c->prec;
+    a.cond_max = c->cond_max;
+    a.n64 = (unsigned long long *)c->shard_ctr.p + (uint64_t)SB_FP64 * NSHARD * SHARD_STRIDE;
     a.ctr = ctr;
     HIP_TRY(hipMemsetAsync(ctr + CTR_LARGE, 0, 3 * sizeof(unsigned long long), s)); // LARGE, LARGE_RECS, BOX
     tic(c, KS_FOLD, s);
@@ -714,6 +723,7 @@ ovx_status ovx_fdg_stats(ovx_fdg_ctx *c, int64_t *out, int n)
         c->stats[10] = (int64_t)c->h_ctr[CTR_LARGE];
         c->stats[11] = (int64_t)c->h_ctr[CTR_LARGE_RECS];
         c->stats[12] = (int64_t)c->h_ctr[CTR_SUM + SB_BOX];
+        c->stats[13] = c->prec ? (int64_t)c->h_ctr[CTR_SUM + SB_FP64] : 0;
     }
     c->stats[6] = held_bytes(c);
     for (int i = 0; i < n && i < OVX_FDG_N_STATS; ++i)
@@ -764,6 +774,19 @@ ovx_status ovx_fdg_set_capacity(ovx_fdg_ctx *c, int64_t events, int64_t hits)
     return OVX_OK;
 }
 
+ovx_status ovx_fdg_set_precision(ovx_fdg_ctx *c, int mode, float cond_max)
+{
+    if (!c || (mode != OVX_PREC_REFERENCE && mode != OVX_PREC_COMPENSATED) || !(cond_max >= 1.0f))
+        return fail(OVX_ERR_INVALID_ARGUMENT, "invalid precision arguments");
+    DeviceGuard dg(c->device);
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (c->pending)
+        return fail(OVX_ERR_STATE, "precision changed between begin and finish");
+    c->prec = mode;
+    c->cond_max = cond_max;
+    return OVX_OK;
+}
+
 int ovx_fdg_kernel_count(void) { return KS_N; }
 
 const char *ovx_fdg_kernel_name(int slot) { return (slot >= 0 && slot < KS_N) ? kslot_names[slot] : ""; }
@@ -787,18 +810,19 @@ ovx_status ovx_fdg_kernel_times(ovx_fdg_ctx *c, float *ms, int n)
     return OVX_OK;
 }
 
-ovx_status ovx_fdg_stage_times(ovx_fdg_ctx *c, float *ms5)
+ovx_status ovx_fdg_stage_times(ovx_fdg_ctx *c, float *ms4)
 {
+    if (!ms4)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "null argument");
     float k[KS_N];
     ovx_status st = ovx_fdg_kernel_times(c, k, KS_N);
     if (st)
         return st;
     auto v = [&](int i) { return k[i] > 0.0f ? k[i] : 0.0f; };
-    ms5[0] = v(KS_PREP) + v(KS_MARK);
-    ms5[1] = v(KS_WORDSCAN) + v(KS_COORDS);
-    ms5[2] = v(KS_EDGES);
-    ms5[3] = v(KS_GEN_EVENTS) + v(KS_FACE) + v(KS_GEN_FHITS) + v(KS_BDRY) + v(KS_GEN_BHITS);
-    ms5[4] = v(KS_ACTIVE) + v(KS_FOLD) + v(KS_FOLD_LARGE) + v(KS_SOLVE) + v(KS_SOLVE_BOX);
+    ms4[0] = v(KS_PREP) + v(KS_MARK) + v(KS_WORDSCAN) + v(KS_COORDS) + v(KS_GEN_EVENTS);
+    ms4[1] = v(KS_FACE) + v(KS_GEN_FHITS);
+    ms4[2] = v(KS_EDGES) + v(KS_BDRY) + v(KS_GEN_BHITS);
+    ms4[3] = v(KS_ACTIVE) + v(KS_FOLD) + v(KS_FOLD_LARGE) + v(KS_SOLVE) + v(KS_SOLVE_BOX);
     return OVX_OK;
 }
 

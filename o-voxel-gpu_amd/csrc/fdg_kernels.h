@@ -36,7 +36,8 @@ enum
     SB_RECS = 3,
     SB_CHUNKS = 4,
     SB_BOX = 5, // voxels queued for the box-constrained search
-    SB_N = 6
+    SB_FP64 = 6, // counters only: voxels solved on the FP64 path (precision variant)
+    SB_N = 7
 };
 
 // Append / allocation counters are split over NSHARD counters 128 bytes
@@ -136,8 +137,7 @@ struct BoxItem
 struct FoldArgs
 {
     const uint32_t *choff;              // [bricks+1] first chunk of each brick in `bchunks`
-    const uint32_t *bchunks;            // chunk indices grouped by brick
-    const Chunk *chunks;
+    const uint2 *gchunks;               // (first record, count) of each chunk, grouped by brick
     const Rec *recs;
     const uint32_t *active;             // bricks with records
     const unsigned long long *n_active; // their count (device)
@@ -149,23 +149,26 @@ struct FoldArgs
     const BEdge *bedges;
     float *dual;
     uint8_t *flags;
-    float *qef; // [N][10] folded, regularised quadric of each voxel
+    float *qef; // [N][10] folded, regularised quadric of each voxel ([N][20], h then l, precision variant)
     float bw, rw;
     ShardBuf<BoxItem> box; // box-search queue
+    int prec;              // 0 reference float path, 1 compensated voxel-frame accumulation
+    float cond_max;        // precision variant: FP64 solve above this condition estimate
+    unsigned long long *n64; // precision variant: sharded counters of FP64 solves
     unsigned long long *ctr;
 };
 
 hipError_t launch_prep(const MeshDev &m, int64_t V, const GridDev &g, float4 *planes, unsigned long long *ctr,
                        hipStream_t s);
 hipError_t launch_scan_events(const MeshDev &m, const GridDev &g, unsigned long long *ctr, const ShardBuf<Event> &eb,
-                              hipStream_t s);
+                              bool merge, hipStream_t s);
 hipError_t launch_face(const MeshDev &m, const GridDev &g, const ShardBuf<FHit> &hb, unsigned long long *ctr,
                        hipStream_t s);
 hipError_t launch_gen_events(const ShardBuf<Event> &ev, const GridDev &g, const BucketArgs &ba, hipStream_t s);
 hipError_t launch_gen_fhits(const ShardBuf<FHit> &h, const GridDev &g, const BucketArgs &ba, hipStream_t s);
 hipError_t launch_gen_bhits(const ShardBuf<BHit> &h, const GridDev &g, const BucketArgs &ba, hipStream_t s);
 hipError_t launch_shard_reduce(unsigned long long *shard_ctr, unsigned long long *ctr, hipStream_t s);
-hipError_t launch_place_chunks(const ShardBuf<Chunk> &chunks, const uint32_t *choff, uint32_t *bchunks,
+hipError_t launch_place_chunks(const ShardBuf<Chunk> &chunks, const uint32_t *choff, uint2 *gchunks,
                                hipStream_t s);
 hipError_t launch_active_bricks(const uint32_t *nch, uint64_t n_bricks, uint32_t *list, unsigned long long *count,
                                 hipStream_t s);

@@ -14,6 +14,8 @@
 // large and small triangles balance across the wave64 lanes.
 #include "fdg_device.h"
 #include "fdg_kernels.h"
+
+#include <type_traits>
 #include "fdg_solve.h"
 
 namespace ovx
@@ -367,8 +369,66 @@ __device__ __forceinline__ void event_voxels(int ax0, int ax1, int x_idx, int y_
         }
 }
 
-// set the occupancy bits of the 4 voxels of a crossing (one atomicOr per word)
-__device__ __forceinline__ void mark_event(const GridDev &g, int ax0, int ax1, int x_idx, int y_idx, int z_idx)
+// Block-local OR-combining table for occupancy words: the crossings of a
+// block's faces fall on few bitset words, so their bits are merged in LDS and
+// each distinct word is written with one global atomicOr when the table is
+// flushed (device-scope atomics are far costlier than LDS ones).
+constexpr int MARK_LOG2 = 11;
+constexpr int MARK_TAB = 1 << MARK_LOG2;
+constexpr uint32_t MARK_EMPTY = 0xffffffffu; // word indices are < 2^32 - 1
+
+struct MarkTable
+{
+    uint32_t key[MARK_TAB];
+    unsigned long long mask[MARK_TAB];
+};
+struct NoMarkTable // kernel variant without merging: no LDS
+{
+    uint32_t key[1];
+    unsigned long long mask[1];
+};
+
+template <class Tab>
+__device__ __forceinline__ void mark_word(const GridDev &g, Tab &T, uint64_t w, uint64_t msk)
+{
+    const uint32_t key = (uint32_t)w;
+    uint32_t h = (key * 2654435761u) >> (32 - MARK_LOG2);
+#pragma unroll 1
+    for (int probe = 0; probe < 16; ++probe)
+    {
+        const uint32_t prev = atomicCAS(&T.key[h], MARK_EMPTY, key);
+        if (prev == MARK_EMPTY || prev == key)
+        {
+            atomicOr(&T.mask[h], (unsigned long long)msk);
+            return;
+        }
+        h = (h + 1u) & (uint32_t)(MARK_TAB - 1);
+    }
+    atomicOr((unsigned long long *)(g.bits + w), (unsigned long long)msk); // table crowded
+}
+
+// writes the table's words to the bitset and empties it; contains barriers
+template <class Tab>
+__device__ __forceinline__ void mark_flush(const GridDev &g, Tab &T)
+{
+    __syncthreads();
+    for (int i = threadIdx.x; i < MARK_TAB; i += BT)
+    {
+        const uint32_t key = T.key[i];
+        if (key != MARK_EMPTY)
+        {
+            atomicOr((unsigned long long *)(g.bits + key), T.mask[i]);
+            T.key[i] = MARK_EMPTY;
+            T.mask[i] = 0ull;
+        }
+    }
+    __syncthreads();
+}
+
+// set the occupancy bits of the 4 voxels of a crossing (one update per word,
+// in the table or directly)
+template <bool MERGE, class Tab>
+__device__ __forceinline__ void mark_event(const GridDev &g, Tab &T, int ax0, int ax1, int x_idx, int y_idx, int z_idx)
 {
     int c[4][3];
     event_voxels(ax0, ax1, x_idx, y_idx, z_idx, c);
@@ -397,9 +457,17 @@ __device__ __forceinline__ void mark_event(const GridDev &g, int ax0, int ax1, i
 #pragma unroll
     for (int k = 0; k < 4; ++k)
         if (msk[k])
-            atomicOr((unsigned long long *)(g.bits + w[k]), (unsigned long long)msk[k]);
+        {
+            if constexpr (MERGE)
+                mark_word(g, T, w[k], msk[k]);
+            else
+                atomicOr((unsigned long long *)(g.bits + w[k]), (unsigned long long)msk[k]);
+        }
 }
 
+// MERGE: the occupancy bits of a block iteration's crossings are merged in an
+// LDS table first (pays when crossings are dense; the table costs occupancy)
+template <bool MERGE>
 __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, GridDev g, unsigned long long *counters,
                                                     ShardBuf<Event> eb)
 {
@@ -410,9 +478,19 @@ __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, GridDev g, unsign
     __shared__ uint32_t incl_items[BT];
     __shared__ uint32_t incl_rows[BT];
     __shared__ uint32_t sh[NWAVE];
+    __shared__ typename std::conditional<MERGE, MarkTable, NoMarkTable>::type T;
     const int tid = threadIdx.x;
     const int64_t n_items = 3 * m.F;
     uint32_t n_ev = 0;
+    if constexpr (MERGE)
+    {
+        for (int i = tid; i < MARK_TAB; i += BT)
+        {
+            T.key[i] = MARK_EMPTY;
+            T.mask[i] = 0ull;
+        }
+        __syncthreads();
+    }
 
     for (int64_t base = (int64_t)blockIdx.x * BT; base < n_items; base += (int64_t)gridDim.x * BT)
     {
@@ -557,7 +635,7 @@ __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, GridDev g, unsign
                         const uint64_t fa = (uint64_t)it.face * 3u + (uint64_t)ax2;
                         const uint64_t keybase = (fa << KEY_FA_SHIFT) | ((uint64_t)(rw.y_idx - ylo) << KEY_Y_SHIFT) |
                                                  ((uint64_t)(x_idx - xlo) << KEY_X_SHIFT);
-                        mark_event(g, ax0, ax1, x_idx, rw.y_idx, z_idx);
+                        mark_event<MERGE>(g, T, ax0, ax1, x_idx, rw.y_idx, z_idx);
                         evr.key = keybase;
                         evr.z = (float)z;
                         evr.zi = z_idx;
@@ -585,7 +663,8 @@ __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, GridDev g, unsign
             }
             __syncthreads();
         }
-        __syncthreads();
+        if constexpr (MERGE)
+            mark_flush(g, T);
     }
     (void)counters;
     (void)n_ev;
@@ -626,6 +705,27 @@ __device__ __forceinline__ bool face_pass(const FaceItem &it, const float vs[3],
         if ((it.nzx[k][0] * p2 + it.nzx[k][1] * p0) + it.dzx[k] < 0.0f)
             return false;
     return true;
+}
+
+// position of the k-th (0-based) set bit of m; k < popcount(m)
+__device__ __forceinline__ int select_bit(uint64_t m, uint32_t k)
+{
+    int pos = 0;
+#pragma unroll
+    for (int half = 32; half >= 1; half >>= 1)
+    {
+        const uint64_t low = m & ((1ull << half) - 1ull);
+        const uint32_t c = (uint32_t)__popcll(low);
+        if (k >= c)
+        {
+            k -= c;
+            m >>= half;
+            pos += half;
+        }
+        else
+            m = low;
+    }
+    return pos;
 }
 
 __device__ __forceinline__ uint64_t row_mask(int lo_bit, int hi_bit)
@@ -776,9 +876,7 @@ __global__ __launch_bounds__(BT) void k_face(MeshDev m, GridDev g, ShardBuf<FHit
                             k -= pc;
                             continue;
                         }
-                        for (uint32_t j = 0; j < k; ++j)
-                            mm &= mm - 1ull;
-                        const int b = __builtin_ctzll(mm);
+                        const int b = select_bit(mm, k);
                         const int x = g.gmin[0] + w * 64 + b;
                         if (face_pass(it, g.vs, x, row.y, row.z))
                         {
@@ -1160,8 +1258,8 @@ __global__ __launch_bounds__(BT) void k_shard_reduce(const unsigned long long *s
     }
 }
 
-// chunk indices grouped by brick: position = first chunk of the brick + rank
-__global__ __launch_bounds__(BT) void k_place_chunks(ShardBuf<Chunk> chunks, const uint32_t *choff, uint32_t *bchunks)
+// chunks grouped by brick as (first record, count): position = first chunk of the brick + rank
+__global__ __launch_bounds__(BT) void k_place_chunks(ShardBuf<Chunk> chunks, const uint32_t *choff, uint2 *gchunks)
 {
     const uint64_t n = (uint64_t)NSHARD * chunks.per;
     const uint64_t stride = (uint64_t)gridDim.x * BT;
@@ -1170,7 +1268,7 @@ __global__ __launch_bounds__(BT) void k_place_chunks(ShardBuf<Chunk> chunks, con
         if (!shard_valid(chunks, c))
             continue;
         const Chunk ch = chunks.items[c];
-        bchunks[choff[ch.brick] + ch.rank] = (uint32_t)c;
+        gchunks[choff[ch.brick] + ch.rank] = make_uint2(ch.start, ch.count);
     }
 }
 
@@ -1311,7 +1409,7 @@ struct FoldState
     float m0, m1, m2, cnt;
     int fl;
 
-    __device__ __forceinline__ void init()
+    __device__ __forceinline__ void init(int, int, int, const FoldArgs &)
     {
 #pragma unroll
         for (int k = 0; k < 10; ++k)
@@ -1404,7 +1502,8 @@ struct FoldState
         }
     }
 
-    __device__ __forceinline__ void finish(int x, int y, int z, const FoldArgs &a)
+    // regularised quadric and flags of the voxel in output row vid
+    __device__ __forceinline__ void finish_at(uint64_t vid, const FoldArgs &a)
     {
         if (a.rw > 0.0f && cnt > 0.0f)
         {
@@ -1415,9 +1514,6 @@ struct FoldState
             for (int k = 0; k < 10; ++k)
                 q[k] = q[k] + s * R[k];
         }
-        int b;
-        const uint64_t w = word_of(a.g, x, y, z, b);
-        const uint64_t vid = (uint64_t)voxel_id(a.g, w, b);
         float *qo = a.qef + 10 * vid;
 #pragma unroll
         for (int i = 0; i < 10; ++i)
@@ -1427,6 +1523,221 @@ struct FoldState
         fo[1] = (uint8_t)((fl >> 1) & 1);
         fo[2] = (uint8_t)((fl >> 2) & 1);
     }
+
+    __device__ __forceinline__ void finish(int x, int y, int z, const FoldArgs &a)
+    {
+        int b;
+        const uint64_t w = word_of(a.g, x, y, z, b);
+        finish_at((uint64_t)voxel_id(a.g, w, b), a);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// precision variant: the quadric of each voxel is accumulated in the voxel's
+// own frame u = (p - c) / vs (c = the voxel's min corner), where its entries
+// are O(vs^2) instead of the O(1) - O(1) cancellations of world coordinates,
+// with compensated FP32 sums (TwoSum) of error-free products (TwoProduct);
+// the world problem min_p [p,1]^T Q [p,1] over the voxel is the same problem
+// over u in [0,1]^3.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e)
+{
+    s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+__device__ __forceinline__ void two_prod(float a, float b, float &p, float &e)
+{
+    p = a * b;
+    e = __fmaf_rn(a, b, -p);
+}
+
+// compensated sum: value = h + l
+struct CSum
+{
+    float h, l;
+    __device__ __forceinline__ void zero() { h = l = 0.0f; }
+    __device__ __forceinline__ void add(float x)
+    {
+        float s, e;
+        two_sum(h, x, s, e);
+        h = s;
+        l = l + e;
+    }
+    __device__ __forceinline__ void add_prod(float a, float b)
+    {
+        float p, e;
+        two_prod(a, b, p, e);
+        add(p);
+        l = l + e;
+    }
+    __device__ __forceinline__ void add_double(double x)
+    {
+        const float xh = (float)x;
+        add(xh);
+        add((float)(x - (double)xh));
+    }
+};
+
+struct FoldStateC
+{
+    CSum q[10];
+    CSum m[3]; // sum of the intersection points in the voxel frame
+    float cnt;
+    int fl;
+    float c[3]; // voxel min corner (world)
+
+    __device__ __forceinline__ void init(int x, int y, int z, const FoldArgs &a)
+    {
+#pragma unroll
+        for (int k = 0; k < 10; ++k)
+            q[k].zero();
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            m[k].zero();
+        cnt = 0.0f;
+        fl = 0;
+        c[0] = (float)x * a.g.vs[0];
+        c[1] = (float)y * a.g.vs[1];
+        c[2] = (float)z * a.g.vs[2];
+    }
+
+    // plane n.p + d = 0 in the voxel frame: (n * vs).u + (n.c + d), the
+    // offset formed from exact products and compensated sums
+    __device__ __forceinline__ void add_plane(const float4 pl4, const FoldArgs &a)
+    {
+        float w[4];
+        w[0] = pl4.x * a.g.vs[0];
+        w[1] = pl4.y * a.g.vs[1];
+        w[2] = pl4.z * a.g.vs[2];
+        CSum d;
+        d.h = pl4.w;
+        d.l = 0.0f;
+        d.add_prod(pl4.x, c[0]);
+        d.add_prod(pl4.y, c[1]);
+        d.add_prod(pl4.z, c[2]);
+        w[3] = d.h + d.l;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = i; j < 4; ++j)
+                q[qidx(i, j)].add_prod(w[i], w[j]);
+    }
+
+    __device__ __forceinline__ void add_pl(uint64_t key, float z, const float4 pl4, const FoldArgs &a)
+    {
+        const int phase = (int)(key >> KEY_PHASE_SHIFT);
+        if (phase == 0)
+        {
+            const uint64_t fa = (key >> KEY_FA_SHIFT) & KEY_FA_MASK;
+            const uint32_t f = (uint32_t)(fa / 3u);
+            const int ax2 = (int)(fa - 3ull * f);
+            const int ax0 = (ax2 + 1) % 3;
+            const int ax1 = (ax2 + 2) % 3;
+            const int y_idx =
+                (int)((key >> KEY_Y_SHIFT) & KEY_IDX_MASK) + sel3i(ax1, a.g.gmin[0], a.g.gmin[1], a.g.gmin[2]);
+            const int x_idx =
+                (int)((key >> KEY_X_SHIFT) & KEY_IDX_MASK) + sel3i(ax0, a.g.gmin[0], a.g.gmin[1], a.g.gmin[2]);
+            const float px = (float)(x_idx + 1) * sel3f(ax0, a.g.vs[0], a.g.vs[1], a.g.vs[2]);
+            const float py = (float)(y_idx + 1) * sel3f(ax1, a.g.vs[0], a.g.vs[1], a.g.vs[2]);
+            const float p[3] = {ax0 == 0 ? px : (ax1 == 0 ? py : z), ax0 == 1 ? px : (ax1 == 1 ? py : z),
+                                ax0 == 2 ? px : (ax1 == 2 ? py : z)};
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                m[i].add((p[i] - c[i]) / a.g.vs[i]);
+            cnt = cnt + 1.0f;
+            add_plane(pl4, a);
+            if ((key & 3ull) == 0ull)
+                fl |= 1 << ax2;
+        }
+        else if (phase == 1)
+            add_plane(pl4, a);
+        else
+        {
+            // bw |A (p - p0)|^2 with A = I - d d^T: in the voxel frame
+            // Q_ij = vs_i vs_j A_ij, Q_i3 = -vs_i (A r0)_i, Q_33 = r0^T A r0, r0 = p0 - c
+            const uint64_t e = key & ((1ull << KEY_PHASE_SHIFT) - 1ull);
+            const BEdge &be = a.bedges[e];
+            double A[3][3], r0[3], Ar[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+            {
+                r0[i] = (double)be.v0[i] - (double)c[i];
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    A[i][j] = (i == j ? 1.0 : 0.0) - be.dir[i] * be.dir[j];
+            }
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                Ar[i] = A[i][0] * r0[0] + A[i][1] * r0[1] + A[i][2] * r0[2];
+            const double bw = (double)a.bw;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+            {
+#pragma unroll
+                for (int j = i; j < 3; ++j)
+                    q[qidx(i, j)].add_double(bw * ((double)a.g.vs[i] * (double)a.g.vs[j] * A[i][j]));
+                q[qidx(i, 3)].add_double(bw * (-(double)a.g.vs[i] * Ar[i]));
+            }
+            q[qidx(3, 3)].add_double(bw * (r0[0] * Ar[0] + r0[1] * Ar[1] + r0[2] * Ar[2]));
+        }
+    }
+
+    __device__ __forceinline__ void add(uint64_t key, float z, const FoldArgs &a)
+    {
+        const uint32_t f = FoldState::plane_of(key);
+        add_pl(key, z, f != FoldState::NO_PLANE ? a.planes[f] : make_float4(0.0f, 0.0f, 0.0f, 0.0f), a);
+    }
+
+    // regularisation rw cnt |p - mean|^2 in the voxel frame, scaling by a
+    // power of two (largest diagonal entry to [1, 2)), quadric (h and l
+    // parts) and flags of the voxel in output row vid
+    __device__ __forceinline__ void finish_at(uint64_t vid, const FoldArgs &a)
+    {
+        if (a.rw > 0.0f && cnt > 0.0f)
+        {
+            const float s = a.rw * cnt;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+            {
+                const float mu = (m[i].h + m[i].l) / cnt;
+                const float t = s * (a.g.vs[i] * a.g.vs[i]);
+                q[qidx(i, i)].add(t);
+                q[qidx(i, 3)].add_prod(-t, mu);
+                float tm, te;
+                two_prod(t, mu, tm, te);
+                q[qidx(3, 3)].add_prod(tm, mu);
+                q[qidx(3, 3)].add_prod(te, mu);
+            }
+        }
+        const float dmax = fmaxf(fmaxf(q[qidx(0, 0)].h, q[qidx(1, 1)].h), q[qidx(2, 2)].h);
+        int ex = 0;
+        if (dmax > 0.0f && dmax <= 3.0e38f)
+            frexpf(dmax, &ex);
+        float *qo = a.qef + 20 * vid;
+#pragma unroll
+        for (int i = 0; i < 10; ++i)
+        {
+            qo[i] = ldexpf(q[i].h, 1 - ex);
+            qo[10 + i] = ldexpf(q[i].l, 1 - ex);
+        }
+        uint8_t *fo = a.flags + 3 * vid;
+        fo[0] = (uint8_t)(fl & 1);
+        fo[1] = (uint8_t)((fl >> 1) & 1);
+        fo[2] = (uint8_t)((fl >> 2) & 1);
+    }
+};
+
+template <int P>
+struct FoldPick
+{
+    using type = FoldState;
+};
+template <>
+struct FoldPick<1>
+{
+    using type = FoldStateC;
 };
 
 constexpr int FOLD_SMALL = 2048; // records of a brick held in LDS by the main fold kernel (4 blocks per CU)
@@ -1452,6 +1763,8 @@ struct FoldLds
     uint32_t off[513];   // first grouped slot of each voxel
     uint32_t cur[512];   // per-voxel record counts
     uint16_t vlist[512]; // voxels with records
+    uint64_t rbits[64];  // occupancy word of each (y, z) row of the brick
+    uint32_t roff[64];   // ... and its first voxel id
     uint32_t sh[NWAVE];
     uint32_t n, m;
     ChunkBatch cb;
@@ -1470,9 +1783,9 @@ __device__ __forceinline__ void for_each_rec(const FoldArgs &a, uint32_t brick, 
         uint32_t cnt = 0;
         if (ci < c1)
         {
-            const Chunk ch = a.chunks[a.bchunks[ci]];
-            cb.start[threadIdx.x] = ch.start;
-            cnt = ch.count;
+            const uint2 ch = a.gchunks[ci];
+            cb.start[threadIdx.x] = ch.x;
+            cnt = ch.y;
         }
         uint32_t tot;
         const uint32_t ex = block_excl_scan(cnt, cb.sh, tot);
@@ -1492,7 +1805,7 @@ __device__ __forceinline__ void for_each_rec(const FoldArgs &a, uint32_t brick, 
 // histogram: group them by voxel (counting sort), order each voxel's records
 // by key (rank sort, keys are unique), replay them in that order, one thread
 // per voxel with records, and solve.  Contains barriers.
-template <int CAP>
+template <int CAP, int P>
 __device__ __forceinline__ void fold_loaded(const FoldArgs &a, FoldLds<CAP> &L, uint32_t n, int x0, int y0, int z0)
 {
     const int tid = threadIdx.x;
@@ -1550,8 +1863,8 @@ __device__ __forceinline__ void fold_loaded(const FoldArgs &a, FoldLds<CAP> &L, 
     {
         const int l = L.vlist[vi];
         const uint32_t s0 = L.off[l], s1 = L.off[l + 1];
-        FoldState st;
-        st.init();
+        typename FoldPick<P>::type st;
+        st.init(x0 + (l & 7), y0 + ((l >> 3) & 7), z0 + (l >> 6), a);
         for (uint32_t i = s0; i < s1; i += 4)
         {
             // the planes of up to 4 records are fetched together, then folded in order
@@ -1579,9 +1892,41 @@ __device__ __forceinline__ void fold_loaded(const FoldArgs &a, FoldLds<CAP> &L, 
                 if (i + u < s1)
                     st.add_pl(kk[u], zz[u], pp[u], a);
         }
-        st.finish(x0 + (l & 7), y0 + ((l >> 3) & 7), z0 + (l >> 6), a);
+        st.finish_at(row_vid(a, L, x0, l), a);
     }
     __syncthreads();
+}
+
+// Output rows of the brick's voxels: the 8 voxels of a (y, z) row of a brick
+// share one 64-bit occupancy word (bricks start at multiples of 8 along x).
+// Loaded by the first wave; read after the block's next barrier.
+template <int CAP>
+__device__ __forceinline__ void load_rows(const FoldArgs &a, FoldLds<CAP> &L, int x0, int y0, int z0)
+{
+    const int r = threadIdx.x;
+    if (r < 64)
+    {
+        const int y = y0 + (r & 7), z = z0 + (r >> 3);
+        uint64_t m = 0;
+        uint32_t o = 0;
+        if (y < a.g.gmax[1] && z < a.g.gmax[2])
+        {
+            int bit;
+            const uint64_t w = word_of(a.g, x0, y, z, bit);
+            m = a.g.bits[w];
+            o = a.g.woff[w];
+        }
+        L.rbits[r] = m;
+        L.roff[r] = o;
+    }
+}
+
+template <int CAP>
+__device__ __forceinline__ uint64_t row_vid(const FoldArgs &a, const FoldLds<CAP> &L, int x0, int l)
+{
+    const int row = l >> 3;
+    const int bit = (x0 - a.g.gmin[0] + (l & 7)) & 63;
+    return (uint64_t)L.roff[row] + (uint64_t)__popcll(L.rbits[row] & ((1ull << bit) - 1ull));
 }
 
 __device__ __forceinline__ void brick_origin(const FoldArgs &a, uint32_t brick, int &x0, int &y0, int &z0)
@@ -1598,6 +1943,7 @@ __device__ __forceinline__ void brick_origin(const FoldArgs &a, uint32_t brick, 
 // One workgroup per brick that received records, for bricks of at most
 // FOLD_SMALL records (the others are listed for k_fold_large): the records
 // are read once into LDS, grouped, replayed and solved there.
+template <int P>
 __global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs a)
 {
     __shared__ FoldLds<FOLD_SMALL> L;
@@ -1607,6 +1953,9 @@ __global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs a)
     {
         const uint32_t brick = a.active[bi];
         const uint32_t c0 = a.choff[brick], c1 = a.choff[brick + 1];
+        int x0, y0, z0;
+        brick_origin(a, brick, x0, y0, z0);
+        load_rows(a, L, x0, y0, z0);
         for (int i = tid; i < 512; i += BT)
             L.cur[i] = 0;
         // the brick's chunks, BT at a time: records appended to the LDS tile
@@ -1618,9 +1967,9 @@ __global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs a)
             uint32_t cnt = 0;
             if (tid < nb)
             {
-                const Chunk ch = a.chunks[a.bchunks[cbeg + tid]];
-                L.cb.start[tid] = ch.start;
-                cnt = ch.count;
+                const uint2 ch = a.gchunks[cbeg + tid];
+                L.cb.start[tid] = ch.x;
+                cnt = ch.y;
             }
             uint32_t tot;
             const uint32_t ex = block_excl_scan(cnt, L.cb.sh, tot);
@@ -1651,9 +2000,7 @@ __global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs a)
             __syncthreads();
             continue;
         }
-        int x0, y0, z0;
-        brick_origin(a, brick, x0, y0, z0);
-        fold_loaded<FOLD_SMALL>(a, L, n, x0, y0, z0);
+        fold_loaded<FOLD_SMALL, P>(a, L, n, x0, y0, z0);
     }
 }
 
@@ -1661,6 +2008,7 @@ __global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs a)
 // whose records fit FOLD_CAP is folded like there; otherwise voxel ranges
 // are sized to the LDS tile, and a voxel with more records than the tile is
 // replayed in key-ordered batches selected by a binary search over the key.
+template <int P>
 __global__ __launch_bounds__(BT) void k_fold_large(FoldArgs a)
 {
     __shared__ FoldLds<FOLD_CAP> L;
@@ -1672,6 +2020,7 @@ __global__ __launch_bounds__(BT) void k_fold_large(FoldArgs a)
         const uint32_t brick = a.large[bi];
         int x0, y0, z0;
         brick_origin(a, brick, x0, y0, z0);
+        load_rows(a, L, x0, y0, z0);
 
         // one pass over the brick's records: per-voxel histogram, and the
         // records themselves into LDS while they fit
@@ -1695,7 +2044,7 @@ __global__ __launch_bounds__(BT) void k_fold_large(FoldArgs a)
             atomicAdd(a.ctr + CTR_LARGE_RECS, (unsigned long long)n_all);
         if (n_all <= (uint32_t)FOLD_CAP)
         {
-            fold_loaded<FOLD_CAP>(a, L, n_all, x0, y0, z0);
+            fold_loaded<FOLD_CAP, P>(a, L, n_all, x0, y0, z0);
             continue;
         }
         {
@@ -1723,8 +2072,8 @@ __global__ __launch_bounds__(BT) void k_fold_large(FoldArgs a)
             {
                 // ---- one voxel with more records than the tile: key-ordered batches
                 const uint32_t lv = (uint32_t)lo;
-                FoldState st;
-                st.init();
+                typename FoldPick<P>::type st;
+                st.init(x0 + (int)(lv & 7), y0 + (int)((lv >> 3) & 7), z0 + (int)(lv >> 6), a);
                 uint64_t last = 0;
                 bool have_last = false;
                 uint32_t done = 0;
@@ -1781,7 +2130,7 @@ __global__ __launch_bounds__(BT) void k_fold_large(FoldArgs a)
                 }
                 if (tid == 0)
                 {
-                    st.finish(x0 + (int)(lv & 7), y0 + (int)((lv >> 3) & 7), z0 + (int)(lv >> 6), a);
+                    st.finish_at(row_vid(a, L, x0, (int)lv), a);
                     atomicAdd(a.ctr + CTR_GIANT, 1ull);
                 }
                 __syncthreads();
@@ -1838,14 +2187,14 @@ __global__ __launch_bounds__(BT) void k_fold_large(FoldArgs a)
                 const uint32_t s0 = L.off[lv] - base, s1 = L.off[lv + 1] - base;
                 if (s1 == s0)
                     continue;
-                FoldState st;
-                st.init();
+                typename FoldPick<P>::type st;
+                st.init(x0 + (lv & 7), y0 + ((lv >> 3) & 7), z0 + (lv >> 6), a);
                 for (uint32_t i = s0; i < s1; ++i)
                 {
                     const uint32_t j = L.perm[i];
                     st.add(L.key[j], L.z[j], a);
                 }
-                st.finish(x0 + (lv & 7), y0 + ((lv >> 3) & 7), z0 + (lv >> 6), a);
+                st.finish_at(row_vid(a, L, x0, lv), a);
             }
             __syncthreads();
             lo = hi;
@@ -1924,6 +2273,102 @@ __global__ __launch_bounds__(BT) void k_solve_voxels(const float *qef, const int
     }
 }
 
+// Dual vertex of every voxel from its voxel-frame quadric (precision
+// variant): FP32 column-pivoted QR; when it finds the system rank-deficient
+// or its condition estimate |R00 / R22| exceeds cond_max, the pivoting /
+// rank decision and the back substitution are redone in FP64 on the
+// compensated (h + l) quadric, in world coordinates like the reference.  A
+// minimiser outside [0,1]^3 goes through the box search (FP32, voxel frame).
+// The voxels that took the FP64 path are counted.
+__global__ __launch_bounds__(BT) void k_solve_voxels_c(const float *qef, const int32_t *coords, uint64_t n, GridDev g,
+                                                       float *dual, float cond_max, unsigned long long *n64)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * BT;
+    const int lane = threadIdx.x & 63;
+    unsigned long long *cnt64 = n64 + (uint64_t)((blockIdx.x * NWAVE + (threadIdx.x >> 6)) % NSHARD) * SHARD_STRIDE;
+    const uint64_t n_round = (n + 63ull) & ~63ull;
+    for (uint64_t vid = (uint64_t)blockIdx.x * BT + threadIdx.x; vid < n_round; vid += stride)
+    {
+        const bool live = vid < n;
+        bool used64 = false;
+        if (live)
+        {
+            float qh[10], ql[10];
+#pragma unroll
+            for (int i = 0; i < 10; ++i)
+            {
+                qh[i] = qef[20 * vid + i];
+                ql[i] = qef[20 * vid + 10 + i];
+            }
+            float Q[4][4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    Q[i][j] = qh[qidx(i, j)] + ql[qidx(i, j)];
+            const float A[3][3] = {{Q[0][0], Q[0][1], Q[0][2]}, {Q[1][0], Q[1][1], Q[1][2]}, {Q[2][0], Q[2][1], Q[2][2]}};
+            const float b[3] = {-Q[0][3], -Q[1][3], -Q[2][3]};
+            float u[3];
+            Cpqr<3> f;
+            f.factor(A);
+            const float r0 = fabsf(f.qr[0][0]), rl = fabsf(f.qr[2][2]);
+            used64 = f.rk < 3 || !(r0 <= cond_max * rl);
+            if (used64)
+            {
+                // FP64 solve in world coordinates x = c + vs u, so that a
+                // rank-deficient system gets the reference's basic solution
+                // (free world coordinates zero): A_w = S^-1 A_u S^-1,
+                // b_w_i = -(Q_u[i][3] - sum_j A_u[i][j] c_j / s_j) / s_i
+                double Qd[4][4], cd[3], sd[3], Ad[3][3], bd[3], xd[3];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        Qd[i][j] = (double)qh[qidx(i, j)] + (double)ql[qidx(i, j)];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                {
+                    sd[i] = (double)g.vs[i];
+                    cd[i] = (double)((float)coords[3 * vid + i] * g.vs[i]);
+                }
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                {
+                    double gi = Qd[i][3];
+#pragma unroll
+                    for (int j = 0; j < 3; ++j)
+                    {
+                        Ad[i][j] = Qd[i][j] / (sd[i] * sd[j]);
+                        gi = gi - Qd[i][j] * (cd[j] / sd[j]);
+                    }
+                    bd[i] = -(gi / sd[i]);
+                }
+                Cpqr<3, double> fd;
+                fd.factor(Ad);
+                fd.solve(bd, xd);
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    u[i] = (float)((xd[i] - cd[i]) / sd[i]);
+            }
+            else
+                f.solve(b, u);
+            const float mn[3] = {0.0f, 0.0f, 0.0f}, mx[3] = {1.0f, 1.0f, 1.0f};
+            if (!(u[0] >= 0.0f && u[0] <= 1.0f && u[1] >= 0.0f && u[1] <= 1.0f && u[2] >= 0.0f && u[2] <= 1.0f))
+                box_search(Q, mn, mx, u);
+            float *o = dual + 3 * vid;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+            {
+                const float c = (float)coords[3 * vid + i] * g.vs[i];
+                o[i] = c + g.vs[i] * u[i];
+            }
+        }
+        const uint64_t m = __ballot(used64);
+        if (m && lane == __ffsll((long long)m) - 1)
+            atomicAdd(cnt64, (unsigned long long)__popcll(m));
+    }
+}
+
 // box-constrained search for the queued voxels, one per lane
 __global__ __launch_bounds__(BT) void k_solve_box(ShardBuf<BoxItem> box, GridDev g, float *dual)
 {
@@ -1965,9 +2410,12 @@ hipError_t launch_prep(const MeshDev &m, int64_t V, const GridDev &g, float4 *pl
 }
 
 hipError_t launch_scan_events(const MeshDev &m, const GridDev &g, unsigned long long *ctr, const ShardBuf<Event> &eb,
-                              hipStream_t s)
+                              bool merge, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_scan_events, dim3(grid_for(3 * m.F)), dim3(BT), 0, s, m, g, ctr, eb);
+    if (merge)
+        hipLaunchKernelGGL(k_scan_events<true>, dim3(grid_for(3 * m.F)), dim3(BT), 0, s, m, g, ctr, eb);
+    else
+        hipLaunchKernelGGL(k_scan_events<false>, dim3(grid_for(3 * m.F)), dim3(BT), 0, s, m, g, ctr, eb);
     return hipGetLastError();
 }
 
@@ -2006,11 +2454,11 @@ hipError_t launch_shard_reduce(unsigned long long *shard_ctr, unsigned long long
     return hipGetLastError();
 }
 
-hipError_t launch_place_chunks(const ShardBuf<Chunk> &chunks, const uint32_t *choff, uint32_t *bchunks,
+hipError_t launch_place_chunks(const ShardBuf<Chunk> &chunks, const uint32_t *choff, uint2 *gchunks,
                                hipStream_t s)
 {
     const uint64_t n = (uint64_t)NSHARD * chunks.per;
-    hipLaunchKernelGGL(k_place_chunks, dim3(grid_for((int64_t)n, 16384)), dim3(BT), 0, s, chunks, choff, bchunks);
+    hipLaunchKernelGGL(k_place_chunks, dim3(grid_for((int64_t)n, 16384)), dim3(BT), 0, s, chunks, choff, gchunks);
     return hipGetLastError();
 }
 
@@ -2027,12 +2475,21 @@ hipError_t launch_fold_bricks(const FoldArgs &a, uint64_t max_bricks, hipStream_
     uint64_t nb = max_bricks < 65536ull ? max_bricks : 65536ull;
     if (nb == 0)
         nb = 1;
-    hipLaunchKernelGGL(k_fold_bricks, dim3((unsigned)nb), dim3(BT), 0, s, a);
+    if (a.prec)
+        hipLaunchKernelGGL(k_fold_bricks<1>, dim3((unsigned)nb), dim3(BT), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_fold_bricks<0>, dim3((unsigned)nb), dim3(BT), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_solve_voxels(const FoldArgs &a, const int32_t *coords, uint64_t n, hipStream_t s)
 {
+    if (a.prec)
+    {
+        hipLaunchKernelGGL(k_solve_voxels_c, dim3(grid_for((int64_t)n, 65536)), dim3(BT), 0, s, (const float *)a.qef,
+                           coords, n, a.g, a.dual, a.cond_max, a.n64);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_solve_voxels, dim3(grid_for((int64_t)n, 65536)), dim3(BT), 0, s, (const float *)a.qef,
                        coords, n, a.g, a.dual, a.box);
     return hipGetLastError();
@@ -2050,7 +2507,10 @@ hipError_t launch_fold_large(const FoldArgs &a, uint64_t max_bricks, hipStream_t
     uint64_t nb = max_bricks < 2048ull ? max_bricks : 2048ull;
     if (nb == 0)
         nb = 1;
-    hipLaunchKernelGGL(k_fold_large, dim3((unsigned)nb), dim3(BT), 0, s, a);
+    if (a.prec)
+        hipLaunchKernelGGL(k_fold_large<1>, dim3((unsigned)nb), dim3(BT), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_fold_large<0>, dim3((unsigned)nb), dim3(BT), 0, s, a);
     return hipGetLastError();
 }
 

@@ -11,55 +11,75 @@ namespace ovx
 {
 
 // ---------------------------------------------------------------------------
-template <int n>
+// float / double constants and functions of the factorisation
+template <typename T>
+struct Num;
+template <>
+struct Num<float>
+{
+    __device__ static constexpr float eps() { return FLT_EPSILON; }
+    __device__ static constexpr float tiny() { return FLT_MIN; }
+    __device__ static float sqrt_(float x) { return sqrtf(x); }
+    __device__ static float abs_(float x) { return fabsf(x); }
+};
+template <>
+struct Num<double>
+{
+    __device__ static constexpr double eps() { return DBL_EPSILON; }
+    __device__ static constexpr double tiny() { return DBL_MIN; }
+    __device__ static double sqrt_(double x) { return sqrt(x); }
+    __device__ static double abs_(double x) { return fabs(x); }
+};
+
+template <int n, typename T = float>
 struct Cpqr
 {
-    float qr[n + 1][n + 1]; // one spare row/column keeps the unrolled (dead) tail indices in bounds
-    float hco[n];
+    T qr[n + 1][n + 1]; // one spare row/column keeps the unrolled (dead) tail indices in bounds
+    T hco[n];
     int perm[n];
     int rk;
 
-    __device__ __forceinline__ void factor(const float (&A)[n][n]);
-    __device__ __forceinline__ void solve(const float (&b)[n], float (&x)[n]) const;
+    __device__ __forceinline__ void factor(const T (&A)[n][n]);
+    __device__ __forceinline__ void solve(const T (&b)[n], T (&x)[n]) const;
 };
 
 // Householder QR with column pivoting of A (the factorisation does not depend
 // on the right-hand side, so one factorisation serves several solves)
-template <int n>
-__device__ __forceinline__ void Cpqr<n>::factor(const float (&A)[n][n])
+template <int n, typename T>
+__device__ __forceinline__ void Cpqr<n, T>::factor(const T (&A)[n][n])
 {
 #pragma unroll
     for (int i = 0; i <= n; ++i)
 #pragma unroll
         for (int j = 0; j <= n; ++j)
-            qr[i][j] = (i < n && j < n) ? A[i][j] : 0.0f;
+            qr[i][j] = (i < n && j < n) ? A[i][j] : T(0);
     int transp[n];
-    float nu[n], nd[n];
+    T nu[n], nd[n];
 #pragma unroll
     for (int k = 0; k < n; ++k)
     {
-        float s;
+        T s;
         if constexpr (n == 3)
             s = qr[0][k] * qr[0][k] + (qr[1][k] * qr[1][k] + qr[2][k] * qr[2][k]);
         else
             s = qr[0][k] * qr[0][k] + qr[1][k] * qr[1][k];
-        nd[k] = sqrtf(s);
+        nd[k] = Num<T>::sqrt_(s);
         nu[k] = nd[k];
     }
-    float maxn = nu[0];
+    T maxn = nu[0];
 #pragma unroll
     for (int k = 1; k < n; ++k)
         if (nu[k] > maxn)
             maxn = nu[k];
-    const float me = maxn * FLT_EPSILON;
-    const float th = (me * me) / (float)n;
-    const float ndt = sqrtf(FLT_EPSILON);
+    const T me = maxn * Num<T>::eps();
+    const T th = (me * me) / (T)n;
+    const T ndt = Num<T>::sqrt_(Num<T>::eps());
     int nzp = n;
 #pragma unroll
     for (int k = 0; k < n; ++k)
     {
         int big = k;
-        float bigv = nu[k];
+        T bigv = nu[k];
 #pragma unroll
         for (int j = k + 1; j < n; ++j)
             if (nu[j] > bigv)
@@ -67,7 +87,7 @@ __device__ __forceinline__ void Cpqr<n>::factor(const float (&A)[n][n])
                 bigv = nu[j];
                 big = j;
             }
-        if (nzp == n && bigv * bigv < th * (float)(n - k))
+        if (nzp == n && bigv * bigv < th * (T)(n - k))
             nzp = k;
         transp[k] = big;
 #pragma unroll
@@ -77,11 +97,11 @@ __device__ __forceinline__ void Cpqr<n>::factor(const float (&A)[n][n])
 #pragma unroll
                 for (int rr = 0; rr < n; ++rr)
                 {
-                    const float t = qr[rr][k];
+                    const T t = qr[rr][k];
                     qr[rr][k] = qr[rr][j];
                     qr[rr][j] = t;
                 }
-                float t = nu[k];
+                T t = nu[k];
                 nu[k] = nu[j];
                 nu[j] = t;
                 t = nd[k];
@@ -89,7 +109,7 @@ __device__ __forceinline__ void Cpqr<n>::factor(const float (&A)[n][n])
                 nd[j] = t;
             }
         const int tl = n - k - 1;
-        float tail_sq = 0.0f;
+        T tail_sq = T(0);
         if (tl > 0)
         {
             tail_sq = qr[k + 1][k] * qr[k + 1][k];
@@ -97,22 +117,22 @@ __device__ __forceinline__ void Cpqr<n>::factor(const float (&A)[n][n])
             for (int i = 1; i < n - k - 1; ++i)
                 tail_sq = tail_sq + qr[k + 1 + i][k] * qr[k + 1 + i][k];
         }
-        const float c0 = qr[k][k];
-        float beta, tau;
-        if (tail_sq <= FLT_MIN)
+        const T c0 = qr[k][k];
+        T beta, tau;
+        if (tail_sq <= Num<T>::tiny())
         {
-            tau = 0.0f;
+            tau = T(0);
             beta = c0;
 #pragma unroll
             for (int i = 0; i < n - k - 1; ++i)
-                qr[k + 1 + i][k] = 0.0f;
+                qr[k + 1 + i][k] = T(0);
         }
         else
         {
-            beta = sqrtf(c0 * c0 + tail_sq);
-            if (c0 >= 0.0f)
+            beta = Num<T>::sqrt_(c0 * c0 + tail_sq);
+            if (c0 >= T(0))
                 beta = -beta;
-            const float den = c0 - beta;
+            const T den = c0 - beta;
 #pragma unroll
             for (int i = 0; i < n - k - 1; ++i)
                 qr[k + 1 + i][k] = qr[k + 1 + i][k] / den;
@@ -120,12 +140,12 @@ __device__ __forceinline__ void Cpqr<n>::factor(const float (&A)[n][n])
         }
         hco[k] = tau;
         qr[k][k] = beta;
-        if (n - k > 1 && tau != 0.0f)
+        if (n - k > 1 && tau != T(0))
         {
 #pragma unroll
             for (int j = k + 1; j < n; ++j)
             {
-                float tmp = qr[k + 1][k] * qr[k + 1][j];
+                T tmp = qr[k + 1][k] * qr[k + 1][j];
 #pragma unroll
                 for (int i = 1; i < n - k - 1; ++i)
                     tmp = tmp + qr[k + 1 + i][k] * qr[k + 1 + i][j];
@@ -134,7 +154,7 @@ __device__ __forceinline__ void Cpqr<n>::factor(const float (&A)[n][n])
 #pragma unroll
                 for (int i = 0; i < n - k - 1; ++i)
                 {
-                    const float te = tau * qr[k + 1 + i][k];
+                    const T te = tau * qr[k + 1 + i][k];
                     qr[k + 1 + i][j] = qr[k + 1 + i][j] - tmp * te;
                 }
             }
@@ -142,31 +162,31 @@ __device__ __forceinline__ void Cpqr<n>::factor(const float (&A)[n][n])
 #pragma unroll
         for (int j = k + 1; j < n; ++j)
         {
-            if (nu[j] != 0.0f)
+            if (nu[j] != T(0))
             {
-                const float a = fabsf(qr[k][j]);
-                float temp = a / nu[j];
-                temp = (1.0f + temp) * (1.0f - temp);
-                temp = temp < 0.0f ? 0.0f : temp;
-                const float ratio = nu[j] / nd[j];
-                const float temp2 = temp * (ratio * ratio);
+                const T a = Num<T>::abs_(qr[k][j]);
+                T temp = a / nu[j];
+                temp = (T(1) + temp) * (T(1) - temp);
+                temp = temp < T(0) ? T(0) : temp;
+                const T ratio = nu[j] / nd[j];
+                const T temp2 = temp * (ratio * ratio);
                 if (temp2 <= ndt)
                 {
-                    float s = 0.0f;
+                    T s = T(0);
                     if (k + 1 < n)
                     {
                         s = qr[k + 1][j] * qr[k + 1][j];
 #pragma unroll
                         for (int i = k + 2; i < n; ++i)
                             s = s + qr[i][j] * qr[i][j];
-                        s = sqrtf(s);
+                        s = Num<T>::sqrt_(s);
                     }
                     nd[j] = s;
                     nu[j] = s;
                 }
                 else
                 {
-                    nu[j] = nu[j] * sqrtf(temp);
+                    nu[j] = nu[j] * Num<T>::sqrt_(temp);
                 }
             }
         }
@@ -189,19 +209,19 @@ __device__ __forceinline__ void Cpqr<n>::factor(const float (&A)[n][n])
     rk = nzp;
 }
 
-template <int n>
-__device__ __forceinline__ void Cpqr<n>::solve(const float (&b)[n], float (&x)[n]) const
+template <int n, typename T>
+__device__ __forceinline__ void Cpqr<n, T>::solve(const T (&b)[n], T (&x)[n]) const
 {
-    float c[n + 1];
+    T c[n + 1];
 #pragma unroll
     for (int i = 0; i < n; ++i)
         c[i] = b[i];
-    c[n] = 0.0f;
+    c[n] = T(0);
     if (rk == 0)
     {
 #pragma unroll
         for (int i = 0; i < n; ++i)
-            x[i] = 0.0f;
+            x[i] = T(0);
         return;
     }
 #pragma unroll
@@ -209,14 +229,14 @@ __device__ __forceinline__ void Cpqr<n>::solve(const float (&b)[n], float (&x)[n
     {
         if (k < rk)
         {
-            const float tau = hco[k];
+            const T tau = hco[k];
             if (n - k == 1)
             {
-                c[k] = c[k] * (1.0f - tau);
+                c[k] = c[k] * (T(1) - tau);
             }
-            else if (tau != 0.0f)
+            else if (tau != T(0))
             {
-                float tmp = qr[k + 1][k] * c[k + 1];
+                T tmp = qr[k + 1][k] * c[k + 1];
 #pragma unroll
                 for (int i = 1; i < n - k - 1; ++i)
                     tmp = tmp + qr[k + 1 + i][k] * c[k + 1 + i];
@@ -225,7 +245,7 @@ __device__ __forceinline__ void Cpqr<n>::solve(const float (&b)[n], float (&x)[n
 #pragma unroll
                 for (int i = 0; i < n - k - 1; ++i)
                 {
-                    const float te = tau * qr[k + 1 + i][k];
+                    const T te = tau * qr[k + 1 + i][k];
                     c[k + 1 + i] = c[k + 1 + i] - tmp * te;
                 }
             }
@@ -234,7 +254,7 @@ __device__ __forceinline__ void Cpqr<n>::solve(const float (&b)[n], float (&x)[n
 #pragma unroll
     for (int i = n - 1; i >= 0; --i)
     {
-        if (i < rk && c[i] != 0.0f)
+        if (i < rk && c[i] != T(0))
         {
             c[i] = c[i] / qr[i][i];
 #pragma unroll
@@ -245,7 +265,7 @@ __device__ __forceinline__ void Cpqr<n>::solve(const float (&b)[n], float (&x)[n
 #pragma unroll
     for (int j = 0; j < n; ++j)
     {
-        float v = 0.0f;
+        T v = T(0);
 #pragma unroll
         for (int i = 0; i < n; ++i)
             if (perm[i] == j && i < rk)
@@ -313,12 +333,8 @@ __device__ __forceinline__ bool solve_free(const float q[10], const int coord[3]
 // the 6 faces (2x2 solves), 12 edges (1x1) and 8 corners, keeping the
 // smallest p^T Q p with a strict <.  v enters as the unconstrained minimiser
 // (kept if no candidate compares smaller, e.g. non-finite quadrics).
-__device__ __forceinline__ void solve_box(const float q[10], const int coord[3], const float vs[3], float v[3])
+__device__ __forceinline__ void box_search(const float (&Q)[4][4], const float (&mn)[3], const float (&mx)[3], float v[3])
 {
-    float Q[4][4];
-    load_q44(q, Q);
-    float mn[3], mx[3];
-    voxel_box(coord, vs, mn, mx);
     float best = __builtin_huge_valf();
     auto consider = [&](const float p[4]) {
         const float err = quad_form(p, Q);
@@ -391,6 +407,15 @@ __device__ __forceinline__ void solve_box(const float q[10], const int coord[3],
                 const float p[4] = {cx ? mn[0] : mx[0], cy ? mn[1] : mx[1], cz ? mn[2] : mx[2], 1.0f};
                 consider(p);
             }
+}
+
+__device__ __forceinline__ void solve_box(const float q[10], const int coord[3], const float vs[3], float v[3])
+{
+    float Q[4][4];
+    load_q44(q, Q);
+    float mn[3], mx[3];
+    voxel_box(coord, vs, mn, mx);
+    box_search(Q, mn, mx, v);
 }
 
 // dual vertex of one voxel: the unconstrained minimiser, else the box search

@@ -83,6 +83,8 @@ def _load():
         L.ovx_fdg_set_timing.argtypes = [vp, ctypes.c_int]
         L.ovx_fdg_stage_times.argtypes = [vp, ctypes.POINTER(f32)]
         L.ovx_fdg_set_capacity.argtypes = [vp, i64, i64]
+        L.ovx_fdg_set_precision.argtypes = [vp, ctypes.c_int, f32]
+        L.ovx_fdg_set_precision.restype = ctypes.c_int
         L.ovx_fdg_set_capacity.restype = ctypes.c_int
         L.ovx_fdg_kernel_count.restype = ctypes.c_int
         L.ovx_fdg_kernel_name.argtypes = [ctypes.c_int]
@@ -159,6 +161,9 @@ def _stream_handle(device: torch.device) -> int:
     return torch.cuda.current_stream(device).cuda_stream
 
 
+_PRECISIONS = {"reference": 0, "compensated": 1}
+
+
 def mesh_to_flexible_dual_grid_cuda(
     vertices: torch.Tensor,
     faces: torch.Tensor,
@@ -167,6 +172,8 @@ def mesh_to_flexible_dual_grid_cuda(
     face_weight: float,
     boundary_weight: float,
     regularization_weight: float,
+    precision: str = "reference",
+    cond_max: float = 1.0e6,
 ) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
     """Device flexible dual grid (reference: src/ext.cpp:19 ->
     src/convert/mesh_to_flexible_dual_grid_gpu/mesh_to_flexible_dual_grid.cu:797-895).
@@ -174,7 +181,15 @@ def mesh_to_flexible_dual_grid_cuda(
     vertices f32 [V,3] grid-local, faces i32 [F,3], both contiguous on one
     device.  Returns (coords i32 [N,3], dual f32 [N,3], intersected bool [N,3])
     with voxels in ascending x + Gx*(y + Gy*z) order.
+
+    ``precision`` (not in the reference): "reference" reproduces the CPU float
+    path bit for bit; "compensated" accumulates each voxel's quadric in the
+    voxel's frame with compensated FP32 sums and solves in FP32 with an FP64
+    fallback above the condition estimate ``cond_max``.
     """
+    mode = _PRECISIONS.get(precision)
+    if mode is None:
+        raise RuntimeError(f"precision must be one of {sorted(_PRECISIONS)}, got {precision!r}")
     _require_device_tensor(vertices, "vertices", torch.float32, 2)
     _require_device_tensor(faces, "faces", torch.int32, 2)
     if vertices.size(1) != 3 or faces.size(1) != 3:
@@ -190,6 +205,7 @@ def mesh_to_flexible_dual_grid_cuda(
     with torch.cuda.device(dev), ctx.lock:
         stream = _stream_handle(dev)
         n = ctypes.c_int64(0)
+        _check(L.ovx_fdg_set_precision(ctx.h, mode, float(cond_max)), "mesh_to_flexible_dual_grid")
         _check(L.ovx_fdg_begin(ctx.h, vertices.data_ptr(), vertices.size(0), faces.data_ptr(), faces.size(0),
                                ctypes.byref(g), float(face_weight), float(boundary_weight),
                                float(regularization_weight), stream, ctypes.byref(n)), "mesh_to_flexible_dual_grid")
@@ -245,6 +261,7 @@ def mesh_to_flexible_dual_grid_cpu(
     boundary_weight: float,
     regularization_weight: float,
     timing: bool,
+    precision: str = "reference",
 ) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
     """Host-tensor entry point (reference: src/ext.cpp:18 ->
     src/convert/flexible_dual_grid.cpp:660-992), run on the current GPU.
@@ -258,11 +275,12 @@ def mesh_to_flexible_dual_grid_cpu(
     if timing:
         set_timing(dev, True)
     out = mesh_to_flexible_dual_grid_cuda(v, f, voxel_size, grid_range, face_weight, boundary_weight,
-                                          regularization_weight)
+                                          regularization_weight, precision=precision)
     if timing:
-        names = ("mark", "compact", "count", "emit", "fold+solve")
+        names = ("Intersect QEF computation", "Face QEF computation", "Boundary QEF computation",
+                 "Dual vertices computation")
         for k, ms in zip(names, stage_times(dev)):
-            print(f"{k} took {ms / 1e3:.6f} seconds.")
+            print(f"{k} took {ms / 1e3} seconds.")
     return tuple(t.cpu() for t in out)
 
 
@@ -282,7 +300,7 @@ def last_stats(device=None) -> dict:
     ctx = _ctx(dev)
     keys = ("voxels", "events", "records", "face_records", "boundary_edges", "boundary_records",
             "workspace_bytes", "giant_voxels", "active_bricks", "chunks", "large_bricks", "large_brick_records",
-            "box_voxels")
+            "box_voxels", "fp64_voxels")
     out = (ctypes.c_int64 * len(keys))()
     with torch.cuda.device(dev), ctx.lock:
         _check(_load().ovx_fdg_stats(ctx.h, out, len(keys)), "ovx_fdg_stats")
@@ -297,10 +315,11 @@ def set_timing(device=None, enable: bool = True):
 
 
 def stage_times(device=None) -> List[float]:
-    """Device ms of [mark, compact, count, emit, fold+solve] of the last conversion."""
+    """Device ms of the reference's timed stages [intersection QEF, face QEF,
+    boundary QEF, dual vertices] (FDG.cpp:711-986) of the last conversion."""
     dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
     ctx = _ctx(dev)
-    out = (ctypes.c_float * 5)()
+    out = (ctypes.c_float * 4)()
     with torch.cuda.device(dev), ctx.lock:
         _check(_load().ovx_fdg_stage_times(ctx.h, out), "ovx_fdg_stage_times")
     return [float(x) for x in out]

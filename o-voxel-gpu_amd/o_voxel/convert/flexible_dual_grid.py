@@ -125,6 +125,7 @@ def mesh_to_flexible_dual_grid(
     boundary_weight: float = 1.0,
     regularization_weight: float = 0.1,
     timing: bool = False,
+    precision: str = "reference",
 ) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
     """Convert a triangle mesh into a sparse flexible dual grid.
 
@@ -137,6 +138,13 @@ def mesh_to_flexible_dual_grid(
     x + Gx*(y + Gy*z) order (the reference CPU path returns first-insertion
     order and its CUDA path an unspecified order; the sets and the per-voxel
     values are the same).
+
+    ``precision`` is an addition to the reference signature: "reference"
+    (default) reproduces the reference CPU float arithmetic bit for bit;
+    "compensated" accumulates each voxel's quadric in the voxel's own frame
+    with compensated FP32 sums and solves in FP32 with an FP64 fallback for
+    ill-conditioned voxels, which brings the dual vertices close to the exact
+    (FP64) solution.  Occupancy and flags do not depend on it.
     """
     _check_mesh(vertices, faces)
     device = vertices.device
@@ -146,9 +154,9 @@ def mesh_to_flexible_dual_grid(
     vertices = vertices - aabb_dev[0].reshape(1, 3)
     if vertices.is_cuda:
         return _C.mesh_to_flexible_dual_grid_cuda(vertices, faces, voxel_size_arg, grid_range_arg, face_weight,
-                                                  boundary_weight, regularization_weight)
+                                                  boundary_weight, regularization_weight, precision=precision)
     return _C.mesh_to_flexible_dual_grid_cpu(vertices, faces, voxel_size_arg, grid_range_arg, face_weight,
-                                             boundary_weight, regularization_weight, timing)
+                                             boundary_weight, regularization_weight, timing, precision=precision)
 
 
 @torch.no_grad()

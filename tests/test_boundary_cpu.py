@@ -56,11 +56,14 @@ def test_public_signatures_match_reference():
     import o_voxel
 
     sig = inspect.signature(o_voxel.convert.mesh_to_flexible_dual_grid)
+    # the reference's parameters in the reference's order, then the one
+    # addition (the precision variant) with a default that keeps the
+    # reference behaviour
     assert list(sig.parameters) == ["vertices", "faces", "voxel_size", "grid_size", "aabb", "face_weight",
-                                    "boundary_weight", "regularization_weight", "timing"]
+                                    "boundary_weight", "regularization_weight", "timing", "precision"]
     d = {k: v.default for k, v in sig.parameters.items()}
     assert d["face_weight"] == 1.0 and d["boundary_weight"] == 1.0 and d["regularization_weight"] == 0.1
-    assert d["timing"] is False
+    assert d["timing"] is False and d["precision"] == "reference"
     sig = inspect.signature(o_voxel.convert.intersect_occ)
     assert list(sig.parameters) == ["vertices", "faces", "voxel_size", "grid_size", "aabb"]
     from o_voxel import _C
@@ -68,7 +71,7 @@ def test_public_signatures_match_reference():
     for name in ("mesh_to_flexible_dual_grid_cuda", "mesh_to_flexible_dual_grid_cpu", "intersect_occ_cuda",
                  "intersect_occ_cpu"):
         assert callable(getattr(_C, name))
-    assert list(inspect.signature(_C.mesh_to_flexible_dual_grid_cpu).parameters)[-1] == "timing"
+    assert list(inspect.signature(_C.mesh_to_flexible_dual_grid_cpu).parameters)[-2:] == ["timing", "precision"]
 
 
 def test_assertions_like_reference():
