@@ -109,6 +109,12 @@ def algorithmic_bytes(slot, st, F, V, n_words):
     return table.get(slot)
 
 
+def meshgen_grid(grid):
+    import meshgen
+
+    return meshgen.grid_args(grid)
+
+
 def cpu_baseline(v, f, grid, workload, max_faces=None):
     """Oracle float path (single thread) on the workload or a bounded face sample."""
     import meshgen
@@ -147,6 +153,9 @@ def main():
     ap.add_argument("--workload", default="scene")
     ap.add_argument("--grid", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision", default="reference", choices=["reference", "compensated"],
+                    help="arithmetic of the QEF accumulation and solve (compensated: voxel-frame compensated FP32, "
+                         "FP64 fallback; its dual error is reported against the FP64 oracle)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, copy a 256 MiB buffer 5 times (a kernel of known traffic "
                          "for scaling hardware byte counters)")
@@ -177,7 +186,7 @@ def main():
     def step():
         return o_voxel.convert.mesh_to_flexible_dual_grid(vt, ft, grid_size=grid, aabb=aabb, face_weight=WEIGHTS[0],
                                                           boundary_weight=WEIGHTS[1],
-                                                          regularization_weight=WEIGHTS[2])
+                                                          regularization_weight=WEIGHTS[2], precision=args.precision)
 
     # cold call (first in the process)
     torch.cuda.synchronize()
@@ -266,6 +275,32 @@ def main():
                     "vs": "oracle float path",
                 })
 
+    precision = None
+    if args.precision == "compensated":
+        precision = {"mode": "compensated", "fp64_fraction": st["fp64_voxels"] / max(1, st["voxels"])}
+        if not args.no_cpu_baseline and F <= 1_000_000:
+            from oracle import oracle as O
+
+            vs, gr, a0 = meshgen_grid(grid)
+            r64 = O.flexible_dual_grid((v - a0).astype(np.float32), f.astype(np.int32), vs, gr, *WEIGHTS, fp64=True)
+            perm = O.canonical_order(r64.coords, gr)
+            ref_plain = o_voxel.convert.mesh_to_flexible_dual_grid(vt, ft, grid_size=grid, aabb=aabb,
+                                                                   face_weight=WEIGHTS[0], boundary_weight=WEIGHTS[1],
+                                                                   regularization_weight=WEIGHTS[2])[1].cpu().numpy()
+
+            def err(d):
+                e = (d.astype(np.float64) - r64.dual64[perm]) / np.asarray(vs, np.float64)[None, :]
+                n = np.sqrt((e ** 2).sum(1))
+                return float(np.sqrt((n ** 2).mean())), float(n.max())
+
+            occ_ok = coords.shape == r64.coords.shape and bool(np.array_equal(coords, r64.coords[perm]))
+            rms, mx = err(dual)
+            rms_p, mx_p = err(ref_plain)
+            precision.update({"vs": "FP64 oracle", "occupancy_match": occ_ok,
+                              "intersected_mismatch": int((inter != r64.flags[perm]).sum()) if occ_ok else None,
+                              "dual_rms_voxel": rms, "dual_max_voxel": mx,
+                              "reference_path_dual_rms_voxel": rms_p, "reference_path_dual_max_voxel": mx_p})
+
     line = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -293,6 +328,9 @@ def main():
         "peak_mem_mb": round(peak_mb, 1),
         "counts": {k: v for k, v in st.items() if k != "workspace_bytes"},
     }
+    if precision is not None:
+        line["precision"] = precision
+        line["config"]["precision"] = "compensated"
     print(json.dumps(line), flush=True)
 
 

@@ -73,7 +73,7 @@ struct ovx_fdg_ctx
     // workspace
     DevBuf planes, bits, woff, tiles, counters;
     DevBuf vcnt, voff, nbr, erank, bcnt, boff, bedge_v, bedges;
-    DevBuf events, recs, chunks, heads, choff, bchunks, active, large, box, qef, bhits, fhits, shard_ctr;
+    DevBuf events, recs, chunks, heads, choff, bchunks, active, large, box, qef, bhits, fhits, shard_ctr, rowq, faceq;
     uint64_t per[SB_N] = {};       // per-shard capacities of the sharded buffers (grow-only)
     uint64_t force_per[SB_N] = {}; // one-shot overrides (tests, tuning)
     double face_per_face = 8.0;           // face records per face seen last time (capacity estimate)
@@ -105,7 +105,7 @@ struct ovx_fdg_ctx
     {
         DevBuf *all[] = {&planes, &bits,   &woff,  &tiles,   &vcnt,   &voff,  &nbr, &erank, &bcnt, &boff,
                          &bedge_v, &bedges, &events, &recs,   &chunks, &heads, &choff, &bchunks, &active, &large,
-                         &box,     &qef,    &bhits,  &fhits,  &shard_ctr};
+                         &box,     &qef,    &bhits,  &fhits,  &shard_ctr, &rowq, &faceq};
         for (DevBuf *b : all)
             fn(*b);
     }
@@ -335,8 +335,13 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int64_t V, const ovx_g
     // plane the crossings are sparse and the table only costs occupancy
     const double rows_per_plane = std::max({(double)g.G[0] * g.G[1], (double)g.G[1] * g.G[2], (double)g.G[0] * g.G[2]});
     const bool merge = (double)m.F <= 2.0 * rows_per_plane;
+    // row chunks of large triangles, shared out over the grid by a second launch
+    if ((st = ensure_shard(c, c->rowq, SB_ROWQ, per_for(std::max<uint64_t>(1ull << 20, (uint64_t)m.F / 4)),
+                           sizeof(RowChunk))))
+        return st;
+    const ShardBuf<RowChunk> rq = shard_view<RowChunk>(c, c->rowq, SB_ROWQ);
     tic(c, KS_MARK, s);
-    HIP_TRY(launch_scan_events(m, g, ctr, eb, merge, s));
+    HIP_TRY(launch_scan_events(m, g, ctr, eb, rq, merge, s));
     toc(c, KS_MARK, s);
     tic(c, KS_WORDSCAN, s);
     HIP_TRY(launch_scan_popc(g.bits, n_words, (uint32_t *)c->woff.p, (uint64_t *)c->tiles.p, ctr + CTR_WORDS_TOTAL,
@@ -390,7 +395,7 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int64_t V, const ovx_g
                 return st;
             ShardBuf<Event> eb2 = shard_view<Event>(c, c->events, SB_EVENTS);
             tic(c, KS_MARK, s);
-            HIP_TRY(launch_scan_events(m, g, nullptr, eb2, merge, s));
+            HIP_TRY(launch_scan_events(m, g, nullptr, eb2, rq, merge, s));
             toc(c, KS_MARK, s);
         }
         c->E = (int64_t)c->h_ctr[CTR_BEDGE_TOTAL];
@@ -448,6 +453,8 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
             (st = ensure_shard(c, c->chunks, SB_CHUNKS, std::min(c->per[SB_RECS] / 2 + 256, lim), sizeof(Chunk))) ||
             (st = ensure_shard(c, c->fhits, SB_FHITS, c->fw > 0.0f ? want_fhit : 1, sizeof(FHit))) ||
             (st = ensure_shard(c, c->bhits, SB_BHITS, c->E > 0 ? want_bhit : 1, sizeof(BHit))) ||
+            (st = ensure_shard(c, c->faceq, SB_FACEQ, per_for(std::max<uint64_t>(1ull << 20, (uint64_t)m.F / 4)),
+                               sizeof(FaceChunk))) ||
             (st = ensure(c->heads, (n_bricks + 1) * 4)))
             return st;
         if ((st = clear_shards(c, SB_FHITS, SB_N - SB_FHITS, s)))
@@ -468,7 +475,7 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
         {
             const ShardBuf<FHit> fb = shard_view<FHit>(c, c->fhits, SB_FHITS);
             tic(c, KS_FACE, s);
-            HIP_TRY(launch_face(m, g, fb, ctr, s));
+            HIP_TRY(launch_face(m, g, fb, ctr, shard_view<FaceChunk>(c, c->faceq, SB_FACEQ), s));
             toc(c, KS_FACE, s);
             tic(c, KS_GEN_FHITS, s);
             HIP_TRY(launch_gen_fhits(fb, g, ba, s));

@@ -23,8 +23,8 @@ enum
     CTR_LARGE_RECS = 11, // their records
     // sum and largest shard of each sharded buffer (filled by k_shard_reduce)
     CTR_SUM = 16,        // CTR_SUM + SB_*  : total items requested
-    CTR_MAX = 24,        // CTR_MAX + SB_*  : largest per-shard request
-    CTR_COUNT = 32
+    CTR_MAX = 32,        // CTR_MAX + SB_*  : largest per-shard request
+    CTR_COUNT = 48
 };
 
 // sharded buffers
@@ -37,7 +37,9 @@ enum
     SB_CHUNKS = 4,
     SB_BOX = 5, // voxels queued for the box-constrained search
     SB_FP64 = 6, // counters only: voxels solved on the FP64 path (precision variant)
-    SB_N = 7
+    SB_ROWQ = 7, // row chunks of large triangles (scanline)
+    SB_FACEQ = 8, // row chunks of large triangles (face test)
+    SB_N = 9
 };
 
 // Append / allocation counters are split over NSHARD counters 128 bytes
@@ -71,6 +73,24 @@ struct Event
     uint64_t key;
     float z;
     int zi;
+};
+
+static_assert(SB_N <= 16, "counter layout: CTR_SUM and CTR_MAX hold 16 entries each");
+
+// Rows [r0, r1) of one (face, axis) scanline item, queued when a triangle has
+// more rows than one block iteration should take (balances large triangles
+// over the grid)
+struct RowChunk
+{
+    uint32_t face;
+    int ax2, r0, r1;
+};
+
+// Bounding-box rows [r0, r1) (linear (y, z) index) of one face for the face test
+struct FaceChunk
+{
+    uint32_t face;
+    uint32_t r0, r1;
 };
 
 // one QEF contribution: order key, crossing z (phase 0), voxel index in its brick
@@ -161,9 +181,9 @@ struct FoldArgs
 hipError_t launch_prep(const MeshDev &m, int64_t V, const GridDev &g, float4 *planes, unsigned long long *ctr,
                        hipStream_t s);
 hipError_t launch_scan_events(const MeshDev &m, const GridDev &g, unsigned long long *ctr, const ShardBuf<Event> &eb,
-                              bool merge, hipStream_t s);
+                              const ShardBuf<RowChunk> &rq, bool merge, hipStream_t s);
 hipError_t launch_face(const MeshDev &m, const GridDev &g, const ShardBuf<FHit> &hb, unsigned long long *ctr,
-                       hipStream_t s);
+                       const ShardBuf<FaceChunk> &fq, hipStream_t s);
 hipError_t launch_gen_events(const ShardBuf<Event> &ev, const GridDev &g, const BucketArgs &ba, hipStream_t s);
 hipError_t launch_gen_fhits(const ShardBuf<FHit> &h, const GridDev &g, const BucketArgs &ba, hipStream_t s);
 hipError_t launch_gen_bhits(const ShardBuf<BHit> &h, const GridDev &g, const BucketArgs &ba, hipStream_t s);

@@ -182,6 +182,31 @@ __device__ __forceinline__ bool shard_valid(const ShardBuf<T> &b, uint64_t i)
     return n <= b.per && local < n;
 }
 
+// Dense numbering of the filled slots of a queue whose claims are
+// all-or-nothing (every slot below a region's counter holds an item): the
+// exclusive prefix of the regions' fill counts, one region per thread.
+// Contains barriers.
+template <typename T>
+__device__ __forceinline__ uint32_t queue_prefix(const ShardBuf<T> &b, uint32_t *pre, uint32_t *sh)
+{
+    static_assert(NSHARD == BT, "one region per thread");
+    const unsigned long long c = b.cnt[(uint64_t)threadIdx.x * SHARD_STRIDE];
+    const uint32_t n = (uint32_t)(c < b.per ? c : b.per);
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan(n, sh, tot);
+    pre[threadIdx.x] = ex + n; // inclusive
+    __syncthreads();
+    return tot;
+}
+
+// slot of the v-th filled item (v < total)
+template <typename T>
+__device__ __forceinline__ uint64_t queue_slot(const ShardBuf<T> &b, const uint32_t *pre, uint32_t v)
+{
+    const int s = upper_find(pre, NSHARD, v);
+    return (uint64_t)s * b.per + (v - (s ? pre[s - 1] : 0u));
+}
+
 // LDS table of one block's records grouped by brick
 template <int LOG2>
 struct BucketLds
@@ -340,10 +365,64 @@ __global__ __launch_bounds__(BT) void k_prep(MeshDev m, int64_t V, GridDev g, fl
 struct ScanItem
 {
     float t[3][3]; // y-sorted vertices in (ax0, ax1, ax2) coordinates
-    int start, mid, end;
+    int start, mid, end; // rows [start, end) handled here (a part of a large triangle's rows); mid splits the halves
     int ax2;
     uint32_t face;
 };
+
+// A scanline item whose rows x columns exceed MARK_VOX_CAP is split into
+// row chunks of about that many cells; all but the last go to the row queue
+constexpr int MARK_VOX_CAP = 512;
+constexpr uint32_t QUEUE_GROUP = 32; // row chunks claimed together (from one shard)
+
+// Sorts the triangle's vertices by the scan axis and returns its row range
+// [s, e) and middle row (FDG.cpp:195-230).
+__device__ __forceinline__ void scan_item_setup(const MeshDev &m, const GridDev &g, int64_t face, int ax2,
+                                                ScanItem &it)
+{
+    const int ax0 = (ax2 + 1) % 3;
+    const int ax1 = (ax2 + 2) % 3;
+    float v[3][3];
+    load_tri(m, face, v[0], v[1], v[2]);
+    float t[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        t[k][0] = sel3f(ax0, v[k][0], v[k][1], v[k][2]);
+        t[k][1] = sel3f(ax1, v[k][0], v[k][1], v[k][2]);
+        t[k][2] = sel3f(ax2, v[k][0], v[k][1], v[k][2]);
+    }
+    // stable sort of 3 by y (std::sort on 3 elements is insertion sort)
+    auto swp = [&](int a, int b) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+        {
+            const float tmp = t[a][i];
+            t[a][i] = t[b][i];
+            t[b][i] = tmp;
+        }
+    };
+    if (t[1][1] < t[0][1])
+        swp(0, 1);
+    if (t[2][1] < t[1][1])
+    {
+        swp(1, 2);
+        if (t[1][1] < t[0][1])
+            swp(0, 1);
+    }
+    const double vs1 = (double)g.vs[ax1];
+    const int lo = g.gmin[ax1], hi = g.gmax[ax1] - 1;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            it.t[k][i] = t[k][i];
+    it.start = iclamp((int)((double)t[0][1] / vs1), lo, hi);
+    it.mid = iclamp((int)((double)t[1][1] / vs1), lo, hi);
+    it.end = iclamp((int)((double)t[2][1] / vs1), lo, hi);
+    it.ax2 = ax2;
+    it.face = (uint32_t)face;
+}
 
 struct ScanRow
 {
@@ -394,7 +473,7 @@ __device__ __forceinline__ void mark_word(const GridDev &g, Tab &T, uint64_t w, 
     const uint32_t key = (uint32_t)w;
     uint32_t h = (key * 2654435761u) >> (32 - MARK_LOG2);
 #pragma unroll 1
-    for (int probe = 0; probe < 16; ++probe)
+    for (int probe = 0; probe < 4; ++probe)
     {
         const uint32_t prev = atomicCAS(&T.key[h], MARK_EMPTY, key);
         if (prev == MARK_EMPTY || prev == key)
@@ -467,9 +546,11 @@ __device__ __forceinline__ void mark_event(const GridDev &g, Tab &T, int ax0, in
 
 // MERGE: the occupancy bits of a block iteration's crossings are merged in an
 // LDS table first (pays when crossings are dense; the table costs occupancy)
-template <bool MERGE>
+// QUEUED = false: one item per (face, axis); QUEUED = true: the queued row
+// chunks of large triangles (items index the queue's slots)
+template <bool MERGE, bool QUEUED>
 __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, GridDev g, unsigned long long *counters,
-                                                    ShardBuf<Event> eb)
+                                                    ShardBuf<Event> eb, ShardBuf<RowChunk> rq)
 {
     __shared__ unsigned long long sbase;
     __shared__ bool sfit;
@@ -479,8 +560,11 @@ __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, GridDev g, unsign
     __shared__ uint32_t incl_rows[BT];
     __shared__ uint32_t sh[NWAVE];
     __shared__ typename std::conditional<MERGE, MarkTable, NoMarkTable>::type T;
+    __shared__ uint32_t qpre[QUEUED ? NSHARD : 1];
     const int tid = threadIdx.x;
-    const int64_t n_items = 3 * m.F;
+    int64_t n_items = 3 * m.F;
+    if constexpr (QUEUED)
+        n_items = (int64_t)queue_prefix(rq, qpre, sh);
     uint32_t n_ev = 0;
     if constexpr (MERGE)
     {
@@ -496,57 +580,61 @@ __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, GridDev g, unsign
     {
         const int64_t item = base + tid;
         uint32_t R = 0;
-        if (item < n_items)
+        if constexpr (!QUEUED)
         {
-            const int64_t face = item / 3;
-            const int ax2 = (int)(item - face * 3);
-            const int ax0 = (ax2 + 1) % 3;
-            const int ax1 = (ax2 + 2) % 3;
-            float v[3][3];
-            load_tri(m, face, v[0], v[1], v[2]);
-            float t[3][3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
+            if (item < n_items)
             {
-                t[k][0] = sel3f(ax0, v[k][0], v[k][1], v[k][2]);
-                t[k][1] = sel3f(ax1, v[k][0], v[k][1], v[k][2]);
-                t[k][2] = sel3f(ax2, v[k][0], v[k][1], v[k][2]);
-            }
-            // stable sort of 3 by y (std::sort on 3 elements is insertion sort)
-            auto swp = [&](int a, int b) {
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
+                const int64_t face = item / 3;
+                ScanItem &it = items[tid];
+                scan_item_setup(m, g, face, (int)(item - face * 3), it);
+                // a non-finite vertex (reported as an input error after this
+                // kernel) can break the sort: an empty range then, never a
+                // negative one
+                R = it.end > it.start ? (uint32_t)(it.end - it.start) : 0u;
+                // columns of a row are bounded by the x extent of the triangle
+                const float vs0 = sel3f((it.ax2 + 1) % 3, g.vs[0], g.vs[1], g.vs[2]);
+                const float xlo = sminf(sminf(it.t[0][0], it.t[1][0]), it.t[2][0]) / vs0;
+                const float xhi = smaxf(smaxf(it.t[0][0], it.t[1][0]), it.t[2][0]) / vs0;
+                const float ext = xhi - xlo + 2.0f;
+                const uint32_t rpc =
+                    ext < (float)MARK_VOX_CAP ? (uint32_t)((float)MARK_VOX_CAP / (ext > 1.0f ? ext : 1.0f)) : 1u;
+                if (R > rpc && rq.items)
                 {
-                    const float tmp = t[a][i];
-                    t[a][i] = t[b][i];
-                    t[b][i] = tmp;
+                    // all but the last rpc rows go to the row queue, in chunks
+                    // of rpc rows, claimed QUEUE_GROUP chunks at a time from
+                    // successive shards; the rows of the first group that does
+                    // not fit stay with this block
+                    const uint32_t nq = (R - 1) / rpc;
+                    uint32_t done = 0;
+                    for (uint32_t c0 = 0; c0 < nq; c0 += QUEUE_GROUP)
+                    {
+                        const uint32_t n = nq - c0 < QUEUE_GROUP ? nq - c0 : QUEUE_GROUP;
+                        uint64_t q0;
+                        if (!shard_claim_fit(rq, (uint32_t)((item + c0 / QUEUE_GROUP) % NSHARD), n, q0))
+                            break;
+                        for (uint32_t c = 0; c < n; ++c)
+                        {
+                            const int r0 = it.start + (int)(rpc * (c0 + c));
+                            rq.items[q0 + c] = RowChunk{it.face, it.ax2, r0, r0 + (int)rpc};
+                        }
+                        done = c0 + n;
+                    }
+                    it.start += (int)(rpc * done);
+                    R = (uint32_t)(it.end - it.start);
                 }
-            };
-            if (t[1][1] < t[0][1])
-                swp(0, 1);
-            if (t[2][1] < t[1][1])
-            {
-                swp(1, 2);
-                if (t[1][1] < t[0][1])
-                    swp(0, 1);
             }
-            const double vs1 = (double)g.vs[ax1];
-            const int lo = g.gmin[ax1], hi = g.gmax[ax1] - 1;
-            const int s = iclamp((int)((double)t[0][1] / vs1), lo, hi);
-            const int mi = iclamp((int)((double)t[1][1] / vs1), lo, hi);
-            const int e = iclamp((int)((double)t[2][1] / vs1), lo, hi);
-            ScanItem &it = items[tid];
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-                    it.t[k][i] = t[k][i];
-            it.start = s;
-            it.mid = mi;
-            it.end = e;
-            it.ax2 = ax2;
-            it.face = (uint32_t)face;
-            R = (uint32_t)(e - s);
+        }
+        else
+        {
+            if (item < n_items)
+            {
+                const RowChunk ch = rq.items[queue_slot(rq, qpre, (uint32_t)item)];
+                ScanItem &it = items[tid];
+                scan_item_setup(m, g, (int64_t)ch.face, ch.ax2, it);
+                it.start = ch.r0;
+                it.end = ch.r1;
+                R = (uint32_t)(ch.r1 - ch.r0);
+            }
         }
         uint32_t tot_r;
         const uint32_t ex_r = block_excl_scan(R, sh, tot_r);
@@ -599,7 +687,7 @@ __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, GridDev g, unsign
                 rw.y_idx = y_idx;
                 rw.line_start = ls;
                 rw.item = L;
-                C = (uint32_t)(le - ls);
+                C = le > ls ? (uint32_t)(le - ls) : 0u; // le < ls only for non-finite input
             }
             uint32_t tot_c;
             const uint32_t ex_c = block_excl_scan(C, sh, tot_c);
@@ -679,12 +767,52 @@ struct FaceItem
     float nxy[3][2], dxy[3], nyz[3][2], dyz[3], nzx[3][2], dzx[3];
     int bx0, bx1, by0, bz0, ny;
     uint32_t face;
+    uint32_t r0; // first bounding-box row (linear (y, z) index) handled here
 };
+// A face whose bounding box holds more than FACE_VOX_CAP voxels is split into
+// chunks of bounding-box rows of about that many voxels; all but the last are queued
+constexpr uint32_t FACE_VOX_CAP = 1024;
 
 struct FaceRow
 {
     int item, y, z;
+    int x0, x1; // candidate voxels [x0, x1) of the row
 };
+
+// Voxels of bounding-box row (y, z) that can pass the slab test of
+// face_pass: with p the voxel's min corner, n.p must lie between -d1 and
+// -d2, an interval of x for |n_x| large enough.  The interval is computed in
+// double and widened by a bound on the FP32 test's rounding (a few ulp of
+// the terms of n.p + d, over |n_x|) plus 2 voxels; for |n_x| < 0.05 the
+// whole bounding-box row is kept.
+__device__ __forceinline__ void face_row_range(const FaceItem &it, const GridDev &g, int y, int z, int &x0, int &x1)
+{
+    x0 = it.bx0;
+    x1 = it.bx1;
+    const double nx = (double)it.n[0];
+    if (!(fabs(nx) >= 0.05))
+        return;
+    const double vs0 = (double)g.vs[0];
+    const double py = (double)g.vs[1] * (double)y, pz = (double)g.vs[2] * (double)z;
+    const double K = (double)it.n[1] * py + (double)it.n[2] * pz;
+    const double a = (-(double)it.d1 - K) / nx, b = (-(double)it.d2 - K) / nx;
+    const double pxm = fmax(fabs((double)it.bx0), fabs((double)it.bx1)) * vs0;
+    const double terms = fabs(nx) * pxm + fabs((double)it.n[1] * py) + fabs((double)it.n[2] * pz) +
+                         fabs((double)it.d1) + fabs((double)it.d2);
+    const double margin = 2.0 + 1.0e-6 * terms / (fabs(nx) * vs0);
+    const double lo = fmax(fmin(a, b) / vs0 - margin, (double)it.bx0 - 1.0);
+    const double hi = fmin(fmax(a, b) / vs0 + margin, (double)it.bx1 + 1.0);
+    if (!(lo <= hi))
+    {
+        x1 = x0; // empty (also for non-finite planes)
+        return;
+    }
+    const int l = (int)floor(lo), h = (int)floor(hi) + 1;
+    x0 = l > x0 ? l : x0;
+    x1 = h < x1 ? h : x1;
+    if (x1 < x0)
+        x1 = x0;
+}
 
 __device__ __forceinline__ bool face_pass(const FaceItem &it, const float vs[3], int x, int y, int z)
 {
@@ -728,6 +856,76 @@ __device__ __forceinline__ int select_bit(uint64_t m, uint32_t k)
     return pos;
 }
 
+// Face-test set-up of one face (FDG.cpp:304-360): plane, slab offsets, the
+// 9 projected edge half-planes and the clamped bounding box; returns the
+// number of bounding-box rows (y, z).
+__device__ __forceinline__ uint64_t face_item_setup(const MeshDev &m, const GridDev &g, int64_t face, FaceItem &it)
+{
+    float v0[3], v1[3], v2[3], pl[4];
+    load_tri(m, face, v0, v1, v2);
+    tri_plane(v0, v1, v2, pl);
+    const float n0 = pl[0], n1 = pl[1], n2 = pl[2];
+    int bmin[3], bmax[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        const float mn = sminf(sminf(v0[i], v1[i]), v2[i]) / g.vs[i];
+        const float mx = smaxf(smaxf(v0[i], v1[i]), v2[i]) / g.vs[i];
+        const int a = (int)mn;
+        const int b = (int)(mx + 1.0f);
+        bmin[i] = a > g.gmin[i] ? a : g.gmin[i];
+        bmax[i] = b < g.gmax[i] ? b : g.gmax[i];
+    }
+    const float c0 = n0 > 0.0f ? g.vs[0] : 0.0f;
+    const float c1 = n1 > 0.0f ? g.vs[1] : 0.0f;
+    const float c2 = n2 > 0.0f ? g.vs[2] : 0.0f;
+    it.n[0] = n0;
+    it.n[1] = n1;
+    it.n[2] = n2;
+    it.d1 = dot3f(n0, n1, n2, c0 - v0[0], c1 - v0[1], c2 - v0[2]);
+    it.d2 = dot3f(n0, n1, n2, (g.vs[0] - c0) - v0[0], (g.vs[1] - c1) - v0[1], (g.vs[2] - c2) - v0[2]);
+    const float mxy = (n2 < 0.0f) ? -1.0f : 1.0f;
+    const float myz = (n0 < 0.0f) ? -1.0f : 1.0f;
+    const float mzx = (n1 < 0.0f) ? -1.0f : 1.0f;
+    const float *P[3] = {v0, v1, v2};
+    float E[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        E[0][i] = v1[i] - v0[i];
+        E[1][i] = v2[i] - v1[i];
+        E[2][i] = v0[i] - v2[i];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        const float *p = P[k];
+        float a0 = (-mxy) * E[k][1], a1 = mxy * E[k][0];
+        it.nxy[k][0] = a0;
+        it.nxy[k][1] = a1;
+        it.dxy[k] = -(a0 * p[0] + a1 * p[1]) + (smaxf(a0, 0.0f) * g.vs[0] + smaxf(a1, 0.0f) * g.vs[1]);
+        a0 = (-myz) * E[k][2];
+        a1 = myz * E[k][1];
+        it.nyz[k][0] = a0;
+        it.nyz[k][1] = a1;
+        it.dyz[k] = -(a0 * p[1] + a1 * p[2]) + (smaxf(a0, 0.0f) * g.vs[1] + smaxf(a1, 0.0f) * g.vs[2]);
+        a0 = (-mzx) * E[k][0];
+        a1 = mzx * E[k][2];
+        it.nzx[k][0] = a0;
+        it.nzx[k][1] = a1;
+        it.dzx[k] = -(a0 * p[2] + a1 * p[0]) + (smaxf(a0, 0.0f) * g.vs[2] + smaxf(a1, 0.0f) * g.vs[0]);
+    }
+    it.bx0 = bmin[0];
+    it.bx1 = bmax[0];
+    it.by0 = bmin[1];
+    it.bz0 = bmin[2];
+    it.ny = bmax[1] - bmin[1];
+    it.face = (uint32_t)face;
+    if (bmax[0] > bmin[0] && bmax[1] > bmin[1] && bmax[2] > bmin[2])
+        return (uint64_t)(bmax[1] - bmin[1]) * (uint64_t)(bmax[2] - bmin[2]);
+    return 0;
+}
+
 __device__ __forceinline__ uint64_t row_mask(int lo_bit, int hi_bit)
 {
     // bits [lo_bit, hi_bit) of a 64-bit word, 0 <= lo_bit < hi_bit <= 64
@@ -736,7 +934,11 @@ __device__ __forceinline__ uint64_t row_mask(int lo_bit, int hi_bit)
     return hi & ~lo;
 }
 
-__global__ __launch_bounds__(BT) void k_face(MeshDev m, GridDev g, ShardBuf<FHit> hb, unsigned long long *ctr)
+// QUEUED = false: one item per face; QUEUED = true: the queued row chunks of
+// large faces (items index the queue's slots)
+template <bool QUEUED>
+__global__ __launch_bounds__(BT) void k_face(MeshDev m, GridDev g, ShardBuf<FHit> hb, unsigned long long *ctr,
+                                             ShardBuf<FaceChunk> fq)
 {
     __shared__ FaceItem items[BT];
     __shared__ FaceRow rows[BT];
@@ -744,78 +946,62 @@ __global__ __launch_bounds__(BT) void k_face(MeshDev m, GridDev g, ShardBuf<FHit
     __shared__ uint32_t incl_rows[BT];
     __shared__ uint64_t sh64[NWAVE];
     __shared__ uint32_t sh32[NWAVE];
+    __shared__ uint32_t qpre[QUEUED ? NSHARD : 1];
     const int tid = threadIdx.x;
     uint32_t n_hit = 0;
+    int64_t n_items = m.F;
+    if constexpr (QUEUED)
+        n_items = (int64_t)queue_prefix(fq, qpre, sh32);
 
-    for (int64_t base = (int64_t)blockIdx.x * BT; base < m.F; base += (int64_t)gridDim.x * BT)
+    for (int64_t base = (int64_t)blockIdx.x * BT; base < n_items; base += (int64_t)gridDim.x * BT)
     {
-        const int64_t face = base + tid;
+        const int64_t item = base + tid;
         uint64_t R = 0;
-        if (face < m.F)
+        if constexpr (!QUEUED)
         {
-            float v0[3], v1[3], v2[3], pl[4];
-            load_tri(m, face, v0, v1, v2);
-            tri_plane(v0, v1, v2, pl);
-            FaceItem &it = items[tid];
-            const float n0 = pl[0], n1 = pl[1], n2 = pl[2];
-            int bmin[3], bmax[3];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
+            if (item < m.F)
             {
-                const float mn = sminf(sminf(v0[i], v1[i]), v2[i]) / g.vs[i];
-                const float mx = smaxf(smaxf(v0[i], v1[i]), v2[i]) / g.vs[i];
-                const int a = (int)mn;
-                const int b = (int)(mx + 1.0f);
-                bmin[i] = a > g.gmin[i] ? a : g.gmin[i];
-                bmax[i] = b < g.gmax[i] ? b : g.gmax[i];
+                FaceItem &it = items[tid];
+                R = face_item_setup(m, g, item, it);
+                it.r0 = 0;
+                const uint32_t ext = (uint32_t)(it.bx1 > it.bx0 ? it.bx1 - it.bx0 : 1);
+                const uint64_t rpc = ext < FACE_VOX_CAP ? FACE_VOX_CAP / ext : 1u;
+                if (R > rpc && fq.items)
+                {
+                    // all but the last rpc rows go to the row queue, in chunks
+                    // of rpc rows, claimed QUEUE_GROUP chunks at a time from
+                    // successive shards; the rows of the first group that does
+                    // not fit stay with this block
+                    const uint64_t nq = (R - 1) / rpc;
+                    uint64_t done = 0;
+                    for (uint64_t c0 = 0; c0 < nq; c0 += QUEUE_GROUP)
+                    {
+                        const uint64_t n = nq - c0 < QUEUE_GROUP ? nq - c0 : QUEUE_GROUP;
+                        uint64_t q0;
+                        if (!shard_claim_fit(fq, (uint32_t)((item + c0 / QUEUE_GROUP) % NSHARD), n, q0))
+                            break;
+                        for (uint64_t c = 0; c < n; ++c)
+                        {
+                            const uint64_t r0 = rpc * (c0 + c);
+                            fq.items[q0 + c] = FaceChunk{it.face, (uint32_t)r0, (uint32_t)(r0 + rpc)};
+                        }
+                        done = c0 + n;
+                    }
+                    it.r0 = (uint32_t)(rpc * done);
+                    R -= rpc * done;
+                }
             }
-            const float c0 = n0 > 0.0f ? g.vs[0] : 0.0f;
-            const float c1 = n1 > 0.0f ? g.vs[1] : 0.0f;
-            const float c2 = n2 > 0.0f ? g.vs[2] : 0.0f;
-            it.n[0] = n0;
-            it.n[1] = n1;
-            it.n[2] = n2;
-            it.d1 = dot3f(n0, n1, n2, c0 - v0[0], c1 - v0[1], c2 - v0[2]);
-            it.d2 = dot3f(n0, n1, n2, (g.vs[0] - c0) - v0[0], (g.vs[1] - c1) - v0[1], (g.vs[2] - c2) - v0[2]);
-            const float mxy = (n2 < 0.0f) ? -1.0f : 1.0f;
-            const float myz = (n0 < 0.0f) ? -1.0f : 1.0f;
-            const float mzx = (n1 < 0.0f) ? -1.0f : 1.0f;
-            const float *P[3] = {v0, v1, v2};
-            float E[3][3];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
+        }
+        else
+        {
+            if (item < n_items)
             {
-                E[0][i] = v1[i] - v0[i];
-                E[1][i] = v2[i] - v1[i];
-                E[2][i] = v0[i] - v2[i];
+                const FaceChunk ch = fq.items[queue_slot(fq, qpre, (uint32_t)item)];
+                FaceItem &it = items[tid];
+                face_item_setup(m, g, (int64_t)ch.face, it);
+                it.r0 = ch.r0;
+                R = (uint64_t)(ch.r1 - ch.r0);
             }
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-            {
-                const float *p = P[k];
-                float a0 = (-mxy) * E[k][1], a1 = mxy * E[k][0];
-                it.nxy[k][0] = a0;
-                it.nxy[k][1] = a1;
-                it.dxy[k] = -(a0 * p[0] + a1 * p[1]) + (smaxf(a0, 0.0f) * g.vs[0] + smaxf(a1, 0.0f) * g.vs[1]);
-                a0 = (-myz) * E[k][2];
-                a1 = myz * E[k][1];
-                it.nyz[k][0] = a0;
-                it.nyz[k][1] = a1;
-                it.dyz[k] = -(a0 * p[1] + a1 * p[2]) + (smaxf(a0, 0.0f) * g.vs[1] + smaxf(a1, 0.0f) * g.vs[2]);
-                a0 = (-mzx) * E[k][0];
-                a1 = mzx * E[k][2];
-                it.nzx[k][0] = a0;
-                it.nzx[k][1] = a1;
-                it.dzx[k] = -(a0 * p[2] + a1 * p[0]) + (smaxf(a0, 0.0f) * g.vs[2] + smaxf(a1, 0.0f) * g.vs[0]);
-            }
-            it.bx0 = bmin[0];
-            it.bx1 = bmax[0];
-            it.by0 = bmin[1];
-            it.bz0 = bmin[2];
-            it.ny = bmax[1] - bmin[1];
-            it.face = (uint32_t)face;
-            if (bmax[0] > bmin[0] && bmax[1] > bmin[1] && bmax[2] > bmin[2])
-                R = (uint64_t)(bmax[1] - bmin[1]) * (uint64_t)(bmax[2] - bmin[2]);
         }
         uint64_t tot_r;
         const uint64_t ex_r = block_excl_scan(R, sh64, tot_r);
@@ -831,16 +1017,17 @@ __global__ __launch_bounds__(BT) void k_face(MeshDev m, GridDev g, ShardBuf<FHit
                 const int L = upper_find(incl_items, BT, gi);
                 const FaceItem &it = items[L];
                 const uint64_t R_L = incl_items[L] - (L ? incl_items[L - 1] : 0ull);
-                const uint64_t rr = gi - (incl_items[L] - R_L);
+                const uint64_t rr = (uint64_t)it.r0 + (gi - (incl_items[L] - R_L));
                 const int y = it.by0 + (int)(rr % (uint64_t)it.ny);
                 const int z = it.bz0 + (int)(rr / (uint64_t)it.ny);
                 FaceRow &row = rows[tid];
                 row.item = L;
                 row.y = y;
                 row.z = z;
-                const int lx0 = it.bx0 - g.gmin[0], lx1 = it.bx1 - g.gmin[0];
+                face_row_range(it, g, y, z, row.x0, row.x1);
+                const int lx0 = row.x0 - g.gmin[0], lx1 = row.x1 - g.gmin[0];
                 const uint64_t rowbase = ((uint64_t)(z - g.gmin[2]) * g.G[1] + (uint64_t)(y - g.gmin[1])) * g.wx;
-                for (int w = lx0 >> 6; w <= ((lx1 - 1) >> 6); ++w)
+                for (int w = lx0 >> 6; lx1 > lx0 && w <= ((lx1 - 1) >> 6); ++w)
                 {
                     const int blo = (w == (lx0 >> 6)) ? (lx0 & 63) : 0;
                     const int bhi = (w == ((lx1 - 1) >> 6)) ? (((lx1 - 1) & 63) + 1) : 64;
@@ -862,7 +1049,7 @@ __global__ __launch_bounds__(BT) void k_face(MeshDev m, GridDev g, ShardBuf<FHit
                     const FaceRow &row = rows[rr];
                     const FaceItem &it = items[row.item];
                     uint32_t k = h - ((rr == 0) ? 0u : incl_rows[rr - 1]);
-                    const int lx0 = it.bx0 - g.gmin[0], lx1 = it.bx1 - g.gmin[0];
+                    const int lx0 = row.x0 - g.gmin[0], lx1 = row.x1 - g.gmin[0];
                     const uint64_t rowbase =
                         ((uint64_t)(row.z - g.gmin[2]) * g.G[1] + (uint64_t)(row.y - g.gmin[1])) * g.wx;
                     for (int w = lx0 >> 6; w <= ((lx1 - 1) >> 6); ++w)
@@ -2410,19 +2597,37 @@ hipError_t launch_prep(const MeshDev &m, int64_t V, const GridDev &g, float4 *pl
 }
 
 hipError_t launch_scan_events(const MeshDev &m, const GridDev &g, unsigned long long *ctr, const ShardBuf<Event> &eb,
-                              bool merge, hipStream_t s)
+                              const ShardBuf<RowChunk> &rq, bool merge, hipStream_t s)
 {
+    // row chunks of large triangles: queue emptied, filled by the first
+    // launch, consumed by the second
+    hipError_t e = hipMemsetAsync(rq.cnt, 0, (size_t)NSHARD * SHARD_STRIDE * sizeof(unsigned long long), s);
+    if (e != hipSuccess)
+        return e;
+    const unsigned gq = 2048; // consumers loop over the filled slots
     if (merge)
-        hipLaunchKernelGGL(k_scan_events<true>, dim3(grid_for(3 * m.F)), dim3(BT), 0, s, m, g, ctr, eb);
+    {
+        hipLaunchKernelGGL((k_scan_events<true, false>), dim3(grid_for(3 * m.F)), dim3(BT), 0, s, m, g, ctr, eb, rq);
+        hipLaunchKernelGGL((k_scan_events<true, true>), dim3(gq), dim3(BT), 0, s, m, g, ctr, eb, rq);
+    }
     else
-        hipLaunchKernelGGL(k_scan_events<false>, dim3(grid_for(3 * m.F)), dim3(BT), 0, s, m, g, ctr, eb);
+    {
+        hipLaunchKernelGGL((k_scan_events<false, false>), dim3(grid_for(3 * m.F)), dim3(BT), 0, s, m, g, ctr, eb, rq);
+        hipLaunchKernelGGL((k_scan_events<false, true>), dim3(gq), dim3(BT), 0, s, m, g, ctr, eb, rq);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_face(const MeshDev &m, const GridDev &g, const ShardBuf<FHit> &hb, unsigned long long *ctr,
-                       hipStream_t s)
+                       const ShardBuf<FaceChunk> &fq, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_face, dim3(grid_for(m.F)), dim3(BT), 0, s, m, g, hb, ctr);
+    // row chunks of large faces: queue emptied, filled by the first launch,
+    // consumed by the second
+    hipError_t e = hipMemsetAsync(fq.cnt, 0, (size_t)NSHARD * SHARD_STRIDE * sizeof(unsigned long long), s);
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_face<false>, dim3(grid_for(m.F)), dim3(BT), 0, s, m, g, hb, ctr, fq);
+    hipLaunchKernelGGL(k_face<true>, dim3(2048), dim3(BT), 0, s, m, g, hb, ctr, fq); // loops over the filled slots
     return hipGetLastError();
 }
 
