@@ -176,6 +176,8 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
 
+    if args.workload == "batch":
+        return run_batch(args, torch, o_voxel, _C, dev)
     v, f, grid, seed = make_workload(args.workload, args.grid)
     F, V = int(f.shape[0]), int(v.shape[0])
     log(f"[bench] workload={args.workload} F={F} V={V} grid={grid}")
@@ -331,6 +333,90 @@ def main():
     if precision is not None:
         line["precision"] = precision
         line["config"]["precision"] = "compensated"
+    print(json.dumps(line), flush=True)
+
+
+def run_batch(args, torch, o_voxel, _C, dev):
+    """256 seeded meshes, faces log-uniform in [500, 500k], grid per mesh from
+    {64, 128, 256, 512}: one batched conversion per step vs the same meshes
+    converted by 256 plain calls."""
+    import meshgen
+
+    meshes = meshgen.batch(n_meshes=256, seed=5)
+    F = sum(len(f) for _, f, _ in meshes)
+    log(f"[bench] batch: {len(meshes)} meshes, {F} faces")
+    aabb = [[-0.5, -0.5, -0.5], [0.5, 0.5, 0.5]]
+    ms_in = [(torch.from_numpy(v).to(dev), torch.from_numpy(f).to(dev)) for v, f, _ in meshes]
+    grids = [g for _, _, g in meshes]
+    kw = dict(face_weight=WEIGHTS[0], boundary_weight=WEIGHTS[1], regularization_weight=WEIGHTS[2])
+
+    def batched():
+        return o_voxel.convert.mesh_to_flexible_dual_grid_batch(ms_in, grid_sizes=grids, aabb=aabb, **kw)
+
+    def sequential():
+        out, lat = [], []
+        for (vt, ft), g in zip(ms_in, grids):
+            t0 = time.perf_counter()
+            out.append(o_voxel.convert.mesh_to_flexible_dual_grid(vt, ft, grid_size=g, aabb=aabb, **kw))
+            torch.cuda.synchronize()
+            lat.append((time.perf_counter() - t0) * 1e3)
+        return out, lat
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batched()
+    torch.cuda.synchronize()
+    cold_ms = (time.perf_counter() - t0) * 1e3
+    for _ in range(args.warmup):
+        batched()
+        sequential()
+    torch.cuda.synchronize()
+    # timed: K batched steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out_b = batched()
+    torch.cuda.synchronize()
+    batch_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    # the same meshes, one plain call each (per-call latency recorded)
+    seq_ms, lat_all = [], []
+    for _ in range(max(1, args.steps // 4)):
+        t0 = time.perf_counter()
+        out_s, lat = sequential()
+        seq_ms.append((time.perf_counter() - t0) * 1e3)
+        lat_all += lat
+    seq_ms = float(np.median(seq_ms))
+    same = all(all(torch.equal(a, b) for a, b in zip(ob, os_)) for ob, os_ in zip(out_b, out_s))
+    st = _C.last_stats(dev)
+    line = {
+        "metric": METRIC,
+        "value": round(F / (batch_ms * 1e-3) / 1e6, 4),
+        "unit": "M faces/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(batch_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"batch-{len(meshes)}meshes-{F}f", "meshes": len(meshes), "faces": F, "seed": 5,
+                   "grids": {str(g): grids.count(g) for g in sorted(set(grids))},
+                   "weights": {"face": WEIGHTS[0], "boundary": WEIGHTS[1], "regularization": WEIGHTS[2]}},
+        "batch": {
+            "batched_ms": round(batch_ms, 3),
+            "batched_meshes_per_s": round(len(meshes) / (batch_ms * 1e-3), 1),
+            "sequential_ms": round(seq_ms, 3),
+            "sequential_meshes_per_s": round(len(meshes) / (seq_ms * 1e-3), 1),
+            "speedup": round(seq_ms / batch_ms, 3),
+            "sequential_p50_ms_per_mesh": round(float(np.percentile(lat_all, 50)), 4),
+            "sequential_p99_ms_per_mesh": round(float(np.percentile(lat_all, 99)), 4),
+            "batched_ms_per_mesh": round(batch_ms / len(meshes), 4),
+            "outputs_identical_to_sequential": bool(same),
+        },
+        "cold_ms": round(cold_ms, 3),
+        "counts": {k: v for k, v in st.items() if k != "workspace_bytes"},
+    }
     print(json.dumps(line), flush=True)
 
 

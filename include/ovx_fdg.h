@@ -87,6 +87,26 @@ ovx_status ovx_fdg_trim(ovx_fdg_ctx *ctx);
 ovx_status ovx_fdg_begin(ovx_fdg_ctx *ctx, const float *d_vertices, int64_t n_vertices, const int32_t *d_faces,
                          int64_t n_faces, const ovx_grid *grid, float face_weight, float boundary_weight,
                          float regularization_weight, void *stream, int64_t *n_voxels);
+/*
+ * Batched conversion: M meshes of any sizes and grids in one pass, every
+ * stage one launch over all of them.  No counterpart in the reference (its
+ * API converts one mesh per call, FDG.py:38-205); each mesh's results are the
+ * ones ovx_fdg_begin / ovx_fdg_finish give for it alone.
+ *   d_vertices  float32 [n_vertices, 3]: the meshes' grid-local vertices, mesh after mesh
+ *   d_faces     int32 [n_faces, 3]: faces, mesh after mesh, indices into d_vertices
+ *               (batch-global: mesh m's indices lie in [vertex_offsets[m], vertex_offsets[m+1]))
+ *   face_offsets, vertex_offsets  host int64 [n_meshes + 1] prefix offsets (first 0, last n_faces / n_vertices)
+ *   grids       host [n_meshes]: each mesh's grid
+ *   n_voxels    host int64 [n_meshes]: each mesh's voxel count
+ * Then ovx_fdg_finish writes all meshes' outputs, mesh after mesh (mesh m's
+ * rows start at the sum of the voxel counts before it).  Blocks once, like
+ * ovx_fdg_begin.
+ */
+ovx_status ovx_fdg_batch_begin(ovx_fdg_ctx *ctx, const float *d_vertices, int64_t n_vertices, const int32_t *d_faces,
+                               int64_t n_faces, int n_meshes, const int64_t *face_offsets,
+                               const int64_t *vertex_offsets, const ovx_grid *grids, float face_weight,
+                               float boundary_weight, float regularization_weight, void *stream, int64_t *n_voxels);
+
 ovx_status ovx_fdg_finish(ovx_fdg_ctx *ctx, int32_t *d_coords, float *d_dual, uint8_t *d_intersected, void *stream);
 
 /*

@@ -9,11 +9,14 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "ovx_fdg.h"
 #include "fdg_kernels.h"
 
 using namespace ovx;
+
+constexpr int BT_HOST = 256; // threads per block of the tiled kernels (scanline items, face test)
 
 namespace
 {
@@ -84,8 +87,19 @@ struct ovx_fdg_ctx
     int pending = 0; // 0 none, 1 fdg, 2 occ
     MeshDev mesh{};
     int64_t V = 0;
-    GridDev grid{};
-    uint64_t n_words = 0;
+    uint64_t n_words = 0;  // occupancy words of all meshes
+    uint64_t n_bricks = 0; // bricks of all meshes
+    // the batch (a single conversion is a batch of one): host tables, their
+    // device copy (segbuf) and the pinned staging of the upload / readback
+    int M = 0;
+    std::vector<int64_t> face_off, vert_off;
+    std::vector<uint64_t> word_off, brick_off, mtile_off, ftile_off;
+    std::vector<MeshSeg> segs;
+    DevBuf segbuf;
+    unsigned char *h_stage = nullptr;
+    size_t h_stage_bytes = 0;
+    uint64_t *h_vox_off = nullptr; // inside h_stage: per-mesh first voxel ids, read back by begin
+    BatchDev bdev{};
     float fw = 0, bw = 0, rw = 0;
     int prec = 0;           // 0 reference float path, 1 compensated voxel-frame accumulation
     float cond_max = 1.0e6f; // precision variant: FP64 solve above this condition estimate
@@ -105,7 +119,7 @@ struct ovx_fdg_ctx
     {
         DevBuf *all[] = {&planes, &bits,   &woff,  &tiles,   &vcnt,   &voff,  &nbr, &erank, &bcnt, &boff,
                          &bedge_v, &bedges, &events, &recs,   &chunks, &heads, &choff, &bchunks, &active, &large,
-                         &box,     &qef,    &bhits,  &fhits,  &shard_ctr, &rowq, &faceq};
+                         &box,     &qef,    &bhits,  &fhits,  &shard_ctr, &rowq, &faceq, &segbuf};
         for (DevBuf *b : all)
             fn(*b);
     }
@@ -267,19 +281,113 @@ ovx_status clear_shards(ovx_fdg_ctx *c, int sb0, int nsb, hipStream_t s)
     return OVX_OK;
 }
 
-// common begin: validation, (planes), mark, compaction count
-ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int64_t V, const ovx_grid *grid, bool planes,
-                        hipStream_t s, int64_t *n_voxels)
+// Batch tables: per-mesh grids, prefix offsets of faces, vertices, bitset
+// words, bricks and block tiles; the device copy is uploaded on stream s.
+ovx_status setup_batch(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *face_off, const int64_t *vert_off,
+                       const ovx_grid *grids, hipStream_t s)
+{
+    if (M < 1)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "a batch needs at least one mesh");
+    if (!face_off || !vert_off || !grids)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "batch tables are null");
+    if (face_off[0] != 0 || vert_off[0] != 0 || face_off[M] != m.F || vert_off[M] != m.V)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "batch offsets must start at 0 and end at the face / vertex counts");
+    c->M = M;
+    c->face_off.assign(face_off, face_off + M + 1);
+    c->vert_off.assign(vert_off, vert_off + M + 1);
+    c->word_off.assign(M + 1, 0);
+    c->brick_off.assign(M + 1, 0);
+    c->mtile_off.assign(M + 1, 0);
+    c->ftile_off.assign(M + 1, 0);
+    c->segs.assign(M, MeshSeg{});
+    for (int i = 0; i < M; ++i)
+    {
+        if (face_off[i + 1] < face_off[i] || vert_off[i + 1] < vert_off[i])
+            return fail(OVX_ERR_INVALID_ARGUMENT, "batch offsets must not decrease (mesh %d)", i);
+        GridDev g{};
+        uint64_t nw = 0;
+        ovx_status st = check_grid(grids + i, g, nw);
+        if (st)
+            return st;
+        MeshSeg &sg = c->segs[i];
+        sg.g = g;
+        for (int a = 0; a < 3; ++a)
+            sg.nb[a] = (g.G[a] + 7) / 8;
+        sg.brick0 = c->brick_off[i];
+        sg.face0 = face_off[i];
+        const int64_t nf = face_off[i + 1] - face_off[i];
+        c->word_off[i + 1] = c->word_off[i] + nw;
+        c->brick_off[i + 1] = c->brick_off[i] + (uint64_t)sg.nb[0] * (uint64_t)sg.nb[1] * (uint64_t)sg.nb[2];
+        c->mtile_off[i + 1] = c->mtile_off[i] + (uint64_t)(3 * nf + BT_HOST - 1) / BT_HOST;
+        c->ftile_off[i + 1] = c->ftile_off[i] + (uint64_t)(nf + BT_HOST - 1) / BT_HOST;
+    }
+    c->n_words = c->word_off[M];
+    c->n_bricks = c->brick_off[M];
+    if (c->n_words >= (1ull << 32))
+        return fail(OVX_ERR_CAPACITY, "occupancy bitset of %llu words exceeds 2^32 - 1", (unsigned long long)c->n_words);
+    if (c->n_bricks >= (1ull << 32))
+        return fail(OVX_ERR_CAPACITY, "brick grids of %llu bricks exceed 2^32", (unsigned long long)c->n_bricks);
+    ovx_status st;
+    if ((st = ensure(c->bits, c->n_words * sizeof(uint64_t))) || (st = ensure(c->woff, c->n_words * sizeof(uint32_t))))
+        return st;
+    for (int i = 0; i < M; ++i)
+    {
+        c->segs[i].g.bits = (uint64_t *)c->bits.p + c->word_off[i];
+        c->segs[i].g.woff = (const uint32_t *)c->woff.p + c->word_off[i];
+    }
+    // device layout: segments, then 7 offset arrays of M + 1 entries
+    const size_t seg_bytes = ((sizeof(MeshSeg) * (size_t)M + 255) / 256) * 256;
+    const size_t arr = (size_t)(M + 1) * 8;
+    const size_t total = seg_bytes + 7 * arr;
+    if (c->h_stage_bytes < total + arr)
+    {
+        if (c->ran)
+            HIP_TRY(hipStreamSynchronize(c->last_stream)); // the staging may still be read by a copy
+        if (c->h_stage)
+            HIP_TRY(hipHostFree(c->h_stage));
+        c->h_stage = nullptr;
+        c->h_stage_bytes = 0;
+        HIP_TRY(hipHostMalloc((void **)&c->h_stage, total + arr));
+        c->h_stage_bytes = total + arr;
+    }
+    if ((st = ensure(c->segbuf, total)))
+        return st;
+    unsigned char *h = c->h_stage;
+    std::memcpy(h, c->segs.data(), sizeof(MeshSeg) * (size_t)M);
+    std::memcpy(h + seg_bytes + 0 * arr, c->face_off.data(), arr);
+    std::memcpy(h + seg_bytes + 1 * arr, c->vert_off.data(), arr);
+    std::memcpy(h + seg_bytes + 2 * arr, c->word_off.data(), arr);
+    std::memcpy(h + seg_bytes + 3 * arr, c->brick_off.data(), arr);
+    std::memset(h + seg_bytes + 4 * arr, 0, arr); // voxel offsets: written on the device
+    std::memcpy(h + seg_bytes + 5 * arr, c->mtile_off.data(), arr);
+    std::memcpy(h + seg_bytes + 6 * arr, c->ftile_off.data(), arr);
+    c->h_vox_off = (uint64_t *)(h + total);
+    HIP_TRY(hipMemcpyAsync(c->segbuf.p, h, total, hipMemcpyHostToDevice, s));
+    unsigned char *d = (unsigned char *)c->segbuf.p;
+    BatchDev &b = c->bdev;
+    b.seg = (const MeshSeg *)d;
+    b.face_off = (const int64_t *)(d + seg_bytes + 0 * arr);
+    b.vert_off = (const int64_t *)(d + seg_bytes + 1 * arr);
+    b.word_off = (const uint64_t *)(d + seg_bytes + 2 * arr);
+    b.brick_off = (const uint64_t *)(d + seg_bytes + 3 * arr);
+    b.vox_off = (const uint64_t *)(d + seg_bytes + 4 * arr);
+    b.mtile_off = (const uint64_t *)(d + seg_bytes + 5 * arr);
+    b.ftile_off = (const uint64_t *)(d + seg_bytes + 6 * arr);
+    b.M = M;
+    return OVX_OK;
+}
+
+// common begin of a batch: validation, (planes), mark, compaction count;
+// n_voxels[0..M) receives each mesh's voxel count
+ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *face_off, const int64_t *vert_off,
+                        const ovx_grid *grids, bool planes, hipStream_t s, int64_t *n_voxels)
 {
     c->pending = 0;
     c->N = 0;
     for (int i = 0; i < OVX_FDG_N_STATS; ++i)
         c->stats[i] = 0;
-    GridDev g{};
-    uint64_t n_words = 0;
-    ovx_status st = check_grid(grid, g, n_words);
-    if (st)
-        return st;
+    ovx_status st;
+    const int64_t V = m.V;
     if (m.F < 0 || V < 0)
         return fail(OVX_ERR_INVALID_ARGUMENT, "negative size");
     if (m.F > 0x7fffffffll)
@@ -291,32 +399,31 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int64_t V, const ovx_g
     if ((st = ensure(c->counters, CTR_COUNT * sizeof(unsigned long long))))
         return st;
     HIP_TRY(hipMemsetAsync(c->counters.p, 0, CTR_COUNT * sizeof(unsigned long long), s));
+    if ((st = setup_batch(c, m, M, face_off, vert_off, grids, s)))
+        return st;
     c->mesh = m;
     c->V = V;
     c->last_stream = s;
     c->ran = true;
+    const uint64_t n_words = c->n_words;
     if (m.F == 0)
     {
-        c->grid = g;
-        c->n_words = n_words;
         c->pending = planes ? 1 : 2;
-        *n_voxels = 0;
+        for (int i = 0; i < M; ++i)
+            n_voxels[i] = 0;
         return OVX_OK;
     }
-    if ((st = ensure(c->bits, n_words * sizeof(uint64_t))) || (st = ensure(c->woff, n_words * sizeof(uint32_t))) ||
-        (st = ensure(c->tiles, (scan_tile_count(n_words) + scan_tile_count((uint64_t)m.F * 3 + 1) + 64) * 8)))
+    if ((st = ensure(c->tiles, (scan_tile_count(n_words) + scan_tile_count((uint64_t)m.F * 3 + 1) + 64) * 8)))
         return st;
     if (planes && (st = ensure(c->planes, (size_t)m.F * sizeof(float4))))
         return st;
-    g.bits = (uint64_t *)c->bits.p;
-    g.woff = (const uint32_t *)c->woff.p;
     auto *ctr = (unsigned long long *)c->counters.p;
 
     for (bool &r : c->kran)
         r = false;
     tic(c, KS_PREP, s);
     HIP_TRY(hipMemsetAsync(c->bits.p, 0, n_words * sizeof(uint64_t), s));
-    HIP_TRY(launch_prep(m, V, g, planes ? (float4 *)c->planes.p : nullptr, ctr, s));
+    HIP_TRY(launch_prep(m, c->bdev, planes ? (float4 *)c->planes.p : nullptr, ctr, s));
     toc(c, KS_PREP, s);
     // crossings are recorded once, here, when the buffer can hold them
     if ((st = ensure(c->shard_ctr, (size_t)SB_N * NSHARD * SHARD_STRIDE * sizeof(unsigned long long))))
@@ -333,19 +440,30 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int64_t V, const ovx_g
     // the LDS merging of occupancy bits pays when crossings are dense (faces
     // spanning several voxel rows); with many more faces than voxel rows per
     // plane the crossings are sparse and the table only costs occupancy
-    const double rows_per_plane = std::max({(double)g.G[0] * g.G[1], (double)g.G[1] * g.G[2], (double)g.G[0] * g.G[2]});
+    double rows_per_plane = 0.0;
+    for (const MeshSeg &sg : c->segs)
+    {
+        const GridDev &gm = sg.g;
+        rows_per_plane += std::max({(double)gm.G[0] * gm.G[1], (double)gm.G[1] * gm.G[2], (double)gm.G[0] * gm.G[2]});
+    }
     const bool merge = (double)m.F <= 2.0 * rows_per_plane;
     // row chunks of large triangles, shared out over the grid by a second launch
     if ((st = ensure_shard(c, c->rowq, SB_ROWQ, per_for(std::max<uint64_t>(1ull << 20, (uint64_t)m.F / 4)),
                            sizeof(RowChunk))))
         return st;
-    const ShardBuf<RowChunk> rq = shard_view<RowChunk>(c, c->rowq, SB_ROWQ);
+    ShardBuf<RowChunk> rq = shard_view<RowChunk>(c, c->rowq, SB_ROWQ);
+    if (M > 1)
+        rq.items = nullptr; // row chunks are for single-mesh conversions (a batch has other work to balance)
     tic(c, KS_MARK, s);
-    HIP_TRY(launch_scan_events(m, g, ctr, eb, rq, merge, s));
+    HIP_TRY(launch_scan_events(m, c->bdev, c->mtile_off[M], ctr, eb, rq, merge, s));
     toc(c, KS_MARK, s);
     tic(c, KS_WORDSCAN, s);
-    HIP_TRY(launch_scan_popc(g.bits, n_words, (uint32_t *)c->woff.p, (uint64_t *)c->tiles.p, ctr + CTR_WORDS_TOTAL,
-                             s));
+    HIP_TRY(launch_scan_popc((const uint64_t *)c->bits.p, n_words, (uint32_t *)c->woff.p, (uint64_t *)c->tiles.p,
+                             ctr + CTR_WORDS_TOTAL, s));
+    // first voxel id of each mesh, for the solve and for the caller
+    uint64_t *d_vox_off = (uint64_t *)c->bdev.vox_off;
+    HIP_TRY(launch_seg_voxels(c->bdev, (const uint32_t *)c->woff.p, n_words, ctr + CTR_WORDS_TOTAL, d_vox_off, s));
+    HIP_TRY(hipMemcpyAsync(c->h_vox_off, d_vox_off, (size_t)(M + 1) * 8, hipMemcpyDeviceToHost, s));
     toc(c, KS_WORDSCAN, s);
     c->E = 0;
     if (planes && c->bw > 0.0f)
@@ -395,7 +513,7 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int64_t V, const ovx_g
                 return st;
             ShardBuf<Event> eb2 = shard_view<Event>(c, c->events, SB_EVENTS);
             tic(c, KS_MARK, s);
-            HIP_TRY(launch_scan_events(m, g, nullptr, eb2, rq, merge, s));
+            HIP_TRY(launch_scan_events(m, c->bdev, c->mtile_off[M], nullptr, eb2, rq, merge, s));
             toc(c, KS_MARK, s);
         }
         c->E = (int64_t)c->h_ctr[CTR_BEDGE_TOTAL];
@@ -410,14 +528,13 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int64_t V, const ovx_g
             HIP_TRY(launch_bedge_prep(m.v, c->E, (const int2 *)c->bedge_v.p, (BEdge *)c->bedges.p, s));
         }
     }
-    c->grid = g;
-    c->n_words = n_words;
     c->N = (int64_t)N;
     c->stats[0] = (int64_t)N;
     c->stats[1] = (int64_t)c->n_events;
     c->stats[4] = c->E;
     c->pending = planes ? 1 : 2;
-    *n_voxels = (int64_t)N;
+    for (int i = 0; i < M; ++i)
+        n_voxels[i] = (int64_t)(c->h_vox_off[i + 1] - c->h_vox_off[i]);
     return OVX_OK;
 }
 
@@ -425,19 +542,13 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
 {
     ovx_status st;
     const MeshDev &m = c->mesh;
-    GridDev g = c->grid;
     auto *ctr = (unsigned long long *)c->counters.p;
     tic(c, KS_COORDS, s);
-    HIP_TRY(launch_emit_coords(g, c->n_words, coords, s));
+    HIP_TRY(launch_emit_coords(c->bdev, c->n_words, coords, s));
     toc(c, KS_COORDS, s);
 
-    // ---- records grouped by 8x8x8 brick
-    int nb[3];
-    for (int a = 0; a < 3; ++a)
-        nb[a] = (g.G[a] + 7) / 8;
-    const uint64_t n_bricks = (uint64_t)nb[0] * (uint64_t)nb[1] * (uint64_t)nb[2];
-    if (n_bricks >= (1ull << 32))
-        return fail(OVX_ERR_CAPACITY, "brick grid exceeds 2^32 bricks");
+    // ---- records grouped by 8x8x8 brick (bricks of all meshes, numbered mesh after mesh)
+    const uint64_t n_bricks = c->n_bricks;
     // phase-0 records are exactly 4 per crossing; face and boundary records are
     // estimated from the previous conversion (an overflow is detected and the
     // generation re-run with exact sizes)
@@ -465,30 +576,33 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
         ba.recs = shard_view<Rec>(c, c->recs, SB_RECS);
         ba.chunks = shard_view<Chunk>(c, c->chunks, SB_CHUNKS);
         ba.brick_nch = (uint32_t *)c->heads.p;
-        for (int a = 0; a < 3; ++a)
-            ba.nb[a] = nb[a];
+        ba.b = c->bdev;
+        ba.bedge_v = (const int2 *)c->bedge_v.p;
         ba.ctr = ctr;
         tic(c, KS_GEN_EVENTS, s);
-        HIP_TRY(launch_gen_events(shard_view<Event>(c, c->events, SB_EVENTS), g, ba, s));
+        HIP_TRY(launch_gen_events(shard_view<Event>(c, c->events, SB_EVENTS), ba, s));
         toc(c, KS_GEN_EVENTS, s);
         if (c->fw > 0.0f)
         {
             const ShardBuf<FHit> fb = shard_view<FHit>(c, c->fhits, SB_FHITS);
             tic(c, KS_FACE, s);
-            HIP_TRY(launch_face(m, g, fb, ctr, shard_view<FaceChunk>(c, c->faceq, SB_FACEQ), s));
+            ShardBuf<FaceChunk> fq = shard_view<FaceChunk>(c, c->faceq, SB_FACEQ);
+            if (c->M > 1)
+                fq.items = nullptr; // row chunks are for single-mesh conversions
+            HIP_TRY(launch_face(m, c->bdev, c->ftile_off[c->M], fb, ctr, fq, s));
             toc(c, KS_FACE, s);
             tic(c, KS_GEN_FHITS, s);
-            HIP_TRY(launch_gen_fhits(fb, g, ba, s));
+            HIP_TRY(launch_gen_fhits(fb, ba, s));
             toc(c, KS_GEN_FHITS, s);
         }
         if (c->E > 0)
         {
             const ShardBuf<BHit> hb = shard_view<BHit>(c, c->bhits, SB_BHITS);
             tic(c, KS_BDRY, s);
-            HIP_TRY(launch_bdry(c->E, (const BEdge *)c->bedges.p, g, hb, ctr, s));
+            HIP_TRY(launch_bdry(c->E, (const BEdge *)c->bedges.p, (const int2 *)c->bedge_v.p, c->bdev, hb, ctr, s));
             toc(c, KS_BDRY, s);
             tic(c, KS_GEN_BHITS, s);
-            HIP_TRY(launch_gen_bhits(hb, g, ba, s));
+            HIP_TRY(launch_gen_bhits(hb, ba, s));
             toc(c, KS_GEN_BHITS, s);
         }
         HIP_TRY(launch_shard_reduce((unsigned long long *)c->shard_ctr.p, ctr, s));
@@ -551,9 +665,7 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
     a.n_active = ctr + CTR_ACTIVE;
     a.large = (uint32_t *)c->large.p;
     a.n_large = ctr + CTR_LARGE;
-    for (int i = 0; i < 3; ++i)
-        a.nb[i] = nb[i];
-    a.g = g;
+    a.b = c->bdev; // grid and brick grid are set per brick from its mesh
     a.planes = (const float4 *)c->planes.p;
     a.bedges = (const BEdge *)c->bedges.p;
     a.dual = dual;
@@ -626,6 +738,8 @@ ovx_status ovx_fdg_destroy(ovx_fdg_ctx *c)
         release(c->counters);
         if (c->h_ctr)
             (void)hipHostFree(c->h_ctr);
+        if (c->h_stage)
+            (void)hipHostFree(c->h_stage);
         for (auto &e : c->kev)
             if (e)
                 (void)hipEventDestroy(e);
@@ -665,7 +779,26 @@ ovx_status ovx_fdg_begin(ovx_fdg_ctx *c, const float *d_vertices, int64_t n_vert
     c->fw = face_weight;
     c->bw = boundary_weight;
     c->rw = regularization_weight;
-    return begin_common(c, m, n_vertices, grid, true, (hipStream_t)stream, n_voxels);
+    const int64_t fo[2] = {0, n_faces}, vo[2] = {0, n_vertices};
+    return begin_common(c, m, 1, fo, vo, grid, true, (hipStream_t)stream, n_voxels);
+}
+
+ovx_status ovx_fdg_batch_begin(ovx_fdg_ctx *c, const float *d_vertices, int64_t n_vertices, const int32_t *d_faces,
+                               int64_t n_faces, int n_meshes, const int64_t *face_offsets,
+                               const int64_t *vertex_offsets, const ovx_grid *grids, float face_weight,
+                               float boundary_weight, float regularization_weight, void *stream, int64_t *n_voxels)
+{
+    if (!c || !n_voxels)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "ctx / n_voxels is null");
+    if (n_faces > 0 && !d_faces)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "faces pointer is null");
+    DeviceGuard dg(c->device);
+    std::lock_guard<std::mutex> lk(c->mu);
+    MeshDev m{d_vertices, d_faces, n_faces, n_vertices};
+    c->fw = face_weight;
+    c->bw = boundary_weight;
+    c->rw = regularization_weight;
+    return begin_common(c, m, n_meshes, face_offsets, vertex_offsets, grids, true, (hipStream_t)stream, n_voxels);
 }
 
 ovx_status ovx_fdg_finish(ovx_fdg_ctx *c, int32_t *d_coords, float *d_dual, uint8_t *d_intersected, void *stream)
@@ -694,7 +827,8 @@ ovx_status ovx_occ_begin(ovx_fdg_ctx *c, const float *d_triangles, int64_t n_tri
     DeviceGuard dg(c->device);
     std::lock_guard<std::mutex> lk(c->mu);
     MeshDev m{d_triangles, nullptr, n_triangles, 3 * n_triangles};
-    return begin_common(c, m, 3 * n_triangles, grid, false, (hipStream_t)stream, n_voxels);
+    const int64_t fo[2] = {0, n_triangles}, vo[2] = {0, 3 * n_triangles};
+    return begin_common(c, m, 1, fo, vo, grid, false, (hipStream_t)stream, n_voxels);
 }
 
 ovx_status ovx_occ_finish(ovx_fdg_ctx *c, int32_t *d_coords, void *stream)
@@ -710,7 +844,7 @@ ovx_status ovx_occ_finish(ovx_fdg_ctx *c, int32_t *d_coords, void *stream)
         return OVX_OK;
     if (!d_coords)
         return fail(OVX_ERR_INVALID_ARGUMENT, "output pointer is null");
-    HIP_TRY(launch_emit_coords(c->grid, c->n_words, d_coords, (hipStream_t)stream));
+    HIP_TRY(launch_emit_coords(c->bdev, c->n_words, d_coords, (hipStream_t)stream));
     return OVX_OK;
 }
 

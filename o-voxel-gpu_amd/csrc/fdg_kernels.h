@@ -125,12 +125,57 @@ struct BHit
     int x, y, z;
 };
 
+// One mesh of a batch (a single conversion is a batch of one).  Its grid's
+// bits / woff point at the mesh's first word of the shared occupancy bitset
+// and word offsets, so word indices inside a mesh are local and voxel ids
+// (word offsets) are global: the meshes' voxels follow each other in mesh order.
+struct MeshSeg
+{
+    GridDev g;
+    int nb[3];      // brick grid (8x8x8 voxels per brick)
+    int pad;
+    uint64_t brick0; // first brick of the mesh in the shared brick numbering
+    int64_t face0;   // first face
+};
+
+// the batch on the device: segments and prefix offsets (M + 1 entries each)
+struct BatchDev
+{
+    const MeshSeg *seg;
+    const int64_t *face_off;   // faces
+    const int64_t *vert_off;   // vertices
+    const uint64_t *word_off;  // bitset words
+    const uint64_t *brick_off; // bricks
+    const uint64_t *vox_off;   // voxels (written on the device after the word scan)
+    const uint64_t *mtile_off; // block tiles of scanline items (ceil(3 F_m / block))
+    const uint64_t *ftile_off; // block tiles of faces (ceil(F_m / block))
+    int M;
+};
+
+// the mesh owning index x of a prefix array (largest m with off[m] <= x;
+// empty meshes are skipped over)
+template <typename T>
+__device__ __forceinline__ int seg_find(const T *off, int M, T x)
+{
+    int lo = 0, hi = M - 1;
+    while (lo < hi)
+    {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= x)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
 struct BucketArgs
 {
     ShardBuf<Rec> recs;
     ShardBuf<Chunk> chunks;
     uint32_t *brick_nch;     // [bricks] chunks per brick
-    int nb[3];               // brick grid (8x8x8 voxels per brick)
+    BatchDev b;              // meshes (grid, brick grid and first brick of each)
+    const int2 *bedge_v;     // boundary edges (vertex pairs), to find a boundary hit's mesh
     unsigned long long *ctr; // device counters
 };
 
@@ -163,8 +208,9 @@ struct FoldArgs
     const unsigned long long *n_active; // their count (device)
     uint32_t *large;                    // bricks too large for the main fold kernel
     unsigned long long *n_large;        // their count (device, zeroed by the host)
-    int nb[3];
-    GridDev g; // bitset + voxel ids, grid
+    BatchDev b;                         // meshes
+    int nb[3];  // brick grid of the brick being folded (set per brick from its mesh)
+    GridDev g;  // grid of the brick's mesh (set per brick)
     const float4 *planes;
     const BEdge *bedges;
     float *dual;
@@ -178,15 +224,14 @@ struct FoldArgs
     unsigned long long *ctr;
 };
 
-hipError_t launch_prep(const MeshDev &m, int64_t V, const GridDev &g, float4 *planes, unsigned long long *ctr,
-                       hipStream_t s);
-hipError_t launch_scan_events(const MeshDev &m, const GridDev &g, unsigned long long *ctr, const ShardBuf<Event> &eb,
-                              const ShardBuf<RowChunk> &rq, bool merge, hipStream_t s);
-hipError_t launch_face(const MeshDev &m, const GridDev &g, const ShardBuf<FHit> &hb, unsigned long long *ctr,
-                       const ShardBuf<FaceChunk> &fq, hipStream_t s);
-hipError_t launch_gen_events(const ShardBuf<Event> &ev, const GridDev &g, const BucketArgs &ba, hipStream_t s);
-hipError_t launch_gen_fhits(const ShardBuf<FHit> &h, const GridDev &g, const BucketArgs &ba, hipStream_t s);
-hipError_t launch_gen_bhits(const ShardBuf<BHit> &h, const GridDev &g, const BucketArgs &ba, hipStream_t s);
+hipError_t launch_prep(const MeshDev &m, const BatchDev &b, float4 *planes, unsigned long long *ctr, hipStream_t s);
+hipError_t launch_scan_events(const MeshDev &m, const BatchDev &b, uint64_t n_tiles, unsigned long long *ctr,
+                              const ShardBuf<Event> &eb, const ShardBuf<RowChunk> &rq, bool merge, hipStream_t s);
+hipError_t launch_face(const MeshDev &m, const BatchDev &b, uint64_t n_tiles, const ShardBuf<FHit> &hb,
+                       unsigned long long *ctr, const ShardBuf<FaceChunk> &fq, hipStream_t s);
+hipError_t launch_gen_events(const ShardBuf<Event> &ev, const BucketArgs &ba, hipStream_t s);
+hipError_t launch_gen_fhits(const ShardBuf<FHit> &h, const BucketArgs &ba, hipStream_t s);
+hipError_t launch_gen_bhits(const ShardBuf<BHit> &h, const BucketArgs &ba, hipStream_t s);
 hipError_t launch_shard_reduce(unsigned long long *shard_ctr, unsigned long long *ctr, hipStream_t s);
 hipError_t launch_place_chunks(const ShardBuf<Chunk> &chunks, const uint32_t *choff, uint2 *gchunks,
                                hipStream_t s);
@@ -201,13 +246,15 @@ hipError_t launch_edge_count(const MeshDev &m, uint32_t *vcnt, uint32_t *rank, c
 hipError_t launch_edge_singletons(int64_t V, const uint32_t *voff, uint32_t *nbr, uint32_t *bcnt, const uint32_t *boff,
                                   int2 *bedge_v, int pass, hipStream_t s);
 hipError_t launch_bedge_prep(const float *verts, int64_t E, const int2 *bedge_v, BEdge *be, hipStream_t s);
-hipError_t launch_bdry(int64_t E, const BEdge *be, const GridDev &g, const ShardBuf<BHit> &hb,
+hipError_t launch_bdry(int64_t E, const BEdge *be, const int2 *bedge_v, const BatchDev &b, const ShardBuf<BHit> &hb,
                        unsigned long long *ctr, hipStream_t s);
 uint64_t scan_tile_count(uint64_t n);
 hipError_t launch_scan_popc(const uint64_t *bits, uint64_t n, uint32_t *out, uint64_t *tile_sums,
                             unsigned long long *total, hipStream_t s);
 hipError_t launch_scan_u32(const uint32_t *in, uint64_t n, uint32_t *out, uint64_t *tile_sums,
                            unsigned long long *total, hipStream_t s);
-hipError_t launch_emit_coords(const GridDev &g, uint64_t n_words, int32_t *coords, hipStream_t s);
+hipError_t launch_seg_voxels(const BatchDev &b, const uint32_t *woff, uint64_t n_words,
+                             const unsigned long long *n_voxels, uint64_t *vox_off, hipStream_t s);
+hipError_t launch_emit_coords(const BatchDev &b, uint64_t n_words, int32_t *coords, hipStream_t s);
 
 } // namespace ovx

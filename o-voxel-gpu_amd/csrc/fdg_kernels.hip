@@ -315,8 +315,7 @@ __device__ __forceinline__ void bucket_emit(const uint32_t (&bk)[K], const Rec (
 // ---------------------------------------------------------------------------
 // prep: validate inputs, compute per-face planes
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BT) void k_prep(MeshDev m, int64_t V, GridDev g, float4 *planes,
-                                             unsigned long long *counters)
+__global__ __launch_bounds__(BT) void k_prep(MeshDev m, BatchDev bt, float4 *planes, unsigned long long *counters)
 {
     const int64_t stride = (int64_t)gridDim.x * BT;
     unsigned flags = 0;
@@ -324,9 +323,12 @@ __global__ __launch_bounds__(BT) void k_prep(MeshDev m, int64_t V, GridDev g, fl
     {
         if (m.f)
         {
+            // indices are batch-global: each must name a vertex of the face's own mesh
+            const int ms = seg_find(bt.face_off, bt.M, i);
+            const int64_t v0 = bt.vert_off[ms], v1 = bt.vert_off[ms + 1];
             const int32_t *fi = m.f + 3 * i;
-            const int32_t a = fi[0], b = fi[1], c = fi[2];
-            if (a < 0 || b < 0 || c < 0 || a >= V || b >= V || c >= V)
+            const int64_t a = fi[0], b = fi[1], c = fi[2];
+            if (a < v0 || b < v0 || c < v0 || a >= v1 || b >= v1 || c >= v1)
             {
                 flags |= ERR_FACE_INDEX;
                 continue;
@@ -342,9 +344,10 @@ __global__ __launch_bounds__(BT) void k_prep(MeshDev m, int64_t V, GridDev g, fl
     }
     // vertices: finite and |v / vs| < 2^30 (so every int conversion of the
     // scanline and DDA is in range)
-    const int64_t nv = m.f ? V : 3 * m.F;
+    const int64_t nv = m.f ? m.V : 3 * m.F;
     for (int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x; i < nv; i += stride)
     {
+        const GridDev &g = bt.seg[seg_find(bt.vert_off, bt.M, i)].g;
 #pragma unroll
         for (int a = 0; a < 3; ++a)
         {
@@ -549,8 +552,9 @@ __device__ __forceinline__ void mark_event(const GridDev &g, Tab &T, int ax0, in
 // QUEUED = false: one item per (face, axis); QUEUED = true: the queued row
 // chunks of large triangles (items index the queue's slots)
 template <bool MERGE, bool QUEUED>
-__global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, GridDev g, unsigned long long *counters,
-                                                    ShardBuf<Event> eb, ShardBuf<RowChunk> rq)
+__global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, BatchDev bt, uint64_t n_tiles,
+                                                    unsigned long long *counters, ShardBuf<Event> eb,
+                                                    ShardBuf<RowChunk> rq)
 {
     __shared__ unsigned long long sbase;
     __shared__ bool sfit;
@@ -562,9 +566,12 @@ __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, GridDev g, unsign
     __shared__ typename std::conditional<MERGE, MarkTable, NoMarkTable>::type T;
     __shared__ uint32_t qpre[QUEUED ? NSHARD : 1];
     const int tid = threadIdx.x;
-    int64_t n_items = 3 * m.F;
+    int64_t n_items = 0;
     if constexpr (QUEUED)
+    {
         n_items = (int64_t)queue_prefix(rq, qpre, sh);
+        n_tiles = (uint64_t)(n_items + BT - 1) / BT;
+    }
     uint32_t n_ev = 0;
     if constexpr (MERGE)
     {
@@ -576,13 +583,32 @@ __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, GridDev g, unsign
         __syncthreads();
     }
 
-    for (int64_t base = (int64_t)blockIdx.x * BT; base < n_items; base += (int64_t)gridDim.x * BT)
+    // a block takes tiles of BT items of one mesh (queued row chunks exist
+    // for single-mesh conversions only)
+    for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x)
     {
-        const int64_t item = base + tid;
+        // the tile's mesh; its grid is read in place (a local copy of the
+        // struct would go to scratch memory under runtime indexing)
+        const int ms = QUEUED ? 0 : seg_find(bt.mtile_off, bt.M, tile);
+        const GridDev &g = bt.seg[ms].g;
+        int64_t item;
+        bool live;
+        if constexpr (!QUEUED)
+        {
+            const int64_t f0 = bt.face_off[ms];
+            const int64_t loc = (int64_t)(tile - bt.mtile_off[ms]) * BT + tid;
+            item = 3 * f0 + loc;
+            live = loc < 3 * (bt.face_off[ms + 1] - f0);
+        }
+        else
+        {
+            item = (int64_t)tile * BT + tid;
+            live = item < n_items;
+        }
         uint32_t R = 0;
         if constexpr (!QUEUED)
         {
-            if (item < n_items)
+            if (live)
             {
                 const int64_t face = item / 3;
                 ScanItem &it = items[tid];
@@ -626,7 +652,7 @@ __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, GridDev g, unsign
         }
         else
         {
-            if (item < n_items)
+            if (live)
             {
                 const RowChunk ch = rq.items[queue_slot(rq, qpre, (uint32_t)item)];
                 ScanItem &it = items[tid];
@@ -937,8 +963,8 @@ __device__ __forceinline__ uint64_t row_mask(int lo_bit, int hi_bit)
 // QUEUED = false: one item per face; QUEUED = true: the queued row chunks of
 // large faces (items index the queue's slots)
 template <bool QUEUED>
-__global__ __launch_bounds__(BT) void k_face(MeshDev m, GridDev g, ShardBuf<FHit> hb, unsigned long long *ctr,
-                                             ShardBuf<FaceChunk> fq)
+__global__ __launch_bounds__(BT) void k_face(MeshDev m, BatchDev bt, uint64_t n_tiles, ShardBuf<FHit> hb,
+                                             unsigned long long *ctr, ShardBuf<FaceChunk> fq)
 {
     __shared__ FaceItem items[BT];
     __shared__ FaceRow rows[BT];
@@ -949,17 +975,38 @@ __global__ __launch_bounds__(BT) void k_face(MeshDev m, GridDev g, ShardBuf<FHit
     __shared__ uint32_t qpre[QUEUED ? NSHARD : 1];
     const int tid = threadIdx.x;
     uint32_t n_hit = 0;
-    int64_t n_items = m.F;
+    int64_t n_items = 0;
     if constexpr (QUEUED)
-        n_items = (int64_t)queue_prefix(fq, qpre, sh32);
-
-    for (int64_t base = (int64_t)blockIdx.x * BT; base < n_items; base += (int64_t)gridDim.x * BT)
     {
-        const int64_t item = base + tid;
+        n_items = (int64_t)queue_prefix(fq, qpre, sh32);
+        n_tiles = (uint64_t)(n_items + BT - 1) / BT;
+    }
+
+    // a block takes tiles of BT faces of one mesh (queued row chunks exist
+    // for single-mesh conversions only)
+    for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x)
+    {
+        // the tile's mesh; its grid is read in place
+        const int ms = QUEUED ? 0 : seg_find(bt.ftile_off, bt.M, tile);
+        const GridDev &g = bt.seg[ms].g;
+        int64_t item;
+        bool live;
+        if constexpr (!QUEUED)
+        {
+            const int64_t f0 = bt.face_off[ms];
+            const int64_t loc = (int64_t)(tile - bt.ftile_off[ms]) * BT + tid;
+            item = f0 + loc;
+            live = loc < bt.face_off[ms + 1] - f0;
+        }
+        else
+        {
+            item = (int64_t)tile * BT + tid;
+            live = item < n_items;
+        }
         uint64_t R = 0;
         if constexpr (!QUEUED)
         {
-            if (item < m.F)
+            if (live)
             {
                 FaceItem &it = items[tid];
                 R = face_item_setup(m, g, item, it);
@@ -994,7 +1041,7 @@ __global__ __launch_bounds__(BT) void k_face(MeshDev m, GridDev g, ShardBuf<FHit
         }
         else
         {
-            if (item < n_items)
+            if (live)
             {
                 const FaceChunk ch = fq.items[queue_slot(fq, qpre, (uint32_t)item)];
                 FaceItem &it = items[tid];
@@ -1240,8 +1287,8 @@ __global__ __launch_bounds__(BT) void k_bedge_prep(const float *verts, int64_t E
 
 // DDA over a boundary segment (FDG.cpp:455-522); one lane per segment.  Every
 // active voxel it passes is appended to `hb` as (edge rank, voxel).
-__global__ __launch_bounds__(BT) void k_bdry(int64_t E, const BEdge *be, GridDev g, ShardBuf<BHit> hb,
-                                             unsigned long long *counters)
+__global__ __launch_bounds__(BT) void k_bdry(int64_t E, const BEdge *be, const int2 *bedge_v, BatchDev bt,
+                                             ShardBuf<BHit> hb, unsigned long long *counters)
 {
     const int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
     uint32_t n_hit = 0;
@@ -1249,6 +1296,7 @@ __global__ __launch_bounds__(BT) void k_bdry(int64_t E, const BEdge *be, GridDev
     if (i < E && be[i].valid)
     {
         const BEdge &e = be[i];
+        const GridDev g = bt.seg[seg_find(bt.vert_off, bt.M, (int64_t)bedge_v[i].x)].g; // the edge's mesh
         int cur[3], step[3];
         double tMax[3], tDelta[3];
         double cap = 16.0;
@@ -1328,7 +1376,7 @@ __global__ __launch_bounds__(BT) void k_bdry(int64_t E, const BEdge *be, GridDev
 // record generation from the recorded crossings and boundary hits
 // ---------------------------------------------------------------------------
 // 4 phase-0 records per crossing, keys (face*3+axis, y, x, dx, dy)
-__global__ __launch_bounds__(BT) void k_gen_events(ShardBuf<Event> ev, GridDev g, BucketArgs ba)
+__global__ __launch_bounds__(BT) void k_gen_events(ShardBuf<Event> ev, BucketArgs ba)
 {
     __shared__ BucketLds<11> L;
     const uint64_t n = (uint64_t)NSHARD * ev.per;
@@ -1343,6 +1391,8 @@ __global__ __launch_bounds__(BT) void k_gen_events(ShardBuf<Event> ev, GridDev g
             const Event e = ev.items[i];
             const uint64_t fa = (e.key >> KEY_FA_SHIFT) & KEY_FA_MASK;
             const uint64_t face = fa / 3u;
+            const MeshSeg &sg = ba.b.seg[seg_find(ba.b.face_off, ba.b.M, (int64_t)face)];
+            const GridDev &g = sg.g;
             const int ax2 = (int)(fa - 3u * face);
             const int ax0 = (ax2 + 1) % 3;
             const int ax1 = (ax2 + 2) % 3;
@@ -1356,7 +1406,8 @@ __global__ __launch_bounds__(BT) void k_gen_events(ShardBuf<Event> ev, GridDev g
             for (int k = 0; k < 4; ++k)
             {
                 uint32_t lvox;
-                brick_of(g, ba.nb, c[k][0], c[k][1], c[k][2], bk[k], lvox);
+                brick_of(g, sg.nb, c[k][0], c[k][1], c[k][2], bk[k], lvox);
+                bk[k] += (uint32_t)sg.brick0;
                 rc[k] = Rec{e.key | (uint64_t)k, e.z, lvox};
                 ok[k] = true;
             }
@@ -1366,7 +1417,7 @@ __global__ __launch_bounds__(BT) void k_gen_events(ShardBuf<Event> ev, GridDev g
 }
 
 // phase-1 records from the face-test hits, key = face index
-__global__ __launch_bounds__(BT) void k_gen_fhits(ShardBuf<FHit> h, GridDev g, BucketArgs ba)
+__global__ __launch_bounds__(BT) void k_gen_fhits(ShardBuf<FHit> h, BucketArgs ba)
 {
     __shared__ BucketLds<11> L;
     const uint64_t n = (uint64_t)NSHARD * h.per;
@@ -1382,8 +1433,11 @@ __global__ __launch_bounds__(BT) void k_gen_fhits(ShardBuf<FHit> h, GridDev g, B
             if (i < n && shard_valid(h, i))
             {
                 const FHit x = h.items[i];
+                const MeshSeg &sg = ba.b.seg[seg_find(ba.b.face_off, ba.b.M, (int64_t)x.face)];
+                const GridDev &g = sg.g;
                 uint32_t lvox;
-                brick_of(g, ba.nb, g.gmin[0] + (int)x.lx, g.gmin[1] + (int)x.ly, g.gmin[2] + (int)x.lz, bk[k], lvox);
+                brick_of(g, sg.nb, g.gmin[0] + (int)x.lx, g.gmin[1] + (int)x.ly, g.gmin[2] + (int)x.lz, bk[k], lvox);
+                bk[k] += (uint32_t)sg.brick0;
                 rc[k] = Rec{(1ull << KEY_PHASE_SHIFT) | (uint64_t)x.face, 0.0f, lvox};
                 ok[k] = true;
             }
@@ -1393,7 +1447,7 @@ __global__ __launch_bounds__(BT) void k_gen_fhits(ShardBuf<FHit> h, GridDev g, B
 }
 
 // phase-2 records from the boundary DDA hits, key = boundary-edge rank
-__global__ __launch_bounds__(BT) void k_gen_bhits(ShardBuf<BHit> h, GridDev g, BucketArgs ba)
+__global__ __launch_bounds__(BT) void k_gen_bhits(ShardBuf<BHit> h, BucketArgs ba)
 {
     __shared__ BucketLds<9> L;
     const uint64_t n = (uint64_t)NSHARD * h.per;
@@ -1406,8 +1460,10 @@ __global__ __launch_bounds__(BT) void k_gen_bhits(ShardBuf<BHit> h, GridDev g, B
         if (i < n && shard_valid(h, i))
         {
             const BHit x = h.items[i];
+            const MeshSeg &sg = ba.b.seg[seg_find(ba.b.vert_off, ba.b.M, (int64_t)ba.bedge_v[x.edge].x)];
             uint32_t lvox;
-            brick_of(g, ba.nb, x.x, x.y, x.z, bk[0], lvox);
+            brick_of(sg.g, sg.nb, x.x, x.y, x.z, bk[0], lvox);
+            bk[0] += (uint32_t)sg.brick0;
             rc[0] = Rec{(2ull << KEY_PHASE_SHIFT) | (uint64_t)x.edge, 0.0f, lvox};
             ok[0] = true;
         }
@@ -1561,11 +1617,14 @@ __global__ __launch_bounds__(BT) void k_scan_down(Load ld, uint64_t n, const uin
 // ---------------------------------------------------------------------------
 // voxel coordinates from the bitset (ascending x + Gx*(y + Gy*z))
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BT) void k_emit_coords(GridDev g, uint64_t n_words, int32_t *coords)
+__global__ __launch_bounds__(BT) void k_emit_coords(BatchDev bt, uint64_t n_words, int32_t *coords)
 {
     const uint64_t stride = (uint64_t)gridDim.x * BT;
-    for (uint64_t w = (uint64_t)blockIdx.x * BT + threadIdx.x; w < n_words; w += stride)
+    for (uint64_t gw = (uint64_t)blockIdx.x * BT + threadIdx.x; gw < n_words; gw += stride)
     {
+        const int ms = seg_find(bt.word_off, bt.M, gw);
+        const GridDev &g = bt.seg[ms].g;
+        const uint64_t w = gw - bt.word_off[ms]; // word of the mesh
         uint64_t m = g.bits[w];
         if (!m)
             continue;
@@ -1585,6 +1644,17 @@ __global__ __launch_bounds__(BT) void k_emit_coords(GridDev g, uint64_t n_words,
             ++id;
         }
     }
+}
+
+// first voxel id of each mesh (the word offset at its first word)
+__global__ __launch_bounds__(BT) void k_seg_voxels(BatchDev bt, const uint32_t *woff, uint64_t n_words,
+                                                   const unsigned long long *n_voxels, uint64_t *vox_off)
+{
+    const int m = (int)(blockIdx.x * BT + threadIdx.x);
+    if (m > bt.M)
+        return;
+    const uint64_t w = bt.word_off[m];
+    vox_off[m] = w < n_words ? (uint64_t)woff[w] : (uint64_t)*n_voxels;
 }
 
 // Running replay of one voxel's contributions in the reference's serial order
@@ -2116,6 +2186,19 @@ __device__ __forceinline__ uint64_t row_vid(const FoldArgs &a, const FoldLds<CAP
     return (uint64_t)L.roff[row] + (uint64_t)__popcll(L.rbits[row] & ((1ull << bit) - 1ull));
 }
 
+// Sets the grid and brick grid of the mesh owning `brick` (a brick of the
+// batch's shared numbering) and returns the brick's index inside its mesh.
+__device__ __forceinline__ uint32_t enter_brick(FoldArgs &a, uint32_t brick)
+{
+    const MeshSeg &sg = a.b.seg[seg_find(a.b.brick_off, a.b.M, (uint64_t)brick)];
+    a.g = sg.g;
+    a.nb[0] = sg.nb[0];
+    a.nb[1] = sg.nb[1];
+    a.nb[2] = sg.nb[2];
+    return brick - (uint32_t)sg.brick0;
+}
+
+// min corner of a brick of the current mesh (lb = index inside the mesh)
 __device__ __forceinline__ void brick_origin(const FoldArgs &a, uint32_t brick, int &x0, int &y0, int &z0)
 {
     const int bx = (int)(brick % (uint32_t)a.nb[0]);
@@ -2131,17 +2214,19 @@ __device__ __forceinline__ void brick_origin(const FoldArgs &a, uint32_t brick, 
 // FOLD_SMALL records (the others are listed for k_fold_large): the records
 // are read once into LDS, grouped, replayed and solved there.
 template <int P>
-__global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs a)
+__global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs ain)
 {
     __shared__ FoldLds<FOLD_SMALL> L;
     const int tid = threadIdx.x;
-    const uint64_t n_active = *a.n_active;
+    const uint64_t n_active = *ain.n_active;
     for (uint64_t bi = blockIdx.x; bi < n_active; bi += gridDim.x)
     {
-        const uint32_t brick = a.active[bi];
+        const uint32_t brick = ain.active[bi];
+        FoldArgs a = ain;
+        const uint32_t lb = enter_brick(a, brick);
         const uint32_t c0 = a.choff[brick], c1 = a.choff[brick + 1];
         int x0, y0, z0;
-        brick_origin(a, brick, x0, y0, z0);
+        brick_origin(a, lb, x0, y0, z0);
         load_rows(a, L, x0, y0, z0);
         for (int i = tid; i < 512; i += BT)
             L.cur[i] = 0;
@@ -2196,17 +2281,19 @@ __global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs a)
 // are sized to the LDS tile, and a voxel with more records than the tile is
 // replayed in key-ordered batches selected by a binary search over the key.
 template <int P>
-__global__ __launch_bounds__(BT) void k_fold_large(FoldArgs a)
+__global__ __launch_bounds__(BT) void k_fold_large(FoldArgs ain)
 {
     __shared__ FoldLds<FOLD_CAP> L;
     const int tid = threadIdx.x;
-    const uint64_t n_large = *a.n_large;
+    const uint64_t n_large = *ain.n_large;
 
     for (uint64_t bi = blockIdx.x; bi < n_large; bi += gridDim.x)
     {
-        const uint32_t brick = a.large[bi];
+        const uint32_t brick = ain.large[bi];
+        FoldArgs a = ain;
+        const uint32_t lb = enter_brick(a, brick);
         int x0, y0, z0;
-        brick_origin(a, brick, x0, y0, z0);
+        brick_origin(a, lb, x0, y0, z0);
         load_rows(a, L, x0, y0, z0);
 
         // one pass over the brick's records: per-voxel histogram, and the
@@ -2395,7 +2482,7 @@ __global__ __launch_bounds__(BT) void k_fold_large(FoldArgs a)
 // queued for the box-constrained search (solved in place when its queue
 // region is full: the claim is all-or-nothing, so the queue holds only
 // complete items).
-__global__ __launch_bounds__(BT) void k_solve_voxels(const float *qef, const int32_t *coords, uint64_t n, GridDev g,
+__global__ __launch_bounds__(BT) void k_solve_voxels(const float *qef, const int32_t *coords, uint64_t n, BatchDev bt,
                                                      float *dual, ShardBuf<BoxItem> box)
 {
     const uint64_t stride = (uint64_t)gridDim.x * BT;
@@ -2409,16 +2496,21 @@ __global__ __launch_bounds__(BT) void k_solve_voxels(const float *qef, const int
         float q[10];
         int c[3] = {0, 0, 0};
         float d[3] = {0.0f, 0.0f, 0.0f};
+        float vs[3] = {1.0f, 1.0f, 1.0f};
         bool inside = true;
         if (live)
         {
+            const GridDev &g = bt.seg[seg_find(bt.vox_off, bt.M, vid)].g; // the voxel's mesh
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                vs[i] = g.vs[i];
 #pragma unroll
             for (int i = 0; i < 10; ++i)
                 q[i] = qef[10 * vid + i];
 #pragma unroll
             for (int i = 0; i < 3; ++i)
                 c[i] = coords[3 * vid + i];
-            inside = solve_free(q, c, g.vs, d);
+            inside = solve_free(q, c, vs, d);
         }
         const uint64_t m = __ballot(!inside);
         uint64_t base = ~0ull;
@@ -2451,7 +2543,7 @@ __global__ __launch_bounds__(BT) void k_solve_voxels(const float *qef, const int
                 e.vid = vid;
                 continue;
             }
-            solve_box(q, c, g.vs, d);
+            solve_box(q, c, vs, d);
         }
         float *o = dual + 3 * vid;
         o[0] = d[0];
@@ -2467,8 +2559,9 @@ __global__ __launch_bounds__(BT) void k_solve_voxels(const float *qef, const int
 // compensated (h + l) quadric, in world coordinates like the reference.  A
 // minimiser outside [0,1]^3 goes through the box search (FP32, voxel frame).
 // The voxels that took the FP64 path are counted.
-__global__ __launch_bounds__(BT) void k_solve_voxels_c(const float *qef, const int32_t *coords, uint64_t n, GridDev g,
-                                                       float *dual, float cond_max, unsigned long long *n64)
+__global__ __launch_bounds__(BT) void k_solve_voxels_c(const float *qef, const int32_t *coords, uint64_t n,
+                                                       BatchDev bt, float *dual, float cond_max,
+                                                       unsigned long long *n64)
 {
     const uint64_t stride = (uint64_t)gridDim.x * BT;
     const int lane = threadIdx.x & 63;
@@ -2480,6 +2573,7 @@ __global__ __launch_bounds__(BT) void k_solve_voxels_c(const float *qef, const i
         bool used64 = false;
         if (live)
         {
+            const GridDev &g = bt.seg[seg_find(bt.vox_off, bt.M, vid)].g; // the voxel's mesh
             float qh[10], ql[10];
 #pragma unroll
             for (int i = 0; i < 10; ++i)
@@ -2557,7 +2651,7 @@ __global__ __launch_bounds__(BT) void k_solve_voxels_c(const float *qef, const i
 }
 
 // box-constrained search for the queued voxels, one per lane
-__global__ __launch_bounds__(BT) void k_solve_box(ShardBuf<BoxItem> box, GridDev g, float *dual)
+__global__ __launch_bounds__(BT) void k_solve_box(ShardBuf<BoxItem> box, BatchDev bt, float *dual)
 {
     const uint64_t n = (uint64_t)NSHARD * box.per;
     const uint64_t stride = (uint64_t)gridDim.x * BT;
@@ -2566,6 +2660,7 @@ __global__ __launch_bounds__(BT) void k_solve_box(ShardBuf<BoxItem> box, GridDev
         if (!shard_valid(box, i))
             continue;
         const BoxItem e = box.items[i];
+        const GridDev &g = bt.seg[seg_find(bt.vox_off, bt.M, e.vid)].g; // the voxel's mesh
         float v[3] = {e.v[0], e.v[1], e.v[2]};
         solve_box(e.q, e.c, g.vs, v);
         float *o = dual + 3 * e.vid;
@@ -2588,17 +2683,17 @@ static inline unsigned grid_for(int64_t n, int64_t cap = 1 << 20)
     return (unsigned)b;
 }
 
-hipError_t launch_prep(const MeshDev &m, int64_t V, const GridDev &g, float4 *planes, unsigned long long *ctr,
-                       hipStream_t s)
+hipError_t launch_prep(const MeshDev &m, const BatchDev &b, float4 *planes, unsigned long long *ctr, hipStream_t s)
 {
-    const int64_t n = m.F > V ? m.F : V;
-    hipLaunchKernelGGL(k_prep, dim3(grid_for(n, 8192)), dim3(BT), 0, s, m, V, g, planes, ctr);
+    const int64_t n = m.F > (m.f ? m.V : 3 * m.F) ? m.F : (m.f ? m.V : 3 * m.F);
+    hipLaunchKernelGGL(k_prep, dim3(grid_for(n)), dim3(BT), 0, s, m, b, planes, ctr);
     return hipGetLastError();
 }
 
-hipError_t launch_scan_events(const MeshDev &m, const GridDev &g, unsigned long long *ctr, const ShardBuf<Event> &eb,
-                              const ShardBuf<RowChunk> &rq, bool merge, hipStream_t s)
+hipError_t launch_scan_events(const MeshDev &m, const BatchDev &b, uint64_t n_tiles, unsigned long long *ctr,
+                              const ShardBuf<Event> &eb, const ShardBuf<RowChunk> &rq, bool merge, hipStream_t s)
 {
+    const unsigned gm = (unsigned)(n_tiles < (1ull << 20) ? (n_tiles ? n_tiles : 1) : (1ull << 20));
     // row chunks of large triangles: queue emptied, filled by the first
     // launch, consumed by the second
     hipError_t e = hipMemsetAsync(rq.cnt, 0, (size_t)NSHARD * SHARD_STRIDE * sizeof(unsigned long long), s);
@@ -2607,48 +2702,49 @@ hipError_t launch_scan_events(const MeshDev &m, const GridDev &g, unsigned long 
     const unsigned gq = 2048; // consumers loop over the filled slots
     if (merge)
     {
-        hipLaunchKernelGGL((k_scan_events<true, false>), dim3(grid_for(3 * m.F)), dim3(BT), 0, s, m, g, ctr, eb, rq);
-        hipLaunchKernelGGL((k_scan_events<true, true>), dim3(gq), dim3(BT), 0, s, m, g, ctr, eb, rq);
+        hipLaunchKernelGGL((k_scan_events<true, false>), dim3(gm), dim3(BT), 0, s, m, b, n_tiles, ctr, eb, rq);
+        hipLaunchKernelGGL((k_scan_events<true, true>), dim3(gq), dim3(BT), 0, s, m, b, 0ull, ctr, eb, rq);
     }
     else
     {
-        hipLaunchKernelGGL((k_scan_events<false, false>), dim3(grid_for(3 * m.F)), dim3(BT), 0, s, m, g, ctr, eb, rq);
-        hipLaunchKernelGGL((k_scan_events<false, true>), dim3(gq), dim3(BT), 0, s, m, g, ctr, eb, rq);
+        hipLaunchKernelGGL((k_scan_events<false, false>), dim3(gm), dim3(BT), 0, s, m, b, n_tiles, ctr, eb, rq);
+        hipLaunchKernelGGL((k_scan_events<false, true>), dim3(gq), dim3(BT), 0, s, m, b, 0ull, ctr, eb, rq);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_face(const MeshDev &m, const GridDev &g, const ShardBuf<FHit> &hb, unsigned long long *ctr,
-                       const ShardBuf<FaceChunk> &fq, hipStream_t s)
+hipError_t launch_face(const MeshDev &m, const BatchDev &b, uint64_t n_tiles, const ShardBuf<FHit> &hb,
+                       unsigned long long *ctr, const ShardBuf<FaceChunk> &fq, hipStream_t s)
 {
+    const unsigned gf = (unsigned)(n_tiles < (1ull << 20) ? (n_tiles ? n_tiles : 1) : (1ull << 20));
     // row chunks of large faces: queue emptied, filled by the first launch,
     // consumed by the second
     hipError_t e = hipMemsetAsync(fq.cnt, 0, (size_t)NSHARD * SHARD_STRIDE * sizeof(unsigned long long), s);
     if (e != hipSuccess)
         return e;
-    hipLaunchKernelGGL(k_face<false>, dim3(grid_for(m.F)), dim3(BT), 0, s, m, g, hb, ctr, fq);
-    hipLaunchKernelGGL(k_face<true>, dim3(2048), dim3(BT), 0, s, m, g, hb, ctr, fq); // loops over the filled slots
+    hipLaunchKernelGGL(k_face<false>, dim3(gf), dim3(BT), 0, s, m, b, n_tiles, hb, ctr, fq);
+    hipLaunchKernelGGL(k_face<true>, dim3(2048), dim3(BT), 0, s, m, b, 0ull, hb, ctr, fq); // loops over the filled slots
     return hipGetLastError();
 }
 
-hipError_t launch_gen_fhits(const ShardBuf<FHit> &h, const GridDev &g, const BucketArgs &ba, hipStream_t s)
+hipError_t launch_gen_fhits(const ShardBuf<FHit> &h, const BucketArgs &ba, hipStream_t s)
 {
     const uint64_t n = (uint64_t)NSHARD * h.per;
-    hipLaunchKernelGGL(k_gen_fhits, dim3(grid_for((int64_t)((n + 3) / 4), 16384)), dim3(BT), 0, s, h, g, ba);
+    hipLaunchKernelGGL(k_gen_fhits, dim3(grid_for((int64_t)((n + 3) / 4), 16384)), dim3(BT), 0, s, h, ba);
     return hipGetLastError();
 }
 
-hipError_t launch_gen_events(const ShardBuf<Event> &ev, const GridDev &g, const BucketArgs &ba, hipStream_t s)
+hipError_t launch_gen_events(const ShardBuf<Event> &ev, const BucketArgs &ba, hipStream_t s)
 {
     const uint64_t n = (uint64_t)NSHARD * ev.per;
-    hipLaunchKernelGGL(k_gen_events, dim3(grid_for((int64_t)n, 16384)), dim3(BT), 0, s, ev, g, ba);
+    hipLaunchKernelGGL(k_gen_events, dim3(grid_for((int64_t)n, 16384)), dim3(BT), 0, s, ev, ba);
     return hipGetLastError();
 }
 
-hipError_t launch_gen_bhits(const ShardBuf<BHit> &h, const GridDev &g, const BucketArgs &ba, hipStream_t s)
+hipError_t launch_gen_bhits(const ShardBuf<BHit> &h, const BucketArgs &ba, hipStream_t s)
 {
     const uint64_t n = (uint64_t)NSHARD * h.per;
-    hipLaunchKernelGGL(k_gen_bhits, dim3(grid_for((int64_t)n, 4096)), dim3(BT), 0, s, h, g, ba);
+    hipLaunchKernelGGL(k_gen_bhits, dim3(grid_for((int64_t)n, 4096)), dim3(BT), 0, s, h, ba);
     return hipGetLastError();
 }
 
@@ -2692,18 +2788,18 @@ hipError_t launch_solve_voxels(const FoldArgs &a, const int32_t *coords, uint64_
     if (a.prec)
     {
         hipLaunchKernelGGL(k_solve_voxels_c, dim3(grid_for((int64_t)n, 65536)), dim3(BT), 0, s, (const float *)a.qef,
-                           coords, n, a.g, a.dual, a.cond_max, a.n64);
+                           coords, n, a.b, a.dual, a.cond_max, a.n64);
         return hipGetLastError();
     }
     hipLaunchKernelGGL(k_solve_voxels, dim3(grid_for((int64_t)n, 65536)), dim3(BT), 0, s, (const float *)a.qef,
-                       coords, n, a.g, a.dual, a.box);
+                       coords, n, a.b, a.dual, a.box);
     return hipGetLastError();
 }
 
 hipError_t launch_solve_box(const FoldArgs &a, hipStream_t s)
 {
     const uint64_t n = (uint64_t)NSHARD * a.box.per;
-    hipLaunchKernelGGL(k_solve_box, dim3(grid_for((int64_t)n, 16384)), dim3(BT), 0, s, a.box, a.g, a.dual);
+    hipLaunchKernelGGL(k_solve_box, dim3(grid_for((int64_t)n, 16384)), dim3(BT), 0, s, a.box, a.b, a.dual);
     return hipGetLastError();
 }
 
@@ -2740,12 +2836,12 @@ hipError_t launch_bedge_prep(const float *verts, int64_t E, const int2 *bedge_v,
     return hipGetLastError();
 }
 
-hipError_t launch_bdry(int64_t E, const BEdge *be, const GridDev &g, const ShardBuf<BHit> &hb,
+hipError_t launch_bdry(int64_t E, const BEdge *be, const int2 *bedge_v, const BatchDev &b, const ShardBuf<BHit> &hb,
                        unsigned long long *ctr, hipStream_t s)
 {
     if (E == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_bdry, dim3(grid_for(E)), dim3(BT), 0, s, E, be, g, hb, ctr);
+    hipLaunchKernelGGL(k_bdry, dim3(grid_for(E)), dim3(BT), 0, s, E, be, bedge_v, b, hb, ctr);
     return hipGetLastError();
 }
 
@@ -2787,9 +2883,17 @@ hipError_t launch_scan_u32(const uint32_t *in, uint64_t n, uint32_t *out, uint64
     return scan_impl(LoadU32{in}, n, out, tile_sums, total, s);
 }
 
-hipError_t launch_emit_coords(const GridDev &g, uint64_t n_words, int32_t *coords, hipStream_t s)
+hipError_t launch_emit_coords(const BatchDev &b, uint64_t n_words, int32_t *coords, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_emit_coords, dim3(grid_for((int64_t)n_words, 65536)), dim3(BT), 0, s, g, n_words, coords);
+    hipLaunchKernelGGL(k_emit_coords, dim3(grid_for((int64_t)n_words, 65536)), dim3(BT), 0, s, b, n_words, coords);
+    return hipGetLastError();
+}
+
+hipError_t launch_seg_voxels(const BatchDev &b, const uint32_t *woff, uint64_t n_words,
+                             const unsigned long long *n_voxels, uint64_t *vox_off, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_seg_voxels, dim3(grid_for((int64_t)b.M + 1)), dim3(BT), 0, s, b, woff, n_words, n_voxels,
+                       vox_off);
     return hipGetLastError();
 }
 

@@ -29,6 +29,7 @@ __all__ = [
     "intersect_occ_cpu",
     "library_path",
     "last_stats",
+    "mesh_to_flexible_dual_grid_batch_cuda",
     "stage_times",
     "set_timing",
     "kernel_times",
@@ -77,6 +78,10 @@ def _load():
         L.ovx_fdg_begin.argtypes = [vp, vp, i64, vp, i64, ctypes.POINTER(_Grid), f32, f32, f32, vp,
                                     ctypes.POINTER(i64)]
         L.ovx_fdg_finish.argtypes = [vp, vp, vp, vp, vp]
+        L.ovx_fdg_batch_begin.argtypes = [vp, vp, i64, vp, i64, ctypes.c_int, ctypes.POINTER(i64),
+                                          ctypes.POINTER(i64), ctypes.POINTER(_Grid), f32, f32, f32, vp,
+                                          ctypes.POINTER(i64)]
+        L.ovx_fdg_batch_begin.restype = ctypes.c_int
         L.ovx_occ_begin.argtypes = [vp, vp, i64, ctypes.POINTER(_Grid), vp, ctypes.POINTER(i64)]
         L.ovx_occ_finish.argtypes = [vp, vp, vp]
         L.ovx_fdg_stats.argtypes = [vp, ctypes.POINTER(i64), ctypes.c_int]
@@ -216,6 +221,72 @@ def mesh_to_flexible_dual_grid_cuda(
         _check(L.ovx_fdg_finish(ctx.h, coords.data_ptr(), dual.data_ptr(), inter.data_ptr(), stream),
                "mesh_to_flexible_dual_grid")
     return coords, dual, inter
+
+
+def mesh_to_flexible_dual_grid_batch_cuda(
+    vertices: List[torch.Tensor],
+    faces: List[torch.Tensor],
+    voxel_sizes: List[List[float]],
+    grid_ranges: List[List[int]],
+    face_weight: float,
+    boundary_weight: float,
+    regularization_weight: float,
+    precision: str = "reference",
+    cond_max: float = 1.0e6,
+) -> List[Tuple[torch.Tensor, torch.Tensor, torch.Tensor]]:
+    """Batched device flexible dual grid: M meshes (grid-local f32 [V_m,3]
+    vertices, i32 [F_m,3] faces with per-mesh indices, each with its own grid)
+    converted together, every pipeline stage one launch over all of them.  No
+    reference counterpart (the reference converts one mesh per call,
+    src/ext.cpp:19); each mesh's outputs equal those of
+    mesh_to_flexible_dual_grid_cuda on it alone.  Returns one (coords, dual,
+    intersected) tuple per mesh (views of shared output tensors)."""
+    mode = _PRECISIONS.get(precision)
+    if mode is None:
+        raise RuntimeError(f"precision must be one of {sorted(_PRECISIONS)}, got {precision!r}")
+    M = len(vertices)
+    if M == 0:
+        return []
+    if len(faces) != M or len(voxel_sizes) != M or len(grid_ranges) != M:
+        raise RuntimeError("vertices, faces, voxel_sizes and grid_ranges must have one entry per mesh")
+    dev = vertices[0].device
+    for k in range(M):
+        _require_device_tensor(vertices[k], f"vertices[{k}]", torch.float32, 2)
+        _require_device_tensor(faces[k], f"faces[{k}]", torch.int32, 2)
+        if vertices[k].size(1) != 3 or faces[k].size(1) != 3:
+            raise RuntimeError(f"mesh {k}: vertices and faces must have 3 columns")
+        if vertices[k].device != dev or faces[k].device != dev:
+            raise RuntimeError("all meshes of a batch must be on the same device")
+    nv = [int(v.size(0)) for v in vertices]
+    nf = [int(f.size(0)) for f in faces]
+    vo = (ctypes.c_int64 * (M + 1))()
+    fo = (ctypes.c_int64 * (M + 1))()
+    for k in range(M):
+        vo[k + 1] = vo[k] + nv[k]
+        fo[k + 1] = fo[k] + nf[k]
+    # one vertex array and one face array with batch-global indices
+    v_all = torch.cat([v.contiguous() for v in vertices], 0) if M > 1 else vertices[0].contiguous()
+    f_all = torch.cat([f + int(vo[k]) if vo[k] else f for k, f in enumerate(faces)], 0).to(torch.int32).contiguous()
+    grids = (_Grid * M)()
+    for k in range(M):
+        grids[k] = _grid(voxel_sizes[k], grid_ranges[k])
+    ctx = _ctx(dev)
+    L = _load()
+    counts = (ctypes.c_int64 * M)()
+    with torch.cuda.device(dev), ctx.lock:
+        stream = _stream_handle(dev)
+        _check(L.ovx_fdg_set_precision(ctx.h, mode, float(cond_max)), "mesh_to_flexible_dual_grid_batch")
+        _check(L.ovx_fdg_batch_begin(ctx.h, v_all.data_ptr(), int(vo[M]), f_all.data_ptr(), int(fo[M]), M, fo, vo,
+                                     grids, float(face_weight), float(boundary_weight),
+                                     float(regularization_weight), stream, counts), "mesh_to_flexible_dual_grid_batch")
+        n = [int(counts[k]) for k in range(M)]
+        N = sum(n)
+        coords = torch.empty((N, 3), dtype=torch.int32, device=dev)
+        dual = torch.empty((N, 3), dtype=torch.float32, device=dev)
+        inter = torch.empty((N, 3), dtype=torch.bool, device=dev)
+        _check(L.ovx_fdg_finish(ctx.h, coords.data_ptr(), dual.data_ptr(), inter.data_ptr(), stream),
+               "mesh_to_flexible_dual_grid_batch")
+    return list(zip(coords.split(n), dual.split(n), inter.split(n)))
 
 
 def intersect_occ_cuda(triangles: torch.Tensor, voxel_size: List[float], grid_range: List[int]) -> torch.Tensor:

@@ -18,6 +18,7 @@ ArrayLike = Union[List[Any], Tuple[Any, ...], np.ndarray, torch.Tensor]
 
 __all__ = [
     "mesh_to_flexible_dual_grid",
+    "mesh_to_flexible_dual_grid_batch",
     "intersect_occ",
 ]
 
@@ -157,6 +158,66 @@ def mesh_to_flexible_dual_grid(
                                                   boundary_weight, regularization_weight, precision=precision)
     return _C.mesh_to_flexible_dual_grid_cpu(vertices, faces, voxel_size_arg, grid_range_arg, face_weight,
                                              boundary_weight, regularization_weight, timing, precision=precision)
+
+
+@torch.no_grad()
+def mesh_to_flexible_dual_grid_batch(
+    meshes,
+    voxel_size: Optional[FloatArrayLike] = None,
+    grid_size: Optional[IntArrayLike] = None,
+    aabb: Optional[ArrayLike] = None,
+    face_weight: float = 1.0,
+    boundary_weight: float = 1.0,
+    regularization_weight: float = 0.1,
+    precision: str = "reference",
+    voxel_sizes=None,
+    grid_sizes=None,
+    aabbs=None,
+) -> List[Tuple[torch.Tensor, torch.Tensor, torch.Tensor]]:
+    """Convert many meshes at once (an addition to the reference API, whose
+    ``mesh_to_flexible_dual_grid`` converts one mesh per call).
+
+    ``meshes`` is a sequence of ``(vertices, faces)`` on one device.
+    ``voxel_size`` / ``grid_size`` / ``aabb`` apply to every mesh; the plural
+    ``voxel_sizes`` / ``grid_sizes`` / ``aabbs`` give one value per mesh
+    instead.  Each mesh's grid is set up exactly as by
+    ``mesh_to_flexible_dual_grid`` and each returned ``(coords, dual_vertices,
+    intersected)`` equals that function's result for the mesh alone; the
+    conversion runs every pipeline stage once over all meshes.
+    """
+    meshes = list(meshes)
+    M = len(meshes)
+
+    def pick(single, plural, k, name):
+        if plural is not None:
+            assert len(plural) == M, f"{name}s must have one entry per mesh"
+            return plural[k]
+        return single
+
+    vs_list, gr_list, v_list, f_list = [], [], [], []
+    device = None
+    for k, (vertices, faces) in enumerate(meshes):
+        _check_mesh(vertices, faces)
+        device = vertices.device if device is None else device
+        assert vertices.device == device, "all meshes must be on the same device"
+        vertices = vertices.to(device=device, dtype=torch.float32).contiguous()
+        faces = faces.to(device=device, dtype=torch.int32).contiguous()
+        vs_k, gr_k, aabb_dev = _grid_setup(vertices, pick(voxel_size, voxel_sizes, k, "voxel_size"),
+                                           pick(grid_size, grid_sizes, k, "grid_size"), pick(aabb, aabbs, k, "aabb"))
+        v_list.append(vertices - aabb_dev[0].reshape(1, 3))
+        f_list.append(faces)
+        vs_list.append(vs_k)
+        gr_list.append(gr_k)
+    if M == 0:
+        return []
+    if device.type == "cuda":
+        return _C.mesh_to_flexible_dual_grid_batch_cuda(v_list, f_list, vs_list, gr_list, face_weight,
+                                                        boundary_weight, regularization_weight, precision=precision)
+    dev = _C._gpu_for_host_call()
+    out = _C.mesh_to_flexible_dual_grid_batch_cuda([v.to(dev) for v in v_list], [f.to(dev) for f in f_list], vs_list,
+                                                   gr_list, face_weight, boundary_weight, regularization_weight,
+                                                   precision=precision)
+    return [tuple(t.cpu() for t in o) for o in out]
 
 
 @torch.no_grad()

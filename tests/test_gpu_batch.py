@@ -1,0 +1,98 @@
+"""GPU: batched conversion (many meshes of mixed size and grid in one pass)
+equals converting each mesh alone, bit for bit; and the batch of one is the
+plain path.  The oracle checks one batch member end to end."""
+
+import numpy as np
+import pytest
+import torch
+
+import meshgen
+
+pytestmark = pytest.mark.gpu
+
+AABB = [[-0.5, -0.5, -0.5], [0.5, 0.5, 0.5]]
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch.device("cuda", 0)
+
+
+def _meshes(n, seed, f_hi=20000):
+    out = meshgen.batch(n_meshes=n, seed=seed, f_lo=200, f_hi=f_hi, grids=(32, 64, 128, 256))
+    return [(v, f.astype(np.int64), g) for v, f, g in out]
+
+
+def _single(v, f, g, dev, **kw):
+    import o_voxel
+
+    vt = torch.from_numpy(v).to(dev)
+    ft = torch.from_numpy(f).to(dev)
+    return [t.cpu().numpy() for t in o_voxel.convert.mesh_to_flexible_dual_grid(
+        vt, ft, grid_size=g, aabb=AABB, face_weight=1.0, boundary_weight=0.2, regularization_weight=1e-2, **kw)]
+
+
+def _batch(meshes, dev, **kw):
+    import o_voxel
+
+    ms = [(torch.from_numpy(v).to(dev), torch.from_numpy(f).to(dev)) for v, f, _ in meshes]
+    out = o_voxel.convert.mesh_to_flexible_dual_grid_batch(
+        ms, grid_sizes=[g for _, _, g in meshes], aabb=AABB, face_weight=1.0, boundary_weight=0.2,
+        regularization_weight=1e-2, **kw)
+    torch.cuda.synchronize()
+    return [[t.cpu().numpy() for t in o] for o in out]
+
+
+@pytest.mark.parametrize("precision", ["reference", "compensated"])
+def test_batch_equals_single_calls(precision):
+    dev = _dev()
+    meshes = _meshes(12, seed=7)
+    # an empty mesh and a mesh entirely outside its grid inside the batch
+    meshes.insert(3, (np.zeros((0, 3), np.float32), np.zeros((0, 3), np.int64), 64))
+    v, f = meshgen.icosphere(2)
+    meshes.insert(7, ((v * 0.1 + 2.0).astype(np.float32), f.astype(np.int64), 64))
+    print(f"[case] batch of {len(meshes)} meshes, precision={precision}, "
+          f"faces={[len(f) for _, f, _ in meshes]}", flush=True)
+    got = _batch(meshes, dev, precision=precision)
+    assert len(got) == len(meshes)
+    for k, (v, f, g) in enumerate(meshes):
+        want = _single(v, f, g, dev, precision=precision)
+        for a, b, name in zip(got[k], want, ("coords", "dual", "intersected")):
+            assert a.shape == b.shape and np.array_equal(a, b), f"mesh {k} ({len(f)} faces, grid {g}): {name} differs"
+
+
+def test_batch_member_matches_oracle(oracle):
+    dev = _dev()
+    meshes = _meshes(6, seed=11, f_hi=8000)
+    got = _batch(meshes, dev)
+    for k in (0, len(meshes) - 1):
+        v, f, g = meshes[k]
+        vs, gr, a0 = meshgen.grid_args(g)
+        ref = oracle.flexible_dual_grid((v - a0).astype(np.float32), f.astype(np.int32), vs, gr, 1.0, 0.2, 1e-2)
+        perm = oracle.canonical_order(ref.coords, gr)
+        c, d, i = got[k]
+        assert np.array_equal(c, ref.coords[perm]) and np.array_equal(i, ref.flags[perm])
+        assert np.array_equal(d, ref.dual[perm])
+
+
+def test_batch_of_one_is_plain_path():
+    dev = _dev()
+    (v, f, g), = _meshes(1, seed=3)
+    got = _batch([(v, f, g)], dev)[0]
+    want = _single(v, f, g, dev)
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)
+
+
+def test_batch_invalid_face_index_raises():
+    """An index that names a vertex of another mesh of the batch is an input error."""
+    dev = _dev()
+    meshes = _meshes(3, seed=5, f_hi=2000)
+    v, f, g = meshes[1]
+    f = f.copy()
+    f[0, 0] = len(v)  # one past this mesh's vertices: a vertex of the next mesh in the batch
+    meshes[1] = (v, f, g)
+    with pytest.raises(RuntimeError, match="vertex index"):
+        _batch(meshes, dev)
+    _batch(_meshes(2, seed=6, f_hi=2000), dev)  # the context keeps working
