@@ -386,6 +386,11 @@ def run_batch(args, torch, o_voxel, _C, dev):
         lat_all += lat
     seq_ms = float(np.median(seq_ms))
     same = all(all(torch.equal(a, b) for a, b in zip(ob, os_)) for ob, os_ in zip(out_b, out_s))
+    # per-kernel device times of one more batched step
+    _C.set_timing(dev, True)
+    batched()
+    kt = _C.kernel_times(dev)
+    _C.set_timing(dev, False)
     st = _C.last_stats(dev)
     line = {
         "metric": METRIC,
@@ -415,6 +420,7 @@ def run_batch(args, torch, o_voxel, _C, dev):
             "outputs_identical_to_sequential": bool(same),
         },
         "cold_ms": round(cold_ms, 3),
+        "kernels_ms": {k: round(x, 4) for k, x in sorted(kt.items(), key=lambda kv: -kv[1]) if x >= 0},
         "counts": {k: v for k, v in st.items() if k != "workspace_bytes"},
     }
     print(json.dumps(line), flush=True)

@@ -167,45 +167,39 @@ __device__ __forceinline__ bool shard_claim_fit(const ShardBuf<T> &b, uint32_t s
     return false;
 }
 
-// Whether global slot i of a sharded buffer holds an item.  A region whose
-// counter passed its capacity had a claim dropped and is skipped whole (its
-// slots below the capacity may be unwritten); the host re-runs the producer
-// with larger regions in that case.
-template <typename T>
-__device__ __forceinline__ bool shard_valid(const ShardBuf<T> &b, uint64_t i)
-{
-    const uint64_t sh = i / b.per;
-    const uint64_t local = i - sh * b.per;
-    if (sh >= (uint64_t)NSHARD)
-        return false;
-    const unsigned long long n = b.cnt[sh * SHARD_STRIDE];
-    return n <= b.per && local < n;
-}
-
-// Dense numbering of the filled slots of a queue whose claims are
-// all-or-nothing (every slot below a region's counter holds an item): the
-// exclusive prefix of the regions' fill counts, one region per thread.
+// Dense numbering of the items of a sharded buffer: the exclusive prefix
+// of the regions' fill counts, one region per thread, so that consumers walk
+// the items rather than the capacity.  A region whose counter passed its
+// capacity had a claim dropped and counts as empty (the host re-runs the
+// producer with larger regions then; all-or-nothing queues never overflow).
 // Contains barriers.
 template <typename T>
-__device__ __forceinline__ uint32_t queue_prefix(const ShardBuf<T> &b, uint32_t *pre, uint32_t *sh)
+__device__ __forceinline__ uint64_t queue_prefix(const ShardBuf<T> &b, uint64_t *pre, uint64_t *sh)
 {
     static_assert(NSHARD == BT, "one region per thread");
     const unsigned long long c = b.cnt[(uint64_t)threadIdx.x * SHARD_STRIDE];
-    const uint32_t n = (uint32_t)(c < b.per ? c : b.per);
-    uint32_t tot;
-    const uint32_t ex = block_excl_scan(n, sh, tot);
+    const uint64_t n = c <= b.per ? (uint64_t)c : 0ull;
+    uint64_t tot;
+    const uint64_t ex = block_excl_scan(n, sh, tot);
     pre[threadIdx.x] = ex + n; // inclusive
     __syncthreads();
     return tot;
 }
 
-// slot of the v-th filled item (v < total)
+// slot of the v-th item (v < total)
 template <typename T>
-__device__ __forceinline__ uint64_t queue_slot(const ShardBuf<T> &b, const uint32_t *pre, uint32_t v)
+__device__ __forceinline__ uint64_t queue_slot(const ShardBuf<T> &b, const uint64_t *pre, uint64_t v)
 {
     const int s = upper_find(pre, NSHARD, v);
-    return (uint64_t)s * b.per + (v - (s ? pre[s - 1] : 0u));
+    return (uint64_t)s * b.per + (v - (s ? pre[s - 1] : 0ull));
 }
+
+// dense numbering in LDS (one block's copy)
+struct QueueMap
+{
+    uint64_t pre[NSHARD];
+    uint64_t sh[NWAVE];
+};
 
 // LDS table of one block's records grouped by brick
 template <int LOG2>
@@ -564,12 +558,12 @@ __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, BatchDev bt, uint
     __shared__ uint32_t incl_rows[BT];
     __shared__ uint32_t sh[NWAVE];
     __shared__ typename std::conditional<MERGE, MarkTable, NoMarkTable>::type T;
-    __shared__ uint32_t qpre[QUEUED ? NSHARD : 1];
+    __shared__ QueueMap qm;
     const int tid = threadIdx.x;
     int64_t n_items = 0;
     if constexpr (QUEUED)
     {
-        n_items = (int64_t)queue_prefix(rq, qpre, sh);
+        n_items = (int64_t)queue_prefix(rq, qm.pre, qm.sh);
         n_tiles = (uint64_t)(n_items + BT - 1) / BT;
     }
     uint32_t n_ev = 0;
@@ -654,7 +648,7 @@ __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, BatchDev bt, uint
         {
             if (live)
             {
-                const RowChunk ch = rq.items[queue_slot(rq, qpre, (uint32_t)item)];
+                const RowChunk ch = rq.items[queue_slot(rq, qm.pre, (uint64_t)item)];
                 ScanItem &it = items[tid];
                 scan_item_setup(m, g, (int64_t)ch.face, ch.ax2, it);
                 it.start = ch.r0;
@@ -972,13 +966,13 @@ __global__ __launch_bounds__(BT) void k_face(MeshDev m, BatchDev bt, uint64_t n_
     __shared__ uint32_t incl_rows[BT];
     __shared__ uint64_t sh64[NWAVE];
     __shared__ uint32_t sh32[NWAVE];
-    __shared__ uint32_t qpre[QUEUED ? NSHARD : 1];
+    __shared__ QueueMap qm;
     const int tid = threadIdx.x;
     uint32_t n_hit = 0;
     int64_t n_items = 0;
     if constexpr (QUEUED)
     {
-        n_items = (int64_t)queue_prefix(fq, qpre, sh32);
+        n_items = (int64_t)queue_prefix(fq, qm.pre, qm.sh);
         n_tiles = (uint64_t)(n_items + BT - 1) / BT;
     }
 
@@ -1043,7 +1037,7 @@ __global__ __launch_bounds__(BT) void k_face(MeshDev m, BatchDev bt, uint64_t n_
         {
             if (live)
             {
-                const FaceChunk ch = fq.items[queue_slot(fq, qpre, (uint32_t)item)];
+                const FaceChunk ch = fq.items[queue_slot(fq, qm.pre, (uint64_t)item)];
                 FaceItem &it = items[tid];
                 face_item_setup(m, g, (int64_t)ch.face, it);
                 it.r0 = ch.r0;
@@ -1379,16 +1373,17 @@ __global__ __launch_bounds__(BT) void k_bdry(int64_t E, const BEdge *be, const i
 __global__ __launch_bounds__(BT) void k_gen_events(ShardBuf<Event> ev, BucketArgs ba)
 {
     __shared__ BucketLds<11> L;
-    const uint64_t n = (uint64_t)NSHARD * ev.per;
+    __shared__ QueueMap qm;
+    const uint64_t n = queue_prefix(ev, qm.pre, qm.sh);
     for (uint64_t base = (uint64_t)blockIdx.x * BT; base < n; base += (uint64_t)gridDim.x * BT)
     {
         const uint64_t i = base + threadIdx.x;
         uint32_t bk[4] = {0, 0, 0, 0};
         Rec rc[4];
         bool ok[4] = {false, false, false, false};
-        if (i < n && shard_valid(ev, i))
+        if (i < n)
         {
-            const Event e = ev.items[i];
+            const Event e = ev.items[queue_slot(ev, qm.pre, i)];
             const uint64_t fa = (e.key >> KEY_FA_SHIFT) & KEY_FA_MASK;
             const uint64_t face = fa / 3u;
             const MeshSeg &sg = ba.b.seg[seg_find(ba.b.face_off, ba.b.M, (int64_t)face)];
@@ -1420,7 +1415,8 @@ __global__ __launch_bounds__(BT) void k_gen_events(ShardBuf<Event> ev, BucketArg
 __global__ __launch_bounds__(BT) void k_gen_fhits(ShardBuf<FHit> h, BucketArgs ba)
 {
     __shared__ BucketLds<11> L;
-    const uint64_t n = (uint64_t)NSHARD * h.per;
+    __shared__ QueueMap qm;
+    const uint64_t n = queue_prefix(h, qm.pre, qm.sh);
     for (uint64_t base = (uint64_t)blockIdx.x * (4 * BT); base < n; base += (uint64_t)gridDim.x * (4 * BT))
     {
         uint32_t bk[4] = {0, 0, 0, 0};
@@ -1430,9 +1426,9 @@ __global__ __launch_bounds__(BT) void k_gen_fhits(ShardBuf<FHit> h, BucketArgs b
         for (int k = 0; k < 4; ++k)
         {
             const uint64_t i = base + (uint64_t)k * BT + threadIdx.x;
-            if (i < n && shard_valid(h, i))
+            if (i < n)
             {
-                const FHit x = h.items[i];
+                const FHit x = h.items[queue_slot(h, qm.pre, i)];
                 const MeshSeg &sg = ba.b.seg[seg_find(ba.b.face_off, ba.b.M, (int64_t)x.face)];
                 const GridDev &g = sg.g;
                 uint32_t lvox;
@@ -1450,16 +1446,17 @@ __global__ __launch_bounds__(BT) void k_gen_fhits(ShardBuf<FHit> h, BucketArgs b
 __global__ __launch_bounds__(BT) void k_gen_bhits(ShardBuf<BHit> h, BucketArgs ba)
 {
     __shared__ BucketLds<9> L;
-    const uint64_t n = (uint64_t)NSHARD * h.per;
+    __shared__ QueueMap qm;
+    const uint64_t n = queue_prefix(h, qm.pre, qm.sh);
     for (uint64_t base = (uint64_t)blockIdx.x * BT; base < n; base += (uint64_t)gridDim.x * BT)
     {
         const uint64_t i = base + threadIdx.x;
         uint32_t bk[1] = {0};
         Rec rc[1];
         bool ok[1] = {false};
-        if (i < n && shard_valid(h, i))
+        if (i < n)
         {
-            const BHit x = h.items[i];
+            const BHit x = h.items[queue_slot(h, qm.pre, i)];
             const MeshSeg &sg = ba.b.seg[seg_find(ba.b.vert_off, ba.b.M, (int64_t)ba.bedge_v[x.edge].x)];
             uint32_t lvox;
             brick_of(sg.g, sg.nb, x.x, x.y, x.z, bk[0], lvox);
@@ -1504,13 +1501,12 @@ __global__ __launch_bounds__(BT) void k_shard_reduce(const unsigned long long *s
 // chunks grouped by brick as (first record, count): position = first chunk of the brick + rank
 __global__ __launch_bounds__(BT) void k_place_chunks(ShardBuf<Chunk> chunks, const uint32_t *choff, uint2 *gchunks)
 {
-    const uint64_t n = (uint64_t)NSHARD * chunks.per;
+    __shared__ QueueMap qm;
+    const uint64_t n = queue_prefix(chunks, qm.pre, qm.sh);
     const uint64_t stride = (uint64_t)gridDim.x * BT;
-    for (uint64_t c = (uint64_t)blockIdx.x * BT + threadIdx.x; c < n; c += stride)
+    for (uint64_t v = (uint64_t)blockIdx.x * BT + threadIdx.x; v < n; v += stride)
     {
-        if (!shard_valid(chunks, c))
-            continue;
-        const Chunk ch = chunks.items[c];
+        const Chunk ch = chunks.items[queue_slot(chunks, qm.pre, v)];
         gchunks[choff[ch.brick] + ch.rank] = make_uint2(ch.start, ch.count);
     }
 }
@@ -2653,13 +2649,12 @@ __global__ __launch_bounds__(BT) void k_solve_voxels_c(const float *qef, const i
 // box-constrained search for the queued voxels, one per lane
 __global__ __launch_bounds__(BT) void k_solve_box(ShardBuf<BoxItem> box, BatchDev bt, float *dual)
 {
-    const uint64_t n = (uint64_t)NSHARD * box.per;
+    __shared__ QueueMap qm;
+    const uint64_t n = queue_prefix(box, qm.pre, qm.sh);
     const uint64_t stride = (uint64_t)gridDim.x * BT;
-    for (uint64_t i = (uint64_t)blockIdx.x * BT + threadIdx.x; i < n; i += stride)
+    for (uint64_t qi = (uint64_t)blockIdx.x * BT + threadIdx.x; qi < n; qi += stride)
     {
-        if (!shard_valid(box, i))
-            continue;
-        const BoxItem e = box.items[i];
+        const BoxItem e = box.items[queue_slot(box, qm.pre, qi)];
         const GridDev &g = bt.seg[seg_find(bt.vox_off, bt.M, e.vid)].g; // the voxel's mesh
         float v[3] = {e.v[0], e.v[1], e.v[2]};
         solve_box(e.q, e.c, g.vs, v);
@@ -2729,22 +2724,19 @@ hipError_t launch_face(const MeshDev &m, const BatchDev &b, uint64_t n_tiles, co
 
 hipError_t launch_gen_fhits(const ShardBuf<FHit> &h, const BucketArgs &ba, hipStream_t s)
 {
-    const uint64_t n = (uint64_t)NSHARD * h.per;
-    hipLaunchKernelGGL(k_gen_fhits, dim3(grid_for((int64_t)((n + 3) / 4), 16384)), dim3(BT), 0, s, h, ba);
+    hipLaunchKernelGGL(k_gen_fhits, dim3(4096), dim3(BT), 0, s, h, ba); // loops over the hits
     return hipGetLastError();
 }
 
 hipError_t launch_gen_events(const ShardBuf<Event> &ev, const BucketArgs &ba, hipStream_t s)
 {
-    const uint64_t n = (uint64_t)NSHARD * ev.per;
-    hipLaunchKernelGGL(k_gen_events, dim3(grid_for((int64_t)n, 16384)), dim3(BT), 0, s, ev, ba);
+    hipLaunchKernelGGL(k_gen_events, dim3(4096), dim3(BT), 0, s, ev, ba); // loops over the crossings
     return hipGetLastError();
 }
 
 hipError_t launch_gen_bhits(const ShardBuf<BHit> &h, const BucketArgs &ba, hipStream_t s)
 {
-    const uint64_t n = (uint64_t)NSHARD * h.per;
-    hipLaunchKernelGGL(k_gen_bhits, dim3(grid_for((int64_t)n, 4096)), dim3(BT), 0, s, h, ba);
+    hipLaunchKernelGGL(k_gen_bhits, dim3(1024), dim3(BT), 0, s, h, ba); // loops over the hits
     return hipGetLastError();
 }
 
@@ -2758,8 +2750,7 @@ hipError_t launch_shard_reduce(unsigned long long *shard_ctr, unsigned long long
 hipError_t launch_place_chunks(const ShardBuf<Chunk> &chunks, const uint32_t *choff, uint2 *gchunks,
                                hipStream_t s)
 {
-    const uint64_t n = (uint64_t)NSHARD * chunks.per;
-    hipLaunchKernelGGL(k_place_chunks, dim3(grid_for((int64_t)n, 16384)), dim3(BT), 0, s, chunks, choff, gchunks);
+    hipLaunchKernelGGL(k_place_chunks, dim3(4096), dim3(BT), 0, s, chunks, choff, gchunks); // loops over the chunks
     return hipGetLastError();
 }
 
@@ -2798,8 +2789,7 @@ hipError_t launch_solve_voxels(const FoldArgs &a, const int32_t *coords, uint64_
 
 hipError_t launch_solve_box(const FoldArgs &a, hipStream_t s)
 {
-    const uint64_t n = (uint64_t)NSHARD * a.box.per;
-    hipLaunchKernelGGL(k_solve_box, dim3(grid_for((int64_t)n, 16384)), dim3(BT), 0, s, a.box, a.b, a.dual);
+    hipLaunchKernelGGL(k_solve_box, dim3(2048), dim3(BT), 0, s, a.box, a.b, a.dual); // loops over the queue
     return hipGetLastError();
 }
 

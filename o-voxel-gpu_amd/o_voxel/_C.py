@@ -30,6 +30,7 @@ __all__ = [
     "library_path",
     "last_stats",
     "mesh_to_flexible_dual_grid_batch_cuda",
+    "mesh_to_flexible_dual_grid_batch_packed_cuda",
     "stage_times",
     "set_timing",
     "kernel_times",
@@ -223,6 +224,72 @@ def mesh_to_flexible_dual_grid_cuda(
     return coords, dual, inter
 
 
+def mesh_to_flexible_dual_grid_batch_packed_cuda(
+    vertices: torch.Tensor,
+    faces: torch.Tensor,
+    vertex_counts: List[int],
+    face_counts: List[int],
+    voxel_sizes: List[List[float]],
+    grid_ranges: List[List[int]],
+    face_weight: float,
+    boundary_weight: float,
+    regularization_weight: float,
+    precision: str = "reference",
+    cond_max: float = 1.0e6,
+) -> List[Tuple[torch.Tensor, torch.Tensor, torch.Tensor]]:
+    """Batched device flexible dual grid on packed inputs: the meshes'
+    grid-local f32 vertices one after another [sum V_m, 3] and their i32
+    faces [sum F_m, 3] with batch-global vertex indices, plus per-mesh counts
+    and grids.  No reference counterpart (the reference converts one mesh per
+    call, src/ext.cpp:19); each mesh's outputs equal those of
+    mesh_to_flexible_dual_grid_cuda on it alone.  Returns one (coords, dual,
+    intersected) tuple per mesh (views of shared output tensors)."""
+    mode = _PRECISIONS.get(precision)
+    if mode is None:
+        raise RuntimeError(f"precision must be one of {sorted(_PRECISIONS)}, got {precision!r}")
+    M = len(vertex_counts)
+    if M == 0:
+        return []
+    if len(face_counts) != M or len(voxel_sizes) != M or len(grid_ranges) != M:
+        raise RuntimeError("vertex_counts, face_counts, voxel_sizes and grid_ranges must have one entry per mesh")
+    _require_device_tensor(vertices, "vertices", torch.float32, 2)
+    _require_device_tensor(faces, "faces", torch.int32, 2)
+    if vertices.size(1) != 3 or faces.size(1) != 3:
+        raise RuntimeError("vertices and faces must have 3 columns")
+    if vertices.device != faces.device:
+        raise RuntimeError("vertices and faces must be on the same device")
+    vo = (ctypes.c_int64 * (M + 1))()
+    fo = (ctypes.c_int64 * (M + 1))()
+    for k in range(M):
+        vo[k + 1] = vo[k] + int(vertex_counts[k])
+        fo[k + 1] = fo[k] + int(face_counts[k])
+    if vo[M] != vertices.size(0) or fo[M] != faces.size(0):
+        raise RuntimeError("vertex / face counts do not add up to the packed arrays")
+    vertices = vertices.contiguous()
+    faces = faces.contiguous()
+    dev = vertices.device
+    grids = (_Grid * M)()
+    for k in range(M):
+        grids[k] = _grid(voxel_sizes[k], grid_ranges[k])
+    ctx = _ctx(dev)
+    L = _load()
+    counts = (ctypes.c_int64 * M)()
+    with torch.cuda.device(dev), ctx.lock:
+        stream = _stream_handle(dev)
+        _check(L.ovx_fdg_set_precision(ctx.h, mode, float(cond_max)), "mesh_to_flexible_dual_grid_batch")
+        _check(L.ovx_fdg_batch_begin(ctx.h, vertices.data_ptr(), int(vo[M]), faces.data_ptr(), int(fo[M]), M, fo, vo,
+                                     grids, float(face_weight), float(boundary_weight),
+                                     float(regularization_weight), stream, counts), "mesh_to_flexible_dual_grid_batch")
+        n = [int(counts[k]) for k in range(M)]
+        N = sum(n)
+        coords = torch.empty((N, 3), dtype=torch.int32, device=dev)
+        dual = torch.empty((N, 3), dtype=torch.float32, device=dev)
+        inter = torch.empty((N, 3), dtype=torch.bool, device=dev)
+        _check(L.ovx_fdg_finish(ctx.h, coords.data_ptr(), dual.data_ptr(), inter.data_ptr(), stream),
+               "mesh_to_flexible_dual_grid_batch")
+    return list(zip(coords.split(n), dual.split(n), inter.split(n)))
+
+
 def mesh_to_flexible_dual_grid_batch_cuda(
     vertices: List[torch.Tensor],
     faces: List[torch.Tensor],
@@ -234,59 +301,28 @@ def mesh_to_flexible_dual_grid_batch_cuda(
     precision: str = "reference",
     cond_max: float = 1.0e6,
 ) -> List[Tuple[torch.Tensor, torch.Tensor, torch.Tensor]]:
-    """Batched device flexible dual grid: M meshes (grid-local f32 [V_m,3]
-    vertices, i32 [F_m,3] faces with per-mesh indices, each with its own grid)
-    converted together, every pipeline stage one launch over all of them.  No
-    reference counterpart (the reference converts one mesh per call,
-    src/ext.cpp:19); each mesh's outputs equal those of
-    mesh_to_flexible_dual_grid_cuda on it alone.  Returns one (coords, dual,
-    intersected) tuple per mesh (views of shared output tensors)."""
-    mode = _PRECISIONS.get(precision)
-    if mode is None:
-        raise RuntimeError(f"precision must be one of {sorted(_PRECISIONS)}, got {precision!r}")
+    """Batched device flexible dual grid on per-mesh tensors (grid-local f32
+    [V_m,3] vertices, i32 [F_m,3] faces with per-mesh indices): packs them
+    (one concatenation each, one index rebase) and runs
+    mesh_to_flexible_dual_grid_batch_packed_cuda."""
     M = len(vertices)
     if M == 0:
         return []
-    if len(faces) != M or len(voxel_sizes) != M or len(grid_ranges) != M:
-        raise RuntimeError("vertices, faces, voxel_sizes and grid_ranges must have one entry per mesh")
-    dev = vertices[0].device
+    if len(faces) != M:
+        raise RuntimeError("vertices and faces must have one entry per mesh")
     for k in range(M):
         _require_device_tensor(vertices[k], f"vertices[{k}]", torch.float32, 2)
         _require_device_tensor(faces[k], f"faces[{k}]", torch.int32, 2)
-        if vertices[k].size(1) != 3 or faces[k].size(1) != 3:
-            raise RuntimeError(f"mesh {k}: vertices and faces must have 3 columns")
-        if vertices[k].device != dev or faces[k].device != dev:
-            raise RuntimeError("all meshes of a batch must be on the same device")
+    dev = vertices[0].device
     nv = [int(v.size(0)) for v in vertices]
     nf = [int(f.size(0)) for f in faces]
-    vo = (ctypes.c_int64 * (M + 1))()
-    fo = (ctypes.c_int64 * (M + 1))()
-    for k in range(M):
-        vo[k + 1] = vo[k] + nv[k]
-        fo[k + 1] = fo[k] + nf[k]
-    # one vertex array and one face array with batch-global indices
-    v_all = torch.cat([v.contiguous() for v in vertices], 0) if M > 1 else vertices[0].contiguous()
-    f_all = torch.cat([f + int(vo[k]) if vo[k] else f for k, f in enumerate(faces)], 0).to(torch.int32).contiguous()
-    grids = (_Grid * M)()
-    for k in range(M):
-        grids[k] = _grid(voxel_sizes[k], grid_ranges[k])
-    ctx = _ctx(dev)
-    L = _load()
-    counts = (ctypes.c_int64 * M)()
-    with torch.cuda.device(dev), ctx.lock:
-        stream = _stream_handle(dev)
-        _check(L.ovx_fdg_set_precision(ctx.h, mode, float(cond_max)), "mesh_to_flexible_dual_grid_batch")
-        _check(L.ovx_fdg_batch_begin(ctx.h, v_all.data_ptr(), int(vo[M]), f_all.data_ptr(), int(fo[M]), M, fo, vo,
-                                     grids, float(face_weight), float(boundary_weight),
-                                     float(regularization_weight), stream, counts), "mesh_to_flexible_dual_grid_batch")
-        n = [int(counts[k]) for k in range(M)]
-        N = sum(n)
-        coords = torch.empty((N, 3), dtype=torch.int32, device=dev)
-        dual = torch.empty((N, 3), dtype=torch.float32, device=dev)
-        inter = torch.empty((N, 3), dtype=torch.bool, device=dev)
-        _check(L.ovx_fdg_finish(ctx.h, coords.data_ptr(), dual.data_ptr(), inter.data_ptr(), stream),
-               "mesh_to_flexible_dual_grid_batch")
-    return list(zip(coords.split(n), dual.split(n), inter.split(n)))
+    v_all = torch.cat(vertices, 0)
+    f_all = torch.cat(faces, 0)
+    base = torch.tensor([0] + nv[:-1], dtype=torch.int64).cumsum(0).to(torch.int32).to(dev)
+    f_all = f_all + torch.repeat_interleave(base, torch.tensor(nf, device=dev), output_size=sum(nf)).unsqueeze(1)
+    return mesh_to_flexible_dual_grid_batch_packed_cuda(v_all, f_all.to(torch.int32), nv, nf, voxel_sizes, grid_ranges,
+                                                        face_weight, boundary_weight, regularization_weight,
+                                                        precision=precision, cond_max=cond_max)
 
 
 def intersect_occ_cuda(triangles: torch.Tensor, voxel_size: List[float], grid_range: List[int]) -> torch.Tensor:

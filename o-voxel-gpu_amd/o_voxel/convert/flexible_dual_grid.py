@@ -187,6 +187,8 @@ def mesh_to_flexible_dual_grid_batch(
     """
     meshes = list(meshes)
     M = len(meshes)
+    if M == 0:
+        return []
 
     def pick(single, plural, k, name):
         if plural is not None:
@@ -194,30 +196,65 @@ def mesh_to_flexible_dual_grid_batch(
             return plural[k]
         return single
 
-    vs_list, gr_list, v_list, f_list = [], [], [], []
-    device = None
+    # grid set-up per mesh, shared by meshes with the same explicit parameters
+    cache = {}
+    vs_list, gr_list, shift_rows, v_list, f_list = [], [], [], [], []
+    device = meshes[0][0].device if isinstance(meshes[0][0], torch.Tensor) else None
     for k, (vertices, faces) in enumerate(meshes):
         _check_mesh(vertices, faces)
-        device = vertices.device if device is None else device
         assert vertices.device == device, "all meshes must be on the same device"
-        vertices = vertices.to(device=device, dtype=torch.float32).contiguous()
-        faces = faces.to(device=device, dtype=torch.int32).contiguous()
-        vs_k, gr_k, aabb_dev = _grid_setup(vertices, pick(voxel_size, voxel_sizes, k, "voxel_size"),
-                                           pick(grid_size, grid_sizes, k, "grid_size"), pick(aabb, aabbs, k, "aabb"))
-        v_list.append(vertices - aabb_dev[0].reshape(1, 3))
+        vertices = vertices.to(dtype=torch.float32)
+        vs_k, gs_k, ab_k = (pick(voxel_size, voxel_sizes, k, "voxel_size"), pick(grid_size, grid_sizes, k, "grid_size"),
+                            pick(aabb, aabbs, k, "aabb"))
+        try:
+            key = _hashable((vs_k, gs_k, ab_k)) if ab_k is not None else None
+        except TypeError:
+            key = None
+        if key is not None and key in cache:
+            vs_l, gr_l, a0 = cache[key]
+        else:
+            vs_l, gr_l, aabb_dev = _grid_setup(vertices, vs_k, gs_k, ab_k)
+            a0 = aabb_dev[0].reshape(1, 3)
+            if key is not None:
+                cache[key] = (vs_l, gr_l, a0)
+        vs_list.append(vs_l)
+        gr_list.append(gr_l)
+        shift_rows.append(a0)
+        v_list.append(vertices)
         f_list.append(faces)
-        vs_list.append(vs_k)
-        gr_list.append(gr_k)
-    if M == 0:
-        return []
+    nv = [int(v.size(0)) for v in v_list]
+    nf = [int(f.size(0)) for f in f_list]
+    # packed grid-local vertices and batch-global face indices: a handful of
+    # whole-batch tensor operations rather than a few per mesh
+    shift = torch.cat(shift_rows, 0).to(device=device)
+    v_all = torch.cat(v_list, 0) - torch.repeat_interleave(shift, torch.tensor(nv, device=device), dim=0,
+                                                          output_size=sum(nv))
+    base = torch.tensor([0] + nv[:-1], dtype=torch.int64).cumsum(0).to(device)
+    f_all = torch.cat([f.to(torch.int64) for f in f_list], 0)
+    f_all = (f_all + torch.repeat_interleave(base, torch.tensor(nf, device=device), output_size=sum(nf)).unsqueeze(1)
+             ).to(torch.int32).contiguous()
     if device.type == "cuda":
-        return _C.mesh_to_flexible_dual_grid_batch_cuda(v_list, f_list, vs_list, gr_list, face_weight,
-                                                        boundary_weight, regularization_weight, precision=precision)
+        return _C.mesh_to_flexible_dual_grid_batch_packed_cuda(v_all.contiguous(), f_all, nv, nf, vs_list, gr_list,
+                                                               face_weight, boundary_weight, regularization_weight,
+                                                               precision=precision)
     dev = _C._gpu_for_host_call()
-    out = _C.mesh_to_flexible_dual_grid_batch_cuda([v.to(dev) for v in v_list], [f.to(dev) for f in f_list], vs_list,
-                                                   gr_list, face_weight, boundary_weight, regularization_weight,
-                                                   precision=precision)
+    out = _C.mesh_to_flexible_dual_grid_batch_packed_cuda(v_all.to(dev).contiguous(), f_all.to(dev), nv, nf, vs_list,
+                                                          gr_list, face_weight, boundary_weight,
+                                                          regularization_weight, precision=precision)
     return [tuple(t.cpu() for t in o) for o in out]
+
+
+def _hashable(x):
+    """Nested plain-value key of grid parameters; None if a value is a device tensor."""
+    if x is None or isinstance(x, (int, float, str)):
+        return x
+    if isinstance(x, torch.Tensor):
+        if x.is_cuda:
+            raise TypeError
+        return ("t", tuple(x.reshape(-1).tolist()), tuple(x.shape), str(x.dtype))
+    if isinstance(x, np.ndarray):
+        return ("a", tuple(x.reshape(-1).tolist()), x.shape, str(x.dtype))
+    return tuple(_hashable(e) for e in x)
 
 
 @torch.no_grad()
