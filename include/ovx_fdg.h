@@ -179,6 +179,69 @@ const char *ovx_fdg_kernel_name(int slot);
 ovx_status ovx_fdg_kernel_times(ovx_fdg_ctx *ctx, float *ms, int n);
 ovx_status ovx_fdg_stage_times(ovx_fdg_ctx *ctx, float *ms4);
 
+/*
+ * flexible_dual_grid_to_mesh, device path: the quads around intersected grid
+ * edges whose four voxels all exist, split into triangles.
+ * Replaces the body of o_voxel.convert.flexible_dual_grid_to_mesh
+ * (o_voxel/convert/flexible_dual_grid.py:422-454: its voxel hashmap built by
+ * _C.hashmap_insert_3d_idx_as_val_cuda, the neighbour lookups by
+ * _C.hashmap_lookup_3d_cuda and the torch glue that forms the quads and
+ * picks each split).
+ *   d_coords      int32   [n_voxels, 3] voxel coordinates
+ *   d_dual        float32 [n_voxels, 3] dual vertices in voxel units
+ *   d_flags       uint8   [n_voxels, 3] intersected-edge flags (nonzero = set)
+ *   grid_size     host [3] (W, H, D): lookups outside [0,W)x[0,H)x[0,D) fail
+ *   voxel_size, aabb_min   host float [3]
+ *   d_vertices    float32 [n_voxels, 3] out: (coords + dual) * voxel_size + aabb_min
+ * begin returns the quad count L in *n_quads (one blocking readback).
+ * finish writes, in (voxel, axis) order,
+ *   d_faces  int32 [2L, 3] (may be null): two triangles per quad, split by the
+ *            larger product of opposite split weights when d_split_weight is
+ *            given (float32 [n_voxels] values; split_weight_type says which
+ *            dtype's multiply rounds the products: OVX_SPLIT_F32 / F16 /
+ *            BF16), otherwise by the reference's normal-alignment rule
+ *   d_quads  int32 [L, 4] (may be null): the quads' voxel rows
+ * Duplicate coordinates resolve to the largest row.  Uses the context's
+ * workspace; not between another begin / finish pair on the same context.
+ */
+#define OVX_SPLIT_F32 0
+#define OVX_SPLIT_F16 1
+#define OVX_SPLIT_BF16 2
+ovx_status ovx_dual_mesh_begin(ovx_fdg_ctx *ctx, const int32_t *d_coords, const float *d_dual, const uint8_t *d_flags,
+                               int64_t n_voxels, const int32_t *grid_size, const float *voxel_size,
+                               const float *aabb_min, float *d_vertices, void *stream, int64_t *n_quads);
+ovx_status ovx_dual_mesh_finish(ovx_fdg_ctx *ctx, const float *d_split_weight, int split_weight_type,
+                                int32_t *d_faces, int32_t *d_quads, void *stream);
+
+/*
+ * Generic device hashmap in the reference's table format
+ * (src/hash/hash.cuh, src/hash/hash.cu; declared src/hash/api.h:27-110,
+ * bound src/ext.cpp:12-16): caller-owned key and value arrays of `capacity`
+ * entries (key_bytes / value_bytes 4 or 8; empty keys all ones), Murmur3
+ * finalizer modulo capacity, linear probing.  Enqueued on `stream` of the
+ * current device; no context.
+ *   ovx_hashmap_insert / _lookup: keys [n] of the table's key type.
+ *   *_3d: coords int32 [n, 4] (b, x, y, z), 16-byte aligned, key
+ *     b*W*H*D + x*H*D + y*D + z; lookups of (x, y, z) outside the grid miss.
+ *   ovx_hashmap_insert_3d_idx_as_val: the value of row i is i.
+ * Lookups write all ones for a missing key.  Inserting an existing key
+ * overwrites its value.  A full table drops the keys it cannot place (the
+ * reference probes forever).
+ */
+ovx_status ovx_hashmap_insert(void *table_keys, int key_bytes, void *table_values, int value_bytes, int64_t capacity,
+                              const void *d_keys, const void *d_values, int64_t n, void *stream);
+ovx_status ovx_hashmap_lookup(const void *table_keys, int key_bytes, const void *table_values, int value_bytes,
+                              int64_t capacity, const void *d_keys, void *d_out, int64_t n, void *stream);
+ovx_status ovx_hashmap_insert_3d(void *table_keys, int key_bytes, void *table_values, int value_bytes,
+                                 int64_t capacity, const int32_t *d_coords, const void *d_values, int64_t n, int W,
+                                 int H, int D, void *stream);
+ovx_status ovx_hashmap_lookup_3d(const void *table_keys, int key_bytes, const void *table_values, int value_bytes,
+                                 int64_t capacity, const int32_t *d_coords, void *d_out, int64_t n, int W, int H,
+                                 int D, void *stream);
+ovx_status ovx_hashmap_insert_3d_idx_as_val(void *table_keys, int key_bytes, void *table_values, int value_bytes,
+                                            int64_t capacity, const int32_t *d_coords, int64_t n, int W, int H,
+                                            int D, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

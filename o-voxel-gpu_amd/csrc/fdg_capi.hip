@@ -13,6 +13,7 @@
 
 #include "ovx_fdg.h"
 #include "fdg_kernels.h"
+#include "dual_mesh.h"
 
 using namespace ovx;
 
@@ -75,6 +76,7 @@ struct ovx_fdg_ctx
     std::mutex mu;
     // workspace
     DevBuf planes, bits, woff, tiles, counters;
+    DevBuf dm_table, dm_cnt, dm_off, dm_mask, dm_tiles; // dual grid -> mesh
     DevBuf vcnt, voff, nbr, erank, bcnt, boff, bedge_v, bedges;
     DevBuf events, recs, chunks, heads, choff, bchunks, active, large, box, qef, bhits, fhits, shard_ctr, rowq, faceq;
     uint64_t per[SB_N] = {};       // per-shard capacities of the sharded buffers (grow-only)
@@ -84,7 +86,7 @@ struct ovx_fdg_ctx
     int64_t E = 0;           // boundary edges of the last begin
     unsigned long long *h_ctr = nullptr; // pinned mirror of the counters
     // state between begin and finish
-    int pending = 0; // 0 none, 1 fdg, 2 occ
+    int pending = 0; // 0 none, 1 fdg, 2 occ, 3 dual mesh
     MeshDev mesh{};
     int64_t V = 0;
     uint64_t n_words = 0;  // occupancy words of all meshes
@@ -110,8 +112,9 @@ struct ovx_fdg_ctx
     bool ran = false;                 // a conversion was started on this context
     // per-kernel timing (events around each launch on the caller's stream)
     bool timing = false;
-    hipEvent_t kev[2 * 16] = {};
-    bool kran[16] = {};
+    hipEvent_t kev[2 * 24] = {};
+    bool kran[24] = {};
+    DualMeshArgs dm{}; // dual grid -> mesh between begin and finish
 
     // every workspace buffer except the counters
     template <class Fn>
@@ -119,7 +122,8 @@ struct ovx_fdg_ctx
     {
         DevBuf *all[] = {&planes, &bits,   &woff,  &tiles,   &vcnt,   &voff,  &nbr, &erank, &bcnt, &boff,
                          &bedge_v, &bedges, &events, &recs,   &chunks, &heads, &choff, &bchunks, &active, &large,
-                         &box,     &qef,    &bhits,  &fhits,  &shard_ctr, &rowq, &faceq, &segbuf};
+                         &box,     &qef,    &bhits,  &fhits,  &shard_ctr, &rowq, &faceq, &segbuf,
+                         &dm_table, &dm_cnt, &dm_off, &dm_mask, &dm_tiles};
         for (DevBuf *b : all)
             fn(*b);
     }
@@ -230,11 +234,17 @@ enum KSlot
     KS_FOLD_LARGE,
     KS_SOLVE,
     KS_SOLVE_BOX,
+    KS_DM_INSERT, // dual grid -> mesh: voxel table
+    KS_DM_COUNT,  // vertices and quad counts
+    KS_DM_SCAN,
+    KS_DM_EMIT,   // quads and triangles
     KS_N
 };
-static_assert(2 * KS_N <= 32, "timing events");
-const char *const kslot_names[KS_N] = {"prep",      "mark", "word_scan", "edges",     "coords", "gen_events", "face",
-                                       "gen_fhits", "bdry", "gen_bhits", "active",    "fold",   "fold_large", "solve", "solve_box"};
+static_assert(2 * KS_N <= 48, "timing events");
+const char *const kslot_names[KS_N] = {"prep",      "mark",     "word_scan", "edges",   "coords",  "gen_events",
+                                       "face",      "gen_fhits", "bdry",     "gen_bhits", "active", "fold",
+                                       "fold_large", "solve",   "solve_box", "dm_insert", "dm_count", "dm_scan",
+                                       "dm_emit"};
 
 void tic(ovx_fdg_ctx *c, int k, hipStream_t s)
 {
@@ -964,6 +974,192 @@ ovx_status ovx_fdg_stage_times(ovx_fdg_ctx *c, float *ms4)
     ms4[1] = v(KS_FACE) + v(KS_GEN_FHITS);
     ms4[2] = v(KS_EDGES) + v(KS_BDRY) + v(KS_GEN_BHITS);
     ms4[3] = v(KS_ACTIVE) + v(KS_FOLD) + v(KS_FOLD_LARGE) + v(KS_SOLVE) + v(KS_SOLVE_BOX);
+    return OVX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// flexible_dual_grid_to_mesh
+// ---------------------------------------------------------------------------
+ovx_status ovx_dual_mesh_begin(ovx_fdg_ctx *c, const int32_t *d_coords, const float *d_dual, const uint8_t *d_flags,
+                               int64_t n_voxels, const int32_t *grid_size, const float *voxel_size,
+                               const float *aabb_min, float *d_vertices, void *stream, int64_t *n_quads)
+{
+    if (!c || !grid_size || !voxel_size || !aabb_min || !n_quads)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "null argument");
+    if (n_voxels < 0 || n_voxels > (1ll << 30))
+        return fail(OVX_ERR_CAPACITY, "n_voxels = %lld outside [0, 2^30]", (long long)n_voxels);
+    if (n_voxels > 0 && (!d_coords || !d_dual || !d_flags || !d_vertices))
+        return fail(OVX_ERR_INVALID_ARGUMENT, "null device pointer");
+    for (int a = 0; a < 3; ++a)
+        if (grid_size[a] < 1)
+            return fail(OVX_ERR_INVALID_ARGUMENT, "grid_size[%d] = %d must be positive", a, grid_size[a]);
+    DeviceGuard dg(c->device);
+    std::lock_guard<std::mutex> lk(c->mu);
+    hipStream_t s = (hipStream_t)stream;
+    c->pending = 0;
+    DualMeshArgs &a = c->dm;
+    a = DualMeshArgs{};
+    a.coords = d_coords;
+    a.dual = d_dual;
+    a.flags = d_flags;
+    a.N = n_voxels;
+    a.W = grid_size[0];
+    a.H = grid_size[1];
+    a.D = grid_size[2];
+    for (int k = 0; k < 3; ++k)
+    {
+        a.vs[k] = voxel_size[k];
+        a.a0[k] = aabb_min[k];
+    }
+    a.verts = d_vertices;
+    const uint64_t cells = (uint64_t)a.W * (uint64_t)a.H * (uint64_t)a.D;
+    a.wide = cells >= (1ull << 32) ? 1 : 0;
+    uint64_t cap = 64;
+    while (cap < 2 * (uint64_t)n_voxels)
+        cap <<= 1;
+    a.cap = cap;
+    *n_quads = 0;
+    if (n_voxels == 0)
+    {
+        c->pending = 3;
+        return OVX_OK;
+    }
+    ovx_status st;
+    if ((st = ensure(c->dm_table, cap * (a.wide ? 16 : 8))) || (st = ensure(c->dm_cnt, (size_t)n_voxels * 4)) ||
+        (st = ensure(c->dm_off, (size_t)n_voxels * 4)) || (st = ensure(c->dm_mask, (size_t)n_voxels)) ||
+        (st = ensure(c->dm_tiles, scan_tile_count((uint64_t)n_voxels) * 8)) ||
+        (st = ensure(c->counters, CTR_COUNT * sizeof(unsigned long long))))
+        return st;
+    a.table = c->dm_table.p;
+    a.cnt = (uint32_t *)c->dm_cnt.p;
+    a.off = (const uint32_t *)c->dm_off.p;
+    a.mask = (uint8_t *)c->dm_mask.p;
+    for (int k = KS_DM_INSERT; k <= KS_DM_EMIT; ++k)
+        c->kran[k] = false;
+    unsigned long long *ctr = (unsigned long long *)c->counters.p;
+    tic(c, KS_DM_INSERT, s);
+    HIP_TRY(launch_dm_clear(a, s));
+    HIP_TRY(launch_dm_insert(a, s));
+    toc(c, KS_DM_INSERT, s);
+    tic(c, KS_DM_COUNT, s);
+    HIP_TRY(launch_dm_count(a, s));
+    toc(c, KS_DM_COUNT, s);
+    tic(c, KS_DM_SCAN, s);
+    HIP_TRY(launch_scan_u32(a.cnt, (uint64_t)n_voxels, (uint32_t *)c->dm_off.p, (uint64_t *)c->dm_tiles.p,
+                            ctr + CTR_DM_QUADS, s));
+    toc(c, KS_DM_SCAN, s);
+    HIP_TRY(hipMemcpyAsync(c->h_ctr + CTR_DM_QUADS, ctr + CTR_DM_QUADS, sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    c->last_stream = s;
+    c->ran = true;
+    *n_quads = (int64_t)c->h_ctr[CTR_DM_QUADS];
+    c->pending = 3;
+    return OVX_OK;
+}
+
+ovx_status ovx_dual_mesh_finish(ovx_fdg_ctx *c, const float *d_split_weight, int split_weight_type, int32_t *d_faces,
+                                int32_t *d_quads, void *stream)
+{
+    if (!c)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "ctx is null");
+    if (split_weight_type < OVX_SPLIT_F32 || split_weight_type > OVX_SPLIT_BF16)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "split_weight_type %d is not one of OVX_SPLIT_*", split_weight_type);
+    if (((uintptr_t)d_quads & 15) != 0)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "d_quads must be 16-byte aligned");
+    DeviceGuard dg(c->device);
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (c->pending != 3)
+        return fail(OVX_ERR_STATE, "ovx_dual_mesh_finish without a successful ovx_dual_mesh_begin");
+    c->pending = 0;
+    if (c->dm.N == 0 || c->h_ctr[CTR_DM_QUADS] == 0 || (!d_faces && !d_quads))
+        return OVX_OK;
+    hipStream_t s = (hipStream_t)stream;
+    DualMeshArgs a = c->dm;
+    a.split_w = d_split_weight;
+    a.faces = d_faces;
+    a.quads = d_quads;
+    tic(c, KS_DM_EMIT, s);
+    HIP_TRY(launch_dm_emit(a, split_weight_type, s));
+    toc(c, KS_DM_EMIT, s);
+    c->last_stream = s;
+    return OVX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// generic hashmap (reference table format)
+// ---------------------------------------------------------------------------
+static ovx_status hash_args(const void *tk, int kb, const void *tv, int vb, int64_t cap, const void *in, int64_t n,
+                            bool coords)
+{
+    if ((kb != 4 && kb != 8) || (vb != 4 && vb != 8))
+        return fail(OVX_ERR_INVALID_ARGUMENT, "key_bytes / value_bytes must be 4 or 8 (got %d / %d)", kb, vb);
+    if (cap <= 0)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "capacity must be positive");
+    if (n < 0)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "n must be non-negative");
+    if (n > 0 && (!tk || !tv || !in))
+        return fail(OVX_ERR_INVALID_ARGUMENT, "null device pointer");
+    if (coords && ((uintptr_t)in & 15) != 0)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "coords must be 16-byte aligned int32 [n, 4]");
+    return OVX_OK;
+}
+
+ovx_status ovx_hashmap_insert(void *tk, int kb, void *tv, int vb, int64_t cap, const void *d_keys, const void *d_values,
+                              int64_t n, void *stream)
+{
+    if (ovx_status st = hash_args(tk, kb, tv, vb, cap, d_keys, n, false))
+        return st;
+    if (n > 0 && !d_values)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "null values");
+    HIP_TRY(launch_hash_insert(tk, kb, tv, vb, (uint64_t)cap, HASH_KEYS, d_keys, d_values, (uint64_t)n, 0, 0, 0,
+                               (hipStream_t)stream));
+    return OVX_OK;
+}
+
+ovx_status ovx_hashmap_lookup(const void *tk, int kb, const void *tv, int vb, int64_t cap, const void *d_keys,
+                              void *d_out, int64_t n, void *stream)
+{
+    if (ovx_status st = hash_args(tk, kb, tv, vb, cap, d_keys, n, false))
+        return st;
+    if (n > 0 && !d_out)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "null output");
+    HIP_TRY(launch_hash_lookup(tk, kb, tv, vb, (uint64_t)cap, HASH_KEYS, d_keys, d_out, (uint64_t)n, 0, 0, 0,
+                               (hipStream_t)stream));
+    return OVX_OK;
+}
+
+ovx_status ovx_hashmap_insert_3d(void *tk, int kb, void *tv, int vb, int64_t cap, const int32_t *d_coords,
+                                 const void *d_values, int64_t n, int W, int H, int D, void *stream)
+{
+    if (ovx_status st = hash_args(tk, kb, tv, vb, cap, d_coords, n, true))
+        return st;
+    if (n > 0 && !d_values)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "null values");
+    HIP_TRY(launch_hash_insert(tk, kb, tv, vb, (uint64_t)cap, HASH_COORDS, d_coords, d_values, (uint64_t)n, W, H, D,
+                               (hipStream_t)stream));
+    return OVX_OK;
+}
+
+ovx_status ovx_hashmap_lookup_3d(const void *tk, int kb, const void *tv, int vb, int64_t cap, const int32_t *d_coords,
+                                 void *d_out, int64_t n, int W, int H, int D, void *stream)
+{
+    if (ovx_status st = hash_args(tk, kb, tv, vb, cap, d_coords, n, true))
+        return st;
+    if (n > 0 && !d_out)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "null output");
+    HIP_TRY(launch_hash_lookup(tk, kb, tv, vb, (uint64_t)cap, HASH_COORDS, d_coords, d_out, (uint64_t)n, W, H, D,
+                               (hipStream_t)stream));
+    return OVX_OK;
+}
+
+ovx_status ovx_hashmap_insert_3d_idx_as_val(void *tk, int kb, void *tv, int vb, int64_t cap, const int32_t *d_coords,
+                                            int64_t n, int W, int H, int D, void *stream)
+{
+    if (ovx_status st = hash_args(tk, kb, tv, vb, cap, d_coords, n, true))
+        return st;
+    HIP_TRY(launch_hash_insert(tk, kb, tv, vb, (uint64_t)cap, HASH_COORDS, d_coords, nullptr, (uint64_t)n, W, H, D,
+                               (hipStream_t)stream));
     return OVX_OK;
 }
 

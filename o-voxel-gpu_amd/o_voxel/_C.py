@@ -36,6 +36,12 @@ __all__ = [
     "kernel_times",
     "set_capacity",
     "workspace_bytes",
+    "flexible_dual_grid_to_mesh_cuda",
+    "hashmap_insert_cuda",
+    "hashmap_lookup_cuda",
+    "hashmap_insert_3d_cuda",
+    "hashmap_lookup_3d_cuda",
+    "hashmap_insert_3d_idx_as_val_cuda",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -96,6 +102,18 @@ def _load():
         L.ovx_fdg_kernel_name.argtypes = [ctypes.c_int]
         L.ovx_fdg_kernel_name.restype = ctypes.c_char_p
         L.ovx_fdg_kernel_times.argtypes = [vp, ctypes.POINTER(f32), ctypes.c_int]
+        i32p, f32p = ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(f32)
+        L.ovx_dual_mesh_begin.argtypes = [vp, vp, vp, vp, i64, i32p, f32p, f32p, vp, vp, ctypes.POINTER(i64)]
+        L.ovx_dual_mesh_finish.argtypes = [vp, vp, ctypes.c_int, vp, vp, vp]
+        ci = ctypes.c_int
+        L.ovx_hashmap_insert.argtypes = [vp, ci, vp, ci, i64, vp, vp, i64, vp]
+        L.ovx_hashmap_lookup.argtypes = [vp, ci, vp, ci, i64, vp, vp, i64, vp]
+        L.ovx_hashmap_insert_3d.argtypes = [vp, ci, vp, ci, i64, vp, vp, i64, ci, ci, ci, vp]
+        L.ovx_hashmap_lookup_3d.argtypes = [vp, ci, vp, ci, i64, vp, vp, i64, ci, ci, ci, vp]
+        L.ovx_hashmap_insert_3d_idx_as_val.argtypes = [vp, ci, vp, ci, i64, vp, i64, ci, ci, ci, vp]
+        for name in ("ovx_dual_mesh_begin", "ovx_dual_mesh_finish", "ovx_hashmap_insert", "ovx_hashmap_lookup",
+                     "ovx_hashmap_insert_3d", "ovx_hashmap_lookup_3d", "ovx_hashmap_insert_3d_idx_as_val"):
+            getattr(L, name).restype = ctypes.c_int
         for name in ("ovx_fdg_create", "ovx_fdg_destroy", "ovx_fdg_trim", "ovx_fdg_begin", "ovx_fdg_finish",
                      "ovx_occ_begin", "ovx_occ_finish", "ovx_fdg_stats", "ovx_fdg_set_timing",
                      "ovx_fdg_stage_times", "ovx_fdg_kernel_times"):
@@ -348,6 +366,192 @@ def intersect_occ_cuda(triangles: torch.Tensor, voxel_size: List[float], grid_ra
         coords = torch.empty((int(n.value), 3), dtype=torch.int32, device=dev)
         _check(L.ovx_occ_finish(ctx.h, coords.data_ptr(), stream), "intersect_occ")
     return coords
+
+
+_SPLIT_TYPES = {torch.float32: 0, torch.float16: 1, torch.bfloat16: 2}
+
+
+def flexible_dual_grid_to_mesh_cuda(
+    coords: torch.Tensor,
+    dual_vertices: torch.Tensor,
+    intersected_flag: torch.Tensor,
+    split_weight,
+    grid_size: List[int],
+    voxel_size: List[float],
+    aabb_min: List[float],
+    faces: bool = True,
+    quads: bool = False,
+):
+    """Device dual grid -> mesh (the body of the reference's
+    flexible_dual_grid_to_mesh, o_voxel/convert/flexible_dual_grid.py:422-454,
+    as one library call instead of its hashmap calls and torch glue).
+
+    coords i32 [N,3], dual_vertices f32/f16/bf16 [N,3] in voxel units,
+    intersected_flag bool/uint8 [N,3], split_weight None or f32/f16/bf16 with
+    N values ([N,1] as the reference uses it, or [N]); grid_size (W, H, D),
+    voxel_size and aabb_min are 3-lists of the float32 values the reference
+    computes.  Returns (vertices f32 [N,3] = (coords + dual) * voxel_size +
+    aabb_min, faces i32 [2L,3] or None, quads i32 [L,4] or None), quads in
+    (voxel, axis) order.
+    """
+    _require_device_tensor(coords, "coords", torch.int32, 2)
+    if not isinstance(dual_vertices, torch.Tensor) or dual_vertices.device != coords.device:
+        raise RuntimeError("dual_vertices must be a tensor on the device of coords")
+    if dual_vertices.dtype not in _SPLIT_TYPES:
+        raise RuntimeError(f"dual_vertices must be float32, float16 or bfloat16, got {dual_vertices.dtype}")
+    if not isinstance(intersected_flag, torch.Tensor) or intersected_flag.device != coords.device:
+        raise RuntimeError("intersected_flag must be a tensor on the device of coords")
+    if intersected_flag.dtype not in (torch.bool, torch.uint8):
+        raise RuntimeError(f"intersected_flag must be bool or uint8, got {intersected_flag.dtype}")
+    N = coords.size(0)
+    for name, t in (("coords", coords), ("dual_vertices", dual_vertices), ("intersected_flag", intersected_flag)):
+        if t.dim() != 2 or t.size(0) != N or t.size(1) != 3:
+            raise RuntimeError(f"{name} must have shape [N, 3] with N = {N}, got {tuple(t.shape)}")
+    wtype = 0
+    if split_weight is not None:
+        if not isinstance(split_weight, torch.Tensor) or split_weight.device != coords.device:
+            raise RuntimeError("split_weight must be a tensor on the device of coords")
+        if split_weight.dtype not in _SPLIT_TYPES:
+            raise RuntimeError(f"split_weight must be float32, float16 or bfloat16, got {split_weight.dtype}")
+        if split_weight.numel() != N:
+            raise RuntimeError(f"split_weight must hold one value per voxel ({N}), got shape {tuple(split_weight.shape)}")
+        wtype = _SPLIT_TYPES[split_weight.dtype]
+        # float16 / bfloat16 -> float32 is exact; the kernel rounds products back
+        split_weight = split_weight.detach().reshape(N).to(torch.float32).contiguous()
+    dev = coords.device
+    coords = coords.contiguous()
+    dual = dual_vertices.detach().to(torch.float32).contiguous()
+    flags = intersected_flag.contiguous().view(torch.uint8)
+    gs = (ctypes.c_int32 * 3)(*[int(x) for x in grid_size])
+    vs = (ctypes.c_float * 3)(*[float(x) for x in voxel_size])
+    a0 = (ctypes.c_float * 3)(*[float(x) for x in aabb_min])
+    ctx = _ctx(dev)
+    L = _load()
+    with torch.cuda.device(dev), ctx.lock:
+        stream = _stream_handle(dev)
+        verts = torch.empty((N, 3), dtype=torch.float32, device=dev)
+        nq = ctypes.c_int64(0)
+        _check(L.ovx_dual_mesh_begin(ctx.h, coords.data_ptr(), dual.data_ptr(), flags.data_ptr(), N, gs, vs, a0,
+                                     verts.data_ptr(), stream, ctypes.byref(nq)), "flexible_dual_grid_to_mesh")
+        nq = int(nq.value)
+        f = torch.empty((2 * nq, 3), dtype=torch.int32, device=dev) if faces else None
+        q = torch.empty((nq, 4), dtype=torch.int32, device=dev) if quads else None
+        _check(L.ovx_dual_mesh_finish(ctx.h, split_weight.data_ptr() if split_weight is not None else None, wtype,
+                                      f.data_ptr() if f is not None else None,
+                                      q.data_ptr() if q is not None else None, stream),
+               "flexible_dual_grid_to_mesh")
+    return verts, f, q
+
+
+# ---------------------------------------------------------------------------
+# generic hashmap (reference: src/hash/api.h:27-110, src/hash/hash.cu, bound
+# src/ext.cpp:12-16).  Tables are caller-owned uint32 / uint64 tensors; the
+# keys tensor starts filled with all ones (empty).
+# ---------------------------------------------------------------------------
+_HASH_BYTES = {torch.uint32: 4, torch.uint64: 8}
+
+
+def _hash_table(hashmap_keys: torch.Tensor, hashmap_values: torch.Tensor):
+    for name, t in (("hashmap_keys", hashmap_keys), ("hashmap_values", hashmap_values)):
+        if not isinstance(t, torch.Tensor) or not t.is_cuda:
+            raise RuntimeError(f"{name} must be a device tensor")
+        if not t.is_contiguous():
+            raise RuntimeError(f"{name} must be contiguous")
+    kb, vb = _HASH_BYTES.get(hashmap_keys.dtype), _HASH_BYTES.get(hashmap_values.dtype)
+    if kb is None or vb is None:
+        raise RuntimeError("Unsupported data type")
+    if hashmap_keys.device != hashmap_values.device:
+        raise RuntimeError("hashmap_keys and hashmap_values must be on the same device")
+    if hashmap_values.numel() < hashmap_keys.numel():
+        raise RuntimeError("hashmap_values must have as many entries as hashmap_keys")
+    return kb, vb
+
+
+def _hash_input(t: torch.Tensor, name: str, dtype: torch.dtype, msg: str, dev: torch.device, cols: int = 0):
+    if not isinstance(t, torch.Tensor) or t.dtype != dtype:
+        raise RuntimeError(msg)
+    if t.device != dev:
+        raise RuntimeError(f"{name} must be on the device of the hashmap")
+    if cols and (t.dim() != 2 or t.size(1) != cols):
+        raise RuntimeError(f"{name} must have shape [M, {cols}], got {tuple(t.shape)}")
+    return t.contiguous()
+
+
+def _type_name(nbytes: int) -> str:
+    return "uint32" if nbytes == 4 else "uint64"
+
+
+def hashmap_insert_cuda(hashmap_keys: torch.Tensor, hashmap_values: torch.Tensor, keys: torch.Tensor,
+                        values: torch.Tensor) -> None:
+    """Insert keys[M] with values[M] (reference src/hash/hash.cu hashmap_insert_cuda)."""
+    kb, vb = _hash_table(hashmap_keys, hashmap_values)
+    dev = hashmap_keys.device
+    keys = _hash_input(keys, "keys", hashmap_keys.dtype, f"Keys must be {_type_name(kb)}", dev)
+    values = _hash_input(values, "values", hashmap_values.dtype, f"Values must be {_type_name(vb)}", dev)
+    M = keys.size(0) if keys.dim() else 0
+    with torch.cuda.device(dev):
+        _check(_load().ovx_hashmap_insert(hashmap_keys.data_ptr(), kb, hashmap_values.data_ptr(), vb,
+                                          hashmap_keys.numel(), keys.data_ptr(), values.data_ptr(), M,
+                                          _stream_handle(dev)), "hashmap_insert")
+
+
+def hashmap_lookup_cuda(hashmap_keys: torch.Tensor, hashmap_values: torch.Tensor, keys: torch.Tensor) -> torch.Tensor:
+    """Values of keys[M], all ones where missing (reference hashmap_lookup_cuda)."""
+    kb, vb = _hash_table(hashmap_keys, hashmap_values)
+    dev = hashmap_keys.device
+    keys = _hash_input(keys, "keys", hashmap_keys.dtype, f"Keys must be {_type_name(kb)}", dev)
+    M = keys.size(0) if keys.dim() else 0
+    out = torch.empty((M,), dtype=hashmap_values.dtype, device=dev)
+    with torch.cuda.device(dev):
+        _check(_load().ovx_hashmap_lookup(hashmap_keys.data_ptr(), kb, hashmap_values.data_ptr(), vb,
+                                          hashmap_keys.numel(), keys.data_ptr(), out.data_ptr(), M,
+                                          _stream_handle(dev)), "hashmap_lookup")
+    return out
+
+
+def hashmap_insert_3d_cuda(hashmap_keys: torch.Tensor, hashmap_values: torch.Tensor, coords: torch.Tensor,
+                           values: torch.Tensor, W: int, H: int, D: int) -> None:
+    """Insert int32 coords[M,4] = (b, x, y, z) under key b*W*H*D + x*H*D + y*D + z
+    (reference hashmap_insert_3d_cuda)."""
+    kb, vb = _hash_table(hashmap_keys, hashmap_values)
+    dev = hashmap_keys.device
+    coords = _hash_input(coords, "coords", torch.int32, "Coords must be int32", dev, cols=4)
+    values = _hash_input(values, "values", hashmap_values.dtype, f"Values must be {_type_name(vb)}", dev)
+    with torch.cuda.device(dev):
+        _check(_load().ovx_hashmap_insert_3d(hashmap_keys.data_ptr(), kb, hashmap_values.data_ptr(), vb,
+                                             hashmap_keys.numel(), coords.data_ptr(), values.data_ptr(),
+                                             coords.size(0), int(W), int(H), int(D), _stream_handle(dev)),
+               "hashmap_insert_3d")
+
+
+def hashmap_lookup_3d_cuda(hashmap_keys: torch.Tensor, hashmap_values: torch.Tensor, coords: torch.Tensor, W: int,
+                           H: int, D: int) -> torch.Tensor:
+    """Values of int32 coords[M,4]; all ones where missing or where (x, y, z)
+    lies outside [0,W) x [0,H) x [0,D) (reference hashmap_lookup_3d_cuda)."""
+    kb, vb = _hash_table(hashmap_keys, hashmap_values)
+    dev = hashmap_keys.device
+    coords = _hash_input(coords, "coords", torch.int32, "Coords must be int32", dev, cols=4)
+    out = torch.empty((coords.size(0),), dtype=hashmap_values.dtype, device=dev)
+    with torch.cuda.device(dev):
+        _check(_load().ovx_hashmap_lookup_3d(hashmap_keys.data_ptr(), kb, hashmap_values.data_ptr(), vb,
+                                             hashmap_keys.numel(), coords.data_ptr(), out.data_ptr(),
+                                             coords.size(0), int(W), int(H), int(D), _stream_handle(dev)),
+               "hashmap_lookup_3d")
+    return out
+
+
+def hashmap_insert_3d_idx_as_val_cuda(hashmap_keys: torch.Tensor, hashmap_values: torch.Tensor, coords: torch.Tensor,
+                                      W: int, H: int, D: int) -> None:
+    """Insert int32 coords[M,4] with each row's index as its value
+    (reference hashmap_insert_3d_idx_as_val_cuda)."""
+    kb, vb = _hash_table(hashmap_keys, hashmap_values)
+    dev = hashmap_keys.device
+    coords = _hash_input(coords, "coords", torch.int32, "Coords must be int32", dev, cols=4)
+    with torch.cuda.device(dev):
+        _check(_load().ovx_hashmap_insert_3d_idx_as_val(hashmap_keys.data_ptr(), kb, hashmap_values.data_ptr(), vb,
+                                                        hashmap_keys.numel(), coords.data_ptr(), coords.size(0),
+                                                        int(W), int(H), int(D), _stream_handle(dev)),
+               "hashmap_insert_3d_idx_as_val")
 
 
 def _gpu_for_host_call() -> torch.device:

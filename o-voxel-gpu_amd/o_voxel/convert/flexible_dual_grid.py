@@ -20,6 +20,7 @@ __all__ = [
     "mesh_to_flexible_dual_grid",
     "mesh_to_flexible_dual_grid_batch",
     "intersect_occ",
+    "flexible_dual_grid_to_mesh",
 ]
 
 
@@ -281,3 +282,124 @@ def intersect_occ(
     if vertices.is_cuda:
         return _C.intersect_occ_cuda(triangles, voxel_size_arg, grid_range_arg)
     return _C.intersect_occ_cpu(triangles, voxel_size_arg, grid_range_arg)
+
+
+# split patterns over a quad's 4 rows plus (train mode) its mid vertex in
+# column 4: the reference's quad_split_1 / quad_split_train
+# (flexible_dual_grid.py:376-381)
+_TRAIN_FAN = (0, 1, 4, 1, 2, 4, 2, 3, 4, 3, 0, 4)
+
+
+def _dual_grid_params(aabb, voxel_size, grid_size, device):
+    """aabb / voxel_size / grid_size as the reference's
+    flexible_dual_grid_to_mesh derives them (flexible_dual_grid.py:386-417):
+    float32 aabb [2,3], float32 voxel_size [3] and int32 grid_size [3]
+    tensors on ``device``, with its assertion messages."""
+    if isinstance(aabb, (list, tuple)):
+        aabb = np.array(aabb)
+    if isinstance(aabb, np.ndarray):
+        aabb = torch.tensor(aabb, dtype=torch.float32, device=device)
+    assert isinstance(aabb, torch.Tensor), f"aabb must be a list, tuple, np.ndarray, or torch.Tensor, but got {type(aabb)}"
+    assert aabb.dim() == 2, f"aabb must be a 2D tensor, but got {aabb.shape}"
+    assert aabb.size(0) == 2, f"aabb must have 2 rows, but got {aabb.size(0)}"
+    assert aabb.size(1) == 3, f"aabb must have 3 columns, but got {aabb.size(1)}"
+    aabb = aabb.to(device=device, dtype=torch.float32)
+    extent = aabb[1] - aabb[0]
+    if voxel_size is not None:
+        if isinstance(voxel_size, float):
+            voxel_size = [voxel_size] * 3
+        if isinstance(voxel_size, (list, tuple)):
+            voxel_size = np.array(voxel_size)
+        if isinstance(voxel_size, np.ndarray):
+            voxel_size = torch.tensor(voxel_size, dtype=torch.float32, device=device)
+        if isinstance(voxel_size, torch.Tensor):
+            voxel_size = voxel_size.to(device=device)
+            grid_size = (extent / voxel_size).round().int()
+    else:
+        assert grid_size is not None, "Either voxel_size or grid_size must be provided"
+        if isinstance(grid_size, int):
+            grid_size = [grid_size] * 3
+        if isinstance(grid_size, (list, tuple)):
+            grid_size = np.array(grid_size)
+        if isinstance(grid_size, np.ndarray):
+            grid_size = torch.tensor(grid_size, dtype=torch.int32, device=device)
+        if isinstance(grid_size, torch.Tensor):
+            grid_size = grid_size.to(device=device)
+            voxel_size = extent / grid_size
+    assert isinstance(voxel_size, torch.Tensor), (
+        f"voxel_size must be a float, list, tuple, np.ndarray, or torch.Tensor, but got {type(voxel_size)}"
+    )
+    assert voxel_size.dim() == 1, f"voxel_size must be a 1D tensor, but got {voxel_size.shape}"
+    assert voxel_size.size(0) == 3, f"voxel_size must have 3 elements, but got {voxel_size.size(0)}"
+    assert isinstance(grid_size, torch.Tensor), (
+        f"grid_size must be an int, list, tuple, np.ndarray, or torch.Tensor, but got {type(grid_size)}"
+    )
+    assert grid_size.dim() == 1, f"grid_size must be a 1D tensor, but got {grid_size.shape}"
+    assert grid_size.size(0) == 3, f"grid_size must have 3 elements, but got {grid_size.size(0)}"
+    return aabb, voxel_size.to(torch.float32), grid_size
+
+
+def flexible_dual_grid_to_mesh(
+    coords: torch.Tensor,
+    dual_vertices: torch.Tensor,
+    intersected_flag: torch.Tensor,
+    split_weight: Optional[torch.Tensor],
+    aabb: ArrayLike,
+    voxel_size: Optional[FloatArrayLike] = None,
+    grid_size: Optional[IntArrayLike] = None,
+    train: bool = False,
+):
+    """Triangle mesh of a flexible dual grid: one quad around every
+    intersected grid edge whose four voxels are all present, split into two
+    triangles (or, with ``train``, four triangles around a split-weighted mid
+    vertex).
+
+    Reference: o_voxel/convert/flexible_dual_grid.py:331-477 (same signature,
+    grid set-up and assertions).  ``dual_vertices`` are in voxel units
+    (the voxel's own [0, 1]^3).  Returns (vertices float32 [N,3] (train:
+    [N+L,3]), faces int32 [2L,3] (train: int64 [4L,3])).  The voxel table,
+    the neighbour lookups, the quads, the vertices and the split choice are
+    one HIP library call; train mode forms its mid vertices with
+    differentiable torch operations on the quads the library returns.
+    Differences: vertices are float32 whatever the aabb dtype, a grid of
+    2^32 cells or more uses 64-bit keys (the reference's int32 cell count
+    wraps), duplicate coordinates resolve to the last row, and CPU inputs
+    run on the current GPU (results returned on the CPU).
+    """
+    host = not coords.is_cuda
+    dev = _C._gpu_for_host_call() if host else coords.device
+    if host:
+        coords = coords.to(dev)
+        dual_vertices = dual_vertices.to(dev)
+        intersected_flag = intersected_flag.to(dev)
+        if split_weight is not None:
+            split_weight = split_weight.to(dev)
+    aabb_t, vs_t, gs_t = _dual_grid_params(aabb, voxel_size, grid_size, dev)
+    if train:
+        assert split_weight is not None, "split_weight must be provided in training mode"
+    N = dual_vertices.shape[0]
+    gs = [int(x) for x in gs_t.tolist()]
+    vs = [float(x) for x in vs_t.tolist()]
+    a0 = [float(x) for x in aabb_t[0].tolist()]
+    verts, faces, quads = _C.flexible_dual_grid_to_mesh_cuda(
+        coords, dual_vertices, intersected_flag, None if train else split_weight, gs, vs, a0,
+        faces=not train, quads=train)
+    if train or (torch.is_grad_enabled() and dual_vertices.requires_grad):
+        # the same values as the library's, through autograd
+        verts = (coords.float() + dual_vertices) * vs_t + aabb_t[0].reshape(1, 3)
+    if train:
+        L = quads.size(0)
+        q = quads.long()
+        corner = verts[q]                                                  # [L, 4, 3]
+        w = split_weight.reshape(N, 1)[q]                                  # [L, 4, 1]
+        w02 = w[:, 0] * w[:, 2]
+        w13 = w[:, 1] * w[:, 3]
+        mid02 = (corner[:, 0] + corner[:, 2]) / 2
+        mid13 = (corner[:, 1] + corner[:, 3]) / 2
+        mid = (w02 * mid02 + w13 * mid13) / (w02 + w13)
+        verts = torch.cat([verts, mid], dim=0)
+        q5 = torch.cat([q, torch.arange(N, N + L, device=dev).unsqueeze(1)], dim=1)
+        faces = q5[:, list(_TRAIN_FAN)].reshape(-1, 3)
+    if host:
+        return verts.cpu(), faces.cpu()
+    return verts, faces
