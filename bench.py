@@ -6,7 +6,8 @@ One step = one call of the public drop-in API
 resident in HBM (float32 vertices, int64 faces, as a caller passes them),
 including the wrapper's dtype cast and aabb shift.  Prints ONE JSON line.
 
-  python bench.py [--steps K] [--warmup W] [--workload scene|scene_large|ico_cap|scene_rankdef] [--grid G]
+  python bench.py [--steps K] [--warmup W] [--workload scene|scene_large|ico_cap|scene_rankdef|batch|to_mesh]
+                  [--grid G]
 
 Default workload: the ~280k-face seeded scene (closed icospheres/tori, open
 patches, 1% slivers) at grid 512, weights face 1.0 / boundary 0.2 /
@@ -178,6 +179,8 @@ def main():
 
     if args.workload == "batch":
         return run_batch(args, torch, o_voxel, _C, dev)
+    if args.workload == "to_mesh":
+        return run_to_mesh(args, torch, o_voxel, _C, dev)
     v, f, grid, seed = make_workload(args.workload, args.grid)
     F, V = int(f.shape[0]), int(v.shape[0])
     log(f"[bench] workload={args.workload} F={F} V={V} grid={grid}")
@@ -422,6 +425,160 @@ def run_batch(args, torch, o_voxel, _C, dev):
         "cold_ms": round(cold_ms, 3),
         "kernels_ms": {k: round(x, 4) for k, x in sorted(kt.items(), key=lambda kv: -kv[1]) if x >= 0},
         "counts": {k: v for k, v in st.items() if k != "workspace_bytes"},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def run_to_mesh(args, torch, o_voxel, _C, dev):
+    """flexible_dual_grid_to_mesh on the flexible dual grid of the 268k-face
+    scene (grid 512 unless --grid), dual vertices in voxel units as the
+    reference's example stores them.  Steps time the public API (fused HIP
+    path); beside it, the reference's own recipe for the same function (its
+    hashmap calls and torch glue, flexible_dual_grid.py:419-451) composed from
+    this build's generic hashmap on the same GPU, and the CPU oracle on a
+    z-slab sample of the voxels."""
+    import meshgen
+    from oracle import dual_mesh as DM
+
+    grid = args.grid or 512
+    v, f = meshgen.scene(seed=2)
+    aabb = [[-0.5, -0.5, -0.5], [0.5, 0.5, 0.5]]
+    coords, dual, inter = o_voxel.convert.mesh_to_flexible_dual_grid(
+        torch.from_numpy(v).to(dev), torch.from_numpy(f).to(dev), grid_size=grid, aabb=aabb,
+        face_weight=WEIGHTS[0], boundary_weight=WEIGHTS[1], regularization_weight=WEIGHTS[2])
+    dual = dual * grid - coords
+    N = int(coords.shape[0])
+    flags = int(inter.sum())
+    log(f"[bench] to_mesh: N={N} flags={flags} grid={grid}")
+
+    def fused():
+        return o_voxel.convert.flexible_dual_grid_to_mesh(coords, dual, inter, None, aabb=aabb, grid_size=grid)
+
+    off = torch.from_numpy(DM.EDGE_NEIGHBORS.astype(np.int32)).to(dev).unsqueeze(0)
+    vs = torch.full((3,), 1.0 / grid, dtype=torch.float32, device=dev)
+    a0 = torch.full((1, 3), -0.5, dtype=torch.float32, device=dev)
+
+    def recipe():
+        tk = torch.full((2 * N,), -1, dtype=torch.int32, device=dev).view(torch.uint32)
+        tv = torch.empty((2 * N,), dtype=torch.uint32, device=dev)
+        _C.hashmap_insert_3d_idx_as_val_cuda(tk, tv, torch.cat([torch.zeros_like(coords[:, :1]), coords], 1),
+                                             grid, grid, grid)
+        cand = (coords.reshape(N, 1, 1, 3) + off)[inter]
+        M = cand.shape[0]
+        keys = torch.cat([torch.zeros((M * 4, 1), dtype=torch.int32, device=dev), cand.reshape(-1, 3)], 1)
+        idx = _C.hashmap_lookup_3d_cuda(tk, tv, keys, grid, grid, grid).reshape(M, 4).int()
+        q = idx[(idx != -1).all(1)]
+        verts = (coords.float() + dual) * vs + a0
+
+        def align(t):
+            p0, p1, p2, p3 = (verts[t[:, k]] for k in range(4))
+            n0 = torch.cross(p1 - p0, p2 - p0, dim=1)
+            n1 = torch.cross(p2 - p1, p3 - p1, dim=1)
+            return (n0 * n1).sum(1, keepdim=True).abs()
+
+        t1, t2 = q[:, [0, 1, 2, 0, 2, 3]], q[:, [0, 1, 3, 3, 1, 2]]
+        return verts, torch.where(align(t1) > align(t2), t1, t2).reshape(-1, 3)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fused()
+    torch.cuda.synchronize()
+    cold_ms = (time.perf_counter() - t0) * 1e3
+    for _ in range(args.warmup):
+        fused()
+        recipe()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        vf, ff = fused()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        vr, fr = recipe()
+    torch.cuda.synchronize()
+    recipe_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    same_v = bool(torch.equal(vf, vr))
+    face_mismatch = int((ff != fr).any(1).sum()) if ff.shape == fr.shape else -1
+    # the quads split differently: how far their first triangle (q0, q1, q2)
+    # is from degenerate, |(v1 - v0) x (v2 - v0)| / (|v1 - v0| |v2 - v0|) in float64
+    diff_sine = None
+    if face_mismatch > 0:
+        qd = (ff != fr).reshape(-1, 6).any(1).nonzero().flatten()
+        _, _, quads = _C.flexible_dual_grid_to_mesh_cuda(coords, dual, inter, None, [grid] * 3, [1.0 / grid] * 3,
+                                                         [-0.5] * 3, faces=False, quads=True)
+        t1 = quads[qd].long()
+        p = vf.double()
+        e1, e2 = p[t1[:, 1]] - p[t1[:, 0]], p[t1[:, 2]] - p[t1[:, 0]]
+        sine = torch.linalg.norm(torch.cross(e1, e2, dim=1), dim=1) / (
+            torch.linalg.norm(e1, dim=1) * torch.linalg.norm(e2, dim=1)).clamp_min(1e-300)
+        diff_sine = float(sine.max())
+    L = int(ff.shape[0]) // 2
+    # per-kernel device times (HIP events on the call's stream), averaged over steps
+    _C.set_timing(dev, True)
+    acc = {}
+    for _ in range(args.steps):
+        fused()
+        for k, x in _C.kernel_times(dev).items():
+            if k.startswith("dm_"):
+                acc[k] = acc.get(k, 0.0) + x
+    _C.set_timing(dev, False)
+    kt = {k: x / args.steps for k, x in acc.items()}
+    if args.calibrate_copy:
+        src = torch.ones(64 * 2 ** 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    cap = 64
+    while cap < 2 * N:
+        cap *= 2
+    alg = {"dm_insert": 12 * N + 8 * cap + 8 * N,
+           "dm_count": 44 * N + 32 * flags,
+           "dm_scan": 8 * N,
+           "dm_emit": 17 * N + 104 * L}
+    dom = max(kt, key=kt.get)
+    achieved = alg[dom] / (kt[dom] * 1e-3) / 1e9
+    roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "algorithmic_bytes_per_launch": alg[dom], "avg_launch_ms": round(kt[dom], 4)}
+    tpath = os.path.join(ROOT, "profiles", f"{dom}_to_mesh_traffic.json")
+    if os.path.exists(tpath):
+        roof["traffic"] = json.load(open(tpath))["hbm_bytes_per_launch"]
+    cpu = None
+    if not args.no_cpu_baseline:
+        n_s = min(N, 1_000_000)  # voxels of the lowest z slabs (canonical order is z-major)
+        c_np, d_np, i_np = coords[:n_s].cpu().numpy(), dual[:n_s].cpu().numpy(), inter[:n_s].cpu().numpy()
+        t0 = time.perf_counter()
+        DM.dual_grid_to_mesh(c_np, d_np, i_np, None, (grid,) * 3, [1.0 / grid] * 3, [-0.5] * 3)
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(n_s / dt / 1e6, 4), "unit": "M voxels/s", "cores": 1, "kind": "port",
+               "sample": f"numpy/Python oracle on the first {n_s} voxels (lowest z slabs), {dt:.2f} s",
+               "cpu_model": platform.processor() or platform.machine(), "nproc": os.cpu_count()}
+    line = {
+        "metric": "flexible_dual_grid_to_mesh M voxels/s",
+        "value": round(N / (ms * 1e-3) / 1e6, 4),
+        "unit": "M voxels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"to_mesh-scene268k@{grid}", "voxels": N, "intersected_flags": flags, "quads": L,
+                   "grid": grid, "seed": 2},
+        "reference_recipe_ms": round(recipe_ms, 4),
+        "speedup_vs_reference_recipe": round(recipe_ms / ms, 3),
+        "parity_vs_recipe": {"vertices_identical": same_v, "faces_differing": face_mismatch,
+                             "max_sine_of_differing_first_triangles": diff_sine},
+        "cold_ms": round(cold_ms, 3),
+        "kernels_ms": {k: round(x, 4) for k, x in sorted(kt.items(), key=lambda kv: -kv[1])},
+        "roofline": roof,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 

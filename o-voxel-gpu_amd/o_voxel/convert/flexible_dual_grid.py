@@ -374,7 +374,9 @@ def flexible_dual_grid_to_mesh(
         intersected_flag = intersected_flag.to(dev)
         if split_weight is not None:
             split_weight = split_weight.to(dev)
-    aabb_t, vs_t, gs_t = _dual_grid_params(aabb, voxel_size, grid_size, dev)
+    # the grid parameters are derived on the host (the same float32 operations
+    # as on the device, without three device round trips per call)
+    aabb_t, vs_t, gs_t = _dual_grid_params(aabb, voxel_size, grid_size, torch.device("cpu"))
     if train:
         assert split_weight is not None, "split_weight must be provided in training mode"
     N = dual_vertices.shape[0]
@@ -386,7 +388,7 @@ def flexible_dual_grid_to_mesh(
         faces=not train, quads=train)
     if train or (torch.is_grad_enabled() and dual_vertices.requires_grad):
         # the same values as the library's, through autograd
-        verts = (coords.float() + dual_vertices) * vs_t + aabb_t[0].reshape(1, 3)
+        verts = (coords.float() + dual_vertices) * vs_t.to(dev) + aabb_t[0].reshape(1, 3).to(dev)
     if train:
         L = quads.size(0)
         q = quads.long()
