@@ -250,7 +250,14 @@ __constant__ int c_edge_nbr[3][4][3] = {
     {{0, 0, 0}, {0, 1, 0}, {1, 1, 0}, {1, 0, 0}},
 };
 
-template <bool WIDE>
+// cell of the direct table: x fastest, so lanes holding consecutive x (the
+// canonical voxel order) read neighbouring words
+__device__ __forceinline__ uint64_t direct_cell(uint32_t x, uint32_t y, uint32_t z, const DualMeshArgs &a)
+{
+    return ((uint64_t)z * (uint32_t)a.H + y) * (uint32_t)a.W + x;
+}
+
+template <int MODE>
 __device__ __forceinline__ uint32_t find_voxel(const DualMeshArgs &a, int x, int y, int z, int ax, int k)
 {
     // neighbour coordinates wrap like the reference's int32 tensor sum; the
@@ -260,11 +267,28 @@ __device__ __forceinline__ uint32_t find_voxel(const DualMeshArgs &a, int x, int
     const uint32_t zz = (uint32_t)z + (uint32_t)c_edge_nbr[ax][k][2];
     if (xx >= (uint32_t)a.W || yy >= (uint32_t)a.H || zz >= (uint32_t)a.D)
         return NONE;
+    if (MODE == DM_DIRECT)
+    {
+        // 0 (empty) -> NONE; a row beyond this call's voxels cannot come from
+        // this call's insertions and is treated as empty as well
+        const uint32_t r = ((const uint32_t *)a.table)[direct_cell(xx, yy, zz, a)] - 1u;
+        return r < (uint64_t)a.N ? r : NONE;
+    }
     const uint64_t key = ((uint64_t)xx * (uint32_t)a.H + yy) * (uint32_t)a.D + zz;
     const uint64_t mask = a.cap - 1;
-    if (WIDE)
+    if (MODE == DM_WIDE)
         return get_wide((const ulonglong2 *)a.table, mask, key);
     return get_narrow((const uint64_t *)a.table, mask, (uint32_t)key);
+}
+
+__global__ __launch_bounds__(HB) void k_dm_restore(uint32_t *table, const uint32_t *touched, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * HB + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t t = touched[i];
+    if (t != NONE)
+        table[t] = 0u;
 }
 
 __global__ __launch_bounds__(HB) void k_dm_clear(void *table, uint64_t n, int wide)
@@ -279,7 +303,7 @@ __global__ __launch_bounds__(HB) void k_dm_clear(void *table, uint64_t n, int wi
     }
 }
 
-template <bool WIDE>
+template <int MODE>
 __global__ __launch_bounds__(HB) void k_dm_insert(DualMeshArgs a)
 {
     const int64_t i = (int64_t)blockIdx.x * HB + threadIdx.x;
@@ -288,11 +312,35 @@ __global__ __launch_bounds__(HB) void k_dm_insert(DualMeshArgs a)
     const int x = a.coords[3 * i], y = a.coords[3 * i + 1], z = a.coords[3 * i + 2];
     // the key the reference inserts for (0, x, y, z), truncated to the key width
     const uint64_t key = flat_key(0, x, y, z, a.W, a.H, a.D);
+    if (MODE == DM_DIRECT)
+    {
+        // a coordinate outside the grid lands where its truncated key names
+        // an in-grid cell (as in the reference's u32-keyed table), else nowhere
+        uint32_t cx = (uint32_t)x, cy = (uint32_t)y, cz = (uint32_t)z;
+        bool in = cx < (uint32_t)a.W && cy < (uint32_t)a.H && cz < (uint32_t)a.D;
+        if (!in)
+        {
+            const uint32_t k32 = (uint32_t)key;
+            const uint32_t hd = (uint32_t)a.H * (uint32_t)a.D;
+            in = k32 < (uint32_t)a.W * hd;
+            cx = k32 / hd;
+            cy = (k32 - cx * hd) / (uint32_t)a.D;
+            cz = k32 - cx * hd - cy * (uint32_t)a.D;
+        }
+        uint32_t t = NONE;
+        if (in)
+        {
+            t = (uint32_t)direct_cell(cx, cy, cz, a);
+            atomicMax((unsigned int *)a.table + t, (unsigned int)i + 1u); // duplicates: the last row
+        }
+        a.touched[i] = t;
+        return;
+    }
     // an all-ones 64-bit key (only from coordinates outside the grid) would
     // read as an empty slot; no in-grid lookup can match it
-    if (WIDE && key == EMPTY8)
+    if (MODE == DM_WIDE && key == EMPTY8)
         return;
-    if (WIDE)
+    if (MODE == DM_WIDE)
         put_wide((ulonglong2 *)a.table, a.cap - 1, key, (uint32_t)i);
     else
         put_narrow((uint64_t *)a.table, a.cap - 1, (uint32_t)key, (uint32_t)i);
@@ -301,7 +349,7 @@ __global__ __launch_bounds__(HB) void k_dm_insert(DualMeshArgs a)
 // mesh vertex (coords + dual) * voxel_size + aabb_min (one rounding per
 // operation, reference flexible_dual_grid.py:437); quads whose four voxels
 // all exist (reference :422-431)
-template <bool WIDE>
+template <int MODE>
 __global__ __launch_bounds__(HB) void k_dm_count(DualMeshArgs a)
 {
     const int64_t i = (int64_t)blockIdx.x * HB + threadIdx.x;
@@ -323,7 +371,7 @@ __global__ __launch_bounds__(HB) void k_dm_count(DualMeshArgs a)
             continue;
         bool ok = true;
         for (int k = 0; k < 4 && ok; ++k)
-            ok = find_voxel<WIDE>(a, c[0], c[1], c[2], ax, k) != NONE;
+            ok = find_voxel<MODE>(a, c[0], c[1], c[2], ax, k) != NONE;
         if (ok)
             m |= 1u << ax;
     }
@@ -380,7 +428,7 @@ __device__ __forceinline__ float weight_product(float a, float b, int wtype)
     return p;
 }
 
-template <bool WIDE>
+template <int MODE>
 __global__ __launch_bounds__(HB) void k_dm_emit(DualMeshArgs a, int wtype)
 {
     const int64_t i = (int64_t)blockIdx.x * HB + threadIdx.x;
@@ -398,7 +446,7 @@ __global__ __launch_bounds__(HB) void k_dm_emit(DualMeshArgs a, int wtype)
         uint32_t q[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            q[k] = find_voxel<WIDE>(a, x, y, z, ax, k);
+            q[k] = find_voxel<MODE>(a, x, y, z, ax, k);
         if (a.quads)
             *(int4 *)(a.quads + 4 * j) = make_int4((int)q[0], (int)q[1], (int)q[2], (int)q[3]);
         if (a.faces)
@@ -461,9 +509,29 @@ hipError_t launch_hash_lookup(const void *tkeys, int key_bytes, const void *tval
                           : hash_lookup_t<uint64_t, uint64_t>(tkeys, tvals, cap, src, keys, out, n, W, H, D, s);
 }
 
+#define DM_DISPATCH(kern, g, ...)                                                                                   \
+    do                                                                                                             \
+    {                                                                                                              \
+        if (a.mode == DM_DIRECT)                                                                                   \
+            hipLaunchKernelGGL(kern<DM_DIRECT>, g, dim3(HB), 0, s, __VA_ARGS__);                                    \
+        else if (a.mode == DM_WIDE)                                                                                \
+            hipLaunchKernelGGL(kern<DM_WIDE>, g, dim3(HB), 0, s, __VA_ARGS__);                                      \
+        else                                                                                                       \
+            hipLaunchKernelGGL(kern<DM_NARROW>, g, dim3(HB), 0, s, __VA_ARGS__);                                    \
+    } while (0)
+
 hipError_t launch_dm_clear(const DualMeshArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_dm_clear, dim3(grid_of(a.cap, 1u << 16)), dim3(HB), 0, s, a.table, a.cap, a.wide);
+    hipLaunchKernelGGL(k_dm_clear, dim3(grid_of(a.cap, 1u << 16)), dim3(HB), 0, s, a.table, a.cap,
+                       a.mode == DM_WIDE ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_dm_restore(uint32_t *table, const uint32_t *touched, int64_t n, hipStream_t s)
+{
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_dm_restore, dim3((unsigned)((n + HB - 1) / HB)), dim3(HB), 0, s, table, touched, n);
     return hipGetLastError();
 }
 
@@ -472,10 +540,7 @@ hipError_t launch_dm_insert(const DualMeshArgs &a, hipStream_t s)
     if (a.N == 0)
         return hipSuccess;
     const dim3 g((unsigned)((a.N + HB - 1) / HB));
-    if (a.wide)
-        hipLaunchKernelGGL(k_dm_insert<true>, g, dim3(HB), 0, s, a);
-    else
-        hipLaunchKernelGGL(k_dm_insert<false>, g, dim3(HB), 0, s, a);
+    DM_DISPATCH(k_dm_insert, g, a);
     return hipGetLastError();
 }
 
@@ -484,10 +549,7 @@ hipError_t launch_dm_count(const DualMeshArgs &a, hipStream_t s)
     if (a.N == 0)
         return hipSuccess;
     const dim3 g((unsigned)((a.N + HB - 1) / HB));
-    if (a.wide)
-        hipLaunchKernelGGL(k_dm_count<true>, g, dim3(HB), 0, s, a);
-    else
-        hipLaunchKernelGGL(k_dm_count<false>, g, dim3(HB), 0, s, a);
+    DM_DISPATCH(k_dm_count, g, a);
     return hipGetLastError();
 }
 
@@ -496,10 +558,7 @@ hipError_t launch_dm_emit(const DualMeshArgs &a, int wtype, hipStream_t s)
     if (a.N == 0)
         return hipSuccess;
     const dim3 g((unsigned)((a.N + HB - 1) / HB));
-    if (a.wide)
-        hipLaunchKernelGGL(k_dm_emit<true>, g, dim3(HB), 0, s, a, wtype);
-    else
-        hipLaunchKernelGGL(k_dm_emit<false>, g, dim3(HB), 0, s, a, wtype);
+    DM_DISPATCH(k_dm_emit, g, a, wtype);
     return hipGetLastError();
 }
 

@@ -77,6 +77,8 @@ struct ovx_fdg_ctx
     // workspace
     DevBuf planes, bits, woff, tiles, counters;
     DevBuf dm_table, dm_cnt, dm_off, dm_mask, dm_tiles; // dual grid -> mesh
+    DevBuf dm_direct, dm_touched; // direct-mapped voxel table (all zero between calls) and the cells a call set
+    int64_t dm_dirty = 0;         // rows in dm_touched whose cells are still set
     DevBuf vcnt, voff, nbr, erank, bcnt, boff, bedge_v, bedges;
     DevBuf events, recs, chunks, heads, choff, bchunks, active, large, box, qef, bhits, fhits, shard_ctr, rowq, faceq;
     uint64_t per[SB_N] = {};       // per-shard capacities of the sharded buffers (grow-only)
@@ -123,7 +125,7 @@ struct ovx_fdg_ctx
         DevBuf *all[] = {&planes, &bits,   &woff,  &tiles,   &vcnt,   &voff,  &nbr, &erank, &bcnt, &boff,
                          &bedge_v, &bedges, &events, &recs,   &chunks, &heads, &choff, &bchunks, &active, &large,
                          &box,     &qef,    &bhits,  &fhits,  &shard_ctr, &rowq, &faceq, &segbuf,
-                         &dm_table, &dm_cnt, &dm_off, &dm_mask, &dm_tiles};
+                         &dm_table, &dm_cnt, &dm_off, &dm_mask, &dm_tiles, &dm_direct, &dm_touched};
         for (DevBuf *b : all)
             fn(*b);
     }
@@ -238,13 +240,14 @@ enum KSlot
     KS_DM_COUNT,  // vertices and quad counts
     KS_DM_SCAN,
     KS_DM_EMIT,   // quads and triangles
+    KS_DM_RESTORE, // direct table: clear the cells of the call
     KS_N
 };
 static_assert(2 * KS_N <= 48, "timing events");
 const char *const kslot_names[KS_N] = {"prep",      "mark",     "word_scan", "edges",   "coords",  "gen_events",
                                        "face",      "gen_fhits", "bdry",     "gen_bhits", "active", "fold",
                                        "fold_large", "solve",   "solve_box", "dm_insert", "dm_count", "dm_scan",
-                                       "dm_emit"};
+                                       "dm_emit",   "dm_restore"};
 
 void tic(ovx_fdg_ctx *c, int k, hipStream_t s)
 {
@@ -772,6 +775,7 @@ ovx_status ovx_fdg_trim(ovx_fdg_ctx *c)
     for (int i = 0; i < SB_N; ++i)
         c->per[i] = 0;
     c->pending = 0;
+    c->dm_dirty = 0; // the direct table is re-zeroed when it is allocated again
     return OVX_OK;
 }
 
@@ -1013,11 +1017,11 @@ ovx_status ovx_dual_mesh_begin(ovx_fdg_ctx *c, const int32_t *d_coords, const fl
     }
     a.verts = d_vertices;
     const uint64_t cells = (uint64_t)a.W * (uint64_t)a.H * (uint64_t)a.D;
-    a.wide = cells >= (1ull << 32) ? 1 : 0;
+    a.mode = cells <= DM_DIRECT_MAX_CELLS ? DM_DIRECT : (cells >= (1ull << 32) ? DM_WIDE : DM_NARROW);
     uint64_t cap = 64;
     while (cap < 2 * (uint64_t)n_voxels)
         cap <<= 1;
-    a.cap = cap;
+    a.cap = a.mode == DM_DIRECT ? cells : cap;
     *n_quads = 0;
     if (n_voxels == 0)
     {
@@ -1025,22 +1029,42 @@ ovx_status ovx_dual_mesh_begin(ovx_fdg_ctx *c, const int32_t *d_coords, const fl
         return OVX_OK;
     }
     ovx_status st;
-    if ((st = ensure(c->dm_table, cap * (a.wide ? 16 : 8))) || (st = ensure(c->dm_cnt, (size_t)n_voxels * 4)) ||
+    if (c->dm_dirty) // a begin whose finish never ran left cells set
+    {
+        HIP_TRY(launch_dm_restore((uint32_t *)c->dm_direct.p, (const uint32_t *)c->dm_touched.p, c->dm_dirty, s));
+        c->dm_dirty = 0;
+    }
+    if (a.mode == DM_DIRECT)
+    {
+        void *old = c->dm_direct.p;
+        const size_t old_bytes = c->dm_direct.bytes;
+        if ((st = ensure(c->dm_direct, cells * 4)) || (st = ensure(c->dm_touched, (size_t)n_voxels * 4)))
+            return st;
+        if (c->dm_direct.p != old || c->dm_direct.bytes != old_bytes)
+            HIP_TRY(hipMemsetAsync(c->dm_direct.p, 0, c->dm_direct.bytes, s));
+    }
+    else if ((st = ensure(c->dm_table, cap * (a.mode == DM_WIDE ? 16 : 8))))
+        return st;
+    if ((st = ensure(c->dm_cnt, (size_t)n_voxels * 4)) ||
         (st = ensure(c->dm_off, (size_t)n_voxels * 4)) || (st = ensure(c->dm_mask, (size_t)n_voxels)) ||
         (st = ensure(c->dm_tiles, scan_tile_count((uint64_t)n_voxels) * 8)) ||
         (st = ensure(c->counters, CTR_COUNT * sizeof(unsigned long long))))
         return st;
-    a.table = c->dm_table.p;
+    a.table = a.mode == DM_DIRECT ? c->dm_direct.p : c->dm_table.p;
+    a.touched = a.mode == DM_DIRECT ? (uint32_t *)c->dm_touched.p : nullptr;
     a.cnt = (uint32_t *)c->dm_cnt.p;
     a.off = (const uint32_t *)c->dm_off.p;
     a.mask = (uint8_t *)c->dm_mask.p;
-    for (int k = KS_DM_INSERT; k <= KS_DM_EMIT; ++k)
+    for (int k = KS_DM_INSERT; k <= KS_DM_RESTORE; ++k)
         c->kran[k] = false;
     unsigned long long *ctr = (unsigned long long *)c->counters.p;
     tic(c, KS_DM_INSERT, s);
-    HIP_TRY(launch_dm_clear(a, s));
+    if (a.mode != DM_DIRECT)
+        HIP_TRY(launch_dm_clear(a, s));
     HIP_TRY(launch_dm_insert(a, s));
     toc(c, KS_DM_INSERT, s);
+    if (a.mode == DM_DIRECT)
+        c->dm_dirty = n_voxels;
     tic(c, KS_DM_COUNT, s);
     HIP_TRY(launch_dm_count(a, s));
     toc(c, KS_DM_COUNT, s);
@@ -1072,16 +1096,24 @@ ovx_status ovx_dual_mesh_finish(ovx_fdg_ctx *c, const float *d_split_weight, int
     if (c->pending != 3)
         return fail(OVX_ERR_STATE, "ovx_dual_mesh_finish without a successful ovx_dual_mesh_begin");
     c->pending = 0;
-    if (c->dm.N == 0 || c->h_ctr[CTR_DM_QUADS] == 0 || (!d_faces && !d_quads))
-        return OVX_OK;
     hipStream_t s = (hipStream_t)stream;
     DualMeshArgs a = c->dm;
-    a.split_w = d_split_weight;
-    a.faces = d_faces;
-    a.quads = d_quads;
-    tic(c, KS_DM_EMIT, s);
-    HIP_TRY(launch_dm_emit(a, split_weight_type, s));
-    toc(c, KS_DM_EMIT, s);
+    if (a.N > 0 && c->h_ctr[CTR_DM_QUADS] > 0 && (d_faces || d_quads))
+    {
+        a.split_w = d_split_weight;
+        a.faces = d_faces;
+        a.quads = d_quads;
+        tic(c, KS_DM_EMIT, s);
+        HIP_TRY(launch_dm_emit(a, split_weight_type, s));
+        toc(c, KS_DM_EMIT, s);
+    }
+    if (c->dm_dirty)
+    {
+        tic(c, KS_DM_RESTORE, s);
+        HIP_TRY(launch_dm_restore((uint32_t *)c->dm_direct.p, (const uint32_t *)c->dm_touched.p, c->dm_dirty, s));
+        toc(c, KS_DM_RESTORE, s);
+        c->dm_dirty = 0;
+    }
     c->last_stream = s;
     return OVX_OK;
 }

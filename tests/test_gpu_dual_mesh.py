@@ -271,3 +271,56 @@ def test_to_mesh_empty_and_host(ov):
     d = torch.full((4, 3), 0.5)
     v, f = ov.convert.flexible_dual_grid_to_mesh(c, d, inter, None, aabb=[[0] * 3, [1] * 3], grid_size=4)
     assert not v.is_cuda and f.tolist() == [[0, 2, 3], [0, 3, 1]]
+
+
+def _oracle_mesh(coords, dual, inter, G):
+    return DM.dual_grid_to_mesh(coords.cpu().numpy(), dual.cpu().numpy(), inter.cpu().numpy(), None, G,
+                                [np.float32(1.0) / np.float32(g) for g in G], [-0.5] * 3)
+
+
+@pytest.mark.parametrize("G", [(64, 64, 64), (1100, 1100, 1100), (2048, 2048, 2048), (40, 3000, 9000)])
+def test_to_mesh_table_modes(ov, G):
+    """Direct-mapped table (grids up to 2^30 cells), packed 8-byte hash
+    (below 2^32 cells) and 16-byte hash (2^32 cells and more), with
+    duplicate rows and coordinates outside the grid whose keys alias cells
+    inside it."""
+    _say(f"to_mesh table mode grid={G}")
+    coords, dual, inter = _fdg_input(ov, 64, seed=23, faces=20000)
+    N = coords.shape[0]
+    g = torch.Generator(device="cpu").manual_seed(4)
+    dup = torch.randint(0, N, (200,), generator=g).to(DEV)
+    W, H, D = G
+    odd = torch.tensor([[0, 1, -1], [0, 0, D], [-1, 0, 0], [W, 0, 0], [2, -3, 5]], dtype=torch.int32, device=DEV)
+    c = torch.cat([coords, coords[dup], odd])
+    d = torch.cat([dual, dual[dup].flip(1), torch.full((5, 3), 0.25, device=DEV)])
+    f = torch.cat([inter, inter[dup], torch.ones((5, 3), dtype=torch.bool, device=DEV)])
+    verts, faces = ov.convert.flexible_dual_grid_to_mesh(c, d, f, None, aabb=[[-0.5] * 3, [0.5] * 3], grid_size=list(G))
+    rv, rf = _oracle_mesh(c, d, f, G)
+    np.testing.assert_array_equal(verts.cpu().numpy(), rv)
+    np.testing.assert_array_equal(faces.cpu().numpy(), rf)
+
+
+def test_to_mesh_begin_without_finish(ov):
+    """A begin whose finish never ran leaves the next call correct (the
+    direct table's cells of the abandoned call are cleared first)."""
+    import ctypes
+
+    _C = ov._C
+    L = _C._load()
+    coords, dual, inter = _fdg_input(ov, 64, seed=29, faces=20000)
+    dev = coords.device
+    ctx = _C._ctx(dev)
+    gs = (ctypes.c_int32 * 3)(64, 64, 64)
+    vs = (ctypes.c_float * 3)(*([1.0 / 64] * 3))
+    a0 = (ctypes.c_float * 3)(*([-0.5] * 3))
+    verts = torch.empty_like(dual)
+    flags = inter.contiguous().view(torch.uint8)
+    nq = ctypes.c_int64(0)
+    with torch.cuda.device(dev), ctx.lock:
+        st = L.ovx_dual_mesh_begin(ctx.h, coords.data_ptr(), dual.data_ptr(), flags.data_ptr(), coords.shape[0], gs,
+                                   vs, a0, verts.data_ptr(), _C._stream_handle(dev), ctypes.byref(nq))
+    assert st == 0 and nq.value > 0
+    # a different voxel set on the same grid
+    keep = torch.arange(coords.shape[0], device=DEV) % 3 != 0
+    _check_mesh(ov, coords[keep], dual[keep], inter[keep], 64)
+    _check_mesh(ov, coords, dual, inter, 64)
