@@ -500,9 +500,12 @@ def run_to_mesh(args, torch, o_voxel, _C, dev):
     recipe_ms = (time.perf_counter() - t0) * 1e3 / args.steps
     same_v = bool(torch.equal(vf, vr))
     face_mismatch = int((ff != fr).any(1).sum()) if ff.shape == fr.shape else -1
-    # the quads split differently: how far their first triangle (q0, q1, q2)
-    # is from degenerate, |(v1 - v0) x (v2 - v0)| / (|v1 - v0| |v2 - v0|) in float64
-    diff_sine = diff_edge = None
+    # the quads split differently: area of their first triangle (q0, q1, q2)
+    # relative to the quad's extent, |(v1 - v0) x (v2 - v0)| / diam^2 in float64
+    # (where it is within rounding of zero, torch's FMA-contracted cross of a
+    # vector with itself no longer gives exactly zero and the recipe's choice
+    # can flip)
+    diff_area = None
     if face_mismatch > 0:
         qd = (ff != fr).reshape(-1, 6).any(1).nonzero().flatten()
         _, _, quads = _C.flexible_dual_grid_to_mesh_cuda(coords, dual, inter, None, [grid] * 3, [1.0 / grid] * 3,
@@ -510,10 +513,9 @@ def run_to_mesh(args, torch, o_voxel, _C, dev):
         t1 = quads[qd].long()
         p = vf.double()
         e1, e2 = p[t1[:, 1]] - p[t1[:, 0]], p[t1[:, 2]] - p[t1[:, 0]]
-        n1, n2 = torch.linalg.norm(e1, dim=1), torch.linalg.norm(e2, dim=1)
-        sine = torch.linalg.norm(torch.cross(e1, e2, dim=1), dim=1) / (n1 * n2).clamp_min(1e-300)
-        diff_sine = float(sine.max())
-        diff_edge = float(torch.maximum(n1, n2).max()) * grid  # in voxels
+        corners = p[t1]  # [m, 4, 3]
+        diam = torch.cdist(corners, corners).amax((1, 2))
+        diff_area = float((torch.linalg.norm(torch.cross(e1, e2, dim=1), dim=1) / diam.clamp_min(1e-300) ** 2).max())
     L = int(ff.shape[0]) // 2
     # per-kernel device times (HIP events on the call's stream), averaged over steps
     _C.set_timing(dev, True)
@@ -575,8 +577,7 @@ def run_to_mesh(args, torch, o_voxel, _C, dev):
         "reference_recipe_ms": round(recipe_ms, 4),
         "speedup_vs_reference_recipe": round(recipe_ms / ms, 3),
         "parity_vs_recipe": {"vertices_identical": same_v, "faces_differing": face_mismatch,
-                             "max_sine_of_differing_first_triangles": diff_sine,
-                             "max_edge_voxels_of_differing_first_triangles": diff_edge},
+                             "max_relative_area_of_differing_first_triangles": diff_area},
         "cold_ms": round(cold_ms, 3),
         "kernels_ms": {k: round(x, 4) for k, x in sorted(kt.items(), key=lambda kv: -kv[1])},
         "roofline": roof,
