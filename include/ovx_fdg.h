@@ -242,6 +242,28 @@ ovx_status ovx_hashmap_insert_3d_idx_as_val(void *table_keys, int key_bytes, voi
                                             int64_t capacity, const int32_t *d_coords, int64_t n, int W, int H,
                                             int D, void *stream);
 
+/*
+ * 30-bit voxel serialization codes (10 bits per axis): Z-order (Morton,
+ * x's bit the most significant of each triple) and Hilbert (Skilling's
+ * transform, then the same interleave).
+ * Replaces z_order_encode/decode_cuda and hilbert_encode/decode_cuda
+ * (src/serialize/api.cu, z_order.cu, hilbert.cu; declared
+ * src/serialize/api.h; bound src/ext.cpp:28-31) and, on host copies, their
+ * _cpu twins (src/ext.cpp:24-27).  Codes are bit-identical to the
+ * reference's for every int32 input (coordinates outside [0, 1024) included).
+ *   d_cols[3], strides[3]  the three coordinate columns (element strides, so
+ *                          the columns of an [N, 3] tensor are read in place)
+ *   elem_bytes             4 (int32) or 8 (int64: the low 32 bits are used)
+ *   d_codes                int32 [n]
+ * Decode writes int32 columns through the same column/stride description.
+ */
+#define OVX_SEQ_Z_ORDER 0
+#define OVX_SEQ_HILBERT 1
+ovx_status ovx_seq_encode(const void *const *d_cols, const int64_t *strides, int elem_bytes, int64_t n, int mode,
+                          int32_t *d_codes, void *stream);
+ovx_status ovx_seq_decode(const int32_t *d_codes, int64_t n, int mode, int32_t *const *d_cols,
+                          const int64_t *strides, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,6 +14,7 @@
 #include "ovx_fdg.h"
 #include "fdg_kernels.h"
 #include "dual_mesh.h"
+#include "serialize.h"
 
 using namespace ovx;
 
@@ -1192,6 +1193,52 @@ ovx_status ovx_hashmap_insert_3d_idx_as_val(void *tk, int kb, void *tv, int vb, 
         return st;
     HIP_TRY(launch_hash_insert(tk, kb, tv, vb, (uint64_t)cap, HASH_COORDS, d_coords, nullptr, (uint64_t)n, W, H, D,
                                (hipStream_t)stream));
+    return OVX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Z-order / Hilbert codes
+// ---------------------------------------------------------------------------
+ovx_status ovx_seq_encode(const void *const *d_cols, const int64_t *strides, int elem_bytes, int64_t n, int mode,
+                          int32_t *d_codes, void *stream)
+{
+    if (!d_cols || !strides || n < 0 || (elem_bytes != 4 && elem_bytes != 8) ||
+        (mode != OVX_SEQ_Z_ORDER && mode != OVX_SEQ_HILBERT))
+        return fail(OVX_ERR_INVALID_ARGUMENT, "invalid ovx_seq_encode arguments");
+    if (n == 0)
+        return OVX_OK;
+    SeqCols in{};
+    for (int k = 0; k < 3; ++k)
+    {
+        if (!d_cols[k])
+            return fail(OVX_ERR_INVALID_ARGUMENT, "null coordinate column %d", k);
+        in.p[k] = d_cols[k];
+        in.stride[k] = strides[k];
+    }
+    if (!d_codes)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "null codes");
+    HIP_TRY(launch_seq_encode(in, elem_bytes, n, mode, d_codes, (hipStream_t)stream));
+    return OVX_OK;
+}
+
+ovx_status ovx_seq_decode(const int32_t *d_codes, int64_t n, int mode, int32_t *const *d_cols, const int64_t *strides,
+                          void *stream)
+{
+    if (!d_cols || !strides || n < 0 || (mode != OVX_SEQ_Z_ORDER && mode != OVX_SEQ_HILBERT))
+        return fail(OVX_ERR_INVALID_ARGUMENT, "invalid ovx_seq_decode arguments");
+    if (n == 0)
+        return OVX_OK;
+    if (!d_codes)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "null codes");
+    SeqCols out{};
+    for (int k = 0; k < 3; ++k)
+    {
+        if (!d_cols[k])
+            return fail(OVX_ERR_INVALID_ARGUMENT, "null coordinate column %d", k);
+        out.p[k] = d_cols[k];
+        out.stride[k] = strides[k];
+    }
+    HIP_TRY(launch_seq_decode(d_codes, n, mode, out, (hipStream_t)stream));
     return OVX_OK;
 }
 

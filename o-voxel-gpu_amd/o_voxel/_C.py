@@ -42,6 +42,14 @@ __all__ = [
     "hashmap_insert_3d_cuda",
     "hashmap_lookup_3d_cuda",
     "hashmap_insert_3d_idx_as_val_cuda",
+    "z_order_encode_cuda",
+    "z_order_decode_cuda",
+    "hilbert_encode_cuda",
+    "hilbert_decode_cuda",
+    "z_order_encode_cpu",
+    "z_order_decode_cpu",
+    "hilbert_encode_cpu",
+    "hilbert_decode_cpu",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -111,6 +119,11 @@ def _load():
         L.ovx_hashmap_insert_3d.argtypes = [vp, ci, vp, ci, i64, vp, vp, i64, ci, ci, ci, vp]
         L.ovx_hashmap_lookup_3d.argtypes = [vp, ci, vp, ci, i64, vp, vp, i64, ci, ci, ci, vp]
         L.ovx_hashmap_insert_3d_idx_as_val.argtypes = [vp, ci, vp, ci, i64, vp, i64, ci, ci, ci, vp]
+        i64p, vpp = ctypes.POINTER(i64), ctypes.POINTER(vp)
+        L.ovx_seq_encode.argtypes = [vpp, i64p, ci, i64, ci, vp, vp]
+        L.ovx_seq_decode.argtypes = [vp, i64, ci, vpp, i64p, vp]
+        for name in ("ovx_seq_encode", "ovx_seq_decode"):
+            getattr(L, name).restype = ctypes.c_int
         for name in ("ovx_dual_mesh_begin", "ovx_dual_mesh_finish", "ovx_hashmap_insert", "ovx_hashmap_lookup",
                      "ovx_hashmap_insert_3d", "ovx_hashmap_lookup_3d", "ovx_hashmap_insert_3d_idx_as_val"):
             getattr(L, name).restype = ctypes.c_int
@@ -552,6 +565,101 @@ def hashmap_insert_3d_idx_as_val_cuda(hashmap_keys: torch.Tensor, hashmap_values
                                                         hashmap_keys.numel(), coords.data_ptr(), coords.size(0),
                                                         int(W), int(H), int(D), _stream_handle(dev)),
                "hashmap_insert_3d_idx_as_val")
+
+
+# ---------------------------------------------------------------------------
+# 30-bit voxel codes (reference: src/serialize/api.cu, bound src/ext.cpp:24-31)
+# ---------------------------------------------------------------------------
+_SEQ_MODES = {"z_order": 0, "hilbert": 1}
+
+
+def seq_encode_columns(cols, mode: str) -> torch.Tensor:
+    """int32 codes [N] of three coordinate columns (1-D int32 / int64 device
+    tensors of one length, any strides; read in place)."""
+    dev = cols[0].device
+    n = cols[0].numel()
+    dt = cols[0].dtype
+    for c in cols:
+        if not isinstance(c, torch.Tensor) or not c.is_cuda or c.device != dev:
+            raise RuntimeError("coordinate columns must be device tensors on one device")
+        if c.dim() != 1 or c.numel() != n or c.dtype != dt or dt not in (torch.int32, torch.int64):
+            raise RuntimeError("coordinate columns must be 1-D int32 or int64 tensors of one length")
+    codes = torch.empty((n,), dtype=torch.int32, device=dev)
+    ptrs = (ctypes.c_void_p * 3)(*[c.data_ptr() for c in cols])
+    strides = (ctypes.c_int64 * 3)(*[c.stride(0) for c in cols])
+    with torch.cuda.device(dev):
+        _check(_load().ovx_seq_encode(ptrs, strides, cols[0].element_size(), n, _SEQ_MODES[mode], codes.data_ptr(),
+                                      _stream_handle(dev)), f"{mode}_encode")
+    return codes
+
+
+def seq_decode_into(codes: torch.Tensor, mode: str, out_cols) -> None:
+    """Write the three int32 coordinate columns of int32 codes [N] into
+    ``out_cols`` (1-D int32 device tensor views of length N, any strides)."""
+    dev = codes.device
+    ptrs = (ctypes.c_void_p * 3)(*[c.data_ptr() for c in out_cols])
+    strides = (ctypes.c_int64 * 3)(*[c.stride(0) for c in out_cols])
+    with torch.cuda.device(dev):
+        _check(_load().ovx_seq_decode(codes.data_ptr(), codes.numel(), _SEQ_MODES[mode], ptrs, strides,
+                                      _stream_handle(dev)), f"{mode}_decode")
+
+
+def _seq_codes_in(codes: torch.Tensor) -> torch.Tensor:
+    if not isinstance(codes, torch.Tensor) or not codes.is_cuda:
+        raise RuntimeError("codes must be a device tensor")
+    if codes.dtype not in (torch.int32, torch.int64):
+        raise RuntimeError(f"codes must be int32, got {codes.dtype}")
+    return codes.reshape(-1).to(torch.int32).contiguous()
+
+
+def _seq_decode(codes: torch.Tensor, mode: str):
+    codes = _seq_codes_in(codes)
+    out = torch.empty((3, codes.numel()), dtype=torch.int32, device=codes.device)
+    seq_decode_into(codes, mode, (out[0], out[1], out[2]))
+    return out[0], out[1], out[2]
+
+
+def z_order_encode_cuda(x: torch.Tensor, y: torch.Tensor, z: torch.Tensor) -> torch.Tensor:
+    """Morton codes of (x, y, z) (reference src/serialize/api.cu z_order_encode_cuda)."""
+    return seq_encode_columns((x, y, z), "z_order")
+
+
+def z_order_decode_cuda(codes: torch.Tensor):
+    """(x, y, z) int32 of Morton codes (reference z_order_decode_cuda)."""
+    return _seq_decode(codes, "z_order")
+
+
+def hilbert_encode_cuda(x: torch.Tensor, y: torch.Tensor, z: torch.Tensor) -> torch.Tensor:
+    """Hilbert codes of (x, y, z) (reference hilbert_encode_cuda)."""
+    return seq_encode_columns((x, y, z), "hilbert")
+
+
+def hilbert_decode_cuda(codes: torch.Tensor):
+    """(x, y, z) int32 of Hilbert codes (reference hilbert_decode_cuda)."""
+    return _seq_decode(codes, "hilbert")
+
+
+def _seq_host(fn, *ts):
+    dev = _gpu_for_host_call()
+    out = fn(*[t.to(dev) for t in ts])
+    return tuple(t.cpu() for t in out) if isinstance(out, tuple) else out.cpu()
+
+
+def z_order_encode_cpu(x, y, z):
+    """Host-tensor twin (reference src/ext.cpp:24), run on the current GPU."""
+    return _seq_host(z_order_encode_cuda, x, y, z)
+
+
+def z_order_decode_cpu(codes):
+    return _seq_host(z_order_decode_cuda, codes)
+
+
+def hilbert_encode_cpu(x, y, z):
+    return _seq_host(hilbert_encode_cuda, x, y, z)
+
+
+def hilbert_decode_cpu(codes):
+    return _seq_host(hilbert_decode_cuda, codes)
 
 
 def _gpu_for_host_call() -> torch.device:

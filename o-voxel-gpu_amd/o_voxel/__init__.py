@@ -1,8 +1,9 @@
 """o_voxel (MI355X build): the flexible-dual-grid conversion path.
 
-Only ``o_voxel.convert.mesh_to_flexible_dual_grid`` / ``intersect_occ`` and
-the ``o_voxel._C`` boundary they call are provided.
+Provided: ``o_voxel.convert`` (mesh_to_flexible_dual_grid, its batched form,
+intersect_occ, flexible_dual_grid_to_mesh), ``o_voxel.serialize``
+(encode_seq / decode_seq) and the ``o_voxel._C`` boundary they call.
 """
-from . import _C, convert
+from . import _C, convert, serialize
 
-__all__ = ["convert", "_C"]
+__all__ = ["convert", "serialize", "_C"]
