@@ -264,6 +264,48 @@ ovx_status ovx_seq_encode(const void *const *d_cols, const int64_t *strides, int
 ovx_status ovx_seq_decode(const int32_t *d_codes, int64_t n, int mode, int32_t *const *d_cols,
                           const int64_t *strides, void *stream);
 
+/*
+ * Sparse voxel octrees and attribute filters of the .vxz file format.
+ * Replace encode/decode_sparse_voxel_octree_cpu (src/io/svo.cpp:17-138) and
+ * encode/decode_sparse_voxel_octree_attr_{parent,neighbor}_cpu
+ * (src/io/filter_parent.cpp, src/io/filter_neighbor.cpp; declared
+ * src/io/api.h), which the reference binds from its extension and calls per
+ * chunk from o_voxel/io/vxz.py.  Octree bytes and filter outputs are
+ * identical to the reference's.
+ *
+ * Encoding runs on the device over all chunks at once:
+ *   d_codes      int32 [n] Morton codes of chunk-local coordinates (< 2^(3 depth)),
+ *                ascending inside each chunk; d_chunk int32 [n] chunk of each code;
+ *                d_chunk_off int64 [n_chunks + 1] first code of each chunk (device)
+ *   begin returns the total octree bytes (one blocking readback); finish
+ *   writes the chunks' octrees one after another into d_svo (4-byte aligned,
+ *   room for n_bytes rounded up to 4) and, if d_chunk_svo_off is given, each
+ *   chunk's byte offset (int64 [n_chunks + 1], device).
+ *   ovx_svo_parent_encode: after finish, the parent-filter rows of the
+ *   chunks (d_attr, d_delta uint8 [n, channels], in code order).
+ * ovx_svo_neighbor_filter: the neighbour filter (decode = 0) or its inverse
+ *   (decode = 1) over chunk-local int32 [n, 3] coordinates in [0, res),
+ *   n_chunks * res^3 <= 2^31.
+ * Reading parses pre-order streams on the host (one chunk per call):
+ *   ovx_svo_decode_host writes the ascending codes (at most capacity);
+ *   ovx_svo_parent_decode_host inverts the parent filter into attr [n, channels].
+ * Codes must be distinct within a chunk (repeated codes add no octree byte;
+ * the parent filter leaves their rows zero).
+ */
+ovx_status ovx_svo_encode_begin(ovx_fdg_ctx *ctx, const int32_t *d_codes, const int32_t *d_chunk,
+                                const int64_t *d_chunk_off, int64_t n, int n_chunks, int depth, void *stream,
+                                int64_t *n_bytes);
+ovx_status ovx_svo_encode_finish(ovx_fdg_ctx *ctx, uint8_t *d_svo, int64_t *d_chunk_svo_off, void *stream);
+ovx_status ovx_svo_parent_encode(ovx_fdg_ctx *ctx, const uint8_t *d_attr, int channels, uint8_t *d_delta,
+                                 void *stream);
+ovx_status ovx_svo_neighbor_filter(ovx_fdg_ctx *ctx, const int32_t *d_coords, const int32_t *d_chunk, int64_t n,
+                                   int n_chunks, int res, const uint8_t *d_in, int channels, int decode,
+                                   uint8_t *d_out, void *stream);
+ovx_status ovx_svo_decode_host(const uint8_t *svo, int64_t n_bytes, int depth, int32_t *codes, int64_t capacity,
+                               int64_t *n_codes);
+ovx_status ovx_svo_parent_decode_host(const uint8_t *svo, int64_t n_bytes, int depth, const uint8_t *delta, int64_t n,
+                                      int channels, uint8_t *attr);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,6 +15,7 @@
 #include "fdg_kernels.h"
 #include "dual_mesh.h"
 #include "serialize.h"
+#include "svo.h"
 
 using namespace ovx;
 
@@ -80,6 +81,12 @@ struct ovx_fdg_ctx
     DevBuf dm_table, dm_cnt, dm_off, dm_mask, dm_tiles; // dual grid -> mesh
     DevBuf dm_direct, dm_touched; // direct-mapped voxel table (all zero between calls) and the cells a call set
     int64_t dm_dirty = 0;         // rows in dm_touched whose cells are still set
+    // sparse voxel octrees (.vxz): per-code state of the last encode, neighbour-filter table
+    DevBuf svo_insf, svo_k, svo_pos, svo_tiles, svo_rows, svo_erow, nb_table, nb_own, nb_nb, nb_acc, nb_jmp;
+    SvoCodes svo{};
+    uint64_t svo_total = 0;
+    uint8_t *svo_bytes = nullptr; // the octree written by the last encode finish
+    int svo_state = 0;            // 1 begun, 2 written
     DevBuf vcnt, voff, nbr, erank, bcnt, boff, bedge_v, bedges;
     DevBuf events, recs, chunks, heads, choff, bchunks, active, large, box, qef, bhits, fhits, shard_ctr, rowq, faceq;
     uint64_t per[SB_N] = {};       // per-shard capacities of the sharded buffers (grow-only)
@@ -126,7 +133,9 @@ struct ovx_fdg_ctx
         DevBuf *all[] = {&planes, &bits,   &woff,  &tiles,   &vcnt,   &voff,  &nbr, &erank, &bcnt, &boff,
                          &bedge_v, &bedges, &events, &recs,   &chunks, &heads, &choff, &bchunks, &active, &large,
                          &box,     &qef,    &bhits,  &fhits,  &shard_ctr, &rowq, &faceq, &segbuf,
-                         &dm_table, &dm_cnt, &dm_off, &dm_mask, &dm_tiles, &dm_direct, &dm_touched};
+                         &dm_table, &dm_cnt, &dm_off, &dm_mask, &dm_tiles, &dm_direct, &dm_touched,
+                         &svo_insf, &svo_k, &svo_pos, &svo_tiles, &svo_rows, &svo_erow, &nb_table, &nb_own,
+                         &nb_nb, &nb_acc, &nb_jmp};
         for (DevBuf *b : all)
             fn(*b);
     }
@@ -1239,6 +1248,169 @@ ovx_status ovx_seq_decode(const int32_t *d_codes, int64_t n, int mode, int32_t *
         out.stride[k] = strides[k];
     }
     HIP_TRY(launch_seq_decode(d_codes, n, mode, out, (hipStream_t)stream));
+    return OVX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// sparse voxel octrees and attribute filters of the .vxz format
+// ---------------------------------------------------------------------------
+ovx_status ovx_svo_encode_begin(ovx_fdg_ctx *c, const int32_t *d_codes, const int32_t *d_chunk,
+                                const int64_t *d_chunk_off, int64_t n, int n_chunks, int depth, void *stream,
+                                int64_t *n_bytes)
+{
+    if (!c || !n_bytes || n < 0 || depth < 1 || depth > 10 || n_chunks < 0 || n > (1ll << 31))
+        return fail(OVX_ERR_INVALID_ARGUMENT, "invalid ovx_svo_encode_begin arguments");
+    if (n > 0 && (!d_codes || !d_chunk || !d_chunk_off || n_chunks < 1))
+        return fail(OVX_ERR_INVALID_ARGUMENT, "null device pointer or no chunk");
+    DeviceGuard dg(c->device);
+    std::lock_guard<std::mutex> lk(c->mu);
+    hipStream_t s = (hipStream_t)stream;
+    c->svo_state = 0;
+    c->svo = SvoCodes{d_codes, d_chunk, d_chunk_off, n, n_chunks, depth};
+    *n_bytes = 0;
+    c->svo_total = 0;
+    if (n > 0)
+    {
+        ovx_status st;
+        if ((st = ensure(c->svo_insf, (size_t)n)) || (st = ensure(c->svo_k, (size_t)n * 4)) ||
+            (st = ensure(c->svo_pos, (size_t)n * 4)) || (st = ensure(c->svo_tiles, scan_tile_count((uint64_t)n) * 8)) ||
+            (st = ensure(c->counters, CTR_COUNT * sizeof(unsigned long long))))
+            return st;
+        unsigned long long *ctr = (unsigned long long *)c->counters.p;
+        HIP_TRY(launch_svo_prep(c->svo, (uint8_t *)c->svo_insf.p, (uint32_t *)c->svo_k.p, s));
+        HIP_TRY(launch_scan_u32((const uint32_t *)c->svo_k.p, (uint64_t)n, (uint32_t *)c->svo_pos.p,
+                                (uint64_t *)c->svo_tiles.p, ctr + CTR_SVO_BYTES, s));
+        HIP_TRY(hipMemcpyAsync(c->h_ctr + CTR_SVO_BYTES, ctr + CTR_SVO_BYTES, sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        c->svo_total = c->h_ctr[CTR_SVO_BYTES];
+        if (c->svo_total >= (1ull << 32) - 4)
+            return fail(OVX_ERR_CAPACITY, "octree of %llu bytes exceeds 2^32 - 4", (unsigned long long)c->svo_total);
+        c->last_stream = s;
+        c->ran = true;
+    }
+    *n_bytes = (int64_t)c->svo_total;
+    c->svo_state = 1;
+    return OVX_OK;
+}
+
+ovx_status ovx_svo_encode_finish(ovx_fdg_ctx *c, uint8_t *d_svo, int64_t *d_chunk_svo_off, void *stream)
+{
+    if (!c)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "ctx is null");
+    if (((uintptr_t)d_svo & 3) != 0)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "d_svo must be 4-byte aligned");
+    DeviceGuard dg(c->device);
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (c->svo_state != 1)
+        return fail(OVX_ERR_STATE, "ovx_svo_encode_finish without a successful ovx_svo_encode_begin");
+    hipStream_t s = (hipStream_t)stream;
+    if (c->svo_total > 0)
+    {
+        if (!d_svo)
+            return fail(OVX_ERR_INVALID_ARGUMENT, "null octree buffer");
+        HIP_TRY(hipMemsetAsync(d_svo, 0, (c->svo_total + 3) & ~3ull, s));
+        HIP_TRY(launch_svo_bits(c->svo, (const uint8_t *)c->svo_insf.p, (const uint32_t *)c->svo_pos.p, d_svo, s));
+    }
+    if (d_chunk_svo_off && c->svo.n_chunks > 0)
+        HIP_TRY(launch_svo_chunk_offsets(c->svo, (const uint32_t *)c->svo_pos.p, c->svo_total, d_chunk_svo_off, s));
+    c->svo_bytes = d_svo;
+    c->svo_state = 2;
+    c->last_stream = s;
+    return OVX_OK;
+}
+
+ovx_status ovx_svo_parent_encode(ovx_fdg_ctx *c, const uint8_t *d_attr, int channels, uint8_t *d_delta, void *stream)
+{
+    if (!c || channels < 1)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "invalid ovx_svo_parent_encode arguments");
+    DeviceGuard dg(c->device);
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (c->svo_state != 2)
+        return fail(OVX_ERR_STATE, "ovx_svo_parent_encode needs the octree of a finished encode on this context");
+    const int64_t n = c->svo.n;
+    if (n == 0)
+        return OVX_OK;
+    if (!d_attr || !d_delta)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "null attribute buffer");
+    hipStream_t s = (hipStream_t)stream;
+    ovx_status st;
+    if ((st = ensure(c->svo_rows, c->svo_total * 4)) || (st = ensure(c->svo_erow, c->svo_total * 4)) ||
+        (st = ensure(c->svo_tiles, scan_tile_count(std::max<uint64_t>(c->svo_total, (uint64_t)n)) * 8)))
+        return st;
+    unsigned long long *ctr = (unsigned long long *)c->counters.p;
+    HIP_TRY(launch_svo_rows(c->svo_bytes, c->svo_total, (uint32_t *)c->svo_rows.p, s));
+    HIP_TRY(launch_scan_u32((const uint32_t *)c->svo_rows.p, c->svo_total, (uint32_t *)c->svo_erow.p,
+                            (uint64_t *)c->svo_tiles.p, ctr + CTR_SVO_BYTES + 1, s));
+    HIP_TRY(hipMemsetAsync(d_delta, 0, (size_t)n * channels, s));
+    HIP_TRY(launch_svo_parent_encode(c->svo, (const uint8_t *)c->svo_insf.p, (const uint32_t *)c->svo_pos.p,
+                                     c->svo_bytes, (const uint32_t *)c->svo_erow.p, d_attr, channels, d_delta, s));
+    c->last_stream = s;
+    return OVX_OK;
+}
+
+ovx_status ovx_svo_neighbor_filter(ovx_fdg_ctx *c, const int32_t *d_coords, const int32_t *d_chunk, int64_t n,
+                                   int n_chunks, int res, const uint8_t *d_in, int channels, int decode,
+                                   uint8_t *d_out, void *stream)
+{
+    if (!c || n < 0 || n_chunks < 0 || res < 1 || channels < 1 || n > (1ll << 31) - 2)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "invalid ovx_svo_neighbor_filter arguments");
+    const uint64_t cells = (uint64_t)n_chunks * (uint64_t)res * (uint64_t)res * (uint64_t)res;
+    if (cells > (1ull << 31))
+        return fail(OVX_ERR_CAPACITY, "neighbour filter over %llu cells exceeds 2^31", (unsigned long long)cells);
+    if (n == 0)
+        return OVX_OK;
+    if (!d_coords || !d_chunk || !d_in || !d_out)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "null device pointer");
+    DeviceGuard dg(c->device);
+    std::lock_guard<std::mutex> lk(c->mu);
+    hipStream_t s = (hipStream_t)stream;
+    ovx_status st;
+    if ((st = ensure(c->nb_table, cells * 4)) || (st = ensure(c->nb_own, (size_t)n * 4)) ||
+        (st = ensure(c->nb_nb, (size_t)n * 4)) ||
+        (decode && ((st = ensure(c->nb_acc, (size_t)n * channels * 2)) || (st = ensure(c->nb_jmp, (size_t)n * 8)))))
+        return st;
+    HIP_TRY(hipMemsetAsync(c->nb_table.p, 0, cells * 4, s));
+    NbArgs a{d_coords, d_chunk, n, n_chunks, res, (uint32_t *)c->nb_table.p, (int32_t *)c->nb_own.p,
+             (int32_t *)c->nb_nb.p};
+    HIP_TRY(launch_nb_insert(a, s));
+    HIP_TRY(launch_nb_link(a, s));
+    if (decode)
+    {
+        uint8_t *acc = (uint8_t *)c->nb_acc.p;
+        int32_t *jmp = (int32_t *)c->nb_jmp.p;
+        HIP_TRY(launch_nb_decode(a, d_in, channels, acc, acc + (size_t)n * channels, jmp, jmp + n, d_out, s));
+    }
+    else
+        HIP_TRY(launch_nb_encode(a, d_in, channels, d_out, s));
+    c->last_stream = s;
+    c->ran = true;
+    return OVX_OK;
+}
+
+ovx_status ovx_svo_decode_host(const uint8_t *svo, int64_t n_bytes, int depth, int32_t *codes, int64_t capacity,
+                               int64_t *n_codes)
+{
+    if (!svo || !codes || !n_codes || n_bytes < 1 || depth < 1 || depth > 10 || capacity < 0)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "invalid ovx_svo_decode_host arguments");
+    const int64_t r = svo_decode_host(svo, n_bytes, depth, codes, capacity);
+    if (r == -2)
+        return fail(OVX_ERR_CAPACITY, "octree holds more than %lld leaves", (long long)capacity);
+    if (r < 0)
+        return fail(OVX_ERR_INPUT, "truncated or malformed octree stream");
+    *n_codes = r;
+    return OVX_OK;
+}
+
+ovx_status ovx_svo_parent_decode_host(const uint8_t *svo, int64_t n_bytes, int depth, const uint8_t *delta, int64_t n,
+                                      int channels, uint8_t *attr)
+{
+    if (!svo || !delta || !attr || n_bytes < 1 || depth < 1 || depth > 10 || n < 1 || channels < 1)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "invalid ovx_svo_parent_decode_host arguments");
+    const int64_t r = svo_parent_decode_host(svo, n_bytes, depth, delta, n, channels, attr);
+    if (r != n)
+        return fail(OVX_ERR_INPUT, "octree and delta rows do not match (%lld leaves for %lld rows)", (long long)r,
+                    (long long)n);
     return OVX_OK;
 }
 

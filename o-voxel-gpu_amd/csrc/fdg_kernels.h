@@ -22,6 +22,7 @@ enum
     CTR_LARGE = 10,      // bricks left to the fold kernel for large bricks
     CTR_LARGE_RECS = 11, // their records
     CTR_DM_QUADS = 12,   // dual grid -> mesh: quads (scan total)
+    CTR_SVO_BYTES = 13,  // octree bytes of an encode (scan total)
     // sum and largest shard of each sharded buffer (filled by k_shard_reduce)
     CTR_SUM = 16,        // CTR_SUM + SB_*  : total items requested
     CTR_MAX = 32,        // CTR_MAX + SB_*  : largest per-shard request

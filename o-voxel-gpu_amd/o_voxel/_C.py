@@ -50,6 +50,12 @@ __all__ = [
     "z_order_decode_cpu",
     "hilbert_encode_cpu",
     "hilbert_decode_cpu",
+    "encode_sparse_voxel_octree_cpu",
+    "decode_sparse_voxel_octree_cpu",
+    "encode_sparse_voxel_octree_attr_parent_cpu",
+    "decode_sparse_voxel_octree_attr_parent_cpu",
+    "encode_sparse_voxel_octree_attr_neighbor_cpu",
+    "decode_sparse_voxel_octree_attr_neighbor_cpu",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -122,7 +128,15 @@ def _load():
         i64p, vpp = ctypes.POINTER(i64), ctypes.POINTER(vp)
         L.ovx_seq_encode.argtypes = [vpp, i64p, ci, i64, ci, vp, vp]
         L.ovx_seq_decode.argtypes = [vp, i64, ci, vpp, i64p, vp]
-        for name in ("ovx_seq_encode", "ovx_seq_decode"):
+        L.ovx_svo_encode_begin.argtypes = [vp, vp, vp, vp, i64, ci, ci, vp, i64p]
+        L.ovx_svo_encode_finish.argtypes = [vp, vp, vp, vp]
+        L.ovx_svo_parent_encode.argtypes = [vp, vp, ci, vp, vp]
+        L.ovx_svo_neighbor_filter.argtypes = [vp, vp, vp, i64, ci, ci, vp, ci, ci, vp, vp]
+        L.ovx_svo_decode_host.argtypes = [vp, i64, ci, vp, i64, i64p]
+        L.ovx_svo_parent_decode_host.argtypes = [vp, i64, ci, vp, i64, ci, vp]
+        for name in ("ovx_seq_encode", "ovx_seq_decode", "ovx_svo_encode_begin", "ovx_svo_encode_finish",
+                     "ovx_svo_parent_encode", "ovx_svo_neighbor_filter", "ovx_svo_decode_host",
+                     "ovx_svo_parent_decode_host"):
             getattr(L, name).restype = ctypes.c_int
         for name in ("ovx_dual_mesh_begin", "ovx_dual_mesh_finish", "ovx_hashmap_insert", "ovx_hashmap_lookup",
                      "ovx_hashmap_insert_3d", "ovx_hashmap_lookup_3d", "ovx_hashmap_insert_3d_idx_as_val"):
@@ -150,6 +164,7 @@ class _Ctx:
         _check(L.ovx_fdg_create(device_index, ctypes.byref(h)), "ovx_fdg_create")
         self.h = h
         self.lock = threading.Lock()
+        self.svo_keep = None  # inputs of the last octree encode (the context refers to them)
 
     def __del__(self):
         try:
@@ -660,6 +675,161 @@ def hilbert_encode_cpu(x, y, z):
 
 def hilbert_decode_cpu(codes):
     return _seq_host(hilbert_decode_cuda, codes)
+
+
+# ---------------------------------------------------------------------------
+# sparse voxel octrees of the .vxz format (reference: src/io/svo.cpp,
+# src/io/filter_parent.cpp, src/io/filter_neighbor.cpp, src/io/api.h)
+# ---------------------------------------------------------------------------
+def svo_encode_chunks(codes: torch.Tensor, chunk: torch.Tensor, chunk_off: torch.Tensor, n_chunks: int, depth: int):
+    """Octrees of all chunks at once on the device.  codes int32 [n] (Morton
+    codes of chunk-local coordinates, ascending inside each chunk), chunk
+    int32 [n], chunk_off int64 [n_chunks + 1] device tensors.  Returns
+    (octree bytes uint8 [total] on the device, per-chunk byte offsets as a
+    host int64 tensor [n_chunks + 1]).  The context keeps the codes'
+    octree state for svo_parent_encode."""
+    dev = codes.device
+    n = codes.numel()
+    codes, chunk, chunk_off = codes.contiguous(), chunk.contiguous(), chunk_off.contiguous()
+    ctx = _ctx(dev)
+    L = _load()
+    with torch.cuda.device(dev), ctx.lock:
+        stream = _stream_handle(dev)
+        nb = ctypes.c_int64(0)
+        _check(L.ovx_svo_encode_begin(ctx.h, codes.data_ptr(), chunk.data_ptr(), chunk_off.data_ptr(), n,
+                                      int(n_chunks), int(depth), stream, ctypes.byref(nb)), "svo_encode")
+        total = int(nb.value)
+        buf = torch.empty(((total + 3) // 4) * 4, dtype=torch.uint8, device=dev)
+        offs = torch.empty((int(n_chunks) + 1,), dtype=torch.int64, device=dev)
+        _check(L.ovx_svo_encode_finish(ctx.h, buf.data_ptr() if total else None,
+                                       offs.data_ptr() if n_chunks else None, stream), "svo_encode")
+        ctx.svo_keep = (codes, chunk, chunk_off, buf)  # the context's octree state refers to these
+    return buf[:total], (offs.cpu() if n_chunks else torch.zeros((1,), dtype=torch.int64))
+
+
+def svo_parent_encode(attr: torch.Tensor) -> torch.Tensor:
+    """Parent-filter rows (uint8 [n, C]) of the codes of the last
+    svo_encode_chunks on attr's device, attr uint8 [n, C] in code order."""
+    dev = attr.device
+    attr = attr.contiguous()
+    ctx = _ctx(dev)
+    out = torch.empty_like(attr)
+    with torch.cuda.device(dev), ctx.lock:
+        _check(_load().ovx_svo_parent_encode(ctx.h, attr.data_ptr(), attr.shape[1], out.data_ptr(),
+                                             _stream_handle(dev)), "svo_parent_encode")
+    return out
+
+
+def svo_neighbor_filter(coords: torch.Tensor, chunk: torch.Tensor, n_chunks: int, res: int, data: torch.Tensor,
+                        decode: bool) -> torch.Tensor:
+    """Neighbour filter (or its inverse) of uint8 [n, C] data over chunk-local
+    int32 [n, 3] coordinates in [0, res) and int32 chunk ids, on the device."""
+    dev = coords.device
+    coords, chunk, data = coords.contiguous(), chunk.contiguous(), data.contiguous()
+    out = torch.empty_like(data)
+    if data.numel() == 0:
+        return out
+    ctx = _ctx(dev)
+    with torch.cuda.device(dev), ctx.lock:
+        _check(_load().ovx_svo_neighbor_filter(ctx.h, coords.data_ptr(), chunk.data_ptr(), coords.shape[0],
+                                               int(n_chunks), int(res), data.data_ptr(), data.shape[1],
+                                               int(bool(decode)), out.data_ptr(), _stream_handle(dev)),
+               "svo_neighbor_filter")
+    return out
+
+
+def svo_decode_host(octree, depth: int):
+    """Ascending codes (numpy int32) of one chunk's octree bytes (host parse)."""
+    import numpy as np
+
+    o = np.ascontiguousarray(np.asarray(octree, dtype=np.uint8))
+    cap = 8 * max(1, o.size)
+    codes = np.empty(cap, np.int32)
+    n = ctypes.c_int64(0)
+    _check(_load().ovx_svo_decode_host(o.ctypes.data, o.size, int(depth), codes.ctypes.data, cap, ctypes.byref(n)),
+           "svo_decode")
+    return codes[:int(n.value)]
+
+
+def svo_parent_decode_host(octree, depth: int, delta):
+    """Attributes (numpy uint8 [n, C]) of one chunk from its octree and
+    parent-filter rows (host parse)."""
+    import numpy as np
+
+    o = np.ascontiguousarray(np.asarray(octree, dtype=np.uint8))
+    d = np.ascontiguousarray(np.asarray(delta, dtype=np.uint8))
+    out = np.empty_like(d)
+    _check(_load().ovx_svo_parent_decode_host(o.ctypes.data, o.size, int(depth), d.ctypes.data, d.shape[0],
+                                              d.shape[1], out.ctypes.data), "svo_parent_decode")
+    return out
+
+
+def _single_chunk(n: int, dev):
+    return (torch.zeros((n,), dtype=torch.int32, device=dev),
+            torch.tensor([0, n], dtype=torch.int64, device=dev))
+
+
+def _check_codes(codes: torch.Tensor, depth: int):
+    if not 1 <= int(depth) <= 10:
+        raise RuntimeError(f"depth must be in [1, 10], got {depth}")
+    if codes.numel():
+        c = codes.long()
+        if bool((c < 0).any()) or bool((c >= (1 << (3 * int(depth)))).any()):
+            raise RuntimeError(f"codes must lie in [0, 2^{3 * int(depth)})")
+        if bool((c[1:] < c[:-1]).any()):
+            raise RuntimeError("codes must be sorted in ascending order")
+
+
+def encode_sparse_voxel_octree_cpu(codes: torch.Tensor, depth: int) -> torch.Tensor:
+    """Octree bytes of ascending Morton codes (reference src/io/svo.cpp:17-73);
+    computed on the current GPU."""
+    dev = _gpu_for_host_call()
+    c = codes.to(device=dev, dtype=torch.int32).reshape(-1)
+    _check_codes(c, depth)
+    if c.numel() == 0:
+        return torch.zeros((1,), dtype=torch.uint8)  # the root alone, as the reference writes it
+    chunk, off = _single_chunk(c.numel(), dev)
+    svo, _ = svo_encode_chunks(c, chunk, off, 1, int(depth))
+    return svo.cpu()
+
+
+def decode_sparse_voxel_octree_cpu(octree: torch.Tensor, depth: int) -> torch.Tensor:
+    """Ascending codes of an octree (reference src/io/svo.cpp:76-137)."""
+    return torch.from_numpy(svo_decode_host(octree.cpu().numpy(), depth).copy())
+
+
+def encode_sparse_voxel_octree_attr_parent_cpu(octree: torch.Tensor, depth: int, attr: torch.Tensor) -> torch.Tensor:
+    """Parent-filter rows (reference src/io/filter_parent.cpp:10-87)."""
+    dev = _gpu_for_host_call()
+    codes = torch.from_numpy(svo_decode_host(octree.cpu().numpy(), depth).copy()).to(dev)
+    a = attr.to(device=dev, dtype=torch.uint8)
+    if a.shape[0] != codes.numel():
+        raise RuntimeError(f"attr has {a.shape[0]} rows for {codes.numel()} octree leaves")
+    chunk, off = _single_chunk(codes.numel(), dev)
+    svo_encode_chunks(codes, chunk, off, 1, int(depth))
+    return svo_parent_encode(a).cpu()
+
+
+def decode_sparse_voxel_octree_attr_parent_cpu(octree: torch.Tensor, depth: int, delta: torch.Tensor) -> torch.Tensor:
+    """Inverse of the parent filter (reference src/io/filter_parent.cpp:90-164)."""
+    return torch.from_numpy(svo_parent_decode_host(octree.cpu().numpy(), depth, delta.cpu().numpy()))
+
+
+def _neighbor_cpu(coord, res, data, decode):
+    dev = _gpu_for_host_call()
+    c = coord.to(device=dev, dtype=torch.int32).reshape(-1, 3)
+    chunk, _ = _single_chunk(c.shape[0], dev)
+    return svo_neighbor_filter(c, chunk, 1, int(res), data.to(device=dev, dtype=torch.uint8), decode).cpu()
+
+
+def encode_sparse_voxel_octree_attr_neighbor_cpu(coord: torch.Tensor, res: int, attr: torch.Tensor) -> torch.Tensor:
+    """Neighbour-filter rows (reference src/io/filter_neighbor.cpp:18-99)."""
+    return _neighbor_cpu(coord, res, attr, False)
+
+
+def decode_sparse_voxel_octree_attr_neighbor_cpu(coord: torch.Tensor, res: int, delta: torch.Tensor) -> torch.Tensor:
+    """Inverse of the neighbour filter (reference src/io/filter_neighbor.cpp:110-178)."""
+    return _neighbor_cpu(coord, res, delta, True)
 
 
 def _gpu_for_host_call() -> torch.device:

@@ -2,8 +2,9 @@
 
 Provided: ``o_voxel.convert`` (mesh_to_flexible_dual_grid, its batched form,
 intersect_occ, flexible_dual_grid_to_mesh), ``o_voxel.serialize``
-(encode_seq / decode_seq) and the ``o_voxel._C`` boundary they call.
+(encode_seq / decode_seq), ``o_voxel.io`` (the .vxz format, .npz) and the
+``o_voxel._C`` boundary they call.
 """
-from . import _C, convert, serialize
+from . import _C, convert, io, serialize
 
-__all__ = ["convert", "serialize", "_C"]
+__all__ = ["convert", "io", "serialize", "_C"]
