@@ -6,8 +6,8 @@ One step = one call of the public drop-in API
 resident in HBM (float32 vertices, int64 faces, as a caller passes them),
 including the wrapper's dtype cast and aabb shift.  Prints ONE JSON line.
 
-  python bench.py [--steps K] [--warmup W] [--workload scene|scene_large|ico_cap|scene_rankdef|batch|to_mesh]
-                  [--grid G]
+  python bench.py [--steps K] [--warmup W]
+                  [--workload scene|scene_large|ico_cap|scene_rankdef|batch|to_mesh|serialize|vxz] [--grid G]
 
 Default workload: the ~280k-face seeded scene (closed icospheres/tori, open
 patches, 1% slivers) at grid 512, weights face 1.0 / boundary 0.2 /
@@ -181,6 +181,8 @@ def main():
         return run_batch(args, torch, o_voxel, _C, dev)
     if args.workload == "to_mesh":
         return run_to_mesh(args, torch, o_voxel, _C, dev)
+    if args.workload in ("serialize", "vxz"):
+        return run_voxel_io(args, torch, o_voxel, _C, dev)
     v, f, grid, seed = make_workload(args.workload, args.grid)
     F, V = int(f.shape[0]), int(v.shape[0])
     log(f"[bench] workload={args.workload} F={F} V={V} grid={grid}")
@@ -583,6 +585,120 @@ def run_to_mesh(args, torch, o_voxel, _C, dev):
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    print(json.dumps(line), flush=True)
+
+
+def _scene_voxels(torch, o_voxel, dev, grid):
+    """The conversion of the 268k-face scene: coords, duals, flags."""
+    import meshgen
+
+    v, f = meshgen.scene(seed=2)
+    return o_voxel.convert.mesh_to_flexible_dual_grid(
+        torch.from_numpy(v).to(dev), torch.from_numpy(f).to(dev), grid_size=grid, aabb=[[-0.5] * 3, [0.5] * 3],
+        face_weight=WEIGHTS[0], boundary_weight=WEIGHTS[1], regularization_weight=WEIGHTS[2])
+
+
+def run_voxel_io(args, torch, o_voxel, _C, dev):
+    """serialize: Z-order and Hilbert encode + decode of the scene
+    conversion's voxels (grid 512).  vxz: the reference example's O-voxel
+    file of that conversion (Morton-sorted voxels, duals quantised to uint8,
+    flags packed to 3 bits, parent filter) written and read back through
+    o_voxel.io, uncompressed and deflate; CPU oracle beside each."""
+    import io as _io
+
+    grid = args.grid or 512
+    coords, dual, inter = _scene_voxels(torch, o_voxel, dev, grid)
+    N = int(coords.shape[0])
+
+    def timed(fn, steps):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            out = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps, out
+
+    line = {"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "data": "synthetic"}
+    if args.workload == "serialize":
+        from oracle import serialize as S
+
+        res = {}
+        for mode in ("z_order", "hilbert"):
+            enc_ms, codes = timed(lambda: o_voxel.serialize.encode_seq(coords, mode=mode), args.steps)
+            dec_ms, back = timed(lambda: o_voxel.serialize.decode_seq(codes, mode=mode), args.steps)
+            c_np = coords.cpu().numpy()
+            t0 = time.perf_counter()
+            ref = (S.z_order_encode if mode == "z_order" else S.hilbert_encode)(c_np[:, 0], c_np[:, 1], c_np[:, 2])
+            cpu_s = time.perf_counter() - t0
+            res[mode] = {"encode_ms": round(enc_ms, 4), "decode_ms": round(dec_ms, 4),
+                         "encode_M_voxels_per_s": round(N / enc_ms / 1e3, 1),
+                         "codes_identical_to_oracle": bool(np.array_equal(codes.cpu().numpy(), ref)),
+                         "decode_round_trip": bool(torch.equal(back, coords)),
+                         "cpu_oracle_encode_s": round(cpu_s, 3)}
+        hb = res["hilbert"]
+        alg = 16 * N  # 12 B of coordinates read + 4 B of code written per voxel
+        achieved = alg / (hb["encode_ms"] * 1e-3) / 1e9
+        line.update({
+            "metric": "voxel codes M voxels/s (Hilbert encode)", "value": hb["encode_M_voxels_per_s"],
+            "unit": "M voxels/s", "ms_per_step": hb["encode_ms"], "dtype": "int32",
+            "config": {"workload": f"serialize-scene268k@{grid}", "voxels": N, "grid": grid, "seed": 2},
+            "modes": res,
+            "roofline": {"kernel": "k_seq_encode<hilbert>", "bound": "hbm", "achieved": round(achieved, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "traffic": None, "algorithmic_bytes_per_launch": alg,
+                         "note": "whole call (one kernel + wrapper), not the kernel alone"},
+            "cpu_baseline": {"value": round(N / res["hilbert"]["cpu_oracle_encode_s"] / 1e6, 3),
+                             "unit": "M voxels/s", "cores": 1, "kind": "port",
+                             "sample": f"numpy oracle, all {N} voxels (Hilbert encode)"},
+        })
+        print(json.dumps(line), flush=True)
+        return
+    # vxz
+    from oracle import vxz as X
+
+    order = torch.argsort(o_voxel.serialize.encode_seq(coords))
+    coords, dual, inter = coords[order], dual[order], inter[order]
+    attr = {"dual_vertices": (torch.clamp(dual * grid - coords, 0, 1) * 255).type(torch.uint8),
+            "intersected": (inter[:, 0:1] + 2 * inter[:, 1:2] + 4 * inter[:, 2:3]).type(torch.uint8)}
+    res = {}
+    for comp in ("none", "deflate"):
+        def write():
+            b = _io.BytesIO()
+            o_voxel.io.write_vxz(b, coords, attr, filter="parent", compression=comp)
+            return b.getvalue()
+
+        steps = max(1, args.steps // 4) if comp == "deflate" else args.steps
+        w_ms, data = timed(write, steps)
+        r_ms, (c2, a2) = timed(lambda: o_voxel.io.read_vxz(_io.BytesIO(data)), steps)
+        same = bool(torch.equal(c2, coords.cpu())) and all(torch.equal(a2[k], attr[k].cpu()) for k in attr)
+        res[comp] = {"write_ms": round(w_ms, 3), "read_ms": round(r_ms, 3), "bytes": len(data),
+                     "write_M_voxels_per_s": round(N / w_ms / 1e3, 2), "read_M_voxels_per_s": round(N / r_ms / 1e3, 2),
+                     "round_trip_exact": same}
+    # CPU oracle writer on one chunk-sized sample (voxels of the lowest Morton codes)
+    n_s = min(N, 60000)
+    c_np = coords[:n_s].cpu().numpy()
+    a_np = {k: v[:n_s].cpu().numpy() for k, v in attr.items()}
+    t0 = time.perf_counter()
+    ref = X.write_vxz_bytes(c_np, a_np, filter="parent", compression="none")
+    cpu_s = time.perf_counter() - t0
+    b = _io.BytesIO()
+    o_voxel.io.write_vxz(b, coords[:n_s], {k: v[:n_s] for k, v in attr.items()}, filter="parent",
+                         compression="none")
+    un = res["none"]
+    line.update({
+        "metric": "O-voxel .vxz write M voxels/s (parent filter, uncompressed)", "value": un["write_M_voxels_per_s"],
+        "unit": "M voxels/s", "ms_per_step": un["write_ms"], "dtype": "u8",
+        "config": {"workload": f"vxz-scene268k@{grid}", "voxels": N, "grid": grid, "chunk_size": 256, "seed": 2},
+        "compression": res,
+        "sample_bytes_identical_to_oracle": b.getvalue() == ref,
+        "roofline": None,
+        "cpu_baseline": {"value": round(n_s / cpu_s / 1e6, 4), "unit": "M voxels/s", "cores": 1, "kind": "port",
+                         "sample": f"Python oracle writer on the first {n_s} Morton-ordered voxels, uncompressed, "
+                                   f"{cpu_s:.2f} s"},
+    })
     print(json.dumps(line), flush=True)
 
 

@@ -108,7 +108,7 @@ def read_vxz(file, num_threads: int = -1):
     data = _read_all(file)
     num_threads = num_threads if num_threads > 0 else os.cpu_count()
     info, start = _parse_header(data)
-    bin_data = data[start:]
+    bin_data = memoryview(data)[start:]  # slices below are views, not copies
     cs = info["chunk_size"]
     depth = np.log2(cs)
     assert depth.is_integer(), f"Chunk size must be a power of 2, got {cs}"
@@ -252,19 +252,22 @@ def write_vxz(
                 ch += chs
         else:
             streams["attr"] = _compress(ak.tobytes(), compression, compression_level)
-        ci, data = {"idx": chunk_xyz[k]}, b""
+        ci, parts, size = {"idx": chunk_xyz[k]}, [], 0
         for name, v in streams.items():
-            ci[name] = [len(data), len(v)]
-            data += v
-        return ci, data
+            ci[name] = [size, len(v)]
+            parts.append(v)
+            size += len(v)
+        return ci, parts, size
 
     with ThreadPoolExecutor(max_workers=num_threads) as ex:
         chunks = list(ex.map(worker, range(n_chunks)))
-    bin_data = b""
-    for ci, data in chunks:
-        ci["ptr"] = [len(bin_data), len(data)]
-        bin_data += data
+    pieces, total = [], 0
+    for ci, parts, size in chunks:
+        ci["ptr"] = [total, size]
+        pieces += parts
+        total += size
         info["chunks"].append(ci)
+    bin_data = b"".join(pieces)
     info["compressed_size"] = len(bin_data)
     info["compress_ratio"] = info["raw_size"] / info["compressed_size"]
     structure = json.dumps(info).encode()
