@@ -105,7 +105,9 @@ struct ovx_fdg_ctx
     // device copy (segbuf) and the pinned staging of the upload / readback
     int M = 0;
     std::vector<int64_t> face_off, vert_off;
-    std::vector<uint64_t> word_off, brick_off, mtile_off, ftile_off;
+    std::vector<uint64_t> word_off, brick_off, mtile_off;
+    std::vector<uint64_t> ftile_off[N_FACE_TILES]; // face-test tiles per mesh, per tile size
+    const uint64_t *d_ftile_off[N_FACE_TILES] = {};
     std::vector<MeshSeg> segs;
     DevBuf segbuf;
     unsigned char *h_stage = nullptr;
@@ -321,7 +323,8 @@ ovx_status setup_batch(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *f
     c->word_off.assign(M + 1, 0);
     c->brick_off.assign(M + 1, 0);
     c->mtile_off.assign(M + 1, 0);
-    c->ftile_off.assign(M + 1, 0);
+    for (auto &v : c->ftile_off)
+        v.assign(M + 1, 0);
     c->segs.assign(M, MeshSeg{});
     for (int i = 0; i < M; ++i)
     {
@@ -342,7 +345,8 @@ ovx_status setup_batch(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *f
         c->word_off[i + 1] = c->word_off[i] + nw;
         c->brick_off[i + 1] = c->brick_off[i] + (uint64_t)sg.nb[0] * (uint64_t)sg.nb[1] * (uint64_t)sg.nb[2];
         c->mtile_off[i + 1] = c->mtile_off[i] + (uint64_t)(3 * nf + BT_HOST - 1) / BT_HOST;
-        c->ftile_off[i + 1] = c->ftile_off[i] + (uint64_t)(nf + BT_HOST - 1) / BT_HOST;
+        for (int t = 0; t < N_FACE_TILES; ++t)
+            c->ftile_off[t][i + 1] = c->ftile_off[t][i] + (uint64_t)(nf + FACE_TILES[t] - 1) / FACE_TILES[t];
     }
     c->n_words = c->word_off[M];
     c->n_bricks = c->brick_off[M];
@@ -358,10 +362,10 @@ ovx_status setup_batch(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *f
         c->segs[i].g.bits = (uint64_t *)c->bits.p + c->word_off[i];
         c->segs[i].g.woff = (const uint32_t *)c->woff.p + c->word_off[i];
     }
-    // device layout: segments, then 7 offset arrays of M + 1 entries
+    // device layout: segments, then 6 + N_FACE_TILES offset arrays of M + 1 entries
     const size_t seg_bytes = ((sizeof(MeshSeg) * (size_t)M + 255) / 256) * 256;
     const size_t arr = (size_t)(M + 1) * 8;
-    const size_t total = seg_bytes + 7 * arr;
+    const size_t total = seg_bytes + (6 + N_FACE_TILES) * arr;
     if (c->h_stage_bytes < total + arr)
     {
         if (c->ran)
@@ -383,7 +387,8 @@ ovx_status setup_batch(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *f
     std::memcpy(h + seg_bytes + 3 * arr, c->brick_off.data(), arr);
     std::memset(h + seg_bytes + 4 * arr, 0, arr); // voxel offsets: written on the device
     std::memcpy(h + seg_bytes + 5 * arr, c->mtile_off.data(), arr);
-    std::memcpy(h + seg_bytes + 6 * arr, c->ftile_off.data(), arr);
+    for (int t = 0; t < N_FACE_TILES; ++t)
+        std::memcpy(h + seg_bytes + (6 + t) * arr, c->ftile_off[t].data(), arr);
     c->h_vox_off = (uint64_t *)(h + total);
     HIP_TRY(hipMemcpyAsync(c->segbuf.p, h, total, hipMemcpyHostToDevice, s));
     unsigned char *d = (unsigned char *)c->segbuf.p;
@@ -395,7 +400,9 @@ ovx_status setup_batch(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *f
     b.brick_off = (const uint64_t *)(d + seg_bytes + 3 * arr);
     b.vox_off = (const uint64_t *)(d + seg_bytes + 4 * arr);
     b.mtile_off = (const uint64_t *)(d + seg_bytes + 5 * arr);
-    b.ftile_off = (const uint64_t *)(d + seg_bytes + 6 * arr);
+    for (int t = 0; t < N_FACE_TILES; ++t)
+        c->d_ftile_off[t] = (const uint64_t *)(d + seg_bytes + (6 + t) * arr);
+    b.ftile_off = c->d_ftile_off[0];
     b.M = M;
     return OVX_OK;
 }
@@ -612,7 +619,14 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
             ShardBuf<FaceChunk> fq = shard_view<FaceChunk>(c, c->faceq, SB_FACEQ);
             if (c->M > 1)
                 fq.items = nullptr; // row chunks are for single-mesh conversions
-            HIP_TRY(launch_face(m, c->bdev, c->ftile_off[c->M], fb, ctr, fq, s));
+            // tile size from the voxels per face: faces covering many voxels
+            // take small tiles, sub-voxel faces larger ones
+            const int64_t F = c->face_off[c->M];
+            const double vpf = F > 0 ? (double)c->N / (double)F : 0.0;
+            const int tile = vpf >= 4.0 ? 0 : (vpf >= 1.0 ? 1 : 2);
+            BatchDev bf = c->bdev;
+            bf.ftile_off = c->d_ftile_off[tile];
+            HIP_TRY(launch_face(m, bf, tile, c->ftile_off[tile][c->M], fb, ctr, fq, s));
             toc(c, KS_FACE, s);
             tic(c, KS_GEN_FHITS, s);
             HIP_TRY(launch_gen_fhits(fb, ba, s));

@@ -9,6 +9,12 @@
 namespace ovx
 {
 
+// faces per tile of the face-test kernel (a block of BT threads takes one
+// tile): small tiles when faces cover many voxels (more, shorter blocks),
+// larger ones for sub-voxel faces (less per-block overhead)
+constexpr int N_FACE_TILES = 3;
+constexpr int FACE_TILES[N_FACE_TILES] = {32, 64, 128};
+
 // device counters (unsigned long long each)
 enum
 {
@@ -150,7 +156,7 @@ struct BatchDev
     const uint64_t *brick_off; // bricks
     const uint64_t *vox_off;   // voxels (written on the device after the word scan)
     const uint64_t *mtile_off; // block tiles of scanline items (ceil(3 F_m / block))
-    const uint64_t *ftile_off; // block tiles of faces (ceil(F_m / block))
+    const uint64_t *ftile_off; // face-test tiles (ceil(F_m / tile)) for the tile size of the launch
     int M;
 };
 
@@ -229,7 +235,8 @@ struct FoldArgs
 hipError_t launch_prep(const MeshDev &m, const BatchDev &b, float4 *planes, unsigned long long *ctr, hipStream_t s);
 hipError_t launch_scan_events(const MeshDev &m, const BatchDev &b, uint64_t n_tiles, unsigned long long *ctr,
                               const ShardBuf<Event> &eb, const ShardBuf<RowChunk> &rq, bool merge, hipStream_t s);
-hipError_t launch_face(const MeshDev &m, const BatchDev &b, uint64_t n_tiles, const ShardBuf<FHit> &hb,
+// tile: index into FACE_TILES; b.ftile_off must hold that tile size's offsets
+hipError_t launch_face(const MeshDev &m, const BatchDev &b, int tile, uint64_t n_tiles, const ShardBuf<FHit> &hb,
                        unsigned long long *ctr, const ShardBuf<FaceChunk> &fq, hipStream_t s);
 hipError_t launch_gen_events(const ShardBuf<Event> &ev, const BucketArgs &ba, hipStream_t s);
 hipError_t launch_gen_fhits(const ShardBuf<FHit> &h, const BucketArgs &ba, hipStream_t s);

@@ -956,13 +956,14 @@ __device__ __forceinline__ uint64_t row_mask(int lo_bit, int hi_bit)
 
 // QUEUED = false: one item per face; QUEUED = true: the queued row chunks of
 // large faces (items index the queue's slots)
-template <bool QUEUED>
+template <bool QUEUED, int FACE_TILE>
 __global__ __launch_bounds__(BT) void k_face(MeshDev m, BatchDev bt, uint64_t n_tiles, ShardBuf<FHit> hb,
                                              unsigned long long *ctr, ShardBuf<FaceChunk> fq)
 {
-    __shared__ FaceItem items[BT];
+    static_assert(FACE_TILE <= BT, "a tile's faces are set up one per thread");
+    __shared__ FaceItem items[FACE_TILE];
     __shared__ FaceRow rows[BT];
-    __shared__ uint64_t incl_items[BT];
+    __shared__ uint64_t incl_items[FACE_TILE];
     __shared__ uint32_t incl_rows[BT];
     __shared__ uint64_t sh64[NWAVE];
     __shared__ uint32_t sh32[NWAVE];
@@ -973,10 +974,10 @@ __global__ __launch_bounds__(BT) void k_face(MeshDev m, BatchDev bt, uint64_t n_
     if constexpr (QUEUED)
     {
         n_items = (int64_t)queue_prefix(fq, qm.pre, qm.sh);
-        n_tiles = (uint64_t)(n_items + BT - 1) / BT;
+        n_tiles = (uint64_t)(n_items + FACE_TILE - 1) / FACE_TILE;
     }
 
-    // a block takes tiles of BT faces of one mesh (queued row chunks exist
+    // a block takes tiles of FACE_TILE faces of one mesh (queued row chunks exist
     // for single-mesh conversions only)
     for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x)
     {
@@ -988,14 +989,14 @@ __global__ __launch_bounds__(BT) void k_face(MeshDev m, BatchDev bt, uint64_t n_
         if constexpr (!QUEUED)
         {
             const int64_t f0 = bt.face_off[ms];
-            const int64_t loc = (int64_t)(tile - bt.ftile_off[ms]) * BT + tid;
+            const int64_t loc = (int64_t)(tile - bt.ftile_off[ms]) * FACE_TILE + tid;
             item = f0 + loc;
-            live = loc < bt.face_off[ms + 1] - f0;
+            live = tid < FACE_TILE && loc < bt.face_off[ms + 1] - f0;
         }
         else
         {
-            item = (int64_t)tile * BT + tid;
-            live = item < n_items;
+            item = (int64_t)tile * FACE_TILE + tid;
+            live = tid < FACE_TILE && item < n_items;
         }
         uint64_t R = 0;
         if constexpr (!QUEUED)
@@ -1046,7 +1047,8 @@ __global__ __launch_bounds__(BT) void k_face(MeshDev m, BatchDev bt, uint64_t n_
         }
         uint64_t tot_r;
         const uint64_t ex_r = block_excl_scan(R, sh64, tot_r);
-        incl_items[tid] = ex_r + R;
+        if (tid < FACE_TILE)
+            incl_items[tid] = ex_r + R;
         __syncthreads();
 
         for (uint64_t rb = 0; rb < tot_r; rb += BT)
@@ -1055,7 +1057,7 @@ __global__ __launch_bounds__(BT) void k_face(MeshDev m, BatchDev bt, uint64_t n_
             uint32_t A = 0;
             if (gi < tot_r)
             {
-                const int L = upper_find(incl_items, BT, gi);
+                const int L = upper_find(incl_items, FACE_TILE, gi);
                 const FaceItem &it = items[L];
                 const uint64_t R_L = incl_items[L] - (L ? incl_items[L - 1] : 0ull);
                 const uint64_t rr = (uint64_t)it.r0 + (gi - (incl_items[L] - R_L));
@@ -2708,7 +2710,16 @@ hipError_t launch_scan_events(const MeshDev &m, const BatchDev &b, uint64_t n_ti
     return hipGetLastError();
 }
 
-hipError_t launch_face(const MeshDev &m, const BatchDev &b, uint64_t n_tiles, const ShardBuf<FHit> &hb,
+template <int TILE>
+static void face_pair(const MeshDev &m, const BatchDev &b, unsigned gf, uint64_t n_tiles, const ShardBuf<FHit> &hb,
+                      unsigned long long *ctr, const ShardBuf<FaceChunk> &fq, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_face<false, TILE>), dim3(gf), dim3(BT), 0, s, m, b, n_tiles, hb, ctr, fq);
+    // loops over the filled slots of the row-chunk queue
+    hipLaunchKernelGGL((k_face<true, TILE>), dim3(2048), dim3(BT), 0, s, m, b, 0ull, hb, ctr, fq);
+}
+
+hipError_t launch_face(const MeshDev &m, const BatchDev &b, int tile, uint64_t n_tiles, const ShardBuf<FHit> &hb,
                        unsigned long long *ctr, const ShardBuf<FaceChunk> &fq, hipStream_t s)
 {
     const unsigned gf = (unsigned)(n_tiles < (1ull << 20) ? (n_tiles ? n_tiles : 1) : (1ull << 20));
@@ -2717,8 +2728,13 @@ hipError_t launch_face(const MeshDev &m, const BatchDev &b, uint64_t n_tiles, co
     hipError_t e = hipMemsetAsync(fq.cnt, 0, (size_t)NSHARD * SHARD_STRIDE * sizeof(unsigned long long), s);
     if (e != hipSuccess)
         return e;
-    hipLaunchKernelGGL(k_face<false>, dim3(gf), dim3(BT), 0, s, m, b, n_tiles, hb, ctr, fq);
-    hipLaunchKernelGGL(k_face<true>, dim3(2048), dim3(BT), 0, s, m, b, 0ull, hb, ctr, fq); // loops over the filled slots
+    static_assert(FACE_TILES[0] == 32 && FACE_TILES[1] == 64 && FACE_TILES[2] == 128, "face tile instantiations");
+    if (tile == 0)
+        face_pair<32>(m, b, gf, n_tiles, hb, ctr, fq, s);
+    else if (tile == 1)
+        face_pair<64>(m, b, gf, n_tiles, hb, ctr, fq, s);
+    else
+        face_pair<128>(m, b, gf, n_tiles, hb, ctr, fq, s);
     return hipGetLastError();
 }
 
