@@ -19,7 +19,6 @@
 
 using namespace ovx;
 
-constexpr int BT_HOST = 256; // threads per block of the tiled kernels (scanline items, face test)
 
 namespace
 {
@@ -105,7 +104,10 @@ struct ovx_fdg_ctx
     // device copy (segbuf) and the pinned staging of the upload / readback
     int M = 0;
     std::vector<int64_t> face_off, vert_off;
-    std::vector<uint64_t> word_off, brick_off, mtile_off;
+    std::vector<uint64_t> word_off, brick_off;
+    std::vector<uint64_t> mtile_off[N_MARK_TILES]; // scanline tiles per mesh, per tile size
+    const uint64_t *d_mtile_off[N_MARK_TILES] = {};
+    int mark_tile = 2;                             // tile size of the current conversion
     std::vector<uint64_t> ftile_off[N_FACE_TILES]; // face-test tiles per mesh, per tile size
     const uint64_t *d_ftile_off[N_FACE_TILES] = {};
     std::vector<MeshSeg> segs;
@@ -322,7 +324,8 @@ ovx_status setup_batch(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *f
     c->vert_off.assign(vert_off, vert_off + M + 1);
     c->word_off.assign(M + 1, 0);
     c->brick_off.assign(M + 1, 0);
-    c->mtile_off.assign(M + 1, 0);
+    for (auto &v : c->mtile_off)
+        v.assign(M + 1, 0);
     for (auto &v : c->ftile_off)
         v.assign(M + 1, 0);
     c->segs.assign(M, MeshSeg{});
@@ -344,7 +347,8 @@ ovx_status setup_batch(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *f
         const int64_t nf = face_off[i + 1] - face_off[i];
         c->word_off[i + 1] = c->word_off[i] + nw;
         c->brick_off[i + 1] = c->brick_off[i] + (uint64_t)sg.nb[0] * (uint64_t)sg.nb[1] * (uint64_t)sg.nb[2];
-        c->mtile_off[i + 1] = c->mtile_off[i] + (uint64_t)(3 * nf + BT_HOST - 1) / BT_HOST;
+        for (int t = 0; t < N_MARK_TILES; ++t)
+            c->mtile_off[t][i + 1] = c->mtile_off[t][i] + (uint64_t)(3 * nf + MARK_TILES[t] - 1) / MARK_TILES[t];
         for (int t = 0; t < N_FACE_TILES; ++t)
             c->ftile_off[t][i + 1] = c->ftile_off[t][i] + (uint64_t)(nf + FACE_TILES[t] - 1) / FACE_TILES[t];
     }
@@ -362,10 +366,11 @@ ovx_status setup_batch(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *f
         c->segs[i].g.bits = (uint64_t *)c->bits.p + c->word_off[i];
         c->segs[i].g.woff = (const uint32_t *)c->woff.p + c->word_off[i];
     }
-    // device layout: segments, then 6 + N_FACE_TILES offset arrays of M + 1 entries
+    // device layout: segments, then 5 + N_MARK_TILES + N_FACE_TILES offset arrays of M + 1 entries
     const size_t seg_bytes = ((sizeof(MeshSeg) * (size_t)M + 255) / 256) * 256;
     const size_t arr = (size_t)(M + 1) * 8;
-    const size_t total = seg_bytes + (6 + N_FACE_TILES) * arr;
+    constexpr int FT0 = 5 + N_MARK_TILES; // first face-tile array
+    const size_t total = seg_bytes + (FT0 + N_FACE_TILES) * arr;
     if (c->h_stage_bytes < total + arr)
     {
         if (c->ran)
@@ -386,9 +391,10 @@ ovx_status setup_batch(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *f
     std::memcpy(h + seg_bytes + 2 * arr, c->word_off.data(), arr);
     std::memcpy(h + seg_bytes + 3 * arr, c->brick_off.data(), arr);
     std::memset(h + seg_bytes + 4 * arr, 0, arr); // voxel offsets: written on the device
-    std::memcpy(h + seg_bytes + 5 * arr, c->mtile_off.data(), arr);
+    for (int t = 0; t < N_MARK_TILES; ++t)
+        std::memcpy(h + seg_bytes + (5 + t) * arr, c->mtile_off[t].data(), arr);
     for (int t = 0; t < N_FACE_TILES; ++t)
-        std::memcpy(h + seg_bytes + (6 + t) * arr, c->ftile_off[t].data(), arr);
+        std::memcpy(h + seg_bytes + (FT0 + t) * arr, c->ftile_off[t].data(), arr);
     c->h_vox_off = (uint64_t *)(h + total);
     HIP_TRY(hipMemcpyAsync(c->segbuf.p, h, total, hipMemcpyHostToDevice, s));
     unsigned char *d = (unsigned char *)c->segbuf.p;
@@ -399,9 +405,21 @@ ovx_status setup_batch(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *f
     b.word_off = (const uint64_t *)(d + seg_bytes + 2 * arr);
     b.brick_off = (const uint64_t *)(d + seg_bytes + 3 * arr);
     b.vox_off = (const uint64_t *)(d + seg_bytes + 4 * arr);
-    b.mtile_off = (const uint64_t *)(d + seg_bytes + 5 * arr);
+    for (int t = 0; t < N_MARK_TILES; ++t)
+        c->d_mtile_off[t] = (const uint64_t *)(d + seg_bytes + (5 + t) * arr);
     for (int t = 0; t < N_FACE_TILES; ++t)
-        c->d_ftile_off[t] = (const uint64_t *)(d + seg_bytes + (6 + t) * arr);
+        c->d_ftile_off[t] = (const uint64_t *)(d + seg_bytes + (FT0 + t) * arr);
+    // scanline tile size from faces per grid cross-section: meshes of few,
+    // large triangles take small tiles, meshes of many small ones large tiles
+    double cross = 0.0;
+    for (int i = 0; i < M; ++i)
+    {
+        const GridDev &g = c->segs[i].g;
+        cross += std::max({(double)g.G[0] * g.G[1], (double)g.G[1] * g.G[2], (double)g.G[0] * g.G[2]});
+    }
+    const double fpc = cross > 0.0 ? (double)face_off[M] / cross : 0.0;
+    c->mark_tile = fpc >= 8.0 ? 2 : (fpc >= 0.5 ? 1 : 0);
+    b.mtile_off = c->d_mtile_off[c->mark_tile];
     b.ftile_off = c->d_ftile_off[0];
     b.M = M;
     return OVX_OK;
@@ -485,7 +503,7 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *
     if (M > 1)
         rq.items = nullptr; // row chunks are for single-mesh conversions (a batch has other work to balance)
     tic(c, KS_MARK, s);
-    HIP_TRY(launch_scan_events(m, c->bdev, c->mtile_off[M], ctr, eb, rq, merge, s));
+    HIP_TRY(launch_scan_events(m, c->bdev, c->mark_tile, c->mtile_off[c->mark_tile][M], ctr, eb, rq, merge, s));
     toc(c, KS_MARK, s);
     tic(c, KS_WORDSCAN, s);
     HIP_TRY(launch_scan_popc((const uint64_t *)c->bits.p, n_words, (uint32_t *)c->woff.p, (uint64_t *)c->tiles.p,
@@ -543,7 +561,8 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *
                 return st;
             ShardBuf<Event> eb2 = shard_view<Event>(c, c->events, SB_EVENTS);
             tic(c, KS_MARK, s);
-            HIP_TRY(launch_scan_events(m, c->bdev, c->mtile_off[M], nullptr, eb2, rq, merge, s));
+            HIP_TRY(launch_scan_events(m, c->bdev, c->mark_tile, c->mtile_off[c->mark_tile][M], nullptr, eb2, rq, merge,
+                                       s));
             toc(c, KS_MARK, s);
         }
         c->E = (int64_t)c->h_ctr[CTR_BEDGE_TOTAL];

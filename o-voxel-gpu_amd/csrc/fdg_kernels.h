@@ -12,6 +12,12 @@ namespace ovx
 // faces per tile of the face-test kernel (a block of BT threads takes one
 // tile): small tiles when faces cover many voxels (more, shorter blocks),
 // larger ones for sub-voxel faces (less per-block overhead)
+// (face, axis) items per tile of the scanline kernel (a block of BT threads
+// takes one tile): small tiles spread meshes with large triangles over more
+// blocks, large ones keep the per-tile merging of occupancy bits for meshes
+// of many small triangles
+constexpr int N_MARK_TILES = 3;
+constexpr int MARK_TILES[N_MARK_TILES] = {64, 128, 256};
 constexpr int N_FACE_TILES = 3;
 constexpr int FACE_TILES[N_FACE_TILES] = {32, 64, 128};
 
@@ -155,7 +161,7 @@ struct BatchDev
     const uint64_t *word_off;  // bitset words
     const uint64_t *brick_off; // bricks
     const uint64_t *vox_off;   // voxels (written on the device after the word scan)
-    const uint64_t *mtile_off; // block tiles of scanline items (ceil(3 F_m / block))
+    const uint64_t *mtile_off; // scanline tiles for the tile size of the launch;// block tiles of scanline items (ceil(3 F_m / block))
     const uint64_t *ftile_off; // face-test tiles (ceil(F_m / tile)) for the tile size of the launch
     int M;
 };
@@ -233,7 +239,8 @@ struct FoldArgs
 };
 
 hipError_t launch_prep(const MeshDev &m, const BatchDev &b, float4 *planes, unsigned long long *ctr, hipStream_t s);
-hipError_t launch_scan_events(const MeshDev &m, const BatchDev &b, uint64_t n_tiles, unsigned long long *ctr,
+// tile: index into MARK_TILES; b.mtile_off must hold that tile size's offsets
+hipError_t launch_scan_events(const MeshDev &m, const BatchDev &b, int tile, uint64_t n_tiles, unsigned long long *ctr,
                               const ShardBuf<Event> &eb, const ShardBuf<RowChunk> &rq, bool merge, hipStream_t s);
 // tile: index into FACE_TILES; b.ftile_off must hold that tile size's offsets
 hipError_t launch_face(const MeshDev &m, const BatchDev &b, int tile, uint64_t n_tiles, const ShardBuf<FHit> &hb,

@@ -545,16 +545,17 @@ __device__ __forceinline__ void mark_event(const GridDev &g, Tab &T, int ax0, in
 // LDS table first (pays when crossings are dense; the table costs occupancy)
 // QUEUED = false: one item per (face, axis); QUEUED = true: the queued row
 // chunks of large triangles (items index the queue's slots)
-template <bool MERGE, bool QUEUED>
+template <bool MERGE, bool QUEUED, int MARK_TILE>
 __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, BatchDev bt, uint64_t n_tiles,
                                                     unsigned long long *counters, ShardBuf<Event> eb,
                                                     ShardBuf<RowChunk> rq)
 {
+    static_assert(MARK_TILE <= BT, "a tile's items are set up one per thread");
     __shared__ unsigned long long sbase;
     __shared__ bool sfit;
-    __shared__ ScanItem items[BT];
+    __shared__ ScanItem items[MARK_TILE];
     __shared__ ScanRow rows[BT];
-    __shared__ uint32_t incl_items[BT];
+    __shared__ uint32_t incl_items[MARK_TILE];
     __shared__ uint32_t incl_rows[BT];
     __shared__ uint32_t sh[NWAVE];
     __shared__ typename std::conditional<MERGE, MarkTable, NoMarkTable>::type T;
@@ -564,7 +565,7 @@ __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, BatchDev bt, uint
     if constexpr (QUEUED)
     {
         n_items = (int64_t)queue_prefix(rq, qm.pre, qm.sh);
-        n_tiles = (uint64_t)(n_items + BT - 1) / BT;
+        n_tiles = (uint64_t)(n_items + MARK_TILE - 1) / MARK_TILE;
     }
     uint32_t n_ev = 0;
     if constexpr (MERGE)
@@ -590,14 +591,14 @@ __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, BatchDev bt, uint
         if constexpr (!QUEUED)
         {
             const int64_t f0 = bt.face_off[ms];
-            const int64_t loc = (int64_t)(tile - bt.mtile_off[ms]) * BT + tid;
+            const int64_t loc = (int64_t)(tile - bt.mtile_off[ms]) * MARK_TILE + tid;
             item = 3 * f0 + loc;
-            live = loc < 3 * (bt.face_off[ms + 1] - f0);
+            live = tid < MARK_TILE && loc < 3 * (bt.face_off[ms + 1] - f0);
         }
         else
         {
-            item = (int64_t)tile * BT + tid;
-            live = item < n_items;
+            item = (int64_t)tile * MARK_TILE + tid;
+            live = tid < MARK_TILE && item < n_items;
         }
         uint32_t R = 0;
         if constexpr (!QUEUED)
@@ -658,7 +659,8 @@ __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, BatchDev bt, uint
         }
         uint32_t tot_r;
         const uint32_t ex_r = block_excl_scan(R, sh, tot_r);
-        incl_items[tid] = ex_r + R;
+        if (tid < MARK_TILE)
+            incl_items[tid] = ex_r + R;
         __syncthreads();
 
         for (uint32_t rb = 0; rb < tot_r; rb += BT)
@@ -667,7 +669,7 @@ __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, BatchDev bt, uint
             uint32_t C = 0;
             if (gi < tot_r)
             {
-                const int L = upper_find(incl_items, BT, gi);
+                const int L = upper_find(incl_items, MARK_TILE, gi);
                 const ScanItem &it = items[L];
                 const uint32_t R_L = (uint32_t)(it.end - it.start);
                 const int y_idx = it.start + (int)(gi - (incl_items[L] - R_L));
@@ -2687,7 +2689,16 @@ hipError_t launch_prep(const MeshDev &m, const BatchDev &b, float4 *planes, unsi
     return hipGetLastError();
 }
 
-hipError_t launch_scan_events(const MeshDev &m, const BatchDev &b, uint64_t n_tiles, unsigned long long *ctr,
+template <bool MERGE, int TILE>
+static void scan_pair(const MeshDev &m, const BatchDev &b, unsigned gm, uint64_t n_tiles, unsigned long long *ctr,
+                      const ShardBuf<Event> &eb, const ShardBuf<RowChunk> &rq, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_scan_events<MERGE, false, TILE>), dim3(gm), dim3(BT), 0, s, m, b, n_tiles, ctr, eb, rq);
+    // consumers loop over the filled slots of the row-chunk queue
+    hipLaunchKernelGGL((k_scan_events<MERGE, true, TILE>), dim3(2048), dim3(BT), 0, s, m, b, 0ull, ctr, eb, rq);
+}
+
+hipError_t launch_scan_events(const MeshDev &m, const BatchDev &b, int tile, uint64_t n_tiles, unsigned long long *ctr,
                               const ShardBuf<Event> &eb, const ShardBuf<RowChunk> &rq, bool merge, hipStream_t s)
 {
     const unsigned gm = (unsigned)(n_tiles < (1ull << 20) ? (n_tiles ? n_tiles : 1) : (1ull << 20));
@@ -2696,16 +2707,24 @@ hipError_t launch_scan_events(const MeshDev &m, const BatchDev &b, uint64_t n_ti
     hipError_t e = hipMemsetAsync(rq.cnt, 0, (size_t)NSHARD * SHARD_STRIDE * sizeof(unsigned long long), s);
     if (e != hipSuccess)
         return e;
-    const unsigned gq = 2048; // consumers loop over the filled slots
+    static_assert(MARK_TILES[0] == 64 && MARK_TILES[1] == 128 && MARK_TILES[2] == 256, "scanline tile instantiations");
     if (merge)
     {
-        hipLaunchKernelGGL((k_scan_events<true, false>), dim3(gm), dim3(BT), 0, s, m, b, n_tiles, ctr, eb, rq);
-        hipLaunchKernelGGL((k_scan_events<true, true>), dim3(gq), dim3(BT), 0, s, m, b, 0ull, ctr, eb, rq);
+        if (tile == 0)
+            scan_pair<true, 64>(m, b, gm, n_tiles, ctr, eb, rq, s);
+        else if (tile == 1)
+            scan_pair<true, 128>(m, b, gm, n_tiles, ctr, eb, rq, s);
+        else
+            scan_pair<true, 256>(m, b, gm, n_tiles, ctr, eb, rq, s);
     }
     else
     {
-        hipLaunchKernelGGL((k_scan_events<false, false>), dim3(gm), dim3(BT), 0, s, m, b, n_tiles, ctr, eb, rq);
-        hipLaunchKernelGGL((k_scan_events<false, true>), dim3(gq), dim3(BT), 0, s, m, b, 0ull, ctr, eb, rq);
+        if (tile == 0)
+            scan_pair<false, 64>(m, b, gm, n_tiles, ctr, eb, rq, s);
+        else if (tile == 1)
+            scan_pair<false, 128>(m, b, gm, n_tiles, ctr, eb, rq, s);
+        else
+            scan_pair<false, 256>(m, b, gm, n_tiles, ctr, eb, rq, s);
     }
     return hipGetLastError();
 }
