@@ -1998,7 +1998,7 @@ struct FoldPick<1>
 };
 
 constexpr int FOLD_SMALL = 2048; // records of a brick held in LDS by the main fold kernel (4 blocks per CU)
-constexpr int FOLD_CAP = 6144;   // records held in LDS at a time by the kernel for large bricks
+constexpr int FOLD_CAP = 4096;   // records held in LDS at a time by the kernel for large bricks (2 blocks per CU)
 
 // LDS view of a batch of up to BT chunks of one brick
 struct ChunkBatch
