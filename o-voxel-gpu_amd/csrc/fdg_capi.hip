@@ -107,7 +107,7 @@ struct ovx_fdg_ctx
     std::vector<uint64_t> word_off, brick_off;
     std::vector<uint64_t> mtile_off[N_MARK_TILES]; // scanline tiles per mesh, per tile size
     const uint64_t *d_mtile_off[N_MARK_TILES] = {};
-    int mark_tile = 2;                             // tile size of the current conversion
+    size_t stage_ftile = 0; // byte offset of the face-tile arrays in the staging and device tables
     std::vector<uint64_t> ftile_off[N_FACE_TILES]; // face-test tiles per mesh, per tile size
     const uint64_t *d_ftile_off[N_FACE_TILES] = {};
     std::vector<MeshSeg> segs;
@@ -347,8 +347,15 @@ ovx_status setup_batch(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *f
         const int64_t nf = face_off[i + 1] - face_off[i];
         c->word_off[i + 1] = c->word_off[i] + nw;
         c->brick_off[i + 1] = c->brick_off[i] + (uint64_t)sg.nb[0] * (uint64_t)sg.nb[1] * (uint64_t)sg.nb[2];
+        // scanline tile size of the mesh from its faces per grid cross-section:
+        // meshes of few, large triangles take small tiles, meshes of many
+        // small ones large tiles; each size is one launch over its meshes
+        const double cross = std::max({(double)g.G[0] * g.G[1], (double)g.G[1] * g.G[2], (double)g.G[0] * g.G[2]});
+        const double fpc = (double)nf / cross;
+        const int mt = fpc >= 8.0 ? 2 : (fpc >= 0.5 ? 1 : 0);
         for (int t = 0; t < N_MARK_TILES; ++t)
-            c->mtile_off[t][i + 1] = c->mtile_off[t][i] + (uint64_t)(3 * nf + MARK_TILES[t] - 1) / MARK_TILES[t];
+            c->mtile_off[t][i + 1] =
+                c->mtile_off[t][i] + (t == mt ? (uint64_t)(3 * nf + MARK_TILES[t] - 1) / MARK_TILES[t] : 0ull);
         for (int t = 0; t < N_FACE_TILES; ++t)
             c->ftile_off[t][i + 1] = c->ftile_off[t][i] + (uint64_t)(nf + FACE_TILES[t] - 1) / FACE_TILES[t];
     }
@@ -409,19 +416,63 @@ ovx_status setup_batch(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *f
         c->d_mtile_off[t] = (const uint64_t *)(d + seg_bytes + (5 + t) * arr);
     for (int t = 0; t < N_FACE_TILES; ++t)
         c->d_ftile_off[t] = (const uint64_t *)(d + seg_bytes + (FT0 + t) * arr);
-    // scanline tile size from faces per grid cross-section: meshes of few,
-    // large triangles take small tiles, meshes of many small ones large tiles
-    double cross = 0.0;
-    for (int i = 0; i < M; ++i)
-    {
-        const GridDev &g = c->segs[i].g;
-        cross += std::max({(double)g.G[0] * g.G[1], (double)g.G[1] * g.G[2], (double)g.G[0] * g.G[2]});
-    }
-    const double fpc = cross > 0.0 ? (double)face_off[M] / cross : 0.0;
-    c->mark_tile = fpc >= 8.0 ? 2 : (fpc >= 0.5 ? 1 : 0);
-    b.mtile_off = c->d_mtile_off[c->mark_tile];
+    c->stage_ftile = seg_bytes + FT0 * arr;
+    b.mtile_off = c->d_mtile_off[0];
     b.ftile_off = c->d_ftile_off[0];
     b.M = M;
+    return OVX_OK;
+}
+
+// the scanline pass: one launch per tile size present in the batch
+ovx_status launch_mark(ovx_fdg_ctx *c, const MeshDev &m, unsigned long long *ctr, const ShardBuf<Event> &eb,
+                       const ShardBuf<RowChunk> &rq, bool merge, hipStream_t s)
+{
+    for (int t = 0; t < N_MARK_TILES; ++t)
+    {
+        const uint64_t nt = c->mtile_off[t][c->M];
+        if (!nt)
+            continue;
+        BatchDev b = c->bdev;
+        b.mtile_off = c->d_mtile_off[t];
+        HIP_TRY(launch_scan_events(m, b, t, nt, ctr, eb, rq, merge, s));
+    }
+    return OVX_OK;
+}
+
+// the face test: the face-tile size of each mesh from its voxels per face
+// (faces covering many voxels take small tiles, sub-voxel faces larger
+// ones); per-size tile offsets are uploaded, then one launch per size
+ovx_status launch_face_tiles(ovx_fdg_ctx *c, const MeshDev &m, const ShardBuf<FHit> &fb, unsigned long long *ctr,
+                             const ShardBuf<FaceChunk> &fq, hipStream_t s)
+{
+    const int M = c->M;
+    for (auto &v : c->ftile_off)
+        v[0] = 0;
+    for (int i = 0; i < M; ++i)
+    {
+        const int64_t nf = c->face_off[i + 1] - c->face_off[i];
+        const double nv = (double)(c->h_vox_off[i + 1] - c->h_vox_off[i]);
+        const double vpf = nf > 0 ? nv / (double)nf : 0.0;
+        const int ft = vpf >= 4.0 ? 0 : (vpf >= 1.0 ? 1 : 2);
+        for (int t = 0; t < N_FACE_TILES; ++t)
+            c->ftile_off[t][i + 1] =
+                c->ftile_off[t][i] + (t == ft ? (uint64_t)(nf + FACE_TILES[t] - 1) / FACE_TILES[t] : 0ull);
+    }
+    // the staging region was last read by the set-up copy, complete since begin's readback
+    const size_t arr = (size_t)(M + 1) * 8;
+    for (int t = 0; t < N_FACE_TILES; ++t)
+        std::memcpy(c->h_stage + c->stage_ftile + t * arr, c->ftile_off[t].data(), arr);
+    HIP_TRY(hipMemcpyAsync((unsigned char *)c->segbuf.p + c->stage_ftile, c->h_stage + c->stage_ftile,
+                           N_FACE_TILES * arr, hipMemcpyHostToDevice, s));
+    for (int t = 0; t < N_FACE_TILES; ++t)
+    {
+        const uint64_t nt = c->ftile_off[t][M];
+        if (!nt)
+            continue;
+        BatchDev b = c->bdev;
+        b.ftile_off = c->d_ftile_off[t];
+        HIP_TRY(launch_face(m, b, t, nt, fb, ctr, fq, s));
+    }
     return OVX_OK;
 }
 
@@ -503,7 +554,8 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *
     if (M > 1)
         rq.items = nullptr; // row chunks are for single-mesh conversions (a batch has other work to balance)
     tic(c, KS_MARK, s);
-    HIP_TRY(launch_scan_events(m, c->bdev, c->mark_tile, c->mtile_off[c->mark_tile][M], ctr, eb, rq, merge, s));
+    if ((st = launch_mark(c, m, ctr, eb, rq, merge, s)))
+        return st;
     toc(c, KS_MARK, s);
     tic(c, KS_WORDSCAN, s);
     HIP_TRY(launch_scan_popc((const uint64_t *)c->bits.p, n_words, (uint32_t *)c->woff.p, (uint64_t *)c->tiles.p,
@@ -561,8 +613,8 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *
                 return st;
             ShardBuf<Event> eb2 = shard_view<Event>(c, c->events, SB_EVENTS);
             tic(c, KS_MARK, s);
-            HIP_TRY(launch_scan_events(m, c->bdev, c->mark_tile, c->mtile_off[c->mark_tile][M], nullptr, eb2, rq, merge,
-                                       s));
+            if ((st = launch_mark(c, m, nullptr, eb2, rq, merge, s)))
+                return st;
             toc(c, KS_MARK, s);
         }
         c->E = (int64_t)c->h_ctr[CTR_BEDGE_TOTAL];
@@ -638,14 +690,8 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
             ShardBuf<FaceChunk> fq = shard_view<FaceChunk>(c, c->faceq, SB_FACEQ);
             if (c->M > 1)
                 fq.items = nullptr; // row chunks are for single-mesh conversions
-            // tile size from the voxels per face: faces covering many voxels
-            // take small tiles, sub-voxel faces larger ones
-            const int64_t F = c->face_off[c->M];
-            const double vpf = F > 0 ? (double)c->N / (double)F : 0.0;
-            const int tile = vpf >= 4.0 ? 0 : (vpf >= 1.0 ? 1 : 2);
-            BatchDev bf = c->bdev;
-            bf.ftile_off = c->d_ftile_off[tile];
-            HIP_TRY(launch_face(m, bf, tile, c->ftile_off[tile][c->M], fb, ctr, fq, s));
+            if ((st = launch_face_tiles(c, m, fb, ctr, fq, s)))
+                return st;
             toc(c, KS_FACE, s);
             tic(c, KS_GEN_FHITS, s);
             HIP_TRY(launch_gen_fhits(fb, ba, s));
