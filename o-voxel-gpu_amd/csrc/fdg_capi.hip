@@ -71,6 +71,20 @@ struct DeviceGuard
 
 } // namespace
 
+// Scanline launch classes: (index into MARK_TILES, LDS merging of the
+// occupancy bits).  A mesh's class comes from its faces per grid
+// cross-section (fpc): meshes of few, large triangles take small tiles and
+// merge their dense crossings in LDS; with many more faces than voxel rows
+// per plane the crossings are sparse, the merge table only costs occupancy,
+// and larger tiles pay.  Each class present is one launch over its meshes.
+constexpr int N_MARK_CLASSES = 4;
+constexpr int MARK_CLASS_TILE[N_MARK_CLASSES] = {0, 1, 1, 2};
+constexpr bool MARK_CLASS_MERGE[N_MARK_CLASSES] = {true, true, false, false};
+int mark_class(double fpc)
+{
+    return fpc < 0.5 ? 0 : (fpc <= 2.0 ? 1 : (fpc < 8.0 ? 2 : 3));
+}
+
 struct ovx_fdg_ctx
 {
     int device = 0;
@@ -105,9 +119,11 @@ struct ovx_fdg_ctx
     int M = 0;
     std::vector<int64_t> face_off, vert_off;
     std::vector<uint64_t> word_off, brick_off;
-    std::vector<uint64_t> mtile_off[N_MARK_TILES]; // scanline tiles per mesh, per tile size
-    const uint64_t *d_mtile_off[N_MARK_TILES] = {};
+    std::vector<uint64_t> mtile_off[N_MARK_CLASSES]; // scanline tiles per mesh, per launch class
+    const uint64_t *d_mtile_off[N_MARK_CLASSES] = {};
     size_t stage_ftile = 0; // byte offset of the face-tile arrays in the staging and device tables
+    size_t stage_gd = 0;    // byte offset of the first guide (after the face-tile arrays), and bytes per guide
+    size_t gd_bytes = 0;
     std::vector<uint64_t> ftile_off[N_FACE_TILES]; // face-test tiles per mesh, per tile size
     const uint64_t *d_ftile_off[N_FACE_TILES] = {};
     std::vector<MeshSeg> segs;
@@ -308,6 +324,28 @@ ovx_status clear_shards(ovx_fdg_ctx *c, int sb0, int nsb, hipStream_t s)
     return OVX_OK;
 }
 
+// Guide of a prefix array of M meshes (BatchDev::gd): the smallest shift
+// that makes ((total - 1) >> shift) + 2 entries fit guide_len(M), and entry
+// j = the owner of index j << shift (largest m < M with off[m] <= j << shift).
+template <typename T>
+int build_guide(const T *off, int M, int32_t *gd)
+{
+    const uint64_t total = (uint64_t)off[M];
+    int sh = 0;
+    while (total > 0 && (int64_t)((total - 1) >> sh) + 2 > guide_len(M))
+        ++sh;
+    const int64_t n = total ? (int64_t)((total - 1) >> sh) + 2 : 2;
+    int m = 0;
+    for (int64_t j = 0; j < n; ++j)
+    {
+        const uint64_t x = (uint64_t)j << sh;
+        while (m + 1 < M && (uint64_t)off[m + 1] <= x)
+            ++m;
+        gd[j] = m;
+    }
+    return sh;
+}
+
 // Batch tables: per-mesh grids, prefix offsets of faces, vertices, bitset
 // words, bricks and block tiles; the device copy is uploaded on stream s.
 ovx_status setup_batch(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *face_off, const int64_t *vert_off,
@@ -347,15 +385,14 @@ ovx_status setup_batch(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *f
         const int64_t nf = face_off[i + 1] - face_off[i];
         c->word_off[i + 1] = c->word_off[i] + nw;
         c->brick_off[i + 1] = c->brick_off[i] + (uint64_t)sg.nb[0] * (uint64_t)sg.nb[1] * (uint64_t)sg.nb[2];
-        // scanline tile size of the mesh from its faces per grid cross-section:
-        // meshes of few, large triangles take small tiles, meshes of many
-        // small ones large tiles; each size is one launch over its meshes
+        // scanline launch class of the mesh from its faces per grid cross-section
         const double cross = std::max({(double)g.G[0] * g.G[1], (double)g.G[1] * g.G[2], (double)g.G[0] * g.G[2]});
-        const double fpc = (double)nf / cross;
-        const int mt = fpc >= 8.0 ? 2 : (fpc >= 0.5 ? 1 : 0);
-        for (int t = 0; t < N_MARK_TILES; ++t)
-            c->mtile_off[t][i + 1] =
-                c->mtile_off[t][i] + (t == mt ? (uint64_t)(3 * nf + MARK_TILES[t] - 1) / MARK_TILES[t] : 0ull);
+        const int mc = mark_class((double)nf / cross);
+        for (int t = 0; t < N_MARK_CLASSES; ++t)
+        {
+            const uint64_t tile = (uint64_t)MARK_TILES[MARK_CLASS_TILE[t]];
+            c->mtile_off[t][i + 1] = c->mtile_off[t][i] + (t == mc ? (3 * (uint64_t)nf + tile - 1) / tile : 0ull);
+        }
         for (int t = 0; t < N_FACE_TILES; ++t)
             c->ftile_off[t][i + 1] = c->ftile_off[t][i] + (uint64_t)(nf + FACE_TILES[t] - 1) / FACE_TILES[t];
     }
@@ -373,11 +410,14 @@ ovx_status setup_batch(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *f
         c->segs[i].g.bits = (uint64_t *)c->bits.p + c->word_off[i];
         c->segs[i].g.woff = (const uint32_t *)c->woff.p + c->word_off[i];
     }
-    // device layout: segments, then 5 + N_MARK_TILES + N_FACE_TILES offset arrays of M + 1 entries
+    // device layout: segments, then 5 + N_MARK_CLASSES + N_FACE_TILES offset
+    // arrays of M + 1 entries, then N_GUIDES guides of guide_len(M) entries
     const size_t seg_bytes = ((sizeof(MeshSeg) * (size_t)M + 255) / 256) * 256;
     const size_t arr = (size_t)(M + 1) * 8;
-    constexpr int FT0 = 5 + N_MARK_TILES; // first face-tile array
-    const size_t total = seg_bytes + (FT0 + N_FACE_TILES) * arr;
+    const size_t garr = (((size_t)guide_len(M) * 4 + 7) / 8) * 8;
+    constexpr int FT0 = 5 + N_MARK_CLASSES; // first face-tile array
+    const size_t gd0 = seg_bytes + (FT0 + N_FACE_TILES) * arr;
+    const size_t total = gd0 + N_GUIDES * garr;
     if (c->h_stage_bytes < total + arr)
     {
         if (c->ran)
@@ -398,10 +438,16 @@ ovx_status setup_batch(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *f
     std::memcpy(h + seg_bytes + 2 * arr, c->word_off.data(), arr);
     std::memcpy(h + seg_bytes + 3 * arr, c->brick_off.data(), arr);
     std::memset(h + seg_bytes + 4 * arr, 0, arr); // voxel offsets: written on the device
-    for (int t = 0; t < N_MARK_TILES; ++t)
+    for (int t = 0; t < N_MARK_CLASSES; ++t)
         std::memcpy(h + seg_bytes + (5 + t) * arr, c->mtile_off[t].data(), arr);
     for (int t = 0; t < N_FACE_TILES; ++t)
         std::memcpy(h + seg_bytes + (FT0 + t) * arr, c->ftile_off[t].data(), arr);
+    int gsh[N_GUIDES] = {};
+    gsh[GD_FACE] = build_guide(c->face_off.data(), M, (int32_t *)(h + gd0 + GD_FACE * garr));
+    gsh[GD_VERT] = build_guide(c->vert_off.data(), M, (int32_t *)(h + gd0 + GD_VERT * garr));
+    gsh[GD_WORD] = build_guide(c->word_off.data(), M, (int32_t *)(h + gd0 + GD_WORD * garr));
+    gsh[GD_BRICK] = build_guide(c->brick_off.data(), M, (int32_t *)(h + gd0 + GD_BRICK * garr));
+    std::memset(h + gd0 + GD_VOX * garr, 0, garr); // voxel guide: built once the voxel offsets are read back
     c->h_vox_off = (uint64_t *)(h + total);
     HIP_TRY(hipMemcpyAsync(c->segbuf.p, h, total, hipMemcpyHostToDevice, s));
     unsigned char *d = (unsigned char *)c->segbuf.p;
@@ -412,29 +458,36 @@ ovx_status setup_batch(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *f
     b.word_off = (const uint64_t *)(d + seg_bytes + 2 * arr);
     b.brick_off = (const uint64_t *)(d + seg_bytes + 3 * arr);
     b.vox_off = (const uint64_t *)(d + seg_bytes + 4 * arr);
-    for (int t = 0; t < N_MARK_TILES; ++t)
+    for (int t = 0; t < N_MARK_CLASSES; ++t)
         c->d_mtile_off[t] = (const uint64_t *)(d + seg_bytes + (5 + t) * arr);
     for (int t = 0; t < N_FACE_TILES; ++t)
         c->d_ftile_off[t] = (const uint64_t *)(d + seg_bytes + (FT0 + t) * arr);
     c->stage_ftile = seg_bytes + FT0 * arr;
+    c->stage_gd = gd0;
+    c->gd_bytes = garr;
     b.mtile_off = c->d_mtile_off[0];
     b.ftile_off = c->d_ftile_off[0];
+    for (int k = 0; k < N_GUIDES; ++k)
+    {
+        b.gd[k] = (const int32_t *)(d + gd0 + k * garr);
+        b.gsh[k] = gsh[k];
+    }
     b.M = M;
     return OVX_OK;
 }
 
-// the scanline pass: one launch per tile size present in the batch
+// the scanline pass: one launch per launch class present in the batch
 ovx_status launch_mark(ovx_fdg_ctx *c, const MeshDev &m, unsigned long long *ctr, const ShardBuf<Event> &eb,
-                       const ShardBuf<RowChunk> &rq, bool merge, hipStream_t s)
+                       const ShardBuf<RowChunk> &rq, hipStream_t s)
 {
-    for (int t = 0; t < N_MARK_TILES; ++t)
+    for (int t = 0; t < N_MARK_CLASSES; ++t)
     {
         const uint64_t nt = c->mtile_off[t][c->M];
         if (!nt)
             continue;
         BatchDev b = c->bdev;
         b.mtile_off = c->d_mtile_off[t];
-        HIP_TRY(launch_scan_events(m, b, t, nt, ctr, eb, rq, merge, s));
+        HIP_TRY(launch_scan_events(m, b, MARK_CLASS_TILE[t], nt, ctr, eb, rq, MARK_CLASS_MERGE[t], s));
     }
     return OVX_OK;
 }
@@ -536,16 +589,6 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *
             return st;
         eb = shard_view<Event>(c, c->events, SB_EVENTS);
     }
-    // the LDS merging of occupancy bits pays when crossings are dense (faces
-    // spanning several voxel rows); with many more faces than voxel rows per
-    // plane the crossings are sparse and the table only costs occupancy
-    double rows_per_plane = 0.0;
-    for (const MeshSeg &sg : c->segs)
-    {
-        const GridDev &gm = sg.g;
-        rows_per_plane += std::max({(double)gm.G[0] * gm.G[1], (double)gm.G[1] * gm.G[2], (double)gm.G[0] * gm.G[2]});
-    }
-    const bool merge = (double)m.F <= 2.0 * rows_per_plane;
     // row chunks of large triangles, shared out over the grid by a second launch
     if ((st = ensure_shard(c, c->rowq, SB_ROWQ, per_for(std::max<uint64_t>(1ull << 20, (uint64_t)m.F / 4)),
                            sizeof(RowChunk))))
@@ -554,7 +597,7 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *
     if (M > 1)
         rq.items = nullptr; // row chunks are for single-mesh conversions (a batch has other work to balance)
     tic(c, KS_MARK, s);
-    if ((st = launch_mark(c, m, ctr, eb, rq, merge, s)))
+    if ((st = launch_mark(c, m, ctr, eb, rq, s)))
         return st;
     toc(c, KS_MARK, s);
     tic(c, KS_WORDSCAN, s);
@@ -613,7 +656,7 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *
                 return st;
             ShardBuf<Event> eb2 = shard_view<Event>(c, c->events, SB_EVENTS);
             tic(c, KS_MARK, s);
-            if ((st = launch_mark(c, m, nullptr, eb2, rq, merge, s)))
+            if ((st = launch_mark(c, m, nullptr, eb2, rq, s)))
                 return st;
             toc(c, KS_MARK, s);
         }
@@ -644,6 +687,13 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
     ovx_status st;
     const MeshDev &m = c->mesh;
     auto *ctr = (unsigned long long *)c->counters.p;
+    if (c->M > 1)
+    {
+        // the voxel guide, from the voxel offsets begin read back
+        const size_t g = c->stage_gd + GD_VOX * c->gd_bytes;
+        c->bdev.gsh[GD_VOX] = build_guide(c->h_vox_off, c->M, (int32_t *)(c->h_stage + g));
+        HIP_TRY(hipMemcpyAsync((unsigned char *)c->segbuf.p + g, c->h_stage + g, c->gd_bytes, hipMemcpyHostToDevice, s));
+    }
     tic(c, KS_COORDS, s);
     HIP_TRY(launch_emit_coords(c->bdev, c->n_words, coords, s));
     toc(c, KS_COORDS, s);

@@ -161,17 +161,37 @@ struct BatchDev
     const uint64_t *word_off;  // bitset words
     const uint64_t *brick_off; // bricks
     const uint64_t *vox_off;   // voxels (written on the device after the word scan)
-    const uint64_t *mtile_off; // scanline tiles for the tile size of the launch;// block tiles of scanline items (ceil(3 F_m / block))
+    const uint64_t *mtile_off; // scanline tiles (ceil(3 F_m / tile)) for the tile size of the launch
     const uint64_t *ftile_off; // face-test tiles (ceil(F_m / tile)) for the tile size of the launch
+    // guides of the face / vertex / word / brick / voxel prefixes:
+    // gd[k][j] = owner of index j << gsh[k], for j up to ((total - 1) >> gsh[k]) + 1
+    const int32_t *gd[5];
+    int gsh[5];
     int M;
 };
 
-// the mesh owning index x of a prefix array (largest m with off[m] <= x;
-// empty meshes are skipped over)
-template <typename T>
-__device__ __forceinline__ int seg_find(const T *off, int M, T x)
+enum SegGuide
 {
-    int lo = 0, hi = M - 1;
+    GD_FACE = 0,
+    GD_VERT = 1,
+    GD_WORD = 2,
+    GD_BRICK = 3,
+    GD_VOX = 4,
+    N_GUIDES = 5
+};
+
+// guide entries per prefix array for M meshes (the shift makes the guide
+// cover the array's total in at most this many steps)
+__host__ __device__ constexpr int64_t guide_len(int M)
+{
+    return 8 * (int64_t)M + 2;
+}
+
+// the mesh owning index x of a prefix array (largest m with off[m] <= x;
+// empty meshes are skipped over), searched inside [lo, hi]
+template <typename T>
+__device__ __forceinline__ int seg_search(const T *off, int lo, int hi, T x)
+{
     while (lo < hi)
     {
         const int mid = (lo + hi + 1) >> 1;
@@ -181,6 +201,44 @@ __device__ __forceinline__ int seg_find(const T *off, int M, T x)
             hi = mid - 1;
     }
     return lo;
+}
+
+template <typename T>
+__device__ __forceinline__ int seg_find(const T *off, int M, T x)
+{
+    return seg_search(off, 0, M - 1, x);
+}
+
+// the same through the prefix's guide: the owner of x lies between the
+// owners of the guide points below and above x, usually one mesh
+template <typename T>
+__device__ __forceinline__ int seg_find(const BatchDev &b, int k, const T *off, T x)
+{
+    if (b.M == 1)
+        return 0;
+    const uint64_t j = (uint64_t)x >> b.gsh[k];
+    return seg_search(off, b.gd[k][j], b.gd[k][j + 1], x);
+}
+
+__device__ __forceinline__ int mesh_of_face(const BatchDev &b, int64_t f)
+{
+    return seg_find(b, GD_FACE, b.face_off, f);
+}
+__device__ __forceinline__ int mesh_of_vert(const BatchDev &b, int64_t v)
+{
+    return seg_find(b, GD_VERT, b.vert_off, v);
+}
+__device__ __forceinline__ int mesh_of_word(const BatchDev &b, uint64_t w)
+{
+    return seg_find(b, GD_WORD, b.word_off, w);
+}
+__device__ __forceinline__ int mesh_of_brick(const BatchDev &b, uint64_t k)
+{
+    return seg_find(b, GD_BRICK, b.brick_off, k);
+}
+__device__ __forceinline__ int mesh_of_voxel(const BatchDev &b, uint64_t v)
+{
+    return seg_find(b, GD_VOX, b.vox_off, v);
 }
 
 struct BucketArgs

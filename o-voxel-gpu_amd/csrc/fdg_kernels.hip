@@ -318,7 +318,7 @@ __global__ __launch_bounds__(BT) void k_prep(MeshDev m, BatchDev bt, float4 *pla
         if (m.f)
         {
             // indices are batch-global: each must name a vertex of the face's own mesh
-            const int ms = seg_find(bt.face_off, bt.M, i);
+            const int ms = mesh_of_face(bt, i);
             const int64_t v0 = bt.vert_off[ms], v1 = bt.vert_off[ms + 1];
             const int32_t *fi = m.f + 3 * i;
             const int64_t a = fi[0], b = fi[1], c = fi[2];
@@ -341,7 +341,7 @@ __global__ __launch_bounds__(BT) void k_prep(MeshDev m, BatchDev bt, float4 *pla
     const int64_t nv = m.f ? m.V : 3 * m.F;
     for (int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x; i < nv; i += stride)
     {
-        const GridDev &g = bt.seg[seg_find(bt.vert_off, bt.M, i)].g;
+        const GridDev &g = bt.seg[mesh_of_vert(bt, i)].g;
 #pragma unroll
         for (int a = 0; a < 3; ++a)
         {
@@ -1294,7 +1294,7 @@ __global__ __launch_bounds__(BT) void k_bdry(int64_t E, const BEdge *be, const i
     if (i < E && be[i].valid)
     {
         const BEdge &e = be[i];
-        const GridDev g = bt.seg[seg_find(bt.vert_off, bt.M, (int64_t)bedge_v[i].x)].g; // the edge's mesh
+        const GridDev g = bt.seg[mesh_of_vert(bt, (int64_t)bedge_v[i].x)].g; // the edge's mesh
         int cur[3], step[3];
         double tMax[3], tDelta[3];
         double cap = 16.0;
@@ -1390,7 +1390,7 @@ __global__ __launch_bounds__(BT) void k_gen_events(ShardBuf<Event> ev, BucketArg
             const Event e = ev.items[queue_slot(ev, qm.pre, i)];
             const uint64_t fa = (e.key >> KEY_FA_SHIFT) & KEY_FA_MASK;
             const uint64_t face = fa / 3u;
-            const MeshSeg &sg = ba.b.seg[seg_find(ba.b.face_off, ba.b.M, (int64_t)face)];
+            const MeshSeg &sg = ba.b.seg[mesh_of_face(ba.b, (int64_t)face)];
             const GridDev &g = sg.g;
             const int ax2 = (int)(fa - 3u * face);
             const int ax0 = (ax2 + 1) % 3;
@@ -1433,7 +1433,7 @@ __global__ __launch_bounds__(BT) void k_gen_fhits(ShardBuf<FHit> h, BucketArgs b
             if (i < n)
             {
                 const FHit x = h.items[queue_slot(h, qm.pre, i)];
-                const MeshSeg &sg = ba.b.seg[seg_find(ba.b.face_off, ba.b.M, (int64_t)x.face)];
+                const MeshSeg &sg = ba.b.seg[mesh_of_face(ba.b, (int64_t)x.face)];
                 const GridDev &g = sg.g;
                 uint32_t lvox;
                 brick_of(g, sg.nb, g.gmin[0] + (int)x.lx, g.gmin[1] + (int)x.ly, g.gmin[2] + (int)x.lz, bk[k], lvox);
@@ -1461,7 +1461,7 @@ __global__ __launch_bounds__(BT) void k_gen_bhits(ShardBuf<BHit> h, BucketArgs b
         if (i < n)
         {
             const BHit x = h.items[queue_slot(h, qm.pre, i)];
-            const MeshSeg &sg = ba.b.seg[seg_find(ba.b.vert_off, ba.b.M, (int64_t)ba.bedge_v[x.edge].x)];
+            const MeshSeg &sg = ba.b.seg[mesh_of_vert(ba.b, (int64_t)ba.bedge_v[x.edge].x)];
             uint32_t lvox;
             brick_of(sg.g, sg.nb, x.x, x.y, x.z, bk[0], lvox);
             bk[0] += (uint32_t)sg.brick0;
@@ -1622,7 +1622,7 @@ __global__ __launch_bounds__(BT) void k_emit_coords(BatchDev bt, uint64_t n_word
     const uint64_t stride = (uint64_t)gridDim.x * BT;
     for (uint64_t gw = (uint64_t)blockIdx.x * BT + threadIdx.x; gw < n_words; gw += stride)
     {
-        const int ms = seg_find(bt.word_off, bt.M, gw);
+        const int ms = mesh_of_word(bt, gw);
         const GridDev &g = bt.seg[ms].g;
         const uint64_t w = gw - bt.word_off[ms]; // word of the mesh
         uint64_t m = g.bits[w];
@@ -1693,10 +1693,43 @@ struct FoldState
         add_pl(key, z, f != NO_PLANE ? a.planes[f] : make_float4(0.0f, 0.0f, 0.0f, 0.0f), a);
     }
 
-    // one record; pl4 = the plane of its face (unused for boundary records)
+    // Start values from the voxel's first record (records are replayed in
+    // key order, so a voxel's intersections come first): the reference stores
+    // the first intersection's point and quadric as they are, which is -0 + x
+    // (exact for every x, signed zeros included); a voxel without
+    // intersections adds its face and boundary terms to +0.
+    __device__ __forceinline__ void begin(uint64_t first_key)
+    {
+        const float q0 = (first_key >> KEY_PHASE_SHIFT) == 0 ? -0.0f : 0.0f;
+#pragma unroll
+        for (int k = 0; k < 10; ++k)
+            q[k] = q0;
+        m0 = m1 = m2 = -0.0f;
+    }
+
+    // one record; pl4 = the plane of its face (unused for boundary records).
+    // The quadric term and its addition are shared by all phases, so a wave
+    // whose lanes are at records of different phases runs them once.
     __device__ __forceinline__ void add_pl(uint64_t key, float z, const float4 pl4, const FoldArgs &a)
     {
         const int phase = (int)(key >> KEY_PHASE_SHIFT);
+        float qf[10];
+        if (phase == 2)
+        {
+            const uint64_t e = key & ((1ull << KEY_PHASE_SHIFT) - 1ull);
+            const float *qe = a.bedges[e].q;
+#pragma unroll
+            for (int k = 0; k < 10; ++k)
+                qf[k] = a.bw * qe[k];
+        }
+        else
+        {
+            const float pl[4] = {pl4.x, pl4.y, pl4.z, pl4.w};
+            plane_quadric(pl, qf);
+        }
+#pragma unroll
+        for (int k = 0; k < 10; ++k)
+            q[k] = q[k] + qf[k];
         if (phase == 0)
         {
             const uint64_t fa = (key >> KEY_FA_SHIFT) & KEY_FA_MASK;
@@ -1710,52 +1743,12 @@ struct FoldState
                 (int)((key >> KEY_X_SHIFT) & KEY_IDX_MASK) + sel3i(ax0, a.g.gmin[0], a.g.gmin[1], a.g.gmin[2]);
             const float px = (float)(x_idx + 1) * sel3f(ax0, a.g.vs[0], a.g.vs[1], a.g.vs[2]);
             const float py = (float)(y_idx + 1) * sel3f(ax1, a.g.vs[0], a.g.vs[1], a.g.vs[2]);
-            const float p0 = ax0 == 0 ? px : (ax1 == 0 ? py : z);
-            const float p1 = ax0 == 1 ? px : (ax1 == 1 ? py : z);
-            const float p2 = ax0 == 2 ? px : (ax1 == 2 ? py : z);
-            const float pl[4] = {pl4.x, pl4.y, pl4.z, pl4.w};
-            float qf[10];
-            plane_quadric(pl, qf);
-            if (cnt == 0.0f)
-            {
-                // first contribution: the reference pushes the point and Q
-                m0 = p0;
-                m1 = p1;
-                m2 = p2;
-                cnt = 1.0f;
-#pragma unroll
-                for (int k = 0; k < 10; ++k)
-                    q[k] = qf[k];
-            }
-            else
-            {
-                m0 = m0 + p0;
-                m1 = m1 + p1;
-                m2 = m2 + p2;
-                cnt = cnt + 1.0f;
-#pragma unroll
-                for (int k = 0; k < 10; ++k)
-                    q[k] = q[k] + qf[k];
-            }
+            m0 = m0 + (ax0 == 0 ? px : (ax1 == 0 ? py : z));
+            m1 = m1 + (ax0 == 1 ? px : (ax1 == 1 ? py : z));
+            m2 = m2 + (ax0 == 2 ? px : (ax1 == 2 ? py : z));
+            cnt = cnt + 1.0f;
             if ((key & 3ull) == 0ull)
                 fl |= 1 << ax2;
-        }
-        else if (phase == 1)
-        {
-            const float pl[4] = {pl4.x, pl4.y, pl4.z, pl4.w};
-            float qf[10];
-            plane_quadric(pl, qf);
-#pragma unroll
-            for (int k = 0; k < 10; ++k)
-                q[k] = q[k] + qf[k];
-        }
-        else
-        {
-            const uint64_t e = key & ((1ull << KEY_PHASE_SHIFT) - 1ull);
-            const float *qe = a.bedges[e].q;
-#pragma unroll
-            for (int k = 0; k < 10; ++k)
-                q[k] = q[k] + a.bw * qe[k];
         }
     }
 
@@ -1860,6 +1853,11 @@ struct FoldStateC
         c[2] = (float)z * a.g.vs[2];
     }
 
+    // the compensated sums start at zero whatever the first record is
+    __device__ __forceinline__ void begin(uint64_t)
+    {
+    }
+
     // plane n.p + d = 0 in the voxel frame: (n * vs).u + (n.c + d), the
     // offset formed from exact products and compensated sums
     __device__ __forceinline__ void add_plane(const float4 pl4, const FoldArgs &a)
@@ -1904,11 +1902,10 @@ struct FoldStateC
             for (int i = 0; i < 3; ++i)
                 m[i].add((p[i] - c[i]) / a.g.vs[i]);
             cnt = cnt + 1.0f;
-            add_plane(pl4, a);
             if ((key & 3ull) == 0ull)
                 fl |= 1 << ax2;
         }
-        else if (phase == 1)
+        if (phase != 2)
             add_plane(pl4, a);
         else
         {
@@ -2122,6 +2119,7 @@ __device__ __forceinline__ void fold_loaded(const FoldArgs &a, FoldLds<CAP> &L, 
         const uint32_t s0 = L.off[l], s1 = L.off[l + 1];
         typename FoldPick<P>::type st;
         st.init(x0 + (l & 7), y0 + ((l >> 3) & 7), z0 + (l >> 6), a);
+        st.begin(L.key[L.perm[s0]]);
         for (uint32_t i = s0; i < s1; i += 4)
         {
             // the planes of up to 4 records are fetched together, then folded in order
@@ -2190,7 +2188,7 @@ __device__ __forceinline__ uint64_t row_vid(const FoldArgs &a, const FoldLds<CAP
 // batch's shared numbering) and returns the brick's index inside its mesh.
 __device__ __forceinline__ uint32_t enter_brick(FoldArgs &a, uint32_t brick)
 {
-    const MeshSeg &sg = a.b.seg[seg_find(a.b.brick_off, a.b.M, (uint64_t)brick)];
+    const MeshSeg &sg = a.b.seg[mesh_of_brick(a.b, (uint64_t)brick)];
     a.g = sg.g;
     a.nb[0] = sg.nb[0];
     a.nb[1] = sg.nb[1];
@@ -2394,7 +2392,11 @@ __global__ __launch_bounds__(BT) void k_fold_large(FoldArgs ain)
                     __syncthreads();
                     if (tid == 0)
                         for (uint32_t i = 0; i < m; ++i)
+                        {
+                            if (done == 0 && i == 0)
+                                st.begin(L.key[L.perm[0]]);
                             st.add(L.key[L.perm[i]], L.z[L.perm[i]], a);
+                        }
                     __syncthreads();
                     last = T;
                     have_last = true;
@@ -2463,6 +2465,7 @@ __global__ __launch_bounds__(BT) void k_fold_large(FoldArgs ain)
                     continue;
                 typename FoldPick<P>::type st;
                 st.init(x0 + (lv & 7), y0 + ((lv >> 3) & 7), z0 + (lv >> 6), a);
+                st.begin(L.key[L.perm[s0]]);
                 for (uint32_t i = s0; i < s1; ++i)
                 {
                     const uint32_t j = L.perm[i];
@@ -2500,7 +2503,7 @@ __global__ __launch_bounds__(BT) void k_solve_voxels(const float *qef, const int
         bool inside = true;
         if (live)
         {
-            const GridDev &g = bt.seg[seg_find(bt.vox_off, bt.M, vid)].g; // the voxel's mesh
+            const GridDev &g = bt.seg[mesh_of_voxel(bt, vid)].g; // the voxel's mesh
 #pragma unroll
             for (int i = 0; i < 3; ++i)
                 vs[i] = g.vs[i];
@@ -2573,7 +2576,7 @@ __global__ __launch_bounds__(BT) void k_solve_voxels_c(const float *qef, const i
         bool used64 = false;
         if (live)
         {
-            const GridDev &g = bt.seg[seg_find(bt.vox_off, bt.M, vid)].g; // the voxel's mesh
+            const GridDev &g = bt.seg[mesh_of_voxel(bt, vid)].g; // the voxel's mesh
             float qh[10], ql[10];
 #pragma unroll
             for (int i = 0; i < 10; ++i)
@@ -2659,7 +2662,7 @@ __global__ __launch_bounds__(BT) void k_solve_box(ShardBuf<BoxItem> box, BatchDe
     for (uint64_t qi = (uint64_t)blockIdx.x * BT + threadIdx.x; qi < n; qi += stride)
     {
         const BoxItem e = box.items[queue_slot(box, qm.pre, qi)];
-        const GridDev &g = bt.seg[seg_find(bt.vox_off, bt.M, e.vid)].g; // the voxel's mesh
+        const GridDev &g = bt.seg[mesh_of_voxel(bt, e.vid)].g; // the voxel's mesh
         float v[3] = {e.v[0], e.v[1], e.v[2]};
         solve_box(e.q, e.c, g.vs, v);
         float *o = dual + 3 * e.vid;
