@@ -17,15 +17,27 @@ namespace ovx
 
 // Record keys: the 2 top bits hold the accumulation phase, the rest the
 // position of the contribution in the CPU's serial order.
-//   phase 0 (intersection): (face*3 + axis) << 28 | ly << 15 | lx << 2 | dx << 1 | dy
+//   phase 0 (intersection): face << 30 | axis << 28 | ly << 15 | lx << 2 | dx << 1 | dy
+//                           (the order of face*3 + axis, with both as plain fields)
 //   phase 1 (face term):    face
 //   phase 2 (boundary):     rank of the (min, max) vertex pair
 constexpr int KEY_PHASE_SHIFT = 62;
-constexpr int KEY_FA_SHIFT = 28;
+constexpr int KEY_FACE_SHIFT = 30;
+constexpr int KEY_AX_SHIFT = 28;
 constexpr int KEY_Y_SHIFT = 15;
 constexpr int KEY_X_SHIFT = 2;
 constexpr uint64_t KEY_IDX_MASK = 0x1fff;  // 13 bits: grid extent <= 8192
-constexpr uint64_t KEY_FA_MASK = (1ull << 34) - 1;
+constexpr uint64_t KEY_FACE_MASK = 0xffffffffull;
+
+// the axes after ax in cyclic order: (ax + 1) % 3 and (ax + 2) % 3
+__host__ __device__ __forceinline__ int axis_next(int ax)
+{
+    return ax == 2 ? 0 : ax + 1;
+}
+__host__ __device__ __forceinline__ int axis_prev(int ax)
+{
+    return ax == 0 ? 2 : ax - 1;
+}
 constexpr int MAX_GRID_EXTENT = 8192;
 
 struct GridDev

@@ -377,8 +377,8 @@ constexpr uint32_t QUEUE_GROUP = 32; // row chunks claimed together (from one sh
 __device__ __forceinline__ void scan_item_setup(const MeshDev &m, const GridDev &g, int64_t face, int ax2,
                                                 ScanItem &it)
 {
-    const int ax0 = (ax2 + 1) % 3;
-    const int ax1 = (ax2 + 2) % 3;
+    const int ax0 = axis_next(ax2);
+    const int ax1 = axis_prev(ax2);
     float v[3][3];
     load_tri(m, face, v[0], v[1], v[2]);
     float t[3][3];
@@ -613,7 +613,7 @@ __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, BatchDev bt, uint
                 // negative one
                 R = it.end > it.start ? (uint32_t)(it.end - it.start) : 0u;
                 // columns of a row are bounded by the x extent of the triangle
-                const float vs0 = sel3f((it.ax2 + 1) % 3, g.vs[0], g.vs[1], g.vs[2]);
+                const float vs0 = sel3f(axis_next(it.ax2), g.vs[0], g.vs[1], g.vs[2]);
                 const float xlo = sminf(sminf(it.t[0][0], it.t[1][0]), it.t[2][0]) / vs0;
                 const float xhi = smaxf(smaxf(it.t[0][0], it.t[1][0]), it.t[2][0]) / vs0;
                 const float ext = xhi - xlo + 2.0f;
@@ -674,8 +674,8 @@ __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, BatchDev bt, uint
                 const uint32_t R_L = (uint32_t)(it.end - it.start);
                 const int y_idx = it.start + (int)(gi - (incl_items[L] - R_L));
                 const int ax2 = it.ax2;
-                const int ax0 = (ax2 + 1) % 3;
-                const int ax1 = (ax2 + 2) % 3;
+                const int ax0 = axis_next(ax2);
+                const int ax1 = axis_prev(ax2);
                 const bool first = y_idx < it.mid;
                 const int i0 = first ? 0 : 2, i2 = first ? 2 : 0;
                 const double t0x = it.t[i0][0], t0y = it.t[i0][1], t0z = it.t[i0][2];
@@ -728,8 +728,8 @@ __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, BatchDev bt, uint
                     const uint32_t ex = (rr == 0) ? 0u : incl_rows[rr - 1];
                     const int x_idx = rw.line_start + (int)(h - ex);
                     const int ax2 = it.ax2;
-                    const int ax0 = (ax2 + 1) % 3;
-                    const int ax1 = (ax2 + 2) % 3;
+                    const int ax0 = axis_next(ax2);
+                    const int ax1 = axis_prev(ax2);
                     const float vs0 = sel3f(ax0, g.vs[0], g.vs[1], g.vs[2]);
                     const float vs2 = sel3f(ax2, g.vs[0], g.vs[1], g.vs[2]);
                     const double x = (double)((float)(x_idx + 1) * vs0);
@@ -742,8 +742,9 @@ __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, BatchDev bt, uint
                         ++n_ev;
                         const int ylo = sel3i(ax1, g.gmin[0], g.gmin[1], g.gmin[2]);
                         const int xlo = sel3i(ax0, g.gmin[0], g.gmin[1], g.gmin[2]);
-                        const uint64_t fa = (uint64_t)it.face * 3u + (uint64_t)ax2;
-                        const uint64_t keybase = (fa << KEY_FA_SHIFT) | ((uint64_t)(rw.y_idx - ylo) << KEY_Y_SHIFT) |
+                        const uint64_t keybase = ((uint64_t)it.face << KEY_FACE_SHIFT) |
+                                                 ((uint64_t)ax2 << KEY_AX_SHIFT) |
+                                                 ((uint64_t)(rw.y_idx - ylo) << KEY_Y_SHIFT) |
                                                  ((uint64_t)(x_idx - xlo) << KEY_X_SHIFT);
                         mark_event<MERGE>(g, T, ax0, ax1, x_idx, rw.y_idx, z_idx);
                         evr.key = keybase;
@@ -1388,13 +1389,12 @@ __global__ __launch_bounds__(BT) void k_gen_events(ShardBuf<Event> ev, BucketArg
         if (i < n)
         {
             const Event e = ev.items[queue_slot(ev, qm.pre, i)];
-            const uint64_t fa = (e.key >> KEY_FA_SHIFT) & KEY_FA_MASK;
-            const uint64_t face = fa / 3u;
+            const uint64_t face = (e.key >> KEY_FACE_SHIFT) & KEY_FACE_MASK;
             const MeshSeg &sg = ba.b.seg[mesh_of_face(ba.b, (int64_t)face)];
             const GridDev &g = sg.g;
-            const int ax2 = (int)(fa - 3u * face);
-            const int ax0 = (ax2 + 1) % 3;
-            const int ax1 = (ax2 + 2) % 3;
+            const int ax2 = (int)((e.key >> KEY_AX_SHIFT) & 3ull);
+            const int ax0 = axis_next(ax2);
+            const int ax1 = axis_prev(ax2);
             const int y_idx =
                 (int)((e.key >> KEY_Y_SHIFT) & KEY_IDX_MASK) + sel3i(ax1, g.gmin[0], g.gmin[1], g.gmin[2]);
             const int x_idx =
@@ -1681,7 +1681,7 @@ struct FoldState
     {
         const int phase = (int)(key >> KEY_PHASE_SHIFT);
         if (phase == 0)
-            return (uint32_t)(((key >> KEY_FA_SHIFT) & KEY_FA_MASK) / 3u);
+            return (uint32_t)((key >> KEY_FACE_SHIFT) & KEY_FACE_MASK);
         if (phase == 1)
             return (uint32_t)(key & 0xffffffffull);
         return NO_PLANE;
@@ -1732,11 +1732,9 @@ struct FoldState
             q[k] = q[k] + qf[k];
         if (phase == 0)
         {
-            const uint64_t fa = (key >> KEY_FA_SHIFT) & KEY_FA_MASK;
-            const uint32_t f = (uint32_t)(fa / 3u);
-            const int ax2 = (int)(fa - 3ull * f);
-            const int ax0 = (ax2 + 1) % 3;
-            const int ax1 = (ax2 + 2) % 3;
+            const int ax2 = (int)((key >> KEY_AX_SHIFT) & 3ull);
+            const int ax0 = axis_next(ax2);
+            const int ax1 = axis_prev(ax2);
             const int y_idx =
                 (int)((key >> KEY_Y_SHIFT) & KEY_IDX_MASK) + sel3i(ax1, a.g.gmin[0], a.g.gmin[1], a.g.gmin[2]);
             const int x_idx =
@@ -1885,11 +1883,9 @@ struct FoldStateC
         const int phase = (int)(key >> KEY_PHASE_SHIFT);
         if (phase == 0)
         {
-            const uint64_t fa = (key >> KEY_FA_SHIFT) & KEY_FA_MASK;
-            const uint32_t f = (uint32_t)(fa / 3u);
-            const int ax2 = (int)(fa - 3ull * f);
-            const int ax0 = (ax2 + 1) % 3;
-            const int ax1 = (ax2 + 2) % 3;
+            const int ax2 = (int)((key >> KEY_AX_SHIFT) & 3ull);
+            const int ax0 = axis_next(ax2);
+            const int ax1 = axis_prev(ax2);
             const int y_idx =
                 (int)((key >> KEY_Y_SHIFT) & KEY_IDX_MASK) + sel3i(ax1, a.g.gmin[0], a.g.gmin[1], a.g.gmin[2]);
             const int x_idx =
@@ -2055,6 +2051,30 @@ __device__ __forceinline__ void for_each_rec(const FoldArgs &a, uint32_t brick, 
     }
 }
 
+// key, crossing coordinate and face plane of the grouped records i .. i+3 below s1
+// (zeros past s1; boundary records have no plane)
+template <int CAP>
+__device__ __forceinline__ void fetch_recs(const FoldArgs &a, const FoldLds<CAP> &L, uint32_t i, uint32_t s1,
+                                           uint64_t kk[4], float zz[4], float4 pp[4])
+{
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+    {
+        kk[u] = 0;
+        zz[u] = 0.0f;
+        pp[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (i + u < s1)
+        {
+            const uint32_t j = L.perm[i + u];
+            kk[u] = L.key[j];
+            zz[u] = L.z[j];
+            const uint32_t f = FoldState::plane_of(kk[u]);
+            if (f != FoldState::NO_PLANE)
+                pp[u] = a.planes[f];
+        }
+    }
+}
+
 // The brick's n records are in L.key/z/lv and L.cur holds their per-voxel
 // histogram: group them by voxel (counting sort), order each voxel's records
 // by key (rank sort, keys are unique), replay them in that order, one thread
@@ -2126,22 +2146,7 @@ __device__ __forceinline__ void fold_loaded(const FoldArgs &a, FoldLds<CAP> &L, 
             uint64_t kk[4];
             float zz[4];
             float4 pp[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-            {
-                kk[u] = 0;
-                zz[u] = 0.0f;
-                pp[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                if (i + u < s1)
-                {
-                    const uint32_t j = L.perm[i + u];
-                    kk[u] = L.key[j];
-                    zz[u] = L.z[j];
-                    const uint32_t f = FoldState::plane_of(kk[u]);
-                    if (f != FoldState::NO_PLANE)
-                        pp[u] = a.planes[f];
-                }
-            }
+            fetch_recs<CAP>(a, L, i, s1, kk, zz, pp);
 #pragma unroll
             for (int u = 0; u < 4; ++u)
                 if (i + u < s1)
