@@ -2160,8 +2160,8 @@ __device__ __forceinline__ void fold_loaded(const FoldArgs &a, FoldLds<CAP> &L, 
 // Output rows of the brick's voxels: the 8 voxels of a (y, z) row of a brick
 // share one 64-bit occupancy word (bricks start at multiples of 8 along x).
 // Loaded by the first wave; read after the block's next barrier.
-template <int CAP>
-__device__ __forceinline__ void load_rows(const FoldArgs &a, FoldLds<CAP> &L, int x0, int y0, int z0)
+template <class LDS>
+__device__ __forceinline__ void load_rows(const FoldArgs &a, LDS &L, int x0, int y0, int z0)
 {
     const int r = threadIdx.x;
     if (r < 64)
@@ -2181,8 +2181,8 @@ __device__ __forceinline__ void load_rows(const FoldArgs &a, FoldLds<CAP> &L, in
     }
 }
 
-template <int CAP>
-__device__ __forceinline__ uint64_t row_vid(const FoldArgs &a, const FoldLds<CAP> &L, int x0, int l)
+template <class LDS>
+__device__ __forceinline__ uint64_t row_vid(const FoldArgs &a, const LDS &L, int x0, int l)
 {
     const int row = l >> 3;
     const int bit = (x0 - a.g.gmin[0] + (l & 7)) & 63;
@@ -2213,13 +2213,129 @@ __device__ __forceinline__ void brick_origin(const FoldArgs &a, uint32_t brick, 
     z0 = a.g.gmin[2] + 8 * bz;
 }
 
+// LDS of the small-brick fold: the brick's records grouped by voxel and,
+// inside each voxel, in key order
+template <int CAP>
+struct FoldLdsG
+{
+    uint64_t key[CAP];
+    float z[CAP];
+    uint32_t off[513];   // first record of each voxel
+    uint32_t cur[512];   // per-voxel record counts, then fill cursors
+    uint16_t vlist[512]; // voxels with records
+    uint64_t rbits[64];  // occupancy word of each (y, z) row of the brick
+    uint32_t roff[64];   // ... and its first voxel id
+    uint32_t sh[NWAVE];
+    ChunkBatch cb;
+};
+
+// The brick's n records are in registers (record g = tid + k*BT in slot k)
+// and L.cur holds their per-voxel histogram: each record goes to its voxel's
+// segment, is ranked there among the voxel's keys (keys are unique) and
+// moved to its rank; then one thread per voxel with records replays them in
+// key order and solves.  Contains barriers.
+template <int CAP, int P>
+__device__ __forceinline__ void fold_grouped(const FoldArgs &a, FoldLdsG<CAP> &L, uint32_t n,
+                                             const uint64_t (&rk)[CAP / BT], const float (&rz)[CAP / BT],
+                                             const uint32_t (&rl)[CAP / BT], int x0, int y0, int z0)
+{
+    constexpr int PER = CAP / BT;
+    const int tid = threadIdx.x;
+    uint32_t nv;
+    {
+        const uint32_t c0 = L.cur[2 * tid], c1 = L.cur[2 * tid + 1];
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan(c0 + c1, L.sh, tot);
+        L.off[2 * tid] = ex;
+        L.off[2 * tid + 1] = ex + c0;
+        L.cur[2 * tid] = ex;
+        L.cur[2 * tid + 1] = ex + c0;
+        if (tid == 0)
+            L.off[512] = tot;
+        const uint32_t nz = (c0 ? 1u : 0u) + (c1 ? 1u : 0u);
+        const uint32_t exv = block_excl_scan(nz, L.sh, nv);
+        if (c0)
+            L.vlist[exv] = (uint16_t)(2 * tid);
+        if (c1)
+            L.vlist[exv + (c0 ? 1u : 0u)] = (uint16_t)(2 * tid + 1);
+    }
+    __syncthreads();
+    // keys into their voxels' segments, in any order
+#pragma unroll
+    for (int k = 0; k < PER; ++k)
+        if ((uint32_t)(tid + k * BT) < n)
+            L.key[atomicAdd(&L.cur[rl[k]], 1u)] = rk[k];
+    __syncthreads();
+    uint32_t pos[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k)
+    {
+        pos[k] = 0;
+        if ((uint32_t)(tid + k * BT) < n)
+        {
+            const uint32_t s0 = L.off[rl[k]], s1 = L.off[rl[k] + 1];
+            uint32_t rank = 0;
+            for (uint32_t t = s0; t < s1; ++t)
+                rank += (L.key[t] < rk[k]) ? 1u : 0u;
+            pos[k] = s0 + rank;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; ++k)
+        if ((uint32_t)(tid + k * BT) < n)
+        {
+            L.key[pos[k]] = rk[k];
+            L.z[pos[k]] = rz[k];
+        }
+    __syncthreads();
+    for (uint32_t vi = tid; vi < nv; vi += BT)
+    {
+        const int l = L.vlist[vi];
+        const uint32_t s0 = L.off[l], s1 = L.off[l + 1];
+        typename FoldPick<P>::type st;
+        st.init(x0 + (l & 7), y0 + ((l >> 3) & 7), z0 + (l >> 6), a);
+        st.begin(L.key[s0]);
+        for (uint32_t i = s0; i < s1; i += 4)
+        {
+            // the planes of up to 4 records are fetched together, then folded in order
+            uint64_t kk[4];
+            float zz[4];
+            float4 pp[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+            {
+                kk[u] = 0;
+                zz[u] = 0.0f;
+                pp[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (i + u < s1)
+                {
+                    kk[u] = L.key[i + u];
+                    zz[u] = L.z[i + u];
+                    const uint32_t f = FoldState::plane_of(kk[u]);
+                    if (f != FoldState::NO_PLANE)
+                        pp[u] = a.planes[f];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (i + u < s1)
+                    st.add_pl(kk[u], zz[u], pp[u], a);
+        }
+        st.finish_at(row_vid(a, L, x0, l), a);
+    }
+    __syncthreads();
+}
+
 // One workgroup per brick that received records, for bricks of at most
 // FOLD_SMALL records (the others are listed for k_fold_large): the records
-// are read once into LDS, grouped, replayed and solved there.
+// are read once into registers, grouped and ordered in LDS, replayed and
+// solved there.
 template <int P>
 __global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs ain)
 {
-    __shared__ FoldLds<FOLD_SMALL> L;
+    __shared__ FoldLdsG<FOLD_SMALL> L;
+    constexpr int PER = FOLD_SMALL / BT;
     const int tid = threadIdx.x;
     const uint64_t n_active = *ain.n_active;
     for (uint64_t bi = blockIdx.x; bi < n_active; bi += gridDim.x)
@@ -2233,7 +2349,18 @@ __global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs ain)
         load_rows(a, L, x0, y0, z0);
         for (int i = tid; i < 512; i += BT)
             L.cur[i] = 0;
-        // the brick's chunks, BT at a time: records appended to the LDS tile
+        // the brick's chunks, BT at a time: record g of the brick goes to
+        // slot g / BT of thread g % BT
+        uint64_t rk[PER];
+        float rz[PER];
+        uint32_t rl[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+        {
+            rk[k] = 0;
+            rz[k] = 0.0f;
+            rl[k] = 0;
+        }
         uint32_t n = 0;
         bool over = false;
         for (uint32_t cbeg = c0; cbeg < c1; cbeg += BT)
@@ -2255,15 +2382,21 @@ __global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs ain)
             }
             L.cb.incl[tid] = ex + cnt;
             __syncthreads();
-            for (uint32_t j = tid; j < tot; j += BT)
+#pragma unroll
+            for (int k = 0; k < PER; ++k)
             {
-                const int k = upper_find(L.cb.incl, nb, j);
-                const uint32_t before = k ? L.cb.incl[k - 1] : 0u;
-                const Rec r = a.recs[(uint64_t)L.cb.start[k] + (j - before)];
-                L.key[n + j] = r.key;
-                L.z[n + j] = r.z;
-                L.lv[n + j] = (uint16_t)r.lvox;
-                atomicAdd(&L.cur[r.lvox], 1u);
+                const uint32_t g = (uint32_t)(tid + k * BT);
+                if (g >= n && g < n + tot)
+                {
+                    const uint32_t j = g - n;
+                    const int c = upper_find(L.cb.incl, nb, j);
+                    const uint32_t before = c ? L.cb.incl[c - 1] : 0u;
+                    const Rec r = a.recs[(uint64_t)L.cb.start[c] + (j - before)];
+                    rk[k] = r.key;
+                    rz[k] = r.z;
+                    rl[k] = r.lvox;
+                    atomicAdd(&L.cur[r.lvox], 1u);
+                }
             }
             __syncthreads();
             n += tot;
@@ -2275,7 +2408,7 @@ __global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs ain)
             __syncthreads();
             continue;
         }
-        fold_loaded<FOLD_SMALL, P>(a, L, n, x0, y0, z0);
+        fold_grouped<FOLD_SMALL, P>(a, L, n, rk, rz, rl, x0, y0, z0);
     }
 }
 
