@@ -206,9 +206,8 @@ template <int LOG2>
 struct BucketLds
 {
     static constexpr int N = 1 << LOG2;
-    uint32_t key[N];
+    uint32_t key[N]; // brick of the slot, then the first record of its chunk
     uint32_t cnt[N];
-    uint32_t base[N];
     unsigned long long sh[NWAVE];
     unsigned long long rbase, cbase;
     bool fits;
@@ -287,10 +286,11 @@ __device__ __forceinline__ void bucket_emit(const uint32_t (&bk)[K], const Rec (
         const uint32_t n = L.cnt[s0 + e];
         if (n && fits)
         {
+            // slots s0 .. s0+PER-1 are this thread's alone here
             const uint32_t b = L.key[s0 + e];
             const uint32_t r = atomicAdd(a.brick_nch + b, 1u);
             a.chunks.items[cpos] = Chunk{(uint32_t)rpos, n, b, r};
-            L.base[s0 + e] = (uint32_t)rpos;
+            L.key[s0 + e] = (uint32_t)rpos;
         }
         rpos += n;
         cpos += n ? 1ull : 0ull;
@@ -301,7 +301,7 @@ __device__ __forceinline__ void bucket_emit(const uint32_t (&bk)[K], const Rec (
 #pragma unroll
         for (int k = 0; k < K; ++k)
             if (ok[k])
-                a.recs.items[(uint64_t)L.base[slot[k]] + rank[k]] = rc[k];
+                a.recs.items[(uint64_t)L.key[slot[k]] + rank[k]] = rc[k];
     }
     __syncthreads();
 }
@@ -1990,7 +1990,7 @@ struct FoldPick<1>
     using type = FoldStateC;
 };
 
-constexpr int FOLD_SMALL = 2048; // records of a brick held in LDS by the main fold kernel (4 blocks per CU)
+constexpr int FOLD_SMALL = 2048; // records of a brick held in LDS by the main fold kernel (5 blocks per CU)
 constexpr int FOLD_CAP = 4096;   // records held in LDS at a time by the kernel for large bricks (2 blocks per CU)
 
 // LDS view of a batch of up to BT chunks of one brick
@@ -2218,9 +2218,11 @@ __device__ __forceinline__ void brick_origin(const FoldArgs &a, uint32_t brick, 
 template <int CAP>
 struct FoldLdsG
 {
+    // 16-bit offsets keep the 2048-record layout under 32 KB: 5 blocks per CU
+    static_assert(CAP <= 65535, "16-bit record offsets");
     uint64_t key[CAP];
     float z[CAP];
-    uint32_t off[513];   // first record of each voxel
+    uint16_t off[513];   // first record of each voxel
     uint32_t cur[512];   // per-voxel record counts, then fill cursors
     uint16_t vlist[512]; // voxels with records
     uint64_t rbits[64];  // occupancy word of each (y, z) row of the brick
@@ -2246,12 +2248,12 @@ __device__ __forceinline__ void fold_grouped(const FoldArgs &a, FoldLdsG<CAP> &L
         const uint32_t c0 = L.cur[2 * tid], c1 = L.cur[2 * tid + 1];
         uint32_t tot;
         const uint32_t ex = block_excl_scan(c0 + c1, L.sh, tot);
-        L.off[2 * tid] = ex;
-        L.off[2 * tid + 1] = ex + c0;
+        L.off[2 * tid] = (uint16_t)ex;
+        L.off[2 * tid + 1] = (uint16_t)(ex + c0);
         L.cur[2 * tid] = ex;
         L.cur[2 * tid + 1] = ex + c0;
         if (tid == 0)
-            L.off[512] = tot;
+            L.off[512] = (uint16_t)tot;
         const uint32_t nz = (c0 ? 1u : 0u) + (c1 ? 1u : 0u);
         const uint32_t exv = block_excl_scan(nz, L.sh, nv);
         if (c0)
@@ -2260,17 +2262,24 @@ __device__ __forceinline__ void fold_grouped(const FoldArgs &a, FoldLdsG<CAP> &L
             L.vlist[exv + (c0 ? 1u : 0u)] = (uint16_t)(2 * tid + 1);
     }
     __syncthreads();
-    // keys into their voxels' segments, in any order
+    // keys into their voxels' segments, in any order (the slot loops stop
+    // at the block's last slot in use)
 #pragma unroll
     for (int k = 0; k < PER; ++k)
+    {
+        if ((uint32_t)(k * BT) >= n)
+            break;
         if ((uint32_t)(tid + k * BT) < n)
             L.key[atomicAdd(&L.cur[rl[k]], 1u)] = rk[k];
+    }
     __syncthreads();
     uint32_t pos[PER];
 #pragma unroll
     for (int k = 0; k < PER; ++k)
     {
         pos[k] = 0;
+        if ((uint32_t)(k * BT) >= n)
+            break;
         if ((uint32_t)(tid + k * BT) < n)
         {
             const uint32_t s0 = L.off[rl[k]], s1 = L.off[rl[k] + 1];
@@ -2283,11 +2292,15 @@ __device__ __forceinline__ void fold_grouped(const FoldArgs &a, FoldLdsG<CAP> &L
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < PER; ++k)
+    {
+        if ((uint32_t)(k * BT) >= n)
+            break;
         if ((uint32_t)(tid + k * BT) < n)
         {
             L.key[pos[k]] = rk[k];
             L.z[pos[k]] = rz[k];
         }
+    }
     __syncthreads();
     for (uint32_t vi = tid; vi < nv; vi += BT)
     {
@@ -2385,6 +2398,11 @@ __global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs ain)
 #pragma unroll
             for (int k = 0; k < PER; ++k)
             {
+                // slots of this batch's records only
+                if ((uint32_t)(k * BT) >= n + tot)
+                    break;
+                if ((uint32_t)((k + 1) * BT) <= n)
+                    continue;
                 const uint32_t g = (uint32_t)(tid + k * BT);
                 if (g >= n && g < n + tot)
                 {
