@@ -449,17 +449,27 @@ __device__ __forceinline__ void event_voxels(int ax0, int ax1, int x_idx, int y_
 // block's faces fall on few bitset words, so their bits are merged in LDS and
 // each distinct word is written with one global atomicOr when the table is
 // flushed (device-scope atomics are far costlier than LDS ones).
-constexpr int MARK_LOG2 = 11;
-constexpr int MARK_TAB = 1 << MARK_LOG2;
+// merge table entries: 2048 for the smallest tiles (meshes of large
+// triangles, whose crossings are densest), 1024 otherwise (a smaller table
+// leaves LDS for a fourth block per CU)
+constexpr int mark_log2(int tile)
+{
+    return tile <= 64 ? 11 : 10;
+}
 constexpr uint32_t MARK_EMPTY = 0xffffffffu; // word indices are < 2^32 - 1
 
+template <int LOG2>
 struct MarkTable
 {
-    uint32_t key[MARK_TAB];
-    unsigned long long mask[MARK_TAB];
+    static constexpr int LOG = LOG2;
+    static constexpr int N = 1 << LOG2;
+    uint32_t key[N];
+    unsigned long long mask[N];
 };
 struct NoMarkTable // kernel variant without merging: no LDS
 {
+    static constexpr int LOG = 0;
+    static constexpr int N = 1;
     uint32_t key[1];
     unsigned long long mask[1];
 };
@@ -468,7 +478,7 @@ template <class Tab>
 __device__ __forceinline__ void mark_word(const GridDev &g, Tab &T, uint64_t w, uint64_t msk)
 {
     const uint32_t key = (uint32_t)w;
-    uint32_t h = (key * 2654435761u) >> (32 - MARK_LOG2);
+    uint32_t h = (key * 2654435761u) >> (32 - Tab::LOG);
 #pragma unroll 1
     for (int probe = 0; probe < 4; ++probe)
     {
@@ -478,7 +488,7 @@ __device__ __forceinline__ void mark_word(const GridDev &g, Tab &T, uint64_t w, 
             atomicOr(&T.mask[h], (unsigned long long)msk);
             return;
         }
-        h = (h + 1u) & (uint32_t)(MARK_TAB - 1);
+        h = (h + 1u) & (uint32_t)(Tab::N - 1);
     }
     atomicOr((unsigned long long *)(g.bits + w), (unsigned long long)msk); // table crowded
 }
@@ -488,7 +498,7 @@ template <class Tab>
 __device__ __forceinline__ void mark_flush(const GridDev &g, Tab &T)
 {
     __syncthreads();
-    for (int i = threadIdx.x; i < MARK_TAB; i += BT)
+    for (int i = threadIdx.x; i < Tab::N; i += BT)
     {
         const uint32_t key = T.key[i];
         if (key != MARK_EMPTY)
@@ -546,7 +556,7 @@ __device__ __forceinline__ void mark_event(const GridDev &g, Tab &T, int ax0, in
 // QUEUED = false: one item per (face, axis); QUEUED = true: the queued row
 // chunks of large triangles (items index the queue's slots)
 template <bool MERGE, bool QUEUED, int MARK_TILE>
-__global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, BatchDev bt, uint64_t n_tiles,
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_scan_events(MeshDev m, BatchDev bt, uint64_t n_tiles,
                                                     unsigned long long *counters, ShardBuf<Event> eb,
                                                     ShardBuf<RowChunk> rq)
 {
@@ -558,7 +568,7 @@ __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, BatchDev bt, uint
     __shared__ uint32_t incl_items[MARK_TILE];
     __shared__ uint32_t incl_rows[BT];
     __shared__ uint32_t sh[NWAVE];
-    __shared__ typename std::conditional<MERGE, MarkTable, NoMarkTable>::type T;
+    __shared__ typename std::conditional<MERGE, MarkTable<mark_log2(MARK_TILE)>, NoMarkTable>::type T;
     __shared__ QueueMap qm;
     const int tid = threadIdx.x;
     int64_t n_items = 0;
@@ -570,7 +580,7 @@ __global__ __launch_bounds__(BT) void k_scan_events(MeshDev m, BatchDev bt, uint
     uint32_t n_ev = 0;
     if constexpr (MERGE)
     {
-        for (int i = tid; i < MARK_TAB; i += BT)
+        for (int i = tid; i < T.N; i += BT)
         {
             T.key[i] = MARK_EMPTY;
             T.mask[i] = 0ull;
@@ -960,7 +970,7 @@ __device__ __forceinline__ uint64_t row_mask(int lo_bit, int hi_bit)
 // QUEUED = false: one item per face; QUEUED = true: the queued row chunks of
 // large faces (items index the queue's slots)
 template <bool QUEUED, int FACE_TILE>
-__global__ __launch_bounds__(BT) void k_face(MeshDev m, BatchDev bt, uint64_t n_tiles, ShardBuf<FHit> hb,
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6))) void k_face(MeshDev m, BatchDev bt, uint64_t n_tiles, ShardBuf<FHit> hb,
                                              unsigned long long *ctr, ShardBuf<FaceChunk> fq)
 {
     static_assert(FACE_TILE <= BT, "a tile's faces are set up one per thread");
