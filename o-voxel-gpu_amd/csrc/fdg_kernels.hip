@@ -1629,29 +1629,65 @@ __global__ __launch_bounds__(BT) void k_scan_down(Load ld, uint64_t n, const uin
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BT) void k_emit_coords(BatchDev bt, uint64_t n_words, int32_t *coords)
 {
-    const uint64_t stride = (uint64_t)gridDim.x * BT;
-    for (uint64_t gw = (uint64_t)blockIdx.x * BT + threadIdx.x; gw < n_words; gw += stride)
+    // Each wave takes 64 consecutive words of the shared numbering (their
+    // voxel ids are consecutive too, across meshes) and its lanes write the
+    // words' voxels at consecutive ids: contiguous stores across the wave.
+    const int lane = threadIdx.x & 63;
+    const uint64_t wave = ((uint64_t)blockIdx.x * BT + (uint64_t)threadIdx.x) >> 6;
+    const uint64_t n_waves = ((uint64_t)gridDim.x * BT) >> 6;
+    for (uint64_t gw0 = wave * 64ull; gw0 < n_words; gw0 += n_waves * 64ull)
     {
-        const int ms = mesh_of_word(bt, gw);
-        const GridDev &g = bt.seg[ms].g;
-        const uint64_t w = gw - bt.word_off[ms]; // word of the mesh
-        uint64_t m = g.bits[w];
-        if (!m)
-            continue;
-        uint32_t id = g.woff[w];
-        const uint64_t row = w / (uint64_t)g.wx;
-        const int xw = (int)(w - row * (uint64_t)g.wx);
-        const int y = (int)(row % (uint64_t)g.G[1]) + g.gmin[1];
-        const int z = (int)(row / (uint64_t)g.G[1]) + g.gmin[2];
-        while (m)
+        const uint64_t gw = gw0 + (uint64_t)lane;
+        uint64_t m = 0;
+        uint32_t id = 0;
+        int xb = 0, y = 0, z = 0;
+        if (gw < n_words)
         {
-            const int b = __builtin_ctzll(m);
-            m &= m - 1ull;
-            int32_t *o = coords + 3 * (uint64_t)id;
-            o[0] = g.gmin[0] + xw * 64 + b;
-            o[1] = y;
-            o[2] = z;
-            ++id;
+            const int ms = mesh_of_word(bt, gw);
+            const GridDev &g = bt.seg[ms].g;
+            const uint64_t w = gw - bt.word_off[ms]; // word of the mesh
+            m = g.bits[w];
+            if (m)
+            {
+                id = g.woff[w];
+                const uint64_t row = w / (uint64_t)g.wx;
+                xb = g.gmin[0] + 64 * (int)(w - row * (uint64_t)g.wx);
+                y = (int)(row % (uint64_t)g.G[1]) + g.gmin[1];
+                z = (int)(row / (uint64_t)g.G[1]) + g.gmin[2];
+            }
+        }
+        const uint32_t pc = (uint32_t)__popcll(m);
+        const uint32_t incl = wave_incl_scan(pc);
+        const uint32_t total = __shfl(incl, 63, 64);
+        if (total == 0)
+            continue; // wave-uniform
+        const uint64_t nz = __ballot(pc != 0u);
+        const uint32_t base = __shfl(id, __ffsll((long long)nz) - 1, 64); // id of the wave's first voxel
+        const uint32_t excl = incl - pc;
+        // every lane takes part in every round (the shuffles read all lanes)
+        for (uint32_t k0 = 0; k0 < total; k0 += 64u)
+        {
+            const uint32_t k = k0 + (uint32_t)lane;
+            // the word of voxel k: lane j = number of lanes with incl <= k
+            int j = 0;
+#pragma unroll
+            for (int step = 32; step >= 1; step >>= 1)
+            {
+                const uint32_t v = __shfl(incl, j + step - 1, 64);
+                j += (v <= k) ? step : 0;
+            }
+            j = j < 63 ? j : 63;
+            const uint64_t mj = __shfl(m, j, 64);
+            const uint32_t ej = __shfl(excl, j, 64);
+            const int xj = __shfl(xb, j, 64), yj = __shfl(y, j, 64), zj = __shfl(z, j, 64);
+            if (k < total)
+            {
+                const int bit = select_bit(mj, k - ej);
+                int32_t *o = coords + 3 * (uint64_t)(base + k);
+                o[0] = xj + bit;
+                o[1] = yj;
+                o[2] = zj;
+            }
         }
     }
 }
