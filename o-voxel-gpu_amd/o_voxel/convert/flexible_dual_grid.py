@@ -47,60 +47,80 @@ def _host_vector(value, name: str, dtype: torch.dtype, kind: str) -> torch.Tenso
     return t
 
 
+def _aabb_host(aabb) -> torch.Tensor:
+    """Explicit aabb -> CPU float32 [2, 3] tensor (reference: flexible_dual_grid.py:129-144)."""
+    if isinstance(aabb, (list, tuple)):
+        aabb = np.array(aabb)
+    if isinstance(aabb, np.ndarray):
+        aabb_cpu = torch.tensor(aabb, dtype=torch.float32)
+    else:
+        assert isinstance(aabb, torch.Tensor), (
+            f"aabb must be a list, tuple, np.ndarray, or torch.Tensor, but got {type(aabb)}"
+        )
+        assert not aabb.is_cuda, "aabb Tensor metadata must be on CPU; pass Python values to avoid implicit CUDA sync"
+        aabb_cpu = aabb.to(dtype=torch.float32).contiguous()
+    assert aabb_cpu.dim() == 2, f"aabb must be a 2D tensor, but got {aabb_cpu.shape}"
+    assert aabb_cpu.size(0) == 2, f"aabb must have 2 rows, but got {aabb_cpu.size(0)}"
+    assert aabb_cpu.size(1) == 3, f"aabb must have 3 columns, but got {aabb_cpu.size(1)}"
+    return aabb_cpu
+
+
+def _grid_host(voxel_size, grid_size, aabb):
+    """Grid of an explicit aabb, derived on the host exactly as the reference
+    wrapper does (flexible_dual_grid.py:88-178): (voxel_size list, grid_range
+    list, aabb CPU [2, 3])."""
+    assert voxel_size is not None or grid_size is not None, "Either voxel_size or grid_size must be provided"
+    vs_cpu = _host_vector(voxel_size, "voxel_size", torch.float32, "float") if voxel_size is not None else None
+    gs_cpu = _host_vector(grid_size, "grid_size", torch.int32, "int") if grid_size is not None else None
+    aabb_cpu = _aabb_host(aabb)
+    if vs_cpu is None:
+        vs_cpu = ((aabb_cpu[1] - aabb_cpu[0]) / gs_cpu).to(dtype=torch.float32).contiguous()
+    if gs_cpu is None:
+        gs_cpu = ((aabb_cpu[1] - aabb_cpu[0]) / vs_cpu).round().to(dtype=torch.int32).contiguous()
+    return [float(x) for x in vs_cpu.tolist()], [0, 0, 0] + [int(x) for x in gs_cpu.tolist()], aabb_cpu
+
+
+def _to_device(t: torch.Tensor, device: torch.device) -> torch.Tensor:
+    """Small host table -> device without waiting for the work already queued
+    there (pinned staging, non-blocking copy)."""
+    if device.type != "cuda":
+        return t.to(device)
+    return t.pin_memory().to(device=device, non_blocking=True)
+
+
 def _grid_setup(vertices: torch.Tensor, voxel_size, grid_size, aabb):
     """Derive (voxel_size list, grid_range list, aabb_min tensor on device) exactly as
     the reference wrapper does (flexible_dual_grid.py:88-178)."""
     assert voxel_size is not None or grid_size is not None, "Either voxel_size or grid_size must be provided"
     device = vertices.device
-    vs_cpu = gs_cpu = aabb_cpu = None
-    vs_dev = gs_dev = None
+    if aabb is not None:
+        vs_l, gr_l, aabb_cpu = _grid_host(voxel_size, grid_size, aabb)
+        return vs_l, gr_l, _to_device(aabb_cpu, device).contiguous()
+    # automatic aabb from the vertices, padded to whole voxels (flexible_dual_grid.py:146-159)
+    vs_cpu = gs_cpu = vs_dev = gs_dev = None
     if voxel_size is not None:
         vs_cpu = _host_vector(voxel_size, "voxel_size", torch.float32, "float")
         vs_dev = vs_cpu.to(device=device)
     if grid_size is not None:
         gs_cpu = _host_vector(grid_size, "grid_size", torch.int32, "int")
         gs_dev = gs_cpu.to(device=device)
-    aabb_dev = None
-    if aabb is not None:
-        if isinstance(aabb, (list, tuple)):
-            aabb = np.array(aabb)
-        if isinstance(aabb, np.ndarray):
-            aabb_cpu = torch.tensor(aabb, dtype=torch.float32)
-        else:
-            assert isinstance(aabb, torch.Tensor), (
-                f"aabb must be a list, tuple, np.ndarray, or torch.Tensor, but got {type(aabb)}"
-            )
-            assert not aabb.is_cuda, "aabb Tensor metadata must be on CPU; pass Python values to avoid implicit CUDA sync"
-            aabb_cpu = aabb.to(dtype=torch.float32).contiguous()
-        aabb_dev = aabb_cpu.to(device=device).contiguous()
-        assert aabb_dev.dim() == 2, f"aabb must be a 2D tensor, but got {aabb_dev.shape}"
-        assert aabb_dev.size(0) == 2, f"aabb must have 2 rows, but got {aabb_dev.size(0)}"
-        assert aabb_dev.size(1) == 3, f"aabb must have 3 columns, but got {aabb_dev.size(1)}"
-
-    if aabb_dev is None:
-        # automatic aabb, padded to whole voxels (flexible_dual_grid.py:146-159)
-        lo = vertices.min(dim=0).values
-        hi = vertices.max(dim=0).values
-        if vs_dev is not None:
-            pad = torch.ceil((hi - lo) / vs_dev) * vs_dev - (hi - lo)
-            lo = lo - pad * 0.5
-            hi = hi + pad * 0.5
-        if gs_dev is not None:
-            pad = (hi - lo) / (gs_dev - 1)
-            lo = lo - pad * 0.5
-            hi = hi + pad * 0.5
-        aabb_dev = torch.stack([lo, hi], dim=0).float().contiguous()
-
+    lo = vertices.min(dim=0).values
+    hi = vertices.max(dim=0).values
+    if vs_dev is not None:
+        pad = torch.ceil((hi - lo) / vs_dev) * vs_dev - (hi - lo)
+        lo = lo - pad * 0.5
+        hi = hi + pad * 0.5
+    if gs_dev is not None:
+        pad = (hi - lo) / (gs_dev - 1)
+        lo = lo - pad * 0.5
+        hi = hi + pad * 0.5
+    aabb_dev = torch.stack([lo, hi], dim=0).float().contiguous()
     if vs_cpu is None:
-        if aabb_cpu is None:
-            assert not aabb_dev.is_cuda, "CUDA inputs require CPU aabb or explicit voxel_size to avoid implicit CUDA sync"
-            aabb_cpu = aabb_dev.detach()
-        vs_cpu = ((aabb_cpu[1] - aabb_cpu[0]) / gs_cpu).to(dtype=torch.float32).contiguous()
+        assert not aabb_dev.is_cuda, "CUDA inputs require CPU aabb or explicit voxel_size to avoid implicit CUDA sync"
+        vs_cpu = ((aabb_dev[1] - aabb_dev[0]) / gs_cpu).to(dtype=torch.float32).contiguous()
     if gs_cpu is None:
-        if aabb_cpu is None:
-            assert not aabb_dev.is_cuda, "CUDA inputs require CPU aabb or explicit grid_size to avoid implicit CUDA sync"
-            aabb_cpu = aabb_dev.detach()
-        gs_cpu = ((aabb_cpu[1] - aabb_cpu[0]) / vs_cpu).round().to(dtype=torch.int32).contiguous()
+        assert not aabb_dev.is_cuda, "CUDA inputs require CPU aabb or explicit grid_size to avoid implicit CUDA sync"
+        gs_cpu = ((aabb_dev[1] - aabb_dev[0]) / vs_cpu).round().to(dtype=torch.int32).contiguous()
     voxel_size_arg = [float(x) for x in vs_cpu.tolist()]
     grid_range_arg = [0, 0, 0] + [int(x) for x in gs_cpu.tolist()]
     return voxel_size_arg, grid_range_arg, aabb_dev
@@ -197,27 +217,47 @@ def mesh_to_flexible_dual_grid_batch(
             return plural[k]
         return single
 
-    # grid set-up per mesh, shared by meshes with the same explicit parameters
+    # grid set-up per mesh, shared by meshes with the same explicit parameters;
+    # explicit aabbs are handled on the host (no device round trip)
     cache = {}
+    keys = {}
+
+    def key_of(value, plural, k):
+        # parameters given once for the whole batch are keyed once
+        if plural is not None:
+            return _hashable(value)
+        if id(value) not in keys:
+            keys[id(value)] = _hashable(value)
+        return keys[id(value)]
+
     vs_list, gr_list, shift_rows, v_list, f_list = [], [], [], [], []
     device = meshes[0][0].device if isinstance(meshes[0][0], torch.Tensor) else None
+    host_shift = True
     for k, (vertices, faces) in enumerate(meshes):
         _check_mesh(vertices, faces)
         assert vertices.device == device, "all meshes must be on the same device"
-        vertices = vertices.to(dtype=torch.float32)
+        if vertices.dtype != torch.float32:
+            vertices = vertices.to(dtype=torch.float32)
         vs_k, gs_k, ab_k = (pick(voxel_size, voxel_sizes, k, "voxel_size"), pick(grid_size, grid_sizes, k, "grid_size"),
                             pick(aabb, aabbs, k, "aabb"))
-        try:
-            key = _hashable((vs_k, gs_k, ab_k)) if ab_k is not None else None
-        except TypeError:
-            key = None
+        key = None
+        if ab_k is not None:
+            try:
+                key = (key_of(vs_k, voxel_sizes, k), key_of(gs_k, grid_sizes, k), key_of(ab_k, aabbs, k))
+            except TypeError:
+                key = None
         if key is not None and key in cache:
             vs_l, gr_l, a0 = cache[key]
         else:
-            vs_l, gr_l, aabb_dev = _grid_setup(vertices, vs_k, gs_k, ab_k)
-            a0 = aabb_dev[0].reshape(1, 3)
+            if ab_k is not None:
+                vs_l, gr_l, aabb_cpu = _grid_host(vs_k, gs_k, ab_k)
+                a0 = aabb_cpu[0].reshape(1, 3)
+            else:
+                vs_l, gr_l, aabb_dev = _grid_setup(vertices, vs_k, gs_k, ab_k)
+                a0 = aabb_dev[0].reshape(1, 3)
             if key is not None:
                 cache[key] = (vs_l, gr_l, a0)
+        host_shift = host_shift and a0.device.type == "cpu"
         vs_list.append(vs_l)
         gr_list.append(gr_l)
         shift_rows.append(a0)
@@ -226,14 +266,19 @@ def mesh_to_flexible_dual_grid_batch(
     nv = [int(v.size(0)) for v in v_list]
     nf = [int(f.size(0)) for f in f_list]
     # packed grid-local vertices and batch-global face indices: a handful of
-    # whole-batch tensor operations rather than a few per mesh
-    shift = torch.cat(shift_rows, 0).to(device=device)
-    v_all = torch.cat(v_list, 0) - torch.repeat_interleave(shift, torch.tensor(nv, device=device), dim=0,
-                                                          output_size=sum(nv))
-    base = torch.tensor([0] + nv[:-1], dtype=torch.int64).cumsum(0).to(device)
-    f_all = torch.cat([f.to(torch.int64) for f in f_list], 0)
-    f_all = (f_all + torch.repeat_interleave(base, torch.tensor(nf, device=device), output_size=sum(nf)).unsqueeze(1)
-             ).to(torch.int32).contiguous()
+    # whole-batch tensor operations; the per-mesh tables go up in one
+    # non-blocking copy
+    nv_t = torch.tensor(nv, dtype=torch.int64)
+    nf_t = torch.tensor(nf, dtype=torch.int64)
+    counts = _to_device(torch.cat([nv_t, nf_t, torch.cumsum(nv_t, 0) - nv_t]), device)
+    nv_d, nf_d, base = counts[:M], counts[M:2 * M], counts[2 * M:]
+    if host_shift:
+        shift = _to_device(torch.cat(shift_rows, 0), device)
+    else:
+        shift = torch.cat([r.to(device=device) for r in shift_rows], 0)
+    v_all = torch.cat(v_list, 0) - torch.repeat_interleave(shift, nv_d, dim=0, output_size=sum(nv))
+    f_all = torch.cat([f if f.dtype == torch.int64 else f.to(torch.int64) for f in f_list], 0)
+    f_all = (f_all + torch.repeat_interleave(base, nf_d, output_size=sum(nf)).unsqueeze(1)).to(torch.int32).contiguous()
     if device.type == "cuda":
         return _C.mesh_to_flexible_dual_grid_batch_packed_cuda(v_all.contiguous(), f_all, nv, nf, vs_list, gr_list,
                                                                face_weight, boundary_weight, regularization_weight,
