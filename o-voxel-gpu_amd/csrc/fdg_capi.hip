@@ -101,7 +101,7 @@ struct ovx_fdg_ctx
     uint8_t *svo_bytes = nullptr; // the octree written by the last encode finish
     int svo_state = 0;            // 1 begun, 2 written
     DevBuf vcnt, voff, nbr, erank, bcnt, boff, bedge_v, bedges;
-    DevBuf events, recs, chunks, heads, choff, bchunks, active, large, box, qef, bhits, fhits, shard_ctr, rowq, faceq;
+    DevBuf events, recs, chunks, heads, choff, bchunks, active, large, huge, box, qef, bhits, fhits, shard_ctr, rowq, faceq;
     uint64_t per[SB_N] = {};       // per-shard capacities of the sharded buffers (grow-only)
     uint64_t force_per[SB_N] = {}; // one-shot overrides (tests, tuning)
     double face_per_face = 8.0;           // face records per face seen last time (capacity estimate)
@@ -151,7 +151,7 @@ struct ovx_fdg_ctx
     void for_each_buf(Fn &&fn)
     {
         DevBuf *all[] = {&planes, &bits,   &woff,  &tiles,   &vcnt,   &voff,  &nbr, &erank, &bcnt, &boff,
-                         &bedge_v, &bedges, &events, &recs,   &chunks, &heads, &choff, &bchunks, &active, &large,
+                         &bedge_v, &bedges, &events, &recs,   &chunks, &heads, &choff, &bchunks, &active, &large, &huge,
                          &box,     &qef,    &bhits,  &fhits,  &shard_ctr, &rowq, &faceq, &segbuf,
                          &dm_table, &dm_cnt, &dm_off, &dm_mask, &dm_tiles, &dm_direct, &dm_touched,
                          &svo_insf, &svo_k, &svo_pos, &svo_tiles, &svo_rows, &svo_erow, &nb_table, &nb_own,
@@ -796,6 +796,7 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
     const uint64_t n_chunks = c->h_ctr[CTR_SUM + SB_CHUNKS];
     if ((st = ensure(c->choff, (n_bricks + 1) * 4)) || (st = ensure(c->bchunks, (n_chunks + 1) * 8)) ||
         (st = ensure(c->active, n_bricks * 4)) || (st = ensure(c->large, n_bricks * 4)) ||
+        (st = ensure(c->huge, n_bricks * 4)) ||
         // box-search queue: a few percent of the voxels typically; the solve
         // kernel solves in place whatever does not fit
         (st = ensure_shard(c, c->box, SB_BOX, per_for((uint64_t)c->N / 8 + 4096), sizeof(BoxItem))) ||
@@ -817,6 +818,8 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
     a.n_active = ctr + CTR_ACTIVE;
     a.large = (uint32_t *)c->large.p;
     a.n_large = ctr + CTR_LARGE;
+    a.huge = (uint32_t *)c->huge.p;
+    a.n_huge = ctr + CTR_HUGE;
     a.b = c->bdev; // grid and brick grid are set per brick from its mesh
     a.planes = (const float4 *)c->planes.p;
     a.bedges = (const BEdge *)c->bedges.p;
@@ -830,11 +833,13 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
     a.cond_max = c->cond_max;
     a.n64 = (unsigned long long *)c->shard_ctr.p + (uint64_t)SB_FP64 * NSHARD * SHARD_STRIDE;
     a.ctr = ctr;
-    HIP_TRY(hipMemsetAsync(ctr + CTR_LARGE, 0, 3 * sizeof(unsigned long long), s)); // LARGE, LARGE_RECS, BOX
+    HIP_TRY(hipMemsetAsync(ctr + CTR_LARGE, 0, 2 * sizeof(unsigned long long), s)); // LARGE, LARGE_RECS
+    HIP_TRY(hipMemsetAsync(ctr + CTR_HUGE, 0, sizeof(unsigned long long), s));
     tic(c, KS_FOLD, s);
     HIP_TRY(launch_fold_bricks(a, n_bricks, s));
     toc(c, KS_FOLD, s);
     tic(c, KS_FOLD_LARGE, s);
+    HIP_TRY(launch_fold_mid(a, n_bricks, s));
     HIP_TRY(launch_fold_large(a, n_bricks, s));
     toc(c, KS_FOLD_LARGE, s);
     tic(c, KS_SOLVE, s);

@@ -35,6 +35,7 @@ enum
     CTR_LARGE_RECS = 11, // their records
     CTR_DM_QUADS = 12,   // dual grid -> mesh: quads (scan total)
     CTR_SVO_BYTES = 13,  // octree bytes of an encode (scan total)
+    CTR_HUGE = 14,       // bricks left to the voxel-range fold (beyond the mid-size tile)
     // sum and largest shard of each sharded buffer (filled by k_shard_reduce)
     CTR_SUM = 16,        // CTR_SUM + SB_*  : total items requested
     CTR_MAX = 32,        // CTR_MAX + SB_*  : largest per-shard request
@@ -278,8 +279,10 @@ struct FoldArgs
     const Rec *recs;
     const uint32_t *active;             // bricks with records
     const unsigned long long *n_active; // their count (device)
-    uint32_t *large;                    // bricks too large for the main fold kernel
+    uint32_t *large;                    // bricks too large for the main fold tile
     unsigned long long *n_large;        // their count (device, zeroed by the host)
+    uint32_t *huge;                     // bricks too large for the mid-size tile
+    unsigned long long *n_huge;         // their count (device, zeroed by the host)
     BatchDev b;                         // meshes
     int nb[3];  // brick grid of the brick being folded (set per brick from its mesh)
     GridDev g;  // grid of the brick's mesh (set per brick)
@@ -312,6 +315,9 @@ hipError_t launch_place_chunks(const ShardBuf<Chunk> &chunks, const uint32_t *ch
 hipError_t launch_active_bricks(const uint32_t *nch, uint64_t n_bricks, uint32_t *list, unsigned long long *count,
                                 hipStream_t s);
 hipError_t launch_fold_bricks(const FoldArgs &a, uint64_t max_bricks, hipStream_t s);
+// bricks of the `large` list up to the mid-size tile; the rest go to `huge`
+hipError_t launch_fold_mid(const FoldArgs &a, uint64_t max_bricks, hipStream_t s);
+// bricks of the `huge` list: whole, by voxel ranges, or a giant voxel in key-ordered batches
 hipError_t launch_fold_large(const FoldArgs &a, uint64_t max_bricks, hipStream_t s);
 hipError_t launch_solve_voxels(const FoldArgs &a, const int32_t *coords, uint64_t n, hipStream_t s);
 hipError_t launch_solve_box(const FoldArgs &a, hipStream_t s);

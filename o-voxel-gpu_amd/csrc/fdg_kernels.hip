@@ -2037,6 +2037,7 @@ struct FoldPick<1>
 };
 
 constexpr int FOLD_SMALL = 2048; // records of a brick held in LDS by the main fold kernel (5 blocks per CU)
+constexpr int FOLD_MID = 3584;   // ... by its second launch over the bricks beyond that (3 blocks per CU)
 constexpr int FOLD_CAP = 4096;   // records held in LDS at a time by the kernel for large bricks (2 blocks per CU)
 
 // LDS view of a batch of up to BT chunks of one brick
@@ -2386,20 +2387,25 @@ __device__ __forceinline__ void fold_grouped(const FoldArgs &a, FoldLdsG<CAP> &L
     __syncthreads();
 }
 
-// One workgroup per brick that received records, for bricks of at most
-// FOLD_SMALL records (the others are listed for k_fold_large): the records
-// are read once into registers, grouped and ordered in LDS, replayed and
-// solved there.
-template <int P>
+// One workgroup per brick of a list, for bricks of at most CAP records: the
+// records are read once into registers, grouped and ordered in LDS,
+// replayed and solved there.  MID = false: the bricks with records, larger
+// ones listed in `large`; MID = true: the `large` list, larger ones listed
+// in `huge` (for k_fold_large).
+template <int P, int CAP, bool MID>
 __global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs ain)
 {
-    __shared__ FoldLdsG<FOLD_SMALL> L;
-    constexpr int PER = FOLD_SMALL / BT;
+    __shared__ FoldLdsG<CAP> L;
+    constexpr int PER = CAP / BT;
+    static_assert(CAP % BT == 0, "record slots per thread");
     const int tid = threadIdx.x;
-    const uint64_t n_active = *ain.n_active;
-    for (uint64_t bi = blockIdx.x; bi < n_active; bi += gridDim.x)
+    const uint32_t *list = MID ? ain.large : ain.active;
+    const uint64_t n_list = MID ? *ain.n_large : *ain.n_active;
+    uint32_t *next = MID ? ain.huge : ain.large;
+    unsigned long long *n_next = MID ? ain.n_huge : ain.n_large;
+    for (uint64_t bi = blockIdx.x; bi < n_list; bi += gridDim.x)
     {
-        const uint32_t brick = ain.active[bi];
+        const uint32_t brick = list[bi];
         FoldArgs a = ain;
         const uint32_t lb = enter_brick(a, brick);
         const uint32_t c0 = a.choff[brick], c1 = a.choff[brick + 1];
@@ -2434,7 +2440,7 @@ __global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs ain)
             }
             uint32_t tot;
             const uint32_t ex = block_excl_scan(cnt, L.cb.sh, tot);
-            if (n + tot > (uint32_t)FOLD_SMALL)
+            if (n + tot > (uint32_t)CAP)
             {
                 over = true;
                 break;
@@ -2468,11 +2474,13 @@ __global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs ain)
         if (over)
         {
             if (tid == 0)
-                a.large[atomicAdd(a.n_large, 1ull)] = brick;
+                next[atomicAdd(n_next, 1ull)] = brick;
             __syncthreads();
             continue;
         }
-        fold_grouped<FOLD_SMALL, P>(a, L, n, rk, rz, rl, x0, y0, z0);
+        if (MID && tid == 0)
+            atomicAdd(a.ctr + CTR_LARGE_RECS, (unsigned long long)n);
+        fold_grouped<CAP, P>(a, L, n, rk, rz, rl, x0, y0, z0);
     }
 }
 
@@ -2485,11 +2493,11 @@ __global__ __launch_bounds__(BT) void k_fold_large(FoldArgs ain)
 {
     __shared__ FoldLds<FOLD_CAP> L;
     const int tid = threadIdx.x;
-    const uint64_t n_large = *ain.n_large;
+    const uint64_t n_huge = *ain.n_huge;
 
-    for (uint64_t bi = blockIdx.x; bi < n_large; bi += gridDim.x)
+    for (uint64_t bi = blockIdx.x; bi < n_huge; bi += gridDim.x)
     {
-        const uint32_t brick = ain.large[bi];
+        const uint32_t brick = ain.huge[bi];
         FoldArgs a = ain;
         const uint32_t lb = enter_brick(a, brick);
         int x0, y0, z0;
@@ -3008,9 +3016,21 @@ hipError_t launch_fold_bricks(const FoldArgs &a, uint64_t max_bricks, hipStream_
     if (nb == 0)
         nb = 1;
     if (a.prec)
-        hipLaunchKernelGGL(k_fold_bricks<1>, dim3((unsigned)nb), dim3(BT), 0, s, a);
+        hipLaunchKernelGGL((k_fold_bricks<1, FOLD_SMALL, false>), dim3((unsigned)nb), dim3(BT), 0, s, a);
     else
-        hipLaunchKernelGGL(k_fold_bricks<0>, dim3((unsigned)nb), dim3(BT), 0, s, a);
+        hipLaunchKernelGGL((k_fold_bricks<0, FOLD_SMALL, false>), dim3((unsigned)nb), dim3(BT), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_fold_mid(const FoldArgs &a, uint64_t max_bricks, hipStream_t s)
+{
+    uint64_t nb = max_bricks < 4096ull ? max_bricks : 4096ull;
+    if (nb == 0)
+        nb = 1;
+    if (a.prec)
+        hipLaunchKernelGGL((k_fold_bricks<1, FOLD_MID, true>), dim3((unsigned)nb), dim3(BT), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_fold_bricks<0, FOLD_MID, true>), dim3((unsigned)nb), dim3(BT), 0, s, a);
     return hipGetLastError();
 }
 
