@@ -140,6 +140,27 @@ ovx_status ovx_occ_finish(ovx_fdg_ctx *ctx, int32_t *d_coords, void *stream);
 ovx_status ovx_fdg_set_precision(ovx_fdg_ctx *ctx, int mode, float cond_max);
 
 /*
+ * Launch-class thresholds of the scanline and face-test kernels for the next
+ * conversions on this context.  No counterpart in the reference (its launch
+ * shapes are fixed); the autotuner o_voxel.convert.autotune_tile_classes sets
+ * them from timed trial conversions.  A mesh's scanline class comes from its
+ * faces per grid cross-section fpc (F / the largest of Gx*Gy, Gy*Gz, Gx*Gz):
+ *   fpc <  mark_fpc3[0]: 64-item tiles with a 2048-word LDS merge table
+ *   fpc <= mark_fpc3[1]: 128-item tiles with a 1024-word merge table
+ *   fpc <  mark_fpc3[2]: 128-item tiles, no table
+ *   otherwise:           256-item tiles, no table
+ * and its face-test class from its voxels per face vpf:
+ *   vpf >= face_vpf2[1]: 32 faces per block; vpf >= face_vpf2[0]: 64; else 128.
+ * Defaults {0.5, 2, 8} and {4, 16} (the autotuner's choice on a mixed batch
+ * of 256 meshes, 500..500k faces, grids 64..512).  Thresholds are finite, >= 0 and
+ * non-decreasing; a null array leaves its set unchanged.  Results do not
+ * depend on them, only the launch shapes do.  Must not be called between
+ * begin and finish.
+ */
+ovx_status ovx_fdg_set_tile_classes(ovx_fdg_ctx *ctx, const double *mark_fpc3, const double *face_vpf2);
+ovx_status ovx_fdg_get_tile_classes(ovx_fdg_ctx *ctx, double *mark_fpc3, double *face_vpf2);
+
+/*
  * Per-stage statistics of the last conversion on this context; the first
  * min(n, OVX_FDG_N_STATS) are written:
  *   out[0] N voxels          out[1] scanline events     out[2] QEF records

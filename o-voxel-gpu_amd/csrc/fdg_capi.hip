@@ -77,12 +77,18 @@ struct DeviceGuard
 // merge their dense crossings in LDS; with many more faces than voxel rows
 // per plane the crossings are sparse, the merge table only costs occupancy,
 // and larger tiles pay.  Each class present is one launch over its meshes.
+// The class thresholds are per context (ovx_fdg_set_tile_classes).
 constexpr int N_MARK_CLASSES = 4;
 constexpr int MARK_CLASS_TILE[N_MARK_CLASSES] = {0, 1, 1, 2};
 constexpr bool MARK_CLASS_MERGE[N_MARK_CLASSES] = {true, true, false, false};
-int mark_class(double fpc)
+static int mark_class(const double t[3], double fpc)
 {
-    return fpc < 0.5 ? 0 : (fpc <= 2.0 ? 1 : (fpc < 8.0 ? 2 : 3));
+    return fpc < t[0] ? 0 : (fpc <= t[1] ? 1 : (fpc < t[2] ? 2 : 3));
+}
+// face-test class from voxels per face: index into FACE_TILES (32, 64, 128 faces per block)
+static int face_class(const double t[2], double vpf)
+{
+    return vpf >= t[1] ? 0 : (vpf >= t[0] ? 1 : 2);
 }
 
 struct ovx_fdg_ctx
@@ -132,6 +138,8 @@ struct ovx_fdg_ctx
     size_t h_stage_bytes = 0;
     uint64_t *h_vox_off = nullptr; // inside h_stage: per-mesh first voxel ids, read back by begin
     BatchDev bdev{};
+    double mark_fpc[3] = {0.5, 2.0, 8.0}; // scanline class thresholds (faces per grid cross-section)
+    double face_vpf[2] = {4.0, 16.0};     // face-test class thresholds (voxels per face)
     float fw = 0, bw = 0, rw = 0;
     int prec = 0;           // 0 reference float path, 1 compensated voxel-frame accumulation
     float cond_max = 1.0e6f; // precision variant: FP64 solve above this condition estimate
@@ -387,7 +395,7 @@ ovx_status setup_batch(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *f
         c->brick_off[i + 1] = c->brick_off[i] + (uint64_t)sg.nb[0] * (uint64_t)sg.nb[1] * (uint64_t)sg.nb[2];
         // scanline launch class of the mesh from its faces per grid cross-section
         const double cross = std::max({(double)g.G[0] * g.G[1], (double)g.G[1] * g.G[2], (double)g.G[0] * g.G[2]});
-        const int mc = mark_class((double)nf / cross);
+        const int mc = mark_class(c->mark_fpc, (double)nf / cross);
         for (int t = 0; t < N_MARK_CLASSES; ++t)
         {
             const uint64_t tile = (uint64_t)MARK_TILES[MARK_CLASS_TILE[t]];
@@ -506,7 +514,7 @@ ovx_status launch_face_tiles(ovx_fdg_ctx *c, const MeshDev &m, const ShardBuf<FH
         const int64_t nf = c->face_off[i + 1] - c->face_off[i];
         const double nv = (double)(c->h_vox_off[i + 1] - c->h_vox_off[i]);
         const double vpf = nf > 0 ? nv / (double)nf : 0.0;
-        const int ft = vpf >= 4.0 ? 0 : (vpf >= 1.0 ? 1 : 2);
+        const int ft = face_class(c->face_vpf, vpf);
         for (int t = 0; t < N_FACE_TILES; ++t)
             c->ftile_off[t][i + 1] =
                 c->ftile_off[t][i] + (t == ft ? (uint64_t)(nf + FACE_TILES[t] - 1) / FACE_TILES[t] : 0ull);
@@ -1083,6 +1091,45 @@ ovx_status ovx_fdg_set_precision(ovx_fdg_ctx *c, int mode, float cond_max)
         return fail(OVX_ERR_STATE, "precision changed between begin and finish");
     c->prec = mode;
     c->cond_max = cond_max;
+    return OVX_OK;
+}
+
+ovx_status ovx_fdg_set_tile_classes(ovx_fdg_ctx *c, const double *mark_fpc3, const double *face_vpf2)
+{
+    if (!c)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "null context");
+    auto ok = [](const double *t, int n) {
+        for (int i = 0; i < n; ++i)
+            if (!(t[i] >= 0.0 && t[i] <= 1e30) || (i && t[i] < t[i - 1]))
+                return false;
+        return true;
+    };
+    if ((mark_fpc3 && !ok(mark_fpc3, 3)) || (face_vpf2 && !ok(face_vpf2, 2)))
+        return fail(OVX_ERR_INVALID_ARGUMENT, "tile class thresholds must be finite, >= 0 and non-decreasing");
+    DeviceGuard dg(c->device);
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (c->pending)
+        return fail(OVX_ERR_STATE, "tile classes changed between begin and finish");
+    if (mark_fpc3)
+        for (int i = 0; i < 3; ++i)
+            c->mark_fpc[i] = mark_fpc3[i];
+    if (face_vpf2)
+        for (int i = 0; i < 2; ++i)
+            c->face_vpf[i] = face_vpf2[i];
+    return OVX_OK;
+}
+
+ovx_status ovx_fdg_get_tile_classes(ovx_fdg_ctx *c, double *mark_fpc3, double *face_vpf2)
+{
+    if (!c)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "null context");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (mark_fpc3)
+        for (int i = 0; i < 3; ++i)
+            mark_fpc3[i] = c->mark_fpc[i];
+    if (face_vpf2)
+        for (int i = 0; i < 2; ++i)
+            face_vpf2[i] = c->face_vpf[i];
     return OVX_OK;
 }
 

@@ -111,6 +111,9 @@ def _load():
         L.ovx_fdg_set_capacity.argtypes = [vp, i64, i64]
         L.ovx_fdg_set_precision.argtypes = [vp, ctypes.c_int, f32]
         L.ovx_fdg_set_precision.restype = ctypes.c_int
+        f64p = ctypes.POINTER(ctypes.c_double)
+        L.ovx_fdg_set_tile_classes.argtypes = [vp, f64p, f64p]
+        L.ovx_fdg_get_tile_classes.argtypes = [vp, f64p, f64p]
         L.ovx_fdg_set_capacity.restype = ctypes.c_int
         L.ovx_fdg_kernel_count.restype = ctypes.c_int
         L.ovx_fdg_kernel_name.argtypes = [ctypes.c_int]
@@ -924,6 +927,32 @@ def kernel_times(device=None) -> dict:
     with torch.cuda.device(dev), ctx.lock:
         _check(L.ovx_fdg_kernel_times(ctx.h, out, n), "ovx_fdg_kernel_times")
     return {L.ovx_fdg_kernel_name(k).decode(): float(out[k]) for k in range(n) if out[k] >= 0.0}
+
+
+def set_tile_classes(device=None, mark_fpc=None, face_vpf=None):
+    """Launch-class thresholds for the next conversions on ``device`` (no
+    reference counterpart): three scanline thresholds on faces per grid
+    cross-section and two face-test thresholds on voxels per face, each
+    non-decreasing (ovx_fdg_set_tile_classes); None keeps a set."""
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    ctx = _ctx(dev)
+    if (mark_fpc is not None and len(mark_fpc) != 3) or (face_vpf is not None and len(face_vpf) != 2):
+        raise RuntimeError("set_tile_classes: mark_fpc needs 3 thresholds and face_vpf 2")
+    m = None if mark_fpc is None else (ctypes.c_double * 3)(*[float(x) for x in mark_fpc])
+    f = None if face_vpf is None else (ctypes.c_double * 2)(*[float(x) for x in face_vpf])
+    with torch.cuda.device(dev), ctx.lock:
+        _check(_load().ovx_fdg_set_tile_classes(ctx.h, m, f), "set_tile_classes")
+
+
+def get_tile_classes(device=None) -> Tuple[List[float], List[float]]:
+    """(scanline thresholds [3], face-test thresholds [2]) of ``device``'s context."""
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    ctx = _ctx(dev)
+    m = (ctypes.c_double * 3)()
+    f = (ctypes.c_double * 2)()
+    with torch.cuda.device(dev), ctx.lock:
+        _check(_load().ovx_fdg_get_tile_classes(ctx.h, m, f), "get_tile_classes")
+    return [float(x) for x in m], [float(x) for x in f]
 
 
 def set_capacity(device=None, events: int = 0, hits: int = 0):

@@ -19,6 +19,7 @@ ArrayLike = Union[List[Any], Tuple[Any, ...], np.ndarray, torch.Tensor]
 __all__ = [
     "mesh_to_flexible_dual_grid",
     "mesh_to_flexible_dual_grid_batch",
+    "autotune_tile_classes",
     "intersect_occ",
     "flexible_dual_grid_to_mesh",
 ]
@@ -288,6 +289,54 @@ def mesh_to_flexible_dual_grid_batch(
                                                           gr_list, face_weight, boundary_weight,
                                                           regularization_weight, precision=precision)
     return [tuple(t.cpu() for t in o) for o in out]
+
+
+def autotune_tile_classes(meshes, scales=(0.25, 0.5, 1.0, 2.0, 4.0), repeats: int = 3, **batch_kwargs) -> dict:
+    """Choose the launch-class thresholds of the scanline and face-test kernels
+    for meshes like ``meshes`` (an addition to the reference API, whose launch
+    shapes are fixed).
+
+    Runs ``mesh_to_flexible_dual_grid_batch(meshes, **batch_kwargs)`` with the
+    current thresholds scaled by each of ``scales`` (one warm-up, then
+    ``repeats`` timed conversions per candidate), first for the scanline
+    thresholds (timing the scanline kernels), then for the face-test thresholds
+    (timing the face kernels), keeps the fastest of each on the device's
+    context and returns them with the measured times.  The outputs of a
+    conversion do not depend on the thresholds, only its launch shapes do.
+    """
+    meshes = list(meshes)
+    assert len(meshes) > 0, "autotune_tile_classes needs at least one mesh"
+    device = meshes[0][0].device
+    assert device.type == "cuda", "autotune_tile_classes times device conversions: pass device tensors"
+    mark0, face0 = _C.get_tile_classes(device)
+    trials = []
+
+    def timed(slot):
+        best = float("inf")
+        mesh_to_flexible_dual_grid_batch(meshes, **batch_kwargs)
+        for _ in range(max(1, int(repeats))):
+            mesh_to_flexible_dual_grid_batch(meshes, **batch_kwargs)
+            best = min(best, _C.kernel_times(device).get(slot, float("inf")))
+        return best
+
+    _C.set_timing(device, True)
+    try:
+        picks = {}
+        for kind, base, slot in (("mark_fpc", mark0, "mark"), ("face_vpf", face0, "face")):
+            best = None
+            for sc in scales:
+                cand = [float(x) * float(sc) for x in base]
+                _C.set_tile_classes(device, **{kind: cand})
+                ms = timed(slot)
+                trials.append({kind: cand, slot + "_ms": ms})
+                if best is None or ms < best[0]:
+                    best = (ms, cand)
+            _C.set_tile_classes(device, **{kind: best[1]})
+            picks[kind] = best
+    finally:
+        _C.set_timing(device, False)
+    return {"mark_fpc": picks["mark_fpc"][1], "mark_ms": picks["mark_fpc"][0],
+            "face_vpf": picks["face_vpf"][1], "face_ms": picks["face_vpf"][0], "trials": trials}
 
 
 def _hashable(x):
