@@ -106,7 +106,8 @@ struct ovx_fdg_ctx
     uint64_t svo_total = 0;
     uint8_t *svo_bytes = nullptr; // the octree written by the last encode finish
     int svo_state = 0;            // 1 begun, 2 written
-    DevBuf vcnt, voff, nbr, erank, bcnt, boff, bedge_v, bedges;
+    DevBuf vcnt, voff, nbr, erank, bcnt, boff, bedge_v, bedges, eslots;
+    bool edge_csr = false; // the last conversion's boundary edges came from the CSR lists (a vertex overflowed its slots)
     DevBuf events, recs, chunks, heads, choff, bchunks, active, large, huge, box, qef, bhits, fhits, shard_ctr, rowq, faceq;
     uint64_t per[SB_N] = {};       // per-shard capacities of the sharded buffers (grow-only)
     uint64_t force_per[SB_N] = {}; // one-shot overrides (tests, tuning)
@@ -158,7 +159,7 @@ struct ovx_fdg_ctx
     template <class Fn>
     void for_each_buf(Fn &&fn)
     {
-        DevBuf *all[] = {&planes, &bits,   &woff,  &tiles,   &vcnt,   &voff,  &nbr, &erank, &bcnt, &boff,
+        DevBuf *all[] = {&planes, &bits,   &woff,  &tiles,   &vcnt,   &voff,  &nbr, &erank, &bcnt, &boff, &eslots,
                          &bedge_v, &bedges, &events, &recs,   &chunks, &heads, &choff, &bchunks, &active, &large, &huge,
                          &box,     &qef,    &bhits,  &fhits,  &shard_ctr, &rowq, &faceq, &segbuf,
                          &dm_table, &dm_cnt, &dm_off, &dm_mask, &dm_tiles, &dm_direct, &dm_touched,
@@ -620,25 +621,25 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *
     if (planes && c->bw > 0.0f)
     {
         // boundary edges (edge multiset of FDG.cpp:728-739, kept in std::map order):
-        // half-edges bucketed by their smaller vertex, lists sorted, singletons counted
+        // half-edges bucketed by their smaller vertex (fixed slots per vertex),
+        // each vertex's list sorted in registers, singletons counted; a vertex
+        // with more half-edges than slots re-runs the CSR lists after the readback
         if ((uint64_t)m.F * 3ull >= (1ull << 32))
             return fail(OVX_ERR_CAPACITY, "more than 2^32 half-edges");
-        if ((st = ensure(c->vcnt, (size_t)(V + 1) * 4)) || (st = ensure(c->voff, (size_t)(V + 1) * 4)) ||
-            (st = ensure(c->nbr, (size_t)m.F * 3 * 4)) || (st = ensure(c->erank, (size_t)m.F * 3 * 4)) ||
+        if ((st = ensure(c->vcnt, (size_t)(V + 1) * 4)) || (st = ensure(c->eslots, (size_t)V * EDGE_SLOTS * 4 + 16)) ||
             (st = ensure(c->bcnt, (size_t)(V + 1) * 4)) ||
             (st = ensure(c->boff, (size_t)(V + 1) * 4)) ||
             (st = ensure(c->tiles, (scan_tile_count(n_words) + 2 * scan_tile_count((uint64_t)V + 1) + 64) * 8)))
             return st;
+        c->edge_csr = false;
         auto *vcnt = (uint32_t *)c->vcnt.p;
-        auto *voff = (uint32_t *)c->voff.p;
         tic(c, KS_EDGES, s);
         HIP_TRY(hipMemsetAsync(vcnt, 0, (size_t)(V + 1) * 4, s));
-        auto *erank = (uint32_t *)c->erank.p;
-        HIP_TRY(launch_edge_count(m, vcnt, erank, voff, (uint32_t *)c->nbr.p, 0, s));
-        HIP_TRY(launch_scan_u32(vcnt, (uint64_t)V + 1, voff, (uint64_t *)c->tiles.p, ctr + CTR_EDGE_TOTAL, s));
-        HIP_TRY(launch_edge_count(m, vcnt, erank, voff, (uint32_t *)c->nbr.p, 1, s));
+        HIP_TRY(hipMemsetAsync(ctr + CTR_EDGE_OVER, 0, sizeof(unsigned long long), s));
+        HIP_TRY(launch_edge_slots(m, vcnt, (uint32_t *)c->eslots.p, s));
         HIP_TRY(hipMemsetAsync(c->bcnt.p, 0, (size_t)(V + 1) * 4, s));
-        HIP_TRY(launch_edge_singletons(V, voff, (uint32_t *)c->nbr.p, (uint32_t *)c->bcnt.p, nullptr, nullptr, 0, s));
+        HIP_TRY(launch_edge_singletons_slots(V, vcnt, (const uint32_t *)c->eslots.p, (uint32_t *)c->bcnt.p, nullptr,
+                                             nullptr, 0, ctr + CTR_EDGE_OVER, s));
         HIP_TRY(launch_scan_u32((uint32_t *)c->bcnt.p, (uint64_t)V + 1, (uint32_t *)c->boff.p,
                                 (uint64_t *)c->tiles.p + scan_tile_count(n_words) + 16, ctr + CTR_BEDGE_TOTAL, s));
         toc(c, KS_EDGES, s);
@@ -668,15 +669,46 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *
                 return st;
             toc(c, KS_MARK, s);
         }
+        if (c->bw > 0.0f && c->h_ctr[CTR_EDGE_OVER])
+        {
+            // a vertex has more half-edges than slots: per-vertex lists of any
+            // length (count, scan, place, sort in place), read back again
+            if ((st = ensure(c->voff, (size_t)(V + 1) * 4)) || (st = ensure(c->nbr, (size_t)m.F * 3 * 4)) ||
+                (st = ensure(c->erank, (size_t)m.F * 3 * 4)))
+                return st;
+            c->edge_csr = true;
+            auto *vcnt = (uint32_t *)c->vcnt.p;
+            auto *voff = (uint32_t *)c->voff.p;
+            auto *erank = (uint32_t *)c->erank.p;
+            tic(c, KS_EDGES, s);
+            HIP_TRY(hipMemsetAsync(vcnt, 0, (size_t)(V + 1) * 4, s));
+            HIP_TRY(launch_edge_count(m, vcnt, erank, voff, (uint32_t *)c->nbr.p, 0, s));
+            HIP_TRY(launch_scan_u32(vcnt, (uint64_t)V + 1, voff, (uint64_t *)c->tiles.p, ctr + CTR_EDGE_TOTAL, s));
+            HIP_TRY(launch_edge_count(m, vcnt, erank, voff, (uint32_t *)c->nbr.p, 1, s));
+            HIP_TRY(hipMemsetAsync(c->bcnt.p, 0, (size_t)(V + 1) * 4, s));
+            HIP_TRY(
+                launch_edge_singletons(V, voff, (uint32_t *)c->nbr.p, (uint32_t *)c->bcnt.p, nullptr, nullptr, 0, s));
+            HIP_TRY(launch_scan_u32((uint32_t *)c->bcnt.p, (uint64_t)V + 1, (uint32_t *)c->boff.p,
+                                    (uint64_t *)c->tiles.p + scan_tile_count(c->n_words) + 16, ctr + CTR_BEDGE_TOTAL,
+                                    s));
+            toc(c, KS_EDGES, s);
+            if ((st = read_counters(c, s)))
+                return st;
+        }
         c->E = (int64_t)c->h_ctr[CTR_BEDGE_TOTAL];
         if (c->E > 0)
         {
             if ((st = ensure(c->bedge_v, (size_t)c->E * sizeof(int2))) ||
                 (st = ensure(c->bedges, (size_t)c->E * sizeof(BEdge))))
                 return st;
-            HIP_TRY(launch_edge_singletons(V, (const uint32_t *)c->voff.p, (uint32_t *)c->nbr.p,
-                                           (uint32_t *)c->bcnt.p, (const uint32_t *)c->boff.p,
-                                           (int2 *)c->bedge_v.p, 1, s));
+            if (c->edge_csr)
+                HIP_TRY(launch_edge_singletons(V, (const uint32_t *)c->voff.p, (uint32_t *)c->nbr.p,
+                                               (uint32_t *)c->bcnt.p, (const uint32_t *)c->boff.p,
+                                               (int2 *)c->bedge_v.p, 1, s));
+            else
+                HIP_TRY(launch_edge_singletons_slots(V, (const uint32_t *)c->vcnt.p, (const uint32_t *)c->eslots.p,
+                                                     (uint32_t *)c->bcnt.p, (const uint32_t *)c->boff.p,
+                                                     (int2 *)c->bedge_v.p, 1, nullptr, s));
             HIP_TRY(launch_bedge_prep(m.v, c->E, (const int2 *)c->bedge_v.p, (BEdge *)c->bedges.p, s));
         }
     }

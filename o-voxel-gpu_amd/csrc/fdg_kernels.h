@@ -36,6 +36,7 @@ enum
     CTR_DM_QUADS = 12,   // dual grid -> mesh: quads (scan total)
     CTR_SVO_BYTES = 13,  // octree bytes of an encode (scan total)
     CTR_HUGE = 14,       // bricks left to the voxel-range fold (beyond the mid-size tile)
+    CTR_EDGE_OVER = 15,  // a vertex has more half-edges than its slots (boundary edges re-run on the CSR lists)
     // sum and largest shard of each sharded buffer (filled by k_shard_reduce)
     CTR_SUM = 16,        // CTR_SUM + SB_*  : total items requested
     CTR_MAX = 32,        // CTR_MAX + SB_*  : largest per-shard request
@@ -323,6 +324,16 @@ hipError_t launch_solve_voxels(const FoldArgs &a, const int32_t *coords, uint64_
 hipError_t launch_solve_box(const FoldArgs &a, hipStream_t s);
 hipError_t launch_edge_count(const MeshDev &m, uint32_t *vcnt, uint32_t *rank, const uint32_t *voff, uint32_t *nbr,
                              int pass, hipStream_t s);
+// Boundary edges through fixed per-vertex slots: each half-edge's larger
+// vertex goes to a slot of its smaller vertex (EDGE_SLOTS per vertex, count
+// in vcnt); a vertex with more sets `over` and the caller re-runs the CSR
+// lists below.  Singletons: pass 0 counts each vertex's boundary edges into
+// bcnt, pass 1 writes them at boff in (min, max) order.
+constexpr int EDGE_SLOTS = 16;
+hipError_t launch_edge_slots(const MeshDev &m, uint32_t *vcnt, uint32_t *slots, hipStream_t s);
+hipError_t launch_edge_singletons_slots(int64_t V, const uint32_t *vcnt, const uint32_t *slots, uint32_t *bcnt,
+                                        const uint32_t *boff, int2 *bedge_v, int pass, unsigned long long *over,
+                                        hipStream_t s);
 hipError_t launch_edge_singletons(int64_t V, const uint32_t *voff, uint32_t *nbr, uint32_t *bcnt, const uint32_t *boff,
                                   int2 *bedge_v, int pass, hipStream_t s);
 hipError_t launch_bedge_prep(const float *verts, int64_t E, const int2 *bedge_v, BEdge *be, hipStream_t s);

@@ -1193,6 +1193,108 @@ __global__ __launch_bounds__(BT) void k_edge_count(MeshDev m, uint32_t *vcnt, ui
     }
 }
 
+// one thread per face: the larger vertex of each of its 3 half-edges goes to
+// a slot of the smaller one
+__global__ __launch_bounds__(BT) void k_edge_slots(MeshDev m, uint32_t *vcnt, uint32_t *slots)
+{
+    const int64_t stride = (int64_t)gridDim.x * BT;
+    for (int64_t f = (int64_t)blockIdx.x * BT + threadIdx.x; f < m.F; f += stride)
+    {
+        const int32_t *fi = m.f + 3 * f;
+        const int32_t v[3] = {fi[0], fi[1], fi[2]};
+        if (!face_ok(m, v[0], v[1], v[2]))
+            continue; // flagged by the prep kernel
+        int32_t a[3], b[3];
+        uint32_t r[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+        {
+            const int32_t p = v[k], q = v[k == 2 ? 0 : k + 1];
+            a[k] = p < q ? p : q;
+            b[k] = p < q ? q : p;
+            r[k] = atomicAdd(vcnt + a[k], 1u);
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (r[k] < (uint32_t)EDGE_SLOTS)
+                slots[(uint64_t)a[k] * EDGE_SLOTS + r[k]] = (uint32_t)b[k];
+    }
+}
+
+// ascending sort of 16 values in registers (bitonic network)
+__device__ __forceinline__ void sort16(uint32_t (&x)[16])
+{
+#pragma unroll
+    for (int size = 2; size <= 16; size <<= 1)
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1)
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+            {
+                const int j = i ^ stride;
+                if (j > i)
+                {
+                    const bool up = (i & size) == 0;
+                    const uint32_t lo = x[i] < x[j] ? x[i] : x[j];
+                    const uint32_t hi = x[i] < x[j] ? x[j] : x[i];
+                    x[i] = up ? lo : hi;
+                    x[j] = up ? hi : lo;
+                }
+            }
+}
+
+// each vertex's slots sorted in registers: count its singletons (pass 0) or
+// write its boundary edges in (min, max) order (pass 1)
+__global__ __launch_bounds__(BT) void k_edge_singletons_slots(int64_t V, const uint32_t *vcnt, const uint32_t *slots,
+                                                               uint32_t *bcnt, const uint32_t *boff, int2 *bedge_v,
+                                                               int pass, unsigned long long *over)
+{
+    const int64_t stride = (int64_t)gridDim.x * BT;
+    for (int64_t v = (int64_t)blockIdx.x * BT + threadIdx.x; v < V; v += stride)
+    {
+        const uint32_t c = vcnt[v];
+        if (c > (uint32_t)EDGE_SLOTS)
+        {
+            if (pass == 0)
+                atomicOr(over, 1ull);
+            continue;
+        }
+        if (c == 0)
+            continue;
+        uint32_t x[16];
+        const uint4 *sv = (const uint4 *)(slots + (uint64_t)v * EDGE_SLOTS);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+        {
+            const uint4 w = sv[q];
+            x[4 * q + 0] = w.x;
+            x[4 * q + 1] = w.y;
+            x[4 * q + 2] = w.z;
+            x[4 * q + 3] = w.w;
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            x[i] = (uint32_t)i < c ? x[i] : 0xffffffffu; // unused slots sort last
+        sort16(x);
+        uint32_t n = 0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+        {
+            if ((uint32_t)i >= c)
+                break;
+            const bool single = (i == 0 || x[i] != x[i - 1]) && ((uint32_t)(i + 1) >= c || x[i] != x[i + 1]);
+            if (single)
+            {
+                if (pass == 1)
+                    bedge_v[boff[v] + n] = make_int2((int)v, (int)x[i]);
+                ++n;
+            }
+        }
+        if (pass == 0)
+            bcnt[v] = n;
+    }
+}
+
 // sort each vertex's neighbour list, count singletons (pass 0) or write the
 // boundary edges in rank order (pass 1)
 __global__ __launch_bounds__(BT) void k_edge_singletons(int64_t V, const uint32_t *voff, uint32_t *nbr, uint32_t *bcnt,
@@ -3069,6 +3171,21 @@ hipError_t launch_edge_count(const MeshDev &m, uint32_t *vcnt, uint32_t *rank, c
                              int pass, hipStream_t s)
 {
     hipLaunchKernelGGL(k_edge_count, dim3(grid_for(3 * m.F, 65536)), dim3(BT), 0, s, m, vcnt, rank, voff, nbr, pass);
+    return hipGetLastError();
+}
+
+hipError_t launch_edge_slots(const MeshDev &m, uint32_t *vcnt, uint32_t *slots, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_edge_slots, dim3(grid_for(m.F, 65536)), dim3(BT), 0, s, m, vcnt, slots);
+    return hipGetLastError();
+}
+
+hipError_t launch_edge_singletons_slots(int64_t V, const uint32_t *vcnt, const uint32_t *slots, uint32_t *bcnt,
+                                        const uint32_t *boff, int2 *bedge_v, int pass, unsigned long long *over,
+                                        hipStream_t s)
+{
+    hipLaunchKernelGGL(k_edge_singletons_slots, dim3(grid_for(V, 65536)), dim3(BT), 0, s, V, vcnt, slots, bcnt, boff,
+                       bedge_v, pass, over);
     return hipGetLastError();
 }
 
