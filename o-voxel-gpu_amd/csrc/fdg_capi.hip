@@ -152,6 +152,8 @@ struct ovx_fdg_ctx
     // per-kernel timing (events around each launch on the caller's stream)
     bool timing = false;
     hipEvent_t kev[2 * 24] = {};
+    hipStream_t side = nullptr;          // boundary-edge stream (default priority), beside the scanline stage
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool kran[24] = {};
     DualMeshArgs dm{}; // dual grid -> mesh between begin and finish
 
@@ -582,6 +584,49 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *
 
     for (bool &r : c->kran)
         r = false;
+    // boundary edges (edge multiset of FDG.cpp:728-739, kept in std::map order)
+    // depend on the faces only: they run on the side stream, beside the
+    // scanline and compaction stages, and join before the readback.
+    // Half-edges are bucketed by their smaller vertex (fixed slots per
+    // vertex), each vertex's list sorted in registers, singletons counted; a
+    // vertex with more half-edges than slots re-runs the CSR lists after the
+    // readback.
+    c->E = 0;
+    bool edges_forked = false;
+    if (planes && c->bw > 0.0f)
+    {
+        if ((uint64_t)m.F * 3ull >= (1ull << 32))
+            return fail(OVX_ERR_CAPACITY, "more than 2^32 half-edges");
+        if ((st = ensure(c->vcnt, (size_t)(V + 1) * 4)) || (st = ensure(c->eslots, (size_t)V * EDGE_SLOTS * 4 + 16)) ||
+            (st = ensure(c->bcnt, (size_t)(V + 1) * 4)) || (st = ensure(c->boff, (size_t)(V + 1) * 4)) ||
+            (st = ensure(c->tiles, (scan_tile_count(n_words) + 2 * scan_tile_count((uint64_t)V + 1) +
+                                    scan_tile_count((uint64_t)m.F * 3 + 1) + 64) * 8)))
+            return st;
+        if (!c->side)
+        {
+            HIP_TRY(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+            HIP_TRY(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+        }
+        const hipStream_t es = c->side;
+        HIP_TRY(hipEventRecord(c->ev_fork, s)); // counters cleared, inputs in place
+        HIP_TRY(hipStreamWaitEvent(es, c->ev_fork, 0));
+        edges_forked = true;
+        c->edge_csr = false;
+        auto *vcnt = (uint32_t *)c->vcnt.p;
+        tic(c, KS_EDGES, es);
+        HIP_TRY(hipMemsetAsync(vcnt, 0, (size_t)(V + 1) * 4, es));
+        HIP_TRY(hipMemsetAsync(ctr + CTR_EDGE_OVER, 0, sizeof(unsigned long long), es));
+        HIP_TRY(launch_edge_slots(m, vcnt, (uint32_t *)c->eslots.p, es));
+        HIP_TRY(hipMemsetAsync(c->bcnt.p, 0, (size_t)(V + 1) * 4, es));
+        HIP_TRY(launch_edge_singletons_slots(V, vcnt, (const uint32_t *)c->eslots.p, (uint32_t *)c->bcnt.p, nullptr,
+                                             nullptr, 0, ctr + CTR_EDGE_OVER, es));
+        // its scan tiles sit after the word scan's
+        HIP_TRY(launch_scan_u32((uint32_t *)c->bcnt.p, (uint64_t)V + 1, (uint32_t *)c->boff.p,
+                                (uint64_t *)c->tiles.p + scan_tile_count(n_words) + 16, ctr + CTR_BEDGE_TOTAL, es));
+        toc(c, KS_EDGES, es);
+        HIP_TRY(hipEventRecord(c->ev_join, es));
+    }
     tic(c, KS_PREP, s);
     HIP_TRY(hipMemsetAsync(c->bits.p, 0, n_words * sizeof(uint64_t), s));
     HIP_TRY(launch_prep(m, c->bdev, planes ? (float4 *)c->planes.p : nullptr, ctr, s));
@@ -617,33 +662,8 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *
     HIP_TRY(launch_seg_voxels(c->bdev, (const uint32_t *)c->woff.p, n_words, ctr + CTR_WORDS_TOTAL, d_vox_off, s));
     HIP_TRY(hipMemcpyAsync(c->h_vox_off, d_vox_off, (size_t)(M + 1) * 8, hipMemcpyDeviceToHost, s));
     toc(c, KS_WORDSCAN, s);
-    c->E = 0;
-    if (planes && c->bw > 0.0f)
-    {
-        // boundary edges (edge multiset of FDG.cpp:728-739, kept in std::map order):
-        // half-edges bucketed by their smaller vertex (fixed slots per vertex),
-        // each vertex's list sorted in registers, singletons counted; a vertex
-        // with more half-edges than slots re-runs the CSR lists after the readback
-        if ((uint64_t)m.F * 3ull >= (1ull << 32))
-            return fail(OVX_ERR_CAPACITY, "more than 2^32 half-edges");
-        if ((st = ensure(c->vcnt, (size_t)(V + 1) * 4)) || (st = ensure(c->eslots, (size_t)V * EDGE_SLOTS * 4 + 16)) ||
-            (st = ensure(c->bcnt, (size_t)(V + 1) * 4)) ||
-            (st = ensure(c->boff, (size_t)(V + 1) * 4)) ||
-            (st = ensure(c->tiles, (scan_tile_count(n_words) + 2 * scan_tile_count((uint64_t)V + 1) + 64) * 8)))
-            return st;
-        c->edge_csr = false;
-        auto *vcnt = (uint32_t *)c->vcnt.p;
-        tic(c, KS_EDGES, s);
-        HIP_TRY(hipMemsetAsync(vcnt, 0, (size_t)(V + 1) * 4, s));
-        HIP_TRY(hipMemsetAsync(ctr + CTR_EDGE_OVER, 0, sizeof(unsigned long long), s));
-        HIP_TRY(launch_edge_slots(m, vcnt, (uint32_t *)c->eslots.p, s));
-        HIP_TRY(hipMemsetAsync(c->bcnt.p, 0, (size_t)(V + 1) * 4, s));
-        HIP_TRY(launch_edge_singletons_slots(V, vcnt, (const uint32_t *)c->eslots.p, (uint32_t *)c->bcnt.p, nullptr,
-                                             nullptr, 0, ctr + CTR_EDGE_OVER, s));
-        HIP_TRY(launch_scan_u32((uint32_t *)c->bcnt.p, (uint64_t)V + 1, (uint32_t *)c->boff.p,
-                                (uint64_t *)c->tiles.p + scan_tile_count(n_words) + 16, ctr + CTR_BEDGE_TOTAL, s));
-        toc(c, KS_EDGES, s);
-    }
+    if (edges_forked)
+        HIP_TRY(hipStreamWaitEvent(s, c->ev_join, 0)); // boundary edges done
     HIP_TRY(launch_shard_reduce((unsigned long long *)c->shard_ctr.p, ctr, s));
     if ((st = read_counters(c, s)))
         return st;
@@ -940,6 +960,13 @@ ovx_status ovx_fdg_destroy(ovx_fdg_ctx *c)
         for (auto &e : c->kev)
             if (e)
                 (void)hipEventDestroy(e);
+        if (c->side)
+        {
+            (void)hipStreamSynchronize(c->side);
+            (void)hipEventDestroy(c->ev_fork);
+            (void)hipEventDestroy(c->ev_join);
+            (void)hipStreamDestroy(c->side);
+        }
     }
     delete c;
     return OVX_OK;
