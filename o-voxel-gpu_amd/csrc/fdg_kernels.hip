@@ -816,14 +816,15 @@ struct FaceRow
 // face_pass: with p the voxel's min corner, n.p must lie between -d1 and
 // -d2, an interval of x for |n_x| large enough.  The interval is computed in
 // double and widened by a bound on the FP32 test's rounding (a few ulp of
-// the terms of n.p + d, over |n_x|) plus 2 voxels; for |n_x| < 0.05 the
-// whole bounding-box row is kept.
+// the terms of n.p + d, over |n_x|) plus 2 voxels; for |n_x| < 0.05, and for
+// rows of at most 4 voxels, the whole bounding-box row is kept.
 __device__ __forceinline__ void face_row_range(const FaceItem &it, const GridDev &g, int y, int z, int &x0, int &x1)
 {
     x0 = it.bx0;
     x1 = it.bx1;
     const double nx = (double)it.n[0];
-    if (!(fabs(nx) >= 0.05))
+    // rows of a few voxels: the bounding-box range costs less than narrowing it
+    if (!(fabs(nx) >= 0.05) || it.bx1 - it.bx0 <= 4)
         return;
     const double vs0 = (double)g.vs[0];
     const double py = (double)g.vs[1] * (double)y, pz = (double)g.vs[2] * (double)z;
