@@ -768,7 +768,9 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
     uint64_t want_fhit = per_for(face_est);
     uint64_t want_bhit = per_for(16ull * (uint64_t)c->E + 4096);
     const uint64_t lim = 0xffffffffull / NSHARD; // record offsets and chunk indices are 32-bit
-    for (int attempt = 0; attempt < 2; ++attempt)
+    // at most 3 attempts: hits that overflowed are dropped, so the record
+    // counts of the attempt that first holds every hit are the exact ones
+    for (int attempt = 0; attempt < 3; ++attempt)
     {
         want_rec = std::min(want_rec, lim);
         if ((st = ensure_shard(c, c->recs, SB_RECS, want_rec, sizeof(Rec))) ||
@@ -837,8 +839,8 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
         if (sum[SB_RECS] >= 0xffffffffull || sum[SB_CHUNKS] >= 0xffffffffull)
             return fail(OVX_ERR_CAPACITY, "%llu QEF records / %llu chunks exceed the 32-bit record index",
                         (unsigned long long)sum[SB_RECS], (unsigned long long)sum[SB_CHUNKS]);
-        if (attempt == 1)
-            return fail(OVX_ERR_STATE, "record buffers overflowed twice (internal error)");
+        if (attempt == 2)
+            return fail(OVX_ERR_STATE, "record buffers overflowed three times (internal error)");
         // sizes are known now: grow and generate again
         want_fhit = mx[SB_FHITS] + mx[SB_FHITS] / 4 + 256;
         want_bhit = mx[SB_BHITS] + mx[SB_BHITS] / 4 + 256;

@@ -981,9 +981,12 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6))) void k_
     __shared__ uint32_t incl_rows[BT];
     __shared__ uint64_t sh64[NWAVE];
     __shared__ uint32_t sh32[NWAVE];
+    __shared__ unsigned long long hbase;
+    __shared__ uint32_t hwave[NWAVE];
     __shared__ QueueMap qm;
     const int tid = threadIdx.x;
     uint32_t n_hit = 0;
+    uint32_t n_claim = 0; // hit claims of this block (block-uniform)
     int64_t n_items = 0;
     if constexpr (QUEUED)
     {
@@ -1132,25 +1135,39 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6))) void k_
                         break;
                     }
                 }
-                // wave-aggregated append into this wave's shard (every lane of
-                // the block runs this iteration)
+                // block-aggregated append (every thread of the block runs this
+                // iteration): the iteration's hits, from the tile's faces, stay
+                // together in the queue, so the records made of them fall in few
+                // bricks per generator block
                 {
+                    // per-wave counts through LDS (hwave and hbase are rewritten
+                    // only after every thread has passed the next barrier)
                     const uint64_t m = __ballot(has);
-                    const int lane = tid & 63;
-                    if (m)
+                    const int lane = tid & 63, w = tid >> 6;
+                    if (lane == 0)
+                        hwave[w] = (uint32_t)__popcll(m);
+                    __syncthreads();
+                    uint32_t before = 0, tot_h = 0;
+#pragma unroll
+                    for (int k = 0; k < NWAVE; ++k)
                     {
-                        const int leader = __ffsll((long long)m) - 1;
-                        unsigned long long base = 0;
-                        bool okk = false;
-                        if (lane == leader)
+                        before += k < w ? hwave[k] : 0u;
+                        tot_h += hwave[k];
+                    }
+                    if (tot_h)
+                    {
+                        if (tid == 0)
                         {
-                            base = shard_claim(hb, (blockIdx.x * NWAVE + (tid >> 6)) % NSHARD,
-                                               (uint64_t)__popcll(m), okk);
-                            base = okk ? base : ~0ull;
+                            // successive claims of a block rotate over NWAVE shards
+                            bool okk = false;
+                            const uint32_t sh = (blockIdx.x * NWAVE + (n_claim & (NWAVE - 1))) % NSHARD;
+                            const unsigned long long b0 = shard_claim(hb, sh, (uint64_t)tot_h, okk);
+                            hbase = okk ? b0 : ~0ull;
                         }
-                        base = __shfl(base, leader, 64);
-                        if (has && base != ~0ull)
-                            hb.items[base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull))] = hit;
+                        ++n_claim;
+                        __syncthreads();
+                        if (has && hbase != ~0ull)
+                            hb.items[hbase + before + (uint64_t)__popcll(m & ((1ull << lane) - 1ull))] = hit;
                     }
                 }
             }
