@@ -152,7 +152,7 @@ struct ovx_fdg_ctx
     // per-kernel timing (events around each launch on the caller's stream)
     bool timing = false;
     hipEvent_t kev[2 * 24] = {};
-    hipStream_t side = nullptr;          // boundary-edge stream (default priority), beside the scanline stage
+    hipStream_t side = nullptr; // second stream (default priority): boundary edges beside the scanline stage, crossing and boundary records beside the face test
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool kran[24] = {};
     DualMeshArgs dm{}; // dual grid -> mesh between begin and finish
@@ -314,6 +314,17 @@ ShardBuf<T> shard_view(ovx_fdg_ctx *c, DevBuf &b, int sb)
 
 // per-shard capacity for `total` items spread over the shards, with headroom
 uint64_t per_for(uint64_t total) { return (total + total / 4) / NSHARD + 256; }
+
+// the side stream (default priority) and its fork / join events, made once per context
+ovx_status ensure_side(ovx_fdg_ctx *c)
+{
+    if (c->side)
+        return OVX_OK;
+    HIP_TRY(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    return OVX_OK;
+}
 
 ovx_status ensure_shard(ovx_fdg_ctx *c, DevBuf &b, int sb, uint64_t want_per, size_t item)
 {
@@ -602,12 +613,8 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *
             (st = ensure(c->tiles, (scan_tile_count(n_words) + 2 * scan_tile_count((uint64_t)V + 1) +
                                     scan_tile_count((uint64_t)m.F * 3 + 1) + 64) * 8)))
             return st;
-        if (!c->side)
-        {
-            HIP_TRY(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
-            HIP_TRY(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
-            HIP_TRY(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
-        }
+        if ((st = ensure_side(c)))
+            return st;
         const hipStream_t es = c->side;
         HIP_TRY(hipEventRecord(c->ev_fork, s)); // counters cleared, inputs in place
         HIP_TRY(hipStreamWaitEvent(es, c->ev_fork, 0));
@@ -792,9 +799,27 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
         ba.b = c->bdev;
         ba.bedge_v = (const int2 *)c->bedge_v.p;
         ba.ctr = ctr;
-        tic(c, KS_GEN_EVENTS, s);
-        HIP_TRY(launch_gen_events(shard_view<Event>(c, c->events, SB_EVENTS), ba, s));
-        toc(c, KS_GEN_EVENTS, s);
+        // records of the crossings and the boundary DDA on the side stream,
+        // beside the face test (which alone needs the main stream's order)
+        if ((st = ensure_side(c)))
+            return st;
+        const hipStream_t es = c->side;
+        HIP_TRY(hipEventRecord(c->ev_fork, s)); // buffers cleared
+        HIP_TRY(hipStreamWaitEvent(es, c->ev_fork, 0));
+        tic(c, KS_GEN_EVENTS, es);
+        HIP_TRY(launch_gen_events(shard_view<Event>(c, c->events, SB_EVENTS), ba, es));
+        toc(c, KS_GEN_EVENTS, es);
+        if (c->E > 0)
+        {
+            const ShardBuf<BHit> hb = shard_view<BHit>(c, c->bhits, SB_BHITS);
+            tic(c, KS_BDRY, es);
+            HIP_TRY(launch_bdry(c->E, (const BEdge *)c->bedges.p, (const int2 *)c->bedge_v.p, c->bdev, hb, ctr, es));
+            toc(c, KS_BDRY, es);
+            tic(c, KS_GEN_BHITS, es);
+            HIP_TRY(launch_gen_bhits(hb, ba, es));
+            toc(c, KS_GEN_BHITS, es);
+        }
+        HIP_TRY(hipEventRecord(c->ev_join, es));
         if (c->fw > 0.0f)
         {
             const ShardBuf<FHit> fb = shard_view<FHit>(c, c->fhits, SB_FHITS);
@@ -809,16 +834,7 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
             HIP_TRY(launch_gen_fhits(fb, ba, s));
             toc(c, KS_GEN_FHITS, s);
         }
-        if (c->E > 0)
-        {
-            const ShardBuf<BHit> hb = shard_view<BHit>(c, c->bhits, SB_BHITS);
-            tic(c, KS_BDRY, s);
-            HIP_TRY(launch_bdry(c->E, (const BEdge *)c->bedges.p, (const int2 *)c->bedge_v.p, c->bdev, hb, ctr, s));
-            toc(c, KS_BDRY, s);
-            tic(c, KS_GEN_BHITS, s);
-            HIP_TRY(launch_gen_bhits(hb, ba, s));
-            toc(c, KS_GEN_BHITS, s);
-        }
+        HIP_TRY(hipStreamWaitEvent(s, c->ev_join, 0));
         HIP_TRY(launch_shard_reduce((unsigned long long *)c->shard_ctr.p, ctr, s));
         if ((st = read_counters(c, s)))
             return st;
