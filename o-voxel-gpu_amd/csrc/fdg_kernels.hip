@@ -2156,8 +2156,9 @@ struct FoldPick<1>
     using type = FoldStateC;
 };
 
-constexpr int FOLD_SMALL = 2048; // records of a brick held in LDS by the main fold kernel (5 blocks per CU)
+constexpr int FOLD_SMALL = 1536; // records of a brick held in LDS by the main fold kernel (25 KB: 6 blocks per CU)
 constexpr int FOLD_MID = 3584;   // ... by its second launch over the bricks beyond that (3 blocks per CU)
+constexpr int FOLD_WAVES = 6;    // occupancy of the reference-path main fold (waves per SIMD: at most 80 VGPRs)
 constexpr int FOLD_CAP = 4096;   // records held in LDS at a time by the kernel for large bricks (2 blocks per CU)
 
 // LDS view of a batch of up to BT chunks of one brick
@@ -2385,7 +2386,7 @@ __device__ __forceinline__ void brick_origin(const FoldArgs &a, uint32_t brick, 
 template <int CAP>
 struct FoldLdsG
 {
-    // 16-bit offsets keep the 2048-record layout under 32 KB: 5 blocks per CU
+    // 12 B per record and 16-bit offsets: 25 KB at 1536 records, 6 blocks per CU
     static_assert(CAP <= 65535, "16-bit record offsets");
     uint64_t key[CAP];
     float z[CAP];
@@ -2513,7 +2514,7 @@ __device__ __forceinline__ void fold_grouped(const FoldArgs &a, FoldLdsG<CAP> &L
 // ones listed in `large`; MID = true: the `large` list, larger ones listed
 // in `huge` (for k_fold_large).
 template <int P, int CAP, bool MID>
-__global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs ain)
+__device__ __forceinline__ void fold_bricks_body(const FoldArgs &ain)
 {
     __shared__ FoldLdsG<CAP> L;
     constexpr int PER = CAP / BT;
@@ -2602,6 +2603,24 @@ __global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs ain)
             atomicAdd(a.ctr + CTR_LARGE_RECS, (unsigned long long)n);
         fold_grouped<CAP, P>(a, L, n, rk, rz, rl, x0, y0, z0);
     }
+}
+
+// the kernels: the main tile for the reference path (6 waves per SIMD: its
+// LDS and at most 80 VGPRs leave room for 6 blocks per CU), the main tile for
+// the precision variant, and the mid-size tile
+template <int P>
+__global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs a)
+{
+    fold_bricks_body<P, FOLD_SMALL, false>(a);
+}
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(FOLD_WAVES))) void k_fold_bricks_ref(FoldArgs a)
+{
+    fold_bricks_body<0, FOLD_SMALL, false>(a);
+}
+template <int P>
+__global__ __launch_bounds__(BT) void k_fold_mid(FoldArgs a)
+{
+    fold_bricks_body<P, FOLD_MID, true>(a);
 }
 
 // Bricks with more records than k_fold_bricks holds.  A brick
@@ -3136,9 +3155,9 @@ hipError_t launch_fold_bricks(const FoldArgs &a, uint64_t max_bricks, hipStream_
     if (nb == 0)
         nb = 1;
     if (a.prec)
-        hipLaunchKernelGGL((k_fold_bricks<1, FOLD_SMALL, false>), dim3((unsigned)nb), dim3(BT), 0, s, a);
+        hipLaunchKernelGGL(k_fold_bricks<1>, dim3((unsigned)nb), dim3(BT), 0, s, a);
     else
-        hipLaunchKernelGGL((k_fold_bricks<0, FOLD_SMALL, false>), dim3((unsigned)nb), dim3(BT), 0, s, a);
+        hipLaunchKernelGGL(k_fold_bricks_ref, dim3((unsigned)nb), dim3(BT), 0, s, a);
     return hipGetLastError();
 }
 
@@ -3148,9 +3167,9 @@ hipError_t launch_fold_mid(const FoldArgs &a, uint64_t max_bricks, hipStream_t s
     if (nb == 0)
         nb = 1;
     if (a.prec)
-        hipLaunchKernelGGL((k_fold_bricks<1, FOLD_MID, true>), dim3((unsigned)nb), dim3(BT), 0, s, a);
+        hipLaunchKernelGGL(k_fold_mid<1>, dim3((unsigned)nb), dim3(BT), 0, s, a);
     else
-        hipLaunchKernelGGL((k_fold_bricks<0, FOLD_MID, true>), dim3((unsigned)nb), dim3(BT), 0, s, a);
+        hipLaunchKernelGGL(k_fold_mid<0>, dim3((unsigned)nb), dim3(BT), 0, s, a);
     return hipGetLastError();
 }
 
