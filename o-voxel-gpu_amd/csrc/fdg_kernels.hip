@@ -971,7 +971,7 @@ __device__ __forceinline__ uint64_t row_mask(int lo_bit, int hi_bit)
 // QUEUED = false: one item per face; QUEUED = true: the queued row chunks of
 // large faces (items index the queue's slots)
 template <bool QUEUED, int FACE_TILE>
-__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6))) void k_face(MeshDev m, BatchDev bt, uint64_t n_tiles, ShardBuf<FHit> hb,
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(7))) void k_face(MeshDev m, BatchDev bt, uint64_t n_tiles, ShardBuf<FHit> hb,
                                              unsigned long long *ctr, ShardBuf<FaceChunk> fq)
 {
     static_assert(FACE_TILE <= BT, "a tile's faces are set up one per thread");
