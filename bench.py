@@ -241,8 +241,9 @@ def main():
     alg = algorithmic_bytes(dom, st, F, V, n_words)
     achieved = alg / (kern_avg[dom] * 1e-3) / 1e9 if alg else None
     traffic = None
+    # the committed traffic was measured on the reference-path kernels
     tpath = os.path.join(ROOT, "profiles", f"{dom}_{args.workload}_traffic.json")
-    if os.path.exists(tpath):
+    if os.path.exists(tpath) and args.precision == "reference":
         with open(tpath) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
     roofline = {
