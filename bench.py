@@ -87,7 +87,7 @@ def algorithmic_bytes(slot, st, F, V, n_words):
         "mark": 12 * F + 12 * V + 8 * n_words + 16 * ev,
         # bitset read, word offsets written
         "word_scan": 8 * n_words + 4 * n_words,
-        # half-edges bucketed by min vertex: faces read, 3F neighbour slots written + read
+        # half-edges bucketed by min vertex: faces read, 3F neighbour entries written + read back
         "edges": 12 * F + 24 * F + 16 * V,
         # bitset + offsets read, coords written
         "coords": 12 * n_words + 12 * N,
