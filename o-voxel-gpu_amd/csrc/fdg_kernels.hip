@@ -241,19 +241,44 @@ __device__ __forceinline__ void bucket_emit(const uint32_t (&bk)[K], const Rec (
     {
         slot[k] = 0;
         rank[k] = 0;
-        if (ok[k])
+    }
+    // a thread's records of one brick (the 4 voxels around a crossing mostly
+    // share one) take one table lookup and one count update together
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+    {
+        if (!ok[k])
+            continue;
+        bool led = false; // an earlier record of this thread has the brick
+#pragma unroll
+        for (int j = 0; j < k; ++j)
+            led = led || (ok[j] && bk[j] == bk[k]);
+        if (led)
+            continue;
+        uint32_t n = 1;
+#pragma unroll
+        for (int j = k + 1; j < K; ++j)
+            n += (ok[j] && bk[j] == bk[k]) ? 1u : 0u;
+        uint32_t h = (bk[k] * 2654435761u) >> (32 - LOG2);
+        while (true)
         {
-            uint32_t h = (bk[k] * 2654435761u) >> (32 - LOG2);
-            while (true)
-            {
-                const uint32_t prev = atomicCAS(&L.key[h], 0xffffffffu, bk[k]);
-                if (prev == 0xffffffffu || prev == bk[k])
-                    break;
-                h = (h + 1u) & (uint32_t)(N - 1);
-            }
-            slot[k] = (int)h;
-            rank[k] = atomicAdd(&L.cnt[h], 1u);
+            const uint32_t prev = atomicCAS(&L.key[h], 0xffffffffu, bk[k]);
+            if (prev == 0xffffffffu || prev == bk[k])
+                break;
+            h = (h + 1u) & (uint32_t)(N - 1);
         }
+        const uint32_t r0 = atomicAdd(&L.cnt[h], n);
+        slot[k] = (int)h;
+        rank[k] = r0;
+        uint32_t i = 1;
+#pragma unroll
+        for (int j = k + 1; j < K; ++j)
+            if (ok[j] && bk[j] == bk[k])
+            {
+                slot[j] = (int)h;
+                rank[j] = r0 + i;
+                ++i;
+            }
     }
     __syncthreads();
     // this thread's entries: PER consecutive table slots
