@@ -1584,7 +1584,7 @@ __global__ __launch_bounds__(BT) void k_gen_fhits(ShardBuf<FHit> h, BucketArgs b
 #pragma unroll
         for (int k = 0; k < 4; ++k)
         {
-            const uint64_t i = base + (uint64_t)k * BT + threadIdx.x;
+            const uint64_t i = base + 4ull * threadIdx.x + (uint64_t)k; // 4 consecutive hits: mostly one face's
             if (i < n)
             {
                 const FHit x = h.items[queue_slot(h, qm.pre, i)];
