@@ -791,7 +791,6 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
         if ((st = clear_shards(c, SB_FHITS, SB_N - SB_FHITS, s)))
             return st;
         HIP_TRY(hipMemsetAsync(c->heads.p, 0, (n_bricks + 1) * 4, s));
-        HIP_TRY(hipMemsetAsync(ctr + CTR_ACTIVE, 0, sizeof(unsigned long long), s));
         BucketArgs ba{};
         ba.recs = shard_view<Rec>(c, c->recs, SB_RECS);
         ba.chunks = shard_view<Chunk>(c, c->chunks, SB_CHUNKS);
@@ -911,8 +910,7 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
     a.cond_max = c->cond_max;
     a.n64 = (unsigned long long *)c->shard_ctr.p + (uint64_t)SB_FP64 * NSHARD * SHARD_STRIDE;
     a.ctr = ctr;
-    HIP_TRY(hipMemsetAsync(ctr + CTR_LARGE, 0, 2 * sizeof(unsigned long long), s)); // LARGE, LARGE_RECS
-    HIP_TRY(hipMemsetAsync(ctr + CTR_HUGE, 0, sizeof(unsigned long long), s));
+    // CTR_ACTIVE, CTR_LARGE, CTR_LARGE_RECS and CTR_HUGE were zeroed with all counters by begin
     tic(c, KS_FOLD, s);
     HIP_TRY(launch_fold_bricks(a, n_bricks, s));
     toc(c, KS_FOLD, s);
