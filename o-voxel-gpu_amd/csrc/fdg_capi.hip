@@ -533,12 +533,17 @@ ovx_status launch_face_tiles(ovx_fdg_ctx *c, const MeshDev &m, const ShardBuf<FH
             c->ftile_off[t][i + 1] =
                 c->ftile_off[t][i] + (t == ft ? (uint64_t)(nf + FACE_TILES[t] - 1) / FACE_TILES[t] : 0ull);
     }
-    // the staging region was last read by the set-up copy, complete since begin's readback
+    // the staging region was last read by the set-up copy, complete since
+    // begin's readback; one mesh needs no upload (the kernels read only entry
+    // 0 of its class's offsets, which is 0 in every class)
     const size_t arr = (size_t)(M + 1) * 8;
-    for (int t = 0; t < N_FACE_TILES; ++t)
-        std::memcpy(c->h_stage + c->stage_ftile + t * arr, c->ftile_off[t].data(), arr);
-    HIP_TRY(hipMemcpyAsync((unsigned char *)c->segbuf.p + c->stage_ftile, c->h_stage + c->stage_ftile,
-                           N_FACE_TILES * arr, hipMemcpyHostToDevice, s));
+    if (M > 1)
+    {
+        for (int t = 0; t < N_FACE_TILES; ++t)
+            std::memcpy(c->h_stage + c->stage_ftile + t * arr, c->ftile_off[t].data(), arr);
+        HIP_TRY(hipMemcpyAsync((unsigned char *)c->segbuf.p + c->stage_ftile, c->h_stage + c->stage_ftile,
+                               N_FACE_TILES * arr, hipMemcpyHostToDevice, s));
+    }
     for (int t = 0; t < N_FACE_TILES; ++t)
     {
         const uint64_t nt = c->ftile_off[t][M];
