@@ -96,3 +96,24 @@ def test_batch_invalid_face_index_raises():
     with pytest.raises(RuntimeError, match="vertex index"):
         _batch(meshes, dev)
     _batch(_meshes(2, seed=6, f_hi=2000), dev)  # the context keeps working
+
+
+def test_batch_with_fan_vertex_equals_single_calls():
+    """A mesh whose centre vertex has 80 spokes (more half-edges than the
+    per-vertex slots of the boundary-edge stage) makes the batch take the
+    general per-vertex lists; every mesh's outputs still equal its single call."""
+    dev = _dev()
+    meshes = _meshes(5, seed=21)
+    n = 80
+    ang = np.linspace(0.0, 1.8 * np.pi, n + 1)
+    ring = np.stack([0.3 * np.cos(ang), 0.3 * np.sin(ang), 0.05 * np.sin(3 * ang)], 1)
+    v = np.concatenate([np.zeros((1, 3)), ring]).astype(np.float32)
+    f = np.stack([np.zeros(n, np.int64), np.arange(1, n + 1), np.arange(2, n + 2)], 1)
+    meshes.insert(2, (v, f, 64))
+    print(f"[case] batch of {len(meshes)} meshes with an {n}-spoke fan", flush=True)
+    got = _batch(meshes, dev)
+    for k, (v, f, g) in enumerate(meshes):
+        want = _single(v, f, g, dev)
+        for a, b in zip(got[k], want):
+            assert a.shape == b.shape
+            assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), k
