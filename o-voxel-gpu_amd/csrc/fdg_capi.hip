@@ -620,24 +620,11 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *
             return st;
         if ((st = ensure_side(c)))
             return st;
-        const hipStream_t es = c->side;
+        // the fork point; the edge kernels themselves are queued after the
+        // scanline launch, so their host-side queueing does not hold it back
         HIP_TRY(hipEventRecord(c->ev_fork, s)); // counters cleared, inputs in place
-        HIP_TRY(hipStreamWaitEvent(es, c->ev_fork, 0));
         edges_forked = true;
         c->edge_csr = false;
-        auto *vcnt = (uint32_t *)c->vcnt.p;
-        tic(c, KS_EDGES, es);
-        HIP_TRY(hipMemsetAsync(vcnt, 0, (size_t)(V + 1) * 4, es));
-        HIP_TRY(hipMemsetAsync(ctr + CTR_EDGE_OVER, 0, sizeof(unsigned long long), es));
-        HIP_TRY(launch_edge_slots(m, vcnt, (uint32_t *)c->eslots.p, es));
-        HIP_TRY(hipMemsetAsync(c->bcnt.p, 0, (size_t)(V + 1) * 4, es));
-        HIP_TRY(launch_edge_singletons_slots(V, vcnt, (const uint32_t *)c->eslots.p, (uint32_t *)c->bcnt.p, nullptr,
-                                             nullptr, 0, ctr + CTR_EDGE_OVER, es));
-        // its scan tiles sit after the word scan's
-        HIP_TRY(launch_scan_u32((uint32_t *)c->bcnt.p, (uint64_t)V + 1, (uint32_t *)c->boff.p,
-                                (uint64_t *)c->tiles.p + scan_tile_count(n_words) + 16, ctr + CTR_BEDGE_TOTAL, es));
-        toc(c, KS_EDGES, es);
-        HIP_TRY(hipEventRecord(c->ev_join, es));
     }
     tic(c, KS_PREP, s);
     HIP_TRY(hipMemsetAsync(c->bits.p, 0, n_words * sizeof(uint64_t), s));
@@ -666,6 +653,24 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *
     if ((st = launch_mark(c, m, ctr, eb, rq, s)))
         return st;
     toc(c, KS_MARK, s);
+    if (edges_forked)
+    {
+        const hipStream_t es = c->side;
+        HIP_TRY(hipStreamWaitEvent(es, c->ev_fork, 0));
+        auto *vcnt = (uint32_t *)c->vcnt.p;
+        tic(c, KS_EDGES, es);
+        HIP_TRY(hipMemsetAsync(vcnt, 0, (size_t)(V + 1) * 4, es));
+        HIP_TRY(hipMemsetAsync(ctr + CTR_EDGE_OVER, 0, sizeof(unsigned long long), es));
+        HIP_TRY(launch_edge_slots(m, vcnt, (uint32_t *)c->eslots.p, es));
+        HIP_TRY(hipMemsetAsync(c->bcnt.p, 0, (size_t)(V + 1) * 4, es));
+        HIP_TRY(launch_edge_singletons_slots(V, vcnt, (const uint32_t *)c->eslots.p, (uint32_t *)c->bcnt.p, nullptr,
+                                             nullptr, 0, ctr + CTR_EDGE_OVER, es));
+        // its scan tiles sit after the word scan's
+        HIP_TRY(launch_scan_u32((uint32_t *)c->bcnt.p, (uint64_t)V + 1, (uint32_t *)c->boff.p,
+                                (uint64_t *)c->tiles.p + scan_tile_count(n_words) + 16, ctr + CTR_BEDGE_TOTAL, es));
+        toc(c, KS_EDGES, es);
+        HIP_TRY(hipEventRecord(c->ev_join, es));
+    }
     tic(c, KS_WORDSCAN, s);
     HIP_TRY(launch_scan_popc((const uint64_t *)c->bits.p, n_words, (uint32_t *)c->woff.p, (uint64_t *)c->tiles.p,
                              ctr + CTR_WORDS_TOTAL, s));
@@ -804,11 +809,24 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
         ba.bedge_v = (const int2 *)c->bedge_v.p;
         ba.ctr = ctr;
         // records of the crossings and the boundary DDA on the side stream,
-        // beside the face test (which alone needs the main stream's order)
+        // beside the face test (which alone needs the main stream's order);
+        // the face test is queued first, so the side stream's host-side
+        // queueing does not hold it back
         if ((st = ensure_side(c)))
             return st;
         const hipStream_t es = c->side;
         HIP_TRY(hipEventRecord(c->ev_fork, s)); // buffers cleared
+        const ShardBuf<FHit> fb = shard_view<FHit>(c, c->fhits, SB_FHITS);
+        if (c->fw > 0.0f)
+        {
+            tic(c, KS_FACE, s);
+            ShardBuf<FaceChunk> fq = shard_view<FaceChunk>(c, c->faceq, SB_FACEQ);
+            if (c->M > 1)
+                fq.items = nullptr; // row chunks are for single-mesh conversions
+            if ((st = launch_face_tiles(c, m, fb, ctr, fq, s)))
+                return st;
+            toc(c, KS_FACE, s);
+        }
         HIP_TRY(hipStreamWaitEvent(es, c->ev_fork, 0));
         tic(c, KS_GEN_EVENTS, es);
         HIP_TRY(launch_gen_events(shard_view<Event>(c, c->events, SB_EVENTS), ba, es));
@@ -826,14 +844,6 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
         HIP_TRY(hipEventRecord(c->ev_join, es));
         if (c->fw > 0.0f)
         {
-            const ShardBuf<FHit> fb = shard_view<FHit>(c, c->fhits, SB_FHITS);
-            tic(c, KS_FACE, s);
-            ShardBuf<FaceChunk> fq = shard_view<FaceChunk>(c, c->faceq, SB_FACEQ);
-            if (c->M > 1)
-                fq.items = nullptr; // row chunks are for single-mesh conversions
-            if ((st = launch_face_tiles(c, m, fb, ctr, fq, s)))
-                return st;
-            toc(c, KS_FACE, s);
             tic(c, KS_GEN_FHITS, s);
             HIP_TRY(launch_gen_fhits(fb, ba, s));
             toc(c, KS_GEN_FHITS, s);
