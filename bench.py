@@ -70,7 +70,9 @@ def make_workload(name, grid):
 
 
 def algorithmic_bytes(slot, st, F, V, n_words):
-    """Compulsory HBM bytes of one launch of a kernel slot."""
+    """Compulsory HBM bytes of one launch of a kernel slot: what the slot's
+    kernels must read and write at least once, per the comments below (the
+    roofline's `achieved` figure divides this by the slot's launch time)."""
     N, R, E, ev = st["voxels"], st["records"], st["boundary_edges"], st["events"]
     FR, BR = st["face_records"], st["boundary_records"]
     # the fold reads every record once, each face plane and boundary quadric
