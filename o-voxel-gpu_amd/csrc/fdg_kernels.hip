@@ -449,9 +449,12 @@ __device__ __forceinline__ void scan_item_setup(const MeshDev &m, const GridDev 
 struct ScanRow
 {
     double t3x, t3z, t4x, t4z;
-    int y_idx;
-    int line_start;
-    int item;
+    // row and first column as offsets from the grid's minimum (rows and
+    // columns are clamped to the grid, whose extent is at most 8192), and
+    // the row's item in the tile: 40 B a row
+    uint16_t y_off;
+    uint16_t x_off;
+    uint16_t item;
 };
 
 // the 4 voxels around a crossing, in the reference's (dx, dy) order
@@ -741,9 +744,9 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 rw.t3z = t3z;
                 rw.t4x = t4x;
                 rw.t4z = t4z;
-                rw.y_idx = y_idx;
-                rw.line_start = ls;
-                rw.item = L;
+                rw.y_off = (uint16_t)(y_idx - sel3i(ax1, g.gmin[0], g.gmin[1], g.gmin[2]));
+                rw.x_off = (uint16_t)(ls - lo);
+                rw.item = (uint16_t)L;
                 C = le > ls ? (uint32_t)(le - ls) : 0u; // le < ls only for non-finite input
             }
             uint32_t tot_c;
@@ -761,10 +764,13 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     const ScanRow &rw = rows[rr];
                     const ScanItem &it = items[rw.item];
                     const uint32_t ex = (rr == 0) ? 0u : incl_rows[rr - 1];
-                    const int x_idx = rw.line_start + (int)(h - ex);
                     const int ax2 = it.ax2;
                     const int ax0 = axis_next(ax2);
                     const int ax1 = axis_prev(ax2);
+                    const int ylo = sel3i(ax1, g.gmin[0], g.gmin[1], g.gmin[2]);
+                    const int xlo = sel3i(ax0, g.gmin[0], g.gmin[1], g.gmin[2]);
+                    const int y_idx = ylo + (int)rw.y_off;
+                    const int x_idx = xlo + (int)rw.x_off + (int)(h - ex);
                     const float vs0 = sel3f(ax0, g.vs[0], g.vs[1], g.vs[2]);
                     const float vs2 = sel3f(ax2, g.vs[0], g.vs[1], g.vs[2]);
                     const double x = (double)((float)(x_idx + 1) * vs0);
@@ -775,13 +781,11 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     if (z_idx >= zlo && z_idx < zhi)
                     {
                         ++n_ev;
-                        const int ylo = sel3i(ax1, g.gmin[0], g.gmin[1], g.gmin[2]);
-                        const int xlo = sel3i(ax0, g.gmin[0], g.gmin[1], g.gmin[2]);
                         const uint64_t keybase = ((uint64_t)it.face << KEY_FACE_SHIFT) |
                                                  ((uint64_t)ax2 << KEY_AX_SHIFT) |
-                                                 ((uint64_t)(rw.y_idx - ylo) << KEY_Y_SHIFT) |
+                                                 ((uint64_t)rw.y_off << KEY_Y_SHIFT) |
                                                  ((uint64_t)(x_idx - xlo) << KEY_X_SHIFT);
-                        mark_event<MERGE>(g, T, ax0, ax1, x_idx, rw.y_idx, z_idx);
+                        mark_event<MERGE>(g, T, ax0, ax1, x_idx, y_idx, z_idx);
                         evr.key = keybase;
                         evr.z = (float)z;
                         evr.zi = z_idx;
