@@ -232,3 +232,22 @@ def test_caller_stream_and_back_to_back(oracle):
         for a, b in zip(got, want):
             assert a.shape == b.shape
             assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+@pytest.mark.parametrize("gmin", [0, -4096])
+def test_full_extent_axis(oracle, gmin):
+    """A zig-zag strip across a grid axis of the maximum extent (8192
+    voxels): scanline rows run along that axis for one scan axis and columns
+    for another, so row / column offsets from the grid minimum reach 8191;
+    also with a negative grid minimum."""
+    n = 300
+    x = (gmin + np.linspace(0.0, 8192.0, n + 1)[:-1] + 0.37) / 8192.0
+    x = np.concatenate([x, [(gmin + 8191.6) / 8192.0]])
+    y = np.where(np.arange(n + 1) % 2 == 0, 0.13, 0.87)
+    z = 0.21 + 0.55 * y + 0.05 * np.sin(np.arange(n + 1))
+    v = np.stack([x, y, z], axis=1).astype(np.float32)
+    f = np.array([[i, i + 1, i + 2] for i in range(n - 1)], np.int32)
+    vs = [1.0 / 8192, 1.0 / 8, 1.0 / 8]
+    gr = [gmin, 0, 0, gmin + 8192, 8, 8]
+    rms, mx, exact = compare(oracle, v, f, vs, gr, label=f"strip over 8192 voxels, gmin {gmin}")
+    assert exact == 1.0
