@@ -10,7 +10,9 @@ passes).  Workloads:
 * ``scene``        48 randomly rotated, anisotropically scaled closed
                    icospheres and tori (radii 0.02-0.2), 32 open planar or
                    quadratic patches with free boundaries, and 1% sliver
-                   triangles (one edge ~1e-6 of the others); ~280k F; grid 512.
+                   triangles (one edge ~1e-6 of the others); texture seams
+                   split the shapes into small charts (~0.95 vertices per
+                   face); ~280k F / ~263k V; grid 512.
 * ``scene_large``  the ``scene`` generator at a larger face budget, then two
                    1-to-4 midpoint subdivisions with Gaussian vertex jitter
                    (sigma 1e-4), trimmed by a seeded face drop to 10.3M F.
@@ -22,6 +24,7 @@ passes).  Workloads:
 
 from __future__ import annotations
 
+import functools
 import math
 from typing import List, Tuple
 
@@ -63,7 +66,9 @@ def subdivide_midpoint(v: np.ndarray, f: np.ndarray) -> Mesh:
     return v2, nf
 
 
+@functools.lru_cache(maxsize=None)
 def icosphere(level: int) -> Mesh:
+    """Unit icosphere; the arrays are cached and shared, callers must not modify them in place."""
     v, f = _icosahedron()
     for _ in range(level):
         v, f = subdivide_midpoint(v, f)
@@ -157,15 +162,28 @@ def ico_cap() -> Mesh:
     return _finish(*_merge([(v1, f1), (v2, f2)]))
 
 
-def scene(seed: int = 2, target_faces: int = 280_000, n_closed: int = 48, n_patches: int = 32,
-          sliver_frac: float = 0.01) -> Mesh:
-    """Closed shapes + open patches + slivers, ~target_faces faces."""
-    rng = np.random.default_rng(seed)
-    n_sliver = int(round(sliver_frac * target_faces))
-    budget = target_faces - n_sliver
-    closed_budget = 0.7 * budget
-    patch_budget = budget - closed_budget
+def split_seams(v: np.ndarray, f: np.ndarray, chart_edges: float) -> Mesh:
+    """Texture seams: faces are grouped into charts (cells of a lattice of
+    ``chart_edges`` mean edge lengths, by face centroid) and every chart gets
+    its own copies of its vertices, as a mesh exported with a fragmented UV
+    atlas has.  Edges along chart borders become boundary edges (their two
+    sides no longer share vertex indices).  ``chart_edges`` <= 0 keeps the
+    mesh welded."""
+    if chart_edges <= 0 or f.shape[0] == 0:
+        return v, f
+    a, b, c = v[f[:, 0]], v[f[:, 1]], v[f[:, 2]]
+    d = b - a
+    el = float(np.sqrt((d * d).sum(axis=1)).mean())
+    cent = (a + b + c) / 3.0
+    cell = np.floor((cent - cent.min(axis=0)) / (chart_edges * max(el, 1e-12))).astype(np.int64)
+    chart = cell[:, 0] + 4096 * (cell[:, 1] + 4096 * cell[:, 2])
+    key = chart[:, None] * int(v.shape[0]) + f
+    uniq, inv = np.unique(key.ravel(), return_inverse=True)
+    return v[uniq % v.shape[0]], inv.reshape(f.shape).astype(np.int64)
 
+
+def _scene_parts(rng: np.random.Generator, closed_budget: float, patch_budget: float, n_closed: int,
+                 n_patches: int, chart_edges: float) -> List[Mesh]:
     radii = rng.uniform(0.02, 0.2, size=n_closed)
     kinds = rng.integers(0, 2, size=n_closed)  # 0 sphere, 1 torus
     areas = np.where(kinds == 0, 4 * np.pi * radii ** 2, 4 * np.pi ** 2 * radii * (0.35 * radii))
@@ -183,6 +201,7 @@ def scene(seed: int = 2, target_faces: int = 280_000, n_closed: int = 48, n_patc
             nu = max(int(np.sqrt(n_tot * r / rr)), 6)
             nv = max(int(n_tot / nu), 4)
             v, f = torus(r, rr, nu, nv)
+        v, f = split_seams(v, f, chart_edges)
         scale = rng.uniform(0.7, 1.3, size=3)
         R = _rotation(rng)
         v = (v * scale[None, :]) @ R.T
@@ -197,6 +216,7 @@ def scene(seed: int = 2, target_faces: int = 280_000, n_closed: int = 48, n_patc
         n = max(int(np.sqrt(per_patch / 2.0)), 2)
         curv = (0.0, 0.0, 0.0) if k % 2 == 0 else tuple(rng.uniform(-0.5, 0.5, size=3))
         v, f = grid_patch(n, curv)
+        v, f = split_seams(v, f, chart_edges)
         half = rng.uniform(0.05, 0.25)
         v = v * half
         R = _rotation(rng)
@@ -206,6 +226,29 @@ def scene(seed: int = 2, target_faces: int = 280_000, n_closed: int = 48, n_patc
         hi = 0.5 - ext - 1e-3
         c = rng.uniform(np.minimum(lo, 0.0), np.maximum(hi, 0.0))
         parts.append((v + c[None, :], f))
+    return parts
+
+
+def scene(seed: int = 2, target_faces: int = 280_000, n_closed: int = 48, n_patches: int = 32,
+          sliver_frac: float = 0.01, chart_edges: float = 3.0) -> Mesh:
+    """Closed shapes + open patches + slivers, ~target_faces faces.
+
+    Shapes are tessellated to a face budget; the budget is rescaled once by
+    the face count of a first draw (same seed), so the result lands within a
+    few percent of ``target_faces``.  Texture seams (``split_seams``, charts of
+    ``chart_edges`` mean edge lengths) give about 0.95 vertices per face, as
+    in a textured mesh export; 0 keeps every shape welded (about 0.5)."""
+    n_sliver = int(round(sliver_frac * target_faces))
+    budget = float(target_faces - n_sliver)
+    # face counts do not depend on the seams (the split draws no random numbers)
+    for _ in range(2):
+        rng = np.random.default_rng(seed)
+        got = sum(int(p[1].shape[0]) for p in _scene_parts(rng, 0.7 * budget, 0.3 * budget, n_closed, n_patches, 0.0))
+        if got == 0 or abs(got - (target_faces - n_sliver)) <= 0.005 * target_faces:
+            break
+        budget *= (target_faces - n_sliver) / got
+    rng = np.random.default_rng(seed)
+    parts = _scene_parts(rng, 0.7 * budget, 0.3 * budget, n_closed, n_patches, chart_edges)
 
     # slivers: one edge ~1e-6 of the other two
     base = rng.uniform(-0.45, 0.45, size=(n_sliver, 3))

@@ -508,6 +508,24 @@ def _hash_input(t: torch.Tensor, name: str, dtype: torch.dtype, msg: str, dev: t
     return t.contiguous()
 
 
+def _check_rows(keys, values, keys_name: str, cols: int = 0):
+    """Shape checks made before any device work: 1-D keys (or [M, cols]
+    coordinates) and at least one value per key row, so that the kernels
+    never read past the end of a short values tensor."""
+    if not isinstance(keys, torch.Tensor):
+        raise RuntimeError(f"{keys_name} must be a torch.Tensor")
+    if cols:
+        if keys.dim() != 2 or keys.size(1) != cols:
+            raise RuntimeError(f"{keys_name} must have shape [M, {cols}], got {tuple(keys.shape)}")
+    elif keys.dim() != 1:
+        raise RuntimeError(f"{keys_name} must be a 1-D tensor, got shape {tuple(keys.shape)}")
+    if values is not None:
+        if not isinstance(values, torch.Tensor):
+            raise RuntimeError("values must be a torch.Tensor")
+        if values.numel() < keys.size(0):
+            raise RuntimeError(f"values has {values.numel()} entries for {keys.size(0)} keys")
+
+
 def _type_name(nbytes: int) -> str:
     return "uint32" if nbytes == 4 else "uint64"
 
@@ -515,11 +533,12 @@ def _type_name(nbytes: int) -> str:
 def hashmap_insert_cuda(hashmap_keys: torch.Tensor, hashmap_values: torch.Tensor, keys: torch.Tensor,
                         values: torch.Tensor) -> None:
     """Insert keys[M] with values[M] (reference src/hash/hash.cu hashmap_insert_cuda)."""
+    _check_rows(keys, values, "keys")
     kb, vb = _hash_table(hashmap_keys, hashmap_values)
     dev = hashmap_keys.device
     keys = _hash_input(keys, "keys", hashmap_keys.dtype, f"Keys must be {_type_name(kb)}", dev)
     values = _hash_input(values, "values", hashmap_values.dtype, f"Values must be {_type_name(vb)}", dev)
-    M = keys.size(0) if keys.dim() else 0
+    M = keys.size(0)
     with torch.cuda.device(dev):
         _check(_load().ovx_hashmap_insert(hashmap_keys.data_ptr(), kb, hashmap_values.data_ptr(), vb,
                                           hashmap_keys.numel(), keys.data_ptr(), values.data_ptr(), M,
@@ -528,10 +547,11 @@ def hashmap_insert_cuda(hashmap_keys: torch.Tensor, hashmap_values: torch.Tensor
 
 def hashmap_lookup_cuda(hashmap_keys: torch.Tensor, hashmap_values: torch.Tensor, keys: torch.Tensor) -> torch.Tensor:
     """Values of keys[M], all ones where missing (reference hashmap_lookup_cuda)."""
+    _check_rows(keys, None, "keys")
     kb, vb = _hash_table(hashmap_keys, hashmap_values)
     dev = hashmap_keys.device
     keys = _hash_input(keys, "keys", hashmap_keys.dtype, f"Keys must be {_type_name(kb)}", dev)
-    M = keys.size(0) if keys.dim() else 0
+    M = keys.size(0)
     out = torch.empty((M,), dtype=hashmap_values.dtype, device=dev)
     with torch.cuda.device(dev):
         _check(_load().ovx_hashmap_lookup(hashmap_keys.data_ptr(), kb, hashmap_values.data_ptr(), vb,
@@ -544,6 +564,7 @@ def hashmap_insert_3d_cuda(hashmap_keys: torch.Tensor, hashmap_values: torch.Ten
                            values: torch.Tensor, W: int, H: int, D: int) -> None:
     """Insert int32 coords[M,4] = (b, x, y, z) under key b*W*H*D + x*H*D + y*D + z
     (reference hashmap_insert_3d_cuda)."""
+    _check_rows(coords, values, "coords", cols=4)
     kb, vb = _hash_table(hashmap_keys, hashmap_values)
     dev = hashmap_keys.device
     coords = _hash_input(coords, "coords", torch.int32, "Coords must be int32", dev, cols=4)
@@ -559,6 +580,7 @@ def hashmap_lookup_3d_cuda(hashmap_keys: torch.Tensor, hashmap_values: torch.Ten
                            H: int, D: int) -> torch.Tensor:
     """Values of int32 coords[M,4]; all ones where missing or where (x, y, z)
     lies outside [0,W) x [0,H) x [0,D) (reference hashmap_lookup_3d_cuda)."""
+    _check_rows(coords, None, "coords", cols=4)
     kb, vb = _hash_table(hashmap_keys, hashmap_values)
     dev = hashmap_keys.device
     coords = _hash_input(coords, "coords", torch.int32, "Coords must be int32", dev, cols=4)
@@ -575,6 +597,7 @@ def hashmap_insert_3d_idx_as_val_cuda(hashmap_keys: torch.Tensor, hashmap_values
                                       W: int, H: int, D: int) -> None:
     """Insert int32 coords[M,4] with each row's index as its value
     (reference hashmap_insert_3d_idx_as_val_cuda)."""
+    _check_rows(coords, None, "coords", cols=4)
     kb, vb = _hash_table(hashmap_keys, hashmap_values)
     dev = hashmap_keys.device
     coords = _hash_input(coords, "coords", torch.int32, "Coords must be int32", dev, cols=4)

@@ -316,7 +316,12 @@ def test_quadrics_psd(oracle):
     r = oracle.flexible_dual_grid((v - a0).astype(np.float32), f, vs, gr, 1.0, 0.5, 0.0, fp64=True, keep_qef=True)
     ev = np.linalg.eigvalsh(r.qef)
     assert (ev.min(axis=1) > -1e-9 * np.abs(ev).max(axis=1)).all()
-    assert np.allclose(r.mean / r.cnt[:, None] - r.mean / r.cnt[:, None], 0)
+    # every intersection point of a voxel lies on one of its edges (a grid line
+    # crossing inside the voxel's z range), so their mean lies in the closed voxel box
+    m = r.mean / r.cnt[:, None]
+    lo = r.coords.astype(np.float64) * np.float64(np.float32(vs[0]))
+    assert (m >= lo - 1e-12).all() and (m <= lo + np.float64(np.float32(vs[0])) + 1e-12).all()
+    assert (r.cnt >= 1).all() and (r.cnt == np.round(r.cnt)).all()
 
 
 # ---------------------------------------------------------------------------
