@@ -9,17 +9,22 @@ including the wrapper's dtype cast and aabb shift.  Prints ONE JSON line.
   python bench.py [--steps K] [--warmup W]
                   [--workload scene|scene_large|ico_cap|scene_rankdef|batch|to_mesh|serialize|vxz] [--grid G]
 
-Default workload: the ~280k-face seeded scene (closed icospheres/tori, open
-patches, 1% slivers) at grid 512, weights face 1.0 / boundary 0.2 /
-regularization 1e-2.
+Default workload: the 10.3M-face seeded scene (the ~280k-face scene generator,
+two 1-to-4 subdivisions with vertex jitter, trimmed to 10.3M faces) at grid
+512, weights face 1.0 / boundary 0.2 / regularization 1e-2; headline in the
+reference float path, the compensated precision timed beside it.
 
 Reported next to the throughput:
-  roofline      the dominant kernel's algorithmic bytes per launch over its
-                average launch time (HIP events on the conversion's stream,
-                recorded around every launch of the timed steps) vs 8 TB/s
+  roofline      the conversion's algorithmic bytes (12V + 12F + 27N) over the
+                dominant kernel's average launch time (HIP events on the
+                conversion's stream around every launch) vs 8 TB/s, beside the
+                kernel's own byte model and the whole-step figure
   cpu_baseline  the CPU oracle (float path, single thread) on the same
-                workload (a bounded face sample for the 10M-face mesh)
+                workload (a seeded 300k-face sample of meshes above 1M faces)
   parity        occupancy / flag mismatches and dual error against the oracle
+                float path (on the same sample)
+  precision     the other precision's time, and the compensated path's dual
+                error against the FP64 oracle (on the same sample)
 """
 
 import argparse
@@ -151,7 +156,7 @@ def cpu_baseline(v, f, grid, workload, max_faces=None):
                     break
     except OSError:
         pass
-    return ref, dt, len(fi), sample, cpu, vs, gr
+    return ref, dt, len(fi), sample, cpu, vs, gr, fi
 
 
 def main():
@@ -159,12 +164,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="scene")
+    ap.add_argument("--workload", default="scene_large")
     ap.add_argument("--grid", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-other-precision", action="store_true",
+                    help="skip timing the precision not selected by --precision")
     ap.add_argument("--precision", default="reference", choices=["reference", "compensated"],
-                    help="arithmetic of the QEF accumulation and solve (compensated: voxel-frame compensated FP32, "
-                         "FP64 fallback; its dual error is reported against the FP64 oracle)")
+                    help="arithmetic of the headline steps (compensated: voxel-frame compensated FP32, FP64 "
+                         "fallback); the other one is timed too and reported in the `precision` block with its "
+                         "dual error against the FP64 oracle")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, copy a 256 MiB buffer 5 times (a kernel of known traffic "
                          "for scaling hardware byte counters)")
@@ -191,65 +199,113 @@ def main():
         return run_to_mesh(args, torch, o_voxel, _C, dev)
     if args.workload in ("serialize", "vxz"):
         return run_voxel_io(args, torch, o_voxel, _C, dev)
+    return run_conversion(args, torch, o_voxel, _C, dev)
+
+
+def _calibrate_copy(torch, dev):
+    """A 256 MiB device copy, 5 times: a kernel of known traffic for scaling hardware byte counters."""
+    src = torch.ones(64 * 2 ** 20, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
+def _dual_error(d, ref, vs):
+    """(rms, max) of the per-voxel Euclidean dual difference, in voxels."""
+    if len(d) == 0:
+        return 0.0, 0.0
+    e = (np.asarray(d, np.float64) - np.asarray(ref, np.float64)) / np.asarray(vs, np.float64)[None, :]
+    n = np.sqrt((e ** 2).sum(1))
+    return float(np.sqrt((n ** 2).mean())), float(n.max())
+
+
+def run_conversion(args, torch, o_voxel, _C, dev):
+    """One step = one public-API conversion of the workload's mesh (inputs in
+    HBM).  The headline precision is --precision (default: the reference
+    float path, the drop-in); the other precision is timed on the same mesh
+    and reported in the line's `precision` block, with dual errors against the
+    FP64 oracle on a seeded face subset (the whole mesh when it has at most
+    300k faces)."""
     v, f, grid, seed = make_workload(args.workload, args.grid)
     F, V = int(f.shape[0]), int(v.shape[0])
     log(f"[bench] workload={args.workload} F={F} V={V} grid={grid}")
     vt = torch.from_numpy(v).to(dev)
     ft = torch.from_numpy(f).to(dev)
     aabb = [[-0.5, -0.5, -0.5], [0.5, 0.5, 0.5]]
+    other = "compensated" if args.precision == "reference" else "reference"
 
-    def step():
-        return o_voxel.convert.mesh_to_flexible_dual_grid(vt, ft, grid_size=grid, aabb=aabb, face_weight=WEIGHTS[0],
+    def step(precision, fts=ft):
+        return o_voxel.convert.mesh_to_flexible_dual_grid(vt, fts, grid_size=grid, aabb=aabb, face_weight=WEIGHTS[0],
                                                           boundary_weight=WEIGHTS[1],
-                                                          regularization_weight=WEIGHTS[2], precision=args.precision)
+                                                          regularization_weight=WEIGHTS[2], precision=precision)
+
+    def timed(precision):
+        """K timed steps back to back: ms per step over the whole region."""
+        torch.cuda.synchronize()
+        t_start = time.perf_counter()
+        for _ in range(args.steps):
+            out = step(precision)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t_start) * 1e3 / args.steps, out
+
+    def latencies(precision, timing):
+        """K more steps, each waited for: per-step latency (ms) and, with
+        timing, per-kernel device times (HIP events around every launch on the
+        conversion's stream)."""
+        lat, per_kernel = [], {}
+        if timing:
+            _C.set_timing(dev, True)
+        for _ in range(args.steps):
+            ts = time.perf_counter()
+            step(precision)
+            torch.cuda.synchronize()
+            lat.append((time.perf_counter() - ts) * 1e3)
+            if timing:
+                for k, ms in _C.kernel_times(dev).items():
+                    per_kernel.setdefault(k, []).append(ms)
+        if timing:
+            _C.set_timing(dev, False)
+        return lat, per_kernel
 
     # cold call (first in the process)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    out = step()
+    out = step(args.precision)
     torch.cuda.synchronize()
     cold_ms = (time.perf_counter() - t0) * 1e3
     for _ in range(args.warmup):
-        out = step()
-    torch.cuda.synchronize()
-    _C.set_timing(dev, True)
+        out = step(args.precision)
     torch.cuda.reset_peak_memory_stats(dev)
-
-    # timed region: K steps; per-kernel HIP event times collected after each step
-    per_kernel = {}
-    step_ms = []
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        ts = time.perf_counter()
-        out = step()
-        kt = _C.kernel_times(dev)  # waits for this step's last kernel
-        step_ms.append((time.perf_counter() - ts) * 1e3)
-        for k, ms in kt.items():
-            per_kernel.setdefault(k, []).append(ms)
-    torch.cuda.synchronize()
-    t_end = time.perf_counter()
-    if args.calibrate_copy:
-        src = torch.ones(64 * 2 ** 20, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(5):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
-    ms_per_step = (t_end - t_start) * 1e3 / args.steps
-    value = F / (ms_per_step * 1e-3) / 1e6
+    # timed region: K steps of the headline precision
+    ms_per_step, out = timed(args.precision)
     st = _C.last_stats(dev)
     peak_mb = torch.cuda.max_memory_allocated(dev) / 2 ** 20 + st["workspace_bytes"] / 2 ** 20
+    step_ms, per_kernel = latencies(args.precision, True)
+    if args.calibrate_copy:
+        _calibrate_copy(torch, dev)
+    # the other precision on the same mesh
+    o_ms = o_lat = None
+    if not args.no_other_precision:
+        for _ in range(max(1, args.warmup)):
+            step(other)
+        o_ms, _ = timed(other)
+        o_st = _C.last_stats(dev)
+        o_lat, _ = latencies(other, False)
     coords, dual, inter = (t.cpu().numpy() for t in out)
+    value = F / (ms_per_step * 1e-3) / 1e6
 
     G = grid
     n_words = G * G * ((G + 63) // 64)
     kern_avg = {k: float(np.mean(x)) for k, x in per_kernel.items()}
     dom = max(kern_avg, key=kern_avg.get)
-    alg = algorithmic_bytes(dom, st, F, V, n_words)
-    achieved = alg / (kern_avg[dom] * 1e-3) / 1e9 if alg else None
+    # canonical algorithmic bytes of one conversion: 12 B per vertex and per
+    # face read, 27 B per voxel written (coords 12, dual 12, flags 3)
+    b_alg = 12 * V + 12 * F + 27 * st["voxels"]
+    model = algorithmic_bytes(dom, st, F, V, n_words)
+    achieved = b_alg / (kern_avg[dom] * 1e-3) / 1e9
     traffic = None
-    # the committed traffic was measured on the reference-path kernels
     tpath = os.path.join(ROOT, "profiles", f"{dom}_{args.workload}_traffic.json")
     if os.path.exists(tpath) and args.precision == "reference":
         with open(tpath) as fh:
@@ -257,65 +313,73 @@ def main():
     roofline = {
         "kernel": dom,
         "bound": "hbm",
-        "achieved": round(achieved, 3) if achieved else None,
+        "achieved": round(achieved, 3),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 6) if achieved else None,
+        "frac": round(achieved / HBM_PEAK_GBS, 6),
         "traffic": traffic,
-        "alg_bytes": int(alg) if alg else None,
+        "alg_bytes": int(b_alg),
+        "alg_bytes_basis": "12V + 12F + 27N of the conversion, all charged to the dominant kernel",
         "avg_ms": round(kern_avg[dom], 4),
+        "kernel_model": {"alg_bytes": int(model) if model else None,
+                         "achieved": round(model / (kern_avg[dom] * 1e-3) / 1e9, 3) if model else None,
+                         "frac": round(model / (kern_avg[dom] * 1e-3) / 1e9 / HBM_PEAK_GBS, 6) if model else None,
+                         "basis": "the kernel's own compulsory reads and writes (records, planes, quadrics)"},
+        "path": {"achieved": round(b_alg / (ms_per_step * 1e-3) / 1e9, 3),
+                 "frac": round(b_alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 6),
+                 "basis": "12V + 12F + 27N over the whole step"},
     }
-    # whole-path algorithmic bytes 12V + 12F + 27N over the step time
-    path_alg = 12 * V + 12 * F + 27 * st["voxels"]
 
-    cpu = None
-    parity = None
+    # CPU baseline, parity and the precision block on one face sample
+    cpu = parity = None
+    max_faces = 300_000 if F > 1_000_000 else None
+    precision = {"mode": other if o_ms is not None else None}
     if not args.no_cpu_baseline:
-        max_faces = 300_000 if F > 1_000_000 else None
-        ref, dt, nf, sample, cpu_model, vs, gr = cpu_baseline(v, f, grid, args.workload, max_faces)
+        from oracle import oracle as O
+
+        ref, dt, nf, sample, cpu_model, vs, gr, fi = cpu_baseline(v, f, grid, args.workload, max_faces)
         cpu = {"value": round(nf / dt / 1e6, 4), "unit": "M faces/s", "cores": 1, "kind": "port",
                "sample": sample, "seconds": round(dt, 3), "cpu": cpu_model, "nproc": os.cpu_count()}
         if nf == F:
-            from oracle import oracle as O
-
-            perm = O.canonical_order(ref.coords, gr)
-            occ_ok = coords.shape == ref.coords.shape and bool(np.array_equal(coords, ref.coords[perm]))
-            parity = {"occupancy_match": occ_ok}
+            s_out = {args.precision: (coords, dual, inter)}
+        else:  # the same face sample converted on the GPU
+            fs = torch.from_numpy(fi.astype(np.int64)).to(dev)
+            s_out = {args.precision: tuple(t.cpu().numpy() for t in step(args.precision, fs))}
+        if o_ms is not None:
+            fs = ft if nf == F else torch.from_numpy(fi.astype(np.int64)).to(dev)
+            s_out[other] = tuple(t.cpu().numpy() for t in step(other, fs))
+        perm = O.canonical_order(ref.coords, gr)
+        c_r, d_r, i_r = s_out["reference"] if "reference" in s_out else (None, None, None)
+        on = "full mesh" if nf == F else f"{nf}-face seeded sample"
+        if c_r is not None:
+            occ_ok = c_r.shape == ref.coords.shape and bool(np.array_equal(c_r, ref.coords[perm]))
+            parity = {"on": on, "occupancy_match": occ_ok, "vs": "oracle float path"}
             if occ_ok:
-                d = (dual.astype(np.float64) - ref.dual[perm].astype(np.float64)) / np.float64(vs[0])
+                rms, mx = _dual_error(d_r, ref.dual[perm], vs)
                 parity.update({
-                    "intersected_mismatch": int((inter != ref.flags[perm]).sum()),
-                    "dual_rms_voxel": float(np.sqrt((d ** 2).sum(1).mean())) if len(d) else 0.0,
-                    "dual_max_voxel": float(np.abs(d).max()) if len(d) else 0.0,
-                    "dual_bit_identical": float(np.mean(np.all(dual == ref.dual[perm], axis=1))) if len(d) else 1.0,
-                    "vs": "oracle float path",
+                    "intersected_mismatch": int((i_r != ref.flags[perm]).sum()),
+                    "dual_rms_voxel": rms, "dual_max_voxel": mx,
+                    "dual_bit_identical": float(np.mean(np.all(d_r == ref.dual[perm], axis=1))) if len(d_r) else 1.0,
                 })
-
-    precision = None
-    if args.precision == "compensated":
-        precision = {"mode": "compensated", "fp64_fraction": st["fp64_voxels"] / max(1, st["voxels"])}
-        if not args.no_cpu_baseline and F <= 1_000_000:
+        if "compensated" in s_out:
             from oracle import oracle as O
 
-            vs, gr, a0 = meshgen_grid(grid)
-            r64 = O.flexible_dual_grid((v - a0).astype(np.float32), f.astype(np.int32), vs, gr, *WEIGHTS, fp64=True)
-            perm = O.canonical_order(r64.coords, gr)
-            ref_plain = o_voxel.convert.mesh_to_flexible_dual_grid(vt, ft, grid_size=grid, aabb=aabb,
-                                                                   face_weight=WEIGHTS[0], boundary_weight=WEIGHTS[1],
-                                                                   regularization_weight=WEIGHTS[2])[1].cpu().numpy()
-
-            def err(d):
-                e = (d.astype(np.float64) - r64.dual64[perm]) / np.asarray(vs, np.float64)[None, :]
-                n = np.sqrt((e ** 2).sum(1))
-                return float(np.sqrt((n ** 2).mean())), float(n.max())
-
-            occ_ok = coords.shape == r64.coords.shape and bool(np.array_equal(coords, r64.coords[perm]))
-            rms, mx = err(dual)
-            rms_p, mx_p = err(ref_plain)
-            precision.update({"vs": "FP64 oracle", "occupancy_match": occ_ok,
-                              "intersected_mismatch": int((inter != r64.flags[perm]).sum()) if occ_ok else None,
-                              "dual_rms_voxel": rms, "dual_max_voxel": mx,
-                              "reference_path_dual_rms_voxel": rms_p, "reference_path_dual_max_voxel": mx_p})
+            r64 = O.flexible_dual_grid((v - meshgen_grid(grid)[2]).astype(np.float32), fi, vs, gr, *WEIGHTS,
+                                       fp64=True)
+            p64 = O.canonical_order(r64.coords, gr)
+            c_c, d_c, i_c = s_out["compensated"]
+            occ_ok = c_c.shape == r64.coords.shape and bool(np.array_equal(c_c, r64.coords[p64]))
+            rms, mx = _dual_error(d_c, r64.dual64[p64], vs) if occ_ok else (None, None)
+            precision.update({"vs": "FP64 oracle", "on": on, "occupancy_match": occ_ok,
+                              "intersected_mismatch": int((i_c != r64.flags[p64]).sum()) if occ_ok else None,
+                              "dual_rms_voxel": rms, "dual_max_voxel": mx})
+            if c_r is not None and occ_ok:
+                rms_p, mx_p = _dual_error(d_r, r64.dual64[p64], vs)
+                precision.update({"reference_path_dual_rms_voxel": rms_p, "reference_path_dual_max_voxel": mx_p})
+    if o_ms is not None:
+        precision.update({"ms_per_step": round(o_ms, 4), "M_faces_per_s": round(F / (o_ms * 1e-3) / 1e6, 4),
+                          "p50_ms": round(float(np.percentile(o_lat, 50)), 4),
+                          "fp64_fraction": o_st["fp64_voxels"] / max(1, o_st["voxels"])})
 
     line = {
         "metric": METRIC,
@@ -331,22 +395,19 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"{args.workload}-{F}f@{grid}", "faces": F, "vertices": V, "grid": grid,
-                   "voxels": st["voxels"], "records": st["records"], "seed": seed,
+                   "voxels": st["voxels"], "records": st["records"], "seed": seed, "precision": args.precision,
                    "weights": {"face": WEIGHTS[0], "boundary": WEIGHTS[1], "regularization": WEIGHTS[2]}},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "parity": parity,
+        "precision": precision,
         "cold_ms": round(cold_ms, 3),
         "p50_ms": round(float(np.percentile(step_ms, 50)), 4),
         "p99_ms": round(float(np.percentile(step_ms, 99)), 4),
-        "path_alg_GBps": round(path_alg / (ms_per_step * 1e-3) / 1e9, 3),
         "kernels_ms": {k: round(x, 4) for k, x in sorted(kern_avg.items(), key=lambda kv: -kv[1])},
         "peak_mem_mb": round(peak_mb, 1),
         "counts": {k: v for k, v in st.items() if k != "workspace_bytes"},
     }
-    if precision is not None:
-        line["precision"] = precision
-        line["config"]["precision"] = "compensated"
     print(json.dumps(line), flush=True)
 
 
@@ -364,45 +425,64 @@ def run_batch(args, torch, o_voxel, _C, dev):
     grids = [g for _, _, g in meshes]
     kw = dict(face_weight=WEIGHTS[0], boundary_weight=WEIGHTS[1], regularization_weight=WEIGHTS[2])
 
-    def batched():
-        return o_voxel.convert.mesh_to_flexible_dual_grid_batch(ms_in, grid_sizes=grids, aabb=aabb, **kw)
+    def batched(precision):
+        return o_voxel.convert.mesh_to_flexible_dual_grid_batch(ms_in, grid_sizes=grids, aabb=aabb,
+                                                                precision=precision, **kw)
 
-    def sequential():
+    def sequential(precision):
         out, lat = [], []
         for (vt, ft), g in zip(ms_in, grids):
             t0 = time.perf_counter()
-            out.append(o_voxel.convert.mesh_to_flexible_dual_grid(vt, ft, grid_size=g, aabb=aabb, **kw))
+            out.append(o_voxel.convert.mesh_to_flexible_dual_grid(vt, ft, grid_size=g, aabb=aabb,
+                                                                  precision=precision, **kw))
             torch.cuda.synchronize()
             lat.append((time.perf_counter() - t0) * 1e3)
         return out, lat
 
+    def measure(precision):
+        """K batched steps timed back to back, then the same meshes by one
+        plain call each (median over K/4 passes, per-call latency recorded)."""
+        for _ in range(args.warmup):
+            batched(precision)
+            sequential(precision)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            out_b = batched(precision)
+        torch.cuda.synchronize()
+        b_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        s_ms, lat_all = [], []
+        for _ in range(max(1, args.steps // 4)):
+            t0 = time.perf_counter()
+            out_s, lat = sequential(precision)
+            s_ms.append((time.perf_counter() - t0) * 1e3)
+            lat_all += lat
+        s_ms = float(np.median(s_ms))
+        same = all(all(torch.equal(a, b) for a, b in zip(ob, os_)) for ob, os_ in zip(out_b, out_s))
+        return {
+            "batched_ms": round(b_ms, 3),
+            "batched_meshes_per_s": round(len(meshes) / (b_ms * 1e-3), 1),
+            "batched_M_faces_per_s": round(F / (b_ms * 1e-3) / 1e6, 3),
+            "sequential_ms": round(s_ms, 3),
+            "sequential_meshes_per_s": round(len(meshes) / (s_ms * 1e-3), 1),
+            "speedup": round(s_ms / b_ms, 3),
+            "sequential_p50_ms_per_mesh": round(float(np.percentile(lat_all, 50)), 4),
+            "sequential_p99_ms_per_mesh": round(float(np.percentile(lat_all, 99)), 4),
+            "batched_ms_per_mesh": round(b_ms / len(meshes), 4),
+            "outputs_identical_to_sequential": bool(same),
+        }
+
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    batched()
+    batched("reference")
     torch.cuda.synchronize()
     cold_ms = (time.perf_counter() - t0) * 1e3
-    for _ in range(args.warmup):
-        batched()
-        sequential()
-    torch.cuda.synchronize()
-    # timed: K batched steps
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        out_b = batched()
-    torch.cuda.synchronize()
-    batch_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    # the same meshes, one plain call each (per-call latency recorded)
-    seq_ms, lat_all = [], []
-    for _ in range(max(1, args.steps // 4)):
-        t0 = time.perf_counter()
-        out_s, lat = sequential()
-        seq_ms.append((time.perf_counter() - t0) * 1e3)
-        lat_all += lat
-    seq_ms = float(np.median(seq_ms))
-    same = all(all(torch.equal(a, b) for a, b in zip(ob, os_)) for ob, os_ in zip(out_b, out_s))
+    res = measure("reference")
+    batch_ms = res["batched_ms"]
+    res_c = None if args.no_other_precision else measure("compensated")
     # per-kernel device times of one more batched step
     _C.set_timing(dev, True)
-    batched()
+    batched("reference")
     kt = _C.kernel_times(dev)
     _C.set_timing(dev, False)
     st = _C.last_stats(dev)
@@ -422,17 +502,8 @@ def run_batch(args, torch, o_voxel, _C, dev):
         "config": {"workload": f"batch-{len(meshes)}meshes-{F}f", "meshes": len(meshes), "faces": F, "seed": 5,
                    "grids": {str(g): grids.count(g) for g in sorted(set(grids))},
                    "weights": {"face": WEIGHTS[0], "boundary": WEIGHTS[1], "regularization": WEIGHTS[2]}},
-        "batch": {
-            "batched_ms": round(batch_ms, 3),
-            "batched_meshes_per_s": round(len(meshes) / (batch_ms * 1e-3), 1),
-            "sequential_ms": round(seq_ms, 3),
-            "sequential_meshes_per_s": round(len(meshes) / (seq_ms * 1e-3), 1),
-            "speedup": round(seq_ms / batch_ms, 3),
-            "sequential_p50_ms_per_mesh": round(float(np.percentile(lat_all, 50)), 4),
-            "sequential_p99_ms_per_mesh": round(float(np.percentile(lat_all, 99)), 4),
-            "batched_ms_per_mesh": round(batch_ms / len(meshes), 4),
-            "outputs_identical_to_sequential": bool(same),
-        },
+        "batch": res,
+        "batch_compensated": res_c,
         "cold_ms": round(cold_ms, 3),
         "kernels_ms": {k: round(x, 4) for k, x in sorted(kt.items(), key=lambda kv: -kv[1]) if x >= 0},
         "counts": {k: v for k, v in st.items() if k != "workspace_bytes"},
