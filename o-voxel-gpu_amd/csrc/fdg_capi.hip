@@ -85,7 +85,8 @@ static int mark_class(const double t[3], double fpc)
 {
     return fpc < t[0] ? 0 : (fpc <= t[1] ? 1 : (fpc < t[2] ? 2 : 3));
 }
-// face-test class from voxels per face: index into FACE_TILES (32, 64, 128 faces per block)
+// face-test class from voxels per face: index into FACE_TILES (32 or 64 faces
+// per block walked cooperatively; below t[0], one lane per face)
 static int face_class(const double t[2], double vpf)
 {
     return vpf >= t[1] ? 0 : (vpf >= t[0] ? 1 : 2);
@@ -96,7 +97,7 @@ struct ovx_fdg_ctx
     int device = 0;
     std::mutex mu;
     // workspace
-    DevBuf planes, bits, woff, tiles, counters;
+    DevBuf planes, tris, bits, woff, tiles, counters;
     DevBuf dm_table, dm_cnt, dm_off, dm_mask, dm_tiles; // dual grid -> mesh
     DevBuf dm_direct, dm_touched; // direct-mapped voxel table (all zero between calls) and the cells a call set
     int64_t dm_dirty = 0;         // rows in dm_touched whose cells are still set
@@ -147,8 +148,10 @@ struct ovx_fdg_ctx
     int64_t N = 0;
     // statistics of the last conversion
     int64_t stats[OVX_FDG_N_STATS] = {};
-    hipStream_t last_stream = nullptr; // stream of the last conversion (the null stream is valid)
-    bool ran = false;                 // a conversion was started on this context
+    hipStream_t last_stream = nullptr; // stream of the last call (the null stream is valid)
+    bool ran = false;                 // a call has enqueued work on this context
+    hipEvent_t ev_done = nullptr;     // end of the last call's work on last_stream
+    bool done_recorded = false;
     // per-kernel timing (events around each launch on the caller's stream)
     bool timing = false;
     hipEvent_t kev[2 * 24] = {};
@@ -161,7 +164,7 @@ struct ovx_fdg_ctx
     template <class Fn>
     void for_each_buf(Fn &&fn)
     {
-        DevBuf *all[] = {&planes, &bits,   &woff,  &tiles,   &vcnt,   &voff,  &nbr, &erank, &bcnt, &boff, &eslots,
+        DevBuf *all[] = {&planes, &tris, &bits,   &woff,  &tiles,   &vcnt,   &voff,  &nbr, &erank, &bcnt, &boff, &eslots,
                          &bedge_v, &bedges, &events, &recs,   &chunks, &heads, &choff, &bchunks, &active, &large, &huge,
                          &box,     &qef,    &bhits,  &fhits,  &shard_ctr, &rowq, &faceq, &segbuf,
                          &dm_table, &dm_cnt, &dm_off, &dm_mask, &dm_tiles, &dm_direct, &dm_touched,
@@ -174,6 +177,34 @@ struct ovx_fdg_ctx
 
 namespace
 {
+
+// Every call on a context shares its workspace, whatever stream it runs on:
+// a call's first enqueue waits for the end of the previous call's work when
+// the streams differ, and the call's end is recorded for the next one.  Work
+// of one stream is ordered by the stream itself.
+struct StreamScope
+{
+    ovx_fdg_ctx *c;
+    hipStream_t s;
+    bool active = false;
+    StreamScope(ovx_fdg_ctx *c_, hipStream_t s_) : c(c_), s(s_) {}
+    hipError_t enter()
+    {
+        active = true;
+        if (c->done_recorded && s != c->last_stream)
+            return hipStreamWaitEvent(s, c->ev_done, 0);
+        return hipSuccess;
+    }
+    ~StreamScope()
+    {
+        if (!active)
+            return;
+        if (hipEventRecord(c->ev_done, s) == hipSuccess)
+            c->done_recorded = true;
+        c->last_stream = s;
+        c->ran = true;
+    }
+};
 
 ovx_status ensure(DevBuf &b, size_t bytes)
 {
@@ -228,6 +259,9 @@ ovx_status check_grid(const ovx_grid *grid, GridDev &g, uint64_t &n_words)
         if (lo < -(1ll << 29) || hi > (1ll << 29))
             return fail(OVX_ERR_CAPACITY, "grid_range along axis %d is outside [-2^29, 2^29]", a);
         g.vs[a] = vs;
+        int ex = 0;
+        g.ivs[a] = (std::frexp(vs, &ex) == 0.5f && std::isnormal(std::ldexp(1.0f, 1 - ex))) ? std::ldexp(1.0f, 1 - ex)
+                                                                                            : 0.0f;
         g.gmin[a] = (int)lo;
         g.gmax[a] = (int)hi;
         g.G[a] = (int)(hi - lo);
@@ -442,8 +476,8 @@ ovx_status setup_batch(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *f
     const size_t total = gd0 + N_GUIDES * garr;
     if (c->h_stage_bytes < total + arr)
     {
-        if (c->ran)
-            HIP_TRY(hipStreamSynchronize(c->last_stream)); // the staging may still be read by a copy
+        if (c->done_recorded)
+            HIP_TRY(hipEventSynchronize(c->ev_done)); // the staging may still be read by a copy
         if (c->h_stage)
             HIP_TRY(hipHostFree(c->h_stage));
         c->h_stage = nullptr;
@@ -551,16 +585,23 @@ ovx_status launch_face_tiles(ovx_fdg_ctx *c, const MeshDev &m, const ShardBuf<FH
             continue;
         BatchDev b = c->bdev;
         b.ftile_off = c->d_ftile_off[t];
-        HIP_TRY(launch_face(m, b, t, nt, fb, ctr, fq, s));
+        // the cooperative classes queue row chunks for single-mesh conversions
+        // only (a batch has other work to balance); the lane class queues the
+        // rows of its rare large faces in any case
+        ShardBuf<FaceChunk> q = fq;
+        if (c->M > 1 && t != FACE_LANE_CLASS)
+            q.items = nullptr;
+        HIP_TRY(launch_face(m, b, t, nt, fb, ctr, q, s));
     }
     return OVX_OK;
 }
 
 // common begin of a batch: validation, (planes), mark, compaction count;
 // n_voxels[0..M) receives each mesh's voxel count
-ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *face_off, const int64_t *vert_off,
+ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m_in, int M, const int64_t *face_off, const int64_t *vert_off,
                         const ovx_grid *grids, bool planes, hipStream_t s, int64_t *n_voxels)
 {
+    MeshDev m = m_in; // gains the gathered triangles once the prep kernel is queued
     c->pending = 0;
     c->N = 0;
     for (int i = 0; i < OVX_FDG_N_STATS; ++i)
@@ -596,6 +637,15 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *
         return st;
     if (planes && (st = ensure(c->planes, (size_t)m.F * sizeof(float4))))
         return st;
+    // indexed meshes: the prep kernel gathers each face's vertices once, later
+    // stages read them in face order (a soup is in that layout already)
+    float *tris = nullptr;
+    if (planes && m.f)
+    {
+        if ((st = ensure(c->tris, (size_t)m.F * 9 * sizeof(float))))
+            return st;
+        tris = (float *)c->tris.p;
+    }
     auto *ctr = (unsigned long long *)c->counters.p;
 
     for (bool &r : c->kran)
@@ -628,8 +678,11 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *
     }
     tic(c, KS_PREP, s);
     HIP_TRY(hipMemsetAsync(c->bits.p, 0, n_words * sizeof(uint64_t), s));
-    HIP_TRY(launch_prep(m, c->bdev, planes ? (float4 *)c->planes.p : nullptr, ctr, s));
+    HIP_TRY(launch_prep(m, c->bdev, planes ? (float4 *)c->planes.p : nullptr, tris, ctr, s));
     toc(c, KS_PREP, s);
+    m.tri = tris;
+    m.pl = planes ? (const float4 *)c->planes.p : nullptr;
+    c->mesh = m;
     // crossings are recorded once, here, when the buffer can hold them
     if ((st = ensure(c->shard_ctr, (size_t)SB_N * NSHARD * SHARD_STRIDE * sizeof(unsigned long long))))
         return st;
@@ -821,8 +874,6 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
         {
             tic(c, KS_FACE, s);
             ShardBuf<FaceChunk> fq = shard_view<FaceChunk>(c, c->faceq, SB_FACEQ);
-            if (c->M > 1)
-                fq.items = nullptr; // row chunks are for single-mesh conversions
             if ((st = launch_face_tiles(c, m, fb, ctr, fq, s)))
                 return st;
             toc(c, KS_FACE, s);
@@ -964,10 +1015,14 @@ ovx_status ovx_fdg_create(int device, ovx_fdg_ctx **out)
     auto *c = new ovx_fdg_ctx();
     c->device = device;
     hipError_t e = hipHostMalloc((void **)&c->h_ctr, CTR_COUNT * sizeof(unsigned long long), hipHostMallocDefault);
+    if (e == hipSuccess)
+        e = hipEventCreateWithFlags(&c->ev_done, hipEventDisableTiming);
     if (e != hipSuccess)
     {
+        if (c->h_ctr)
+            (void)hipHostFree(c->h_ctr);
         delete c;
-        return fail(OVX_ERR_HIP, "hipHostMalloc failed: %s", hipGetErrorString(e));
+        return fail(OVX_ERR_HIP, "context set-up failed: %s", hipGetErrorString(e));
     }
     *out = c;
     return OVX_OK;
@@ -980,10 +1035,12 @@ ovx_status ovx_fdg_destroy(ovx_fdg_ctx *c)
     {
         DeviceGuard dg(c->device);
         std::lock_guard<std::mutex> lk(c->mu);
-        if (c->ran)
-            (void)hipStreamSynchronize(c->last_stream);
+        if (c->done_recorded)
+            (void)hipEventSynchronize(c->ev_done);
         c->for_each_buf([](DevBuf &b) { release(b); });
         release(c->counters);
+        if (c->ev_done)
+            (void)hipEventDestroy(c->ev_done);
         if (c->h_ctr)
             (void)hipHostFree(c->h_ctr);
         if (c->h_stage)
@@ -1011,8 +1068,8 @@ ovx_status ovx_fdg_trim(ovx_fdg_ctx *c)
         return fail(OVX_ERR_INVALID_ARGUMENT, "ctx is null");
     DeviceGuard dg(c->device);
     std::lock_guard<std::mutex> lk(c->mu);
-    if (c->ran)
-        HIP_TRY(hipStreamSynchronize(c->last_stream));
+    if (c->done_recorded)
+        HIP_TRY(hipEventSynchronize(c->ev_done));
     c->for_each_buf([](DevBuf &b) { release(b); });
     for (int i = 0; i < SB_N; ++i)
         c->per[i] = 0;
@@ -1036,6 +1093,8 @@ ovx_status ovx_fdg_begin(ovx_fdg_ctx *c, const float *d_vertices, int64_t n_vert
     c->bw = boundary_weight;
     c->rw = regularization_weight;
     const int64_t fo[2] = {0, n_faces}, vo[2] = {0, n_vertices};
+    StreamScope ss(c, (hipStream_t)stream);
+    HIP_TRY(ss.enter());
     return begin_common(c, m, 1, fo, vo, grid, true, (hipStream_t)stream, n_voxels);
 }
 
@@ -1054,6 +1113,8 @@ ovx_status ovx_fdg_batch_begin(ovx_fdg_ctx *c, const float *d_vertices, int64_t 
     c->fw = face_weight;
     c->bw = boundary_weight;
     c->rw = regularization_weight;
+    StreamScope ss(c, (hipStream_t)stream);
+    HIP_TRY(ss.enter());
     return begin_common(c, m, n_meshes, face_offsets, vertex_offsets, grids, true, (hipStream_t)stream, n_voxels);
 }
 
@@ -1070,8 +1131,8 @@ ovx_status ovx_fdg_finish(ovx_fdg_ctx *c, int32_t *d_coords, float *d_dual, uint
         return OVX_OK;
     if (!d_coords || !d_dual || !d_intersected)
         return fail(OVX_ERR_INVALID_ARGUMENT, "output pointer is null");
-    c->last_stream = (hipStream_t)stream;
-    c->ran = true;
+    StreamScope ss(c, (hipStream_t)stream);
+    HIP_TRY(ss.enter());
     return finish_fdg(c, d_coords, d_dual, d_intersected, (hipStream_t)stream);
 }
 
@@ -1084,6 +1145,8 @@ ovx_status ovx_occ_begin(ovx_fdg_ctx *c, const float *d_triangles, int64_t n_tri
     std::lock_guard<std::mutex> lk(c->mu);
     MeshDev m{d_triangles, nullptr, n_triangles, 3 * n_triangles};
     const int64_t fo[2] = {0, n_triangles}, vo[2] = {0, 3 * n_triangles};
+    StreamScope ss(c, (hipStream_t)stream);
+    HIP_TRY(ss.enter());
     return begin_common(c, m, 1, fo, vo, grid, false, (hipStream_t)stream, n_voxels);
 }
 
@@ -1100,6 +1163,8 @@ ovx_status ovx_occ_finish(ovx_fdg_ctx *c, int32_t *d_coords, void *stream)
         return OVX_OK;
     if (!d_coords)
         return fail(OVX_ERR_INVALID_ARGUMENT, "output pointer is null");
+    StreamScope ss(c, (hipStream_t)stream);
+    HIP_TRY(ss.enter());
     HIP_TRY(launch_emit_coords(c->bdev, c->n_words, d_coords, (hipStream_t)stream));
     return OVX_OK;
 }
@@ -1282,6 +1347,8 @@ ovx_status ovx_dual_mesh_begin(ovx_fdg_ctx *c, const int32_t *d_coords, const fl
     std::lock_guard<std::mutex> lk(c->mu);
     hipStream_t s = (hipStream_t)stream;
     c->pending = 0;
+    StreamScope ss(c, s);
+    HIP_TRY(ss.enter());
     DualMeshArgs &a = c->dm;
     a = DualMeshArgs{};
     a.coords = d_coords;
@@ -1378,6 +1445,8 @@ ovx_status ovx_dual_mesh_finish(ovx_fdg_ctx *c, const float *d_split_weight, int
         return fail(OVX_ERR_STATE, "ovx_dual_mesh_finish without a successful ovx_dual_mesh_begin");
     c->pending = 0;
     hipStream_t s = (hipStream_t)stream;
+    StreamScope ss(c, s);
+    HIP_TRY(ss.enter());
     DualMeshArgs a = c->dm;
     if (a.N > 0 && c->h_ctr[CTR_DM_QUADS] > 0 && (d_faces || d_quads))
     {
@@ -1537,6 +1606,8 @@ ovx_status ovx_svo_encode_begin(ovx_fdg_ctx *c, const int32_t *d_codes, const in
     std::lock_guard<std::mutex> lk(c->mu);
     hipStream_t s = (hipStream_t)stream;
     c->svo_state = 0;
+    StreamScope ss(c, s);
+    HIP_TRY(ss.enter());
     c->svo = SvoCodes{d_codes, d_chunk, d_chunk_off, n, n_chunks, depth};
     *n_bytes = 0;
     c->svo_total = 0;
@@ -1576,6 +1647,8 @@ ovx_status ovx_svo_encode_finish(ovx_fdg_ctx *c, uint8_t *d_svo, int64_t *d_chun
     if (c->svo_state != 1)
         return fail(OVX_ERR_STATE, "ovx_svo_encode_finish without a successful ovx_svo_encode_begin");
     hipStream_t s = (hipStream_t)stream;
+    StreamScope ss(c, s);
+    HIP_TRY(ss.enter());
     if (c->svo_total > 0)
     {
         if (!d_svo)
@@ -1605,6 +1678,8 @@ ovx_status ovx_svo_parent_encode(ovx_fdg_ctx *c, const uint8_t *d_attr, int chan
     if (!d_attr || !d_delta)
         return fail(OVX_ERR_INVALID_ARGUMENT, "null attribute buffer");
     hipStream_t s = (hipStream_t)stream;
+    StreamScope ss(c, s);
+    HIP_TRY(ss.enter());
     ovx_status st;
     if ((st = ensure(c->svo_rows, c->svo_total * 4)) || (st = ensure(c->svo_erow, c->svo_total * 4)) ||
         (st = ensure(c->svo_tiles, scan_tile_count(std::max<uint64_t>(c->svo_total, (uint64_t)n)) * 8)))
@@ -1636,6 +1711,8 @@ ovx_status ovx_svo_neighbor_filter(ovx_fdg_ctx *c, const int32_t *d_coords, cons
     DeviceGuard dg(c->device);
     std::lock_guard<std::mutex> lk(c->mu);
     hipStream_t s = (hipStream_t)stream;
+    StreamScope ss(c, s);
+    HIP_TRY(ss.enter());
     ovx_status st;
     if ((st = ensure(c->nb_table, cells * 4)) || (st = ensure(c->nb_own, (size_t)n * 4)) ||
         (st = ensure(c->nb_nb, (size_t)n * 4)) ||

@@ -43,6 +43,7 @@ constexpr int MAX_GRID_EXTENT = 8192;
 struct GridDev
 {
     float vs[3];
+    float ivs[3];        // 1 / vs where vs is a power of two (x / vs == x * ivs exactly), else 0
     int gmin[3];
     int gmax[3];
     int G[3];
@@ -57,6 +58,8 @@ struct MeshDev
     const int32_t *f;   // [F,3], or nullptr for a triangle soup (v is [F,3,3])
     int64_t F;
     int64_t V;          // vertices (>= 1 when f is set and F > 0)
+    const float *tri = nullptr; // [F,9] the faces' vertices gathered once (by the prep kernel), or nullptr
+    const float4 *pl = nullptr; // [F] the faces' planes (prep kernel), or nullptr
 };
 
 // A face with a vertex index outside [0, V) is read as vertex 0 three times
@@ -71,7 +74,13 @@ __device__ __forceinline__ bool face_ok(const MeshDev &m, int32_t a, int32_t b, 
 __device__ __forceinline__ void load_tri(const MeshDev &m, int64_t face, float v0[3], float v1[3], float v2[3])
 {
     const float *p0, *p1, *p2;
-    if (m.f)
+    if (m.tri)
+    {
+        p0 = m.tri + 9 * face;
+        p1 = p0 + 3;
+        p2 = p0 + 6;
+    }
+    else if (m.f)
     {
         const int32_t *fi = m.f + 3 * face;
         int32_t a = fi[0], b = fi[1], c = fi[2];
@@ -99,6 +108,17 @@ __device__ __forceinline__ void load_tri(const MeshDev &m, int64_t face, float v
 __device__ __forceinline__ int iclamp(int v, int lo, int hi) { return v < lo ? lo : (hi < v ? hi : v); }
 
 // FDG.cpp:32-39
+// x / vs along axis a: a product by the exact reciprocal where vs is a power
+// of two (the same value, without the division sequence)
+__device__ __forceinline__ double div_vs(double x, const GridDev &g, int a)
+{
+    return g.ivs[a] != 0.0f ? x * (double)g.ivs[a] : x / (double)g.vs[a];
+}
+__device__ __forceinline__ float div_vs(float x, const GridDev &g, int a)
+{
+    return g.ivs[a] != 0.0f ? x * g.ivs[a] : x / g.vs[a];
+}
+
 __device__ __forceinline__ double lerp1(double a, double b, double t, double va, double vb)
 {
     if (a == b)

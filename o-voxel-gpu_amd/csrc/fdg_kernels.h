@@ -18,8 +18,11 @@ namespace ovx
 // of many small triangles
 constexpr int N_MARK_TILES = 3;
 constexpr int MARK_TILES[N_MARK_TILES] = {64, 128, 256};
+// face-test launch classes: blocks of 32 or 64 faces walked cooperatively,
+// or 256 faces walked one per lane (k_face_lane)
 constexpr int N_FACE_TILES = 3;
-constexpr int FACE_TILES[N_FACE_TILES] = {32, 64, 128};
+constexpr int FACE_TILES[N_FACE_TILES] = {32, 64, 256};
+constexpr int FACE_LANE_CLASS = 2;
 
 // device counters (unsigned long long each)
 enum
@@ -107,6 +110,7 @@ struct FaceChunk
 {
     uint32_t face;
     uint32_t r0, r1;
+    uint32_t ms; // the face's mesh
 };
 
 // one QEF contribution: order key, crossing z (phase 0), voxel index in its brick
@@ -300,7 +304,8 @@ struct FoldArgs
     unsigned long long *ctr;
 };
 
-hipError_t launch_prep(const MeshDev &m, const BatchDev &b, float4 *planes, unsigned long long *ctr, hipStream_t s);
+hipError_t launch_prep(const MeshDev &m, const BatchDev &b, float4 *planes, float *tri, unsigned long long *ctr,
+                       hipStream_t s);
 // tile: index into MARK_TILES; b.mtile_off must hold that tile size's offsets
 hipError_t launch_scan_events(const MeshDev &m, const BatchDev &b, int tile, uint64_t n_tiles, unsigned long long *ctr,
                               const ShardBuf<Event> &eb, const ShardBuf<RowChunk> &rq, bool merge, hipStream_t s);

@@ -334,7 +334,8 @@ __device__ __forceinline__ void bucket_emit(const uint32_t (&bk)[K], const Rec (
 // ---------------------------------------------------------------------------
 // prep: validate inputs, compute per-face planes
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BT) void k_prep(MeshDev m, BatchDev bt, float4 *planes, unsigned long long *counters)
+__global__ __launch_bounds__(BT) void k_prep(MeshDev m, BatchDev bt, float4 *planes, float *tri,
+                                             unsigned long long *counters)
 {
     const int64_t stride = (int64_t)gridDim.x * BT;
     unsigned flags = 0;
@@ -359,6 +360,18 @@ __global__ __launch_bounds__(BT) void k_prep(MeshDev m, BatchDev bt, float4 *pla
             load_tri(m, i, v0, v1, v2);
             tri_plane(v0, v1, v2, pl);
             planes[i] = make_float4(pl[0], pl[1], pl[2], pl[3]);
+            if (tri)
+            {
+                // the gathered triangle, read coalesced by the later stages
+                float *t = tri + 9 * i;
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                {
+                    t[k] = v0[k];
+                    t[3 + k] = v1[k];
+                    t[6 + k] = v2[k];
+                }
+            }
         }
     }
     // vertices: finite and |v / vs| < 2^30 (so every int conversion of the
@@ -373,7 +386,7 @@ __global__ __launch_bounds__(BT) void k_prep(MeshDev m, BatchDev bt, float4 *pla
             const float x = m.v[3 * i + a];
             if (!(fabsf(x) <= 3.0e38f))
                 flags |= ERR_NONFINITE;
-            else if (!(fabs((double)x / (double)g.vs[a]) < 1073741824.0))
+            else if (!(fabs(div_vs((double)x, g, a)) < 1073741824.0))
                 flags |= ERR_RANGE;
         }
     }
@@ -432,16 +445,15 @@ __device__ __forceinline__ void scan_item_setup(const MeshDev &m, const GridDev 
         if (t[1][1] < t[0][1])
             swp(0, 1);
     }
-    const double vs1 = (double)g.vs[ax1];
     const int lo = g.gmin[ax1], hi = g.gmax[ax1] - 1;
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
         for (int i = 0; i < 3; ++i)
             it.t[k][i] = t[k][i];
-    it.start = iclamp((int)((double)t[0][1] / vs1), lo, hi);
-    it.mid = iclamp((int)((double)t[1][1] / vs1), lo, hi);
-    it.end = iclamp((int)((double)t[2][1] / vs1), lo, hi);
+    it.start = iclamp((int)div_vs((double)t[0][1], g, ax1), lo, hi);
+    it.mid = iclamp((int)div_vs((double)t[1][1], g, ax1), lo, hi);
+    it.end = iclamp((int)div_vs((double)t[2][1], g, ax1), lo, hi);
     it.ax2 = ax2;
     it.face = (uint32_t)face;
 }
@@ -651,9 +663,9 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 // negative one
                 R = it.end > it.start ? (uint32_t)(it.end - it.start) : 0u;
                 // columns of a row are bounded by the x extent of the triangle
-                const float vs0 = sel3f(axis_next(it.ax2), g.vs[0], g.vs[1], g.vs[2]);
-                const float xlo = sminf(sminf(it.t[0][0], it.t[1][0]), it.t[2][0]) / vs0;
-                const float xhi = smaxf(smaxf(it.t[0][0], it.t[1][0]), it.t[2][0]) / vs0;
+                const int a0 = axis_next(it.ax2);
+                const float xlo = div_vs(sminf(sminf(it.t[0][0], it.t[1][0]), it.t[2][0]), g, a0);
+                const float xhi = div_vs(smaxf(smaxf(it.t[0][0], it.t[1][0]), it.t[2][0]), g, a0);
                 const float ext = xhi - xlo + 2.0f;
                 const uint32_t rpc =
                     ext < (float)MARK_VOX_CAP ? (uint32_t)((float)MARK_VOX_CAP / (ext > 1.0f ? ext : 1.0f)) : 1u;
@@ -737,8 +749,8 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 }
                 const int lo = sel3i(ax0, g.gmin[0], g.gmin[1], g.gmin[2]);
                 const int hi = sel3i(ax0, g.gmax[0], g.gmax[1], g.gmax[2]) - 1;
-                const int ls = iclamp((int)(t3x / (double)vs0), lo, hi);
-                const int le = iclamp((int)(t4x / (double)vs0), lo, hi);
+                const int ls = iclamp((int)div_vs(t3x, g, ax0), lo, hi);
+                const int le = iclamp((int)div_vs(t4x, g, ax0), lo, hi);
                 ScanRow &rw = rows[tid];
                 rw.t3x = t3x;
                 rw.t3z = t3z;
@@ -775,7 +787,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     const float vs2 = sel3f(ax2, g.vs[0], g.vs[1], g.vs[2]);
                     const double x = (double)((float)(x_idx + 1) * vs0);
                     const double z = lerp1(rw.t3x, rw.t4x, x, rw.t3z, rw.t4z);
-                    const int z_idx = (int)(z / (double)vs2);
+                    const int z_idx = (int)div_vs(z, g, ax2);
                     const int zlo = sel3i(ax2, g.gmin[0], g.gmin[1], g.gmin[2]);
                     const int zhi = sel3i(ax2, g.gmax[0], g.gmax[1], g.gmax[2]);
                     if (z_idx >= zlo && z_idx < zhi)
@@ -830,6 +842,7 @@ struct FaceItem
     int bx0, bx1, by0, bz0, ny;
     uint32_t face;
     uint32_t r0; // first bounding-box row (linear (y, z) index) handled here
+    uint32_t ms; // the face's mesh (queued row chunks of a batch mix meshes)
 };
 // A face whose bounding box holds more than FACE_VOX_CAP voxels is split into
 // chunks of bounding-box rows of about that many voxels; all but the last are queued
@@ -858,13 +871,14 @@ __device__ __forceinline__ void face_row_range(const FaceItem &it, const GridDev
     const double vs0 = (double)g.vs[0];
     const double py = (double)g.vs[1] * (double)y, pz = (double)g.vs[2] * (double)z;
     const double K = (double)it.n[1] * py + (double)it.n[2] * pz;
-    const double a = (-(double)it.d1 - K) / nx, b = (-(double)it.d2 - K) / nx;
+    const double inx = 1.0 / nx;
+    const double a = (-(double)it.d1 - K) * inx, b = (-(double)it.d2 - K) * inx;
     const double pxm = fmax(fabs((double)it.bx0), fabs((double)it.bx1)) * vs0;
     const double terms = fabs(nx) * pxm + fabs((double)it.n[1] * py) + fabs((double)it.n[2] * pz) +
                          fabs((double)it.d1) + fabs((double)it.d2);
     const double margin = 2.0 + 1.0e-6 * terms / (fabs(nx) * vs0);
-    const double lo = fmax(fmin(a, b) / vs0 - margin, (double)it.bx0 - 1.0);
-    const double hi = fmin(fmax(a, b) / vs0 + margin, (double)it.bx1 + 1.0);
+    const double lo = fmax(div_vs(fmin(a, b), g, 0) - margin, (double)it.bx0 - 1.0);
+    const double hi = fmin(div_vs(fmax(a, b), g, 0) + margin, (double)it.bx1 + 1.0);
     if (!(lo <= hi))
     {
         x1 = x0; // empty (also for non-finite planes)
@@ -926,14 +940,23 @@ __device__ __forceinline__ uint64_t face_item_setup(const MeshDev &m, const Grid
 {
     float v0[3], v1[3], v2[3], pl[4];
     load_tri(m, face, v0, v1, v2);
-    tri_plane(v0, v1, v2, pl);
+    if (m.pl)
+    {
+        // the plane the prep kernel computed with the same operations
+        const float4 p4 = m.pl[face];
+        pl[0] = p4.x;
+        pl[1] = p4.y;
+        pl[2] = p4.z;
+    }
+    else
+        tri_plane(v0, v1, v2, pl);
     const float n0 = pl[0], n1 = pl[1], n2 = pl[2];
     int bmin[3], bmax[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
     {
-        const float mn = sminf(sminf(v0[i], v1[i]), v2[i]) / g.vs[i];
-        const float mx = smaxf(smaxf(v0[i], v1[i]), v2[i]) / g.vs[i];
+        const float mn = div_vs(sminf(sminf(v0[i], v1[i]), v2[i]), g, i);
+        const float mx = div_vs(smaxf(smaxf(v0[i], v1[i]), v2[i]), g, i);
         const int a = (int)mn;
         const int b = (int)(mx + 1.0f);
         bmin[i] = a > g.gmin[i] ? a : g.gmin[i];
@@ -1027,9 +1050,9 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(7))) void k_
     // for single-mesh conversions only)
     for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x)
     {
-        // the tile's mesh; its grid is read in place
+        // the tile's mesh and grid (a queued item names its own)
         const int ms = QUEUED ? 0 : seg_find(bt.ftile_off, bt.M, tile);
-        const GridDev &g = bt.seg[ms].g;
+        const GridDev &gt = bt.seg[ms].g;
         int64_t item;
         bool live;
         if constexpr (!QUEUED)
@@ -1050,7 +1073,8 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(7))) void k_
             if (live)
             {
                 FaceItem &it = items[tid];
-                R = face_item_setup(m, g, item, it);
+                R = face_item_setup(m, gt, item, it);
+                it.ms = (uint32_t)ms;
                 it.r0 = 0;
                 const uint32_t ext = (uint32_t)(it.bx1 > it.bx0 ? it.bx1 - it.bx0 : 1);
                 const uint64_t rpc = ext < FACE_VOX_CAP ? FACE_VOX_CAP / ext : 1u;
@@ -1071,7 +1095,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(7))) void k_
                         for (uint64_t c = 0; c < n; ++c)
                         {
                             const uint64_t r0 = rpc * (c0 + c);
-                            fq.items[q0 + c] = FaceChunk{it.face, (uint32_t)r0, (uint32_t)(r0 + rpc)};
+                            fq.items[q0 + c] = FaceChunk{it.face, (uint32_t)r0, (uint32_t)(r0 + rpc), (uint32_t)ms};
                         }
                         done = c0 + n;
                     }
@@ -1086,7 +1110,8 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(7))) void k_
             {
                 const FaceChunk ch = fq.items[queue_slot(fq, qm.pre, (uint64_t)item)];
                 FaceItem &it = items[tid];
-                face_item_setup(m, g, (int64_t)ch.face, it);
+                face_item_setup(m, bt.seg[ch.ms].g, (int64_t)ch.face, it);
+                it.ms = ch.ms;
                 it.r0 = ch.r0;
                 R = (uint64_t)(ch.r1 - ch.r0);
             }
@@ -1105,6 +1130,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(7))) void k_
             {
                 const int L = upper_find(incl_items, FACE_TILE, gi);
                 const FaceItem &it = items[L];
+                const GridDev &g = QUEUED ? bt.seg[it.ms].g : gt;
                 const uint64_t R_L = incl_items[L] - (L ? incl_items[L - 1] : 0ull);
                 const uint64_t rr = (uint64_t)it.r0 + (gi - (incl_items[L] - R_L));
                 const int y = it.by0 + (int)(rr % (uint64_t)it.ny);
@@ -1137,6 +1163,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(7))) void k_
                     const int rr = upper_find(incl_rows, BT, h);
                     const FaceRow &row = rows[rr];
                     const FaceItem &it = items[row.item];
+                    const GridDev &g = QUEUED ? bt.seg[it.ms].g : gt;
                     uint32_t k = h - ((rr == 0) ? 0u : incl_rows[rr - 1]);
                     const int lx0 = row.x0 - g.gmin[0], lx1 = row.x1 - g.gmin[0];
                     const uint64_t rowbase =
@@ -1206,6 +1233,181 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(7))) void k_
     }
     (void)ctr;
     (void)n_hit;
+}
+
+// Walks the candidate voxels of bounding-box rows [0, n_rows) of one face
+// (the active voxels of each row's slab range) and returns the number of
+// faces-test hits; with WRITE the hits are also stored from `out` on.
+template <bool WRITE>
+__device__ __forceinline__ uint32_t face_walk(const FaceItem &it, const GridDev &g, uint64_t n_rows, FHit *out)
+{
+    uint32_t n = 0;
+    for (uint64_t rr = 0; rr < n_rows; ++rr)
+    {
+        const int y = it.by0 + (int)(rr % (uint64_t)it.ny);
+        const int z = it.bz0 + (int)(rr / (uint64_t)it.ny);
+        int x0, x1;
+        face_row_range(it, g, y, z, x0, x1);
+        const int lx0 = x0 - g.gmin[0], lx1 = x1 - g.gmin[0];
+        if (lx1 <= lx0)
+            continue;
+        const uint64_t rowbase = ((uint64_t)(z - g.gmin[2]) * g.G[1] + (uint64_t)(y - g.gmin[1])) * g.wx;
+        for (int w = lx0 >> 6; w <= ((lx1 - 1) >> 6); ++w)
+        {
+            const int blo = (w == (lx0 >> 6)) ? (lx0 & 63) : 0;
+            const int bhi = (w == ((lx1 - 1) >> 6)) ? (((lx1 - 1) & 63) + 1) : 64;
+            uint64_t mm = g.bits[rowbase + w] & row_mask(blo, bhi);
+            while (mm)
+            {
+                const int b = __builtin_ctzll(mm);
+                mm &= mm - 1ull;
+                const int x = g.gmin[0] + w * 64 + b;
+                if (face_pass(it, g.vs, x, y, z))
+                {
+                    if constexpr (WRITE)
+                        out[n] = FHit{it.face, (uint16_t)(x - g.gmin[0]), (uint16_t)(y - g.gmin[1]),
+                                      (uint16_t)(z - g.gmin[2]), 0};
+                    ++n;
+                }
+            }
+        }
+    }
+    return n;
+}
+
+// Hits of a face whose bounding box has at most 64 voxels in at most
+// FACE_MASK_ROWS rows, each row inside one bitset word, as a bit mask over
+// the box (bit r * ext + (x - bx0) for row r): the rows' bitset words are
+// loaded together, then each active voxel is tested once.
+constexpr int FACE_MASK_ROWS = 8;
+__device__ __forceinline__ bool face_mask_fits(const FaceItem &it, const GridDev &g, uint64_t R, int ext)
+{
+    const int sh = (it.bx0 - g.gmin[0]) & 63;
+    return R > 0 && R <= (uint64_t)FACE_MASK_ROWS && R * (uint64_t)ext <= 64 && sh + ext <= 64;
+}
+
+__device__ __forceinline__ uint64_t face_mask_hits(const FaceItem &it, const GridDev &g, int R, int ext)
+{
+    const int lx0 = it.bx0 - g.gmin[0];
+    const int w0 = lx0 >> 6, sh = lx0 & 63;
+    uint64_t wv[FACE_MASK_ROWS];
+#pragma unroll
+    for (int r = 0; r < FACE_MASK_ROWS; ++r)
+    {
+        wv[r] = 0ull;
+        if (r < R)
+        {
+            const int y = it.by0 + r % it.ny, z = it.bz0 + r / it.ny;
+            wv[r] = g.bits[((uint64_t)(z - g.gmin[2]) * g.G[1] + (uint64_t)(y - g.gmin[1])) * g.wx + w0];
+        }
+    }
+    const uint64_t emask = ext >= 64 ? ~0ull : ((1ull << ext) - 1ull);
+    uint64_t hits = 0ull;
+#pragma unroll
+    for (int r = 0; r < FACE_MASK_ROWS; ++r)
+    {
+        // active voxels of the row's box range, bit j = voxel bx0 + j
+        uint64_t act = (wv[r] >> sh) & emask;
+        if (r >= R || !act)
+            continue;
+        const int y = it.by0 + r % it.ny, z = it.bz0 + r / it.ny;
+        int x0, x1;
+        face_row_range(it, g, y, z, x0, x1);
+        const int j0 = x0 - it.bx0, j1 = x1 - it.bx0;
+        act &= (j1 >= 64 ? ~0ull : ((1ull << j1) - 1ull)) & ~((1ull << j0) - 1ull);
+        while (act)
+        {
+            const int j = __builtin_ctzll(act);
+            act &= act - 1ull;
+            if (face_pass(it, g.vs, it.bx0 + j, y, z))
+                hits |= 1ull << (r * ext + j);
+        }
+    }
+    return hits;
+}
+
+// The face test with one lane per face, for meshes whose faces cover few
+// voxels (block-cooperative walking costs more than it balances there).  A
+// block takes BT consecutive faces of one mesh.  A face of at most 64 box
+// voxels is tested once into a hit mask; a larger one is walked twice, to
+// count its hits and, after one claim per wave, to store them.  A face whose
+// bounding box holds more than FACE_LANE_CAP voxels goes to the row-chunk
+// queue, which the block-cooperative kernel consumes; if the queue is full
+// the lane walks it.
+constexpr uint32_t FACE_LANE_CAP = 192;
+__global__ __launch_bounds__(BT) void k_face_lane(MeshDev m, BatchDev bt, uint64_t n_tiles, ShardBuf<FHit> hb,
+                                                  ShardBuf<FaceChunk> fq)
+{
+    const int tid = threadIdx.x, lane = tid & 63;
+    for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x)
+    {
+        const int ms = seg_find(bt.ftile_off, bt.M, tile);
+        const GridDev &g = bt.seg[ms].g;
+        const int64_t f0 = bt.face_off[ms];
+        const int64_t loc = (int64_t)(tile - bt.ftile_off[ms]) * BT + tid;
+        FaceItem it;
+        uint64_t R = 0, hits = 0;
+        int ext = 1;
+        bool masked = false;
+        if (loc < bt.face_off[ms + 1] - f0)
+        {
+            R = face_item_setup(m, g, f0 + loc, it);
+            ext = it.bx1 > it.bx0 ? it.bx1 - it.bx0 : 1;
+            if (face_mask_fits(it, g, R, ext))
+            {
+                hits = face_mask_hits(it, g, (int)R, ext);
+                masked = true;
+            }
+            else if (R * (uint64_t)ext > FACE_LANE_CAP && fq.items)
+            {
+                // chunks of rpc rows of about FACE_VOX_CAP voxels, all queued or none
+                const uint64_t rpc = (uint64_t)ext < FACE_VOX_CAP ? FACE_VOX_CAP / (uint64_t)ext : 1u;
+                const uint64_t nq = (R + rpc - 1) / rpc;
+                uint64_t q0;
+                if (shard_claim_fit(fq, it.face % NSHARD, nq, q0))
+                {
+                    for (uint64_t c = 0; c < nq; ++c)
+                    {
+                        const uint64_t r0 = rpc * c, r1 = r0 + rpc < R ? r0 + rpc : R;
+                        fq.items[q0 + c] = FaceChunk{it.face, (uint32_t)r0, (uint32_t)r1, (uint32_t)ms};
+                    }
+                    R = 0;
+                }
+            }
+        }
+        const uint32_t cnt = masked ? (uint32_t)__popcll(hits) : (R ? face_walk<false>(it, g, R, nullptr) : 0u);
+        // one claim per wave for the wave's hits (consecutive faces stay together)
+        const uint32_t incl = wave_incl_scan(cnt);
+        const uint32_t total = __shfl(incl, 63, 64);
+        if (total == 0)
+            continue;
+        unsigned long long base = 0;
+        if (lane == 63)
+        {
+            bool ok = false;
+            const uint64_t b0 = shard_claim(hb, (uint32_t)((blockIdx.x * NWAVE + (tid >> 6)) % NSHARD), total, ok);
+            base = ok ? (unsigned long long)b0 : ~0ull;
+        }
+        base = __shfl(base, 63, 64);
+        if (!cnt || base == ~0ull)
+            continue;
+        FHit *out = hb.items + base + (incl - cnt);
+        if (masked)
+        {
+            uint32_t k = 0;
+            while (hits)
+            {
+                const int b = __builtin_ctzll(hits);
+                hits &= hits - 1ull;
+                const int r = b / ext, j = b - r * ext;
+                const int x = it.bx0 + j, y = it.by0 + r % it.ny, z = it.bz0 + r / it.ny;
+                out[k++] = FHit{it.face, (uint16_t)(x - g.gmin[0]), (uint16_t)(y - g.gmin[1]), (uint16_t)(z - g.gmin[2]),
+                                0};
+            }
+        }
+        else
+            face_walk<true>(it, g, R, out);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1461,7 +1663,7 @@ __global__ __launch_bounds__(BT) void k_bdry(int64_t E, const BEdge *be, const i
 #pragma unroll
         for (int a = 0; a < 3; ++a)
         {
-            cur[a] = (int)floorf(e.v0[a] / g.vs[a]);
+            cur[a] = (int)floorf(div_vs(e.v0[a], g, a));
             step[a] = e.dir[a] > 0.0 ? 1 : -1;
             if (e.dir[a] == 0.0)
             {
@@ -2091,7 +2293,7 @@ struct FoldStateC
                                 ax0 == 2 ? px : (ax1 == 2 ? py : z)};
 #pragma unroll
             for (int i = 0; i < 3; ++i)
-                m[i].add((p[i] - c[i]) / a.g.vs[i]);
+                m[i].add(div_vs(p[i] - c[i], a.g, i));
             cnt = cnt + 1.0f;
             if ((key & 3ull) == 0ull)
                 fl |= 1 << ax2;
@@ -3063,10 +3265,13 @@ static inline unsigned grid_for(int64_t n, int64_t cap = 1 << 20)
     return (unsigned)b;
 }
 
-hipError_t launch_prep(const MeshDev &m, const BatchDev &b, float4 *planes, unsigned long long *ctr, hipStream_t s)
+hipError_t launch_prep(const MeshDev &m, const BatchDev &b, float4 *planes, float *tri, unsigned long long *ctr,
+                       hipStream_t s)
 {
     const int64_t n = m.F > (m.f ? m.V : 3 * m.F) ? m.F : (m.f ? m.V : 3 * m.F);
-    hipLaunchKernelGGL(k_prep, dim3(grid_for(n)), dim3(BT), 0, s, m, b, planes, ctr);
+    MeshDev mi = m;
+    mi.tri = nullptr; // the prep kernel writes the gathered triangles
+    hipLaunchKernelGGL(k_prep, dim3(grid_for(n)), dim3(BT), 0, s, mi, b, planes, tri, ctr);
     return hipGetLastError();
 }
 
@@ -3128,13 +3333,18 @@ hipError_t launch_face(const MeshDev &m, const BatchDev &b, int tile, uint64_t n
     hipError_t e = hipMemsetAsync(fq.cnt, 0, (size_t)NSHARD * SHARD_STRIDE * sizeof(unsigned long long), s);
     if (e != hipSuccess)
         return e;
-    static_assert(FACE_TILES[0] == 32 && FACE_TILES[1] == 64 && FACE_TILES[2] == 128, "face tile instantiations");
+    static_assert(FACE_TILES[0] == 32 && FACE_TILES[1] == 64 && FACE_TILES[FACE_LANE_CLASS] == BT,
+                  "face tile instantiations");
     if (tile == 0)
         face_pair<32>(m, b, gf, n_tiles, hb, ctr, fq, s);
     else if (tile == 1)
         face_pair<64>(m, b, gf, n_tiles, hb, ctr, fq, s);
     else
-        face_pair<128>(m, b, gf, n_tiles, hb, ctr, fq, s);
+    {
+        hipLaunchKernelGGL(k_face_lane, dim3(gf), dim3(BT), 0, s, m, b, n_tiles, hb, fq);
+        // the queued faces of many voxels, block-cooperatively
+        hipLaunchKernelGGL((k_face<true, 64>), dim3(2048), dim3(BT), 0, s, m, b, 0ull, hb, ctr, fq);
+    }
     return hipGetLastError();
 }
 

@@ -234,6 +234,41 @@ def test_caller_stream_and_back_to_back(oracle):
             assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
 
 
+def test_alternating_streams_share_the_context():
+    """Conversions alternating between two caller streams with no host wait
+    between them (the library's workspace is one per device): each call's
+    work is ordered after the previous call's, so every result equals the
+    single-stream one bit for bit."""
+    import o_voxel
+
+    dev = _dev()
+    aabb = [[-0.5] * 3, [0.5] * 3]
+    meshes = []
+    for seed, nf, g in ((41, 30000, 128), (42, 8000, 64), (43, 60000, 256)):
+        v, f = meshgen.scene(seed=seed, target_faces=nf, n_closed=6, n_patches=6)
+        meshes.append((torch.from_numpy(v).to(dev), torch.from_numpy(f).to(dev), g))
+
+    def conv(m):
+        return o_voxel.convert.mesh_to_flexible_dual_grid(m[0], m[1], grid_size=m[2], aabb=aabb, face_weight=1.0,
+                                                          boundary_weight=0.2, regularization_weight=1e-2)
+
+    want = []
+    for m in meshes:
+        want.append([t.cpu() for t in conv(m)])
+    streams = [torch.cuda.Stream(dev), torch.cuda.Stream(dev)]  # default priority
+    print("[case] 12 conversions alternating between two streams", flush=True)
+    outs = []
+    for k in range(12):
+        with torch.cuda.stream(streams[k % 2]):
+            outs.append((k % 3, conv(meshes[k % 3])))
+    for s in streams:
+        s.synchronize()
+    for k, out in outs:
+        for a, b in zip(out, want[k]):
+            a = a.cpu()
+            assert a.shape == b.shape and torch.equal(a.view(torch.uint8), b.view(torch.uint8))
+
+
 @pytest.mark.parametrize("gmin", [0, -4096])
 def test_full_extent_axis(oracle, gmin):
     """A zig-zag strip across a grid axis of the maximum extent (8192
