@@ -585,13 +585,9 @@ ovx_status launch_face_tiles(ovx_fdg_ctx *c, const MeshDev &m, const ShardBuf<FH
             continue;
         BatchDev b = c->bdev;
         b.ftile_off = c->d_ftile_off[t];
-        // the cooperative classes queue row chunks for single-mesh conversions
-        // only (a batch has other work to balance); the lane class queues the
-        // rows of its rare large faces in any case
-        ShardBuf<FaceChunk> q = fq;
-        if (c->M > 1 && t != FACE_LANE_CLASS)
-            q.items = nullptr;
-        HIP_TRY(launch_face(m, b, t, nt, fb, ctr, q, s));
+        // faces of many voxels put their rows in chunks on a queue that a
+        // second launch shares out over the grid (row chunks carry their mesh)
+        HIP_TRY(launch_face(m, b, t, nt, fb, ctr, fq, s));
     }
     return OVX_OK;
 }

@@ -1275,18 +1275,40 @@ __device__ __forceinline__ uint32_t face_walk(const FaceItem &it, const GridDev 
     return n;
 }
 
-// Hits of a face whose bounding box has at most 64 voxels in at most
-// FACE_MASK_ROWS rows, each row inside one bitset word, as a bit mask over
-// the box (bit r * ext + (x - bx0) for row r): the rows' bitset words are
-// loaded together, then each active voxel is tested once.
+// Hits of a face whose bounding box is at most 8 voxels long along x, with
+// at most FACE_MASK_ROWS (y, z) rows, each row inside one bitset word, as a
+// bit mask over the box (bit 8 r + (x - bx0) for row r).  The rows' words
+// are loaded together and their active voxels gathered into one candidate
+// mask, so a wave's lanes test their candidates in one loop (as many steps
+// as the lane with the most candidates, not that maximum per row summed).
 constexpr int FACE_MASK_ROWS = 8;
 __device__ __forceinline__ bool face_mask_fits(const FaceItem &it, const GridDev &g, uint64_t R, int ext)
 {
     const int sh = (it.bx0 - g.gmin[0]) & 63;
-    return R > 0 && R <= (uint64_t)FACE_MASK_ROWS && R * (uint64_t)ext <= 64 && sh + ext <= 64;
+    return R > 0 && R <= (uint64_t)FACE_MASK_ROWS && ext <= 8 && sh + ext <= 64;
 }
 
-__device__ __forceinline__ uint64_t face_mask_hits(const FaceItem &it, const GridDev &g, int R, int ext)
+// (y, z) offsets of the box's rows r < 8 from its min corner, 4 bits each,
+// row r in byte r (rows run y fastest)
+__device__ __forceinline__ uint64_t face_mask_rows(const FaceItem &it)
+{
+    uint64_t t = 0ull;
+    int dy = 0, dz = 0;
+#pragma unroll
+    for (int r = 0; r < FACE_MASK_ROWS; ++r)
+    {
+        t |= (uint64_t)(dy | (dz << 4)) << (8 * r);
+        if (++dy == it.ny)
+        {
+            dy = 0;
+            ++dz;
+        }
+    }
+    return t;
+}
+
+__device__ __forceinline__ uint64_t face_mask_hits(const FaceItem &it, const GridDev &g, int R, int ext,
+                                                   uint64_t ryz)
 {
     const int lx0 = it.bx0 - g.gmin[0];
     const int w0 = lx0 >> 6, sh = lx0 & 63;
@@ -1297,45 +1319,51 @@ __device__ __forceinline__ uint64_t face_mask_hits(const FaceItem &it, const Gri
         wv[r] = 0ull;
         if (r < R)
         {
-            const int y = it.by0 + r % it.ny, z = it.bz0 + r / it.ny;
+            const int y = it.by0 + (int)((ryz >> (8 * r)) & 15u), z = it.bz0 + (int)((ryz >> (8 * r + 4)) & 15u);
             wv[r] = g.bits[((uint64_t)(z - g.gmin[2]) * g.G[1] + (uint64_t)(y - g.gmin[1])) * g.wx + w0];
         }
     }
-    const uint64_t emask = ext >= 64 ? ~0ull : ((1ull << ext) - 1ull);
-    uint64_t hits = 0ull;
+    const uint64_t emask = (1ull << ext) - 1ull;
+    // long rows are narrowed to the plane's slab (face_row_range); at most 8
+    // voxels here, so only rows of 5 .. 8 voxels
+    const bool narrow = ext > 4 && fabsf(it.n[0]) >= 0.05f;
+    uint64_t cand = 0ull;
 #pragma unroll
     for (int r = 0; r < FACE_MASK_ROWS; ++r)
     {
-        // active voxels of the row's box range, bit j = voxel bx0 + j
         uint64_t act = (wv[r] >> sh) & emask;
-        if (r >= R || !act)
-            continue;
-        const int y = it.by0 + r % it.ny, z = it.bz0 + r / it.ny;
-        int x0, x1;
-        face_row_range(it, g, y, z, x0, x1);
-        const int j0 = x0 - it.bx0, j1 = x1 - it.bx0;
-        act &= (j1 >= 64 ? ~0ull : ((1ull << j1) - 1ull)) & ~((1ull << j0) - 1ull);
-        while (act)
+        if (narrow && r < R && act)
         {
-            const int j = __builtin_ctzll(act);
-            act &= act - 1ull;
-            if (face_pass(it, g.vs, it.bx0 + j, y, z))
-                hits |= 1ull << (r * ext + j);
+            int x0, x1;
+            face_row_range(it, g, it.by0 + (int)((ryz >> (8 * r)) & 15u), it.bz0 + (int)((ryz >> (8 * r + 4)) & 15u),
+                           x0, x1);
+            const int j0 = x0 - it.bx0, j1 = x1 - it.bx0;
+            act &= ((1ull << j1) - 1ull) & ~((1ull << j0) - 1ull);
         }
+        cand |= act << (8 * r);
+    }
+    uint64_t hits = 0ull;
+    while (cand)
+    {
+        const int b = __builtin_ctzll(cand);
+        cand &= cand - 1ull;
+        const uint32_t yz = (uint32_t)(ryz >> (b & ~7)) & 0xffu;
+        if (face_pass(it, g.vs, it.bx0 + (b & 7), it.by0 + (int)(yz & 15u), it.bz0 + (int)(yz >> 4)))
+            hits |= 1ull << b;
     }
     return hits;
 }
 
 // The face test with one lane per face, for meshes whose faces cover few
 // voxels (block-cooperative walking costs more than it balances there).  A
-// block takes BT consecutive faces of one mesh.  A face of at most 64 box
-// voxels is tested once into a hit mask; a larger one is walked twice, to
+// block takes BT consecutive faces of one mesh.  A face of a small box (at
+// most 8 x 8 voxels) is tested once into a hit mask; a larger one is walked twice, to
 // count its hits and, after one claim per wave, to store them.  A face whose
 // bounding box holds more than FACE_LANE_CAP voxels goes to the row-chunk
 // queue, which the block-cooperative kernel consumes; if the queue is full
 // the lane walks it.
 constexpr uint32_t FACE_LANE_CAP = 192;
-__global__ __launch_bounds__(BT) void k_face_lane(MeshDev m, BatchDev bt, uint64_t n_tiles, ShardBuf<FHit> hb,
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_face_lane(MeshDev m, BatchDev bt, uint64_t n_tiles, ShardBuf<FHit> hb,
                                                   ShardBuf<FaceChunk> fq)
 {
     const int tid = threadIdx.x, lane = tid & 63;
@@ -1346,7 +1374,7 @@ __global__ __launch_bounds__(BT) void k_face_lane(MeshDev m, BatchDev bt, uint64
         const int64_t f0 = bt.face_off[ms];
         const int64_t loc = (int64_t)(tile - bt.ftile_off[ms]) * BT + tid;
         FaceItem it;
-        uint64_t R = 0, hits = 0;
+        uint64_t R = 0, hits = 0, ryz = 0;
         int ext = 1;
         bool masked = false;
         if (loc < bt.face_off[ms + 1] - f0)
@@ -1355,7 +1383,8 @@ __global__ __launch_bounds__(BT) void k_face_lane(MeshDev m, BatchDev bt, uint64
             ext = it.bx1 > it.bx0 ? it.bx1 - it.bx0 : 1;
             if (face_mask_fits(it, g, R, ext))
             {
-                hits = face_mask_hits(it, g, (int)R, ext);
+                ryz = face_mask_rows(it);
+                hits = face_mask_hits(it, g, (int)R, ext, ryz);
                 masked = true;
             }
             else if (R * (uint64_t)ext > FACE_LANE_CAP && fq.items)
@@ -1399,8 +1428,8 @@ __global__ __launch_bounds__(BT) void k_face_lane(MeshDev m, BatchDev bt, uint64
             {
                 const int b = __builtin_ctzll(hits);
                 hits &= hits - 1ull;
-                const int r = b / ext, j = b - r * ext;
-                const int x = it.bx0 + j, y = it.by0 + r % it.ny, z = it.bz0 + r / it.ny;
+                const uint32_t yz = (uint32_t)(ryz >> (b & ~7)) & 0xffu;
+                const int x = it.bx0 + (b & 7), y = it.by0 + (int)(yz & 15u), z = it.bz0 + (int)(yz >> 4);
                 out[k++] = FHit{it.face, (uint16_t)(x - g.gmin[0]), (uint16_t)(y - g.gmin[1]), (uint16_t)(z - g.gmin[2]),
                                 0};
             }
@@ -2304,30 +2333,29 @@ struct FoldStateC
         {
             // bw |A (p - p0)|^2 with A = I - d d^T: in the voxel frame
             // Q_ij = vs_i vs_j A_ij, Q_i3 = -vs_i (A r0)_i, Q_33 = r0^T A r0, r0 = p0 - c
+            // entries formed one row of A at a time (few doubles live at once)
             const uint64_t e = key & ((1ull << KEY_PHASE_SHIFT) - 1ull);
             const BEdge &be = a.bedges[e];
-            double A[3][3], r0[3], Ar[3];
+            const double d[3] = {be.dir[0], be.dir[1], be.dir[2]};
+            const double r0[3] = {(double)be.v0[0] - (double)c[0], (double)be.v0[1] - (double)c[1],
+                                  (double)be.v0[2] - (double)c[2]};
+            const double bw = (double)a.bw;
+            double rAr = 0.0;
 #pragma unroll
             for (int i = 0; i < 3; ++i)
             {
-                r0[i] = (double)be.v0[i] - (double)c[i];
+                double Ai[3];
 #pragma unroll
                 for (int j = 0; j < 3; ++j)
-                    A[i][j] = (i == j ? 1.0 : 0.0) - be.dir[i] * be.dir[j];
-            }
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-                Ar[i] = A[i][0] * r0[0] + A[i][1] * r0[1] + A[i][2] * r0[2];
-            const double bw = (double)a.bw;
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-            {
+                    Ai[j] = (i == j ? 1.0 : 0.0) - d[i] * d[j];
+                const double Ar = Ai[0] * r0[0] + Ai[1] * r0[1] + Ai[2] * r0[2];
 #pragma unroll
                 for (int j = i; j < 3; ++j)
-                    q[qidx(i, j)].add_double(bw * ((double)a.g.vs[i] * (double)a.g.vs[j] * A[i][j]));
-                q[qidx(i, 3)].add_double(bw * (-(double)a.g.vs[i] * Ar[i]));
+                    q[qidx(i, j)].add_double(bw * ((double)a.g.vs[i] * (double)a.g.vs[j] * Ai[j]));
+                q[qidx(i, 3)].add_double(bw * (-(double)a.g.vs[i] * Ar));
+                rAr = i == 0 ? r0[0] * Ar : rAr + r0[i] * Ar;
             }
-            q[qidx(3, 3)].add_double(bw * (r0[0] * Ar[0] + r0[1] * Ar[1] + r0[2] * Ar[2]));
+            q[qidx(3, 3)].add_double(bw * rAr);
         }
     }
 
@@ -2450,14 +2478,14 @@ __device__ __forceinline__ void for_each_rec(const FoldArgs &a, uint32_t brick, 
     }
 }
 
-// key, crossing coordinate and face plane of the grouped records i .. i+3 below s1
+// key, crossing coordinate and face plane of the grouped records i .. i+FETCH-1 below s1
 // (zeros past s1; boundary records have no plane)
-template <int CAP>
+template <int CAP, int FETCH>
 __device__ __forceinline__ void fetch_recs(const FoldArgs &a, const FoldLds<CAP> &L, uint32_t i, uint32_t s1,
-                                           uint64_t kk[4], float zz[4], float4 pp[4])
+                                           uint64_t kk[FETCH], float zz[FETCH], float4 pp[FETCH])
 {
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < FETCH; ++u)
     {
         kk[u] = 0;
         zz[u] = 0.0f;
@@ -2539,15 +2567,16 @@ __device__ __forceinline__ void fold_loaded(const FoldArgs &a, FoldLds<CAP> &L, 
         typename FoldPick<P>::type st;
         st.init(x0 + (l & 7), y0 + ((l >> 3) & 7), z0 + (l >> 6), a);
         st.begin(L.key[L.perm[s0]]);
-        for (uint32_t i = s0; i < s1; i += 4)
+        // the planes of up to FETCH records are fetched together, then folded in order
+        constexpr int FETCH = P ? 1 : 4;
+        for (uint32_t i = s0; i < s1; i += FETCH)
         {
-            // the planes of up to 4 records are fetched together, then folded in order
-            uint64_t kk[4];
-            float zz[4];
-            float4 pp[4];
-            fetch_recs<CAP>(a, L, i, s1, kk, zz, pp);
+            uint64_t kk[FETCH];
+            float zz[FETCH];
+            float4 pp[FETCH];
+            fetch_recs<CAP, FETCH>(a, L, i, s1, kk, zz, pp);
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < FETCH; ++u)
                 if (i + u < s1)
                     st.add_pl(kk[u], zz[u], pp[u], a);
         }
@@ -2708,14 +2737,17 @@ __device__ __forceinline__ void fold_grouped(const FoldArgs &a, FoldLdsG<CAP> &L
         typename FoldPick<P>::type st;
         st.init(x0 + (l & 7), y0 + ((l >> 3) & 7), z0 + (l >> 6), a);
         st.begin(L.key[s0]);
-        for (uint32_t i = s0; i < s1; i += 4)
+        // the planes of up to FETCH records are fetched together, then folded
+        // in order (one at a time for the precision variant, whose larger
+        // state leaves no registers for more)
+        constexpr int FETCH = P ? 1 : 4;
+        for (uint32_t i = s0; i < s1; i += FETCH)
         {
-            // the planes of up to 4 records are fetched together, then folded in order
-            uint64_t kk[4];
-            float zz[4];
-            float4 pp[4];
+            uint64_t kk[FETCH];
+            float zz[FETCH];
+            float4 pp[FETCH];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < FETCH; ++u)
             {
                 kk[u] = 0;
                 zz[u] = 0.0f;
@@ -2730,7 +2762,7 @@ __device__ __forceinline__ void fold_grouped(const FoldArgs &a, FoldLdsG<CAP> &L
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < FETCH; ++u)
                 if (i + u < s1)
                     st.add_pl(kk[u], zz[u], pp[u], a);
         }
@@ -2840,7 +2872,7 @@ __device__ __forceinline__ void fold_bricks_body(const FoldArgs &ain)
 // LDS and at most 80 VGPRs leave room for 6 blocks per CU), the main tile for
 // the precision variant, and the mid-size tile
 template <int P>
-__global__ __launch_bounds__(BT) void k_fold_bricks(FoldArgs a)
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(5))) void k_fold_bricks(FoldArgs a)
 {
     fold_bricks_body<P, FOLD_SMALL, false>(a);
 }
