@@ -161,6 +161,20 @@ ovx_status ovx_fdg_set_tile_classes(ovx_fdg_ctx *ctx, const double *mark_fpc3, c
 ovx_status ovx_fdg_get_tile_classes(ovx_fdg_ctx *ctx, double *mark_fpc3, double *face_vpf2);
 
 /*
+ * Brick size of the record grouping and fold for the next conversions on
+ * this context.  No counterpart in the reference (its brick hash uses 8x8x8
+ * bricks, src/convert/mesh_to_flexible_dual_grid_gpu/types.cuh:47-55).  A
+ * mesh with at least `faces_per_voxel` faces per occupied voxel is grouped in
+ * 4x4x4 bricks (its records are dense: an 8x8x8 brick would exceed the fold's
+ * tiles), any other in 8x8x8 bricks.  Default 4; the autotuner
+ * o_voxel.convert.autotune_tile_classes sets it from timed trial
+ * conversions.  Results do not depend on it.  Must not be called between
+ * begin and finish.
+ */
+ovx_status ovx_fdg_set_brick_class(ovx_fdg_ctx *ctx, double faces_per_voxel);
+ovx_status ovx_fdg_get_brick_class(ovx_fdg_ctx *ctx, double *faces_per_voxel);
+
+/*
  * Per-stage statistics of the last conversion on this context; the first
  * min(n, OVX_FDG_N_STATS) are written:
  *   out[0] N voxels          out[1] scanline events     out[2] QEF records

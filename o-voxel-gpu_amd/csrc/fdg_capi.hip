@@ -142,6 +142,7 @@ struct ovx_fdg_ctx
     BatchDev bdev{};
     double mark_fpc[3] = {0.5, 2.0, 8.0}; // scanline class thresholds (faces per grid cross-section)
     double face_vpf[2] = {4.0, 16.0};     // face-test class thresholds (voxels per face)
+    double brick_fpv = 3.0;               // 4x4x4 bricks for meshes of at least this many faces per voxel
     float fw = 0, bw = 0, rw = 0;
     int prec = 0;           // 0 reference float path, 1 compensated voxel-frame accumulation
     float cond_max = 1.0e6f; // precision variant: FP64 solve above this condition estimate
@@ -436,6 +437,7 @@ ovx_status setup_batch(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *f
         sg.g = g;
         for (int a = 0; a < 3; ++a)
             sg.nb[a] = (g.G[a] + 7) / 8;
+        sg.bs = 3; // 8x8x8 until finish chooses (choose_bricks)
         sg.brick0 = c->brick_off[i];
         sg.face0 = face_off[i];
         const int64_t nf = face_off[i + 1] - face_off[i];
@@ -808,11 +810,54 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m_in, int M, const int64_
     return OVX_OK;
 }
 
+// Brick size of each mesh for grouping and folding its records: 4x4x4 voxels
+// where the faces per voxel reach brick_fpv (records are dense there: an
+// 8x8x8 brick would hold more than the fold's tiles), else 8x8x8.  Set-up
+// uploaded 8x8x8 for every mesh; the segments, brick offsets and brick guide
+// are uploaded again when a mesh changes.
+ovx_status choose_bricks(ovx_fdg_ctx *c, hipStream_t s)
+{
+    const int M = c->M;
+    bool changed = false;
+    for (int i = 0; i < M; ++i)
+    {
+        MeshSeg &sg = c->segs[i];
+        const double nv = (double)(c->h_vox_off[i + 1] - c->h_vox_off[i]);
+        const double nf = (double)(c->face_off[i + 1] - c->face_off[i]);
+        const int bs = (nv > 0.0 && nf >= c->brick_fpv * nv) ? 2 : 3;
+        changed |= bs != sg.bs;
+        sg.bs = bs;
+        for (int a = 0; a < 3; ++a)
+            sg.nb[a] = (sg.g.G[a] + (1 << bs) - 1) >> bs;
+        sg.brick0 = c->brick_off[i];
+        c->brick_off[i + 1] = c->brick_off[i] + (uint64_t)sg.nb[0] * (uint64_t)sg.nb[1] * (uint64_t)sg.nb[2];
+    }
+    c->n_bricks = c->brick_off[M];
+    if (c->n_bricks >= (1ull << 32))
+        return fail(OVX_ERR_CAPACITY, "brick grids of %llu bricks exceed 2^32", (unsigned long long)c->n_bricks);
+    if (!changed)
+        return OVX_OK;
+    // the staging regions were last read by begin's set-up copy, complete since its readback
+    const size_t seg_bytes = ((sizeof(MeshSeg) * (size_t)M + 255) / 256) * 256;
+    const size_t arr = (size_t)(M + 1) * 8;
+    const size_t gb = c->stage_gd + GD_BRICK * c->gd_bytes;
+    unsigned char *h = c->h_stage, *d = (unsigned char *)c->segbuf.p;
+    std::memcpy(h, c->segs.data(), sizeof(MeshSeg) * (size_t)M);
+    std::memcpy(h + seg_bytes + 3 * arr, c->brick_off.data(), arr);
+    c->bdev.gsh[GD_BRICK] = build_guide(c->brick_off.data(), M, (int32_t *)(h + gb));
+    HIP_TRY(hipMemcpyAsync(d, h, sizeof(MeshSeg) * (size_t)M, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d + seg_bytes + 3 * arr, h + seg_bytes + 3 * arr, arr, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d + gb, h + gb, c->gd_bytes, hipMemcpyHostToDevice, s));
+    return OVX_OK;
+}
+
 ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *flags, hipStream_t s)
 {
     ovx_status st;
     const MeshDev &m = c->mesh;
     auto *ctr = (unsigned long long *)c->counters.p;
+    if ((st = choose_bricks(c, s)))
+        return st;
     if (c->M > 1)
     {
         // the voxel guide, from the voxel offsets begin read back
@@ -1281,6 +1326,29 @@ ovx_status ovx_fdg_get_tile_classes(ovx_fdg_ctx *c, double *mark_fpc3, double *f
     if (face_vpf2)
         for (int i = 0; i < 2; ++i)
             face_vpf2[i] = c->face_vpf[i];
+    return OVX_OK;
+}
+
+ovx_status ovx_fdg_set_brick_class(ovx_fdg_ctx *c, double faces_per_voxel)
+{
+    if (!c)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "null context");
+    if (!(faces_per_voxel >= 0.0 && faces_per_voxel <= 1e30))
+        return fail(OVX_ERR_INVALID_ARGUMENT, "brick class threshold must be finite and >= 0");
+    DeviceGuard dg(c->device);
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (c->pending)
+        return fail(OVX_ERR_STATE, "brick class changed between begin and finish");
+    c->brick_fpv = faces_per_voxel;
+    return OVX_OK;
+}
+
+ovx_status ovx_fdg_get_brick_class(ovx_fdg_ctx *c, double *faces_per_voxel)
+{
+    if (!c || !faces_per_voxel)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    *faces_per_voxel = c->brick_fpv;
     return OVX_OK;
 }
 

@@ -152,8 +152,8 @@ struct BHit
 struct MeshSeg
 {
     GridDev g;
-    int nb[3];      // brick grid (8x8x8 voxels per brick)
-    int pad;
+    int nb[3];      // brick grid
+    int bs;         // log2 of the brick edge: 3 (8x8x8 voxels) or 2 (4x4x4, meshes of dense records)
     uint64_t brick0; // first brick of the mesh in the shared brick numbering
     int64_t face0;   // first face
 };
@@ -290,6 +290,7 @@ struct FoldArgs
     unsigned long long *n_huge;         // their count (device, zeroed by the host)
     BatchDev b;                         // meshes
     int nb[3];  // brick grid of the brick being folded (set per brick from its mesh)
+    int bs;     // log2 of its brick edge
     GridDev g;  // grid of the brick's mesh (set per brick)
     const float4 *planes;
     const BEdge *bedges;

@@ -108,14 +108,16 @@ __device__ __forceinline__ uint32_t voxel_id(const GridDev &g, uint64_t w, int b
     return g.woff[w] + (uint32_t)__popcll(m & ((1ull << bit) - 1ull));
 }
 
-// brick of an absolute voxel coordinate: id in the dense brick grid and the
-// voxel's index inside its 8x8x8 brick
-__device__ __forceinline__ void brick_of(const GridDev &g, const int nb[3], int c0, int c1, int c2, uint32_t &brick,
-                                         uint32_t &lvox)
+// brick of an absolute voxel coordinate: id in the dense brick grid of the
+// mesh (bricks of 2^bs voxels along each axis) and the voxel's index inside
+// its brick, laid out as in an 8x8x8 brick (x | y << 3 | z << 6) for either size
+__device__ __forceinline__ void brick_of(const MeshSeg &sg, int c0, int c1, int c2, uint32_t &brick, uint32_t &lvox)
 {
+    const GridDev &g = sg.g;
     const int lx = c0 - g.gmin[0], ly = c1 - g.gmin[1], lz = c2 - g.gmin[2];
-    brick = (uint32_t)(lx >> 3) + (uint32_t)nb[0] * ((uint32_t)(ly >> 3) + (uint32_t)nb[1] * (uint32_t)(lz >> 3));
-    lvox = (uint32_t)((lx & 7) | ((ly & 7) << 3) | ((lz & 7) << 6));
+    const int bs = sg.bs, m = (1 << bs) - 1;
+    brick = (uint32_t)(lx >> bs) + (uint32_t)sg.nb[0] * ((uint32_t)(ly >> bs) + (uint32_t)sg.nb[1] * (uint32_t)(lz >> bs));
+    lvox = (uint32_t)((lx & m) | ((ly & m) << 3) | ((lz & m) << 6));
 }
 
 // Wave-aggregated append: one atomic per wave.  All lanes of the wave must call it.
@@ -732,7 +734,6 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 const double t1x = it.t[1][0], t1y = it.t[1][1], t1z = it.t[1][2];
                 const double t2x = it.t[i2][0], t2y = it.t[i2][1], t2z = it.t[i2][2];
                 const float vs1 = sel3f(ax1, g.vs[0], g.vs[1], g.vs[2]);
-                const float vs0 = sel3f(ax0, g.vs[0], g.vs[1], g.vs[2]);
                 const double y = (double)((float)(y_idx + 1) * vs1);
                 double t3x = lerp1(t0y, t1y, y, t0x, t1x);
                 double t3z = lerp1(t0y, t1y, y, t0z, t1z);
@@ -784,7 +785,6 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     const int y_idx = ylo + (int)rw.y_off;
                     const int x_idx = xlo + (int)rw.x_off + (int)(h - ex);
                     const float vs0 = sel3f(ax0, g.vs[0], g.vs[1], g.vs[2]);
-                    const float vs2 = sel3f(ax2, g.vs[0], g.vs[1], g.vs[2]);
                     const double x = (double)((float)(x_idx + 1) * vs0);
                     const double z = lerp1(rw.t3x, rw.t4x, x, rw.t3z, rw.t4z);
                     const int z_idx = (int)div_vs(z, g, ax2);
@@ -1795,7 +1795,7 @@ __global__ __launch_bounds__(BT) void k_gen_events(ShardBuf<Event> ev, BucketArg
             for (int k = 0; k < 4; ++k)
             {
                 uint32_t lvox;
-                brick_of(g, sg.nb, c[k][0], c[k][1], c[k][2], bk[k], lvox);
+                brick_of(sg, c[k][0], c[k][1], c[k][2], bk[k], lvox);
                 bk[k] += (uint32_t)sg.brick0;
                 rc[k] = Rec{e.key | (uint64_t)k, e.z, lvox};
                 ok[k] = true;
@@ -1826,7 +1826,7 @@ __global__ __launch_bounds__(BT) void k_gen_fhits(ShardBuf<FHit> h, BucketArgs b
                 const MeshSeg &sg = ba.b.seg[mesh_of_face(ba.b, (int64_t)x.face)];
                 const GridDev &g = sg.g;
                 uint32_t lvox;
-                brick_of(g, sg.nb, g.gmin[0] + (int)x.lx, g.gmin[1] + (int)x.ly, g.gmin[2] + (int)x.lz, bk[k], lvox);
+                brick_of(sg, g.gmin[0] + (int)x.lx, g.gmin[1] + (int)x.ly, g.gmin[2] + (int)x.lz, bk[k], lvox);
                 bk[k] += (uint32_t)sg.brick0;
                 rc[k] = Rec{(1ull << KEY_PHASE_SHIFT) | (uint64_t)x.face, 0.0f, lvox};
                 ok[k] = true;
@@ -1853,7 +1853,7 @@ __global__ __launch_bounds__(BT) void k_gen_bhits(ShardBuf<BHit> h, BucketArgs b
             const BHit x = h.items[queue_slot(h, qm.pre, i)];
             const MeshSeg &sg = ba.b.seg[mesh_of_vert(ba.b, (int64_t)ba.bedge_v[x.edge].x)];
             uint32_t lvox;
-            brick_of(sg.g, sg.nb, x.x, x.y, x.z, bk[0], lvox);
+            brick_of(sg, x.x, x.y, x.z, bk[0], lvox);
             bk[0] += (uint32_t)sg.brick0;
             rc[0] = Rec{(2ull << KEY_PHASE_SHIFT) | (uint64_t)x.edge, 0.0f, lvox};
             ok[0] = true;
@@ -2626,6 +2626,7 @@ __device__ __forceinline__ uint32_t enter_brick(FoldArgs &a, uint32_t brick)
     a.nb[0] = sg.nb[0];
     a.nb[1] = sg.nb[1];
     a.nb[2] = sg.nb[2];
+    a.bs = sg.bs;
     return brick - (uint32_t)sg.brick0;
 }
 
@@ -2636,9 +2637,9 @@ __device__ __forceinline__ void brick_origin(const FoldArgs &a, uint32_t brick, 
     const int byz = (int)(brick / (uint32_t)a.nb[0]);
     const int by = byz % a.nb[1];
     const int bz = byz / a.nb[1];
-    x0 = a.g.gmin[0] + 8 * bx;
-    y0 = a.g.gmin[1] + 8 * by;
-    z0 = a.g.gmin[2] + 8 * bz;
+    x0 = a.g.gmin[0] + (bx << a.bs);
+    y0 = a.g.gmin[1] + (by << a.bs);
+    z0 = a.g.gmin[2] + (bz << a.bs);
 }
 
 // LDS of the small-brick fold: the brick's records grouped by voxel and,

@@ -35,6 +35,8 @@ __all__ = [
     "set_timing",
     "kernel_times",
     "set_capacity",
+    "set_brick_class",
+    "get_brick_class",
     "workspace_bytes",
     "flexible_dual_grid_to_mesh_cuda",
     "hashmap_insert_cuda",
@@ -114,6 +116,10 @@ def _load():
         f64p = ctypes.POINTER(ctypes.c_double)
         L.ovx_fdg_set_tile_classes.argtypes = [vp, f64p, f64p]
         L.ovx_fdg_get_tile_classes.argtypes = [vp, f64p, f64p]
+        L.ovx_fdg_set_brick_class.argtypes = [vp, ctypes.c_double]
+        L.ovx_fdg_set_brick_class.restype = ctypes.c_int
+        L.ovx_fdg_get_brick_class.argtypes = [vp, f64p]
+        L.ovx_fdg_get_brick_class.restype = ctypes.c_int
         L.ovx_fdg_set_capacity.restype = ctypes.c_int
         L.ovx_fdg_kernel_count.restype = ctypes.c_int
         L.ovx_fdg_kernel_name.argtypes = [ctypes.c_int]
@@ -976,6 +982,26 @@ def get_tile_classes(device=None) -> Tuple[List[float], List[float]]:
     with torch.cuda.device(dev), ctx.lock:
         _check(_load().ovx_fdg_get_tile_classes(ctx.h, m, f), "get_tile_classes")
     return [float(x) for x in m], [float(x) for x in f]
+
+
+def set_brick_class(device=None, faces_per_voxel: float = 3.0):
+    """Meshes with at least ``faces_per_voxel`` faces per occupied voxel group
+    and fold their records in 4x4x4 bricks, others in 8x8x8 ones, for the next
+    conversions on ``device`` (ovx_fdg_set_brick_class; results do not change)."""
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    ctx = _ctx(dev)
+    with torch.cuda.device(dev), ctx.lock:
+        _check(_load().ovx_fdg_set_brick_class(ctx.h, float(faces_per_voxel)), "set_brick_class")
+
+
+def get_brick_class(device=None) -> float:
+    """The brick-size threshold (faces per voxel) of ``device``'s context."""
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    ctx = _ctx(dev)
+    out = ctypes.c_double()
+    with torch.cuda.device(dev), ctx.lock:
+        _check(_load().ovx_fdg_get_brick_class(ctx.h, ctypes.byref(out)), "get_brick_class")
+    return float(out.value)
 
 
 def set_capacity(device=None, events: int = 0, hits: int = 0):

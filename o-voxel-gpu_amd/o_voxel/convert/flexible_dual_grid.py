@@ -293,50 +293,68 @@ def mesh_to_flexible_dual_grid_batch(
 
 def autotune_tile_classes(meshes, scales=(0.25, 0.5, 1.0, 2.0, 4.0), repeats: int = 3, **batch_kwargs) -> dict:
     """Choose the launch-class thresholds of the scanline and face-test kernels
-    for meshes like ``meshes`` (an addition to the reference API, whose launch
-    shapes are fixed).
+    and the brick-size threshold of the record grouping for meshes like
+    ``meshes`` (an addition to the reference API, whose launch shapes and
+    8x8x8 bricks are fixed).
 
     Runs ``mesh_to_flexible_dual_grid_batch(meshes, **batch_kwargs)`` with the
     current thresholds scaled by each of ``scales`` (one warm-up, then
     ``repeats`` timed conversions per candidate), first for the scanline
     thresholds (timing the scanline kernels), then for the face-test thresholds
-    (timing the face kernels), keeps the fastest of each on the device's
-    context and returns them with the measured times.  The outputs of a
-    conversion do not depend on the thresholds, only its launch shapes do.
+    (timing the face kernels), then for the faces-per-voxel threshold above
+    which a mesh's records are grouped in 4x4x4 instead of 8x8x8 bricks
+    (timing the record generation and fold kernels), keeps the fastest of each
+    on the device's context and returns them with the measured times.  The
+    outputs of a conversion do not depend on the thresholds, only its launch
+    shapes do.
     """
     meshes = list(meshes)
     assert len(meshes) > 0, "autotune_tile_classes needs at least one mesh"
     device = meshes[0][0].device
     assert device.type == "cuda", "autotune_tile_classes times device conversions: pass device tensors"
     mark0, face0 = _C.get_tile_classes(device)
+    brick0 = _C.get_brick_class(device)
     trials = []
 
-    def timed(slot):
+    def timed(slots):
         best = float("inf")
         mesh_to_flexible_dual_grid_batch(meshes, **batch_kwargs)
         for _ in range(max(1, int(repeats))):
             mesh_to_flexible_dual_grid_batch(meshes, **batch_kwargs)
-            best = min(best, _C.kernel_times(device).get(slot, float("inf")))
+            kt = _C.kernel_times(device)
+            best = min(best, sum(kt.get(s, 0.0) for s in slots) if any(s in kt for s in slots) else float("inf"))
         return best
 
     _C.set_timing(device, True)
     try:
         picks = {}
-        for kind, base, slot in (("mark_fpc", mark0, "mark"), ("face_vpf", face0, "face")):
+        for kind, base, slots in (("mark_fpc", mark0, ("mark",)), ("face_vpf", face0, ("face",))):
             best = None
             for sc in scales:
                 cand = [float(x) * float(sc) for x in base]
                 _C.set_tile_classes(device, **{kind: cand})
-                ms = timed(slot)
-                trials.append({kind: cand, slot + "_ms": ms})
+                ms = timed(slots)
+                trials.append({kind: cand, slots[0] + "_ms": ms})
                 if best is None or ms < best[0]:
                     best = (ms, cand)
             _C.set_tile_classes(device, **{kind: best[1]})
             picks[kind] = best
+        best = None
+        slots = ("gen_events", "gen_fhits", "gen_bhits", "active", "fold", "fold_large")
+        for sc in scales:
+            cand = float(brick0) * float(sc)
+            _C.set_brick_class(device, cand)
+            ms = timed(slots)
+            trials.append({"brick_fpv": cand, "fold_ms": ms})
+            if best is None or ms < best[0]:
+                best = (ms, cand)
+        _C.set_brick_class(device, best[1])
+        picks["brick_fpv"] = best
     finally:
         _C.set_timing(device, False)
     return {"mark_fpc": picks["mark_fpc"][1], "mark_ms": picks["mark_fpc"][0],
-            "face_vpf": picks["face_vpf"][1], "face_ms": picks["face_vpf"][0], "trials": trials}
+            "face_vpf": picks["face_vpf"][1], "face_ms": picks["face_vpf"][0],
+            "brick_fpv": picks["brick_fpv"][1], "fold_ms": picks["brick_fpv"][0], "trials": trials}
 
 
 def _hashable(x):

@@ -1,7 +1,8 @@
 """GPU: the launch-class thresholds of the scanline and face-test kernels
-change launch shapes only.  Every mesh forced into each extreme class gives
-the same outputs, bit for bit, as the default classes; the autotuner keeps one
-of its candidates on the context; invalid thresholds are rejected."""
+and the brick-size threshold of the record grouping change launch shapes
+only.  Every mesh forced into each extreme class gives the same outputs, bit
+for bit, as the default classes; the autotuner keeps one of its candidates on
+the context; invalid thresholds are rejected."""
 
 import numpy as np
 import pytest
@@ -14,6 +15,7 @@ pytestmark = pytest.mark.gpu
 AABB = [[-0.5, -0.5, -0.5], [0.5, 0.5, 0.5]]
 DEFAULT_MARK = [0.5, 2.0, 8.0]
 DEFAULT_FACE = [4.0, 16.0]
+DEFAULT_BRICK = 3.0
 
 
 def _dev():
@@ -53,7 +55,7 @@ EXTREMES = [
     ("all 256-item scanline tiles", [0.0, 0.0, 0.0], None),
     ("all 32-face tiles", None, [0.0, 0.0]),
     ("all 64-face tiles", None, [0.0, 1e30]),
-    ("all 128-face tiles", None, [1e30, 1e30]),
+    ("all one-lane-per-face tiles", None, [1e30, 1e30]),
 ]
 
 
@@ -70,8 +72,13 @@ def test_tile_classes_do_not_change_results():
             _C.set_tile_classes(dev, mark, face)
             _same(_run(ms, grids), want)
             _C.set_tile_classes(dev, DEFAULT_MARK, DEFAULT_FACE)
+        for fpv in (0.0, 1e30):
+            print(f"[case] every mesh in {'4x4x4' if fpv == 0.0 else '8x8x8'} bricks", flush=True)
+            _C.set_brick_class(dev, fpv)
+            _same(_run(ms, grids), want)
     finally:
         _C.set_tile_classes(dev, DEFAULT_MARK, DEFAULT_FACE)
+        _C.set_brick_class(dev, DEFAULT_BRICK)
 
 
 def test_autotune_keeps_a_candidate():
@@ -90,11 +97,15 @@ def test_autotune_keeps_a_candidate():
         marks = [[x * s for x in DEFAULT_MARK] for s in (0.5, 1.0, 2.0)]
         faces = [[x * s for x in DEFAULT_FACE] for s in (0.5, 1.0, 2.0)]
         assert res["mark_fpc"] in marks and res["face_vpf"] in faces
-        assert len(res["trials"]) == 6 and all(np.isfinite(t.get("mark_ms", t.get("face_ms"))) for t in res["trials"])
+        assert res["brick_fpv"] in [DEFAULT_BRICK * s for s in (0.5, 1.0, 2.0)]
+        assert len(res["trials"]) == 9
+        assert all(np.isfinite(t.get("mark_ms", t.get("face_ms", t.get("fold_ms")))) for t in res["trials"])
         assert _C.get_tile_classes(dev) == (res["mark_fpc"], res["face_vpf"])
+        assert _C.get_brick_class(dev) == res["brick_fpv"]
         _same(_run(ms, grids), want)
     finally:
         _C.set_tile_classes(dev, DEFAULT_MARK, DEFAULT_FACE)
+        _C.set_brick_class(dev, DEFAULT_BRICK)
 
 
 @pytest.mark.parametrize("mark,face", [([2.0, 1.0, 8.0], None), ([-1.0, 2.0, 8.0], None), (None, [4.0, 1.0]),
@@ -106,3 +117,13 @@ def test_invalid_tile_classes(mark, face):
     with pytest.raises(RuntimeError):
         _C.set_tile_classes(dev, mark, face)
     assert _C.get_tile_classes(dev) == (DEFAULT_MARK, DEFAULT_FACE)
+
+
+@pytest.mark.parametrize("fpv", [-1.0, float("nan"), float("inf")])
+def test_invalid_brick_class(fpv):
+    from o_voxel import _C
+
+    dev = _dev()
+    with pytest.raises(RuntimeError):
+        _C.set_brick_class(dev, fpv)
+    assert _C.get_brick_class(dev) == DEFAULT_BRICK
