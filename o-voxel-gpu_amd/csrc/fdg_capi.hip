@@ -113,6 +113,7 @@ struct ovx_fdg_ctx
     uint64_t per[SB_N] = {};       // per-shard capacities of the sharded buffers (grow-only)
     uint64_t force_per[SB_N] = {}; // one-shot overrides (tests, tuning)
     double face_per_face = 8.0;           // face records per face seen last time (capacity estimate)
+    double chunk_per_rec = 0.5;           // record chunks per record seen last time (capacity estimate)
     uint64_t n_events = 0;                // crossings recorded by the last mark pass
     int64_t E = 0;           // boundary edges of the last begin
     unsigned long long *h_ctr = nullptr; // pinned mirror of the counters
@@ -851,148 +852,82 @@ ovx_status choose_bricks(ovx_fdg_ctx *c, hipStream_t s)
     return OVX_OK;
 }
 
-ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *flags, hipStream_t s)
+// The device work of a conversion after its voxel set: coordinates, face
+// test, records, grouping, fold and solve.  Every buffer is sized before (no
+// allocation here) and nothing waits on the host, so the sequence can be
+// captured in a graph; buffers are cleared by fill kernels.  A record-stage
+// buffer that overflows sets CTR_OVER, and the grouping, fold and solve
+// kernels then return at once (the host sizes the buffers and runs it again).
+ovx_status enqueue_finish(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *flags, uint64_t grouped,
+                          hipStream_t s)
 {
     ovx_status st;
     const MeshDev &m = c->mesh;
     auto *ctr = (unsigned long long *)c->counters.p;
-    if ((st = choose_bricks(c, s)))
-        return st;
-    if (c->M > 1)
-    {
-        // the voxel guide, from the voxel offsets begin read back
-        const size_t g = c->stage_gd + GD_VOX * c->gd_bytes;
-        c->bdev.gsh[GD_VOX] = build_guide(c->h_vox_off, c->M, (int32_t *)(c->h_stage + g));
-        HIP_TRY(hipMemcpyAsync((unsigned char *)c->segbuf.p + g, c->h_stage + g, c->gd_bytes, hipMemcpyHostToDevice, s));
-    }
+    const uint64_t n_bricks = c->n_bricks;
+    const hipStream_t es = c->side;
+
     tic(c, KS_COORDS, s);
     HIP_TRY(launch_emit_coords(c->bdev, c->n_words, coords, s));
     toc(c, KS_COORDS, s);
-
-    // ---- records grouped by 8x8x8 brick (bricks of all meshes, numbered mesh after mesh)
-    const uint64_t n_bricks = c->n_bricks;
-    // phase-0 records are exactly 4 per crossing; face and boundary records are
-    // estimated from the previous conversion (an overflow is detected and the
-    // generation re-run with exact sizes)
-    const uint64_t face_est = (uint64_t)(c->face_per_face * 1.25 * (double)m.F) + 4096;
-    uint64_t want_rec = per_for(4ull * c->n_events + face_est + 16ull * (uint64_t)c->E);
-    uint64_t want_fhit = per_for(face_est);
-    uint64_t want_bhit = per_for(16ull * (uint64_t)c->E + 4096);
-    const uint64_t lim = 0xffffffffull / NSHARD; // record offsets and chunk indices are 32-bit
-    // at most 3 attempts: hits that overflowed are dropped, so the record
-    // counts of the attempt that first holds every hit are the exact ones
-    for (int attempt = 0; attempt < 3; ++attempt)
+    HIP_TRY(launch_fill_u64((unsigned long long *)c->shard_ctr.p + (uint64_t)SB_FHITS * NSHARD * SHARD_STRIDE,
+                            (uint64_t)(SB_N - SB_FHITS) * NSHARD * SHARD_STRIDE, 0ull, s));
+    HIP_TRY(launch_fill_u32(c->heads.p, n_bricks + 1, 0u, s));
+    BucketArgs ba{};
+    ba.recs = shard_view<Rec>(c, c->recs, SB_RECS);
+    ba.chunks = shard_view<Chunk>(c, c->chunks, SB_CHUNKS);
+    ba.brick_nch = (uint32_t *)c->heads.p;
+    ba.b = c->bdev;
+    ba.bedge_v = (const int2 *)c->bedge_v.p;
+    ba.ctr = ctr;
+    // records of the crossings and the boundary DDA on the side stream,
+    // beside the face test (which alone needs the main stream's order);
+    // the face test is queued first, so the side stream's host-side
+    // queueing does not hold it back
+    HIP_TRY(hipEventRecord(c->ev_fork, s)); // buffers cleared
+    const ShardBuf<FHit> fb = shard_view<FHit>(c, c->fhits, SB_FHITS);
+    if (c->fw > 0.0f)
     {
-        want_rec = std::min(want_rec, lim);
-        if ((st = ensure_shard(c, c->recs, SB_RECS, want_rec, sizeof(Rec))) ||
-            (st = ensure_shard(c, c->chunks, SB_CHUNKS, std::min(c->per[SB_RECS] / 2 + 256, lim), sizeof(Chunk))) ||
-            (st = ensure_shard(c, c->fhits, SB_FHITS, c->fw > 0.0f ? want_fhit : 1, sizeof(FHit))) ||
-            (st = ensure_shard(c, c->bhits, SB_BHITS, c->E > 0 ? want_bhit : 1, sizeof(BHit))) ||
-            (st = ensure_shard(c, c->faceq, SB_FACEQ, per_for(std::max<uint64_t>(1ull << 20, (uint64_t)m.F / 4)),
-                               sizeof(FaceChunk))) ||
-            (st = ensure(c->heads, (n_bricks + 1) * 4)))
+        tic(c, KS_FACE, s);
+        ShardBuf<FaceChunk> fq = shard_view<FaceChunk>(c, c->faceq, SB_FACEQ);
+        if ((st = launch_face_tiles(c, m, fb, ctr, fq, s)))
             return st;
-        if ((st = clear_shards(c, SB_FHITS, SB_N - SB_FHITS, s)))
-            return st;
-        HIP_TRY(hipMemsetAsync(c->heads.p, 0, (n_bricks + 1) * 4, s));
-        BucketArgs ba{};
-        ba.recs = shard_view<Rec>(c, c->recs, SB_RECS);
-        ba.chunks = shard_view<Chunk>(c, c->chunks, SB_CHUNKS);
-        ba.brick_nch = (uint32_t *)c->heads.p;
-        ba.b = c->bdev;
-        ba.bedge_v = (const int2 *)c->bedge_v.p;
-        ba.ctr = ctr;
-        // records of the crossings and the boundary DDA on the side stream,
-        // beside the face test (which alone needs the main stream's order);
-        // the face test is queued first, so the side stream's host-side
-        // queueing does not hold it back
-        if ((st = ensure_side(c)))
-            return st;
-        const hipStream_t es = c->side;
-        HIP_TRY(hipEventRecord(c->ev_fork, s)); // buffers cleared
-        const ShardBuf<FHit> fb = shard_view<FHit>(c, c->fhits, SB_FHITS);
-        if (c->fw > 0.0f)
-        {
-            tic(c, KS_FACE, s);
-            ShardBuf<FaceChunk> fq = shard_view<FaceChunk>(c, c->faceq, SB_FACEQ);
-            if ((st = launch_face_tiles(c, m, fb, ctr, fq, s)))
-                return st;
-            toc(c, KS_FACE, s);
-        }
-        HIP_TRY(hipStreamWaitEvent(es, c->ev_fork, 0));
-        tic(c, KS_GEN_EVENTS, es);
-        HIP_TRY(launch_gen_events(shard_view<Event>(c, c->events, SB_EVENTS), ba, es));
-        toc(c, KS_GEN_EVENTS, es);
-        if (c->E > 0)
-        {
-            const ShardBuf<BHit> hb = shard_view<BHit>(c, c->bhits, SB_BHITS);
-            tic(c, KS_BDRY, es);
-            HIP_TRY(launch_bdry(c->E, (const BEdge *)c->bedges.p, (const int2 *)c->bedge_v.p, c->bdev, hb, ctr, es));
-            toc(c, KS_BDRY, es);
-            tic(c, KS_GEN_BHITS, es);
-            HIP_TRY(launch_gen_bhits(hb, ba, es));
-            toc(c, KS_GEN_BHITS, es);
-        }
-        HIP_TRY(hipEventRecord(c->ev_join, es));
-        if (c->fw > 0.0f)
-        {
-            tic(c, KS_GEN_FHITS, s);
-            HIP_TRY(launch_gen_fhits(fb, ba, s));
-            toc(c, KS_GEN_FHITS, s);
-        }
-        HIP_TRY(hipStreamWaitEvent(s, c->ev_join, 0));
-        HIP_TRY(launch_shard_reduce((unsigned long long *)c->shard_ctr.p, ctr, s));
-        if ((st = read_counters(c, s)))
-            return st;
-        if (c->h_ctr[CTR_ERRORS])
-            return input_error(c->h_ctr[CTR_ERRORS]);
-        const unsigned long long *mx = c->h_ctr + CTR_MAX, *sum = c->h_ctr + CTR_SUM;
-        const bool hits_over = mx[SB_FHITS] > c->per[SB_FHITS] || mx[SB_BHITS] > c->per[SB_BHITS];
-        const bool over = hits_over || mx[SB_RECS] > c->per[SB_RECS] || mx[SB_CHUNKS] > c->per[SB_CHUNKS];
-        c->face_per_face = m.F ? (double)sum[SB_FHITS] / (double)m.F : 8.0;
-        if (!over)
-        {
-            c->stats[2] = (int64_t)sum[SB_RECS];
-            c->stats[3] = (int64_t)sum[SB_FHITS];
-            c->stats[5] = (int64_t)sum[SB_BHITS];
-            c->stats[9] = (int64_t)sum[SB_CHUNKS];
-            break;
-        }
-        if (sum[SB_RECS] >= 0xffffffffull || sum[SB_CHUNKS] >= 0xffffffffull)
-            return fail(OVX_ERR_CAPACITY, "%llu QEF records / %llu chunks exceed the 32-bit record index",
-                        (unsigned long long)sum[SB_RECS], (unsigned long long)sum[SB_CHUNKS]);
-        if (attempt == 2)
-            return fail(OVX_ERR_STATE, "record buffers overflowed three times (internal error)");
-        // sizes are known now: grow and generate again
-        want_fhit = mx[SB_FHITS] + mx[SB_FHITS] / 4 + 256;
-        want_bhit = mx[SB_BHITS] + mx[SB_BHITS] / 4 + 256;
-        want_rec = mx[SB_RECS] + mx[SB_RECS] / 4 + 256;
-        if (hits_over) // records of dropped hits were never requested: size from all the hits
-            want_rec = std::max(want_rec, per_for(4ull * c->n_events + sum[SB_FHITS] + sum[SB_BHITS]) * 2);
-        // a region's chunks never outnumber its records (both are claimed by the same block)
-        if (hits_over)
-            c->per[SB_CHUNKS] = std::max<uint64_t>(c->per[SB_CHUNKS], std::min(want_rec, lim));
-        else if (mx[SB_CHUNKS] > c->per[SB_CHUNKS])
-            c->per[SB_CHUNKS] = std::min<uint64_t>(mx[SB_CHUNKS] + mx[SB_CHUNKS] / 4 + 256, lim);
+        toc(c, KS_FACE, s);
     }
+    HIP_TRY(hipStreamWaitEvent(es, c->ev_fork, 0));
+    tic(c, KS_GEN_EVENTS, es);
+    HIP_TRY(launch_gen_events(shard_view<Event>(c, c->events, SB_EVENTS), ba, es));
+    toc(c, KS_GEN_EVENTS, es);
+    if (c->E > 0)
+    {
+        const ShardBuf<BHit> hb = shard_view<BHit>(c, c->bhits, SB_BHITS);
+        tic(c, KS_BDRY, es);
+        HIP_TRY(launch_bdry(c->E, (const BEdge *)c->bedges.p, (const int2 *)c->bedge_v.p, c->bdev, hb, ctr, es));
+        toc(c, KS_BDRY, es);
+        tic(c, KS_GEN_BHITS, es);
+        HIP_TRY(launch_gen_bhits(hb, ba, es));
+        toc(c, KS_GEN_BHITS, es);
+    }
+    HIP_TRY(hipEventRecord(c->ev_join, es));
+    if (c->fw > 0.0f)
+    {
+        tic(c, KS_GEN_FHITS, s);
+        HIP_TRY(launch_gen_fhits(fb, ba, s));
+        toc(c, KS_GEN_FHITS, s);
+    }
+    HIP_TRY(hipStreamWaitEvent(s, c->ev_join, 0));
+    HIP_TRY(launch_shard_reduce((unsigned long long *)c->shard_ctr.p, ctr, s));
+    const OverCaps caps{c->per[SB_FHITS], c->per[SB_BHITS], c->per[SB_RECS], c->per[SB_CHUNKS], grouped};
+    HIP_TRY(launch_over_check(ctr, caps, s));
 
     // ---- chunks grouped by brick, bricks with records, then the per-brick ordered fold + solve
-    const uint64_t n_chunks = c->h_ctr[CTR_SUM + SB_CHUNKS];
-    if ((st = ensure(c->choff, (n_bricks + 1) * 4)) || (st = ensure(c->bchunks, (n_chunks + 1) * 8)) ||
-        (st = ensure(c->active, n_bricks * 4)) || (st = ensure(c->large, n_bricks * 4)) ||
-        (st = ensure(c->huge, n_bricks * 4)) ||
-        // box-search queue: a few percent of the voxels typically; the solve
-        // kernel solves in place whatever does not fit
-        (st = ensure_shard(c, c->box, SB_BOX, per_for((uint64_t)c->N / 8 + 4096), sizeof(BoxItem))) ||
-        (st = ensure(c->qef, (size_t)c->N * (c->prec ? 80 : 40))) ||
-        (st = ensure(c->tiles, (scan_tile_count(n_bricks + 1) + 64) * 8)))
-        return st;
     tic(c, KS_ACTIVE, s);
     HIP_TRY(launch_scan_u32((const uint32_t *)c->heads.p, n_bricks + 1, (uint32_t *)c->choff.p, (uint64_t *)c->tiles.p,
                             ctr + CTR_CHUNK_SCAN, s));
     HIP_TRY(launch_place_chunks(shard_view<Chunk>(c, c->chunks, SB_CHUNKS), (const uint32_t *)c->choff.p,
-                                (uint2 *)c->bchunks.p, s));
-    HIP_TRY(launch_active_bricks((const uint32_t *)c->heads.p, n_bricks, (uint32_t *)c->active.p, ctr + CTR_ACTIVE, s));
+                                (uint2 *)c->bchunks.p, ctr + CTR_OVER, s));
+    HIP_TRY(launch_active_bricks((const uint32_t *)c->heads.p, n_bricks, (uint32_t *)c->active.p, ctr + CTR_ACTIVE,
+                                 ctr + CTR_OVER, s));
     toc(c, KS_ACTIVE, s);
     FoldArgs a{};
     a.choff = (const uint32_t *)c->choff.p;
@@ -1017,7 +952,8 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
     a.cond_max = c->cond_max;
     a.n64 = (unsigned long long *)c->shard_ctr.p + (uint64_t)SB_FP64 * NSHARD * SHARD_STRIDE;
     a.ctr = ctr;
-    // CTR_ACTIVE, CTR_LARGE, CTR_LARGE_RECS and CTR_HUGE were zeroed with all counters by begin
+    // CTR_ACTIVE, CTR_LARGE, CTR_LARGE_RECS and CTR_HUGE are zeroed with all counters by begin (an
+    // attempt whose record buffers overflow leaves them at zero)
     tic(c, KS_FOLD, s);
     HIP_TRY(launch_fold_bricks(a, n_bricks, s));
     toc(c, KS_FOLD, s);
@@ -1033,6 +969,99 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
     toc(c, KS_SOLVE_BOX, s);
     // totals of the queue regions, for the statistics
     HIP_TRY(launch_shard_reduce((unsigned long long *)c->shard_ctr.p, ctr, s));
+    return OVX_OK;
+}
+
+ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *flags, hipStream_t s)
+{
+    ovx_status st;
+    const MeshDev &m = c->mesh;
+    if ((st = choose_bricks(c, s)))
+        return st;
+    if (c->M > 1)
+    {
+        // the voxel guide, from the voxel offsets begin read back
+        const size_t g = c->stage_gd + GD_VOX * c->gd_bytes;
+        c->bdev.gsh[GD_VOX] = build_guide(c->h_vox_off, c->M, (int32_t *)(c->h_stage + g));
+        HIP_TRY(hipMemcpyAsync((unsigned char *)c->segbuf.p + g, c->h_stage + g, c->gd_bytes, hipMemcpyHostToDevice, s));
+    }
+    if ((st = ensure_side(c)))
+        return st;
+
+    // ---- records grouped by brick (bricks of all meshes, numbered mesh after mesh)
+    const uint64_t n_bricks = c->n_bricks;
+    // phase-0 records are exactly 4 per crossing; face and boundary records and
+    // the chunks are estimated from the previous conversion (an overflow is
+    // detected on the device and the record stages run again with exact sizes)
+    const uint64_t face_est = (uint64_t)(c->face_per_face * 1.25 * (double)m.F) + 4096;
+    const uint64_t rec_est = 4ull * c->n_events + face_est + 16ull * (uint64_t)c->E;
+    uint64_t want_rec = per_for(rec_est);
+    uint64_t want_fhit = per_for(face_est);
+    uint64_t want_bhit = per_for(16ull * (uint64_t)c->E + 4096);
+    uint64_t grouped = (uint64_t)(c->chunk_per_rec * 1.25 * (double)rec_est) + 65536;
+    const uint64_t lim = 0xffffffffull / NSHARD; // record offsets and chunk indices are 32-bit
+    // at most 3 attempts: hits that overflowed are dropped, so the record
+    // counts of the attempt that first holds every hit are the exact ones
+    for (int attempt = 0; attempt < 3; ++attempt)
+    {
+        want_rec = std::min(want_rec, lim);
+        if ((st = ensure_shard(c, c->recs, SB_RECS, want_rec, sizeof(Rec))) ||
+            (st = ensure_shard(c, c->chunks, SB_CHUNKS, std::min(c->per[SB_RECS] / 2 + 256, lim), sizeof(Chunk))) ||
+            (st = ensure_shard(c, c->fhits, SB_FHITS, c->fw > 0.0f ? want_fhit : 1, sizeof(FHit))) ||
+            (st = ensure_shard(c, c->bhits, SB_BHITS, c->E > 0 ? want_bhit : 1, sizeof(BHit))) ||
+            (st = ensure_shard(c, c->faceq, SB_FACEQ, per_for(std::max<uint64_t>(1ull << 20, (uint64_t)m.F / 4)),
+                               sizeof(FaceChunk))) ||
+            (st = ensure(c->heads, (n_bricks + 1) * 4)))
+            return st;
+        grouped = std::min<uint64_t>(grouped, c->per[SB_CHUNKS] * NSHARD);
+        if ((st = ensure(c->choff, (n_bricks + 1) * 4)) || (st = ensure(c->bchunks, (grouped + 1) * 8)) ||
+            (st = ensure(c->active, n_bricks * 4)) || (st = ensure(c->large, n_bricks * 4)) ||
+            (st = ensure(c->huge, n_bricks * 4)) ||
+            // box-search queue: a few percent of the voxels typically; the solve
+            // kernel solves in place whatever does not fit
+            (st = ensure_shard(c, c->box, SB_BOX, per_for((uint64_t)c->N / 8 + 4096), sizeof(BoxItem))) ||
+            (st = ensure(c->qef, (size_t)c->N * (c->prec ? 80 : 40))) ||
+            (st = ensure(c->tiles, (scan_tile_count(n_bricks + 1) + 64) * 8)))
+            return st;
+        if ((st = enqueue_finish(c, coords, dual, flags, grouped, s)))
+            return st;
+        // the conversion's one read-back of finish: errors, overflow, statistics
+        if ((st = read_counters(c, s)))
+            return st;
+        if (c->h_ctr[CTR_ERRORS])
+            return input_error(c->h_ctr[CTR_ERRORS]);
+        const unsigned long long *mx = c->h_ctr + CTR_MAX, *sum = c->h_ctr + CTR_SUM;
+        c->face_per_face = m.F ? (double)sum[SB_FHITS] / (double)m.F : 8.0;
+        if (!c->h_ctr[CTR_OVER])
+        {
+            c->stats[2] = (int64_t)sum[SB_RECS];
+            c->stats[3] = (int64_t)sum[SB_FHITS];
+            c->stats[5] = (int64_t)sum[SB_BHITS];
+            c->stats[9] = (int64_t)sum[SB_CHUNKS];
+            c->chunk_per_rec = sum[SB_RECS] ? (double)sum[SB_CHUNKS] / (double)sum[SB_RECS] : 0.5;
+            return OVX_OK;
+        }
+        const bool hits_over = mx[SB_FHITS] > c->per[SB_FHITS] || mx[SB_BHITS] > c->per[SB_BHITS];
+        if (sum[SB_RECS] >= 0xffffffffull || sum[SB_CHUNKS] >= 0xffffffffull)
+            return fail(OVX_ERR_CAPACITY, "%llu QEF records / %llu chunks exceed the 32-bit record index",
+                        (unsigned long long)sum[SB_RECS], (unsigned long long)sum[SB_CHUNKS]);
+        if (attempt == 2)
+            return fail(OVX_ERR_STATE, "record buffers overflowed three times (internal error)");
+        // sizes are known now: grow and generate again
+        want_fhit = mx[SB_FHITS] + mx[SB_FHITS] / 4 + 256;
+        want_bhit = mx[SB_BHITS] + mx[SB_BHITS] / 4 + 256;
+        want_rec = mx[SB_RECS] + mx[SB_RECS] / 4 + 256;
+        if (hits_over) // records of dropped hits were never requested: size from all the hits
+            want_rec = std::max(want_rec, per_for(4ull * c->n_events + sum[SB_FHITS] + sum[SB_BHITS]) * 2);
+        // a region's chunks never outnumber its records (both are claimed by the same block)
+        if (hits_over)
+            c->per[SB_CHUNKS] = std::max<uint64_t>(c->per[SB_CHUNKS], std::min(want_rec, lim));
+        else if (mx[SB_CHUNKS] > c->per[SB_CHUNKS])
+            c->per[SB_CHUNKS] = std::min<uint64_t>(mx[SB_CHUNKS] + mx[SB_CHUNKS] / 4 + 256, lim);
+        // all chunks requested by this attempt (dropped ones included), or as many as records when hits were dropped
+        grouped = hits_over ? std::max<uint64_t>(grouped, want_rec * NSHARD)
+                            : std::max<uint64_t>(grouped, sum[SB_CHUNKS] + sum[SB_CHUNKS] / 8 + 256);
+    }
     return OVX_OK;
 }
 

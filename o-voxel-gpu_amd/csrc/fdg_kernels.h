@@ -29,6 +29,7 @@ enum
 {
     CTR_ERRORS = 1,
     CTR_WORDS_TOTAL = 2, // total of the bitset popcount scan (= N)
+    CTR_OVER = 3,        // a record-stage buffer of the conversion overflowed (the fold and solve stages skip it)
     CTR_GIANT = 5,
     CTR_EDGE_TOTAL = 6,  // half-edge count scan total
     CTR_BEDGE_TOTAL = 7, // boundary edges
@@ -317,10 +318,23 @@ hipError_t launch_gen_events(const ShardBuf<Event> &ev, const BucketArgs &ba, hi
 hipError_t launch_gen_fhits(const ShardBuf<FHit> &h, const BucketArgs &ba, hipStream_t s);
 hipError_t launch_gen_bhits(const ShardBuf<BHit> &h, const BucketArgs &ba, hipStream_t s);
 hipError_t launch_shard_reduce(unsigned long long *shard_ctr, unsigned long long *ctr, hipStream_t s);
+// fills n 64-bit / 32-bit words with v (a kernel: capturable in a graph)
+hipError_t launch_fill_u64(void *p, uint64_t n, uint64_t v, hipStream_t s);
+hipError_t launch_fill_u32(void *p, uint64_t n, uint32_t v, hipStream_t s);
+// Sets ctr[CTR_OVER] when a record-stage buffer overflowed: the largest
+// region request of the face hits, boundary hits, records or chunks above its
+// per-region capacity, or the chunks above the grouped-chunk capacity.  Runs
+// after k_shard_reduce.
+struct OverCaps
+{
+    uint64_t fhits, bhits, recs, chunks; // per-region capacities
+    uint64_t grouped;                    // chunks the grouped list holds
+};
+hipError_t launch_over_check(unsigned long long *ctr, const OverCaps &caps, hipStream_t s);
 hipError_t launch_place_chunks(const ShardBuf<Chunk> &chunks, const uint32_t *choff, uint2 *gchunks,
-                               hipStream_t s);
+                               const unsigned long long *over, hipStream_t s);
 hipError_t launch_active_bricks(const uint32_t *nch, uint64_t n_bricks, uint32_t *list, unsigned long long *count,
-                                hipStream_t s);
+                                const unsigned long long *over, hipStream_t s);
 hipError_t launch_fold_bricks(const FoldArgs &a, uint64_t max_bricks, hipStream_t s);
 // bricks of the `large` list up to the mid-size tile; the rest go to `huge`
 hipError_t launch_fold_mid(const FoldArgs &a, uint64_t max_bricks, hipStream_t s);

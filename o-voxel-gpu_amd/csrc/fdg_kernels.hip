@@ -1893,9 +1893,12 @@ __global__ __launch_bounds__(BT) void k_shard_reduce(const unsigned long long *s
 }
 
 // chunks grouped by brick as (first record, count): position = first chunk of the brick + rank
-__global__ __launch_bounds__(BT) void k_place_chunks(ShardBuf<Chunk> chunks, const uint32_t *choff, uint2 *gchunks)
+__global__ __launch_bounds__(BT) void k_place_chunks(ShardBuf<Chunk> chunks, const uint32_t *choff, uint2 *gchunks,
+                                                     const unsigned long long *over)
 {
     __shared__ QueueMap qm;
+    if (*over)
+        return; // dropped chunks: the host sizes the buffers again and re-runs
     const uint64_t n = queue_prefix(chunks, qm.pre, qm.sh);
     const uint64_t stride = (uint64_t)gridDim.x * BT;
     for (uint64_t v = (uint64_t)blockIdx.x * BT + threadIdx.x; v < n; v += stride)
@@ -1909,8 +1912,10 @@ __global__ __launch_bounds__(BT) void k_place_chunks(ShardBuf<Chunk> chunks, con
 // consecutive bricks per thread and appends them with one atomic
 constexpr int ACT_PER = 8;
 __global__ __launch_bounds__(BT) void k_active_bricks(const uint32_t *nch, uint64_t n_bricks, uint32_t *list,
-                                                      unsigned long long *count)
+                                                      unsigned long long *count, const unsigned long long *over)
 {
+    if (*over)
+        return;
     __shared__ uint32_t sh[NWAVE];
     __shared__ unsigned long long sbase;
     const uint64_t span = (uint64_t)BT * ACT_PER;
@@ -2872,18 +2877,26 @@ __device__ __forceinline__ void fold_bricks_body(const FoldArgs &ain)
 // the kernels: the main tile for the reference path (6 waves per SIMD: its
 // LDS and at most 80 VGPRs leave room for 6 blocks per CU), the main tile for
 // the precision variant, and the mid-size tile
+// (a conversion whose record buffers overflowed skips the fold: the host
+// sizes the buffers again and re-runs its record stages)
 template <int P>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(5))) void k_fold_bricks(FoldArgs a)
 {
+    if (a.ctr[CTR_OVER])
+        return;
     fold_bricks_body<P, FOLD_SMALL, false>(a);
 }
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(FOLD_WAVES))) void k_fold_bricks_ref(FoldArgs a)
 {
+    if (a.ctr[CTR_OVER])
+        return;
     fold_bricks_body<0, FOLD_SMALL, false>(a);
 }
 template <int P>
 __global__ __launch_bounds__(BT) void k_fold_mid(FoldArgs a)
 {
+    if (a.ctr[CTR_OVER])
+        return;
     fold_bricks_body<P, FOLD_MID, true>(a);
 }
 
@@ -2896,6 +2909,8 @@ __global__ __launch_bounds__(BT) void k_fold_large(FoldArgs ain)
 {
     __shared__ FoldLds<FOLD_CAP> L;
     const int tid = threadIdx.x;
+    if (ain.ctr[CTR_OVER])
+        return;
     const uint64_t n_huge = *ain.n_huge;
 
     for (uint64_t bi = blockIdx.x; bi < n_huge; bi += gridDim.x)
@@ -3099,8 +3114,10 @@ __global__ __launch_bounds__(BT) void k_fold_large(FoldArgs ain)
 // region is full: the claim is all-or-nothing, so the queue holds only
 // complete items).
 __global__ __launch_bounds__(BT) void k_solve_voxels(const float *qef, const int32_t *coords, uint64_t n, BatchDev bt,
-                                                     float *dual, ShardBuf<BoxItem> box)
+                                                     float *dual, ShardBuf<BoxItem> box, const unsigned long long *over)
 {
+    if (*over)
+        return;
     const uint64_t stride = (uint64_t)gridDim.x * BT;
     const int lane = threadIdx.x & 63;
     const uint32_t shard = (blockIdx.x * NWAVE + (threadIdx.x >> 6)) % NSHARD;
@@ -3177,8 +3194,10 @@ __global__ __launch_bounds__(BT) void k_solve_voxels(const float *qef, const int
 // The voxels that took the FP64 path are counted.
 __global__ __launch_bounds__(BT) void k_solve_voxels_c(const float *qef, const int32_t *coords, uint64_t n,
                                                        BatchDev bt, float *dual, float cond_max,
-                                                       unsigned long long *n64)
+                                                       unsigned long long *n64, const unsigned long long *over)
 {
+    if (*over)
+        return;
     const uint64_t stride = (uint64_t)gridDim.x * BT;
     const int lane = threadIdx.x & 63;
     unsigned long long *cnt64 = n64 + (uint64_t)((blockIdx.x * NWAVE + (threadIdx.x >> 6)) % NSHARD) * SHARD_STRIDE;
@@ -3267,9 +3286,12 @@ __global__ __launch_bounds__(BT) void k_solve_voxels_c(const float *qef, const i
 }
 
 // box-constrained search for the queued voxels, one per lane
-__global__ __launch_bounds__(BT) void k_solve_box(ShardBuf<BoxItem> box, BatchDev bt, float *dual)
+__global__ __launch_bounds__(BT) void k_solve_box(ShardBuf<BoxItem> box, BatchDev bt, float *dual,
+                                                  const unsigned long long *over)
 {
     __shared__ QueueMap qm;
+    if (*over)
+        return;
     const uint64_t n = queue_prefix(box, qm.pre, qm.sh);
     const uint64_t stride = (uint64_t)gridDim.x * BT;
     for (uint64_t qi = (uint64_t)blockIdx.x * BT + threadIdx.x; qi < n; qi += stride)
@@ -3363,7 +3385,7 @@ hipError_t launch_face(const MeshDev &m, const BatchDev &b, int tile, uint64_t n
     const unsigned gf = (unsigned)(n_tiles < (1ull << 20) ? (n_tiles ? n_tiles : 1) : (1ull << 20));
     // row chunks of large faces: queue emptied, filled by the first launch,
     // consumed by the second
-    hipError_t e = hipMemsetAsync(fq.cnt, 0, (size_t)NSHARD * SHARD_STRIDE * sizeof(unsigned long long), s);
+    hipError_t e = launch_fill_u64(fq.cnt, (uint64_t)NSHARD * SHARD_STRIDE, 0ull, s);
     if (e != hipSuccess)
         return e;
     static_assert(FACE_TILES[0] == 32 && FACE_TILES[1] == 64 && FACE_TILES[FACE_LANE_CLASS] == BT,
@@ -3399,6 +3421,48 @@ hipError_t launch_gen_bhits(const ShardBuf<BHit> &h, const BucketArgs &ba, hipSt
     return hipGetLastError();
 }
 
+__global__ __launch_bounds__(BT) void k_fill64(uint64_t *p, uint64_t n, uint64_t v)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * BT;
+    for (uint64_t i = (uint64_t)blockIdx.x * BT + threadIdx.x; i < n; i += stride)
+        p[i] = v;
+}
+
+__global__ __launch_bounds__(BT) void k_fill32(uint32_t *p, uint64_t n, uint32_t v)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * BT;
+    for (uint64_t i = (uint64_t)blockIdx.x * BT + threadIdx.x; i < n; i += stride)
+        p[i] = v;
+}
+
+hipError_t launch_fill_u64(void *p, uint64_t n, uint64_t v, hipStream_t s)
+{
+    if (n)
+        hipLaunchKernelGGL(k_fill64, dim3(grid_for((int64_t)n, 4096)), dim3(BT), 0, s, (uint64_t *)p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(void *p, uint64_t n, uint32_t v, hipStream_t s)
+{
+    if (n)
+        hipLaunchKernelGGL(k_fill32, dim3(grid_for((int64_t)n, 4096)), dim3(BT), 0, s, (uint32_t *)p, n, v);
+    return hipGetLastError();
+}
+
+__global__ void k_over_check(unsigned long long *ctr, OverCaps caps)
+{
+    const unsigned long long *mx = ctr + CTR_MAX;
+    const bool over = mx[SB_FHITS] > caps.fhits || mx[SB_BHITS] > caps.bhits || mx[SB_RECS] > caps.recs ||
+                      mx[SB_CHUNKS] > caps.chunks || ctr[CTR_SUM + SB_CHUNKS] > caps.grouped;
+    ctr[CTR_OVER] = over ? 1ull : 0ull;
+}
+
+hipError_t launch_over_check(unsigned long long *ctr, const OverCaps &caps, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_over_check, dim3(1), dim3(1), 0, s, ctr, caps);
+    return hipGetLastError();
+}
+
 hipError_t launch_shard_reduce(unsigned long long *shard_ctr, unsigned long long *ctr, hipStream_t s)
 {
     static_assert(NSHARD == BT, "one thread per shard");
@@ -3407,17 +3471,18 @@ hipError_t launch_shard_reduce(unsigned long long *shard_ctr, unsigned long long
 }
 
 hipError_t launch_place_chunks(const ShardBuf<Chunk> &chunks, const uint32_t *choff, uint2 *gchunks,
-                               hipStream_t s)
+                               const unsigned long long *over, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_place_chunks, dim3(4096), dim3(BT), 0, s, chunks, choff, gchunks); // loops over the chunks
+    hipLaunchKernelGGL(k_place_chunks, dim3(4096), dim3(BT), 0, s, chunks, choff, gchunks,
+                       (const unsigned long long *)over); // loops over the chunks
     return hipGetLastError();
 }
 
 hipError_t launch_active_bricks(const uint32_t *nch, uint64_t n_bricks, uint32_t *list, unsigned long long *count,
-                                hipStream_t s)
+                                const unsigned long long *over, hipStream_t s)
 {
     hipLaunchKernelGGL(k_active_bricks, dim3(grid_for((int64_t)((n_bricks + ACT_PER - 1) / ACT_PER), 65536)), dim3(BT),
-                       0, s, nch, n_bricks, list, count);
+                       0, s, nch, n_bricks, list, count, over);
     return hipGetLastError();
 }
 
@@ -3450,17 +3515,18 @@ hipError_t launch_solve_voxels(const FoldArgs &a, const int32_t *coords, uint64_
     if (a.prec)
     {
         hipLaunchKernelGGL(k_solve_voxels_c, dim3(grid_for((int64_t)n, 65536)), dim3(BT), 0, s, (const float *)a.qef,
-                           coords, n, a.b, a.dual, a.cond_max, a.n64);
+                           coords, n, a.b, a.dual, a.cond_max, a.n64, (const unsigned long long *)(a.ctr + CTR_OVER));
         return hipGetLastError();
     }
     hipLaunchKernelGGL(k_solve_voxels, dim3(grid_for((int64_t)n, 65536)), dim3(BT), 0, s, (const float *)a.qef,
-                       coords, n, a.b, a.dual, a.box);
+                       coords, n, a.b, a.dual, a.box, (const unsigned long long *)(a.ctr + CTR_OVER));
     return hipGetLastError();
 }
 
 hipError_t launch_solve_box(const FoldArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_solve_box, dim3(2048), dim3(BT), 0, s, a.box, a.b, a.dual); // loops over the queue
+    hipLaunchKernelGGL(k_solve_box, dim3(2048), dim3(BT), 0, s, a.box, a.b, a.dual,
+                       (const unsigned long long *)(a.ctr + CTR_OVER)); // loops over the queue
     return hipGetLastError();
 }
 
