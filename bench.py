@@ -480,6 +480,24 @@ def run_batch(args, torch, o_voxel, _C, dev):
     res = measure("reference")
     batch_ms = res["batched_ms"]
     res_c = None if args.no_other_precision else measure("compensated")
+    # graph mode: the same batched steps with the device work after the voxel
+    # set replayed from HIP graphs (captured at the first step of a plan)
+    eager = [[t.clone() for t in o] for o in batched("reference")]
+    _C.set_graph(dev, True)
+    for _ in range(max(2, args.warmup)):
+        batched("reference")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out_g = batched("reference")
+    torch.cuda.synchronize()
+    g_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    gstats = _C.graph_stats(dev)
+    _C.set_graph(dev, False)
+    same_g = all(all(torch.equal(a, b) for a, b in zip(og, oe)) for og, oe in zip(out_g, eager))
+    del eager, out_g
+    graph = {"batched_ms": round(g_ms, 3), "eager_batched_ms": round(batch_ms, 3),
+             "gain_ms": round(batch_ms - g_ms, 3), "outputs_identical_to_eager": bool(same_g), **gstats}
     # per-kernel device times of one more batched step
     _C.set_timing(dev, True)
     batched("reference")
@@ -504,6 +522,7 @@ def run_batch(args, torch, o_voxel, _C, dev):
                    "weights": {"face": WEIGHTS[0], "boundary": WEIGHTS[1], "regularization": WEIGHTS[2]}},
         "batch": res,
         "batch_compensated": res_c,
+        "batch_graph": graph,
         "cold_ms": round(cold_ms, 3),
         "kernels_ms": {k: round(x, 4) for k, x in sorted(kt.items(), key=lambda kv: -kv[1]) if x >= 0},
         "counts": {k: v for k, v in st.items() if k != "workspace_bytes"},

@@ -11,10 +11,17 @@
  *     serialised by an internal lock; use one context per concurrent caller
  *     to run conversions in parallel.
  *   - A conversion is two calls: *_begin computes the occupied voxel set and
- *     returns its size N (one blocking readback); the caller allocates the
+ *     returns its size N (one blocking read-back); the caller allocates the
  *     [N,3] outputs and calls *_finish, which enqueues the rest on `stream`
- *     and returns without waiting.  The input buffers must stay valid until
- *     *_finish's work has completed on the stream.
+ *     and blocks once, at its end, on a read-back of the error and overflow
+ *     flags (when a record buffer overflowed it sizes the buffers exactly and
+ *     runs the record stages again, blocking once more).  The outputs are
+ *     complete when *_finish returns OVX_OK; the work of ovx_occ_finish,
+ *     ovx_dual_mesh_finish and the svo calls is only enqueued.
+ *   - Calls on one context may use different streams: a call's work waits
+ *     for the previous call's work on the context (an event recorded at the
+ *     end of every call), since they share the workspace.  Work on other
+ *     streams of the caller is never waited for.
  *
  * Drop-in boundary: these entry points are what the reference's extension
  * module binds for this path (src/ext.cpp:18-21), split into begin/finish so
@@ -173,6 +180,22 @@ ovx_status ovx_fdg_get_tile_classes(ovx_fdg_ctx *ctx, double *mark_fpc3, double 
  */
 ovx_status ovx_fdg_set_brick_class(ovx_fdg_ctx *ctx, double faces_per_voxel);
 ovx_status ovx_fdg_get_brick_class(ovx_fdg_ctx *ctx, double *faces_per_voxel);
+
+/*
+ * Graph mode for the next conversions on this context.  No counterpart in the
+ * reference (it launches every kernel from the host).  With enable != 0,
+ * ovx_fdg_finish enqueues its device work (every launch between the voxel
+ * set and the final read-back) as a HIP graph: captured the first time a plan
+ * occurs, replayed when a conversion has the same plan (the same mesh sizes,
+ * grids, weights, buffers and output pointers, whatever the vertex values).
+ * Up to 4 graphs are kept per context.  Outputs are identical to the ones
+ * without graphs.  Disabling releases the graphs.  Timing
+ * (ovx_fdg_set_timing) bypasses graphs.  Must not be called between begin and
+ * finish.  ovx_fdg_graph_stats: graph launches that replayed a kept graph,
+ * and graphs captured, since the context was created.
+ */
+ovx_status ovx_fdg_set_graph(ovx_fdg_ctx *ctx, int enable);
+ovx_status ovx_fdg_graph_stats(ovx_fdg_ctx *ctx, int64_t *replays, int64_t *captures);
 
 /*
  * Per-stage statistics of the last conversion on this context; the first

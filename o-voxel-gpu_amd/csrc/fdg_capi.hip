@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -158,6 +159,21 @@ struct ovx_fdg_ctx
     bool timing = false;
     hipEvent_t kev[2 * 24] = {};
     hipStream_t side = nullptr; // second stream (default priority): boundary edges beside the scanline stage, crossing and boundary records beside the face test
+    // graphs of finish's device work (ovx_fdg_set_graph): the plan key of
+    // each (every host-side input of the captured launches), its executable
+    // graph and last use; a stream of the context to capture on (the null
+    // stream cannot be captured)
+    struct GraphEntry
+    {
+        std::vector<uint64_t> key;
+        hipGraphExec_t exec = nullptr;
+        uint64_t used = 0;
+    };
+    bool graph_mode = false;
+    std::vector<GraphEntry> graphs;
+    hipStream_t cap = nullptr;
+    uint64_t graph_clock = 0;
+    int64_t graph_replays = 0, graph_captures = 0;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool kran[24] = {};
     DualMeshArgs dm{}; // dual grid -> mesh between begin and finish
@@ -208,6 +224,10 @@ struct StreamScope
     }
 };
 
+// generation of the workspaces: advances whenever a buffer is allocated or
+// released, so a graph captured over the old buffers is never replayed
+std::atomic<uint64_t> g_ws_gen{1};
+
 ovx_status ensure(DevBuf &b, size_t bytes)
 {
     if (bytes == 0)
@@ -222,6 +242,7 @@ ovx_status ensure(DevBuf &b, size_t bytes)
         b.bytes = 0;
     }
     const size_t want = bytes + bytes / 8 + 256;
+    g_ws_gen.fetch_add(1);
     HIP_TRY(hipMalloc(&b.p, want));
     b.bytes = want;
     return OVX_OK;
@@ -229,6 +250,7 @@ ovx_status ensure(DevBuf &b, size_t bytes)
 
 void release(DevBuf &b)
 {
+    g_ws_gen.fetch_add(1);
     if (b.p)
         (void)hipFree(b.p);
     b.p = nullptr;
@@ -554,8 +576,10 @@ ovx_status launch_mark(ovx_fdg_ctx *c, const MeshDev &m, unsigned long long *ctr
 // the face test: the face-tile size of each mesh from its voxels per face
 // (faces covering many voxels take small tiles, sub-voxel faces larger
 // ones); per-size tile offsets are uploaded, then one launch per size
-ovx_status launch_face_tiles(ovx_fdg_ctx *c, const MeshDev &m, const ShardBuf<FHit> &fb, unsigned long long *ctr,
-                             const ShardBuf<FaceChunk> &fq, hipStream_t s)
+// the face-test class of each mesh from its voxels per face; per-class tile
+// offsets staged and uploaded for a batch (the kernels of one mesh read only
+// entry 0 of its class's offsets, which is 0 in every class)
+ovx_status stage_face_tiles(ovx_fdg_ctx *c, hipStream_t s)
 {
     const int M = c->M;
     for (auto &v : c->ftile_off)
@@ -581,6 +605,14 @@ ovx_status launch_face_tiles(ovx_fdg_ctx *c, const MeshDev &m, const ShardBuf<FH
         HIP_TRY(hipMemcpyAsync((unsigned char *)c->segbuf.p + c->stage_ftile, c->h_stage + c->stage_ftile,
                                N_FACE_TILES * arr, hipMemcpyHostToDevice, s));
     }
+    return OVX_OK;
+}
+
+// the face-test launches of the classes present (offsets staged by stage_face_tiles)
+ovx_status launch_face_tiles(ovx_fdg_ctx *c, const MeshDev &m, const ShardBuf<FHit> &fb, unsigned long long *ctr,
+                             const ShardBuf<FaceChunk> &fq, hipStream_t s)
+{
+    const int M = c->M;
     for (int t = 0; t < N_FACE_TILES; ++t)
     {
         const uint64_t nt = c->ftile_off[t][M];
@@ -972,6 +1004,96 @@ ovx_status enqueue_finish(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t 
     return OVX_OK;
 }
 
+// Plan key of finish's device work: every host-side value its launches
+// depend on (sizes, buffer and output pointers, weights, capacities, the
+// staged per-mesh tables) and the workspace generation.  Equal keys enqueue
+// identical launches, so a graph captured for one is replayed for the other.
+std::vector<uint64_t> finish_key(ovx_fdg_ctx *c, const int32_t *coords, const float *dual, const uint8_t *flags,
+                                 uint64_t grouped)
+{
+    std::vector<uint64_t> k;
+    auto fbits = [](double d) {
+        uint64_t u;
+        std::memcpy(&u, &d, sizeof u);
+        return u;
+    };
+    const MeshDev &m = c->mesh;
+    k.insert(k.end(), {g_ws_gen.load(), (uint64_t)m.v, (uint64_t)m.f, (uint64_t)m.F, (uint64_t)m.V, (uint64_t)c->M,
+                       (uint64_t)c->N, c->n_words, c->n_bricks, c->n_events, (uint64_t)c->E, (uint64_t)c->edge_csr,
+                       fbits(c->fw), fbits(c->bw), fbits(c->rw), (uint64_t)c->prec, fbits(c->cond_max),
+                       (uint64_t)coords, (uint64_t)dual, (uint64_t)flags, grouped, fbits(c->face_vpf[0]),
+                       fbits(c->face_vpf[1])});
+    for (int i = 0; i < SB_N; ++i)
+        k.push_back(c->per[i]);
+    // the staging tables (segments, offsets, guides, face-test tiles, voxel offsets), byte for byte
+    const size_t n = c->h_stage_bytes;
+    k.push_back((uint64_t)n);
+    const size_t w = k.size();
+    k.resize(w + (n + 7) / 8, 0ull);
+    std::memcpy(k.data() + w, c->h_stage, n);
+    return k;
+}
+
+// Enqueues finish's device work on s through a graph: replayed when a graph
+// of the same plan key exists, otherwise captured on the context's capture
+// stream and instantiated first (at most GRAPHS_KEPT graphs are kept, the
+// least recently used one is dropped).
+constexpr size_t GRAPHS_KEPT = 4;
+ovx_status graph_finish(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *flags, uint64_t grouped,
+                        hipStream_t s)
+{
+    std::vector<uint64_t> key = finish_key(c, coords, dual, flags, grouped);
+    ++c->graph_clock;
+    for (auto &e : c->graphs)
+        if (e.key == key)
+        {
+            e.used = c->graph_clock;
+            ++c->graph_replays;
+            HIP_TRY(hipGraphLaunch(e.exec, s));
+            return OVX_OK;
+        }
+    if (!c->cap)
+        HIP_TRY(hipStreamCreateWithFlags(&c->cap, hipStreamNonBlocking));
+    // nothing runs during the capture; the graph is launched on s, after the work queued there
+    HIP_TRY(hipStreamBeginCapture(c->cap, hipStreamCaptureModeThreadLocal));
+    ovx_status st = enqueue_finish(c, coords, dual, flags, grouped, c->cap);
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(c->cap, &g);
+    if (st)
+    {
+        if (g)
+            (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (ee != hipSuccess)
+        return fail(OVX_ERR_HIP, "graph capture of the conversion failed: %s", hipGetErrorString(ee));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess)
+        return fail(OVX_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ie));
+    if (c->graphs.size() >= GRAPHS_KEPT)
+    {
+        size_t old = 0;
+        for (size_t i = 1; i < c->graphs.size(); ++i)
+            if (c->graphs[i].used < c->graphs[old].used)
+                old = i;
+        (void)hipGraphExecDestroy(c->graphs[old].exec);
+        c->graphs.erase(c->graphs.begin() + (long)old);
+    }
+    c->graphs.push_back(ovx_fdg_ctx::GraphEntry{std::move(key), ex, c->graph_clock});
+    ++c->graph_captures;
+    HIP_TRY(hipGraphLaunch(ex, s));
+    return OVX_OK;
+}
+
+void drop_graphs(ovx_fdg_ctx *c)
+{
+    for (auto &e : c->graphs)
+        (void)hipGraphExecDestroy(e.exec);
+    c->graphs.clear();
+}
+
 ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *flags, hipStream_t s)
 {
     ovx_status st;
@@ -986,6 +1108,8 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
         HIP_TRY(hipMemcpyAsync((unsigned char *)c->segbuf.p + g, c->h_stage + g, c->gd_bytes, hipMemcpyHostToDevice, s));
     }
     if ((st = ensure_side(c)))
+        return st;
+    if (c->fw > 0.0f && (st = stage_face_tiles(c, s)))
         return st;
 
     // ---- records grouped by brick (bricks of all meshes, numbered mesh after mesh)
@@ -1014,7 +1138,12 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
             (st = ensure(c->heads, (n_bricks + 1) * 4)))
             return st;
         grouped = std::min<uint64_t>(grouped, c->per[SB_CHUNKS] * NSHARD);
-        if ((st = ensure(c->choff, (n_bricks + 1) * 4)) || (st = ensure(c->bchunks, (grouped + 1) * 8)) ||
+        if ((st = ensure(c->bchunks, (grouped + 1) * 8)))
+            return st;
+        // the whole grouped-chunk buffer is usable: its capacity (stable until it grows) keeps the plan of
+        // repeated conversions the same while the chunk count varies with the order of the block claims
+        grouped = std::min<uint64_t>(c->bchunks.bytes / 8 - 1, c->per[SB_CHUNKS] * NSHARD);
+        if ((st = ensure(c->choff, (n_bricks + 1) * 4)) ||
             (st = ensure(c->active, n_bricks * 4)) || (st = ensure(c->large, n_bricks * 4)) ||
             (st = ensure(c->huge, n_bricks * 4)) ||
             // box-search queue: a few percent of the voxels typically; the solve
@@ -1023,7 +1152,12 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
             (st = ensure(c->qef, (size_t)c->N * (c->prec ? 80 : 40))) ||
             (st = ensure(c->tiles, (scan_tile_count(n_bricks + 1) + 64) * 8)))
             return st;
-        if ((st = enqueue_finish(c, coords, dual, flags, grouped, s)))
+        // first attempt through a graph when enabled (timing events stay outside graphs)
+        if (c->graph_mode && !c->timing && attempt == 0)
+            st = graph_finish(c, coords, dual, flags, grouped, s);
+        else
+            st = enqueue_finish(c, coords, dual, flags, grouped, s);
+        if (st)
             return st;
         // the conversion's one read-back of finish: errors, overflow, statistics
         if ((st = read_counters(c, s)))
@@ -1109,6 +1243,9 @@ ovx_status ovx_fdg_destroy(ovx_fdg_ctx *c)
             (void)hipEventSynchronize(c->ev_done);
         c->for_each_buf([](DevBuf &b) { release(b); });
         release(c->counters);
+        drop_graphs(c);
+        if (c->cap)
+            (void)hipStreamDestroy(c->cap);
         if (c->ev_done)
             (void)hipEventDestroy(c->ev_done);
         if (c->h_ctr)
@@ -1140,6 +1277,7 @@ ovx_status ovx_fdg_trim(ovx_fdg_ctx *c)
     std::lock_guard<std::mutex> lk(c->mu);
     if (c->done_recorded)
         HIP_TRY(hipEventSynchronize(c->ev_done));
+    drop_graphs(c);
     c->for_each_buf([](DevBuf &b) { release(b); });
     for (int i = 0; i < SB_N; ++i)
         c->per[i] = 0;
@@ -1378,6 +1516,34 @@ ovx_status ovx_fdg_get_brick_class(ovx_fdg_ctx *c, double *faces_per_voxel)
         return fail(OVX_ERR_INVALID_ARGUMENT, "null argument");
     std::lock_guard<std::mutex> lk(c->mu);
     *faces_per_voxel = c->brick_fpv;
+    return OVX_OK;
+}
+
+ovx_status ovx_fdg_set_graph(ovx_fdg_ctx *c, int enable)
+{
+    if (!c)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "null context");
+    DeviceGuard dg(c->device);
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (c->pending)
+        return fail(OVX_ERR_STATE, "graph mode changed between begin and finish");
+    c->graph_mode = enable != 0;
+    if (!c->graph_mode)
+    {
+        if (c->done_recorded)
+            HIP_TRY(hipEventSynchronize(c->ev_done));
+        drop_graphs(c);
+    }
+    return OVX_OK;
+}
+
+ovx_status ovx_fdg_graph_stats(ovx_fdg_ctx *c, int64_t *replays, int64_t *captures)
+{
+    if (!c || !replays || !captures)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    *replays = c->graph_replays;
+    *captures = c->graph_captures;
     return OVX_OK;
 }
 

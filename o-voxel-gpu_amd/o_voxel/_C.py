@@ -37,6 +37,8 @@ __all__ = [
     "set_capacity",
     "set_brick_class",
     "get_brick_class",
+    "set_graph",
+    "graph_stats",
     "workspace_bytes",
     "flexible_dual_grid_to_mesh_cuda",
     "hashmap_insert_cuda",
@@ -120,6 +122,10 @@ def _load():
         L.ovx_fdg_set_brick_class.restype = ctypes.c_int
         L.ovx_fdg_get_brick_class.argtypes = [vp, f64p]
         L.ovx_fdg_get_brick_class.restype = ctypes.c_int
+        L.ovx_fdg_set_graph.argtypes = [vp, ctypes.c_int]
+        L.ovx_fdg_set_graph.restype = ctypes.c_int
+        L.ovx_fdg_graph_stats.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+        L.ovx_fdg_graph_stats.restype = ctypes.c_int
         L.ovx_fdg_set_capacity.restype = ctypes.c_int
         L.ovx_fdg_kernel_count.restype = ctypes.c_int
         L.ovx_fdg_kernel_name.argtypes = [ctypes.c_int]
@@ -1002,6 +1008,27 @@ def get_brick_class(device=None) -> float:
     with torch.cuda.device(dev), ctx.lock:
         _check(_load().ovx_fdg_get_brick_class(ctx.h, ctypes.byref(out)), "get_brick_class")
     return float(out.value)
+
+
+def set_graph(device=None, enable: bool = True):
+    """Graph mode of ``device``'s context (ovx_fdg_set_graph): each
+    conversion's device work after the voxel set is captured as a HIP graph
+    the first time its plan occurs and replayed for conversions of the same
+    plan.  Outputs do not change."""
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    ctx = _ctx(dev)
+    with torch.cuda.device(dev), ctx.lock:
+        _check(_load().ovx_fdg_set_graph(ctx.h, int(bool(enable))), "set_graph")
+
+
+def graph_stats(device=None) -> dict:
+    """{"replays": graph launches of a kept graph, "captures": graphs captured} on ``device``."""
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    ctx = _ctx(dev)
+    r, c = ctypes.c_int64(0), ctypes.c_int64(0)
+    with torch.cuda.device(dev), ctx.lock:
+        _check(_load().ovx_fdg_graph_stats(ctx.h, ctypes.byref(r), ctypes.byref(c)), "graph_stats")
+    return {"replays": int(r.value), "captures": int(c.value)}
 
 
 def set_capacity(device=None, events: int = 0, hits: int = 0):
