@@ -49,6 +49,7 @@ struct DevBuf
 {
     void *p = nullptr;
     size_t bytes = 0;
+    bool pooled = false; // stream-ordered allocation (freed stream-ordered too)
 };
 
 // restores the caller's current device on scope exit
@@ -110,11 +111,17 @@ struct ovx_fdg_ctx
     int svo_state = 0;            // 1 begun, 2 written
     DevBuf vcnt, voff, nbr, erank, bcnt, boff, bedge_v, bedges, eslots;
     bool edge_csr = false; // the last conversion's boundary edges came from the CSR lists (a vertex overflowed its slots)
-    DevBuf events, recs, chunks, heads, choff, bchunks, active, large, huge, box, qef, bhits, fhits, shard_ctr, rowq, faceq;
+    DevBuf events, recs, chunks, heads, choff, bchunks, active, large, huge, shard_ctr, rowq, faceq;
+    // face and boundary hits, and, once the records are made from them, the
+    // per-voxel QEFs and the box-search queue in the same bytes
+    DevBuf hitqef;
+    size_t off_bhits = 0, off_box = 0; // byte offsets inside hitqef
     uint64_t per[SB_N] = {};       // per-shard capacities of the sharded buffers (grow-only)
     uint64_t force_per[SB_N] = {}; // one-shot overrides (tests, tuning)
-    double face_per_face = 8.0;           // face records per face seen last time (capacity estimate)
-    double chunk_per_rec = 0.5;           // record chunks per record seen last time (capacity estimate)
+    double face_hit_ratio = 1.0;          // face hits of the last conversion over the prep's estimate (capacity estimate)
+    double bhit_per_edge = 16.0;          // boundary hits per boundary edge seen last time (capacity estimate)
+    double chunk_per_rec = 0.1;           // record chunks per record (decaying maximum of the conversions seen; capacity estimate)
+    uint64_t face_geo = 0;                // face-test hits expected from the faces' sizes (prep of the last begin)
     uint64_t n_events = 0;                // crossings recorded by the last mark pass
     int64_t E = 0;           // boundary edges of the last begin
     unsigned long long *h_ctr = nullptr; // pinned mirror of the counters
@@ -184,7 +191,7 @@ struct ovx_fdg_ctx
     {
         DevBuf *all[] = {&planes, &tris, &bits,   &woff,  &tiles,   &vcnt,   &voff,  &nbr, &erank, &bcnt, &boff, &eslots,
                          &bedge_v, &bedges, &events, &recs,   &chunks, &heads, &choff, &bchunks, &active, &large, &huge,
-                         &box,     &qef,    &bhits,  &fhits,  &shard_ctr, &rowq, &faceq, &segbuf,
+                         &hitqef,  &shard_ctr, &rowq, &faceq, &segbuf,
                          &dm_table, &dm_cnt, &dm_off, &dm_mask, &dm_tiles, &dm_direct, &dm_touched,
                          &svo_insf, &svo_k, &svo_pos, &svo_tiles, &svo_rows, &svo_erow, &nb_table, &nb_own,
                          &nb_nb, &nb_acc, &nb_jmp};
@@ -195,6 +202,13 @@ struct ovx_fdg_ctx
 
 namespace
 {
+
+// Stream of the API call in progress on this thread (set by StreamScope):
+// workspace growth inside a call is stream-ordered on it, so a buffer is
+// freed after the work already queued on the context and no other stream
+// of the device waits.
+thread_local hipStream_t tl_ws_stream = nullptr;
+thread_local bool tl_ws_stream_set = false;
 
 // Every call on a context shares its workspace, whatever stream it runs on:
 // a call's first enqueue waits for the end of the previous call's work when
@@ -209,6 +223,8 @@ struct StreamScope
     hipError_t enter()
     {
         active = true;
+        tl_ws_stream = s;
+        tl_ws_stream_set = true;
         if (c->done_recorded && s != c->last_stream)
             return hipStreamWaitEvent(s, c->ev_done, 0);
         return hipSuccess;
@@ -217,6 +233,8 @@ struct StreamScope
     {
         if (!active)
             return;
+        tl_ws_stream_set = false;
+        tl_ws_stream = nullptr;
         if (hipEventRecord(c->ev_done, s) == hipSuccess)
             c->done_recorded = true;
         c->last_stream = s;
@@ -228,33 +246,51 @@ struct StreamScope
 // released, so a graph captured over the old buffers is never replayed
 std::atomic<uint64_t> g_ws_gen{1};
 
-ovx_status ensure(DevBuf &b, size_t bytes)
+// Grows b to at least `bytes` (grow-only).  `slack` adds 1/8 for buffers
+// whose sizes creep up from call to call; buffers sized with their own
+// margin take none.
+ovx_status ensure(DevBuf &b, size_t bytes, bool slack = true)
 {
     if (bytes == 0)
         bytes = 256;
     if (b.bytes >= bytes)
         return OVX_OK;
+    g_ws_gen.fetch_add(1);
     if (b.p)
     {
-        HIP_TRY(hipDeviceSynchronize());
-        HIP_TRY(hipFree(b.p));
+        if (b.pooled)
+            HIP_TRY(hipFreeAsync(b.p, tl_ws_stream));
+        else
+            HIP_TRY(hipFree(b.p));
         b.p = nullptr;
         b.bytes = 0;
     }
-    const size_t want = bytes + bytes / 8 + 256;
-    g_ws_gen.fetch_add(1);
-    HIP_TRY(hipMalloc(&b.p, want));
+    const size_t want = (slack ? bytes + bytes / 8 : bytes) + 256;
+    if (tl_ws_stream_set)
+    {
+        HIP_TRY(hipMallocAsync(&b.p, want, tl_ws_stream));
+        b.pooled = true;
+    }
+    else
+    {
+        HIP_TRY(hipMalloc(&b.p, want));
+        b.pooled = false;
+    }
     b.bytes = want;
     return OVX_OK;
 }
 
+// Frees b; callers have waited for the context's work (stream-ordered
+// buffers are freed on the null stream, which has nothing of it left to
+// wait for).
 void release(DevBuf &b)
 {
     g_ws_gen.fetch_add(1);
     if (b.p)
-        (void)hipFree(b.p);
+        (void)(b.pooled ? hipFreeAsync(b.p, nullptr) : hipFree(b.p));
     b.p = nullptr;
     b.bytes = 0;
+    b.pooled = false;
 }
 
 int64_t held_bytes(ovx_fdg_ctx *c)
@@ -365,13 +401,17 @@ void toc(ovx_fdg_ctx *c, int k, hipStream_t s)
 
 // ShardBuf view of a buffer of `per` items per shard
 template <typename T>
-ShardBuf<T> shard_view(ovx_fdg_ctx *c, DevBuf &b, int sb)
+ShardBuf<T> shard_view(ovx_fdg_ctx *c, DevBuf &b, int sb, size_t off = 0)
 {
-    return ShardBuf<T>{(T *)b.p, c->per[sb], (unsigned long long *)c->shard_ctr.p + (uint64_t)sb * NSHARD * SHARD_STRIDE};
+    return ShardBuf<T>{(T *)((unsigned char *)b.p + off), c->per[sb],
+                       (unsigned long long *)c->shard_ctr.p + (uint64_t)sb * NSHARD * SHARD_STRIDE};
 }
 
 // per-shard capacity for `total` items spread over the shards, with headroom
-uint64_t per_for(uint64_t total) { return (total + total / 4) / NSHARD + 256; }
+// (1/margin above the shard mean: 4 -> 25%)
+uint64_t per_for(uint64_t total, int margin = 4) { return (total + total / margin) / NSHARD + 256; }
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 // the side stream (default priority) and its fork / join events, made once per context
 ovx_status ensure_side(ovx_fdg_ctx *c)
@@ -384,7 +424,9 @@ ovx_status ensure_side(ovx_fdg_ctx *c)
     return OVX_OK;
 }
 
-ovx_status ensure_shard(ovx_fdg_ctx *c, DevBuf &b, int sb, uint64_t want_per, size_t item)
+// Sets the per-shard capacity of sharded buffer sb (grow-only, or a one-shot
+// override) and returns the bytes of its regions.
+size_t size_shard(ovx_fdg_ctx *c, int sb, uint64_t want_per, size_t item)
 {
     if (c->force_per[sb])
     {
@@ -394,7 +436,13 @@ ovx_status ensure_shard(ovx_fdg_ctx *c, DevBuf &b, int sb, uint64_t want_per, si
     }
     else if (want_per > c->per[sb])
         c->per[sb] = want_per;
-    return ensure(b, (size_t)c->per[sb] * NSHARD * item);
+    return (size_t)c->per[sb] * NSHARD * item;
+}
+
+// a sharded buffer of its own (its capacity carries its own margin)
+ovx_status ensure_shard(ovx_fdg_ctx *c, DevBuf &b, int sb, uint64_t want_per, size_t item)
+{
+    return ensure(b, size_shard(c, sb, want_per, item), false);
 }
 
 ovx_status clear_shards(ovx_fdg_ctx *c, int sb0, int nsb, hipStream_t s)
@@ -707,16 +755,18 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m_in, int M, const int64_
         edges_forked = true;
         c->edge_csr = false;
     }
+    if ((st = ensure(c->shard_ctr, (size_t)SB_N * NSHARD * SHARD_STRIDE * sizeof(unsigned long long))) ||
+        (st = clear_shards(c, SB_FACE_EST, 1, s)))
+        return st;
     tic(c, KS_PREP, s);
     HIP_TRY(hipMemsetAsync(c->bits.p, 0, n_words * sizeof(uint64_t), s));
-    HIP_TRY(launch_prep(m, c->bdev, planes ? (float4 *)c->planes.p : nullptr, tris, ctr, s));
+    HIP_TRY(launch_prep(m, c->bdev, planes ? (float4 *)c->planes.p : nullptr, tris, ctr,
+                        (unsigned long long *)c->shard_ctr.p + (uint64_t)SB_FACE_EST * NSHARD * SHARD_STRIDE, s));
     toc(c, KS_PREP, s);
     m.tri = tris;
     m.pl = planes ? (const float4 *)c->planes.p : nullptr;
     c->mesh = m;
     // crossings are recorded once, here, when the buffer can hold them
-    if ((st = ensure(c->shard_ctr, (size_t)SB_N * NSHARD * SHARD_STRIDE * sizeof(unsigned long long))))
-        return st;
     ShardBuf<Event> eb{};
     if (planes)
     {
@@ -774,6 +824,7 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m_in, int M, const int64_
     if (N >= (1ull << 32) - 1)
         return fail(OVX_ERR_CAPACITY, "%llu occupied voxels exceed the 32-bit voxel index", N);
     c->n_events = planes ? c->h_ctr[CTR_SUM + SB_EVENTS] : 0;
+    c->face_geo = planes ? c->h_ctr[CTR_SUM + SB_FACE_EST] : 0;
     if (planes)
     {
         if (c->h_ctr[CTR_MAX + SB_EVENTS] > c->per[SB_EVENTS])
@@ -917,7 +968,7 @@ ovx_status enqueue_finish(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t 
     // the face test is queued first, so the side stream's host-side
     // queueing does not hold it back
     HIP_TRY(hipEventRecord(c->ev_fork, s)); // buffers cleared
-    const ShardBuf<FHit> fb = shard_view<FHit>(c, c->fhits, SB_FHITS);
+    const ShardBuf<FHit> fb = shard_view<FHit>(c, c->hitqef, SB_FHITS);
     if (c->fw > 0.0f)
     {
         tic(c, KS_FACE, s);
@@ -932,7 +983,7 @@ ovx_status enqueue_finish(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t 
     toc(c, KS_GEN_EVENTS, es);
     if (c->E > 0)
     {
-        const ShardBuf<BHit> hb = shard_view<BHit>(c, c->bhits, SB_BHITS);
+        const ShardBuf<BHit> hb = shard_view<BHit>(c, c->hitqef, SB_BHITS, c->off_bhits);
         tic(c, KS_BDRY, es);
         HIP_TRY(launch_bdry(c->E, (const BEdge *)c->bedges.p, (const int2 *)c->bedge_v.p, c->bdev, hb, ctr, es));
         toc(c, KS_BDRY, es);
@@ -978,8 +1029,10 @@ ovx_status enqueue_finish(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t 
     a.flags = flags;
     a.bw = c->bw;
     a.rw = c->rw;
-    a.qef = (float *)c->qef.p;
-    a.box = shard_view<BoxItem>(c, c->box, SB_BOX);
+    // the hits are all made into records by now (the side stream is joined):
+    // their bytes hold the QEFs and the box queue
+    a.qef = (float *)c->hitqef.p;
+    a.box = shard_view<BoxItem>(c, c->hitqef, SB_BOX, c->off_box);
     a.prec = c->prec;
     a.cond_max = c->cond_max;
     a.n64 = (unsigned long long *)c->shard_ctr.p + (uint64_t)SB_FP64 * NSHARD * SHARD_STRIDE;
@@ -1022,7 +1075,7 @@ std::vector<uint64_t> finish_key(ovx_fdg_ctx *c, const int32_t *coords, const fl
                        (uint64_t)c->N, c->n_words, c->n_bricks, c->n_events, (uint64_t)c->E, (uint64_t)c->edge_csr,
                        fbits(c->fw), fbits(c->bw), fbits(c->rw), (uint64_t)c->prec, fbits(c->cond_max),
                        (uint64_t)coords, (uint64_t)dual, (uint64_t)flags, grouped, fbits(c->face_vpf[0]),
-                       fbits(c->face_vpf[1])});
+                       fbits(c->face_vpf[1]), (uint64_t)c->off_bhits, (uint64_t)c->off_box});
     for (int i = 0; i < SB_N; ++i)
         k.push_back(c->per[i]);
     // the staging tables (segments, offsets, guides, face-test tiles, voxel offsets), byte for byte
@@ -1114,15 +1167,23 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
 
     // ---- records grouped by brick (bricks of all meshes, numbered mesh after mesh)
     const uint64_t n_bricks = c->n_bricks;
-    // phase-0 records are exactly 4 per crossing; face and boundary records and
-    // the chunks are estimated from the previous conversion (an overflow is
-    // detected on the device and the record stages run again with exact sizes)
-    const uint64_t face_est = (uint64_t)(c->face_per_face * 1.25 * (double)m.F) + 4096;
-    const uint64_t rec_est = 4ull * c->n_events + face_est + 16ull * (uint64_t)c->E;
-    uint64_t want_rec = per_for(rec_est);
-    uint64_t want_fhit = per_for(face_est);
-    uint64_t want_bhit = per_for(16ull * (uint64_t)c->E + 4096);
-    uint64_t grouped = (uint64_t)(c->chunk_per_rec * 1.25 * (double)rec_est) + 65536;
+    // phase-0 records are exactly 4 per crossing; face hits are estimated from
+    // the faces' sizes (prep) scaled by the last conversion's hits over that
+    // estimate, boundary hits from the last conversion's hits per boundary
+    // edge, the chunks from its chunks per record.  Hit regions take 15%
+    // above the shard mean for the uneven spread of the claims; an overflow
+    // is detected on the device and the record stages run again with exact
+    // sizes.
+    const uint64_t face_est = (uint64_t)(c->face_hit_ratio * (double)c->face_geo) + 4096;
+    const uint64_t bhit_est = (uint64_t)(c->bhit_per_edge * (double)c->E) + 4096;
+    const uint64_t rec_est = 4ull * c->n_events + face_est + bhit_est;
+    uint64_t want_rec = per_for(rec_est, 8);
+    uint64_t want_fhit = per_for(face_est + face_est * 15 / 100);
+    uint64_t want_bhit = per_for(bhit_est + bhit_est * 15 / 100);
+    // chunks: the last conversion's chunks per record with a margin (they vary with the order of the claims)
+    const double cpr = std::min(1.0, std::max(0.02, c->chunk_per_rec) * 1.5);
+    uint64_t want_chunk = per_for((uint64_t)(cpr * (double)rec_est));
+    uint64_t grouped = (uint64_t)(cpr * (double)rec_est) + 65536;
     const uint64_t lim = 0xffffffffull / NSHARD; // record offsets and chunk indices are 32-bit
     // at most 3 attempts: hits that overflowed are dropped, so the record
     // counts of the attempt that first holds every hit are the exact ones
@@ -1130,9 +1191,7 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
     {
         want_rec = std::min(want_rec, lim);
         if ((st = ensure_shard(c, c->recs, SB_RECS, want_rec, sizeof(Rec))) ||
-            (st = ensure_shard(c, c->chunks, SB_CHUNKS, std::min(c->per[SB_RECS] / 2 + 256, lim), sizeof(Chunk))) ||
-            (st = ensure_shard(c, c->fhits, SB_FHITS, c->fw > 0.0f ? want_fhit : 1, sizeof(FHit))) ||
-            (st = ensure_shard(c, c->bhits, SB_BHITS, c->E > 0 ? want_bhit : 1, sizeof(BHit))) ||
+            (st = ensure_shard(c, c->chunks, SB_CHUNKS, std::min(want_chunk, lim), sizeof(Chunk))) ||
             (st = ensure_shard(c, c->faceq, SB_FACEQ, per_for(std::max<uint64_t>(1ull << 20, (uint64_t)m.F / 4)),
                                sizeof(FaceChunk))) ||
             (st = ensure(c->heads, (n_bricks + 1) * 4)))
@@ -1148,10 +1207,20 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
             (st = ensure(c->huge, n_bricks * 4)) ||
             // box-search queue: a few percent of the voxels typically; the solve
             // kernel solves in place whatever does not fit
-            (st = ensure_shard(c, c->box, SB_BOX, per_for((uint64_t)c->N / 8 + 4096), sizeof(BoxItem))) ||
-            (st = ensure(c->qef, (size_t)c->N * (c->prec ? 80 : 40))) ||
             (st = ensure(c->tiles, (scan_tile_count(n_bricks + 1) + 64) * 8)))
             return st;
+        {
+            // hits first, then (after the records are made) the QEFs and the box-search queue, which is a
+            // few percent of the voxels typically: the solve kernel solves in place whatever does not fit
+            const size_t fh = align_up(size_shard(c, SB_FHITS, c->fw > 0.0f ? want_fhit : 1, sizeof(FHit)), 256);
+            const size_t bh = size_shard(c, SB_BHITS, c->E > 0 ? want_bhit : 1, sizeof(BHit));
+            const size_t q = align_up((size_t)c->N * (c->prec ? 80 : 40), 256);
+            const size_t bx = size_shard(c, SB_BOX, per_for((uint64_t)c->N / 8 + 4096), sizeof(BoxItem));
+            if ((st = ensure(c->hitqef, std::max(fh + bh, q + bx), false)))
+                return st;
+            c->off_bhits = fh;
+            c->off_box = q;
+        }
         // first attempt through a graph when enabled (timing events stay outside graphs)
         if (c->graph_mode && !c->timing && attempt == 0)
             st = graph_finish(c, coords, dual, flags, grouped, s);
@@ -1165,14 +1234,19 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
         if (c->h_ctr[CTR_ERRORS])
             return input_error(c->h_ctr[CTR_ERRORS]);
         const unsigned long long *mx = c->h_ctr + CTR_MAX, *sum = c->h_ctr + CTR_SUM;
-        c->face_per_face = m.F ? (double)sum[SB_FHITS] / (double)m.F : 8.0;
         if (!c->h_ctr[CTR_OVER])
         {
+            if (c->fw > 0.0f && c->face_geo)
+                c->face_hit_ratio = std::min(4.0, std::max(0.05, (double)sum[SB_FHITS] / (double)c->face_geo));
+            if (c->E > 0)
+                c->bhit_per_edge = std::min(1.0e4, std::max(1.0, (double)sum[SB_BHITS] / (double)c->E));
             c->stats[2] = (int64_t)sum[SB_RECS];
             c->stats[3] = (int64_t)sum[SB_FHITS];
             c->stats[5] = (int64_t)sum[SB_BHITS];
             c->stats[9] = (int64_t)sum[SB_CHUNKS];
-            c->chunk_per_rec = sum[SB_RECS] ? (double)sum[SB_CHUNKS] / (double)sum[SB_RECS] : 0.5;
+            // a slowly decaying maximum: conversions of other meshes on the context do not overflow at once
+            if (sum[SB_RECS])
+                c->chunk_per_rec = std::max((double)sum[SB_CHUNKS] / (double)sum[SB_RECS], 0.9 * c->chunk_per_rec);
             return OVX_OK;
         }
         const bool hits_over = mx[SB_FHITS] > c->per[SB_FHITS] || mx[SB_BHITS] > c->per[SB_BHITS];
@@ -1182,19 +1256,24 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
         if (attempt == 2)
             return fail(OVX_ERR_STATE, "record buffers overflowed three times (internal error)");
         // sizes are known now: grow and generate again
-        want_fhit = mx[SB_FHITS] + mx[SB_FHITS] / 4 + 256;
-        want_bhit = mx[SB_BHITS] + mx[SB_BHITS] / 4 + 256;
-        want_rec = mx[SB_RECS] + mx[SB_RECS] / 4 + 256;
-        if (hits_over) // records of dropped hits were never requested: size from all the hits
-            want_rec = std::max(want_rec, per_for(4ull * c->n_events + sum[SB_FHITS] + sum[SB_BHITS]) * 2);
-        // a region's chunks never outnumber its records (both are claimed by the same block)
+        want_fhit = mx[SB_FHITS] + mx[SB_FHITS] / 8 + 256;
+        want_bhit = mx[SB_BHITS] + mx[SB_BHITS] / 8 + 256;
+        want_rec = mx[SB_RECS] + mx[SB_RECS] / 8 + 256;
         if (hits_over)
-            c->per[SB_CHUNKS] = std::max<uint64_t>(c->per[SB_CHUNKS], std::min(want_rec, lim));
-        else if (mx[SB_CHUNKS] > c->per[SB_CHUNKS])
-            c->per[SB_CHUNKS] = std::min<uint64_t>(mx[SB_CHUNKS] + mx[SB_CHUNKS] / 4 + 256, lim);
-        // all chunks requested by this attempt (dropped ones included), or as many as records when hits were dropped
-        grouped = hits_over ? std::max<uint64_t>(grouped, want_rec * NSHARD)
-                            : std::max<uint64_t>(grouped, sum[SB_CHUNKS] + sum[SB_CHUNKS] / 8 + 256);
+        {
+            // records of dropped hits were never requested: size records and
+            // chunks from all the hits (every hit was counted)
+            const uint64_t all = 4ull * c->n_events + sum[SB_FHITS] + sum[SB_BHITS];
+            want_rec = std::max(want_rec, per_for(all, 8));
+            want_chunk = std::max(want_chunk, per_for((uint64_t)(cpr * (double)all)));
+            grouped = std::max<uint64_t>(grouped, (uint64_t)(cpr * (double)all) + 65536);
+        }
+        else
+        {
+            // every chunk of this attempt was counted (dropped ones included)
+            want_chunk = std::max<uint64_t>(want_chunk, mx[SB_CHUNKS] + mx[SB_CHUNKS] / 4 + 256);
+            grouped = std::max<uint64_t>(grouped, sum[SB_CHUNKS] + sum[SB_CHUNKS] / 8 + 256);
+        }
     }
     return OVX_OK;
 }
@@ -1439,7 +1518,7 @@ ovx_status ovx_fdg_set_capacity(ovx_fdg_ctx *c, int64_t events, int64_t hits)
         }
         release(c->recs);
         release(c->chunks);
-        release(c->fhits);
+        release(c->hitqef);
     }
     return OVX_OK;
 }

@@ -59,7 +59,8 @@ enum
     SB_FP64 = 6, // counters only: voxels solved on the FP64 path (precision variant)
     SB_ROWQ = 7, // row chunks of large triangles (scanline)
     SB_FACEQ = 8, // row chunks of large triangles (face test)
-    SB_N = 9
+    SB_FACE_EST = 9, // counters only: face-test hits expected from the faces' sizes (prep; sizes the face hits)
+    SB_N = 10
 };
 
 // Append / allocation counters are split over NSHARD counters 128 bytes
@@ -306,8 +307,9 @@ struct FoldArgs
     unsigned long long *ctr;
 };
 
+// est_ctr: the NSHARD counters of SB_FACE_EST (cleared by the caller)
 hipError_t launch_prep(const MeshDev &m, const BatchDev &b, float4 *planes, float *tri, unsigned long long *ctr,
-                       hipStream_t s);
+                       unsigned long long *est_ctr, hipStream_t s);
 // tile: index into MARK_TILES; b.mtile_off must hold that tile size's offsets
 hipError_t launch_scan_events(const MeshDev &m, const BatchDev &b, int tile, uint64_t n_tiles, unsigned long long *ctr,
                               const ShardBuf<Event> &eb, const ShardBuf<RowChunk> &rq, bool merge, hipStream_t s);

@@ -337,16 +337,17 @@ __device__ __forceinline__ void bucket_emit(const uint32_t (&bk)[K], const Rec (
 // prep: validate inputs, compute per-face planes
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BT) void k_prep(MeshDev m, BatchDev bt, float4 *planes, float *tri,
-                                             unsigned long long *counters)
+                                             unsigned long long *counters, unsigned long long *est_ctr)
 {
     const int64_t stride = (int64_t)gridDim.x * BT;
     unsigned flags = 0;
+    unsigned long long est = 0; // face-test hits expected of this thread's faces
     for (int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x; i < m.F; i += stride)
     {
+        const int ms = mesh_of_face(bt, i);
         if (m.f)
         {
             // indices are batch-global: each must name a vertex of the face's own mesh
-            const int ms = mesh_of_face(bt, i);
             const int64_t v0 = bt.vert_off[ms], v1 = bt.vert_off[ms + 1];
             const int32_t *fi = m.f + 3 * i;
             const int64_t a = fi[0], b = fi[1], c = fi[2];
@@ -362,6 +363,29 @@ __global__ __launch_bounds__(BT) void k_prep(MeshDev m, BatchDev bt, float4 *pla
             load_tri(m, i, v0, v1, v2);
             tri_plane(v0, v1, v2, pl);
             planes[i] = make_float4(pl[0], pl[1], pl[2], pl[3]);
+            {
+                // voxels a triangle overlaps: about the sum of its areas
+                // projected on the three axis planes, half its perimeter
+                // measured along the axes, and one (all in voxel units); sizes
+                // the face-hit buffer, whose overflow is detected anyway
+                const GridDev &g = bt.seg[ms].g;
+                float e0[3], e1[3], e2[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                {
+                    e0[a] = div_vs(v1[a] - v0[a], g, a);
+                    e1[a] = div_vs(v2[a] - v1[a], g, a);
+                    e2[a] = div_vs(v0[a] - v2[a], g, a);
+                }
+                const float area = fabsf(e0[1] * e2[2] - e0[2] * e2[1]) + fabsf(e0[2] * e2[0] - e0[0] * e2[2]) +
+                                   fabsf(e0[0] * e2[1] - e0[1] * e2[0]);
+                float per = 0.0f;
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    per += fabsf(e0[a]) + fabsf(e1[a]) + fabsf(e2[a]);
+                const float x = 0.5f * (area + per) + 1.0f;
+                est += x < 1.0e9f ? (unsigned long long)x + 1ull : 1000000000ull;
+            }
             if (tri)
             {
                 // the gathered triangle, read coalesced by the later stages
@@ -394,6 +418,11 @@ __global__ __launch_bounds__(BT) void k_prep(MeshDev m, BatchDev bt, float4 *pla
     }
     if (flags)
         atomicOr(counters + CTR_ERRORS, (unsigned long long)flags);
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1)
+        est += __shfl_xor(est, d, 64);
+    if ((threadIdx.x & 63) == 0 && est)
+        atomicAdd(est_ctr + (uint64_t)((blockIdx.x * NWAVE + (threadIdx.x >> 6)) % NSHARD) * SHARD_STRIDE, est);
 }
 
 // ---------------------------------------------------------------------------
@@ -3321,12 +3350,12 @@ static inline unsigned grid_for(int64_t n, int64_t cap = 1 << 20)
 }
 
 hipError_t launch_prep(const MeshDev &m, const BatchDev &b, float4 *planes, float *tri, unsigned long long *ctr,
-                       hipStream_t s)
+                       unsigned long long *est_ctr, hipStream_t s)
 {
     const int64_t n = m.F > (m.f ? m.V : 3 * m.F) ? m.F : (m.f ? m.V : 3 * m.F);
     MeshDev mi = m;
     mi.tri = nullptr; // the prep kernel writes the gathered triangles
-    hipLaunchKernelGGL(k_prep, dim3(grid_for(n)), dim3(BT), 0, s, mi, b, planes, tri, ctr);
+    hipLaunchKernelGGL(k_prep, dim3(grid_for(n)), dim3(BT), 0, s, mi, b, planes, tri, ctr, est_ctr);
     return hipGetLastError();
 }
 
