@@ -117,6 +117,33 @@ ovx_status ovx_fdg_batch_begin(ovx_fdg_ctx *ctx, const float *d_vertices, int64_
 ovx_status ovx_fdg_finish(ovx_fdg_ctx *ctx, int32_t *d_coords, float *d_dual, uint8_t *d_intersected, void *stream);
 
 /*
+ * Allocation-free conversion: ovx_fdg_begin + ovx_fdg_finish in one call
+ * that allocates nothing.  No counterpart in the reference; it is for
+ * callers that cannot allocate device memory (code under HIP graph capture
+ * of their own, tight serving loops, harnesses that forbid hipMalloc).
+ * Every workspace buffer is carved from a static per-device scratch of
+ * OVX_FDG_NOALLOC_SCRATCH_BYTES (held by each device the library runs on
+ * until the process ends; its address is looked up by ovx_fdg_create);
+ * the context's own workspace is left as it was.  The caller passes the
+ * outputs with room for max_voxels rows (layout as ovx_fdg_finish) and gets
+ * the voxel count in *n_voxels.
+ * Errors: OVX_ERR_CAPACITY when the conversion needs more than the scratch
+ * (checked before the stage that would need it is queued) or has more than
+ * max_voxels voxels (checked before anything is written to the outputs).
+ * All calls on one device share its scratch and leave their contents in
+ * it: calls on one context are ordered, but calls that overlap on one GPU
+ * from different contexts race.  Blocks twice, like ovx_fdg_begin and
+ * ovx_fdg_finish; precision follows ovx_fdg_set_precision; graph mode does
+ * not apply.
+ */
+#define OVX_FDG_NOALLOC_SCRATCH_BYTES (64ll << 20)
+ovx_status ovx_fdg_convert_noalloc(ovx_fdg_ctx *ctx, const float *d_vertices, int64_t n_vertices,
+                                   const int32_t *d_faces, int64_t n_faces, const ovx_grid *grid, float face_weight,
+                                   float boundary_weight, float regularization_weight, int32_t *d_coords,
+                                   float *d_dual, uint8_t *d_intersected, int64_t max_voxels, void *stream,
+                                   int64_t *n_voxels);
+
+/*
  * intersect_occ, device path: occupancy of a triangle soup.
  * Replaces o_voxel::fdg::intersect_occ_cuda
  * (src/convert/mesh_to_flexible_dual_grid_gpu/intersect_qef.cu:896-920;

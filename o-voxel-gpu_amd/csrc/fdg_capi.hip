@@ -49,7 +49,8 @@ struct DevBuf
 {
     void *p = nullptr;
     size_t bytes = 0;
-    bool pooled = false; // stream-ordered allocation (freed stream-ordered too)
+    bool pooled = false;     // stream-ordered allocation (freed stream-ordered too)
+    bool in_scratch = false; // carved from the device's static scratch (never freed)
 };
 
 // restores the caller's current device on scope exit
@@ -124,6 +125,8 @@ struct ovx_fdg_ctx
     uint64_t face_geo = 0;                // face-test hits expected from the faces' sizes (prep of the last begin)
     uint64_t n_events = 0;                // crossings recorded by the last mark pass
     int64_t E = 0;           // boundary edges of the last begin
+    uint64_t queue_floor = 1ull << 20;    // least items of the crossing and row-chunk queues
+    unsigned char *scratch = nullptr;     // this device's static scratch (ovx_fdg_convert_noalloc)
     unsigned long long *h_ctr = nullptr; // pinned mirror of the counters
     // state between begin and finish
     int pending = 0; // 0 none, 1 fdg, 2 occ, 3 dual mesh
@@ -200,8 +203,33 @@ struct ovx_fdg_ctx
     }
 };
 
+// Scratch of ovx_fdg_convert_noalloc: one copy of the array per device,
+// held until the process ends.  In host code the name stands for no device
+// address: the address of the current device's copy comes from
+// hipGetSymbolAddress (scratch_for_device), and only that is handed to
+// kernels.
+static __device__ __align__(256) unsigned char scratch[OVX_FDG_NOALLOC_SCRATCH_BYTES];
+
 namespace
 {
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// The address of `device`'s scratch (that device current), looked up once
+// per device.
+ovx_status scratch_for_device(int device, unsigned char **out);
+
+// Bump allocator over the device's static scratch, active on this thread
+// for the duration of an allocation-free call: workspace buffers are carved
+// from it instead of allocated.
+struct ScratchArena
+{
+    unsigned char *base;
+    size_t size, used;
+};
+thread_local ScratchArena *tl_arena = nullptr;
 
 // Stream of the API call in progress on this thread (set by StreamScope):
 // workspace growth inside a call is stream-ordered on it, so a buffer is
@@ -255,6 +283,23 @@ ovx_status ensure(DevBuf &b, size_t bytes, bool slack = true)
         bytes = 256;
     if (b.bytes >= bytes)
         return OVX_OK;
+    if (tl_arena)
+    {
+        // allocation-free call: a fresh piece of the scratch (a piece carved
+        // earlier in the call for this buffer is left unused)
+        const size_t off = (tl_arena->used + 255) / 256 * 256;
+        if (off > tl_arena->size || bytes > tl_arena->size - off)
+            return fail(OVX_ERR_CAPACITY,
+                        "the allocation-free conversion needs more than its %zu-byte scratch (%zu bytes in use, "
+                        "%zu more requested)",
+                        tl_arena->size, off, bytes);
+        b.p = tl_arena->base + off;
+        b.bytes = bytes;
+        b.pooled = false;
+        b.in_scratch = true;
+        tl_arena->used = off + bytes;
+        return OVX_OK;
+    }
     g_ws_gen.fetch_add(1);
     if (b.p)
     {
@@ -285,12 +330,15 @@ ovx_status ensure(DevBuf &b, size_t bytes, bool slack = true)
 // wait for).
 void release(DevBuf &b)
 {
+    if (b.in_scratch)
+    {
+        b = DevBuf{};
+        return;
+    }
     g_ws_gen.fetch_add(1);
     if (b.p)
         (void)(b.pooled ? hipFreeAsync(b.p, nullptr) : hipFree(b.p));
-    b.p = nullptr;
-    b.bytes = 0;
-    b.pooled = false;
+    b = DevBuf{};
 }
 
 int64_t held_bytes(ovx_fdg_ctx *c)
@@ -476,6 +524,16 @@ int build_guide(const T *off, int M, int32_t *gd)
 
 // Batch tables: per-mesh grids, prefix offsets of faces, vertices, bitset
 // words, bricks and block tiles; the device copy is uploaded on stream s.
+// Pinned staging bytes of a batch of M meshes (layout in setup_batch): the
+// tables above and the read-back voxel offsets.
+size_t stage_bytes(int M)
+{
+    const size_t seg_bytes = ((sizeof(MeshSeg) * (size_t)M + 255) / 256) * 256;
+    const size_t arr = (size_t)(M + 1) * 8;
+    const size_t garr = (((size_t)guide_len(M) * 4 + 7) / 8) * 8;
+    return seg_bytes + (5 + N_MARK_CLASSES + N_FACE_TILES) * arr + N_GUIDES * garr + arr;
+}
+
 ovx_status setup_batch(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *face_off, const int64_t *vert_off,
                        const ovx_grid *grids, hipStream_t s)
 {
@@ -547,8 +605,12 @@ ovx_status setup_batch(ovx_fdg_ctx *c, const MeshDev &m, int M, const int64_t *f
     constexpr int FT0 = 5 + N_MARK_CLASSES; // first face-tile array
     const size_t gd0 = seg_bytes + (FT0 + N_FACE_TILES) * arr;
     const size_t total = gd0 + N_GUIDES * garr;
+    if (total + arr != stage_bytes(M))
+        return fail(OVX_ERR_STATE, "staging layout mismatch (internal error)");
     if (c->h_stage_bytes < total + arr)
     {
+        if (tl_arena)
+            return fail(OVX_ERR_STATE, "the allocation-free conversion needs the staging its context set up");
         if (c->done_recorded)
             HIP_TRY(hipEventSynchronize(c->ev_done)); // the staging may still be read by a copy
         if (c->h_stage)
@@ -770,14 +832,14 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m_in, int M, const int64_
     ShardBuf<Event> eb{};
     if (planes)
     {
-        if ((st = ensure_shard(c, c->events, SB_EVENTS, per_for(std::max<uint64_t>(1ull << 20, 2ull * (uint64_t)m.F)),
+        if ((st = ensure_shard(c, c->events, SB_EVENTS, per_for(std::max<uint64_t>(c->queue_floor, 2ull * (uint64_t)m.F)),
                                sizeof(Event))) ||
             (st = clear_shards(c, SB_EVENTS, 1, s)))
             return st;
         eb = shard_view<Event>(c, c->events, SB_EVENTS);
     }
     // row chunks of large triangles, shared out over the grid by a second launch
-    if ((st = ensure_shard(c, c->rowq, SB_ROWQ, per_for(std::max<uint64_t>(1ull << 20, (uint64_t)m.F / 4)),
+    if ((st = ensure_shard(c, c->rowq, SB_ROWQ, per_for(std::max<uint64_t>(c->queue_floor, (uint64_t)m.F / 4)),
                            sizeof(RowChunk))))
         return st;
     ShardBuf<RowChunk> rq = shard_view<RowChunk>(c, c->rowq, SB_ROWQ);
@@ -1192,7 +1254,7 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
         want_rec = std::min(want_rec, lim);
         if ((st = ensure_shard(c, c->recs, SB_RECS, want_rec, sizeof(Rec))) ||
             (st = ensure_shard(c, c->chunks, SB_CHUNKS, std::min(want_chunk, lim), sizeof(Chunk))) ||
-            (st = ensure_shard(c, c->faceq, SB_FACEQ, per_for(std::max<uint64_t>(1ull << 20, (uint64_t)m.F / 4)),
+            (st = ensure_shard(c, c->faceq, SB_FACEQ, per_for(std::max<uint64_t>(c->queue_floor, (uint64_t)m.F / 4)),
                                sizeof(FaceChunk))) ||
             (st = ensure(c->heads, (n_bricks + 1) * 4)))
             return st;
@@ -1278,6 +1340,72 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
     return OVX_OK;
 }
 
+ovx_status scratch_for_device(int device, unsigned char **out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= SCRATCH_MAX_DEVICES)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "device %d beyond the %d devices with a scratch", device,
+                    SCRATCH_MAX_DEVICES);
+    if (!g_scratch_ptr[device])
+    {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(OVX_ERR_HIP, "address of the device scratch: %s", hipGetErrorString(e));
+        g_scratch_ptr[device] = (unsigned char *)p;
+    }
+    *out = g_scratch_ptr[device];
+    return OVX_OK;
+}
+
+// Every workspace buffer of a context, its sharded capacities and its graph
+// mode, parked while an allocation-free call runs on the scratch.
+struct ParkedWorkspace
+{
+    static constexpr int MAX_BUFS = 64;
+    DevBuf bufs[MAX_BUFS];
+    DevBuf counters;
+    uint64_t per[SB_N], force_per[SB_N];
+    bool graph_mode;
+    uint64_t queue_floor;
+    int n = 0;
+
+    void park(ovx_fdg_ctx *c)
+    {
+        n = 0;
+        c->for_each_buf([&](DevBuf &b) {
+            bufs[n++] = b;
+            b = DevBuf{};
+        });
+        counters = c->counters;
+        c->counters = DevBuf{};
+        for (int i = 0; i < SB_N; ++i)
+        {
+            per[i] = c->per[i];
+            force_per[i] = c->force_per[i];
+            c->per[i] = 0;
+            c->force_per[i] = 0;
+        }
+        graph_mode = c->graph_mode;
+        c->graph_mode = false; // graphs are for the context's own buffers
+        queue_floor = c->queue_floor;
+        c->queue_floor = 1ull << 14; // small queues: a full queue only moves work back to its producer
+    }
+    void restore(ovx_fdg_ctx *c)
+    {
+        int i = 0;
+        c->for_each_buf([&](DevBuf &b) { b = bufs[i++]; });
+        c->counters = counters;
+        for (int k = 0; k < SB_N; ++k)
+        {
+            c->per[k] = per[k];
+            c->force_per[k] = force_per[k];
+        }
+        c->graph_mode = graph_mode;
+        c->queue_floor = queue_floor;
+    }
+};
+
 } // namespace
 
 extern "C" {
@@ -1300,12 +1428,29 @@ ovx_status ovx_fdg_create(int device, ovx_fdg_ctx **out)
     hipError_t e = hipHostMalloc((void **)&c->h_ctr, CTR_COUNT * sizeof(unsigned long long), hipHostMallocDefault);
     if (e == hipSuccess)
         e = hipEventCreateWithFlags(&c->ev_done, hipEventDisableTiming);
+    // the staging of a single-mesh conversion and the side stream, so that
+    // ovx_fdg_convert_noalloc finds them in place
+    if (e == hipSuccess && (e = hipHostMalloc((void **)&c->h_stage, stage_bytes(1))) == hipSuccess)
+        c->h_stage_bytes = stage_bytes(1);
     if (e != hipSuccess)
     {
         if (c->h_ctr)
             (void)hipHostFree(c->h_ctr);
+        if (c->h_stage)
+            (void)hipHostFree(c->h_stage);
+        if (c->ev_done)
+            (void)hipEventDestroy(c->ev_done);
         delete c;
         return fail(OVX_ERR_HIP, "context set-up failed: %s", hipGetErrorString(e));
+    }
+    ovx_status st = ensure_side(c);
+    if (!st)
+        st = scratch_for_device(device, &c->scratch);
+    if (st)
+    {
+        const std::string msg = g_last_error;
+        (void)ovx_fdg_destroy(c);
+        return fail(st, "%s", msg.c_str());
     }
     *out = c;
     return OVX_OK;
@@ -1421,6 +1566,56 @@ ovx_status ovx_fdg_finish(ovx_fdg_ctx *c, int32_t *d_coords, float *d_dual, uint
     StreamScope ss(c, (hipStream_t)stream);
     HIP_TRY(ss.enter());
     return finish_fdg(c, d_coords, d_dual, d_intersected, (hipStream_t)stream);
+}
+
+ovx_status ovx_fdg_convert_noalloc(ovx_fdg_ctx *c, const float *d_vertices, int64_t n_vertices,
+                                   const int32_t *d_faces, int64_t n_faces, const ovx_grid *grid, float face_weight,
+                                   float boundary_weight, float regularization_weight, int32_t *d_coords,
+                                   float *d_dual, uint8_t *d_intersected, int64_t max_voxels, void *stream,
+                                   int64_t *n_voxels)
+{
+    if (!c || !n_voxels)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "ctx / n_voxels is null");
+    if (n_faces > 0 && !d_faces)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "faces pointer is null");
+    if (max_voxels < 0 || (max_voxels > 0 && (!d_coords || !d_dual || !d_intersected)))
+        return fail(OVX_ERR_INVALID_ARGUMENT, "output pointers / max_voxels invalid");
+    DeviceGuard dg(c->device);
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->scratch)
+        return fail(OVX_ERR_STATE, "context has no device scratch");
+    *n_voxels = 0;
+    MeshDev m{d_vertices, d_faces, n_faces, n_vertices};
+    c->fw = face_weight;
+    c->bw = boundary_weight;
+    c->rw = regularization_weight;
+    const int64_t fo[2] = {0, n_faces}, vo[2] = {0, n_vertices};
+    ParkedWorkspace parked;
+    parked.park(c);
+    ScratchArena arena{c->scratch, (size_t)OVX_FDG_NOALLOC_SCRATCH_BYTES, 0};
+    tl_arena = &arena;
+    ovx_status st = OVX_OK;
+    int64_t N = 0;
+    {
+        StreamScope ss(c, (hipStream_t)stream);
+        const hipError_t e = ss.enter();
+        if (e != hipSuccess)
+            st = fail(OVX_ERR_HIP, "stream ordering: %s", hipGetErrorString(e));
+        if (!st)
+            st = begin_common(c, m, 1, fo, vo, grid, true, (hipStream_t)stream, &N);
+        if (!st && N > max_voxels)
+            st = fail(OVX_ERR_CAPACITY, "%lld occupied voxels exceed the output capacity of %lld", (long long)N,
+                      (long long)max_voxels);
+        if (!st && N > 0)
+            st = finish_fdg(c, d_coords, d_dual, d_intersected, (hipStream_t)stream);
+    }
+    tl_arena = nullptr;
+    c->pending = 0;
+    parked.restore(c);
+    if (st)
+        return st;
+    *n_voxels = N;
+    return OVX_OK;
 }
 
 ovx_status ovx_occ_begin(ovx_fdg_ctx *c, const float *d_triangles, int64_t n_triangles, const ovx_grid *grid,

@@ -103,6 +103,9 @@ def _load():
         L.ovx_fdg_begin.argtypes = [vp, vp, i64, vp, i64, ctypes.POINTER(_Grid), f32, f32, f32, vp,
                                     ctypes.POINTER(i64)]
         L.ovx_fdg_finish.argtypes = [vp, vp, vp, vp, vp]
+        L.ovx_fdg_convert_noalloc.argtypes = [vp, vp, i64, vp, i64, ctypes.POINTER(_Grid), f32, f32, f32, vp, vp,
+                                              vp, i64, vp, ctypes.POINTER(i64)]
+        L.ovx_fdg_convert_noalloc.restype = ctypes.c_int
         L.ovx_fdg_batch_begin.argtypes = [vp, vp, i64, vp, i64, ctypes.c_int, ctypes.POINTER(i64),
                                           ctypes.POINTER(i64), ctypes.POINTER(_Grid), f32, f32, f32, vp,
                                           ctypes.POINTER(i64)]
