@@ -879,8 +879,10 @@ constexpr uint32_t FACE_VOX_CAP = 1024;
 
 struct FaceRow
 {
+    uint64_t word; // the row's active candidates when they lie in one bitset word (else 0)
     int item, y, z;
     int x0, x1; // candidate voxels [x0, x1) of the row
+    int one;    // 1 when the candidates lie in one bitset word (`word`)
 };
 
 // Voxels of bounding-box row (y, z) that can pass the slab test of
@@ -888,27 +890,151 @@ struct FaceRow
 // -d2, an interval of x for |n_x| large enough.  The interval is computed in
 // double and widened by a bound on the FP32 test's rounding (a few ulp of
 // the terms of n.p + d, over |n_x|) plus 2 voxels; for |n_x| < 0.05, and for
-// rows of at most 4 voxels, the whole bounding-box row is kept.
-__device__ __forceinline__ void face_row_range(const FaceItem &it, const GridDev &g, int y, int z, int &x0, int &x1)
+// rows of at most 4 voxels, the whole bounding-box row is kept.  The terms
+// that do not depend on the row are formed once per face (face_row_pre).
+struct RowNarrow
+{
+    double k;     // 1 / (n_x vs_x): voxels of x per unit of -(d + n_y p_y + n_z p_z)
+    double base;  // |n_x| max(|bx0|, |bx1|) vs_x + |d1| + |d2|: the row-independent rounding terms
+    double scale; // 1e-6 / (|n_x| vs_x): rounding terms -> voxels
+    bool on;      // narrowing applies to this face
+};
+
+__device__ __forceinline__ RowNarrow face_row_pre(const FaceItem &it, const GridDev &g)
+{
+    RowNarrow r;
+    const double nx = (double)it.n[0];
+    r.on = fabs(nx) >= 0.05 && it.bx1 - it.bx0 > 4;
+    const double vs0 = (double)g.vs[0];
+    const double anv = fabs(nx) * vs0;
+    r.k = r.on ? 1.0 / (nx * vs0) : 0.0;
+    r.base = anv * fmax(fabs((double)it.bx0), fabs((double)it.bx1)) + fabs((double)it.d1) + fabs((double)it.d2);
+    r.scale = r.on ? 1.0e-6 / anv : 0.0;
+    return r;
+}
+
+__device__ __forceinline__ void face_row_range(const FaceItem &it, const GridDev &g, const RowNarrow &r, int y, int z,
+                                               int &x0, int &x1)
 {
     x0 = it.bx0;
     x1 = it.bx1;
-    const double nx = (double)it.n[0];
-    // rows of a few voxels: the bounding-box range costs less than narrowing it
-    if (!(fabs(nx) >= 0.05) || it.bx1 - it.bx0 <= 4)
+    if (!r.on)
         return;
-    const double vs0 = (double)g.vs[0];
     const double py = (double)g.vs[1] * (double)y, pz = (double)g.vs[2] * (double)z;
-    const double K = (double)it.n[1] * py + (double)it.n[2] * pz;
-    const double inx = 1.0 / nx;
-    const double a = (-(double)it.d1 - K) * inx, b = (-(double)it.d2 - K) * inx;
-    const double pxm = fmax(fabs((double)it.bx0), fabs((double)it.bx1)) * vs0;
-    const double terms = fabs(nx) * pxm + fabs((double)it.n[1] * py) + fabs((double)it.n[2] * pz) +
-                         fabs((double)it.d1) + fabs((double)it.d2);
-    const double margin = 2.0 + 1.0e-6 * terms / (fabs(nx) * vs0);
-    const double lo = fmax(div_vs(fmin(a, b), g, 0) - margin, (double)it.bx0 - 1.0);
-    const double hi = fmin(div_vs(fmax(a, b), g, 0) + margin, (double)it.bx1 + 1.0);
+    const double ky = (double)it.n[1] * py, kz = (double)it.n[2] * pz;
+    const double K = ky + kz;
+    const double a = (-(double)it.d1 - K) * r.k, b = (-(double)it.d2 - K) * r.k;
+    const double margin = 2.0 + (r.base + fabs(ky) + fabs(kz)) * r.scale;
+    const double lo = fmax(fmin(a, b) - margin, (double)it.bx0 - 1.0);
+    const double hi = fmin(fmax(a, b) + margin, (double)it.bx1 + 1.0);
     if (!(lo <= hi))
+    {
+        x1 = x0; // empty (also for non-finite planes)
+        return;
+    }
+    const int l = (int)floor(lo), h = (int)floor(hi) + 1;
+    x0 = l > x0 ? l : x0;
+    x1 = h < x1 ? h : x1;
+    if (x1 < x0)
+        x1 = x0;
+}
+
+// Candidate range of bounding-box row (y, z) for the block-cooperative
+// face test, whose faces have long rows: besides the slab, the row is cut to
+// where the 6 projected-edge half-planes that depend on x can hold, and
+// dropped when one of the 3 that do not (the yz ones, evaluated exactly as
+// face_pass does) fails.  Each x bound comes from the exact line a vs_x x =
+// -(c + d) moved by 1e-6 of the test's terms (the FP32 evaluation errs by
+// less than 4 ulp of them) and by 2 more voxels, so every voxel that passes
+// face_pass stays in the range; face_pass re-tests them all.
+struct RowNarrowFull
+{
+    double ks;    // 1 / (n_x vs_x), 0 when n_x = 0
+    double sbase; // |n_x| pxm + |d1| + |d2|
+    double pxm;   // vs_x max(|bx0|, |bx1|): bound on |p_x| over the box
+    double ke[6]; // 1 / (a vs_x) of the xy edges' and the zx edges' x coefficient a, 0 when a = 0
+};
+
+__device__ __forceinline__ RowNarrowFull face_row_pre_full(const FaceItem &it, const GridDev &g)
+{
+    RowNarrowFull r;
+    const double vs0 = (double)g.vs[0];
+    const double nx = (double)it.n[0];
+    r.pxm = vs0 * fmax(fabs((double)it.bx0), fabs((double)it.bx1));
+    r.ks = nx != 0.0 ? 1.0 / (nx * vs0) : 0.0;
+    r.sbase = fabs(nx) * r.pxm + fabs((double)it.d1) + fabs((double)it.d2);
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        const double a = (double)it.nxy[k][0], b = (double)it.nzx[k][1];
+        r.ke[k] = a != 0.0 ? 1.0 / (a * vs0) : 0.0;
+        r.ke[3 + k] = b != 0.0 ? 1.0 / (b * vs0) : 0.0;
+    }
+    return r;
+}
+
+// one x-dependent half-plane a p_x + c + d >= 0 (c the FP32 product of the
+// other coordinate's term, as face_pass forms it); false when the row fails
+// it for every x (a = 0 and the exact FP32 test fails)
+__device__ __forceinline__ bool row_edge_bound(float a, float c, float d, double ke, double pxm, float p0, double &lo,
+                                               double &hi)
+{
+    if (ke == 0.0)
+        return !((a * p0 + c) + d < 0.0f);
+    const double m = 1.0e-6 * (fabs((double)a) * pxm + fabs((double)c) + fabs((double)d));
+    const double t = (-((double)c + (double)d) - m) * ke;
+    if (a > 0.0f)
+        lo = fmax(lo, t - 2.0);
+    else
+        hi = fmin(hi, t + 2.0);
+    return true;
+}
+
+__device__ __forceinline__ void face_row_range_full(const FaceItem &it, const GridDev &g, const RowNarrowFull &r, int y,
+                                                    int z, int &x0, int &x1)
+{
+    x0 = it.bx0;
+    x1 = it.bx1;
+    const float p1 = g.vs[1] * (float)y, p2 = g.vs[2] * (float)z;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        if ((it.nyz[k][0] * p1 + it.nyz[k][1] * p2) + it.dyz[k] < 0.0f)
+        {
+            x1 = x0;
+            return;
+        }
+    if (it.bx1 - it.bx0 <= 4)
+        return; // short rows: the box range costs less than narrowing it
+    double lo = (double)it.bx0, hi = (double)(it.bx1 - 1); // inclusive
+    {
+        // slab: n_x vs_x x within [-d1 - K, -d2 - K] (either order), K = n_y p_y + n_z p_z
+        const double ky = (double)it.n[1] * ((double)g.vs[1] * (double)y);
+        const double kz = (double)it.n[2] * ((double)g.vs[2] * (double)z);
+        const double K = ky + kz;
+        const double m = 1.0e-6 * (r.sbase + fabs(ky) + fabs(kz));
+        const double u = -(double)it.d1 - K, v = -(double)it.d2 - K;
+        const double tlo = fmin(u, v) - m, thi = fmax(u, v) + m;
+        if (r.ks != 0.0)
+        {
+            const double a = tlo * r.ks, b = thi * r.ks;
+            lo = fmax(lo, fmin(a, b) - 2.0);
+            hi = fmin(hi, fmax(a, b) + 2.0);
+        }
+        else if (!(tlo <= 0.0 && 0.0 <= thi))
+        {
+            x1 = x0;
+            return;
+        }
+    }
+    const float p0 = g.vs[0] * (float)it.bx0; // only for half-planes that do not depend on x
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        ok = ok && row_edge_bound(it.nxy[k][0], it.nxy[k][1] * p1, it.dxy[k], r.ke[k], r.pxm, p0, lo, hi);
+        ok = ok && row_edge_bound(it.nzx[k][1], it.nzx[k][0] * p2, it.dzx[k], r.ke[3 + k], r.pxm, p0, lo, hi);
+    }
+    if (!ok || !(lo <= hi))
     {
         x1 = x0; // empty (also for non-finite planes)
         return;
@@ -1057,17 +1183,15 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(7))) void k_
 {
     static_assert(FACE_TILE <= BT, "a tile's faces are set up one per thread");
     __shared__ FaceItem items[FACE_TILE];
+    __shared__ RowNarrowFull narrows[FACE_TILE];
     __shared__ FaceRow rows[BT];
     __shared__ uint64_t incl_items[FACE_TILE];
     __shared__ uint32_t incl_rows[BT];
     __shared__ uint64_t sh64[NWAVE];
     __shared__ uint32_t sh32[NWAVE];
-    __shared__ unsigned long long hbase;
-    __shared__ uint32_t hwave[NWAVE];
     __shared__ QueueMap qm;
     const int tid = threadIdx.x;
     uint32_t n_hit = 0;
-    uint32_t n_claim = 0; // hit claims of this block (block-uniform)
     int64_t n_items = 0;
     if constexpr (QUEUED)
     {
@@ -1103,6 +1227,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(7))) void k_
             {
                 FaceItem &it = items[tid];
                 R = face_item_setup(m, gt, item, it);
+                narrows[tid] = face_row_pre_full(it, gt);
                 it.ms = (uint32_t)ms;
                 it.r0 = 0;
                 const uint32_t ext = (uint32_t)(it.bx1 > it.bx0 ? it.bx1 - it.bx0 : 1);
@@ -1140,6 +1265,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(7))) void k_
                 const FaceChunk ch = fq.items[queue_slot(fq, qm.pre, (uint64_t)item)];
                 FaceItem &it = items[tid];
                 face_item_setup(m, bt.seg[ch.ms].g, (int64_t)ch.face, it);
+                narrows[tid] = face_row_pre_full(it, bt.seg[ch.ms].g);
                 it.ms = ch.ms;
                 it.r0 = ch.r0;
                 R = (uint64_t)(ch.r1 - ch.r0);
@@ -1168,15 +1294,20 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(7))) void k_
                 row.item = L;
                 row.y = y;
                 row.z = z;
-                face_row_range(it, g, y, z, row.x0, row.x1);
+                face_row_range_full(it, g, narrows[L], y, z, row.x0, row.x1);
                 const int lx0 = row.x0 - g.gmin[0], lx1 = row.x1 - g.gmin[0];
                 const uint64_t rowbase = ((uint64_t)(z - g.gmin[2]) * g.G[1] + (uint64_t)(y - g.gmin[1])) * g.wx;
+                uint64_t last = 0ull;
                 for (int w = lx0 >> 6; lx1 > lx0 && w <= ((lx1 - 1) >> 6); ++w)
                 {
                     const int blo = (w == (lx0 >> 6)) ? (lx0 & 63) : 0;
                     const int bhi = (w == ((lx1 - 1) >> 6)) ? (((lx1 - 1) & 63) + 1) : 64;
-                    A += (uint32_t)__popcll(g.bits[rowbase + w] & row_mask(blo, bhi));
+                    last = g.bits[rowbase + w] & row_mask(blo, bhi);
+                    A += (uint32_t)__popcll(last);
                 }
+                // a row inside one word keeps its candidates for the test below
+                row.one = (lx1 > lx0 && (lx0 >> 6) == ((lx1 - 1) >> 6)) ? 1 : 0;
+                row.word = row.one ? last : 0ull;
             }
             uint32_t tot_a;
             const uint32_t ex_a = block_excl_scan(A, sh32, tot_a);
@@ -1195,65 +1326,57 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(7))) void k_
                     const GridDev &g = QUEUED ? bt.seg[it.ms].g : gt;
                     uint32_t k = h - ((rr == 0) ? 0u : incl_rows[rr - 1]);
                     const int lx0 = row.x0 - g.gmin[0], lx1 = row.x1 - g.gmin[0];
-                    const uint64_t rowbase =
-                        ((uint64_t)(row.z - g.gmin[2]) * g.G[1] + (uint64_t)(row.y - g.gmin[1])) * g.wx;
-                    for (int w = lx0 >> 6; w <= ((lx1 - 1) >> 6); ++w)
+                    int x = 0;
+                    bool found = row.one != 0; // grid coordinates may be negative: no sentinel x
+                    if (found)
+                        x = g.gmin[0] + (lx0 & ~63) + select_bit(row.word, k);
+                    else
                     {
-                        const int blo = (w == (lx0 >> 6)) ? (lx0 & 63) : 0;
-                        const int bhi = (w == ((lx1 - 1) >> 6)) ? (((lx1 - 1) & 63) + 1) : 64;
-                        uint64_t mm = g.bits[rowbase + w] & row_mask(blo, bhi);
-                        const uint32_t pc = (uint32_t)__popcll(mm);
-                        if (k >= pc)
+                        const uint64_t rowbase =
+                            ((uint64_t)(row.z - g.gmin[2]) * g.G[1] + (uint64_t)(row.y - g.gmin[1])) * g.wx;
+                        for (int w = lx0 >> 6; w <= ((lx1 - 1) >> 6); ++w)
                         {
-                            k -= pc;
-                            continue;
+                            const int blo = (w == (lx0 >> 6)) ? (lx0 & 63) : 0;
+                            const int bhi = (w == ((lx1 - 1) >> 6)) ? (((lx1 - 1) & 63) + 1) : 64;
+                            const uint64_t mm = g.bits[rowbase + w] & row_mask(blo, bhi);
+                            const uint32_t pc = (uint32_t)__popcll(mm);
+                            if (k >= pc)
+                            {
+                                k -= pc;
+                                continue;
+                            }
+                            x = g.gmin[0] + w * 64 + select_bit(mm, k);
+                            found = true;
+                            break;
                         }
-                        const int b = select_bit(mm, k);
-                        const int x = g.gmin[0] + w * 64 + b;
-                        if (face_pass(it, g.vs, x, row.y, row.z))
-                        {
-                            ++n_hit;
-                            hit = FHit{it.face, (uint16_t)(x - g.gmin[0]), (uint16_t)(row.y - g.gmin[1]),
-                                       (uint16_t)(row.z - g.gmin[2]), 0};
-                            has = true;
-                        }
-                        break;
+                    }
+                    if (found && face_pass(it, g.vs, x, row.y, row.z))
+                    {
+                        ++n_hit;
+                        hit = FHit{it.face, (uint16_t)(x - g.gmin[0]), (uint16_t)(row.y - g.gmin[1]),
+                                   (uint16_t)(row.z - g.gmin[2]), 0};
+                        has = true;
                     }
                 }
-                // block-aggregated append (every thread of the block runs this
-                // iteration): the iteration's hits, from the tile's faces, stay
-                // together in the queue, so the records made of them fall in few
-                // bricks per generator block
+                // one claim per wave for the wave's hits (no block barrier):
+                // a wave's hits come from consecutive candidates of the tile's
+                // rows, so the records made of them fall in few bricks per
+                // generator block
+                const uint64_t m = __ballot(has);
+                if (m)
                 {
-                    // per-wave counts through LDS (hwave and hbase are rewritten
-                    // only after every thread has passed the next barrier)
-                    const uint64_t m = __ballot(has);
-                    const int lane = tid & 63, w = tid >> 6;
+                    const int lane = tid & 63;
+                    unsigned long long base = 0;
                     if (lane == 0)
-                        hwave[w] = (uint32_t)__popcll(m);
-                    __syncthreads();
-                    uint32_t before = 0, tot_h = 0;
-#pragma unroll
-                    for (int k = 0; k < NWAVE; ++k)
                     {
-                        before += k < w ? hwave[k] : 0u;
-                        tot_h += hwave[k];
+                        bool okk = false;
+                        const uint32_t sh = (blockIdx.x * NWAVE + (uint32_t)(tid >> 6)) % NSHARD;
+                        const unsigned long long b0 = shard_claim(hb, sh, (uint64_t)__popcll(m), okk);
+                        base = okk ? b0 : ~0ull;
                     }
-                    if (tot_h)
-                    {
-                        if (tid == 0)
-                        {
-                            // successive claims of a block rotate over NWAVE shards
-                            bool okk = false;
-                            const uint32_t sh = (blockIdx.x * NWAVE + (n_claim & (NWAVE - 1))) % NSHARD;
-                            const unsigned long long b0 = shard_claim(hb, sh, (uint64_t)tot_h, okk);
-                            hbase = okk ? b0 : ~0ull;
-                        }
-                        ++n_claim;
-                        __syncthreads();
-                        if (has && hbase != ~0ull)
-                            hb.items[hbase + before + (uint64_t)__popcll(m & ((1ull << lane) - 1ull))] = hit;
-                    }
+                    base = __shfl(base, 0, 64);
+                    if (has && base != ~0ull)
+                        hb.items[base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull))] = hit;
                 }
             }
             __syncthreads();
@@ -1271,12 +1394,13 @@ template <bool WRITE>
 __device__ __forceinline__ uint32_t face_walk(const FaceItem &it, const GridDev &g, uint64_t n_rows, FHit *out)
 {
     uint32_t n = 0;
+    const RowNarrow nr = face_row_pre(it, g);
     for (uint64_t rr = 0; rr < n_rows; ++rr)
     {
         const int y = it.by0 + (int)(rr % (uint64_t)it.ny);
         const int z = it.bz0 + (int)(rr / (uint64_t)it.ny);
         int x0, x1;
-        face_row_range(it, g, y, z, x0, x1);
+        face_row_range(it, g, nr, y, z, x0, x1);
         const int lx0 = x0 - g.gmin[0], lx1 = x1 - g.gmin[0];
         if (lx1 <= lx0)
             continue;
@@ -1355,7 +1479,8 @@ __device__ __forceinline__ uint64_t face_mask_hits(const FaceItem &it, const Gri
     const uint64_t emask = (1ull << ext) - 1ull;
     // long rows are narrowed to the plane's slab (face_row_range); at most 8
     // voxels here, so only rows of 5 .. 8 voxels
-    const bool narrow = ext > 4 && fabsf(it.n[0]) >= 0.05f;
+    const RowNarrow nr = face_row_pre(it, g);
+    const bool narrow = nr.on;
     uint64_t cand = 0ull;
 #pragma unroll
     for (int r = 0; r < FACE_MASK_ROWS; ++r)
@@ -1364,8 +1489,8 @@ __device__ __forceinline__ uint64_t face_mask_hits(const FaceItem &it, const Gri
         if (narrow && r < R && act)
         {
             int x0, x1;
-            face_row_range(it, g, it.by0 + (int)((ryz >> (8 * r)) & 15u), it.bz0 + (int)((ryz >> (8 * r + 4)) & 15u),
-                           x0, x1);
+            face_row_range(it, g, nr, it.by0 + (int)((ryz >> (8 * r)) & 15u),
+                           it.bz0 + (int)((ryz >> (8 * r + 4)) & 15u), x0, x1);
             const int j0 = x0 - it.bx0, j1 = x1 - it.bx0;
             act &= ((1ull << j1) - 1ull) & ~((1ull << j0) - 1ull);
         }
