@@ -1,5 +1,5 @@
 """Voxel file I/O (reference o_voxel/io/__init__.py): .vxz (the O-voxel
-format), .npz, and .ply (needs plyfile)."""
+format), .npz, and .ply (own PLY codec, no plyfile needed)."""
 
 from typing import Dict
 
