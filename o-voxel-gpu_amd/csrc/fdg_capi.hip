@@ -112,7 +112,7 @@ struct ovx_fdg_ctx
     int svo_state = 0;            // 1 begun, 2 written
     DevBuf vcnt, voff, nbr, erank, bcnt, boff, bedge_v, bedges, eslots;
     bool edge_csr = false; // the last conversion's boundary edges came from the CSR lists (a vertex overflowed its slots)
-    DevBuf events, recs, chunks, heads, choff, bchunks, active, large, huge, shard_ctr, rowq, faceq;
+    DevBuf events, recs, chunks, heads, choff, bchunks, active, medium, large, huge, shard_ctr, rowq, faceq;
     // face and boundary hits, and, once the records are made from them, the
     // per-voxel QEFs and the box-search queue in the same bytes
     DevBuf hitqef;
@@ -193,7 +193,7 @@ struct ovx_fdg_ctx
     void for_each_buf(Fn &&fn)
     {
         DevBuf *all[] = {&planes, &tris, &bits,   &woff,  &tiles,   &vcnt,   &voff,  &nbr, &erank, &bcnt, &boff, &eslots,
-                         &bedge_v, &bedges, &events, &recs,   &chunks, &heads, &choff, &bchunks, &active, &large, &huge,
+                         &bedge_v, &bedges, &events, &recs,   &chunks, &heads, &choff, &bchunks, &active, &medium, &large, &huge,
                          &hitqef,  &shard_ctr, &rowq, &faceq, &segbuf,
                          &dm_table, &dm_cnt, &dm_off, &dm_mask, &dm_tiles, &dm_direct, &dm_touched,
                          &svo_insf, &svo_k, &svo_pos, &svo_tiles, &svo_rows, &svo_erow, &nb_table, &nb_own,
@@ -1017,11 +1017,11 @@ ovx_status enqueue_finish(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t 
     toc(c, KS_COORDS, s);
     HIP_TRY(launch_fill_u64((unsigned long long *)c->shard_ctr.p + (uint64_t)SB_FHITS * NSHARD * SHARD_STRIDE,
                             (uint64_t)(SB_N - SB_FHITS) * NSHARD * SHARD_STRIDE, 0ull, s));
-    HIP_TRY(launch_fill_u32(c->heads.p, n_bricks + 1, 0u, s));
+    HIP_TRY(launch_fill_u64(c->heads.p, n_bricks + 1, 0ull, s));
     BucketArgs ba{};
     ba.recs = shard_view<Rec>(c, c->recs, SB_RECS);
     ba.chunks = shard_view<Chunk>(c, c->chunks, SB_CHUNKS);
-    ba.brick_nch = (uint32_t *)c->heads.p;
+    ba.brick_cnt = (unsigned long long *)c->heads.p;
     ba.b = c->bdev;
     ba.bedge_v = (const int2 *)c->bedge_v.p;
     ba.ctr = ctr;
@@ -1067,12 +1067,20 @@ ovx_status enqueue_finish(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t 
 
     // ---- chunks grouped by brick, bricks with records, then the per-brick ordered fold + solve
     tic(c, KS_ACTIVE, s);
-    HIP_TRY(launch_scan_u32((const uint32_t *)c->heads.p, n_bricks + 1, (uint32_t *)c->choff.p, (uint64_t *)c->tiles.p,
-                            ctr + CTR_CHUNK_SCAN, s));
+    HIP_TRY(launch_scan_lo32((const unsigned long long *)c->heads.p, n_bricks + 1, (uint32_t *)c->choff.p,
+                             (uint64_t *)c->tiles.p, ctr + CTR_CHUNK_SCAN, s));
     HIP_TRY(launch_place_chunks(shard_view<Chunk>(c, c->chunks, SB_CHUNKS), (const uint32_t *)c->choff.p,
                                 (uint2 *)c->bchunks.p, ctr + CTR_OVER, s));
-    HIP_TRY(launch_active_bricks((const uint32_t *)c->heads.p, n_bricks, (uint32_t *)c->active.p, ctr + CTR_ACTIVE,
-                                 ctr + CTR_OVER, s));
+    TierLists tl;
+    tl.list[0] = (uint32_t *)c->active.p;
+    tl.list[1] = (uint32_t *)c->medium.p;
+    tl.list[2] = (uint32_t *)c->large.p;
+    tl.list[3] = (uint32_t *)c->huge.p;
+    tl.count[0] = ctr + CTR_ACTIVE;
+    tl.count[1] = ctr + CTR_MEDIUM;
+    tl.count[2] = ctr + CTR_LARGE;
+    tl.count[3] = ctr + CTR_HUGE;
+    HIP_TRY(launch_active_bricks((const unsigned long long *)c->heads.p, n_bricks, tl, ctr + CTR_OVER, s));
     toc(c, KS_ACTIVE, s);
     FoldArgs a{};
     a.choff = (const uint32_t *)c->choff.p;
@@ -1080,6 +1088,8 @@ ovx_status enqueue_finish(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t 
     a.recs = (const Rec *)c->recs.p;
     a.active = (const uint32_t *)c->active.p;
     a.n_active = ctr + CTR_ACTIVE;
+    a.medium = (uint32_t *)c->medium.p;
+    a.n_medium = ctr + CTR_MEDIUM;
     a.large = (uint32_t *)c->large.p;
     a.n_large = ctr + CTR_LARGE;
     a.huge = (uint32_t *)c->huge.p;
@@ -1099,7 +1109,7 @@ ovx_status enqueue_finish(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t 
     a.cond_max = c->cond_max;
     a.n64 = (unsigned long long *)c->shard_ctr.p + (uint64_t)SB_FP64 * NSHARD * SHARD_STRIDE;
     a.ctr = ctr;
-    // CTR_ACTIVE, CTR_LARGE, CTR_LARGE_RECS and CTR_HUGE are zeroed with all counters by begin (an
+    // CTR_ACTIVE, CTR_MEDIUM, CTR_LARGE, CTR_LARGE_RECS and CTR_HUGE are zeroed with all counters by begin (an
     // attempt whose record buffers overflow leaves them at zero)
     tic(c, KS_FOLD, s);
     HIP_TRY(launch_fold_bricks(a, n_bricks, s));
@@ -1256,7 +1266,7 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
             (st = ensure_shard(c, c->chunks, SB_CHUNKS, std::min(want_chunk, lim), sizeof(Chunk))) ||
             (st = ensure_shard(c, c->faceq, SB_FACEQ, per_for(std::max<uint64_t>(c->queue_floor, (uint64_t)m.F / 4)),
                                sizeof(FaceChunk))) ||
-            (st = ensure(c->heads, (n_bricks + 1) * 4)))
+            (st = ensure(c->heads, (n_bricks + 1) * 8)))
             return st;
         grouped = std::min<uint64_t>(grouped, c->per[SB_CHUNKS] * NSHARD);
         if ((st = ensure(c->bchunks, (grouped + 1) * 8)))
@@ -1265,7 +1275,8 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
         // repeated conversions the same while the chunk count varies with the order of the block claims
         grouped = std::min<uint64_t>(c->bchunks.bytes / 8 - 1, c->per[SB_CHUNKS] * NSHARD);
         if ((st = ensure(c->choff, (n_bricks + 1) * 4)) ||
-            (st = ensure(c->active, n_bricks * 4)) || (st = ensure(c->large, n_bricks * 4)) ||
+            (st = ensure(c->active, n_bricks * 4)) || (st = ensure(c->medium, n_bricks * 4)) ||
+            (st = ensure(c->large, n_bricks * 4)) ||
             (st = ensure(c->huge, n_bricks * 4)) ||
             // box-search queue: a few percent of the voxels typically; the solve
             // kernel solves in place whatever does not fit
@@ -1663,8 +1674,9 @@ ovx_status ovx_fdg_stats(ovx_fdg_ctx *c, int64_t *out, int n)
         if (st)
             return st;
         c->stats[7] = (int64_t)c->h_ctr[CTR_GIANT];
-        c->stats[8] = (int64_t)c->h_ctr[CTR_ACTIVE];
-        c->stats[10] = (int64_t)c->h_ctr[CTR_LARGE];
+        c->stats[8] = (int64_t)(c->h_ctr[CTR_ACTIVE] + c->h_ctr[CTR_MEDIUM] + c->h_ctr[CTR_LARGE] +
+                                c->h_ctr[CTR_HUGE]);
+        c->stats[10] = (int64_t)(c->h_ctr[CTR_LARGE] + c->h_ctr[CTR_HUGE]);
         c->stats[11] = (int64_t)c->h_ctr[CTR_LARGE_RECS];
         c->stats[12] = (int64_t)c->h_ctr[CTR_SUM + SB_BOX];
         c->stats[13] = c->prec ? (int64_t)c->h_ctr[CTR_SUM + SB_FP64] : 0;

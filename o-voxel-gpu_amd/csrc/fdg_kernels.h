@@ -30,6 +30,7 @@ enum
     CTR_ERRORS = 1,
     CTR_WORDS_TOTAL = 2, // total of the bitset popcount scan (= N)
     CTR_OVER = 3,        // a record-stage buffer of the conversion overflowed (the fold and solve stages skip it)
+    CTR_MEDIUM = 4,      // bricks left to the second fold tier
     CTR_GIANT = 5,
     CTR_EDGE_TOTAL = 6,  // half-edge count scan total
     CTR_BEDGE_TOTAL = 7, // boundary edges
@@ -253,7 +254,7 @@ struct BucketArgs
 {
     ShardBuf<Rec> recs;
     ShardBuf<Chunk> chunks;
-    uint32_t *brick_nch;     // [bricks] chunks per brick
+    unsigned long long *brick_cnt; // [bricks] chunks (low 32 bits) and records (high 32 bits) per brick
     BatchDev b;              // meshes (grid, brick grid and first brick of each)
     const int2 *bedge_v;     // boundary edges (vertex pairs), to find a boundary hit's mesh
     unsigned long long *ctr; // device counters
@@ -286,6 +287,8 @@ struct FoldArgs
     const Rec *recs;
     const uint32_t *active;             // bricks with records
     const unsigned long long *n_active; // their count (device)
+    uint32_t *medium;                   // bricks too large for the first fold tier
+    unsigned long long *n_medium;       // their count (device, zeroed by the host)
     uint32_t *large;                    // bricks too large for the main fold tile
     unsigned long long *n_large;        // their count (device, zeroed by the host)
     uint32_t *huge;                     // bricks too large for the mid-size tile
@@ -335,7 +338,15 @@ struct OverCaps
 hipError_t launch_over_check(unsigned long long *ctr, const OverCaps &caps, hipStream_t s);
 hipError_t launch_place_chunks(const ShardBuf<Chunk> &chunks, const uint32_t *choff, uint2 *gchunks,
                                const unsigned long long *over, hipStream_t s);
-hipError_t launch_active_bricks(const uint32_t *nch, uint64_t n_bricks, uint32_t *list, unsigned long long *count,
+// Bricks with records, each listed for the fold tier its record count
+// selects: lists[t] / counts[t] for t = 0 (at most FOLD_SMALL_CAP records),
+// 1 (FOLD_SMALL), 2 (FOLD_MID), 3 (more).
+struct TierLists
+{
+    uint32_t *list[4];
+    unsigned long long *count[4];
+};
+hipError_t launch_active_bricks(const unsigned long long *cnt, uint64_t n_bricks, TierLists t,
                                 const unsigned long long *over, hipStream_t s);
 hipError_t launch_fold_bricks(const FoldArgs &a, uint64_t max_bricks, hipStream_t s);
 // bricks of the `large` list up to the mid-size tile; the rest go to `huge`
@@ -363,6 +374,9 @@ hipError_t launch_bdry(int64_t E, const BEdge *be, const int2 *bedge_v, const Ba
                        unsigned long long *ctr, hipStream_t s);
 uint64_t scan_tile_count(uint64_t n);
 hipError_t launch_scan_popc(const uint64_t *bits, uint64_t n, uint32_t *out, uint64_t *tile_sums,
+                            unsigned long long *total, hipStream_t s);
+// exclusive scan of the low 32 bits of 64-bit values (the chunk counts of `brick_cnt`)
+hipError_t launch_scan_lo32(const unsigned long long *in, uint64_t n, uint32_t *out, uint64_t *tile_sums,
                             unsigned long long *total, hipStream_t s);
 hipError_t launch_scan_u32(const uint32_t *in, uint64_t n, uint32_t *out, uint64_t *tile_sums,
                            unsigned long long *total, hipStream_t s);

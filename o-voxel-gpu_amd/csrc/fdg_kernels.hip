@@ -41,10 +41,10 @@ __device__ __forceinline__ T wave_incl_scan(T x)
     return x;
 }
 
-// Exclusive block scan; `sh` holds NWAVE elements of LDS.  Contains barriers:
-// every thread of the block must call it.
-template <typename T>
-__device__ __forceinline__ T block_excl_scan(T v, T *sh, T &total)
+// Exclusive scan over a block of NW waves; `sh` holds NW elements of LDS.
+// Contains barriers: every thread of the block must call it.
+template <int NW, typename T>
+__device__ __forceinline__ T block_excl_scan_w(T v, T *sh, T &total)
 {
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
@@ -54,7 +54,7 @@ __device__ __forceinline__ T block_excl_scan(T v, T *sh, T &total)
     __syncthreads();
     T pre = 0, tot = 0;
 #pragma unroll
-    for (int w = 0; w < NWAVE; ++w)
+    for (int w = 0; w < NW; ++w)
     {
         const T t = sh[w];
         pre += (w < wid) ? t : T(0);
@@ -63,6 +63,13 @@ __device__ __forceinline__ T block_excl_scan(T v, T *sh, T &total)
     __syncthreads();
     total = tot;
     return pre + inc - v;
+}
+
+// the same for blocks of BT threads
+template <typename T>
+__device__ __forceinline__ T block_excl_scan(T v, T *sh, T &total)
+{
+    return block_excl_scan_w<NWAVE>(v, sh, total);
 }
 
 template <typename T>
@@ -315,7 +322,8 @@ __device__ __forceinline__ void bucket_emit(const uint32_t (&bk)[K], const Rec (
         {
             // slots s0 .. s0+PER-1 are this thread's alone here
             const uint32_t b = L.key[s0 + e];
-            const uint32_t r = atomicAdd(a.brick_nch + b, 1u);
+            // one chunk and n records more for the brick; the old chunk count is this chunk's rank
+            const uint32_t r = (uint32_t)atomicAdd(a.brick_cnt + b, 1ull | ((unsigned long long)n << 32));
             a.chunks.items[cpos] = Chunk{(uint32_t)rpos, n, b, r};
             L.key[s0 + e] = (uint32_t)rpos;
         }
@@ -2062,35 +2070,56 @@ __global__ __launch_bounds__(BT) void k_place_chunks(ShardBuf<Chunk> chunks, con
     }
 }
 
-// list of bricks that received records: each block takes ACT_PER
-// consecutive bricks per thread and appends them with one atomic
+// fold tiers by records per brick (held in LDS by the tier's kernel)
+constexpr int FOLD_SMALL_CAP = 768; // first tier: blocks of FOLD_SMALL_NT threads (15 KB of LDS)
+constexpr int FOLD_SMALL_NT = 128;
+constexpr int FOLD_SMALL = 1536; // second tier (25 KB: 6 blocks per CU)
+constexpr int FOLD_MID = 3584;   // third tier (3 blocks per CU); larger bricks: k_fold_large
+
+// bricks that received records, each appended to the list of the fold tier
+// its record count selects: each block takes ACT_PER consecutive bricks per
+// thread and appends them with one atomic per tier
 constexpr int ACT_PER = 8;
-__global__ __launch_bounds__(BT) void k_active_bricks(const uint32_t *nch, uint64_t n_bricks, uint32_t *list,
-                                                      unsigned long long *count, const unsigned long long *over)
+__global__ __launch_bounds__(BT) void k_active_bricks(const unsigned long long *cnt, uint64_t n_bricks, TierLists t,
+                                                      const unsigned long long *over)
 {
     if (*over)
         return;
     __shared__ uint32_t sh[NWAVE];
-    __shared__ unsigned long long sbase;
+    __shared__ unsigned long long sbase[4];
     const uint64_t span = (uint64_t)BT * ACT_PER;
     for (uint64_t b0 = (uint64_t)blockIdx.x * span; b0 < n_bricks; b0 += (uint64_t)gridDim.x * span)
     {
         const uint64_t first = b0 + (uint64_t)threadIdx.x * ACT_PER;
-        uint32_t mine = 0;
+        uint32_t mine[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
         for (int e = 0; e < ACT_PER; ++e)
-            mine |= (first + e < n_bricks && nch[first + e] != 0u) ? (1u << e) : 0u;
-        uint32_t tot;
-        const uint32_t ex = block_excl_scan((uint32_t)__popc(mine), sh, tot);
-        if (threadIdx.x == 0)
-            sbase = tot ? atomicAdd(count, (unsigned long long)tot) : 0ull;
-        __syncthreads();
-        uint64_t pos = sbase + ex;
+        {
+            const unsigned long long c = first + e < n_bricks ? cnt[first + e] : 0ull;
+            if ((uint32_t)c == 0u)
+                continue;
+            const uint32_t nr = (uint32_t)(c >> 32);
+            const int tier = nr <= (uint32_t)FOLD_SMALL_CAP ? 0
+                             : nr <= (uint32_t)FOLD_SMALL ? 1
+                             : nr <= (uint32_t)FOLD_MID   ? 2
+                                                          : 3;
+            mine[tier] |= 1u << e;
+        }
 #pragma unroll
-        for (int e = 0; e < ACT_PER; ++e)
-            if (mine & (1u << e))
-                list[pos++] = (uint32_t)(first + e);
-        __syncthreads();
+        for (int tier = 0; tier < 4; ++tier)
+        {
+            uint32_t tot;
+            const uint32_t ex = block_excl_scan((uint32_t)__popc(mine[tier]), sh, tot);
+            if (threadIdx.x == 0)
+                sbase[tier] = tot ? atomicAdd(t.count[tier], (unsigned long long)tot) : 0ull;
+            __syncthreads();
+            uint64_t pos = sbase[tier] + ex;
+#pragma unroll
+            for (int e = 0; e < ACT_PER; ++e)
+                if (mine[tier] & (1u << e))
+                    t.list[tier][pos++] = (uint32_t)(first + e);
+            __syncthreads();
+        }
     }
 }
 
@@ -2574,18 +2603,18 @@ struct FoldPick<1>
     using type = FoldStateC;
 };
 
-constexpr int FOLD_SMALL = 1536; // records of a brick held in LDS by the main fold kernel (25 KB: 6 blocks per CU)
-constexpr int FOLD_MID = 3584;   // ... by its second launch over the bricks beyond that (3 blocks per CU)
 constexpr int FOLD_WAVES = 6;    // occupancy of the reference-path main fold (waves per SIMD: at most 80 VGPRs)
 constexpr int FOLD_CAP = 4096;   // records held in LDS at a time by the kernel for large bricks (2 blocks per CU)
 
-// LDS view of a batch of up to BT chunks of one brick
-struct ChunkBatch
+// LDS view of a batch of up to NT chunks of one brick (NT = threads of the block)
+template <int NT>
+struct ChunkBatchT
 {
-    uint32_t start[BT];
-    uint32_t incl[BT];
-    uint32_t sh[NWAVE];
+    uint32_t start[NT];
+    uint32_t incl[NT];
+    uint32_t sh[NT / 64];
 };
+using ChunkBatch = ChunkBatchT<BT>;
 
 // one brick's records in LDS and the per-voxel grouping built over them
 template <int CAP>
@@ -2801,12 +2830,13 @@ __device__ __forceinline__ void brick_origin(const FoldArgs &a, uint32_t brick, 
     z0 = a.g.gmin[2] + (bz << a.bs);
 }
 
-// LDS of the small-brick fold: the brick's records grouped by voxel and,
-// inside each voxel, in key order
-template <int CAP>
+// LDS of the register-grouped brick fold (blocks of NT threads): the brick's
+// records grouped by voxel and, inside each voxel, in key order
+template <int CAP, int NT = BT>
 struct FoldLdsG
 {
-    // 12 B per record and 16-bit offsets: 25 KB at 1536 records, 6 blocks per CU
+    // 12 B per record and 16-bit offsets: 25 KB at 1536 records (6 blocks
+    // of 256 threads per CU), 15 KB at 768 (10 blocks of 128 threads)
     static_assert(CAP <= 65535, "16-bit record offsets");
     uint64_t key[CAP];
     float z[CAP];
@@ -2815,8 +2845,8 @@ struct FoldLdsG
     uint16_t vlist[512]; // voxels with records
     uint64_t rbits[64];  // occupancy word of each (y, z) row of the brick
     uint32_t roff[64];   // ... and its first voxel id
-    uint32_t sh[NWAVE];
-    ChunkBatch cb;
+    uint32_t sh[NT / 64];
+    ChunkBatchT<NT> cb;
 };
 
 // The brick's n records are in registers (record g = tid + k*BT in slot k)
@@ -2824,30 +2854,41 @@ struct FoldLdsG
 // segment, is ranked there among the voxel's keys (keys are unique) and
 // moved to its rank; then one thread per voxel with records replays them in
 // key order and solves.  Contains barriers.
-template <int CAP, int P>
-__device__ __forceinline__ void fold_grouped(const FoldArgs &a, FoldLdsG<CAP> &L, uint32_t n,
-                                             const uint64_t (&rk)[CAP / BT], const float (&rz)[CAP / BT],
-                                             const uint32_t (&rl)[CAP / BT], int x0, int y0, int z0)
+template <int CAP, int P, int NT = BT>
+__device__ __forceinline__ void fold_grouped(const FoldArgs &a, FoldLdsG<CAP, NT> &L, uint32_t n,
+                                             const uint64_t (&rk)[CAP / NT], const float (&rz)[CAP / NT],
+                                             const uint32_t (&rl)[CAP / NT], int x0, int y0, int z0)
 {
-    constexpr int PER = CAP / BT;
+    constexpr int PER = CAP / NT;
+    constexpr int NW = NT / 64;
+    constexpr int VPT = 512 / NT; // voxel slots of the brick per thread
     const int tid = threadIdx.x;
     uint32_t nv;
     {
-        const uint32_t c0 = L.cur[2 * tid], c1 = L.cur[2 * tid + 1];
+        uint32_t c[VPT], sum = 0, nz = 0;
+#pragma unroll
+        for (int j = 0; j < VPT; ++j)
+        {
+            c[j] = L.cur[VPT * tid + j];
+            sum += c[j];
+            nz += c[j] ? 1u : 0u;
+        }
         uint32_t tot;
-        const uint32_t ex = block_excl_scan(c0 + c1, L.sh, tot);
-        L.off[2 * tid] = (uint16_t)ex;
-        L.off[2 * tid + 1] = (uint16_t)(ex + c0);
-        L.cur[2 * tid] = ex;
-        L.cur[2 * tid + 1] = ex + c0;
+        uint32_t ex = block_excl_scan_w<NW>(sum, L.sh, tot);
+#pragma unroll
+        for (int j = 0; j < VPT; ++j)
+        {
+            L.off[VPT * tid + j] = (uint16_t)ex;
+            L.cur[VPT * tid + j] = ex;
+            ex += c[j];
+        }
         if (tid == 0)
             L.off[512] = (uint16_t)tot;
-        const uint32_t nz = (c0 ? 1u : 0u) + (c1 ? 1u : 0u);
-        const uint32_t exv = block_excl_scan(nz, L.sh, nv);
-        if (c0)
-            L.vlist[exv] = (uint16_t)(2 * tid);
-        if (c1)
-            L.vlist[exv + (c0 ? 1u : 0u)] = (uint16_t)(2 * tid + 1);
+        uint32_t exv = block_excl_scan_w<NW>(nz, L.sh, nv);
+#pragma unroll
+        for (int j = 0; j < VPT; ++j)
+            if (c[j])
+                L.vlist[exv++] = (uint16_t)(VPT * tid + j);
     }
     __syncthreads();
     // keys into their voxels' segments, in any order (the slot loops stop
@@ -2855,9 +2896,9 @@ __device__ __forceinline__ void fold_grouped(const FoldArgs &a, FoldLdsG<CAP> &L
 #pragma unroll
     for (int k = 0; k < PER; ++k)
     {
-        if ((uint32_t)(k * BT) >= n)
+        if ((uint32_t)(k * NT) >= n)
             break;
-        if ((uint32_t)(tid + k * BT) < n)
+        if ((uint32_t)(tid + k * NT) < n)
             L.key[atomicAdd(&L.cur[rl[k]], 1u)] = rk[k];
     }
     __syncthreads();
@@ -2866,9 +2907,9 @@ __device__ __forceinline__ void fold_grouped(const FoldArgs &a, FoldLdsG<CAP> &L
     for (int k = 0; k < PER; ++k)
     {
         pos[k] = 0;
-        if ((uint32_t)(k * BT) >= n)
+        if ((uint32_t)(k * NT) >= n)
             break;
-        if ((uint32_t)(tid + k * BT) < n)
+        if ((uint32_t)(tid + k * NT) < n)
         {
             const uint32_t s0 = L.off[rl[k]], s1 = L.off[rl[k] + 1];
             uint32_t rank = 0;
@@ -2881,16 +2922,16 @@ __device__ __forceinline__ void fold_grouped(const FoldArgs &a, FoldLdsG<CAP> &L
 #pragma unroll
     for (int k = 0; k < PER; ++k)
     {
-        if ((uint32_t)(k * BT) >= n)
+        if ((uint32_t)(k * NT) >= n)
             break;
-        if ((uint32_t)(tid + k * BT) < n)
+        if ((uint32_t)(tid + k * NT) < n)
         {
             L.key[pos[k]] = rk[k];
             L.z[pos[k]] = rz[k];
         }
     }
     __syncthreads();
-    for (uint32_t vi = tid; vi < nv; vi += BT)
+    for (uint32_t vi = tid; vi < nv; vi += NT)
     {
         const int l = L.vlist[vi];
         const uint32_t s0 = L.off[l], s1 = L.off[l + 1];
@@ -2931,22 +2972,20 @@ __device__ __forceinline__ void fold_grouped(const FoldArgs &a, FoldLdsG<CAP> &L
     __syncthreads();
 }
 
-// One workgroup per brick of a list, for bricks of at most CAP records: the
-// records are read once into registers, grouped and ordered in LDS,
-// replayed and solved there.  MID = false: the bricks with records, larger
-// ones listed in `large`; MID = true: the `large` list, larger ones listed
-// in `huge` (for k_fold_large).
-template <int P, int CAP, bool MID>
-__device__ __forceinline__ void fold_bricks_body(const FoldArgs &ain)
+// One workgroup of NT threads per brick of `list`, for bricks of at most CAP
+// records: the records are read once into registers, grouped and ordered
+// in LDS, replayed and solved there.  Larger bricks are appended to `next`
+// for the next tier; with count_recs the records folded here are counted.
+template <int P, int CAP, int NT>
+__device__ __forceinline__ void fold_bricks_body(const FoldArgs &ain, const uint32_t *list, uint64_t n_list,
+                                                 uint32_t *next, unsigned long long *n_next, bool count_recs)
 {
-    __shared__ FoldLdsG<CAP> L;
-    constexpr int PER = CAP / BT;
-    static_assert(CAP % BT == 0, "record slots per thread");
+    __shared__ FoldLdsG<CAP, NT> L;
+    constexpr int PER = CAP / NT;
+    constexpr int NW = NT / 64;
+    static_assert(CAP % NT == 0, "record slots per thread");
+    static_assert(NT >= 64 && NT <= BT && 512 % NT == 0, "block size");
     const int tid = threadIdx.x;
-    const uint32_t *list = MID ? ain.large : ain.active;
-    const uint64_t n_list = MID ? *ain.n_large : *ain.n_active;
-    uint32_t *next = MID ? ain.huge : ain.large;
-    unsigned long long *n_next = MID ? ain.n_huge : ain.n_large;
     for (uint64_t bi = blockIdx.x; bi < n_list; bi += gridDim.x)
     {
         const uint32_t brick = list[bi];
@@ -2956,10 +2995,10 @@ __device__ __forceinline__ void fold_bricks_body(const FoldArgs &ain)
         int x0, y0, z0;
         brick_origin(a, lb, x0, y0, z0);
         load_rows(a, L, x0, y0, z0);
-        for (int i = tid; i < 512; i += BT)
+        for (int i = tid; i < 512; i += NT)
             L.cur[i] = 0;
-        // the brick's chunks, BT at a time: record g of the brick goes to
-        // slot g / BT of thread g % BT
+        // the brick's chunks, NT at a time: record g of the brick goes to
+        // slot g / NT of thread g % NT
         uint64_t rk[PER];
         float rz[PER];
         uint32_t rl[PER];
@@ -2972,9 +3011,9 @@ __device__ __forceinline__ void fold_bricks_body(const FoldArgs &ain)
         }
         uint32_t n = 0;
         bool over = false;
-        for (uint32_t cbeg = c0; cbeg < c1; cbeg += BT)
+        for (uint32_t cbeg = c0; cbeg < c1; cbeg += NT)
         {
-            const int nb = (int)(c1 - cbeg < (uint32_t)BT ? c1 - cbeg : (uint32_t)BT);
+            const int nb = (int)(c1 - cbeg < (uint32_t)NT ? c1 - cbeg : (uint32_t)NT);
             uint32_t cnt = 0;
             if (tid < nb)
             {
@@ -2983,7 +3022,7 @@ __device__ __forceinline__ void fold_bricks_body(const FoldArgs &ain)
                 cnt = ch.y;
             }
             uint32_t tot;
-            const uint32_t ex = block_excl_scan(cnt, L.cb.sh, tot);
+            const uint32_t ex = block_excl_scan_w<NW>(cnt, L.cb.sh, tot);
             if (n + tot > (uint32_t)CAP)
             {
                 over = true;
@@ -2995,11 +3034,11 @@ __device__ __forceinline__ void fold_bricks_body(const FoldArgs &ain)
             for (int k = 0; k < PER; ++k)
             {
                 // slots of this batch's records only
-                if ((uint32_t)(k * BT) >= n + tot)
+                if ((uint32_t)(k * NT) >= n + tot)
                     break;
-                if ((uint32_t)((k + 1) * BT) <= n)
+                if ((uint32_t)((k + 1) * NT) <= n)
                     continue;
-                const uint32_t g = (uint32_t)(tid + k * BT);
+                const uint32_t g = (uint32_t)(tid + k * NT);
                 if (g >= n && g < n + tot)
                 {
                     const uint32_t j = g - n;
@@ -3022,36 +3061,49 @@ __device__ __forceinline__ void fold_bricks_body(const FoldArgs &ain)
             __syncthreads();
             continue;
         }
-        if (MID && tid == 0)
+        if (count_recs && tid == 0)
             atomicAdd(a.ctr + CTR_LARGE_RECS, (unsigned long long)n);
-        fold_grouped<CAP, P>(a, L, n, rk, rz, rl, x0, y0, z0);
+        fold_grouped<CAP, P, NT>(a, L, n, rk, rz, rl, x0, y0, z0);
     }
 }
 
-// the kernels: the main tile for the reference path (6 waves per SIMD: its
-// LDS and at most 80 VGPRs leave room for 6 blocks per CU), the main tile for
-// the precision variant, and the mid-size tile
-// (a conversion whose record buffers overflowed skips the fold: the host
-// sizes the buffers again and re-runs its record stages)
+// The kernels, one per tier of bricks by record count (a conversion whose
+// record buffers overflowed skips the fold: the host sizes the buffers
+// again and re-runs its record stages):
+//  - k_fold_small: bricks of up to FOLD_SMALL_CAP records, in blocks of 128
+//    threads (5 waves per SIMD and 15 KB of LDS: 10 blocks per CU), so that
+//    a CU folds 10 bricks at a time;
+//  - k_fold_bricks(_ref): bricks of up to FOLD_SMALL records (25 KB of LDS:
+//    6 blocks of 256 threads per CU);
+//  - k_fold_mid: bricks of up to FOLD_MID records (3 blocks per CU).
+// Each brick is listed for its tier by k_active_bricks; a brick too large
+// for a tier's tile (not expected: the counts are exact) goes to the next.
+template <int P>
+__global__ __launch_bounds__(FOLD_SMALL_NT) __attribute__((amdgpu_waves_per_eu(5))) void k_fold_small(FoldArgs a)
+{
+    if (a.ctr[CTR_OVER])
+        return;
+    fold_bricks_body<P, FOLD_SMALL_CAP, FOLD_SMALL_NT>(a, a.active, *a.n_active, a.medium, a.n_medium, false);
+}
 template <int P>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(5))) void k_fold_bricks(FoldArgs a)
 {
     if (a.ctr[CTR_OVER])
         return;
-    fold_bricks_body<P, FOLD_SMALL, false>(a);
+    fold_bricks_body<P, FOLD_SMALL, BT>(a, a.medium, *a.n_medium, a.large, a.n_large, false);
 }
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(FOLD_WAVES))) void k_fold_bricks_ref(FoldArgs a)
 {
     if (a.ctr[CTR_OVER])
         return;
-    fold_bricks_body<0, FOLD_SMALL, false>(a);
+    fold_bricks_body<0, FOLD_SMALL, BT>(a, a.medium, *a.n_medium, a.large, a.n_large, false);
 }
 template <int P>
 __global__ __launch_bounds__(BT) void k_fold_mid(FoldArgs a)
 {
     if (a.ctr[CTR_OVER])
         return;
-    fold_bricks_body<P, FOLD_MID, true>(a);
+    fold_bricks_body<P, FOLD_MID, BT>(a, a.large, *a.n_large, a.huge, a.n_huge, true);
 }
 
 // Bricks with more records than k_fold_bricks holds.  A brick
@@ -3632,16 +3684,23 @@ hipError_t launch_place_chunks(const ShardBuf<Chunk> &chunks, const uint32_t *ch
     return hipGetLastError();
 }
 
-hipError_t launch_active_bricks(const uint32_t *nch, uint64_t n_bricks, uint32_t *list, unsigned long long *count,
+hipError_t launch_active_bricks(const unsigned long long *cnt, uint64_t n_bricks, TierLists t,
                                 const unsigned long long *over, hipStream_t s)
 {
     hipLaunchKernelGGL(k_active_bricks, dim3(grid_for((int64_t)((n_bricks + ACT_PER - 1) / ACT_PER), 65536)), dim3(BT),
-                       0, s, nch, n_bricks, list, count, over);
+                       0, s, cnt, n_bricks, t, over);
     return hipGetLastError();
 }
 
 hipError_t launch_fold_bricks(const FoldArgs &a, uint64_t max_bricks, hipStream_t s)
 {
+    uint64_t ns = max_bricks < 131072ull ? max_bricks : 131072ull;
+    if (ns == 0)
+        ns = 1;
+    if (a.prec)
+        hipLaunchKernelGGL(k_fold_small<1>, dim3((unsigned)ns), dim3(FOLD_SMALL_NT), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_fold_small<0>, dim3((unsigned)ns), dim3(FOLD_SMALL_NT), 0, s, a);
     uint64_t nb = max_bricks < 65536ull ? max_bricks : 65536ull;
     if (nb == 0)
         nb = 1;
@@ -3751,6 +3810,11 @@ struct LoadU32
     const uint32_t *a;
     __device__ __forceinline__ uint32_t operator()(uint64_t i) const { return a[i]; }
 };
+struct LoadLo32
+{
+    const unsigned long long *a;
+    __device__ __forceinline__ uint32_t operator()(uint64_t i) const { return (uint32_t)a[i]; }
+};
 
 template <class Load>
 static hipError_t scan_impl(Load ld, uint64_t n, uint32_t *out, uint64_t *tile_sums, unsigned long long *total,
@@ -3771,6 +3835,12 @@ hipError_t launch_scan_popc(const uint64_t *bits, uint64_t n, uint32_t *out, uin
                             unsigned long long *total, hipStream_t s)
 {
     return scan_impl(LoadPopc{bits}, n, out, tile_sums, total, s);
+}
+
+hipError_t launch_scan_lo32(const unsigned long long *in, uint64_t n, uint32_t *out, uint64_t *tile_sums,
+                            unsigned long long *total, hipStream_t s)
+{
+    return scan_impl(LoadLo32{in}, n, out, tile_sums, total, s);
 }
 
 hipError_t launch_scan_u32(const uint32_t *in, uint64_t n, uint32_t *out, uint64_t *tile_sums,
