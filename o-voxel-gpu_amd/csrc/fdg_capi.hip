@@ -1118,9 +1118,13 @@ ovx_status enqueue_finish(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t 
     HIP_TRY(launch_fold_mid(a, n_bricks, s));
     HIP_TRY(launch_fold_large(a, n_bricks, s));
     toc(c, KS_FOLD_LARGE, s);
-    tic(c, KS_SOLVE, s);
-    HIP_TRY(launch_solve_voxels(a, coords, (uint64_t)c->N, s));
-    toc(c, KS_SOLVE, s);
+    // the reference path's folds solved every voxel; the precision variant solves from its stored QEFs
+    if (c->prec)
+    {
+        tic(c, KS_SOLVE, s);
+        HIP_TRY(launch_solve_voxels(a, coords, (uint64_t)c->N, s));
+        toc(c, KS_SOLVE, s);
+    }
     tic(c, KS_SOLVE_BOX, s);
     HIP_TRY(launch_solve_box(a, s));
     toc(c, KS_SOLVE_BOX, s);
@@ -1283,11 +1287,12 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
             (st = ensure(c->tiles, (scan_tile_count(n_bricks + 1) + 64) * 8)))
             return st;
         {
-            // hits first, then (after the records are made) the QEFs and the box-search queue, which is a
-            // few percent of the voxels typically: the solve kernel solves in place whatever does not fit
+            // hits first, then (after the records are made) the box-search queue, which is a few percent of
+            // the voxels typically (whatever does not fit is solved in place), after the QEFs of the precision
+            // variant (the reference path solves each voxel where it is folded and stores no QEF)
             const size_t fh = align_up(size_shard(c, SB_FHITS, c->fw > 0.0f ? want_fhit : 1, sizeof(FHit)), 256);
             const size_t bh = size_shard(c, SB_BHITS, c->E > 0 ? want_bhit : 1, sizeof(BHit));
-            const size_t q = align_up((size_t)c->N * (c->prec ? 80 : 40), 256);
+            const size_t q = c->prec ? align_up((size_t)c->N * 80, 256) : 0;
             const size_t bx = size_shard(c, SB_BOX, per_for((uint64_t)c->N / 8 + 4096), sizeof(BoxItem));
             if ((st = ensure(c->hitqef, std::max(fh + bh, q + bx), false)))
                 return st;

@@ -353,6 +353,7 @@ hipError_t launch_fold_bricks(const FoldArgs &a, uint64_t max_bricks, hipStream_
 hipError_t launch_fold_mid(const FoldArgs &a, uint64_t max_bricks, hipStream_t s);
 // bricks of the `huge` list: whole, by voxel ranges, or a giant voxel in key-ordered batches
 hipError_t launch_fold_large(const FoldArgs &a, uint64_t max_bricks, hipStream_t s);
+// dual vertices of the precision variant from its stored QEFs (the reference path solves in its folds)
 hipError_t launch_solve_voxels(const FoldArgs &a, const int32_t *coords, uint64_t n, hipStream_t s);
 hipError_t launch_solve_box(const FoldArgs &a, hipStream_t s);
 hipError_t launch_edge_count(const MeshDev &m, uint32_t *vcnt, uint32_t *rank, const uint32_t *voff, uint32_t *nbr,

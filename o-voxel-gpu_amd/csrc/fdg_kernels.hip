@@ -2073,7 +2073,7 @@ __global__ __launch_bounds__(BT) void k_place_chunks(ShardBuf<Chunk> chunks, con
 // fold tiers by records per brick (held in LDS by the tier's kernel)
 constexpr int FOLD_SMALL_CAP = 768; // first tier: blocks of FOLD_SMALL_NT threads (15 KB of LDS)
 constexpr int FOLD_SMALL_NT = 128;
-constexpr int FOLD_SMALL = 1536; // second tier (25 KB: 6 blocks per CU)
+constexpr int FOLD_SMALL = 1536; // second tier (25 KB of LDS, 5 blocks per CU)
 constexpr int FOLD_MID = 3584;   // third tier (3 blocks per CU); larger bricks: k_fold_large
 
 // bricks that received records, each appended to the list of the fold tier
@@ -2392,6 +2392,72 @@ struct FoldState
         const uint64_t w = word_of(a.g, x, y, z, b);
         finish_at((uint64_t)voxel_id(a.g, w, b), a);
     }
+
+    // Regularised quadric and flags of voxel (x, y, z) in output row vid,
+    // and its dual vertex solved here from the registers (FDG.cpp:763-983):
+    // the unconstrained minimiser when it lies in the voxel, else the voxel
+    // is queued for the box-constrained search (solved in place when its
+    // queue region is full).  The active lanes of the wave claim queue space
+    // together (any subset of the wave's lanes may call it).
+    __device__ __forceinline__ void finish_out(uint64_t vid, int x, int y, int z, const FoldArgs &a)
+    {
+        if (a.rw > 0.0f && cnt > 0.0f)
+        {
+            const float p0 = m0 / cnt, p1 = m1 / cnt, p2 = m2 / cnt;
+            const float s = a.rw * cnt;
+            const float R[10] = {1.0f, 0.0f, 0.0f, -p0, 1.0f, 0.0f, -p1, 1.0f, -p2, dot3f(p0, p1, p2, p0, p1, p2)};
+#pragma unroll
+            for (int k = 0; k < 10; ++k)
+                q[k] = q[k] + s * R[k];
+        }
+        uint8_t *fo = a.flags + 3 * vid;
+        fo[0] = (uint8_t)(fl & 1);
+        fo[1] = (uint8_t)((fl >> 1) & 1);
+        fo[2] = (uint8_t)((fl >> 2) & 1);
+        const int c[3] = {x, y, z};
+        const float vs[3] = {a.g.vs[0], a.g.vs[1], a.g.vs[2]};
+        float d[3];
+        const bool inside = solve_free(q, c, vs, d);
+        const uint64_t m = __ballot(!inside);
+        if (m)
+        {
+            const int leader = __ffsll((long long)m) - 1;
+            const int lane = (int)(threadIdx.x & 63);
+            unsigned long long base = ~0ull;
+            if (lane == leader)
+            {
+                uint64_t first;
+                const uint32_t shard = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % NSHARD;
+                base = shard_claim_fit(a.box, shard, (uint64_t)__popcll(m), first) ? first : ~0ull;
+            }
+            const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)base, leader);
+            const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(base >> 32), leader);
+            base = ((unsigned long long)hi << 32) | lo;
+            if (!inside)
+            {
+                if (base != ~0ull)
+                {
+                    BoxItem &e = a.box.items[base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull))];
+#pragma unroll
+                    for (int i = 0; i < 10; ++i)
+                        e.q[i] = q[i];
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+                    {
+                        e.v[i] = d[i];
+                        e.c[i] = c[i];
+                    }
+                    e.vid = vid;
+                    return;
+                }
+                solve_box(q, c, vs, d);
+            }
+        }
+        float *o = a.dual + 3 * vid;
+        o[0] = d[0];
+        o[1] = d[1];
+        o[2] = d[2];
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -2590,6 +2656,13 @@ struct FoldStateC
         fo[1] = (uint8_t)((fl >> 1) & 1);
         fo[2] = (uint8_t)((fl >> 2) & 1);
     }
+
+    // quadric (h and l parts) and flags of voxel (x, y, z) in output row vid;
+    // its dual vertex is solved by k_solve_voxels_c
+    __device__ __forceinline__ void finish_out(uint64_t vid, int, int, int, const FoldArgs &a)
+    {
+        finish_at(vid, a);
+    }
 };
 
 template <int P>
@@ -2603,7 +2676,7 @@ struct FoldPick<1>
     using type = FoldStateC;
 };
 
-constexpr int FOLD_WAVES = 6;    // occupancy of the reference-path main fold (waves per SIMD: at most 80 VGPRs)
+constexpr int FOLD_WAVES = 5;    // occupancy of the reference-path main fold (waves per SIMD: at most 102 VGPRs)
 constexpr int FOLD_CAP = 4096;   // records held in LDS at a time by the kernel for large bricks (2 blocks per CU)
 
 // LDS view of a batch of up to NT chunks of one brick (NT = threads of the block)
@@ -2768,7 +2841,7 @@ __device__ __forceinline__ void fold_loaded(const FoldArgs &a, FoldLds<CAP> &L, 
                 if (i + u < s1)
                     st.add_pl(kk[u], zz[u], pp[u], a);
         }
-        st.finish_at(row_vid(a, L, x0, l), a);
+        st.finish_out(row_vid(a, L, x0, l), x0 + (l & 7), y0 + ((l >> 3) & 7), z0 + (l >> 6), a);
     }
     __syncthreads();
 }
@@ -2967,7 +3040,7 @@ __device__ __forceinline__ void fold_grouped(const FoldArgs &a, FoldLdsG<CAP, NT
                 if (i + u < s1)
                     st.add_pl(kk[u], zz[u], pp[u], a);
         }
-        st.finish_at(row_vid(a, L, x0, l), a);
+        st.finish_out(row_vid(a, L, x0, l), x0 + (l & 7), y0 + ((l >> 3) & 7), z0 + (l >> 6), a);
     }
     __syncthreads();
 }
@@ -3074,7 +3147,7 @@ __device__ __forceinline__ void fold_bricks_body(const FoldArgs &ain, const uint
 //    threads (5 waves per SIMD and 15 KB of LDS: 10 blocks per CU), so that
 //    a CU folds 10 bricks at a time;
 //  - k_fold_bricks(_ref): bricks of up to FOLD_SMALL records (25 KB of LDS:
-//    6 blocks of 256 threads per CU);
+//    5 blocks of 256 threads per CU);
 //  - k_fold_mid: bricks of up to FOLD_MID records (3 blocks per CU).
 // Each brick is listed for its tier by k_active_bricks; a brick too large
 // for a tier's tile (not expected: the counts are exact) goes to the next.
@@ -3240,7 +3313,8 @@ __global__ __launch_bounds__(BT) void k_fold_large(FoldArgs ain)
                 }
                 if (tid == 0)
                 {
-                    st.finish_at(row_vid(a, L, x0, (int)lv), a);
+                    st.finish_out(row_vid(a, L, x0, (int)lv), x0 + (int)(lv & 7), y0 + (int)((lv >> 3) & 7),
+                                  z0 + (int)(lv >> 6), a);
                     atomicAdd(a.ctr + CTR_GIANT, 1ull);
                 }
                 __syncthreads();
@@ -3305,89 +3379,12 @@ __global__ __launch_bounds__(BT) void k_fold_large(FoldArgs ain)
                     const uint32_t j = L.perm[i];
                     st.add(L.key[j], L.z[j], a);
                 }
-                st.finish_at(row_vid(a, L, x0, lv), a);
+                st.finish_out(row_vid(a, L, x0, lv), x0 + (lv & 7), y0 + ((lv >> 3) & 7), z0 + (lv >> 6), a);
             }
             __syncthreads();
             lo = hi;
         }
         __syncthreads();
-    }
-}
-
-// Dual vertex of every voxel, one per lane, from its folded quadric: the
-// unconstrained minimiser when it lies in the voxel, else the voxel is
-// queued for the box-constrained search (solved in place when its queue
-// region is full: the claim is all-or-nothing, so the queue holds only
-// complete items).
-__global__ __launch_bounds__(BT) void k_solve_voxels(const float *qef, const int32_t *coords, uint64_t n, BatchDev bt,
-                                                     float *dual, ShardBuf<BoxItem> box, const unsigned long long *over)
-{
-    if (*over)
-        return;
-    const uint64_t stride = (uint64_t)gridDim.x * BT;
-    const int lane = threadIdx.x & 63;
-    const uint32_t shard = (blockIdx.x * NWAVE + (threadIdx.x >> 6)) % NSHARD;
-    // every lane of a wave runs the same number of iterations (the claim is wave-wide)
-    const uint64_t n_round = (n + 63ull) & ~63ull;
-    for (uint64_t vid = (uint64_t)blockIdx.x * BT + threadIdx.x; vid < n_round; vid += stride)
-    {
-        const bool live = vid < n;
-        float q[10];
-        int c[3] = {0, 0, 0};
-        float d[3] = {0.0f, 0.0f, 0.0f};
-        float vs[3] = {1.0f, 1.0f, 1.0f};
-        bool inside = true;
-        if (live)
-        {
-            const GridDev &g = bt.seg[mesh_of_voxel(bt, vid)].g; // the voxel's mesh
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-                vs[i] = g.vs[i];
-#pragma unroll
-            for (int i = 0; i < 10; ++i)
-                q[i] = qef[10 * vid + i];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-                c[i] = coords[3 * vid + i];
-            inside = solve_free(q, c, vs, d);
-        }
-        const uint64_t m = __ballot(!inside);
-        uint64_t base = ~0ull;
-        if (m)
-        {
-            const int leader = __ffsll((long long)m) - 1;
-            if (lane == leader)
-            {
-                uint64_t first;
-                base = shard_claim_fit(box, shard, (uint64_t)__popcll(m), first) ? first : ~0ull;
-            }
-            base = __shfl(base, leader, 64);
-        }
-        if (!live)
-            continue;
-        if (!inside)
-        {
-            if (base != ~0ull)
-            {
-                BoxItem &e = box.items[base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull))];
-#pragma unroll
-                for (int i = 0; i < 10; ++i)
-                    e.q[i] = q[i];
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-                {
-                    e.v[i] = d[i];
-                    e.c[i] = c[i];
-                }
-                e.vid = vid;
-                continue;
-            }
-            solve_box(q, c, vs, d);
-        }
-        float *o = dual + 3 * vid;
-        o[0] = d[0];
-        o[1] = d[1];
-        o[2] = d[2];
     }
 }
 
@@ -3725,14 +3722,9 @@ hipError_t launch_fold_mid(const FoldArgs &a, uint64_t max_bricks, hipStream_t s
 
 hipError_t launch_solve_voxels(const FoldArgs &a, const int32_t *coords, uint64_t n, hipStream_t s)
 {
-    if (a.prec)
-    {
-        hipLaunchKernelGGL(k_solve_voxels_c, dim3(grid_for((int64_t)n, 65536)), dim3(BT), 0, s, (const float *)a.qef,
-                           coords, n, a.b, a.dual, a.cond_max, a.n64, (const unsigned long long *)(a.ctr + CTR_OVER));
-        return hipGetLastError();
-    }
-    hipLaunchKernelGGL(k_solve_voxels, dim3(grid_for((int64_t)n, 65536)), dim3(BT), 0, s, (const float *)a.qef,
-                       coords, n, a.b, a.dual, a.box, (const unsigned long long *)(a.ctr + CTR_OVER));
+    // precision variant only: the reference path's folds solve each voxel where it is folded
+    hipLaunchKernelGGL(k_solve_voxels_c, dim3(grid_for((int64_t)n, 65536)), dim3(BT), 0, s, (const float *)a.qef,
+                       coords, n, a.b, a.dual, a.cond_max, a.n64, (const unsigned long long *)(a.ctr + CTR_OVER));
     return hipGetLastError();
 }
 
