@@ -122,6 +122,7 @@ struct ovx_fdg_ctx
     double face_hit_ratio = 1.0;          // face hits of the last conversion over the prep's estimate (capacity estimate)
     double bhit_per_edge = 16.0;          // boundary hits per boundary edge seen last time (capacity estimate)
     double chunk_per_rec = 0.1;           // record chunks per record (decaying maximum of the conversions seen; capacity estimate)
+    double rec_per_event = 1.5;           // crossing records (one per crossing and brick) per crossing (same; 1..4)
     uint64_t face_geo = 0;                // face-test hits expected from the faces' sizes (prep of the last begin)
     uint64_t n_events = 0;                // crossings recorded by the last mark pass
     int64_t E = 0;           // boundary edges of the last begin
@@ -1243,16 +1244,19 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
 
     // ---- records grouped by brick (bricks of all meshes, numbered mesh after mesh)
     const uint64_t n_bricks = c->n_bricks;
-    // phase-0 records are exactly 4 per crossing; face hits are estimated from
-    // the faces' sizes (prep) scaled by the last conversion's hits over that
-    // estimate, boundary hits from the last conversion's hits per boundary
-    // edge, the chunks from its chunks per record.  Hit regions take 15%
-    // above the shard mean for the uneven spread of the claims; an overflow
-    // is detected on the device and the record stages run again with exact
-    // sizes.
+    // phase-0 records are 1 to 4 per crossing (one per brick of its voxels),
+    // estimated from the records per crossing seen before; face hits are
+    // estimated from the faces' sizes (prep) scaled by the last conversion's
+    // hits over that estimate, boundary hits from the last conversion's hits
+    // per boundary edge, the chunks from its chunks per record.  Hit regions
+    // take 15% above the shard mean for the uneven spread of the claims; an
+    // overflow is detected on the device and the record stages run again
+    // with exact sizes.
     const uint64_t face_est = (uint64_t)(c->face_hit_ratio * (double)c->face_geo) + 4096;
     const uint64_t bhit_est = (uint64_t)(c->bhit_per_edge * (double)c->E) + 4096;
-    const uint64_t rec_est = 4ull * c->n_events + face_est + bhit_est;
+    const uint64_t ev_est = std::min<uint64_t>(4ull * c->n_events, (uint64_t)(c->rec_per_event * (double)c->n_events) +
+                                                                       c->n_events / 16 + 4096);
+    const uint64_t rec_est = ev_est + face_est + bhit_est;
     uint64_t want_rec = per_for(rec_est, 8);
     uint64_t want_fhit = per_for(face_est + face_est * 15 / 100);
     uint64_t want_bhit = per_for(bhit_est + bhit_est * 15 / 100);
@@ -1325,6 +1329,11 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
             // a slowly decaying maximum: conversions of other meshes on the context do not overflow at once
             if (sum[SB_RECS])
                 c->chunk_per_rec = std::max((double)sum[SB_CHUNKS] / (double)sum[SB_RECS], 0.9 * c->chunk_per_rec);
+            // every hit made one record: the rest are the crossings'
+            if (c->n_events && sum[SB_RECS] >= sum[SB_FHITS] + sum[SB_BHITS])
+                c->rec_per_event = std::min(
+                    4.0, std::max({1.0, (double)(sum[SB_RECS] - sum[SB_FHITS] - sum[SB_BHITS]) / (double)c->n_events,
+                                   0.9 * c->rec_per_event}));
             return OVX_OK;
         }
         const bool hits_over = mx[SB_FHITS] > c->per[SB_FHITS] || mx[SB_BHITS] > c->per[SB_BHITS];

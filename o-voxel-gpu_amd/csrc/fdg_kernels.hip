@@ -223,16 +223,20 @@ struct BucketLds
 };
 
 // Block-level bucketed emission.  Each thread passes K records with their
-// brick ids and validity flags.  Records of one brick are counted in LDS; the
-// block reserves space for all of its records and chunks with one atomic each
-// (on the counters of its shard), gives every brick it touched one contiguous
-// chunk, and writes the records contiguously.  Capacity misses are counted
-// (the totals stay exact) and nothing is written out of bounds.  Contains
-// barriers: every thread of the block must call it.
+// brick ids, validity flags and voxel entries (the voxels of the brick a
+// record stands for: 1 to 4 for a crossing, 1 otherwise).  Records of one
+// brick are counted in LDS (records in the low 16 bits of a table slot's
+// count, entries in the high 16); the block reserves space for all of its
+// records and chunks with one atomic each (on the counters of its shard),
+// gives every brick it touched one contiguous chunk, adds the chunk and its
+// entries to the brick's counts, and writes the records contiguously.
+// Capacity misses are counted (the totals stay exact) and nothing is written
+// out of bounds.  Contains barriers: every thread of the block must call it.
 template <int K, int LOG2>
 __device__ __forceinline__ void bucket_emit(const uint32_t (&bk)[K], const Rec (&rc)[K], const bool (&ok)[K],
-                                            const BucketArgs &a, BucketLds<LOG2> &L)
+                                            const uint32_t (&ent)[K], const BucketArgs &a, BucketLds<LOG2> &L)
 {
+    static_assert(BT * K < 65536 && 4 * BT * K < 65536, "records and entries of one emission in 16 bits each");
     constexpr int N = BucketLds<LOG2>::N;
     constexpr int PER = N / BT; // table entries per thread
     static_assert(N > BT * K, "bucket table too small");
@@ -264,10 +268,14 @@ __device__ __forceinline__ void bucket_emit(const uint32_t (&bk)[K], const Rec (
             led = led || (ok[j] && bk[j] == bk[k]);
         if (led)
             continue;
-        uint32_t n = 1;
+        uint32_t n = 1, ne = ent[k];
 #pragma unroll
         for (int j = k + 1; j < K; ++j)
-            n += (ok[j] && bk[j] == bk[k]) ? 1u : 0u;
+            if (ok[j] && bk[j] == bk[k])
+            {
+                ++n;
+                ne += ent[j];
+            }
         uint32_t h = (bk[k] * 2654435761u) >> (32 - LOG2);
         while (true)
         {
@@ -276,7 +284,7 @@ __device__ __forceinline__ void bucket_emit(const uint32_t (&bk)[K], const Rec (
                 break;
             h = (h + 1u) & (uint32_t)(N - 1);
         }
-        const uint32_t r0 = atomicAdd(&L.cnt[h], n);
+        const uint32_t r0 = atomicAdd(&L.cnt[h], n | (ne << 16)) & 0xffffu;
         slot[k] = (int)h;
         rank[k] = r0;
         uint32_t i = 1;
@@ -296,7 +304,7 @@ __device__ __forceinline__ void bucket_emit(const uint32_t (&bk)[K], const Rec (
 #pragma unroll
     for (int e = 0; e < PER; ++e)
     {
-        const uint32_t n = L.cnt[s0 + e];
+        const uint32_t n = L.cnt[s0 + e] & 0xffffu;
         my_n += n;
         my_c += n ? 1ull : 0ull;
     }
@@ -317,13 +325,13 @@ __device__ __forceinline__ void bucket_emit(const uint32_t (&bk)[K], const Rec (
 #pragma unroll
     for (int e = 0; e < PER; ++e)
     {
-        const uint32_t n = L.cnt[s0 + e];
+        const uint32_t n = L.cnt[s0 + e] & 0xffffu, ne = L.cnt[s0 + e] >> 16;
         if (n && fits)
         {
             // slots s0 .. s0+PER-1 are this thread's alone here
             const uint32_t b = L.key[s0 + e];
-            // one chunk and n records more for the brick; the old chunk count is this chunk's rank
-            const uint32_t r = (uint32_t)atomicAdd(a.brick_cnt + b, 1ull | ((unsigned long long)n << 32));
+            // one chunk and ne voxel entries more for the brick; the old chunk count is this chunk's rank
+            const uint32_t r = (uint32_t)atomicAdd(a.brick_cnt + b, 1ull | ((unsigned long long)ne << 32));
             a.chunks.items[cpos] = Chunk{(uint32_t)rpos, n, b, r};
             L.key[s0 + e] = (uint32_t)rpos;
         }
@@ -1926,7 +1934,10 @@ __global__ __launch_bounds__(BT) void k_bdry(int64_t E, const BEdge *be, const i
 // ---------------------------------------------------------------------------
 // record generation from the recorded crossings and boundary hits
 // ---------------------------------------------------------------------------
-// 4 phase-0 records per crossing, keys (face*3+axis, y, x, dx, dy)
+// phase-0 records: one per (crossing, brick of its 4 voxels), the crossing's
+// key without (dx, dy), its z and its voxel index zi along the axis (in the
+// lvox field); the fold expands it into its voxels inside the brick
+// (crossing_voxels), whose keys (face*3+axis, y, x, dx, dy) order them
 __global__ __launch_bounds__(BT) void k_gen_events(ShardBuf<Event> ev, BucketArgs ba)
 {
     __shared__ BucketLds<11> L;
@@ -1936,6 +1947,7 @@ __global__ __launch_bounds__(BT) void k_gen_events(ShardBuf<Event> ev, BucketArg
     {
         const uint64_t i = base + threadIdx.x;
         uint32_t bk[4] = {0, 0, 0, 0};
+        uint32_t ent[4] = {0, 0, 0, 0};
         Rec rc[4];
         bool ok[4] = {false, false, false, false};
         if (i < n)
@@ -1959,11 +1971,23 @@ __global__ __launch_bounds__(BT) void k_gen_events(ShardBuf<Event> ev, BucketArg
                 uint32_t lvox;
                 brick_of(sg, c[k][0], c[k][1], c[k][2], bk[k], lvox);
                 bk[k] += (uint32_t)sg.brick0;
-                rc[k] = Rec{e.key | (uint64_t)k, e.z, lvox};
-                ok[k] = true;
+                rc[k] = Rec{e.key, e.z, (uint32_t)e.zi};
+            }
+            // the first of the 4 voxels in each distinct brick stands for the brick's voxels
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+            {
+                bool first = true;
+#pragma unroll
+                for (int j = 0; j < k; ++j)
+                    first = first && bk[j] != bk[k];
+                ok[k] = first;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    ent[k] += bk[j] == bk[k] ? 1u : 0u;
             }
         }
-        bucket_emit<4, 11>(bk, rc, ok, ba, L);
+        bucket_emit<4, 11>(bk, rc, ok, ent, ba, L);
     }
 }
 
@@ -1994,33 +2018,40 @@ __global__ __launch_bounds__(BT) void k_gen_fhits(ShardBuf<FHit> h, BucketArgs b
                 ok[k] = true;
             }
         }
-        bucket_emit<4, 11>(bk, rc, ok, ba, L);
+        const uint32_t ent[4] = {1u, 1u, 1u, 1u};
+        bucket_emit<4, 11>(bk, rc, ok, ent, ba, L);
     }
 }
 
 // phase-2 records from the boundary DDA hits, key = boundary-edge rank
+// (4 consecutive hits per thread: mostly one edge's, in one or two bricks)
 __global__ __launch_bounds__(BT) void k_gen_bhits(ShardBuf<BHit> h, BucketArgs ba)
 {
-    __shared__ BucketLds<9> L;
+    __shared__ BucketLds<11> L;
     __shared__ QueueMap qm;
     const uint64_t n = queue_prefix(h, qm.pre, qm.sh);
-    for (uint64_t base = (uint64_t)blockIdx.x * BT; base < n; base += (uint64_t)gridDim.x * BT)
+    for (uint64_t base = (uint64_t)blockIdx.x * (4 * BT); base < n; base += (uint64_t)gridDim.x * (4 * BT))
     {
-        const uint64_t i = base + threadIdx.x;
-        uint32_t bk[1] = {0};
-        Rec rc[1];
-        bool ok[1] = {false};
-        if (i < n)
+        uint32_t bk[4] = {0, 0, 0, 0};
+        Rec rc[4];
+        bool ok[4] = {false, false, false, false};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
         {
-            const BHit x = h.items[queue_slot(h, qm.pre, i)];
-            const MeshSeg &sg = ba.b.seg[mesh_of_vert(ba.b, (int64_t)ba.bedge_v[x.edge].x)];
-            uint32_t lvox;
-            brick_of(sg, x.x, x.y, x.z, bk[0], lvox);
-            bk[0] += (uint32_t)sg.brick0;
-            rc[0] = Rec{(2ull << KEY_PHASE_SHIFT) | (uint64_t)x.edge, 0.0f, lvox};
-            ok[0] = true;
+            const uint64_t i = base + 4ull * threadIdx.x + (uint64_t)k;
+            if (i < n)
+            {
+                const BHit x = h.items[queue_slot(h, qm.pre, i)];
+                const MeshSeg &sg = ba.b.seg[mesh_of_vert(ba.b, (int64_t)ba.bedge_v[x.edge].x)];
+                uint32_t lvox;
+                brick_of(sg, x.x, x.y, x.z, bk[k], lvox);
+                bk[k] += (uint32_t)sg.brick0;
+                rc[k] = Rec{(2ull << KEY_PHASE_SHIFT) | (uint64_t)x.edge, 0.0f, lvox};
+                ok[k] = true;
+            }
         }
-        bucket_emit<1, 9>(bk, rc, ok, ba, L);
+        const uint32_t ent[4] = {1u, 1u, 1u, 1u};
+        bucket_emit<4, 11>(bk, rc, ok, ent, ba, L);
     }
 }
 
@@ -2708,6 +2739,39 @@ struct FoldLds
     ChunkBatch cb;
 };
 
+// Voxel entries of a record in the brick with min corner (x0, y0, z0) of the
+// current mesh: returns a mask of entries k (bit k) and their keys' low bits
+// and brick-local voxel indices.  A crossing record (phase 0) stands for the
+// crossing's 4 voxels (k = dx * 2 + dy, FDG.cpp:239-261) that lie in the
+// brick, with keys key | k; any other record is one entry, its own voxel.
+__device__ __forceinline__ uint32_t rec_entries(const FoldArgs &a, uint64_t key, uint32_t lvox, int x0, int y0,
+                                                int z0, uint32_t (&lv)[4])
+{
+    if ((key >> KEY_PHASE_SHIFT) != 0)
+    {
+        lv[0] = lvox;
+        lv[1] = lv[2] = lv[3] = 0u;
+        return 1u;
+    }
+    const int ax2 = (int)((key >> KEY_AX_SHIFT) & 3ull);
+    const int ax0 = axis_next(ax2);
+    const int ax1 = axis_prev(ax2);
+    const int y_idx = (int)((key >> KEY_Y_SHIFT) & KEY_IDX_MASK) + sel3i(ax1, a.g.gmin[0], a.g.gmin[1], a.g.gmin[2]);
+    const int x_idx = (int)((key >> KEY_X_SHIFT) & KEY_IDX_MASK) + sel3i(ax0, a.g.gmin[0], a.g.gmin[1], a.g.gmin[2]);
+    int c[4][3];
+    event_voxels(ax0, ax1, x_idx, y_idx, (int)lvox, c);
+    const uint32_t e = 1u << a.bs;
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+        const uint32_t lx = (uint32_t)(c[k][0] - x0), ly = (uint32_t)(c[k][1] - y0), lz = (uint32_t)(c[k][2] - z0);
+        lv[k] = lx | (ly << 3) | (lz << 6);
+        m |= (lx < e && ly < e && lz < e) ? (1u << k) : 0u;
+    }
+    return m;
+}
+
 // Calls f(rec) for every record of `brick`, all threads cooperating.
 // Contains barriers: every thread of the block must call it.
 template <class Fn>
@@ -2737,6 +2801,22 @@ __device__ __forceinline__ void for_each_rec(const FoldArgs &a, uint32_t brick, 
         }
         __syncthreads();
     }
+}
+
+// Calls f(key, z, lvox) for every voxel entry of the records of `brick`
+// (min corner x0, y0, z0), all threads cooperating.  Contains barriers.
+template <class Fn>
+__device__ __forceinline__ void for_each_entry(const FoldArgs &a, uint32_t brick, ChunkBatch &cb, int x0, int y0,
+                                               int z0, Fn &&f)
+{
+    for_each_rec(a, brick, cb, [&](const Rec &r) {
+        uint32_t lv[4];
+        const uint32_t m = rec_entries(a, r.key, r.lvox, x0, y0, z0, lv);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (m & (1u << k))
+                f(r.key | (uint64_t)k, r.z, lv[k]);
+    });
 }
 
 // key, crossing coordinate and face plane of the grouped records i .. i+FETCH-1 below s1
@@ -2922,13 +3002,15 @@ struct FoldLdsG
     ChunkBatchT<NT> cb;
 };
 
-// The brick's n records are in registers (record g = tid + k*BT in slot k)
-// and L.cur holds their per-voxel histogram: each record goes to its voxel's
-// segment, is ranked there among the voxel's keys (keys are unique) and
-// moved to its rank; then one thread per voxel with records replays them in
-// key order and solves.  Contains barriers.
+// The brick's n records are in registers (record g = tid + k*NT in slot k)
+// and L.cur holds the per-voxel histogram of their voxel entries
+// (rec_entries): each entry goes to its voxel's segment, is ranked there
+// among the voxel's keys (keys are unique) and moved to its rank; then one
+// thread per voxel with records replays them in key order and solves.
+// Returns false, before anything is folded, when the entries exceed CAP.
+// Contains barriers.
 template <int CAP, int P, int NT = BT>
-__device__ __forceinline__ void fold_grouped(const FoldArgs &a, FoldLdsG<CAP, NT> &L, uint32_t n,
+__device__ __forceinline__ bool fold_grouped(const FoldArgs &a, FoldLdsG<CAP, NT> &L, uint32_t n,
                                              const uint64_t (&rk)[CAP / NT], const float (&rz)[CAP / NT],
                                              const uint32_t (&rl)[CAP / NT], int x0, int y0, int z0)
 {
@@ -2936,7 +3018,7 @@ __device__ __forceinline__ void fold_grouped(const FoldArgs &a, FoldLdsG<CAP, NT
     constexpr int NW = NT / 64;
     constexpr int VPT = 512 / NT; // voxel slots of the brick per thread
     const int tid = threadIdx.x;
-    uint32_t nv;
+    uint32_t nv, tot;
     {
         uint32_t c[VPT], sum = 0, nz = 0;
 #pragma unroll
@@ -2946,8 +3028,9 @@ __device__ __forceinline__ void fold_grouped(const FoldArgs &a, FoldLdsG<CAP, NT
             sum += c[j];
             nz += c[j] ? 1u : 0u;
         }
-        uint32_t tot;
         uint32_t ex = block_excl_scan_w<NW>(sum, L.sh, tot);
+        if (tot > (uint32_t)CAP)
+            return false; // block-uniform
 #pragma unroll
         for (int j = 0; j < VPT; ++j)
         {
@@ -2964,7 +3047,7 @@ __device__ __forceinline__ void fold_grouped(const FoldArgs &a, FoldLdsG<CAP, NT
                 L.vlist[exv++] = (uint16_t)(VPT * tid + j);
     }
     __syncthreads();
-    // keys into their voxels' segments, in any order (the slot loops stop
+    // entries into their voxels' segments, in any order (the slot loops stop
     // at the block's last slot in use)
 #pragma unroll
     for (int k = 0; k < PER; ++k)
@@ -2972,35 +3055,56 @@ __device__ __forceinline__ void fold_grouped(const FoldArgs &a, FoldLdsG<CAP, NT
         if ((uint32_t)(k * NT) >= n)
             break;
         if ((uint32_t)(tid + k * NT) < n)
-            L.key[atomicAdd(&L.cur[rl[k]], 1u)] = rk[k];
+        {
+            uint32_t lv[4];
+            const uint32_t em = rec_entries(a, rk[k], rl[k], x0, y0, z0, lv);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (em & (1u << q))
+                {
+                    const uint32_t u = atomicAdd(&L.cur[lv[q]], 1u);
+                    L.key[u] = rk[k] | (uint64_t)q;
+                    L.z[u] = rz[k];
+                }
+        }
     }
     __syncthreads();
+    // rank of the entry at each position among its voxel's keys
+    uint64_t ek[PER];
+    float ez[PER];
     uint32_t pos[PER];
 #pragma unroll
-    for (int k = 0; k < PER; ++k)
+    for (int i = 0; i < PER; ++i)
     {
-        pos[k] = 0;
-        if ((uint32_t)(k * NT) >= n)
+        pos[i] = 0;
+        ek[i] = 0;
+        ez[i] = 0.0f;
+        if ((uint32_t)(i * NT) >= tot)
             break;
-        if ((uint32_t)(tid + k * NT) < n)
+        const uint32_t u = (uint32_t)(tid + i * NT);
+        if (u < tot)
         {
-            const uint32_t s0 = L.off[rl[k]], s1 = L.off[rl[k] + 1];
+            const uint64_t key = L.key[u];
+            ek[i] = key;
+            ez[i] = L.z[u];
+            const int v = upper_find(L.off + 1, 512, (uint16_t)u); // the voxel whose segment holds u
+            const uint32_t s0 = L.off[v], s1 = L.off[v + 1];
             uint32_t rank = 0;
             for (uint32_t t = s0; t < s1; ++t)
-                rank += (L.key[t] < rk[k]) ? 1u : 0u;
-            pos[k] = s0 + rank;
+                rank += (L.key[t] < key) ? 1u : 0u;
+            pos[i] = s0 + rank;
         }
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < PER; ++k)
+    for (int i = 0; i < PER; ++i)
     {
-        if ((uint32_t)(k * NT) >= n)
+        if ((uint32_t)(i * NT) >= tot)
             break;
-        if ((uint32_t)(tid + k * NT) < n)
+        if ((uint32_t)(tid + i * NT) < tot)
         {
-            L.key[pos[k]] = rk[k];
-            L.z[pos[k]] = rz[k];
+            L.key[pos[i]] = ek[i];
+            L.z[pos[i]] = ez[i];
         }
     }
     __syncthreads();
@@ -3043,6 +3147,7 @@ __device__ __forceinline__ void fold_grouped(const FoldArgs &a, FoldLdsG<CAP, NT
         st.finish_out(row_vid(a, L, x0, l), x0 + (l & 7), y0 + ((l >> 3) & 7), z0 + (l >> 6), a);
     }
     __syncthreads();
+    return true;
 }
 
 // One workgroup of NT threads per brick of `list`, for bricks of at most CAP
@@ -3121,7 +3226,12 @@ __device__ __forceinline__ void fold_bricks_body(const FoldArgs &ain, const uint
                     rk[k] = r.key;
                     rz[k] = r.z;
                     rl[k] = r.lvox;
-                    atomicAdd(&L.cur[r.lvox], 1u);
+                    uint32_t lv[4];
+                    const uint32_t em = rec_entries(a, r.key, r.lvox, x0, y0, z0, lv);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        if (em & (1u << q))
+                            atomicAdd(&L.cur[lv[q]], 1u);
                 }
             }
             __syncthreads();
@@ -3134,9 +3244,17 @@ __device__ __forceinline__ void fold_bricks_body(const FoldArgs &ain, const uint
             __syncthreads();
             continue;
         }
+        if (!fold_grouped<CAP, P, NT>(a, L, n, rk, rz, rl, x0, y0, z0))
+        {
+            // more voxel entries than the tile holds (not expected: bricks are
+            // listed by their exact entry counts): the next tier folds it
+            if (tid == 0)
+                next[atomicAdd(n_next, 1ull)] = brick;
+            __syncthreads();
+            continue;
+        }
         if (count_recs && tid == 0)
             atomicAdd(a.ctr + CTR_LARGE_RECS, (unsigned long long)n);
-        fold_grouped<CAP, P, NT>(a, L, n, rk, rz, rl, x0, y0, z0);
     }
 }
 
@@ -3208,14 +3326,14 @@ __global__ __launch_bounds__(BT) void k_fold_large(FoldArgs ain)
         if (tid == 0)
             L.n = 0;
         __syncthreads();
-        for_each_rec(a, brick, L.cb, [&](const Rec &r) {
-            atomicAdd(&L.cur[r.lvox], 1u);
+        for_each_entry(a, brick, L.cb, x0, y0, z0, [&](uint64_t key, float z, uint32_t lvox) {
+            atomicAdd(&L.cur[lvox], 1u);
             const uint32_t i = atomicAdd(&L.n, 1u);
             if (i < (uint32_t)FOLD_CAP)
             {
-                L.key[i] = r.key;
-                L.z[i] = r.z;
-                L.lv[i] = (uint16_t)r.lvox;
+                L.key[i] = key;
+                L.z[i] = z;
+                L.lv[i] = (uint16_t)lvox;
             }
         });
         const uint32_t n_all = L.n;
@@ -3265,8 +3383,8 @@ __global__ __launch_bounds__(BT) void k_fold_large(FoldArgs ain)
                         const uint64_t d = thi - tlo;
                         const uint64_t mid = tlo + d / 2 + (d & 1ull);
                         uint32_t cnt = 0;
-                        for_each_rec(a, brick, L.cb, [&](const Rec &r) {
-                            cnt += (r.lvox == lv && (!have_last || r.key > last) && r.key <= mid) ? 1u : 0u;
+                        for_each_entry(a, brick, L.cb, x0, y0, z0, [&](uint64_t key, float, uint32_t lvox) {
+                            cnt += (lvox == lv && (!have_last || key > last) && key <= mid) ? 1u : 0u;
                         });
                         uint32_t tot;
                         block_excl_scan(cnt, L.sh, tot);
@@ -3279,12 +3397,12 @@ __global__ __launch_bounds__(BT) void k_fold_large(FoldArgs ain)
                     if (tid == 0)
                         L.m = 0;
                     __syncthreads();
-                    for_each_rec(a, brick, L.cb, [&](const Rec &r) {
-                        if (r.lvox == lv && (!have_last || r.key > last) && r.key <= T)
+                    for_each_entry(a, brick, L.cb, x0, y0, z0, [&](uint64_t key, float z, uint32_t lvox) {
+                        if (lvox == lv && (!have_last || key > last) && key <= T)
                         {
                             const uint32_t p = atomicAdd(&L.m, 1u);
-                            L.key[p] = r.key;
-                            L.z[p] = r.z;
+                            L.key[p] = key;
+                            L.z[p] = z;
                         }
                     });
                     const uint32_t m = L.m;
@@ -3338,12 +3456,12 @@ __global__ __launch_bounds__(BT) void k_fold_large(FoldArgs ain)
                 L.cur[i] = 0;
             __syncthreads();
             // counting sort of the range's records by voxel
-            for_each_rec(a, brick, L.cb, [&](const Rec &r) {
-                if (r.lvox >= (uint32_t)lo && r.lvox < (uint32_t)hi)
+            for_each_entry(a, brick, L.cb, x0, y0, z0, [&](uint64_t key, float z, uint32_t lvox) {
+                if (lvox >= (uint32_t)lo && lvox < (uint32_t)hi)
                 {
-                    const uint32_t p = L.off[r.lvox] - base + atomicAdd(&L.cur[r.lvox], 1u);
-                    L.key[p] = r.key;
-                    L.z[p] = r.z;
+                    const uint32_t p = L.off[lvox] - base + atomicAdd(&L.cur[lvox], 1u);
+                    L.key[p] = key;
+                    L.z[p] = z;
                 }
             });
             // rank of each record among its voxel's keys (keys are unique)
