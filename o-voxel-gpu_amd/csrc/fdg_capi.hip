@@ -1046,13 +1046,11 @@ ovx_status enqueue_finish(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t 
     toc(c, KS_GEN_EVENTS, es);
     if (c->E > 0)
     {
+        // the boundary DDA makes its records directly; its hits are only counted
         const ShardBuf<BHit> hb = shard_view<BHit>(c, c->hitqef, SB_BHITS, c->off_bhits);
         tic(c, KS_BDRY, es);
-        HIP_TRY(launch_bdry(c->E, (const BEdge *)c->bedges.p, (const int2 *)c->bedge_v.p, c->bdev, hb, ctr, es));
+        HIP_TRY(launch_bdry(c->E, (const BEdge *)c->bedges.p, (const int2 *)c->bedge_v.p, c->bdev, ba, hb, ctr, es));
         toc(c, KS_BDRY, es);
-        tic(c, KS_GEN_BHITS, es);
-        HIP_TRY(launch_gen_bhits(hb, ba, es));
-        toc(c, KS_GEN_BHITS, es);
     }
     HIP_TRY(hipEventRecord(c->ev_join, es));
     if (c->fw > 0.0f)
@@ -1063,7 +1061,7 @@ ovx_status enqueue_finish(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t 
     }
     HIP_TRY(hipStreamWaitEvent(s, c->ev_join, 0));
     HIP_TRY(launch_shard_reduce((unsigned long long *)c->shard_ctr.p, ctr, s));
-    const OverCaps caps{c->per[SB_FHITS], c->per[SB_BHITS], c->per[SB_RECS], c->per[SB_CHUNKS], grouped};
+    const OverCaps caps{c->per[SB_FHITS], ~0ull, c->per[SB_RECS], c->per[SB_CHUNKS], grouped}; // boundary hits: counts only
     HIP_TRY(launch_over_check(ctr, caps, s));
 
     // ---- chunks grouped by brick, bricks with records, then the per-brick ordered fold + solve
@@ -1259,7 +1257,6 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
     const uint64_t rec_est = ev_est + face_est + bhit_est;
     uint64_t want_rec = per_for(rec_est, 8);
     uint64_t want_fhit = per_for(face_est + face_est * 15 / 100);
-    uint64_t want_bhit = per_for(bhit_est + bhit_est * 15 / 100);
     // chunks: the last conversion's chunks per record with a margin (they vary with the order of the claims)
     const double cpr = std::min(1.0, std::max(0.02, c->chunk_per_rec) * 1.5);
     uint64_t want_chunk = per_for((uint64_t)(cpr * (double)rec_est));
@@ -1295,7 +1292,7 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
             // the voxels typically (whatever does not fit is solved in place), after the QEFs of the precision
             // variant (the reference path solves each voxel where it is folded and stores no QEF)
             const size_t fh = align_up(size_shard(c, SB_FHITS, c->fw > 0.0f ? want_fhit : 1, sizeof(FHit)), 256);
-            const size_t bh = size_shard(c, SB_BHITS, c->E > 0 ? want_bhit : 1, sizeof(BHit));
+            const size_t bh = size_shard(c, SB_BHITS, 1, sizeof(BHit)); // counters only
             const size_t q = c->prec ? align_up((size_t)c->N * 80, 256) : 0;
             const size_t bx = size_shard(c, SB_BOX, per_for((uint64_t)c->N / 8 + 4096), sizeof(BoxItem));
             if ((st = ensure(c->hitqef, std::max(fh + bh, q + bx), false)))
@@ -1336,7 +1333,7 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
                                    0.9 * c->rec_per_event}));
             return OVX_OK;
         }
-        const bool hits_over = mx[SB_FHITS] > c->per[SB_FHITS] || mx[SB_BHITS] > c->per[SB_BHITS];
+        const bool hits_over = mx[SB_FHITS] > c->per[SB_FHITS];
         if (sum[SB_RECS] >= 0xffffffffull || sum[SB_CHUNKS] >= 0xffffffffull)
             return fail(OVX_ERR_CAPACITY, "%llu QEF records / %llu chunks exceed the 32-bit record index",
                         (unsigned long long)sum[SB_RECS], (unsigned long long)sum[SB_CHUNKS]);
@@ -1344,7 +1341,6 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
             return fail(OVX_ERR_STATE, "record buffers overflowed three times (internal error)");
         // sizes are known now: grow and generate again
         want_fhit = mx[SB_FHITS] + mx[SB_FHITS] / 8 + 256;
-        want_bhit = mx[SB_BHITS] + mx[SB_BHITS] / 8 + 256;
         want_rec = mx[SB_RECS] + mx[SB_RECS] / 8 + 256;
         if (hits_over)
         {

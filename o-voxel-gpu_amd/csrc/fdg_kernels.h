@@ -321,7 +321,6 @@ hipError_t launch_face(const MeshDev &m, const BatchDev &b, int tile, uint64_t n
                        unsigned long long *ctr, const ShardBuf<FaceChunk> &fq, hipStream_t s);
 hipError_t launch_gen_events(const ShardBuf<Event> &ev, const BucketArgs &ba, hipStream_t s);
 hipError_t launch_gen_fhits(const ShardBuf<FHit> &h, const BucketArgs &ba, hipStream_t s);
-hipError_t launch_gen_bhits(const ShardBuf<BHit> &h, const BucketArgs &ba, hipStream_t s);
 hipError_t launch_shard_reduce(unsigned long long *shard_ctr, unsigned long long *ctr, hipStream_t s);
 // fills n 64-bit / 32-bit words with v (a kernel: capturable in a graph)
 hipError_t launch_fill_u64(void *p, uint64_t n, uint64_t v, hipStream_t s);
@@ -371,8 +370,9 @@ hipError_t launch_edge_singletons_slots(int64_t V, const uint32_t *vcnt, const u
 hipError_t launch_edge_singletons(int64_t V, const uint32_t *voff, uint32_t *nbr, uint32_t *bcnt, const uint32_t *boff,
                                   int2 *bedge_v, int pass, hipStream_t s);
 hipError_t launch_bedge_prep(const float *verts, int64_t E, const int2 *bedge_v, BEdge *be, hipStream_t s);
-hipError_t launch_bdry(int64_t E, const BEdge *be, const int2 *bedge_v, const BatchDev &b, const ShardBuf<BHit> &hb,
-                       unsigned long long *ctr, hipStream_t s);
+// boundary DDA and its phase-2 records (hits counted on hcount's counters only)
+hipError_t launch_bdry(int64_t E, const BEdge *be, const int2 *bedge_v, const BatchDev &b, const BucketArgs &ba,
+                       const ShardBuf<BHit> &hcount, unsigned long long *ctr, hipStream_t s);
 uint64_t scan_tile_count(uint64_t n);
 hipError_t launch_scan_popc(const uint64_t *bits, uint64_t n, uint32_t *out, uint64_t *tile_sums,
                             unsigned long long *total, hipStream_t s);

@@ -1844,89 +1844,131 @@ __global__ __launch_bounds__(BT) void k_bedge_prep(const float *verts, int64_t E
     be[i] = out;
 }
 
-// DDA over a boundary segment (FDG.cpp:455-522); one lane per segment.  Every
-// active voxel it passes is appended to `hb` as (edge rank, voxel).
+// Boundary QEF terms (FDG.cpp:455-522): one thread per boundary segment walks
+// the voxels the segment passes with the reference's DDA (FP64 steps), and
+// every occupied voxel on the way gets a phase-2 record (key = the edge's
+// rank).  The block emits its threads' records in rounds of up to 4 each
+// (bucketed by brick, like the other record generators) until every segment
+// of the block is walked; the hits are only counted (SB_BHITS counters:
+// statistics and record-buffer estimates).
 __global__ __launch_bounds__(BT) void k_bdry(int64_t E, const BEdge *be, const int2 *bedge_v, BatchDev bt,
-                                             ShardBuf<BHit> hb, unsigned long long *counters)
+                                             BucketArgs ba, ShardBuf<BHit> hcount, unsigned long long *counters)
 {
-    const int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
-    uint32_t n_hit = 0;
+    __shared__ BucketLds<11> L;
+    const int tid = threadIdx.x;
     unsigned err = 0;
-    if (i < E && be[i].valid)
+    uint32_t n_hit = 0;
+    for (int64_t base = (int64_t)blockIdx.x * BT; base < E; base += (int64_t)gridDim.x * BT)
     {
-        const BEdge &e = be[i];
-        const GridDev g = bt.seg[mesh_of_vert(bt, (int64_t)bedge_v[i].x)].g; // the edge's mesh
-        int cur[3], step[3];
-        double tMax[3], tDelta[3];
-        double cap = 16.0;
+        const int64_t i = base + tid;
+        bool alive = false;
+        int ms = 0;
+        int cur[3] = {0, 0, 0}, step[3] = {0, 0, 0};
+        double tMax[3] = {0.0, 0.0, 0.0}, tDelta[3] = {0.0, 0.0, 0.0};
+        double len = 0.0;
+        int64_t steps = 0, max_steps = 0;
+        if (i < E && be[i].valid)
+        {
+            const BEdge &e = be[i];
+            ms = mesh_of_vert(bt, (int64_t)bedge_v[i].x); // the edge's mesh
+            const GridDev &g = bt.seg[ms].g;
+            double cap = 16.0;
 #pragma unroll
-        for (int a = 0; a < 3; ++a)
-        {
-            cur[a] = (int)floorf(div_vs(e.v0[a], g, a));
-            step[a] = e.dir[a] > 0.0 ? 1 : -1;
-            if (e.dir[a] == 0.0)
+            for (int a = 0; a < 3; ++a)
             {
-                tMax[a] = __builtin_huge_val();
-                tDelta[a] = __builtin_huge_val();
-            }
-            else
-            {
-                const float border = g.vs[a] * (float)(cur[a] + (step[a] > 0 ? 1 : 0));
-                tMax[a] = (double)(border - e.v0[a]) / e.dir[a];
-                tDelta[a] = (double)g.vs[a] / fabs(e.dir[a]);
-                cap += ceil(e.len * fabs(e.dir[a]) / (double)g.vs[a]) + 2.0;
-            }
-        }
-        int64_t steps = 0;
-        const int64_t max_steps = (int64_t)(cap < 4.0e9 ? cap : 4.0e9);
-        while (true)
-        {
-            if (cur[0] >= g.gmin[0] && cur[0] < g.gmax[0] && cur[1] >= g.gmin[1] && cur[1] < g.gmax[1] &&
-                cur[2] >= g.gmin[2] && cur[2] < g.gmax[2])
-            {
-                int b;
-                const uint64_t w = word_of(g, cur[0], cur[1], cur[2], b);
-                const uint64_t word = g.bits[w];
-                if ((word >> b) & 1ull)
+                cur[a] = (int)floorf(div_vs(e.v0[a], g, a));
+                step[a] = e.dir[a] > 0.0 ? 1 : -1;
+                if (e.dir[a] == 0.0)
                 {
-                    ++n_hit;
-                    bool okk;
-                    const uint64_t slot = shard_claim(hb, (uint32_t)(i % NSHARD), 1, okk);
-                    if (okk)
-                        hb.items[slot] = BHit{(uint32_t)i, cur[0], cur[1], cur[2]};
+                    tMax[a] = __builtin_huge_val();
+                    tDelta[a] = __builtin_huge_val();
+                }
+                else
+                {
+                    const float border = g.vs[a] * (float)(cur[a] + (step[a] > 0 ? 1 : 0));
+                    tMax[a] = (double)(border - e.v0[a]) / e.dir[a];
+                    tDelta[a] = (double)g.vs[a] / fabs(e.dir[a]);
+                    cap += ceil(e.len * fabs(e.dir[a]) / (double)g.vs[a]) + 2.0;
                 }
             }
-            int axis;
-            if (tMax[0] < tMax[1])
-                axis = (tMax[0] < tMax[2]) ? 0 : 2;
-            else
-                axis = (tMax[1] < tMax[2]) ? 1 : 2;
-            const double tm = axis == 0 ? tMax[0] : (axis == 1 ? tMax[1] : tMax[2]);
-            if (tm > e.len)
-                break;
-            if (++steps > max_steps)
+            len = e.len;
+            max_steps = (int64_t)(cap < 4.0e9 ? cap : 4.0e9);
+            alive = true;
+        }
+        while (__syncthreads_or(alive ? 1 : 0))
+        {
+            uint32_t bk[4] = {0, 0, 0, 0};
+            Rec rc[4];
+            bool ok[4] = {false, false, false, false};
+            int nh = 0;
+            if (alive)
             {
-                err |= ERR_DDA;
-                break;
+                const MeshSeg &sg = bt.seg[ms];
+                const GridDev &g = sg.g;
+                // one voxel per iteration: its record when it is occupied, then the step to the next
+                while (alive && nh < 4)
+                {
+                    if (cur[0] >= g.gmin[0] && cur[0] < g.gmax[0] && cur[1] >= g.gmin[1] && cur[1] < g.gmax[1] &&
+                        cur[2] >= g.gmin[2] && cur[2] < g.gmax[2])
+                    {
+                        int b;
+                        const uint64_t w = word_of(g, cur[0], cur[1], cur[2], b);
+                        if ((g.bits[w] >> b) & 1ull)
+                        {
+                            uint32_t lvox, brick;
+                            brick_of(sg, cur[0], cur[1], cur[2], brick, lvox);
+#pragma unroll
+                            for (int k = 0; k < 4; ++k)
+                                if (k == nh)
+                                {
+                                    bk[k] = brick + (uint32_t)sg.brick0;
+                                    rc[k] = Rec{(2ull << KEY_PHASE_SHIFT) | (uint64_t)i, 0.0f, lvox};
+                                    ok[k] = true;
+                                }
+                            ++nh;
+                        }
+                    }
+                    int axis;
+                    if (tMax[0] < tMax[1])
+                        axis = (tMax[0] < tMax[2]) ? 0 : 2;
+                    else
+                        axis = (tMax[1] < tMax[2]) ? 1 : 2;
+                    const double tm = axis == 0 ? tMax[0] : (axis == 1 ? tMax[1] : tMax[2]);
+                    if (tm > len)
+                        alive = false;
+                    else if (++steps > max_steps)
+                    {
+                        err |= ERR_DDA;
+                        alive = false;
+                    }
+                    else if (axis == 0)
+                    {
+                        cur[0] += step[0];
+                        tMax[0] += tDelta[0];
+                    }
+                    else if (axis == 1)
+                    {
+                        cur[1] += step[1];
+                        tMax[1] += tDelta[1];
+                    }
+                    else
+                    {
+                        cur[2] += step[2];
+                        tMax[2] += tDelta[2];
+                    }
+                }
             }
-            if (axis == 0)
-            {
-                cur[0] += step[0];
-                tMax[0] += tDelta[0];
-            }
-            else if (axis == 1)
-            {
-                cur[1] += step[1];
-                tMax[1] += tDelta[1];
-            }
-            else
-            {
-                cur[2] += step[2];
-                tMax[2] += tDelta[2];
-            }
+            n_hit += (uint32_t)nh;
+            const uint32_t ent[4] = {1u, 1u, 1u, 1u};
+            bucket_emit<4, 11>(bk, rc, ok, ent, ba, L);
         }
     }
-    (void)n_hit;
+    // the block's hits onto its shard's counter (no items are stored)
+    uint32_t tot;
+    __shared__ uint32_t sh[NWAVE];
+    block_excl_scan(n_hit, sh, tot);
+    if (tid == 0 && tot)
+        atomicAdd(hcount.cnt + (uint64_t)(blockIdx.x % NSHARD) * SHARD_STRIDE, (unsigned long long)tot);
     if (counters && err)
         atomicOr(counters + CTR_ERRORS, (unsigned long long)err);
 }
@@ -2015,38 +2057,6 @@ __global__ __launch_bounds__(BT) void k_gen_fhits(ShardBuf<FHit> h, BucketArgs b
                 brick_of(sg, g.gmin[0] + (int)x.lx, g.gmin[1] + (int)x.ly, g.gmin[2] + (int)x.lz, bk[k], lvox);
                 bk[k] += (uint32_t)sg.brick0;
                 rc[k] = Rec{(1ull << KEY_PHASE_SHIFT) | (uint64_t)x.face, 0.0f, lvox};
-                ok[k] = true;
-            }
-        }
-        const uint32_t ent[4] = {1u, 1u, 1u, 1u};
-        bucket_emit<4, 11>(bk, rc, ok, ent, ba, L);
-    }
-}
-
-// phase-2 records from the boundary DDA hits, key = boundary-edge rank
-// (4 consecutive hits per thread: mostly one edge's, in one or two bricks)
-__global__ __launch_bounds__(BT) void k_gen_bhits(ShardBuf<BHit> h, BucketArgs ba)
-{
-    __shared__ BucketLds<11> L;
-    __shared__ QueueMap qm;
-    const uint64_t n = queue_prefix(h, qm.pre, qm.sh);
-    for (uint64_t base = (uint64_t)blockIdx.x * (4 * BT); base < n; base += (uint64_t)gridDim.x * (4 * BT))
-    {
-        uint32_t bk[4] = {0, 0, 0, 0};
-        Rec rc[4];
-        bool ok[4] = {false, false, false, false};
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-        {
-            const uint64_t i = base + 4ull * threadIdx.x + (uint64_t)k;
-            if (i < n)
-            {
-                const BHit x = h.items[queue_slot(h, qm.pre, i)];
-                const MeshSeg &sg = ba.b.seg[mesh_of_vert(ba.b, (int64_t)ba.bedge_v[x.edge].x)];
-                uint32_t lvox;
-                brick_of(sg, x.x, x.y, x.z, bk[k], lvox);
-                bk[k] += (uint32_t)sg.brick0;
-                rc[k] = Rec{(2ull << KEY_PHASE_SHIFT) | (uint64_t)x.edge, 0.0f, lvox};
                 ok[k] = true;
             }
         }
@@ -3736,12 +3746,6 @@ hipError_t launch_gen_events(const ShardBuf<Event> &ev, const BucketArgs &ba, hi
     return hipGetLastError();
 }
 
-hipError_t launch_gen_bhits(const ShardBuf<BHit> &h, const BucketArgs &ba, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_gen_bhits, dim3(1024), dim3(BT), 0, s, h, ba); // loops over the hits
-    return hipGetLastError();
-}
-
 __global__ __launch_bounds__(BT) void k_fill64(uint64_t *p, uint64_t n, uint64_t v)
 {
     const uint64_t stride = (uint64_t)gridDim.x * BT;
@@ -3901,12 +3905,12 @@ hipError_t launch_bedge_prep(const float *verts, int64_t E, const int2 *bedge_v,
     return hipGetLastError();
 }
 
-hipError_t launch_bdry(int64_t E, const BEdge *be, const int2 *bedge_v, const BatchDev &b, const ShardBuf<BHit> &hb,
-                       unsigned long long *ctr, hipStream_t s)
+hipError_t launch_bdry(int64_t E, const BEdge *be, const int2 *bedge_v, const BatchDev &b, const BucketArgs &ba,
+                       const ShardBuf<BHit> &hcount, unsigned long long *ctr, hipStream_t s)
 {
     if (E == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_bdry, dim3(grid_for(E)), dim3(BT), 0, s, E, be, bedge_v, b, hb, ctr);
+    hipLaunchKernelGGL(k_bdry, dim3(grid_for(E, 65536)), dim3(BT), 0, s, E, be, bedge_v, b, ba, hcount, ctr);
     return hipGetLastError();
 }
 
