@@ -138,8 +138,10 @@ struct Chunk
 struct FHit
 {
     uint32_t face;
-    uint16_t lx, ly, lz, pad;
+    uint16_t lx, ly, lz;
+    uint16_t ms; // the face's mesh (FHIT_NO_MESH in batches of more meshes: looked up from the face)
 };
+constexpr uint16_t FHIT_NO_MESH = 0xffff;
 
 // boundary DDA hit: boundary-edge rank and voxel
 struct BHit

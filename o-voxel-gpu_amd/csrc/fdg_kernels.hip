@@ -1369,8 +1369,9 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(7))) void k_
                     if (found && face_pass(it, g.vs, x, row.y, row.z))
                     {
                         ++n_hit;
+                        const int hm = QUEUED ? (int)it.ms : ms;
                         hit = FHit{it.face, (uint16_t)(x - g.gmin[0]), (uint16_t)(row.y - g.gmin[1]),
-                                   (uint16_t)(row.z - g.gmin[2]), 0};
+                                   (uint16_t)(row.z - g.gmin[2]), bt.M <= 65535 ? (uint16_t)hm : FHIT_NO_MESH};
                         has = true;
                     }
                 }
@@ -1407,7 +1408,8 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(7))) void k_
 // (the active voxels of each row's slab range) and returns the number of
 // faces-test hits; with WRITE the hits are also stored from `out` on.
 template <bool WRITE>
-__device__ __forceinline__ uint32_t face_walk(const FaceItem &it, const GridDev &g, uint64_t n_rows, FHit *out)
+__device__ __forceinline__ uint32_t face_walk(const FaceItem &it, const GridDev &g, uint64_t n_rows, FHit *out,
+                                              uint16_t hm = FHIT_NO_MESH)
 {
     uint32_t n = 0;
     const RowNarrow nr = face_row_pre(it, g);
@@ -1435,7 +1437,7 @@ __device__ __forceinline__ uint32_t face_walk(const FaceItem &it, const GridDev 
                 {
                     if constexpr (WRITE)
                         out[n] = FHit{it.face, (uint16_t)(x - g.gmin[0]), (uint16_t)(y - g.gmin[1]),
-                                      (uint16_t)(z - g.gmin[2]), 0};
+                                      (uint16_t)(z - g.gmin[2]), hm};
                     ++n;
                 }
             }
@@ -1591,6 +1593,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         if (!cnt || base == ~0ull)
             continue;
         FHit *out = hb.items + base + (incl - cnt);
+        const uint16_t hm = bt.M <= 65535 ? (uint16_t)ms : FHIT_NO_MESH;
         if (masked)
         {
             uint32_t k = 0;
@@ -1601,11 +1604,11 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 const uint32_t yz = (uint32_t)(ryz >> (b & ~7)) & 0xffu;
                 const int x = it.bx0 + (b & 7), y = it.by0 + (int)(yz & 15u), z = it.bz0 + (int)(yz >> 4);
                 out[k++] = FHit{it.face, (uint16_t)(x - g.gmin[0]), (uint16_t)(y - g.gmin[1]), (uint16_t)(z - g.gmin[2]),
-                                0};
+                                hm};
             }
         }
         else
-            face_walk<true>(it, g, R, out);
+            face_walk<true>(it, g, R, out, hm);
     }
 }
 
@@ -2051,7 +2054,7 @@ __global__ __launch_bounds__(BT) void k_gen_fhits(ShardBuf<FHit> h, BucketArgs b
             if (i < n)
             {
                 const FHit x = h.items[queue_slot(h, qm.pre, i)];
-                const MeshSeg &sg = ba.b.seg[mesh_of_face(ba.b, (int64_t)x.face)];
+                const MeshSeg &sg = ba.b.seg[x.ms != FHIT_NO_MESH ? (int)x.ms : mesh_of_face(ba.b, (int64_t)x.face)];
                 const GridDev &g = sg.g;
                 uint32_t lvox;
                 brick_of(sg, g.gmin[0] + (int)x.lx, g.gmin[1] + (int)x.ly, g.gmin[2] + (int)x.lz, bk[k], lvox);
