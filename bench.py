@@ -74,22 +74,29 @@ def make_workload(name, grid):
     return v, f, g, seed
 
 
-def algorithmic_bytes(slot, st, F, V, n_words):
+def algorithmic_bytes(slot, st, F, V, n_words, precision="reference"):
     """Compulsory HBM bytes of one launch of a kernel slot: what the slot's
     kernels must read and write at least once, per the comments below (the
     roofline's `achieved` figure divides this by the slot's launch time)."""
     N, R, E, ev = st["voxels"], st["records"], st["boundary_edges"], st["events"]
     FR, BR = st["face_records"], st["boundary_records"]
-    # the fold reads every record once, each face plane and boundary quadric
+    R0 = max(0, R - FR - BR)  # crossing records: one per (crossing, brick of its voxels)
+    comp = precision == "compensated"
+    # the fold reads every record once, each face plane and boundary segment
     # once, the (first record, count) pair of each chunk and the occupancy word
     # + first voxel id of each brick row (64 rows of 12 B per brick), and writes
-    # the folded quadric (40 B) and flags (3 B) of every voxel; the two fold
-    # kernels split it in proportion to the records each folds
-    fold_all = (16 * R + 16 * F + 88 * E + 8 * st["chunks"] + 768 * st["active_bricks"] + 43 * N)
-    large_share = st["large_brick_records"] / R if R else 0.0
+    # the flags (3 B) and, on the reference path, the dual vertex (12 B; the
+    # voxel is solved where it is folded) and the queued box-search items (72 B),
+    # on the precision variant the compensated quadric (80 B); the fold tiers
+    # split it in proportion to the voxel entries each folds
+    fold_out = (80 * N + 3 * N) if comp else (15 * N + 72 * st["box_voxels"])
+    fold_all = (16 * R + 16 * F + 88 * E + 8 * st["chunks"] + 768 * st["active_bricks"] + fold_out)
+    # voxel entries: 4 per crossing, 1 per face or boundary record; the large
+    # tiers' count is in entries
+    large_share = min(1.0, st["large_brick_records"] / max(1, 4 * ev + FR + BR))
     table = {
-        # faces (12 B) + vertices (12 B) read once, 16 B plane per face written
-        "prep": 12 * F + 12 * V + 16 * F,
+        # faces (12 B) + vertices (12 B) read once, 16 B plane + 36 B gathered triangle per face written
+        "prep": 12 * F + 12 * V + 16 * F + 36 * F,
         # faces + vertices read, occupancy bitset written once, 16 B per crossing recorded
         "mark": 12 * F + 12 * V + 8 * n_words + 16 * ev,
         # bitset read, word offsets written
@@ -98,19 +105,18 @@ def algorithmic_bytes(slot, st, F, V, n_words):
         "edges": 12 * F + 24 * F + 16 * V,
         # bitset + offsets read, coords written
         "coords": 12 * n_words + 12 * N,
-        # crossings read, 4 records of 16 B written per crossing
-        "gen_events": 16 * ev + 64 * ev,
+        # crossings read (16 B), one 16 B record per (crossing, brick) written
+        "gen_events": 16 * ev + 16 * R0,
         # faces + vertices + bitset read, one 12 B hit per face / voxel overlap written
         "face": 12 * F + 12 * V + 8 * n_words + 12 * FR,
         # hits read, one 16 B record per hit written
         "gen_fhits": 12 * FR + 16 * FR,
-        # boundary segments (88 B) read, one 16 B hit per voxel on the segment written
+        # boundary segments (88 B) read, one 16 B record per voxel on the segment written
         "bdry": 88 * E + 16 * BR,
-        "gen_bhits": 16 * BR + 16 * BR,
         "fold": (1.0 - large_share) * fold_all,
         "fold_large": large_share * fold_all,
-        # quadric (40 B) + coords (12 B) read, dual (12 B) written, 72 B per queued voxel
-        "solve": 64 * N + 72 * st["box_voxels"],
+        # precision variant: quadric (80 B) + coords (12 B) read, dual (12 B) written, 72 B per queued voxel
+        "solve": 104 * N + 72 * st["box_voxels"],
         # queued voxels (72 B) read, dual (12 B) written
         "solve_box": 84 * st["box_voxels"],
     }
@@ -325,7 +331,7 @@ def run_conversion(args, torch, o_voxel, _C, dev):
     # canonical algorithmic bytes of one conversion: 12 B per vertex and per
     # face read, 27 B per voxel written (coords 12, dual 12, flags 3)
     b_alg = 12 * V + 12 * F + 27 * st["voxels"]
-    model = algorithmic_bytes(dom, st, F, V, n_words)
+    model = algorithmic_bytes(dom, st, F, V, n_words, args.precision)
     achieved = b_alg / (kern_avg[dom] * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"{dom}_{args.workload}_traffic.json")
@@ -554,7 +560,7 @@ def run_batch(args, torch, o_voxel, _C, dev):
 
 
 def run_to_mesh(args, torch, o_voxel, _C, dev):
-    """flexible_dual_grid_to_mesh on the flexible dual grid of the 268k-face
+    """flexible_dual_grid_to_mesh on the flexible dual grid of the ~280k-face
     scene (grid 512 unless --grid), dual vertices in voxel units as the
     reference's example stores them.  Steps time the public API (fused HIP
     path); beside it, the reference's own recipe for the same function (its
@@ -697,7 +703,7 @@ def run_to_mesh(args, torch, o_voxel, _C, dev):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"to_mesh-scene268k@{grid}", "voxels": N, "intersected_flags": flags, "quads": L,
+        "config": {"workload": f"to_mesh-scene280k@{grid}", "voxels": N, "intersected_flags": flags, "quads": L,
                    "grid": grid, "seed": 2},
         "reference_recipe_ms": round(recipe_ms, 4),
         "speedup_vs_reference_recipe": round(recipe_ms / ms, 3),
@@ -712,7 +718,7 @@ def run_to_mesh(args, torch, o_voxel, _C, dev):
 
 
 def _scene_voxels(torch, o_voxel, dev, grid):
-    """The conversion of the 268k-face scene: coords, duals, flags."""
+    """The conversion of the ~280k-face scene: coords, duals, flags."""
     import meshgen
 
     v, f = meshgen.scene(seed=2)
@@ -767,7 +773,7 @@ def run_voxel_io(args, torch, o_voxel, _C, dev):
         line.update({
             "metric": "voxel codes M voxels/s (Hilbert encode)", "value": hb["encode_M_voxels_per_s"],
             "unit": "M voxels/s", "ms_per_step": hb["encode_ms"], "dtype": "int32",
-            "config": {"workload": f"serialize-scene268k@{grid}", "voxels": N, "grid": grid, "seed": 2},
+            "config": {"workload": f"serialize-scene280k@{grid}", "voxels": N, "grid": grid, "seed": 2},
             "modes": res,
             "roofline": {"kernel": "k_seq_encode<hilbert>", "bound": "hbm", "achieved": round(achieved, 1),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -814,7 +820,7 @@ def run_voxel_io(args, torch, o_voxel, _C, dev):
     line.update({
         "metric": "O-voxel .vxz write M voxels/s (parent filter, uncompressed)", "value": un["write_M_voxels_per_s"],
         "unit": "M voxels/s", "ms_per_step": un["write_ms"], "dtype": "u8",
-        "config": {"workload": f"vxz-scene268k@{grid}", "voxels": N, "grid": grid, "chunk_size": 256, "seed": 2},
+        "config": {"workload": f"vxz-scene280k@{grid}", "voxels": N, "grid": grid, "chunk_size": 256, "seed": 2},
         "compression": res,
         "sample_bytes_identical_to_oracle": b.getvalue() == ref,
         "roofline": None,
