@@ -123,7 +123,8 @@ ovx_status ovx_fdg_finish(ovx_fdg_ctx *ctx, int32_t *d_coords, float *d_dual, ui
  * of their own, tight serving loops, harnesses that forbid hipMalloc).
  * Every workspace buffer is carved from a static per-device scratch of
  * OVX_FDG_NOALLOC_SCRATCH_BYTES (held by each device the library runs on
- * until the process ends; its address is looked up by ovx_fdg_create);
+ * until the process ends; its address is looked up by the context's first
+ * allocation-free call, which also pays for placing it on the device);
  * the context's own workspace is left as it was.  The caller passes the
  * outputs with room for max_voxels rows (layout as ovx_fdg_finish) and gets
  * the voxel count in *n_voxels.
@@ -235,9 +236,11 @@ ovx_status ovx_fdg_graph_stats(ovx_fdg_ctx *ctx, int64_t *replays, int64_t *capt
  *   out[12] voxels queued for the box-constrained search (the rest of the voxels
  *           whose minimiser left the box are searched in place)
  *   out[13] voxels solved on the FP64 path (precision variant; 0 otherwise)
+ *   out[14] times finish generated the records (1, or more when a record
+ *           buffer overflowed and was sized again)
  * Reading them synchronises the context's last stream.
  */
-#define OVX_FDG_N_STATS 14
+#define OVX_FDG_N_STATS 15
 ovx_status ovx_fdg_stats(ovx_fdg_ctx *ctx, int64_t *out, int n);
 
 /*

@@ -208,7 +208,8 @@ struct ovx_fdg_ctx
 // held until the process ends.  In host code the name stands for no device
 // address: the address of the current device's copy comes from
 // hipGetSymbolAddress (scratch_for_device), and only that is handed to
-// kernels.
+// kernels.  This file holds no kernel, so its code object (with the array)
+// is loaded only by that lookup.
 static __device__ __align__(256) unsigned char scratch[OVX_FDG_NOALLOC_SCRATCH_BYTES];
 
 namespace
@@ -1310,6 +1311,7 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
         // the conversion's one read-back of finish: errors, overflow, statistics
         if ((st = read_counters(c, s)))
             return st;
+        c->stats[14] = attempt + 1;
         if (c->h_ctr[CTR_ERRORS])
             return input_error(c->h_ctr[CTR_ERRORS]);
         const unsigned long long *mx = c->h_ctr + CTR_MAX, *sum = c->h_ctr + CTR_SUM;
@@ -1465,8 +1467,6 @@ ovx_status ovx_fdg_create(int device, ovx_fdg_ctx **out)
         return fail(OVX_ERR_HIP, "context set-up failed: %s", hipGetErrorString(e));
     }
     ovx_status st = ensure_side(c);
-    if (!st)
-        st = scratch_for_device(device, &c->scratch);
     if (st)
     {
         const std::string msg = g_last_error;
@@ -1603,8 +1603,16 @@ ovx_status ovx_fdg_convert_noalloc(ovx_fdg_ctx *c, const float *d_vertices, int6
         return fail(OVX_ERR_INVALID_ARGUMENT, "output pointers / max_voxels invalid");
     DeviceGuard dg(c->device);
     std::lock_guard<std::mutex> lk(c->mu);
+    // the device's scratch, looked up by the first allocation-free call on
+    // the context (its device current): the code object holding the array
+    // is loaded, and the array placed, then, so that a process that makes no
+    // such call does not pay for it (about 25 ms for 64 MiB)
     if (!c->scratch)
-        return fail(OVX_ERR_STATE, "context has no device scratch");
+    {
+        const ovx_status sst = scratch_for_device(c->device, &c->scratch);
+        if (sst)
+            return sst;
+    }
     *n_voxels = 0;
     MeshDev m{d_vertices, d_faces, n_faces, n_vertices};
     c->fw = face_weight;

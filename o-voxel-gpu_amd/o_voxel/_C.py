@@ -930,7 +930,7 @@ def last_stats(device=None) -> dict:
     ctx = _ctx(dev)
     keys = ("voxels", "events", "records", "face_records", "boundary_edges", "boundary_records",
             "workspace_bytes", "giant_voxels", "active_bricks", "chunks", "large_bricks", "large_brick_records",
-            "box_voxels", "fp64_voxels")
+            "box_voxels", "fp64_voxels", "record_attempts")
     out = (ctypes.c_int64 * len(keys))()
     with torch.cuda.device(dev), ctx.lock:
         _check(_load().ovx_fdg_stats(ctx.h, out, len(keys)), "ovx_fdg_stats")
