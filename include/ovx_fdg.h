@@ -72,6 +72,22 @@ int64_t ovx_fdg_workspace_bytes(const ovx_fdg_ctx *ctx);
 ovx_status ovx_fdg_trim(ovx_fdg_ctx *ctx);
 
 /*
+ * A conversion's inputs from the caller's arrays, as the public wrapper
+ * prepares them before the extension call (o_voxel/convert/flexible_dual_grid.py:89-90
+ * faces cast to int32, :181 vertices - aabb[0]), in one kernel on `stream`
+ * (the current device's), so that a conversion queues no framework kernel:
+ *   d_out_vertices[i][k] = d_vertices[i][k] - d_origin[k]   (FP32, n_vertices rows)
+ *   d_out_faces[j][k]    = d_faces[j][k] as int32           (n_faces rows; an index
+ *                          outside [0, 2^31) becomes -1, which ovx_fdg_begin
+ *                          rejects as out of range)
+ * d_faces may be NULL (faces already int32: only the vertices are written).
+ * Needs no context; returns without waiting.
+ */
+ovx_status ovx_fdg_prepare_mesh(const float *d_vertices, const float *d_origin, int64_t n_vertices,
+                                const int64_t *d_faces, int64_t n_faces, float *d_out_vertices, int32_t *d_out_faces,
+                                void *stream);
+
+/*
  * mesh_to_flexible_dual_grid, device path.
  * Replaces o_voxel::fdg::mesh_to_flexible_dual_grid_cuda
  * (src/convert/mesh_to_flexible_dual_grid_gpu/mesh_to_flexible_dual_grid.cu:797-895;

@@ -1512,6 +1512,26 @@ ovx_status ovx_fdg_destroy(ovx_fdg_ctx *c)
     return OVX_OK;
 }
 
+ovx_status ovx_fdg_prepare_mesh(const float *d_vertices, const float *d_origin, int64_t n_vertices,
+                                const int64_t *d_faces, int64_t n_faces, float *d_out_vertices, int32_t *d_out_faces,
+                                void *stream)
+{
+    if (n_vertices < 0 || n_faces < 0)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "negative vertex or face count");
+    if (n_vertices > (1ll << 40) || n_faces > (1ll << 40))
+        return fail(OVX_ERR_CAPACITY, "vertex or face count beyond 2^40");
+    if (n_vertices > 0 && (!d_vertices || !d_origin || !d_out_vertices))
+        return fail(OVX_ERR_INVALID_ARGUMENT, "vertex, origin or output vertex pointer is null");
+    if (d_faces && n_faces > 0 && !d_out_faces)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "output face pointer is null");
+    const uint64_t nf3 = d_faces ? 3ull * (uint64_t)n_faces : 0ull;
+    if (n_vertices == 0 && nf3 == 0)
+        return OVX_OK;
+    HIP_TRY(launch_prepare_mesh(d_vertices, d_origin, (uint64_t)n_vertices, d_out_vertices, d_faces, nf3, d_out_faces,
+                                (hipStream_t)stream));
+    return OVX_OK;
+}
+
 int64_t ovx_fdg_workspace_bytes(const ovx_fdg_ctx *c) { return c ? held_bytes(const_cast<ovx_fdg_ctx *>(c)) : 0; }
 
 ovx_status ovx_fdg_trim(ovx_fdg_ctx *c)

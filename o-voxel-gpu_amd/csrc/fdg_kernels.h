@@ -324,6 +324,10 @@ hipError_t launch_face(const MeshDev &m, const BatchDev &b, int tile, uint64_t n
 hipError_t launch_gen_events(const ShardBuf<Event> &ev, const BucketArgs &ba, hipStream_t s);
 hipError_t launch_gen_fhits(const ShardBuf<FHit> &h, const BucketArgs &ba, hipStream_t s);
 hipError_t launch_shard_reduce(unsigned long long *shard_ctr, unsigned long long *ctr, hipStream_t s);
+// vout = v - origin per vertex (nv vertices); f32 = f64 (nf3 indices, -1 where
+// outside [0, 2^31)); either part may be empty
+hipError_t launch_prepare_mesh(const float *v, const float *origin, uint64_t nv, float *vout, const int64_t *f64,
+                               uint64_t nf3, int32_t *f32, hipStream_t s);
 // fills n 64-bit / 32-bit words with v (a kernel: capturable in a graph)
 hipError_t launch_fill_u64(void *p, uint64_t n, uint64_t v, hipStream_t s);
 hipError_t launch_fill_u32(void *p, uint64_t n, uint32_t v, hipStream_t s);

@@ -3763,6 +3763,73 @@ __global__ __launch_bounds__(BT) void k_fill32(uint32_t *p, uint64_t n, uint32_t
         p[i] = v;
 }
 
+// A conversion's inputs from the caller's arrays, in one pass: grid-local
+// vertices (each coordinate minus the aabb's min corner, one FP32
+// subtraction) and, when the faces are 64-bit, their indices as 32-bit ones;
+// an index outside [0, 2^31) becomes -1, which the conversion rejects as a
+// face index out of range.  nvf = 3 x vertices floats, nf3 = 3 x faces
+// indices; with vec, 16-byte loads (4 floats, 2 indices) and the few
+// elements past the last whole vector one at a time.
+__device__ __forceinline__ int32_t face_index_i32(int64_t x)
+{
+    return (x >= 0 && x <= (int64_t)INT32_MAX) ? (int32_t)x : -1;
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(BT) void k_prepare_mesh(const float *v, const float *origin, uint64_t nvf, float *vout,
+                                                     const int64_t *f64, uint64_t nf3, int32_t *f32)
+{
+    // float k of the vertex array is coordinate k % 3 of its vertex
+    const float o0 = origin[0], o1 = origin[1], o2 = origin[2];
+    const uint64_t stride = (uint64_t)gridDim.x * BT;
+    const uint64_t t0 = (uint64_t)blockIdx.x * BT + threadIdx.x;
+    uint64_t vdone = 0, fdone = 0;
+    if (VEC)
+    {
+        const uint64_t n4 = nvf / 4, n2 = nf3 / 2;
+        for (uint64_t i = t0; i < n4; i += stride)
+        {
+            const float4 x = reinterpret_cast<const float4 *>(v)[i];
+            const int b = (int)((4 * i) % 3); // coordinate of the vector's first float
+            const float p0 = sel3f(b, o0, o1, o2);
+            const float p1 = sel3f(b == 2 ? 0 : b + 1, o0, o1, o2);
+            const float p2 = sel3f(b == 0 ? 2 : b - 1, o0, o1, o2);
+            float4 y;
+            y.x = x.x - p0;
+            y.y = x.y - p1;
+            y.z = x.z - p2;
+            y.w = x.w - p0;
+            reinterpret_cast<float4 *>(vout)[i] = y;
+        }
+        for (uint64_t i = t0; i < n2; i += stride)
+        {
+            const longlong2 x = reinterpret_cast<const longlong2 *>(f64)[i];
+            reinterpret_cast<int2 *>(f32)[i] = make_int2(face_index_i32(x.x), face_index_i32(x.y));
+        }
+        vdone = 4 * n4;
+        fdone = 2 * n2;
+    }
+    for (uint64_t k = vdone + t0; k < nvf; k += stride)
+        vout[k] = v[k] - sel3f((int)(k % 3), o0, o1, o2);
+    for (uint64_t j = fdone + t0; j < nf3; j += stride)
+        f32[j] = face_index_i32(f64[j]);
+}
+
+hipError_t launch_prepare_mesh(const float *v, const float *origin, uint64_t nv, float *vout, const int64_t *f64,
+                               uint64_t nf3, int32_t *f32, hipStream_t s)
+{
+    const uint64_t nvf = 3 * nv;
+    const uint64_t n = nvf > nf3 ? nvf : nf3;
+    const bool vec = ((uintptr_t)v % 16 == 0) && ((uintptr_t)vout % 16 == 0) && ((uintptr_t)f64 % 16 == 0) &&
+                     ((uintptr_t)f32 % 8 == 0);
+    const unsigned grid = grid_for((int64_t)(vec ? n / 2 + 1 : n), 8192);
+    if (vec)
+        hipLaunchKernelGGL(k_prepare_mesh<true>, dim3(grid), dim3(BT), 0, s, v, origin, nvf, vout, f64, nf3, f32);
+    else
+        hipLaunchKernelGGL(k_prepare_mesh<false>, dim3(grid), dim3(BT), 0, s, v, origin, nvf, vout, f64, nf3, f32);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_u64(void *p, uint64_t n, uint64_t v, hipStream_t s)
 {
     if (n)

@@ -100,6 +100,7 @@ def _load():
         L.ovx_fdg_workspace_bytes.argtypes = [vp]
         L.ovx_fdg_workspace_bytes.restype = i64
         L.ovx_fdg_trim.argtypes = [vp]
+        L.ovx_fdg_prepare_mesh.argtypes = [vp, vp, i64, vp, i64, vp, vp, vp]
         L.ovx_fdg_begin.argtypes = [vp, vp, i64, vp, i64, ctypes.POINTER(_Grid), f32, f32, f32, vp,
                                     ctypes.POINTER(i64)]
         L.ovx_fdg_finish.argtypes = [vp, vp, vp, vp, vp]
@@ -232,6 +233,37 @@ def _stream_handle(device: torch.device) -> int:
 
 
 _PRECISIONS = {"reference": 0, "compensated": 1}
+
+
+def _prepare_mesh(vertices: torch.Tensor, faces: torch.Tensor, origin: torch.Tensor):
+    """(vertices - origin as float32, faces as int32) of device tensors, in
+    one library kernel on the current stream: the preparation the public
+    wrapper does before the extension call (reference
+    o_voxel/convert/flexible_dual_grid.py:89-90,181), with no framework
+    kernel.  vertices float32 [V,3] and faces int32 / int64 [F,3], contiguous;
+    origin float32 [3] on the same device.  int32 faces are returned as they
+    are; an int64 index outside [0, 2^31) becomes -1 (rejected by the
+    conversion as out of range)."""
+    _require_device_tensor(vertices, "vertices", torch.float32, 2)
+    if faces.dtype not in (torch.int32, torch.int64):
+        raise RuntimeError(f"faces must be int32 or int64, got {faces.dtype}")
+    if not faces.is_cuda or faces.device != vertices.device or origin.device != vertices.device:
+        raise RuntimeError("vertices, faces and origin must be on one HIP device")
+    if origin.dtype != torch.float32 or origin.numel() != 3 or not origin.is_contiguous():
+        raise RuntimeError("origin must be 3 contiguous float32 values")
+    if not vertices.is_contiguous() or not faces.is_contiguous():
+        raise RuntimeError("vertices and faces must be contiguous")
+    dev = vertices.device
+    L = _load()
+    v_out = torch.empty_like(vertices)
+    f64 = faces.dtype == torch.int64
+    f_out = torch.empty(faces.shape, dtype=torch.int32, device=dev) if f64 else faces
+    with torch.cuda.device(dev):
+        _check(L.ovx_fdg_prepare_mesh(vertices.data_ptr(), origin.data_ptr(), vertices.size(0),
+                                      faces.data_ptr() if f64 else None, faces.size(0) if f64 else 0,
+                                      v_out.data_ptr(), f_out.data_ptr() if f64 else None, _stream_handle(dev)),
+               "ovx_fdg_prepare_mesh")
+    return v_out, f_out
 
 
 def mesh_to_flexible_dual_grid_cuda(

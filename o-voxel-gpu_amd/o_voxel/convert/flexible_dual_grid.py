@@ -172,12 +172,20 @@ def mesh_to_flexible_dual_grid(
     _check_mesh(vertices, faces)
     device = vertices.device
     vertices = vertices.to(device=device, dtype=torch.float32).contiguous()
+    if vertices.is_cuda:
+        # the int32 cast and the shift by the aabb's min corner in one library
+        # kernel (no framework kernel, whose first launch in a process loads
+        # its code): the same values as the reference's torch operations
+        if faces.dtype not in (torch.int32, torch.int64):
+            faces = faces.to(dtype=torch.int64)
+        faces = faces.contiguous()
+        voxel_size_arg, grid_range_arg, aabb_dev = _grid_setup(vertices, voxel_size, grid_size, aabb)
+        vertices, faces = _C._prepare_mesh(vertices, faces, aabb_dev[0].contiguous())
+        return _C.mesh_to_flexible_dual_grid_cuda(vertices, faces, voxel_size_arg, grid_range_arg, face_weight,
+                                                  boundary_weight, regularization_weight, precision=precision)
     faces = faces.to(device=device, dtype=torch.int32).contiguous()
     voxel_size_arg, grid_range_arg, aabb_dev = _grid_setup(vertices, voxel_size, grid_size, aabb)
     vertices = vertices - aabb_dev[0].reshape(1, 3)
-    if vertices.is_cuda:
-        return _C.mesh_to_flexible_dual_grid_cuda(vertices, faces, voxel_size_arg, grid_range_arg, face_weight,
-                                                  boundary_weight, regularization_weight, precision=precision)
     return _C.mesh_to_flexible_dual_grid_cpu(vertices, faces, voxel_size_arg, grid_range_arg, face_weight,
                                              boundary_weight, regularization_weight, timing, precision=precision)
 

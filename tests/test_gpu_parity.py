@@ -157,6 +157,47 @@ def test_public_api_matches_c_boundary(oracle):
     assert np.array_equal(i.cpu().numpy(), ref.flags[perm])
 
 
+def test_prepare_mesh_matches_torch():
+    """The wrapper's input preparation (one library kernel) gives exactly
+    torch's `vertices - aabb[0]` and `faces.to(int32)`; an int64 index outside
+    [0, 2^31) becomes -1, which the public API reports as out of range
+    (a plain int32 cast would wrap 2^32 to the valid index 0)."""
+    dev = _dev()
+    rng = np.random.default_rng(11)
+    v = torch.from_numpy(rng.normal(0.0, 3.0, (100003, 3)).astype(np.float32)).to(dev)
+    o = torch.from_numpy(rng.normal(0.0, 1.0, 3).astype(np.float32)).to(dev)
+    f = torch.from_numpy(rng.integers(0, 100003, (77777, 3))).to(dev)
+    print("[case] prepare: int64 faces", flush=True)
+    vp, fp = _C()._prepare_mesh(v, f, o)
+    torch.cuda.synchronize()
+    assert torch.equal(vp, v - o.reshape(1, 3))
+    assert fp.dtype == torch.int32 and torch.equal(fp, f.to(torch.int32))
+    print("[case] prepare: int32 faces", flush=True)
+    f32 = f.to(torch.int32)
+    vp2, fp2 = _C()._prepare_mesh(v, f32, o)
+    assert fp2 is f32 and torch.equal(vp2, vp)
+    print("[case] prepare: views not 16-byte aligned (element-wise path)", flush=True)
+    vp3, fp3 = _C()._prepare_mesh(v[1:], f[1:], o)
+    assert torch.equal(vp3, v[1:] - o.reshape(1, 3)) and torch.equal(fp3, f[1:].to(torch.int32))
+    print("[case] prepare: int64 indices outside [0, 2^31)", flush=True)
+    fb = f.clone()
+    fb[5, 1] = 1 << 33
+    fb[9, 2] = -7
+    fb[11, 0] = 1 << 31
+    _, fpb = _C()._prepare_mesh(v, fb, o)
+    assert fpb[5, 1].item() == -1 and fpb[9, 2].item() == -1 and fpb[11, 0].item() == -1
+    assert torch.equal(fpb[12:], fp[12:])
+    import o_voxel
+
+    vv, ff = meshgen.ico_cap()
+    ff = ff.astype(np.int64).copy()
+    ff[3, 0] = 1 << 32
+    print("[case] public API: int64 index 2^32", flush=True)
+    with pytest.raises(RuntimeError, match="vertex index"):
+        o_voxel.convert.mesh_to_flexible_dual_grid(torch.from_numpy(vv).to(dev), torch.from_numpy(ff).to(dev),
+                                                   grid_size=64, aabb=[[-0.5] * 3, [0.5] * 3])
+
+
 def test_determinism(oracle):
     v, f = meshgen.scene(seed=4, target_faces=15000)
     vs, gr, a0 = meshgen.grid_args(128)
