@@ -88,6 +88,34 @@ ovx_status ovx_fdg_prepare_mesh(const float *d_vertices, const float *d_origin, 
                                 void *stream);
 
 /*
+ * Packed inputs of a batched conversion (ovx_fdg_batch_begin) from each
+ * mesh's own arrays, in one kernel on `stream`: what the batch wrapper
+ * needs (no counterpart in the reference, which converts one mesh per call,
+ * FDG.py:89-90,181 per mesh).  d_meshes is a DEVICE array of n_meshes
+ * records; mesh m's vertices minus its origin go to rows
+ * [vertex_offset, vertex_offset + n_vertices) of d_out_vertices
+ * [total_vertices, 3], its faces plus vertex_offset, as int32, to rows
+ * [face_offset, face_offset + n_faces) of d_out_faces [total_faces, 3].
+ * Offsets ascend mesh after mesh.  A face index outside [0, n_vertices of
+ * its mesh) becomes -1, which ovx_fdg_batch_begin rejects as out of range.
+ * Writes stay inside the outputs whatever the table holds.  total_vertices
+ * must be < 2^31 (OVX_ERR_CAPACITY).  Needs no context; returns without
+ * waiting.
+ */
+typedef struct ovx_mesh_src
+{
+    const float *vertices; /* device, [n_vertices, 3] float32 */
+    const void *faces;     /* device, [n_faces, 3] int32, or int64 when faces_int64 */
+    const float *origin;   /* device, 3 floats subtracted from the mesh's vertices */
+    int64_t n_vertices, n_faces;
+    int64_t vertex_offset, face_offset; /* first packed rows of the mesh */
+    int32_t faces_int64;
+    int32_t reserved;
+} ovx_mesh_src;
+ovx_status ovx_fdg_batch_prepare(const ovx_mesh_src *d_meshes, int n_meshes, int64_t total_vertices,
+                                 int64_t total_faces, float *d_out_vertices, int32_t *d_out_faces, void *stream);
+
+/*
  * mesh_to_flexible_dual_grid, device path.
  * Replaces o_voxel::fdg::mesh_to_flexible_dual_grid_cuda
  * (src/convert/mesh_to_flexible_dual_grid_gpu/mesh_to_flexible_dual_grid.cu:797-895;

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <atomic>
+#include <cstddef>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -1529,6 +1530,32 @@ ovx_status ovx_fdg_prepare_mesh(const float *d_vertices, const float *d_origin, 
         return OVX_OK;
     HIP_TRY(launch_prepare_mesh(d_vertices, d_origin, (uint64_t)n_vertices, d_out_vertices, d_faces, nf3, d_out_faces,
                                 (hipStream_t)stream));
+    return OVX_OK;
+}
+
+static_assert(sizeof(MeshSrc) == sizeof(ovx_mesh_src) && offsetof(MeshSrc, origin) == offsetof(ovx_mesh_src, origin) &&
+                  offsetof(MeshSrc, n_vertices) == offsetof(ovx_mesh_src, n_vertices) &&
+                  offsetof(MeshSrc, face_offset) == offsetof(ovx_mesh_src, face_offset) &&
+                  offsetof(MeshSrc, faces_int64) == offsetof(ovx_mesh_src, faces_int64),
+              "ovx_mesh_src layout");
+
+ovx_status ovx_fdg_batch_prepare(const ovx_mesh_src *d_meshes, int n_meshes, int64_t total_vertices,
+                                 int64_t total_faces, float *d_out_vertices, int32_t *d_out_faces, void *stream)
+{
+    if (n_meshes < 0 || total_vertices < 0 || total_faces < 0)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "negative mesh, vertex or face count");
+    if (n_meshes == 0 || (total_vertices == 0 && total_faces == 0))
+        return OVX_OK;
+    if (!d_meshes)
+        return fail(OVX_ERR_INVALID_ARGUMENT, "mesh table pointer is null");
+    if (total_vertices > (int64_t)INT32_MAX)
+        return fail(OVX_ERR_CAPACITY, "%lld vertices: packed face indices are 32-bit", (long long)total_vertices);
+    if (total_faces > (1ll << 40))
+        return fail(OVX_ERR_CAPACITY, "face count beyond 2^40");
+    if ((total_vertices > 0 && !d_out_vertices) || (total_faces > 0 && !d_out_faces))
+        return fail(OVX_ERR_INVALID_ARGUMENT, "output pointer is null");
+    HIP_TRY(launch_batch_prepare((const MeshSrc *)d_meshes, n_meshes, (uint64_t)total_vertices, (uint64_t)total_faces,
+                                 d_out_vertices, d_out_faces, (hipStream_t)stream));
     return OVX_OK;
 }
 

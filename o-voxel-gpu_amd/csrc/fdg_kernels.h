@@ -324,6 +324,20 @@ hipError_t launch_face(const MeshDev &m, const BatchDev &b, int tile, uint64_t n
 hipError_t launch_gen_events(const ShardBuf<Event> &ev, const BucketArgs &ba, hipStream_t s);
 hipError_t launch_gen_fhits(const ShardBuf<FHit> &h, const BucketArgs &ba, hipStream_t s);
 hipError_t launch_shard_reduce(unsigned long long *shard_ctr, unsigned long long *ctr, hipStream_t s);
+// One mesh of a batch to pack (the layout of the C ABI's ovx_mesh_src)
+struct MeshSrc
+{
+    const float *vertices; // [n_vertices, 3]
+    const void *faces;     // [n_faces, 3], int32 or int64
+    const float *origin;   // 3 floats subtracted from the vertices
+    int64_t n_vertices, n_faces;
+    int64_t vertex_offset, face_offset; // first packed rows
+    int32_t faces_int64;
+    int32_t reserved;
+};
+// packed vertices (nv rows) and faces (nf rows) of the M meshes of table t (device)
+hipError_t launch_batch_prepare(const MeshSrc *t, int M, uint64_t nv, uint64_t nf, float *vout, int32_t *fout,
+                                hipStream_t s);
 // vout = v - origin per vertex (nv vertices); f32 = f64 (nf3 indices, -1 where
 // outside [0, 2^31)); either part may be empty
 hipError_t launch_prepare_mesh(const float *v, const float *origin, uint64_t nv, float *vout, const int64_t *f64,

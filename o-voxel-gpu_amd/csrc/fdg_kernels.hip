@@ -3830,6 +3830,99 @@ hipError_t launch_prepare_mesh(const float *v, const float *origin, uint64_t nv,
     return hipGetLastError();
 }
 
+// Packed inputs of a batched conversion from each mesh's own arrays (the
+// concatenation, the shift by each mesh's origin and the rebase of the face
+// indices the batch wrapper needs) in one pass.  Block b writes the packed
+// vertex floats and the packed face indices [b * PACK_CH, (b + 1) * PACK_CH):
+// it finds the mesh of its first element by a binary search over the meshes'
+// first rows and walks on across mesh boundaries.  A face index outside
+// [0, the mesh's vertex count) becomes -1, which the conversion rejects as
+// out of range.  Writes stay inside the packed outputs whatever the table
+// holds; reads stay inside each mesh's arrays given its counts.
+constexpr int PACK_CH = 4096;
+
+template <int FACES>
+__device__ __forceinline__ int pack_first_mesh(const MeshSrc *t, int M, uint64_t e0)
+{
+    int lo = 0, hi = M - 1;
+    while (lo < hi)
+    {
+        const int mid = (lo + hi + 1) >> 1;
+        if (3ull * (uint64_t)(FACES ? t[mid].face_offset : t[mid].vertex_offset) <= e0)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename I>
+__device__ __forceinline__ void pack_faces(const I *f, uint64_t mb, uint64_t lo, uint64_t hi, int64_t nv, int64_t base,
+                                           int32_t *fout)
+{
+    for (uint64_t k = lo + threadIdx.x; k < hi; k += BT)
+    {
+        const int64_t x = (int64_t)f[k - mb];
+        fout[k] = (x >= 0 && x < nv && x + base <= (int64_t)INT32_MAX) ? (int32_t)(x + base) : -1;
+    }
+}
+
+__global__ __launch_bounds__(BT) void k_batch_prepare(const MeshSrc *t, int M, uint64_t nvf, uint64_t nf3, float *vout,
+                                                      int32_t *fout)
+{
+    const uint64_t e0 = (uint64_t)blockIdx.x * PACK_CH;
+    if (e0 < nvf)
+    {
+        const uint64_t e1 = e0 + PACK_CH < nvf ? e0 + PACK_CH : nvf;
+        for (int m = pack_first_mesh<0>(t, M, e0); m < M; ++m)
+        {
+            const MeshSrc s = t[m];
+            const uint64_t mb = 3ull * (uint64_t)s.vertex_offset, me = mb + 3ull * (uint64_t)s.n_vertices;
+            if (mb >= e1)
+                break;
+            const uint64_t lo = e0 > mb ? e0 : mb, hi = e1 < me ? e1 : me;
+            if (hi <= lo)
+                continue;
+            const float o0 = s.origin[0], o1 = s.origin[1], o2 = s.origin[2];
+            for (uint64_t k = lo + threadIdx.x; k < hi; k += BT)
+            {
+                const uint64_t i = k - mb;
+                vout[k] = s.vertices[i] - sel3f((int)(i % 3), o0, o1, o2);
+            }
+        }
+    }
+    if (e0 < nf3)
+    {
+        const uint64_t e1 = e0 + PACK_CH < nf3 ? e0 + PACK_CH : nf3;
+        for (int m = pack_first_mesh<1>(t, M, e0); m < M; ++m)
+        {
+            const MeshSrc s = t[m];
+            const uint64_t mb = 3ull * (uint64_t)s.face_offset, me = mb + 3ull * (uint64_t)s.n_faces;
+            if (mb >= e1)
+                break;
+            const uint64_t lo = e0 > mb ? e0 : mb, hi = e1 < me ? e1 : me;
+            if (hi <= lo)
+                continue;
+            if (s.faces_int64)
+                pack_faces((const int64_t *)s.faces, mb, lo, hi, s.n_vertices, s.vertex_offset, fout);
+            else
+                pack_faces((const int32_t *)s.faces, mb, lo, hi, s.n_vertices, s.vertex_offset, fout);
+        }
+    }
+}
+
+hipError_t launch_batch_prepare(const MeshSrc *t, int M, uint64_t nv, uint64_t nf, float *vout, int32_t *fout,
+                                hipStream_t s)
+{
+    const uint64_t nvf = 3 * nv, nf3 = 3 * nf;
+    const uint64_t n = nvf > nf3 ? nvf : nf3;
+    if (n == 0 || M <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_batch_prepare, dim3((unsigned)((n + PACK_CH - 1) / PACK_CH)), dim3(BT), 0, s, t, M, nvf, nf3,
+                       vout, fout);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_u64(void *p, uint64_t n, uint64_t v, hipStream_t s)
 {
     if (n)

@@ -20,6 +20,7 @@ import os
 import threading
 from typing import List, Tuple
 
+import numpy as np
 import torch
 
 __all__ = [
@@ -101,6 +102,7 @@ def _load():
         L.ovx_fdg_workspace_bytes.restype = i64
         L.ovx_fdg_trim.argtypes = [vp]
         L.ovx_fdg_prepare_mesh.argtypes = [vp, vp, i64, vp, i64, vp, vp, vp]
+        L.ovx_fdg_batch_prepare.argtypes = [vp, ctypes.c_int, i64, i64, vp, vp, vp]
         L.ovx_fdg_begin.argtypes = [vp, vp, i64, vp, i64, ctypes.POINTER(_Grid), f32, f32, f32, vp,
                                     ctypes.POINTER(i64)]
         L.ovx_fdg_finish.argtypes = [vp, vp, vp, vp, vp]
@@ -266,6 +268,58 @@ def _prepare_mesh(vertices: torch.Tensor, faces: torch.Tensor, origin: torch.Ten
     return v_out, f_out
 
 
+def _batch_prepare(vertices: List[torch.Tensor], faces: List[torch.Tensor], origins=None):
+    """Packed inputs of a batched conversion from per-mesh device tensors, in
+    one library kernel on the current stream: (vertices - origins[m], stacked,
+    float32 [sum V_m, 3]; faces + first vertex row of their mesh, stacked,
+    int32 [sum F_m, 3]).  vertices[m] float32 [V_m, 3], faces[m] int32 or
+    int64 [F_m, 3], contiguous; origins float32 [M, 3] on the device, or None
+    for no shift.  A face index outside [0, V_m) becomes -1 (rejected by the
+    conversion as out of range)."""
+    M = len(vertices)
+    if len(faces) != M:
+        raise RuntimeError("vertices and faces must have one entry per mesh")
+    dev = vertices[0].device
+    for k in range(M):
+        _require_device_tensor(vertices[k], f"vertices[{k}]", torch.float32, 2)
+        f = faces[k]
+        if not isinstance(f, torch.Tensor) or f.dtype not in (torch.int32, torch.int64) or f.dim() != 2:
+            raise RuntimeError(f"faces[{k}] must be a 2-D int32 or int64 tensor")
+        if vertices[k].size(1) != 3 or f.size(1) != 3:
+            raise RuntimeError("vertices and faces must have 3 columns")
+        if vertices[k].device != dev or f.device != dev:
+            raise RuntimeError("all meshes must be on one HIP device")
+        if not vertices[k].is_contiguous() or not f.is_contiguous():
+            raise RuntimeError("vertices and faces must be contiguous")
+    nv = [int(v.size(0)) for v in vertices]
+    nf = [int(f.size(0)) for f in faces]
+    V, F = sum(nv), sum(nf)
+    if V > 2 ** 31 - 1:
+        raise RuntimeError(f"{V} vertices in the batch: packed face indices are 32-bit")
+    if origins is None:
+        origins = torch.zeros((M, 3), dtype=torch.float32).pin_memory().to(dev, non_blocking=True)
+    if (origins.dtype != torch.float32 or tuple(origins.shape) != (M, 3) or not origins.is_contiguous()
+            or origins.device != dev):
+        raise RuntimeError("origins must be a contiguous float32 [M, 3] tensor on the meshes' device")
+    table = np.zeros((M, 8), dtype=np.int64)
+    table[:, 0] = [v.data_ptr() for v in vertices]
+    table[:, 1] = [f.data_ptr() for f in faces]
+    table[:, 2] = origins.data_ptr() + 12 * np.arange(M, dtype=np.int64)
+    table[:, 3] = nv
+    table[:, 4] = nf
+    table[:, 5] = np.cumsum([0] + nv[:-1])
+    table[:, 6] = np.cumsum([0] + nf[:-1])
+    table[:, 7] = [1 if f.dtype == torch.int64 else 0 for f in faces]  # faces_int64, then a zero int32
+    tab = torch.from_numpy(table).pin_memory().to(dev, non_blocking=True)
+    v_all = torch.empty((V, 3), dtype=torch.float32, device=dev)
+    f_all = torch.empty((F, 3), dtype=torch.int32, device=dev)
+    L = _load()
+    with torch.cuda.device(dev):
+        _check(L.ovx_fdg_batch_prepare(tab.data_ptr(), M, V, F, v_all.data_ptr(), f_all.data_ptr(), _stream_handle(dev)),
+               "ovx_fdg_batch_prepare")
+    return v_all, f_all, nv, nf
+
+
 def mesh_to_flexible_dual_grid_cuda(
     vertices: torch.Tensor,
     faces: torch.Tensor,
@@ -409,16 +463,10 @@ def mesh_to_flexible_dual_grid_batch_cuda(
     for k in range(M):
         _require_device_tensor(vertices[k], f"vertices[{k}]", torch.float32, 2)
         _require_device_tensor(faces[k], f"faces[{k}]", torch.int32, 2)
-    dev = vertices[0].device
-    nv = [int(v.size(0)) for v in vertices]
-    nf = [int(f.size(0)) for f in faces]
-    v_all = torch.cat(vertices, 0)
-    f_all = torch.cat(faces, 0)
-    base = torch.tensor([0] + nv[:-1], dtype=torch.int64).cumsum(0).to(torch.int32).to(dev)
-    f_all = f_all + torch.repeat_interleave(base, torch.tensor(nf, device=dev), output_size=sum(nf)).unsqueeze(1)
-    return mesh_to_flexible_dual_grid_batch_packed_cuda(v_all, f_all.to(torch.int32), nv, nf, voxel_sizes, grid_ranges,
-                                                        face_weight, boundary_weight, regularization_weight,
-                                                        precision=precision, cond_max=cond_max)
+    v_all, f_all, nv, nf = _batch_prepare([v.contiguous() for v in vertices], [f.contiguous() for f in faces])
+    return mesh_to_flexible_dual_grid_batch_packed_cuda(v_all, f_all, nv, nf, voxel_sizes, grid_ranges, face_weight,
+                                                        boundary_weight, regularization_weight, precision=precision,
+                                                        cond_max=cond_max)
 
 
 def intersect_occ_cuda(triangles: torch.Tensor, voxel_size: List[float], grid_range: List[int]) -> torch.Tensor:

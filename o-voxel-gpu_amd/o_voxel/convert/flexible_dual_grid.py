@@ -272,11 +272,24 @@ def mesh_to_flexible_dual_grid_batch(
         shift_rows.append(a0)
         v_list.append(vertices)
         f_list.append(faces)
+    if device.type == "cuda":
+        # packed grid-local vertices and batch-global int32 face indices in
+        # one library kernel (no framework kernel); each mesh's shift row goes
+        # up with the others in one non-blocking copy
+        if host_shift:
+            shift = _to_device(torch.cat(shift_rows, 0), device)
+        else:
+            shift = torch.cat([r.to(device=device) for r in shift_rows], 0).contiguous()
+        f_list = [(f if f.dtype in (torch.int32, torch.int64) else f.to(dtype=torch.int64)).contiguous()
+                  for f in f_list]
+        v_all, f_all, nv, nf = _C._batch_prepare([v.contiguous() for v in v_list], f_list, shift)
+        return _C.mesh_to_flexible_dual_grid_batch_packed_cuda(v_all, f_all, nv, nf, vs_list, gr_list, face_weight,
+                                                               boundary_weight, regularization_weight,
+                                                               precision=precision)
     nv = [int(v.size(0)) for v in v_list]
     nf = [int(f.size(0)) for f in f_list]
-    # packed grid-local vertices and batch-global face indices: a handful of
-    # whole-batch tensor operations; the per-mesh tables go up in one
-    # non-blocking copy
+    # host meshes: packed with whole-batch tensor operations, then converted
+    # on the current GPU
     nv_t = torch.tensor(nv, dtype=torch.int64)
     nf_t = torch.tensor(nf, dtype=torch.int64)
     counts = _to_device(torch.cat([nv_t, nf_t, torch.cumsum(nv_t, 0) - nv_t]), device)
@@ -288,10 +301,6 @@ def mesh_to_flexible_dual_grid_batch(
     v_all = torch.cat(v_list, 0) - torch.repeat_interleave(shift, nv_d, dim=0, output_size=sum(nv))
     f_all = torch.cat([f if f.dtype == torch.int64 else f.to(torch.int64) for f in f_list], 0)
     f_all = (f_all + torch.repeat_interleave(base, nf_d, output_size=sum(nf)).unsqueeze(1)).to(torch.int32).contiguous()
-    if device.type == "cuda":
-        return _C.mesh_to_flexible_dual_grid_batch_packed_cuda(v_all.contiguous(), f_all, nv, nf, vs_list, gr_list,
-                                                               face_weight, boundary_weight, regularization_weight,
-                                                               precision=precision)
     dev = _C._gpu_for_host_call()
     out = _C.mesh_to_flexible_dual_grid_batch_packed_cuda(v_all.to(dev).contiguous(), f_all.to(dev), nv, nf, vs_list,
                                                           gr_list, face_weight, boundary_weight,

@@ -85,6 +85,46 @@ def test_batch_of_one_is_plain_path():
         assert np.array_equal(a, b)
 
 
+def test_batch_prepare_matches_torch():
+    """The batch wrapper's packing kernel gives exactly torch's concatenation
+    of `vertices - origin` and of the faces rebased by their mesh's first
+    vertex row, over meshes with int32 and int64 faces, an empty mesh, a mesh
+    without faces and meshes spanning several kernel blocks; an index outside
+    its own mesh becomes -1."""
+    from o_voxel import _C
+
+    dev = _dev()
+    rng = np.random.default_rng(17)
+    sizes = [(5000, 9000), (0, 0), (3, 1), (1777, 0), (20011, 31337), (1, 1), (4096, 1366)]
+    vs, fs = [], []
+    for k, (nv, nf) in enumerate(sizes):
+        vs.append(torch.from_numpy(rng.normal(0.0, 2.0, (nv, 3)).astype(np.float32)).to(dev))
+        f = rng.integers(0, max(nv, 1), (nf, 3))
+        fs.append(torch.from_numpy(f.astype(np.int64 if k % 2 else np.int32)).to(dev))
+    org = torch.from_numpy(rng.normal(0.0, 1.0, (len(sizes), 3)).astype(np.float32)).to(dev)
+    print("[case] batch prepare: 7 meshes", flush=True)
+    v_all, f_all, nv, nf = _C._batch_prepare(vs, fs, org)
+    torch.cuda.synchronize()
+    want_v = torch.cat([v - org[k].reshape(1, 3) for k, v in enumerate(vs)], 0)
+    base = np.cumsum([0] + nv[:-1])
+    want_f = torch.cat([(f.to(torch.int64) + int(base[k])).to(torch.int32) for k, f in enumerate(fs)], 0)
+    assert nv == [s[0] for s in sizes] and nf == [s[1] for s in sizes]
+    assert torch.equal(v_all, want_v) and torch.equal(f_all, want_f)
+    print("[case] batch prepare: indices outside their mesh", flush=True)
+    fs[4] = fs[4].clone()
+    fs[4][7, 2] = 20011          # first vertex of the next mesh: outside this one
+    fs[4][9, 0] = -1
+    fs[5] = fs[5].clone()        # int64 faces
+    fs[5][0, 1] = 1 << 40
+    _, f_bad, _, _ = _C._batch_prepare(vs, fs, org)
+    r4 = int(sum(nf[:4]))
+    r5 = int(sum(nf[:5]))
+    assert f_bad[r4 + 7, 2].item() == -1 and f_bad[r4 + 9, 0].item() == -1 and f_bad[r5, 1].item() == -1
+    print("[case] batch prepare: no origins", flush=True)
+    v0, _, _, _ = _C._batch_prepare(vs, fs, None)
+    assert torch.equal(v0, torch.cat(vs, 0))
+
+
 def test_batch_invalid_face_index_raises():
     """An index that names a vertex of another mesh of the batch is an input error."""
     dev = _dev()
