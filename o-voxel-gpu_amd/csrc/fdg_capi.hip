@@ -113,7 +113,7 @@ struct ovx_fdg_ctx
     int svo_state = 0;            // 1 begun, 2 written
     DevBuf vcnt, voff, nbr, erank, bcnt, boff, bedge_v, bedges, eslots;
     bool edge_csr = false; // the last conversion's boundary edges came from the CSR lists (a vertex overflowed its slots)
-    DevBuf events, recs, chunks, heads, choff, bchunks, active, medium, large, huge, shard_ctr, rowq, faceq;
+    DevBuf events, recs, chunks, heads, choff, bchunks, tiny, active, medium, large, huge, shard_ctr, rowq, faceq;
     // face and boundary hits, and, once the records are made from them, the
     // per-voxel QEFs and the box-search queue in the same bytes
     DevBuf hitqef;
@@ -195,7 +195,7 @@ struct ovx_fdg_ctx
     void for_each_buf(Fn &&fn)
     {
         DevBuf *all[] = {&planes, &tris, &bits,   &woff,  &tiles,   &vcnt,   &voff,  &nbr, &erank, &bcnt, &boff, &eslots,
-                         &bedge_v, &bedges, &events, &recs,   &chunks, &heads, &choff, &bchunks, &active, &medium, &large, &huge,
+                         &bedge_v, &bedges, &events, &recs,   &chunks, &heads, &choff, &bchunks, &tiny, &active, &medium, &large, &huge,
                          &hitqef,  &shard_ctr, &rowq, &faceq, &segbuf,
                          &dm_table, &dm_cnt, &dm_off, &dm_mask, &dm_tiles, &dm_direct, &dm_touched,
                          &svo_insf, &svo_k, &svo_pos, &svo_tiles, &svo_rows, &svo_erow, &nb_table, &nb_own,
@@ -1073,21 +1073,25 @@ ovx_status enqueue_finish(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t 
     HIP_TRY(launch_place_chunks(shard_view<Chunk>(c, c->chunks, SB_CHUNKS), (const uint32_t *)c->choff.p,
                                 (uint2 *)c->bchunks.p, ctr + CTR_OVER, s));
     TierLists tl;
-    tl.list[0] = (uint32_t *)c->active.p;
-    tl.list[1] = (uint32_t *)c->medium.p;
-    tl.list[2] = (uint32_t *)c->large.p;
-    tl.list[3] = (uint32_t *)c->huge.p;
-    tl.count[0] = ctr + CTR_ACTIVE;
-    tl.count[1] = ctr + CTR_MEDIUM;
-    tl.count[2] = ctr + CTR_LARGE;
-    tl.count[3] = ctr + CTR_HUGE;
+    tl.list[0] = (uint32_t *)c->tiny.p;
+    tl.list[1] = (uint32_t *)c->active.p;
+    tl.list[2] = (uint32_t *)c->medium.p;
+    tl.list[3] = (uint32_t *)c->large.p;
+    tl.list[4] = (uint32_t *)c->huge.p;
+    tl.count[0] = ctr + CTR_TINY;
+    tl.count[1] = ctr + CTR_ACTIVE;
+    tl.count[2] = ctr + CTR_MEDIUM;
+    tl.count[3] = ctr + CTR_LARGE;
+    tl.count[4] = ctr + CTR_HUGE;
     HIP_TRY(launch_active_bricks((const unsigned long long *)c->heads.p, n_bricks, tl, ctr + CTR_OVER, s));
     toc(c, KS_ACTIVE, s);
     FoldArgs a{};
     a.choff = (const uint32_t *)c->choff.p;
     a.gchunks = (const uint2 *)c->bchunks.p;
     a.recs = (const Rec *)c->recs.p;
-    a.active = (const uint32_t *)c->active.p;
+    a.tiny = (const uint32_t *)c->tiny.p;
+    a.n_tiny = ctr + CTR_TINY;
+    a.active = (uint32_t *)c->active.p;
     a.n_active = ctr + CTR_ACTIVE;
     a.medium = (uint32_t *)c->medium.p;
     a.n_medium = ctr + CTR_MEDIUM;
@@ -1110,7 +1114,7 @@ ovx_status enqueue_finish(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t 
     a.cond_max = c->cond_max;
     a.n64 = (unsigned long long *)c->shard_ctr.p + (uint64_t)SB_FP64 * NSHARD * SHARD_STRIDE;
     a.ctr = ctr;
-    // CTR_ACTIVE, CTR_MEDIUM, CTR_LARGE, CTR_LARGE_RECS and CTR_HUGE are zeroed with all counters by begin (an
+    // CTR_TINY, CTR_ACTIVE, CTR_MEDIUM, CTR_LARGE, CTR_LARGE_RECS and CTR_HUGE are zeroed with all counters by begin (an
     // attempt whose record buffers overflow leaves them at zero)
     tic(c, KS_FOLD, s);
     HIP_TRY(launch_fold_bricks(a, n_bricks, s));
@@ -1282,7 +1286,8 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
         // repeated conversions the same while the chunk count varies with the order of the block claims
         grouped = std::min<uint64_t>(c->bchunks.bytes / 8 - 1, c->per[SB_CHUNKS] * NSHARD);
         if ((st = ensure(c->choff, (n_bricks + 1) * 4)) ||
-            (st = ensure(c->active, n_bricks * 4)) || (st = ensure(c->medium, n_bricks * 4)) ||
+            (st = ensure(c->tiny, n_bricks * 4)) || (st = ensure(c->active, n_bricks * 4)) ||
+            (st = ensure(c->medium, n_bricks * 4)) ||
             (st = ensure(c->large, n_bricks * 4)) ||
             (st = ensure(c->huge, n_bricks * 4)) ||
             // box-search queue: a few percent of the voxels typically; the solve
@@ -1739,7 +1744,7 @@ ovx_status ovx_fdg_stats(ovx_fdg_ctx *c, int64_t *out, int n)
         if (st)
             return st;
         c->stats[7] = (int64_t)c->h_ctr[CTR_GIANT];
-        c->stats[8] = (int64_t)(c->h_ctr[CTR_ACTIVE] + c->h_ctr[CTR_MEDIUM] + c->h_ctr[CTR_LARGE] +
+        c->stats[8] = (int64_t)(c->h_ctr[CTR_TINY] + c->h_ctr[CTR_ACTIVE] + c->h_ctr[CTR_MEDIUM] + c->h_ctr[CTR_LARGE] +
                                 c->h_ctr[CTR_HUGE]);
         c->stats[10] = (int64_t)(c->h_ctr[CTR_LARGE] + c->h_ctr[CTR_HUGE]);
         c->stats[11] = (int64_t)c->h_ctr[CTR_LARGE_RECS];

@@ -27,10 +27,11 @@ constexpr int FACE_LANE_CLASS = 2;
 // device counters (unsigned long long each)
 enum
 {
+    CTR_TINY = 0,        // bricks listed for the first fold tier (one wave per brick)
     CTR_ERRORS = 1,
     CTR_WORDS_TOTAL = 2, // total of the bitset popcount scan (= N)
     CTR_OVER = 3,        // a record-stage buffer of the conversion overflowed (the fold and solve stages skip it)
-    CTR_MEDIUM = 4,      // bricks left to the second fold tier
+    CTR_MEDIUM = 4,      // bricks left to the third fold tier
     CTR_GIANT = 5,
     CTR_EDGE_TOTAL = 6,  // half-edge count scan total
     CTR_BEDGE_TOTAL = 7, // boundary edges
@@ -287,9 +288,11 @@ struct FoldArgs
     const uint32_t *choff;              // [bricks+1] first chunk of each brick in `bchunks`
     const uint2 *gchunks;               // (first record, count) of each chunk, grouped by brick
     const Rec *recs;
-    const uint32_t *active;             // bricks with records
-    const unsigned long long *n_active; // their count (device)
-    uint32_t *medium;                   // bricks too large for the first fold tier
+    const uint32_t *tiny;               // bricks with the fewest voxel entries (first fold tier)
+    const unsigned long long *n_tiny;   // their count (device)
+    uint32_t *active;                   // bricks of the second fold tier
+    unsigned long long *n_active;       // their count (device, zeroed by the host)
+    uint32_t *medium;                   // bricks too large for the second fold tier
     unsigned long long *n_medium;       // their count (device, zeroed by the host)
     uint32_t *large;                    // bricks too large for the main fold tile
     unsigned long long *n_large;        // their count (device, zeroed by the host)
@@ -357,13 +360,14 @@ struct OverCaps
 hipError_t launch_over_check(unsigned long long *ctr, const OverCaps &caps, hipStream_t s);
 hipError_t launch_place_chunks(const ShardBuf<Chunk> &chunks, const uint32_t *choff, uint2 *gchunks,
                                const unsigned long long *over, hipStream_t s);
-// Bricks with records, each listed for the fold tier its record count
-// selects: lists[t] / counts[t] for t = 0 (at most FOLD_SMALL_CAP records),
-// 1 (FOLD_SMALL), 2 (FOLD_MID), 3 (more).
+// Bricks with records, each listed for the fold tier its voxel-entry count
+// selects: lists[t] / counts[t] for t = 0 (at most FOLD_TINY_CAP entries),
+// 1 (FOLD_SMALL_CAP), 2 (FOLD_SMALL), 3 (FOLD_MID), 4 (more).
+constexpr int N_FOLD_TIERS = 5;
 struct TierLists
 {
-    uint32_t *list[4];
-    unsigned long long *count[4];
+    uint32_t *list[N_FOLD_TIERS];
+    unsigned long long *count[N_FOLD_TIERS];
 };
 hipError_t launch_active_bricks(const unsigned long long *cnt, uint64_t n_bricks, TierLists t,
                                 const unsigned long long *over, hipStream_t s);

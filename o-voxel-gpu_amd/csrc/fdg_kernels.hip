@@ -2114,11 +2114,14 @@ __global__ __launch_bounds__(BT) void k_place_chunks(ShardBuf<Chunk> chunks, con
     }
 }
 
-// fold tiers by records per brick (held in LDS by the tier's kernel)
-constexpr int FOLD_SMALL_CAP = 768; // first tier: blocks of FOLD_SMALL_NT threads (15 KB of LDS)
+// fold tiers by voxel entries per brick (held in LDS by the tier's kernel)
+constexpr int FOLD_TINY_CAP = 384;  // first tier: one wave per brick (10 KB of LDS, 16 per CU)
+constexpr int FOLD_TINY_NT = 64;
+constexpr int FOLD_TINY_WAVES = 4;  // waves per SIMD the first tier is compiled for
+constexpr int FOLD_SMALL_CAP = 768; // second tier: blocks of FOLD_SMALL_NT threads (15 KB of LDS)
 constexpr int FOLD_SMALL_NT = 128;
-constexpr int FOLD_SMALL = 1536; // second tier (25 KB of LDS, 5 blocks per CU)
-constexpr int FOLD_MID = 3584;   // third tier (3 blocks per CU); larger bricks: k_fold_large
+constexpr int FOLD_SMALL = 1536; // third tier (25 KB of LDS, 5 blocks per CU)
+constexpr int FOLD_MID = 3584;   // fourth tier (3 blocks per CU); larger bricks: k_fold_large
 
 // bricks that received records, each appended to the list of the fold tier
 // its record count selects: each block takes ACT_PER consecutive bricks per
@@ -2130,12 +2133,12 @@ __global__ __launch_bounds__(BT) void k_active_bricks(const unsigned long long *
     if (*over)
         return;
     __shared__ uint32_t sh[NWAVE];
-    __shared__ unsigned long long sbase[4];
+    __shared__ unsigned long long sbase[N_FOLD_TIERS];
     const uint64_t span = (uint64_t)BT * ACT_PER;
     for (uint64_t b0 = (uint64_t)blockIdx.x * span; b0 < n_bricks; b0 += (uint64_t)gridDim.x * span)
     {
         const uint64_t first = b0 + (uint64_t)threadIdx.x * ACT_PER;
-        uint32_t mine[4] = {0u, 0u, 0u, 0u};
+        uint32_t mine[N_FOLD_TIERS] = {0u, 0u, 0u, 0u, 0u};
 #pragma unroll
         for (int e = 0; e < ACT_PER; ++e)
         {
@@ -2143,14 +2146,15 @@ __global__ __launch_bounds__(BT) void k_active_bricks(const unsigned long long *
             if ((uint32_t)c == 0u)
                 continue;
             const uint32_t nr = (uint32_t)(c >> 32);
-            const int tier = nr <= (uint32_t)FOLD_SMALL_CAP ? 0
-                             : nr <= (uint32_t)FOLD_SMALL ? 1
-                             : nr <= (uint32_t)FOLD_MID   ? 2
-                                                          : 3;
+            const int tier = nr <= (uint32_t)FOLD_TINY_CAP    ? 0
+                             : nr <= (uint32_t)FOLD_SMALL_CAP ? 1
+                             : nr <= (uint32_t)FOLD_SMALL     ? 2
+                             : nr <= (uint32_t)FOLD_MID       ? 3
+                                                              : 4;
             mine[tier] |= 1u << e;
         }
 #pragma unroll
-        for (int tier = 0; tier < 4; ++tier)
+        for (int tier = 0; tier < N_FOLD_TIERS; ++tier)
         {
             uint32_t tot;
             const uint32_t ex = block_excl_scan((uint32_t)__popc(mine[tier]), sh, tot);
@@ -3271,10 +3275,12 @@ __device__ __forceinline__ void fold_bricks_body(const FoldArgs &ain, const uint
     }
 }
 
-// The kernels, one per tier of bricks by record count (a conversion whose
+// The kernels, one per tier of bricks by voxel entries (a conversion whose
 // record buffers overflowed skips the fold: the host sizes the buffers
 // again and re-runs its record stages):
-//  - k_fold_small: bricks of up to FOLD_SMALL_CAP records, in blocks of 128
+//  - k_fold_tiny: bricks of up to FOLD_TINY_CAP entries, one wave each
+//    (10 KB of LDS: 16 bricks per CU, and no barrier waits on another wave);
+//  - k_fold_small: bricks of up to FOLD_SMALL_CAP entries, in blocks of 128
 //    threads (5 waves per SIMD and 15 KB of LDS: 10 blocks per CU), so that
 //    a CU folds 10 bricks at a time;
 //  - k_fold_bricks(_ref): bricks of up to FOLD_SMALL records (25 KB of LDS:
@@ -3282,6 +3288,13 @@ __device__ __forceinline__ void fold_bricks_body(const FoldArgs &ain, const uint
 //  - k_fold_mid: bricks of up to FOLD_MID records (3 blocks per CU).
 // Each brick is listed for its tier by k_active_bricks; a brick too large
 // for a tier's tile (not expected: the counts are exact) goes to the next.
+template <int P>
+__global__ __launch_bounds__(FOLD_TINY_NT) __attribute__((amdgpu_waves_per_eu(FOLD_TINY_WAVES))) void k_fold_tiny(FoldArgs a)
+{
+    if (a.ctr[CTR_OVER])
+        return;
+    fold_bricks_body<P, FOLD_TINY_CAP, FOLD_TINY_NT>(a, a.tiny, *a.n_tiny, a.active, a.n_active, false);
+}
 template <int P>
 __global__ __launch_bounds__(FOLD_SMALL_NT) __attribute__((amdgpu_waves_per_eu(5))) void k_fold_small(FoldArgs a)
 {
@@ -3976,6 +3989,13 @@ hipError_t launch_active_bricks(const unsigned long long *cnt, uint64_t n_bricks
 
 hipError_t launch_fold_bricks(const FoldArgs &a, uint64_t max_bricks, hipStream_t s)
 {
+    uint64_t nt = max_bricks < 262144ull ? max_bricks : 262144ull;
+    if (nt == 0)
+        nt = 1;
+    if (a.prec)
+        hipLaunchKernelGGL(k_fold_tiny<1>, dim3((unsigned)nt), dim3(FOLD_TINY_NT), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_fold_tiny<0>, dim3((unsigned)nt), dim3(FOLD_TINY_NT), 0, s, a);
     uint64_t ns = max_bricks < 131072ull ? max_bricks : 131072ull;
     if (ns == 0)
         ns = 1;
