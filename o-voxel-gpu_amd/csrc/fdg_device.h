@@ -179,10 +179,14 @@ __device__ __forceinline__ void plane_quadric(const float p[4], float q[10])
     q[9] = p[3] * p[3];
 }
 
-// index of (i,j) in the Q10 layout
+// index of (i,j) in the Q10 layout.  Not recursive: a recursive form is
+// turned into a loop before the unrolled callers' constant indices reach it,
+// which leaves the index dynamic and puts the indexed quadric in scratch
+// memory.
 __host__ __device__ constexpr int qidx(int i, int j)
 {
-    return (i > j) ? qidx(j, i) : (i == 0 ? j : (i == 1 ? 3 + j : (i == 2 ? 5 + j : 9)));
+    const int a = i < j ? i : j, b = i < j ? j : i;
+    return a == 0 ? b : (a == 1 ? 3 + b : (a == 2 ? 5 + b : 9));
 }
 
 } // namespace ovx
