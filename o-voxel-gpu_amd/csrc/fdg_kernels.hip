@@ -1303,9 +1303,10 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(7))) void k_
                 const FaceItem &it = items[L];
                 const GridDev &g = QUEUED ? bt.seg[it.ms].g : gt;
                 const uint64_t R_L = incl_items[L] - (L ? incl_items[L - 1] : 0ull);
-                const uint64_t rr = (uint64_t)it.r0 + (gi - (incl_items[L] - R_L));
-                const int y = it.by0 + (int)(rr % (uint64_t)it.ny);
-                const int z = it.bz0 + (int)(rr / (uint64_t)it.ny);
+                // a bounding box has fewer than 2^26 rows: 32-bit division
+                const uint32_t rr = it.r0 + (uint32_t)(gi - (incl_items[L] - R_L));
+                const int y = it.by0 + (int)(rr % (uint32_t)it.ny);
+                const int z = it.bz0 + (int)(rr / (uint32_t)it.ny);
                 FaceRow &row = rows[tid];
                 row.item = L;
                 row.y = y;
@@ -1413,10 +1414,16 @@ __device__ __forceinline__ uint32_t face_walk(const FaceItem &it, const GridDev 
 {
     uint32_t n = 0;
     const RowNarrow nr = face_row_pre(it, g);
+    int ry = 0, rz = 0; // row rr = rz * ny + ry, stepped instead of divided
     for (uint64_t rr = 0; rr < n_rows; ++rr)
     {
-        const int y = it.by0 + (int)(rr % (uint64_t)it.ny);
-        const int z = it.bz0 + (int)(rr / (uint64_t)it.ny);
+        const int y = it.by0 + ry;
+        const int z = it.bz0 + rz;
+        if (++ry == it.ny)
+        {
+            ry = 0;
+            ++rz;
+        }
         int x0, x1;
         face_row_range(it, g, nr, y, z, x0, x1);
         const int lx0 = x0 - g.gmin[0], lx1 = x1 - g.gmin[0];
@@ -2266,10 +2273,11 @@ __global__ __launch_bounds__(BT) void k_emit_coords(BatchDev bt, uint64_t n_word
             if (m)
             {
                 id = g.woff[w];
-                const uint64_t row = w / (uint64_t)g.wx;
-                xb = g.gmin[0] + 64 * (int)(w - row * (uint64_t)g.wx);
-                y = (int)(row % (uint64_t)g.G[1]) + g.gmin[1];
-                z = (int)(row / (uint64_t)g.G[1]) + g.gmin[2];
+                // a mesh has fewer than 2^32 bitset words: 32-bit division
+                const uint32_t w32 = (uint32_t)w, row = w32 / (uint32_t)g.wx;
+                xb = g.gmin[0] + 64 * (int)(w32 - row * (uint32_t)g.wx);
+                y = (int)(row % (uint32_t)g.G[1]) + g.gmin[1];
+                z = (int)(row / (uint32_t)g.G[1]) + g.gmin[2];
             }
         }
         const uint32_t pc = (uint32_t)__popcll(m);
