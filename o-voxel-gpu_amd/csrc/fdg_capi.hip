@@ -117,7 +117,7 @@ struct ovx_fdg_ctx
     // face and boundary hits, and, once the records are made from them, the
     // per-voxel QEFs and the box-search queue in the same bytes
     DevBuf hitqef;
-    size_t off_bhits = 0, off_box = 0; // byte offsets inside hitqef
+    size_t off_bhits = 0, off_box = 0, off_qbits = 0; // byte offsets inside hitqef
     uint64_t per[SB_N] = {};       // per-shard capacities of the sharded buffers (grow-only)
     uint64_t force_per[SB_N] = {}; // one-shot overrides (tests, tuning)
     double face_hit_ratio = 1.0;          // face hits of the last conversion over the prep's estimate (capacity estimate)
@@ -1109,6 +1109,7 @@ ovx_status enqueue_finish(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t 
     // the hits are all made into records by now (the side stream is joined):
     // their bytes hold the QEFs and the box queue
     a.qef = (float *)c->hitqef.p;
+    a.solve_bits = c->prec ? (uint32_t *)((unsigned char *)c->hitqef.p + c->off_qbits) : nullptr;
     a.box = shard_view<BoxItem>(c, c->hitqef, SB_BOX, c->off_box);
     a.prec = c->prec;
     a.cond_max = c->cond_max;
@@ -1116,6 +1117,8 @@ ovx_status enqueue_finish(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t 
     a.ctr = ctr;
     // CTR_TINY, CTR_ACTIVE, CTR_MEDIUM, CTR_LARGE, CTR_LARGE_RECS and CTR_HUGE are zeroed with all counters by begin (an
     // attempt whose record buffers overflow leaves them at zero)
+    if (c->prec)
+        HIP_TRY(launch_fill_u32(a.solve_bits, ((uint64_t)c->N + 31) / 32, 0u, s)); // the hits' bytes before
     tic(c, KS_FOLD, s);
     HIP_TRY(launch_fold_bricks(a, n_bricks, s));
     toc(c, KS_FOLD, s);
@@ -1300,12 +1303,15 @@ ovx_status finish_fdg(ovx_fdg_ctx *c, int32_t *coords, float *dual, uint8_t *fla
             // variant (the reference path solves each voxel where it is folded and stores no QEF)
             const size_t fh = align_up(size_shard(c, SB_FHITS, c->fw > 0.0f ? want_fhit : 1, sizeof(FHit)), 256);
             const size_t bh = size_shard(c, SB_BHITS, 1, sizeof(BHit)); // counters only
+            // precision variant: quadrics of the voxels its folds leave unsolved, and their flags
             const size_t q = c->prec ? align_up((size_t)c->N * 80, 256) : 0;
+            const size_t qb = c->prec ? align_up(((size_t)c->N + 31) / 32 * 4, 256) : 0;
             const size_t bx = size_shard(c, SB_BOX, per_for((uint64_t)c->N / 8 + 4096), sizeof(BoxItem));
-            if ((st = ensure(c->hitqef, std::max(fh + bh, q + bx), false)))
+            if ((st = ensure(c->hitqef, std::max(fh + bh, q + qb + bx), false)))
                 return st;
             c->off_bhits = fh;
-            c->off_box = q;
+            c->off_qbits = q;
+            c->off_box = q + qb;
         }
         // first attempt through a graph when enabled (timing events stay outside graphs)
         if (c->graph_mode && !c->timing && attempt == 0)

@@ -306,7 +306,8 @@ struct FoldArgs
     const BEdge *bedges;
     float *dual;
     uint8_t *flags;
-    float *qef; // [N][10] folded, regularised quadric of each voxel ([N][20], h then l, precision variant)
+    float *qef; // precision variant: [N][20] scaled quadric (h then l) of the voxels its folds leave unsolved
+    uint32_t *solve_bits; // precision variant: [ceil(N / 32)] those voxels (cleared before the folds)
     float bw, rw;
     ShardBuf<BoxItem> box; // box-search queue
     int prec;              // 0 reference float path, 1 compensated voxel-frame accumulation
@@ -376,7 +377,8 @@ hipError_t launch_fold_bricks(const FoldArgs &a, uint64_t max_bricks, hipStream_
 hipError_t launch_fold_mid(const FoldArgs &a, uint64_t max_bricks, hipStream_t s);
 // bricks of the `huge` list: whole, by voxel ranges, or a giant voxel in key-ordered batches
 hipError_t launch_fold_large(const FoldArgs &a, uint64_t max_bricks, hipStream_t s);
-// dual vertices of the precision variant from its stored QEFs (the reference path solves in its folds)
+// dual vertices of the precision variant's voxels that its folds left unsolved (flagged in a.solve_bits, their
+// quadrics in a.qef): the FP64 fallback and the box search in the voxel frame
 hipError_t launch_solve_voxels(const FoldArgs &a, const int32_t *coords, uint64_t n, hipStream_t s);
 hipError_t launch_solve_box(const FoldArgs &a, hipStream_t s);
 hipError_t launch_edge_count(const MeshDev &m, uint32_t *vcnt, uint32_t *rank, const uint32_t *voff, uint32_t *nbr,

@@ -2700,21 +2700,61 @@ struct FoldStateC
         int ex = 0;
         if (dmax > 0.0f && dmax <= 3.0e38f)
             frexpf(dmax, &ex);
-        float *qo = a.qef + 20 * vid;
+        float qh[10], ql[10];
 #pragma unroll
         for (int i = 0; i < 10; ++i)
         {
-            qo[i] = ldexpf(q[i].h, 1 - ex);
-            qo[10 + i] = ldexpf(q[i].l, 1 - ex);
+            qh[i] = ldexpf(q[i].h, 1 - ex);
+            ql[i] = ldexpf(q[i].l, 1 - ex);
         }
         uint8_t *fo = a.flags + 3 * vid;
         fo[0] = (uint8_t)(fl & 1);
         fo[1] = (uint8_t)((fl >> 1) & 1);
         fo[2] = (uint8_t)((fl >> 2) & 1);
+        // the FP32 solve of k_solve_voxels_c, done here when the system is
+        // well conditioned and its minimiser lies in the voxel (the common
+        // case); otherwise the scaled quadric is stored and the voxel
+        // flagged for that kernel (FP64 fallback, box search)
+        float Q[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                Q[i][j] = qh[qidx(i, j)] + ql[qidx(i, j)];
+        const float A[3][3] = {{Q[0][0], Q[0][1], Q[0][2]}, {Q[1][0], Q[1][1], Q[1][2]}, {Q[2][0], Q[2][1], Q[2][2]}};
+        const float b[3] = {-Q[0][3], -Q[1][3], -Q[2][3]};
+        Cpqr<3> f;
+        f.factor(A);
+        const float r0 = fabsf(f.qr[0][0]), rl = fabsf(f.qr[2][2]);
+        bool solved = false;
+        if (!(f.rk < 3 || !(r0 <= a.cond_max * rl)))
+        {
+            float u[3];
+            f.solve(b, u);
+            if (u[0] >= 0.0f && u[0] <= 1.0f && u[1] >= 0.0f && u[1] <= 1.0f && u[2] >= 0.0f && u[2] <= 1.0f)
+            {
+                float *o = a.dual + 3 * vid;
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    o[i] = c[i] + a.g.vs[i] * u[i];
+                solved = true;
+            }
+        }
+        if (!solved)
+        {
+            float *qo = a.qef + 20 * vid;
+#pragma unroll
+            for (int i = 0; i < 10; ++i)
+            {
+                qo[i] = qh[i];
+                qo[10 + i] = ql[i];
+            }
+            atomicOr(a.solve_bits + (vid >> 5), 1u << (uint32_t)(vid & 31ull));
+        }
     }
 
-    // quadric (h and l parts) and flags of voxel (x, y, z) in output row vid;
-    // its dual vertex is solved by k_solve_voxels_c
+    // flags of voxel (x, y, z) in output row vid and its dual vertex, or its
+    // quadric for k_solve_voxels_c
     __device__ __forceinline__ void finish_out(uint64_t vid, int, int, int, const FoldArgs &a)
     {
         finish_at(vid, a);
@@ -3547,19 +3587,42 @@ __global__ __launch_bounds__(BT) void k_fold_large(FoldArgs ain)
 // compensated (h + l) quadric, in world coordinates like the reference.  A
 // minimiser outside [0,1]^3 goes through the box search (FP32, voxel frame).
 // The voxels that took the FP64 path are counted.
-__global__ __launch_bounds__(BT) void k_solve_voxels_c(const float *qef, const int32_t *coords, uint64_t n,
-                                                       BatchDev bt, float *dual, float cond_max,
-                                                       unsigned long long *n64, const unsigned long long *over)
+__global__ __launch_bounds__(BT) void k_solve_voxels_c(const float *qef, const uint32_t *solve_bits,
+                                                       const int32_t *coords, uint64_t n, BatchDev bt, float *dual,
+                                                       float cond_max, unsigned long long *n64,
+                                                       const unsigned long long *over)
 {
     if (*over)
         return;
-    const uint64_t stride = (uint64_t)gridDim.x * BT;
     const int lane = threadIdx.x & 63;
     unsigned long long *cnt64 = n64 + (uint64_t)((blockIdx.x * NWAVE + (threadIdx.x >> 6)) % NSHARD) * SHARD_STRIDE;
-    const uint64_t n_round = (n + 63ull) & ~63ull;
-    for (uint64_t vid = (uint64_t)blockIdx.x * BT + threadIdx.x; vid < n_round; vid += stride)
+    // each wave takes 64 consecutive words of the flags (2048 voxels) and its
+    // lanes take the flagged voxels among them one each, in rounds of 64
+    const uint64_t n_words = (n + 31ull) / 32ull;
+    const uint64_t wave = ((uint64_t)blockIdx.x * BT + (uint64_t)threadIdx.x) >> 6;
+    const uint64_t n_waves = ((uint64_t)gridDim.x * BT) >> 6;
+    for (uint64_t w0 = wave * 64ull; w0 < n_words; w0 += n_waves * 64ull)
+    for (uint32_t k0 = 0, total = 1; k0 < total; k0 += 64u)
     {
-        const bool live = vid < n;
+        const uint64_t wi = w0 + (uint64_t)lane;
+        const uint32_t word = wi < n_words ? solve_bits[wi] : 0u;
+        const uint32_t pc = (uint32_t)__popc(word);
+        const uint32_t incl = wave_incl_scan(pc);
+        total = __shfl(incl, 63, 64);
+        const uint32_t k = k0 + (uint32_t)lane;
+        // the word of flagged voxel k: lane j = number of lanes with incl <= k
+        int jw = 0;
+#pragma unroll
+        for (int step = 32; step >= 1; step >>= 1)
+        {
+            const uint32_t v = __shfl(incl, jw + step - 1, 64);
+            jw += (v <= k) ? step : 0;
+        }
+        jw = jw < 63 ? jw : 63;
+        const uint32_t mj = __shfl(word, jw, 64);
+        const uint32_t ej = __shfl(incl - pc, jw, 64);
+        const bool live = k < total;
+        const uint64_t vid = live ? (w0 + (uint64_t)jw) * 32ull + (uint64_t)select_bit((uint64_t)mj, k - ej) : 0ull;
         bool used64 = false;
         if (live)
         {
@@ -4036,8 +4099,9 @@ hipError_t launch_fold_mid(const FoldArgs &a, uint64_t max_bricks, hipStream_t s
 hipError_t launch_solve_voxels(const FoldArgs &a, const int32_t *coords, uint64_t n, hipStream_t s)
 {
     // precision variant only: the reference path's folds solve each voxel where it is folded
-    hipLaunchKernelGGL(k_solve_voxels_c, dim3(grid_for((int64_t)n, 65536)), dim3(BT), 0, s, (const float *)a.qef,
-                       coords, n, a.b, a.dual, a.cond_max, a.n64, (const unsigned long long *)(a.ctr + CTR_OVER));
+    hipLaunchKernelGGL(k_solve_voxels_c, dim3(grid_for((int64_t)((n + 31) / 32), 4096)), dim3(BT), 0, s,
+                       (const float *)a.qef, (const uint32_t *)a.solve_bits, coords, n, a.b, a.dual, a.cond_max, a.n64,
+                       (const unsigned long long *)(a.ctr + CTR_OVER));
     return hipGetLastError();
 }
 
