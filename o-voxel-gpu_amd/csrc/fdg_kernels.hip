@@ -2519,10 +2519,11 @@ struct FoldState
 // ---------------------------------------------------------------------------
 // precision variant: the quadric of each voxel is accumulated in the voxel's
 // own frame u = (p - c) / vs (c = the voxel's min corner), where its entries
-// are O(vs^2) instead of the O(1) - O(1) cancellations of world coordinates,
-// with compensated FP32 sums (TwoSum) of error-free products (TwoProduct);
-// the world problem min_p [p,1]^T Q [p,1] over the voxel is the same problem
-// over u in [0,1]^3.
+// are O(vs^2) instead of the O(1) - O(1) cancellations of world coordinates:
+// FP64 sums of the exact (FP64) products of FP32 plane coefficients, whose
+// offsets n.c + d, like the intersection means, are compensated FP32 sums
+// (TwoSum) of error-free products (TwoProduct); the world problem
+// min_p [p,1]^T Q [p,1] over the voxel is the same problem over u in [0,1]^3.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void two_sum(float a, float b, float &s, float &e)
 {
@@ -2564,9 +2565,23 @@ struct CSum
     }
 };
 
+// FP64 accumulator of a quadric entry: products of two FP32 values are exact
+// in FP64 (48 significant bits); read back as an unevaluated FP32 pair (h, l).
+// Half the instructions of a compensated FP32 sum of TwoProducts.
+struct DSum
+{
+    double v;
+    __device__ __forceinline__ void zero() { v = 0.0; }
+    __device__ __forceinline__ void add(float x) { v = v + (double)x; }
+    __device__ __forceinline__ void add_prod(float a, float b) { v = v + (double)a * (double)b; }
+    __device__ __forceinline__ void add_double(double x) { v = v + x; }
+    __device__ __forceinline__ float hi() const { return (float)v; }
+    __device__ __forceinline__ float lo() const { return (float)(v - (double)(float)v); }
+};
+
 struct FoldStateC
 {
-    CSum q[10];
+    DSum q[10];
     CSum m[3]; // sum of the intersection points in the voxel frame
     float cnt;
     int fl;
@@ -2696,7 +2711,7 @@ struct FoldStateC
                 q[qidx(3, 3)].add_prod(te, mu);
             }
         }
-        const float dmax = fmaxf(fmaxf(q[qidx(0, 0)].h, q[qidx(1, 1)].h), q[qidx(2, 2)].h);
+        const float dmax = fmaxf(fmaxf(q[qidx(0, 0)].hi(), q[qidx(1, 1)].hi()), q[qidx(2, 2)].hi());
         int ex = 0;
         if (dmax > 0.0f && dmax <= 3.0e38f)
             frexpf(dmax, &ex);
@@ -2704,8 +2719,8 @@ struct FoldStateC
 #pragma unroll
         for (int i = 0; i < 10; ++i)
         {
-            qh[i] = ldexpf(q[i].h, 1 - ex);
-            ql[i] = ldexpf(q[i].l, 1 - ex);
+            qh[i] = ldexpf(q[i].hi(), 1 - ex);
+            ql[i] = ldexpf(q[i].lo(), 1 - ex);
         }
         uint8_t *fo = a.flags + 3 * vid;
         fo[0] = (uint8_t)(fl & 1);
