@@ -279,6 +279,8 @@ def _batch_prepare(vertices: List[torch.Tensor], faces: List[torch.Tensor], orig
     M = len(vertices)
     if len(faces) != M:
         raise RuntimeError("vertices and faces must have one entry per mesh")
+    if M == 0:
+        raise RuntimeError("a batch needs at least one mesh")
     dev = vertices[0].device
     for k in range(M):
         _require_device_tensor(vertices[k], f"vertices[{k}]", torch.float32, 2)

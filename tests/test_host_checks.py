@@ -53,3 +53,19 @@ def test_hashmap_table_must_be_on_device():
     keys = torch.arange(4, dtype=torch.int64).to(torch.uint32)
     with pytest.raises(RuntimeError, match="device tensor"):
         _C().hashmap_insert_cuda(table_k, table_v, keys, keys.clone())
+
+
+def test_input_preparation_rejects_bad_arguments():
+    """The wrapper's preparation entry points check their tensors on the host:
+    host tensors, wrong dtypes, an empty batch."""
+    v = torch.zeros((4, 3), dtype=torch.float32)
+    f = torch.zeros((2, 3), dtype=torch.int64)
+    o = torch.zeros(3, dtype=torch.float32)
+    with pytest.raises(RuntimeError):
+        _C()._prepare_mesh(v, f, o)  # host tensors
+    with pytest.raises(RuntimeError, match="at least one mesh"):
+        _C()._batch_prepare([], [])
+    with pytest.raises(RuntimeError, match="one entry per mesh"):
+        _C()._batch_prepare([v], [])
+    with pytest.raises(RuntimeError):
+        _C()._batch_prepare([v], [f])  # host tensors
