@@ -3242,7 +3242,7 @@ __device__ __forceinline__ void fold_bricks_body(const FoldArgs &ain, const uint
     constexpr int PER = CAP / NT;
     constexpr int NW = NT / 64;
     static_assert(CAP % NT == 0, "record slots per thread");
-    static_assert(NT >= 64 && NT <= BT && 512 % NT == 0, "block size");
+    static_assert(NT >= 64 && NT <= 512 && 512 % NT == 0, "block size");
     const int tid = threadIdx.x;
     for (uint64_t bi = blockIdx.x; bi < n_list; bi += gridDim.x)
     {
@@ -3348,7 +3348,7 @@ __device__ __forceinline__ void fold_bricks_body(const FoldArgs &ain, const uint
 //    a CU folds 10 bricks at a time;
 //  - k_fold_bricks(_ref): bricks of up to FOLD_SMALL records (25 KB of LDS:
 //    5 blocks of 256 threads per CU);
-//  - k_fold_mid: bricks of up to FOLD_MID records (3 blocks per CU).
+//  - k_fold_mid(_ref): bricks of up to FOLD_MID records (3 blocks per CU).
 // Each brick is listed for its tier by k_active_bricks; a brick too large
 // for a tier's tile (not expected: the counts are exact) goes to the next.
 template <int P>
@@ -3384,6 +3384,16 @@ __global__ __launch_bounds__(BT) void k_fold_mid(FoldArgs a)
     if (a.ctr[CTR_OVER])
         return;
     fold_bricks_body<P, FOLD_MID, BT>(a, a.large, *a.n_large, a.huge, a.n_huge, true);
+}
+// the reference path's mid tier in blocks of FOLD_MID_REF_NT threads: the
+// same 3 bricks per CU (LDS), twice the waves to hide their loads, and every
+// voxel of a brick replayed in one round
+constexpr int FOLD_MID_REF_NT = 512;
+__global__ __launch_bounds__(FOLD_MID_REF_NT) __attribute__((amdgpu_waves_per_eu(6))) void k_fold_mid_ref(FoldArgs a)
+{
+    if (a.ctr[CTR_OVER])
+        return;
+    fold_bricks_body<0, FOLD_MID, FOLD_MID_REF_NT>(a, a.large, *a.n_large, a.huge, a.n_huge, true);
 }
 
 // Bricks with more records than k_fold_bricks holds.  A brick
@@ -4107,7 +4117,7 @@ hipError_t launch_fold_mid(const FoldArgs &a, uint64_t max_bricks, hipStream_t s
     if (a.prec)
         hipLaunchKernelGGL(k_fold_mid<1>, dim3((unsigned)nb), dim3(BT), 0, s, a);
     else
-        hipLaunchKernelGGL(k_fold_mid<0>, dim3((unsigned)nb), dim3(BT), 0, s, a);
+        hipLaunchKernelGGL(k_fold_mid_ref, dim3((unsigned)nb), dim3(FOLD_MID_REF_NT), 0, s, a);
     return hipGetLastError();
 }
 
