@@ -2787,7 +2787,7 @@ struct FoldPick<1>
     using type = FoldStateC;
 };
 
-constexpr int FOLD_WAVES = 5;    // occupancy of the reference-path main fold (waves per SIMD: at most 102 VGPRs)
+constexpr int FOLD_WAVES = 6;    // occupancy of the reference-path 256-thread fold (waves per SIMD: at most 85 VGPRs; 6 blocks per CU, its LDS limit)
 constexpr int FOLD_CAP = 4096;   // records held in LDS at a time by the kernel for large bricks (2 blocks per CU)
 
 // LDS view of a batch of up to NT chunks of one brick (NT = threads of the block)
@@ -3347,7 +3347,7 @@ __device__ __forceinline__ void fold_bricks_body(const FoldArgs &ain, const uint
 //    threads (5 waves per SIMD and 15 KB of LDS: 10 blocks per CU), so that
 //    a CU folds 10 bricks at a time;
 //  - k_fold_bricks(_ref): bricks of up to FOLD_SMALL records (25 KB of LDS:
-//    5 blocks of 256 threads per CU);
+//    6 blocks of 256 threads per CU on the reference path, 5 otherwise);
 //  - k_fold_mid(_ref): bricks of up to FOLD_MID records (3 blocks per CU).
 // Each brick is listed for its tier by k_active_bricks; a brick too large
 // for a tier's tile (not expected: the counts are exact) goes to the next.
