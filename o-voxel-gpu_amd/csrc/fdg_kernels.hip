@@ -3068,14 +3068,33 @@ __device__ __forceinline__ void brick_origin(const FoldArgs &a, uint32_t brick, 
 template <int CAP, int NT = BT>
 struct FoldLdsG
 {
-    // 12 B per record and 16-bit offsets: 25 KB at 1536 records (6 blocks
-    // of 256 threads per CU), 15 KB at 768 (10 blocks of 128 threads)
+    // 12 B per record and 16-bit offsets and counters: 24 KB at 1536
+    // records (6 blocks of 256 threads per CU), 14 KB at 768 (11 blocks of
+    // 128 threads)
     static_assert(CAP <= 65535, "16-bit record offsets");
     uint64_t key[CAP];
     float z[CAP];
     uint16_t off[513];   // first record of each voxel
-    uint32_t cur[512];   // per-voxel record counts, then fill cursors
+    // per-voxel record counts, then fill cursors: in blocks of at most 256
+    // threads 16 bits each (voxel l in half l & 1 of word l >> 1), 1 KB less
+    // LDS; 32 bits in larger blocks, whose threads own one voxel each
+    static constexpr bool PACK = NT <= 256;
+    static constexpr int CUR_WORDS = PACK ? 256 : 512;
+    uint32_t cur[CUR_WORDS];
     uint16_t vlist[512]; // voxels with records
+    __device__ __forceinline__ uint32_t cnt_get(int l) const
+    {
+        return PACK ? (cur[l >> 1] >> ((l & 1) * 16)) & 0xffffu : cur[l];
+    }
+    // adds one to voxel l's counter and returns its previous value (the
+    // counters stay below CAP < 2^16, so no carry reaches the other half)
+    __device__ __forceinline__ uint32_t cnt_inc(uint32_t l)
+    {
+        if constexpr (PACK)
+            return (atomicAdd(&cur[l >> 1], 1u << ((l & 1u) * 16u)) >> ((l & 1u) * 16u)) & 0xffffu;
+        else
+            return atomicAdd(&cur[l], 1u);
+    }
     uint64_t rbits[64];  // occupancy word of each (y, z) row of the brick
     uint32_t roff[64];   // ... and its first voxel id
     uint32_t sh[NT / 64];
@@ -3104,19 +3123,34 @@ __device__ __forceinline__ bool fold_grouped(const FoldArgs &a, FoldLdsG<CAP, NT
 #pragma unroll
         for (int j = 0; j < VPT; ++j)
         {
-            c[j] = L.cur[VPT * tid + j];
+            c[j] = L.cnt_get(VPT * tid + j);
             sum += c[j];
             nz += c[j] ? 1u : 0u;
         }
         uint32_t ex = block_excl_scan_w<NW>(sum, L.sh, tot);
         if (tot > (uint32_t)CAP)
             return false; // block-uniform
+        uint32_t exj[VPT];
 #pragma unroll
         for (int j = 0; j < VPT; ++j)
         {
             L.off[VPT * tid + j] = (uint16_t)ex;
-            L.cur[VPT * tid + j] = ex;
+            exj[j] = ex;
             ex += c[j];
+        }
+        // the fill cursors start at the segment offsets
+        if constexpr (FoldLdsG<CAP, NT>::PACK)
+        {
+            static_assert(VPT % 2 == 0, "whole counter words per thread");
+#pragma unroll
+            for (int j = 0; j < VPT; j += 2)
+                L.cur[(VPT * tid + j) >> 1] = exj[j] | (exj[j + 1] << 16);
+        }
+        else
+        {
+#pragma unroll
+            for (int j = 0; j < VPT; ++j)
+                L.cur[VPT * tid + j] = exj[j];
         }
         if (tid == 0)
             L.off[512] = (uint16_t)tot;
@@ -3142,7 +3176,7 @@ __device__ __forceinline__ bool fold_grouped(const FoldArgs &a, FoldLdsG<CAP, NT
             for (int q = 0; q < 4; ++q)
                 if (em & (1u << q))
                 {
-                    const uint32_t u = atomicAdd(&L.cur[lv[q]], 1u);
+                    const uint32_t u = L.cnt_inc(lv[q]);
                     L.key[u] = rk[k] | (uint64_t)q;
                     L.z[u] = rz[k];
                 }
@@ -3253,7 +3287,7 @@ __device__ __forceinline__ void fold_bricks_body(const FoldArgs &ain, const uint
         int x0, y0, z0;
         brick_origin(a, lb, x0, y0, z0);
         load_rows(a, L, x0, y0, z0);
-        for (int i = tid; i < 512; i += NT)
+        for (int i = tid; i < FoldLdsG<CAP, NT>::CUR_WORDS; i += NT)
             L.cur[i] = 0;
         // the brick's chunks, NT at a time: record g of the brick goes to
         // slot g / NT of thread g % NT
@@ -3311,7 +3345,7 @@ __device__ __forceinline__ void fold_bricks_body(const FoldArgs &ain, const uint
 #pragma unroll
                     for (int q = 0; q < 4; ++q)
                         if (em & (1u << q))
-                            atomicAdd(&L.cur[lv[q]], 1u);
+                            L.cnt_inc(lv[q]);
                 }
             }
             __syncthreads();
