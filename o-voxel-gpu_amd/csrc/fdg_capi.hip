@@ -111,7 +111,7 @@ struct ovx_fdg_ctx
     uint64_t svo_total = 0;
     uint8_t *svo_bytes = nullptr; // the octree written by the last encode finish
     int svo_state = 0;            // 1 begun, 2 written
-    DevBuf vcnt, voff, nbr, erank, bcnt, boff, bedge_v, bedges, eslots;
+    DevBuf vcnt, voff, nbr, erank, bcnt, boff, bedge_v, bedges, ebins, ehe;
     bool edge_csr = false; // the last conversion's boundary edges came from the CSR lists (a vertex overflowed its slots)
     DevBuf events, recs, chunks, heads, choff, bchunks, tiny, active, medium, large, huge, shard_ctr, rowq, faceq;
     // face and boundary hits, and, once the records are made from them, the
@@ -194,7 +194,7 @@ struct ovx_fdg_ctx
     template <class Fn>
     void for_each_buf(Fn &&fn)
     {
-        DevBuf *all[] = {&planes, &tris, &bits,   &woff,  &tiles,   &vcnt,   &voff,  &nbr, &erank, &bcnt, &boff, &eslots,
+        DevBuf *all[] = {&planes, &tris, &bits,   &woff,  &tiles,   &vcnt,   &voff,  &nbr, &erank, &bcnt, &boff, &ebins, &ehe,
                          &bedge_v, &bedges, &events, &recs,   &chunks, &heads, &choff, &bchunks, &tiny, &active, &medium, &large, &huge,
                          &hitqef,  &shard_ctr, &rowq, &faceq, &segbuf,
                          &dm_table, &dm_cnt, &dm_off, &dm_mask, &dm_tiles, &dm_direct, &dm_touched,
@@ -449,6 +449,18 @@ void toc(ovx_fdg_ctx *c, int k, hipStream_t s)
     }
 }
 
+// the boundary-edge bins' arrays (n_bins + 1 entries each, in c->ebins) and
+// the half-edges binned by smaller vertex (c->ehe)
+struct EdgeBins
+{
+    uint32_t *cnt, *off, *bcnt, *boff;
+    uint2 *he;
+};
+EdgeBins edge_bins(ovx_fdg_ctx *c, uint64_t n_bins)
+{
+    auto *p = (uint32_t *)c->ebins.p;
+    return EdgeBins{p, p + (n_bins + 1), p + 2 * (n_bins + 1), p + 3 * (n_bins + 1), (uint2 *)c->ehe.p};
+}
 
 // ShardBuf view of a buffer of `per` items per shard
 template <typename T>
@@ -797,18 +809,19 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m_in, int M, const int64_
     // boundary edges (edge multiset of FDG.cpp:728-739, kept in std::map order)
     // depend on the faces only: they run on the side stream, beside the
     // scanline and compaction stages, and join before the readback.
-    // Half-edges are bucketed by their smaller vertex (fixed slots per
-    // vertex), each vertex's list sorted in registers, singletons counted; a
-    // vertex with more half-edges than slots re-runs the CSR lists after the
-    // readback.
+    // Half-edges are binned by their smaller vertex (bins of EBIN_VB
+    // vertices), each bin's half-edges bucketed by vertex in LDS, each
+    // vertex's list sorted in registers, singletons kept; a vertex with more
+    // than EDGE_SLOTS half-edges re-runs the CSR lists after the readback.
     c->E = 0;
     bool edges_forked = false;
+    const uint64_t n_ebins = ((uint64_t)V + EBIN_VB - 1) / EBIN_VB;
     if (planes && c->bw > 0.0f)
     {
         if ((uint64_t)m.F * 3ull >= (1ull << 32))
             return fail(OVX_ERR_CAPACITY, "more than 2^32 half-edges");
-        if ((st = ensure(c->vcnt, (size_t)(V + 1) * 4)) || (st = ensure(c->eslots, (size_t)V * EDGE_SLOTS * 4 + 16)) ||
-            (st = ensure(c->bcnt, (size_t)(V + 1) * 4)) || (st = ensure(c->boff, (size_t)(V + 1) * 4)) ||
+        if ((st = ensure(c->ebins, (size_t)(n_ebins + 1) * 4 * 4)) ||
+            (st = ensure(c->ehe, (size_t)m.F * 3 * sizeof(uint2) + 16)) ||
             (st = ensure(c->tiles, (scan_tile_count(n_words) + 2 * scan_tile_count((uint64_t)V + 1) +
                                     scan_tile_count((uint64_t)m.F * 3 + 1) + 64) * 8)))
             return st;
@@ -856,17 +869,17 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m_in, int M, const int64_
     {
         const hipStream_t es = c->side;
         HIP_TRY(hipStreamWaitEvent(es, c->ev_fork, 0));
-        auto *vcnt = (uint32_t *)c->vcnt.p;
-        tic(c, KS_EDGES, es);
-        HIP_TRY(hipMemsetAsync(vcnt, 0, (size_t)(V + 1) * 4, es));
-        HIP_TRY(hipMemsetAsync(ctr + CTR_EDGE_OVER, 0, sizeof(unsigned long long), es));
-        HIP_TRY(launch_edge_slots(m, vcnt, (uint32_t *)c->eslots.p, es));
-        HIP_TRY(hipMemsetAsync(c->bcnt.p, 0, (size_t)(V + 1) * 4, es));
-        HIP_TRY(launch_edge_singletons_slots(V, vcnt, (const uint32_t *)c->eslots.p, (uint32_t *)c->bcnt.p, nullptr,
-                                             nullptr, 0, ctr + CTR_EDGE_OVER, es));
+        EdgeBins eb = edge_bins(c, n_ebins);
         // its scan tiles sit after the word scan's
-        HIP_TRY(launch_scan_u32((uint32_t *)c->bcnt.p, (uint64_t)V + 1, (uint32_t *)c->boff.p,
-                                (uint64_t *)c->tiles.p + scan_tile_count(n_words) + 16, ctr + CTR_BEDGE_TOTAL, es));
+        uint64_t *etiles = (uint64_t *)c->tiles.p + scan_tile_count(n_words) + 16;
+        tic(c, KS_EDGES, es);
+        HIP_TRY(hipMemsetAsync(c->ebins.p, 0, (size_t)(n_ebins + 1) * 4 * 4, es));
+        HIP_TRY(hipMemsetAsync(ctr + CTR_EDGE_OVER, 0, sizeof(unsigned long long), es));
+        HIP_TRY(launch_ebin(m, false, eb.cnt, eb.off, eb.he, es));
+        HIP_TRY(launch_scan_u32(eb.cnt, n_ebins + 1, eb.off, etiles, ctr + CTR_EDGE_TOTAL, es));
+        HIP_TRY(launch_ebin(m, true, eb.cnt, eb.off, eb.he, es));
+        HIP_TRY(launch_ebin_fold(n_ebins, eb.off, eb.he, eb.bcnt, ctr + CTR_EDGE_OVER, es));
+        HIP_TRY(launch_scan_u32(eb.bcnt, n_ebins + 1, eb.boff, etiles, ctr + CTR_BEDGE_TOTAL, es));
         toc(c, KS_EDGES, es);
         HIP_TRY(hipEventRecord(c->ev_join, es));
     }
@@ -910,8 +923,9 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m_in, int M, const int64_
         {
             // a vertex has more half-edges than slots: per-vertex lists of any
             // length (count, scan, place, sort in place), read back again
-            if ((st = ensure(c->voff, (size_t)(V + 1) * 4)) || (st = ensure(c->nbr, (size_t)m.F * 3 * 4)) ||
-                (st = ensure(c->erank, (size_t)m.F * 3 * 4)))
+            if ((st = ensure(c->vcnt, (size_t)(V + 1) * 4)) || (st = ensure(c->bcnt, (size_t)(V + 1) * 4)) ||
+                (st = ensure(c->boff, (size_t)(V + 1) * 4)) || (st = ensure(c->voff, (size_t)(V + 1) * 4)) ||
+                (st = ensure(c->nbr, (size_t)m.F * 3 * 4)) || (st = ensure(c->erank, (size_t)m.F * 3 * 4)))
                 return st;
             c->edge_csr = true;
             auto *vcnt = (uint32_t *)c->vcnt.p;
@@ -943,9 +957,10 @@ ovx_status begin_common(ovx_fdg_ctx *c, const MeshDev &m_in, int M, const int64_
                                                (uint32_t *)c->bcnt.p, (const uint32_t *)c->boff.p,
                                                (int2 *)c->bedge_v.p, 1, s));
             else
-                HIP_TRY(launch_edge_singletons_slots(V, (const uint32_t *)c->vcnt.p, (const uint32_t *)c->eslots.p,
-                                                     (uint32_t *)c->bcnt.p, (const uint32_t *)c->boff.p,
-                                                     (int2 *)c->bedge_v.p, 1, nullptr, s));
+            {
+                const EdgeBins eb = edge_bins(c, n_ebins);
+                HIP_TRY(launch_ebin_gather(n_ebins, eb.off, eb.boff, eb.he, (int2 *)c->bedge_v.p, s));
+            }
             HIP_TRY(launch_bedge_prep(m.v, c->E, (const int2 *)c->bedge_v.p, (BEdge *)c->bedges.p, s));
         }
     }

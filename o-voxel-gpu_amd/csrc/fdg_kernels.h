@@ -383,16 +383,26 @@ hipError_t launch_solve_voxels(const FoldArgs &a, const int32_t *coords, uint64_
 hipError_t launch_solve_box(const FoldArgs &a, hipStream_t s);
 hipError_t launch_edge_count(const MeshDev &m, uint32_t *vcnt, uint32_t *rank, const uint32_t *voff, uint32_t *nbr,
                              int pass, hipStream_t s);
-// Boundary edges through fixed per-vertex slots: each half-edge's larger
-// vertex goes to a slot of its smaller vertex (EDGE_SLOTS per vertex, count
-// in vcnt); a vertex with more sets `over` and the caller re-runs the CSR
-// lists below.  Singletons: pass 0 counts each vertex's boundary edges into
-// bcnt, pass 1 writes them at boff in (min, max) order.
+// Boundary edges by bins of EBIN_VB consecutive vertices (a half-edge goes
+// to the bin of its smaller vertex).  bin_cnt, bin_off, bin_bcnt and bin_boff
+// hold n_bins + 1 entries each, bin_cnt and bin_bcnt zeroed first; he holds
+// every half-edge (3 per face).  Order on one stream:
+//   launch_ebin(place = false)          half-edges per bin into bin_cnt
+//   scan bin_cnt -> bin_off              each bin's range of he
+//   launch_ebin(place = true)           the half-edges into their ranges (bin_cnt drops to 0)
+//   launch_ebin_fold                     each bin's singletons at the start of its range, counts in bin_bcnt;
+//                                        a vertex with more than EDGE_SLOTS half-edges sets `over` (the
+//                                        caller then re-runs the CSR lists below)
+//   scan bin_bcnt -> bin_boff            the boundary-edge total
+//   launch_ebin_gather                   the boundary edges in (min, max) order into bedge_v
 constexpr int EDGE_SLOTS = 16;
-hipError_t launch_edge_slots(const MeshDev &m, uint32_t *vcnt, uint32_t *slots, hipStream_t s);
-hipError_t launch_edge_singletons_slots(int64_t V, const uint32_t *vcnt, const uint32_t *slots, uint32_t *bcnt,
-                                        const uint32_t *boff, int2 *bedge_v, int pass, unsigned long long *over,
-                                        hipStream_t s);
+constexpr int EBIN_VB = 512;
+hipError_t launch_ebin(const MeshDev &m, bool place, uint32_t *bin_cnt, const uint32_t *bin_off, uint2 *he,
+                       hipStream_t s);
+hipError_t launch_ebin_fold(uint64_t n_bins, const uint32_t *bin_off, uint2 *he, uint32_t *bin_bcnt,
+                            unsigned long long *over, hipStream_t s);
+hipError_t launch_ebin_gather(uint64_t n_bins, const uint32_t *bin_off, const uint32_t *bin_boff, const uint2 *he,
+                              int2 *bedge_v, hipStream_t s);
 hipError_t launch_edge_singletons(int64_t V, const uint32_t *voff, uint32_t *nbr, uint32_t *bcnt, const uint32_t *boff,
                                   int2 *bedge_v, int pass, hipStream_t s);
 hipError_t launch_bedge_prep(const float *verts, int64_t E, const int2 *bedge_v, BEdge *be, hipStream_t s);

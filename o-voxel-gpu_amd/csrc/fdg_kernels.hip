@@ -1651,31 +1651,142 @@ __global__ __launch_bounds__(BT) void k_edge_count(MeshDev m, uint32_t *vcnt, ui
     }
 }
 
-// one thread per face: the larger vertex of each of its 3 half-edges goes to
-// a slot of the smaller one
-__global__ __launch_bounds__(BT) void k_edge_slots(MeshDev m, uint32_t *vcnt, uint32_t *slots)
+// Boundary edges by vertex bins: half-edges are binned by their smaller
+// vertex, EBIN_VB consecutive vertices per bin, so that no half-edge takes a
+// returning global atomic of its own.
+//   k_ebin<false>  a block of faces counts its half-edges per bin in an LDS
+//                  table and adds each bin's count once;
+//   (scan of the bin counts: each bin's range of the half-edge array)
+//   k_ebin<true>   the same blocks claim one range per bin and store their
+//                  half-edges (smaller, larger vertex) there;
+//   k_ebin_fold    one block per bin buckets the bin's half-edges by vertex
+//                  in LDS (at most EDGE_SLOTS per vertex), sorts each vertex's
+//                  list in registers and stores the singletons, vertex by
+//                  vertex, over the bin's own half-edges;
+//   (scan of the bins' singleton counts: the boundary-edge total)
+//   k_ebin_gather  concatenates the bins' singletons: the boundary edges in
+//                  (min, max) order.
+constexpr int EBIN_LOG_VB = 9;
+static_assert((1 << EBIN_LOG_VB) == EBIN_VB, "bin width");
+static_assert(EBIN_VB == 2 * BT, "a bin's vertices are sorted two per thread");
+constexpr int EBIN_FPT = 2;       // faces per thread of the binning kernels
+constexpr int EBIN_LOG_HASH = 11; // LDS table slots of a binning block
+constexpr int EBIN_HASH = 1 << EBIN_LOG_HASH;
+static_assert(EBIN_HASH >= 3 * BT * EBIN_FPT, "every bin of a block's half-edges finds a table slot");
+
+template <bool PLACE>
+__global__ __launch_bounds__(BT) void k_ebin(MeshDev m, uint32_t *bin_cnt, const uint32_t *bin_off, uint2 *he)
 {
-    const int64_t stride = (int64_t)gridDim.x * BT;
-    for (int64_t f = (int64_t)blockIdx.x * BT + threadIdx.x; f < m.F; f += stride)
+    constexpr int K = 3 * EBIN_FPT;
+    __shared__ uint32_t key[EBIN_HASH];
+    __shared__ uint32_t cnt[EBIN_HASH]; // half-edges per bin; once claimed (PLACE), the block's first position
+    for (int i = threadIdx.x; i < EBIN_HASH; i += BT)
     {
-        const int32_t *fi = m.f + 3 * f;
-        const int32_t v[3] = {fi[0], fi[1], fi[2]};
-        if (!face_ok(m, v[0], v[1], v[2]))
-            continue; // flagged by the prep kernel
-        int32_t a[3], b[3];
-        uint32_t r[3];
+        key[i] = 0xffffffffu;
+        cnt[i] = 0;
+    }
+    __syncthreads();
+    const int64_t fbase = (int64_t)blockIdx.x * (BT * EBIN_FPT) + threadIdx.x;
+    int32_t ea[K], eb[K];
+    bool ok[K];
+#pragma unroll
+    for (int j = 0; j < EBIN_FPT; ++j)
+    {
+        const int64_t f = fbase + (int64_t)j * BT;
+        int32_t v[3] = {0, 0, 0};
+        bool live = f < m.F;
+        if (live)
+        {
+            const int32_t *fi = m.f + 3 * f;
+            v[0] = fi[0];
+            v[1] = fi[1];
+            v[2] = fi[2];
+            live = face_ok(m, v[0], v[1], v[2]); // else flagged by the prep kernel
+        }
 #pragma unroll
         for (int k = 0; k < 3; ++k)
         {
             const int32_t p = v[k], q = v[k == 2 ? 0 : k + 1];
-            a[k] = p < q ? p : q;
-            b[k] = p < q ? q : p;
-            r[k] = atomicAdd(vcnt + a[k], 1u);
+            ea[3 * j + k] = p < q ? p : q;
+            eb[3 * j + k] = p < q ? q : p;
+            ok[3 * j + k] = live;
         }
+    }
+    // a thread's half-edges of one bin (a face's mostly share one) take one
+    // table lookup and one count update together
+    uint32_t slot[K], rank[K];
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
-            if (r[k] < (uint32_t)EDGE_SLOTS)
-                slots[(uint64_t)a[k] * EDGE_SLOTS + r[k]] = (uint32_t)b[k];
+    for (int k = 0; k < K; ++k)
+    {
+        slot[k] = 0;
+        rank[k] = 0;
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+    {
+        if (!ok[k])
+            continue;
+        const uint32_t bin = (uint32_t)ea[k] >> EBIN_LOG_VB;
+        bool led = false;
+#pragma unroll
+        for (int j = 0; j < k; ++j)
+            led = led || (ok[j] && ((uint32_t)ea[j] >> EBIN_LOG_VB) == bin);
+        if (led)
+            continue;
+        uint32_t n = 1;
+#pragma unroll
+        for (int j = k + 1; j < K; ++j)
+            n += (ok[j] && ((uint32_t)ea[j] >> EBIN_LOG_VB) == bin) ? 1u : 0u;
+        uint32_t h = (bin * 2654435761u) >> (32 - EBIN_LOG_HASH);
+        while (true)
+        {
+            const uint32_t prev = atomicCAS(&key[h], 0xffffffffu, bin);
+            if (prev == 0xffffffffu || prev == bin)
+                break;
+            h = (h + 1u) & (uint32_t)(EBIN_HASH - 1);
+        }
+        const uint32_t r0 = atomicAdd(&cnt[h], n);
+        slot[k] = h;
+        rank[k] = r0;
+        uint32_t i = 1;
+#pragma unroll
+        for (int j = k + 1; j < K; ++j)
+            if (ok[j] && ((uint32_t)ea[j] >> EBIN_LOG_VB) == bin)
+            {
+                slot[j] = h;
+                rank[j] = r0 + i;
+                ++i;
+            }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < EBIN_HASH; i += BT)
+    {
+        const uint32_t n = cnt[i];
+        if (!n)
+            continue;
+        const uint32_t bin = key[i];
+        if constexpr (!PLACE)
+            atomicAdd(bin_cnt + bin, n);
+        else
+        {
+            // each block lowers the bin's count by its share: its range ends
+            // where the count stood (the count pass saw the same half-edges)
+            const uint32_t old = atomicSub(bin_cnt + bin, n);
+            cnt[i] = old >= n ? bin_off[bin] + (old - n) : 0xffffffffu;
+        }
+    }
+    if constexpr (PLACE)
+    {
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+        {
+            if (!ok[k])
+                continue;
+            const uint32_t b0 = cnt[slot[k]];
+            if (b0 != 0xffffffffu)
+                he[(uint64_t)b0 + rank[k]] = make_uint2((uint32_t)ea[k], (uint32_t)eb[k]);
+        }
     }
 }
 
@@ -1701,55 +1812,88 @@ __device__ __forceinline__ void sort16(uint32_t (&x)[16])
             }
 }
 
-// each vertex's slots sorted in registers: count its singletons (pass 0) or
-// write its boundary edges in (min, max) order (pass 1)
-__global__ __launch_bounds__(BT) void k_edge_singletons_slots(int64_t V, const uint32_t *vcnt, const uint32_t *slots,
-                                                               uint32_t *bcnt, const uint32_t *boff, int2 *bedge_v,
-                                                               int pass, unsigned long long *over)
+// one block per bin: the bin's half-edges bucketed by their smaller vertex
+// in LDS, each vertex's list sorted in registers; the singletons (boundary
+// edges) are stored in vertex order from the bin's first half-edge on, and
+// their count in bin_bcnt.  A vertex with more than EDGE_SLOTS half-edges
+// sets `over` (the conversion then re-runs the stage on per-vertex lists of
+// any length).
+__global__ __launch_bounds__(BT) void k_ebin_fold(const uint32_t *bin_off, uint2 *he, uint32_t *bin_bcnt,
+                                                  unsigned long long *over)
 {
-    const int64_t stride = (int64_t)gridDim.x * BT;
-    for (int64_t v = (int64_t)blockIdx.x * BT + threadIdx.x; v < V; v += stride)
+    __shared__ uint32_t slots[EDGE_SLOTS * EBIN_VB]; // slot r of vertex l at r * EBIN_VB + l
+    __shared__ uint32_t vc[EBIN_VB];
+    __shared__ uint32_t sh[NWAVE];
+    const int tid = threadIdx.x;
+    const uint32_t bin = blockIdx.x;
+    const uint32_t s = bin_off[bin], e = bin_off[bin + 1];
+    const uint32_t v0 = bin << EBIN_LOG_VB;
+    vc[tid] = 0;
+    vc[tid + BT] = 0;
+    __syncthreads();
+    for (uint32_t i = s + tid; i < e; i += BT)
     {
-        const uint32_t c = vcnt[v];
+        const uint2 h = he[i];
+        const uint32_t l = h.x - v0; // < EBIN_VB: the half-edge was binned by h.x
+        if (l < (uint32_t)EBIN_VB)
+        {
+            const uint32_t r = atomicAdd(&vc[l], 1u);
+            if (r < (uint32_t)EDGE_SLOTS)
+                slots[r * EBIN_VB + l] = h.y;
+        }
+    }
+    __syncthreads(); // every half-edge of the bin is read: its range may be overwritten
+    uint32_t base = 0;
+#pragma unroll 1
+    for (int half = 0; half < 2; ++half)
+    {
+        const int l = half * BT + tid;
+        uint32_t c = vc[l];
         if (c > (uint32_t)EDGE_SLOTS)
         {
-            if (pass == 0)
-                atomicOr(over, 1ull);
-            continue;
+            atomicOr(over, 1ull);
+            c = 0;
         }
-        if (c == 0)
-            continue;
         uint32_t x[16];
-        const uint4 *sv = (const uint4 *)(slots + (uint64_t)v * EDGE_SLOTS);
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-        {
-            const uint4 w = sv[q];
-            x[4 * q + 0] = w.x;
-            x[4 * q + 1] = w.y;
-            x[4 * q + 2] = w.z;
-            x[4 * q + 3] = w.w;
-        }
 #pragma unroll
         for (int i = 0; i < 16; ++i)
-            x[i] = (uint32_t)i < c ? x[i] : 0xffffffffu; // unused slots sort last
-        sort16(x);
+            x[i] = (uint32_t)i < c ? slots[i * EBIN_VB + l] : 0xffffffffu; // unused slots sort last
+        if (c > 1)
+            sort16(x);
+        bool single[16];
         uint32_t n = 0;
 #pragma unroll
         for (int i = 0; i < 16; ++i)
         {
-            if ((uint32_t)i >= c)
-                break;
-            const bool single = (i == 0 || x[i] != x[i - 1]) && ((uint32_t)(i + 1) >= c || x[i] != x[i + 1]);
-            if (single)
-            {
-                if (pass == 1)
-                    bedge_v[boff[v] + n] = make_int2((int)v, (int)x[i]);
-                ++n;
-            }
+            single[i] = (uint32_t)i < c && (i == 0 || x[i] != x[i - 1]) && ((uint32_t)(i + 1) >= c || x[i] != x[i + 1]);
+            n += single[i] ? 1u : 0u;
         }
-        if (pass == 0)
-            bcnt[v] = n;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan(n, sh, tot);
+        uint32_t o = s + base + ex;
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            if (single[i])
+                he[o++] = make_uint2(v0 + (uint32_t)l, x[i]);
+        base += tot;
+    }
+    if (tid == 0)
+        bin_bcnt[bin] = base;
+}
+
+// one wave per bin: the bin's singletons to their place in the boundary-edge list
+__global__ __launch_bounds__(BT) void k_ebin_gather(uint64_t n_bins, const uint32_t *bin_off, const uint32_t *bin_boff,
+                                                    const uint2 *he, int2 *bedge_v)
+{
+    const int lane = threadIdx.x & 63;
+    for (uint64_t bin = (uint64_t)blockIdx.x * NWAVE + (threadIdx.x >> 6); bin < n_bins; bin += (uint64_t)gridDim.x * NWAVE)
+    {
+        const uint32_t s = bin_off[bin], o = bin_boff[bin], n = bin_boff[bin + 1] - o;
+        for (uint32_t i = lane; i < n; i += 64)
+        {
+            const uint2 h = he[s + i];
+            bedge_v[o + i] = make_int2((int)h.x, (int)h.y);
+        }
     }
 }
 
@@ -2105,19 +2249,37 @@ __global__ __launch_bounds__(BT) void k_shard_reduce(const unsigned long long *s
     }
 }
 
-// chunks grouped by brick as (first record, count): position = first chunk of the brick + rank
+// chunks grouped by brick as (first record, count): position = first chunk of
+// the brick + rank.  PLACE_SUB blocks walk each shard's chunks (no lookup of
+// a dense index's shard), PLACE_UNROLL chunks in flight per thread.
+constexpr int PLACE_SUB = 16;
+constexpr int PLACE_UNROLL = 4;
 __global__ __launch_bounds__(BT) void k_place_chunks(ShardBuf<Chunk> chunks, const uint32_t *choff, uint2 *gchunks,
                                                      const unsigned long long *over)
 {
-    __shared__ QueueMap qm;
     if (*over)
         return; // dropped chunks: the host sizes the buffers again and re-runs
-    const uint64_t n = queue_prefix(chunks, qm.pre, qm.sh);
-    const uint64_t stride = (uint64_t)gridDim.x * BT;
-    for (uint64_t v = (uint64_t)blockIdx.x * BT + threadIdx.x; v < n; v += stride)
+    const uint32_t sh = blockIdx.x % NSHARD, sub = blockIdx.x / NSHARD;
+    const unsigned long long c = chunks.cnt[(uint64_t)sh * SHARD_STRIDE];
+    const uint64_t n = c <= chunks.per ? (uint64_t)c : 0ull;
+    const Chunk *src = chunks.items + (uint64_t)sh * chunks.per;
+    const uint64_t step = (uint64_t)(gridDim.x / NSHARD) * BT;
+    for (uint64_t i0 = (uint64_t)sub * BT + threadIdx.x; i0 < n; i0 += step * PLACE_UNROLL)
     {
-        const Chunk ch = chunks.items[queue_slot(chunks, qm.pre, v)];
-        gchunks[choff[ch.brick] + ch.rank] = make_uint2(ch.start, ch.count);
+        Chunk ch[PLACE_UNROLL];
+        uint32_t off[PLACE_UNROLL];
+#pragma unroll
+        for (int u = 0; u < PLACE_UNROLL; ++u)
+            if (i0 + u * step < n)
+                ch[u] = src[i0 + u * step];
+#pragma unroll
+        for (int u = 0; u < PLACE_UNROLL; ++u)
+            if (i0 + u * step < n)
+                off[u] = choff[ch[u].brick];
+#pragma unroll
+        for (int u = 0; u < PLACE_UNROLL; ++u)
+            if (i0 + u * step < n)
+                gchunks[off[u] + ch[u].rank] = make_uint2(ch[u].start, ch[u].count);
     }
 }
 
@@ -4104,7 +4266,7 @@ hipError_t launch_shard_reduce(unsigned long long *shard_ctr, unsigned long long
 hipError_t launch_place_chunks(const ShardBuf<Chunk> &chunks, const uint32_t *choff, uint2 *gchunks,
                                const unsigned long long *over, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_place_chunks, dim3(4096), dim3(BT), 0, s, chunks, choff, gchunks,
+    hipLaunchKernelGGL(k_place_chunks, dim3(NSHARD * PLACE_SUB), dim3(BT), 0, s, chunks, choff, gchunks,
                        (const unsigned long long *)over); // loops over the chunks
     return hipGetLastError();
 }
@@ -4190,18 +4352,36 @@ hipError_t launch_edge_count(const MeshDev &m, uint32_t *vcnt, uint32_t *rank, c
     return hipGetLastError();
 }
 
-hipError_t launch_edge_slots(const MeshDev &m, uint32_t *vcnt, uint32_t *slots, hipStream_t s)
+hipError_t launch_ebin(const MeshDev &m, bool place, uint32_t *bin_cnt, const uint32_t *bin_off, uint2 *he,
+                       hipStream_t s)
 {
-    hipLaunchKernelGGL(k_edge_slots, dim3(grid_for(m.F, 65536)), dim3(BT), 0, s, m, vcnt, slots);
+    if (m.F <= 0)
+        return hipSuccess;
+    const int64_t per = (int64_t)BT * EBIN_FPT;
+    const unsigned grid = (unsigned)((m.F + per - 1) / per);
+    if (place)
+        hipLaunchKernelGGL(k_ebin<true>, dim3(grid), dim3(BT), 0, s, m, bin_cnt, bin_off, he);
+    else
+        hipLaunchKernelGGL(k_ebin<false>, dim3(grid), dim3(BT), 0, s, m, bin_cnt, bin_off, he);
     return hipGetLastError();
 }
 
-hipError_t launch_edge_singletons_slots(int64_t V, const uint32_t *vcnt, const uint32_t *slots, uint32_t *bcnt,
-                                        const uint32_t *boff, int2 *bedge_v, int pass, unsigned long long *over,
-                                        hipStream_t s)
+hipError_t launch_ebin_fold(uint64_t n_bins, const uint32_t *bin_off, uint2 *he, uint32_t *bin_bcnt,
+                            unsigned long long *over, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_edge_singletons_slots, dim3(grid_for(V, 65536)), dim3(BT), 0, s, V, vcnt, slots, bcnt, boff,
-                       bedge_v, pass, over);
+    if (n_bins == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_ebin_fold, dim3((unsigned)n_bins), dim3(BT), 0, s, bin_off, he, bin_bcnt, over);
+    return hipGetLastError();
+}
+
+hipError_t launch_ebin_gather(uint64_t n_bins, const uint32_t *bin_off, const uint32_t *bin_boff, const uint2 *he,
+                              int2 *bedge_v, hipStream_t s)
+{
+    if (n_bins == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_ebin_gather, dim3(grid_for((int64_t)((n_bins + NWAVE - 1) / NWAVE) * BT, 65536)), dim3(BT), 0,
+                       s, n_bins, bin_off, bin_boff, he, bedge_v);
     return hipGetLastError();
 }
 
