@@ -46,6 +46,7 @@ enum
     // sum and largest shard of each sharded buffer (filled by k_shard_reduce)
     CTR_SUM = 16,        // CTR_SUM + SB_*  : total items requested
     CTR_MAX = 32,        // CTR_MAX + SB_*  : largest per-shard request
+    CTR_HUGE_NEXT = 42,  // next entry of the huge-brick list to fold (blocks of k_fold_large take bricks one by one)
     CTR_COUNT = 48
 };
 

@@ -3600,13 +3600,25 @@ template <int P>
 __global__ __launch_bounds__(BT) void k_fold_large(FoldArgs ain)
 {
     __shared__ FoldLds<FOLD_CAP> L;
+    __shared__ uint64_t next_bi;
     const int tid = threadIdx.x;
     if (ain.ctr[CTR_OVER])
         return;
     const uint64_t n_huge = *ain.n_huge;
+    if (blockIdx.x >= n_huge)
+        return; // no brick would be left for this block
 
-    for (uint64_t bi = blockIdx.x; bi < n_huge; bi += gridDim.x)
+    // bricks are taken one at a time from a shared counter (their sizes vary
+    // widely: a fixed share per block leaves a long tail)
+    for (;;)
     {
+        __syncthreads(); // every thread has read the last brick's index
+        if (tid == 0)
+            next_bi = (uint64_t)atomicAdd(ain.ctr + CTR_HUGE_NEXT, 1ull);
+        __syncthreads();
+        const uint64_t bi = next_bi;
+        if (bi >= n_huge)
+            break;
         const uint32_t brick = ain.huge[bi];
         FoldArgs a = ain;
         const uint32_t lb = enter_brick(a, brick);
@@ -4335,7 +4347,8 @@ hipError_t launch_solve_box(const FoldArgs &a, hipStream_t s)
 
 hipError_t launch_fold_large(const FoldArgs &a, uint64_t max_bricks, hipStream_t s)
 {
-    uint64_t nb = max_bricks < 2048ull ? max_bricks : 2048ull;
+    // the resident blocks (2 per CU) take the bricks one by one
+    uint64_t nb = max_bricks < 512ull ? max_bricks : 512ull;
     if (nb == 0)
         nb = 1;
     if (a.prec)
