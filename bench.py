@@ -101,8 +101,9 @@ def algorithmic_bytes(slot, st, F, V, n_words, precision="reference"):
         "mark": 12 * F + 12 * V + 8 * n_words + 16 * ev,
         # bitset read, word offsets written
         "word_scan": 8 * n_words + 4 * n_words,
-        # half-edges bucketed by min vertex: faces read, 3F neighbour entries written + read back
-        "edges": 12 * F + 24 * F + 16 * V,
+        # half-edges binned by smaller vertex: faces read by the count and place passes, 3F 8 B half-edges
+        # written and read back, 8 B per boundary edge written
+        "edges": 24 * F + 48 * F + 8 * E,
         # bitset + offsets read, coords written
         "coords": 12 * n_words + 12 * N,
         # crossings read (16 B), one 16 B record per (crossing, brick) written
