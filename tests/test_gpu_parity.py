@@ -327,3 +327,39 @@ def test_full_extent_axis(oracle, gmin):
     gr = [gmin, 0, 0, gmin + 8192, 8, 8]
     rms, mx, exact = compare(oracle, v, f, vs, gr, label=f"strip over 8192 voxels, gmin {gmin}")
     assert exact == 1.0
+
+
+def _fan_mesh(k_closed, seed):
+    """Fans whose centre vertex is the smaller vertex of all its spokes: a
+    closed fan of k triangles gives it 2k half-edges, an open one of k
+    triangles 2k as well (two of them boundary edges).  Centres sit on both
+    sides of the 512-vertex bins' borders, among unreferenced vertices."""
+    rng = np.random.default_rng(seed)
+    V = 1500
+    pts = rng.uniform(0.05, 0.95, size=(V, 3))
+    faces = []
+    for n, c in enumerate((0, 500, 511, 1000, 1023, 1400)):
+        closed = n % 2 == 0
+        k = k_closed if (closed and n == 2) else 8
+        ring = list(range(c + 1, c + 1 + (k if closed else k + 1)))
+        centre = rng.uniform(0.2, 0.8, size=3)
+        q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        ang = np.linspace(0.0, 2.0 * np.pi, len(ring), endpoint=False) if closed else np.linspace(0.0, 1.6 * np.pi, len(ring))
+        r = rng.uniform(0.05, 0.12)
+        local = np.stack([r * np.cos(ang), r * np.sin(ang), 0.3 * r * np.sin(3.0 * ang)], 1)
+        pts[c] = centre
+        pts[ring] = centre + local @ q.T
+        for i in range(len(ring) if closed else len(ring) - 1):
+            faces.append([c, ring[i], ring[(i + 1) % len(ring)]])
+    return pts.astype(np.float32), np.array(faces, np.int32)
+
+
+@pytest.mark.parametrize("k_closed", [8, 9])
+def test_boundary_edges_at_bin_borders_and_slot_limit(oracle, k_closed):
+    """A centre vertex with 16 half-edges fits the per-vertex slots of the
+    binned boundary-edge stage; with 18 the conversion falls back to per-vertex
+    lists of any length.  Both equal the oracle."""
+    v, f = _fan_mesh(k_closed, seed=31 + k_closed)
+    vs = [1.0 / 64] * 3
+    gr = [0, 0, 0, 64, 64, 64]
+    compare(oracle, v, f, vs, gr, label=f"fans, {2 * k_closed} half-edges at one vertex")
