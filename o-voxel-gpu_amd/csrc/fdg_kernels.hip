@@ -2252,8 +2252,8 @@ __global__ __launch_bounds__(BT) void k_shard_reduce(const unsigned long long *s
 // chunks grouped by brick as (first record, count): position = first chunk of
 // the brick + rank.  PLACE_SUB blocks walk each shard's chunks (no lookup of
 // a dense index's shard), PLACE_UNROLL chunks in flight per thread.
-constexpr int PLACE_SUB = 16;
-constexpr int PLACE_UNROLL = 4;
+constexpr int PLACE_SUB = 32;
+constexpr int PLACE_UNROLL = 8;
 __global__ __launch_bounds__(BT) void k_place_chunks(ShardBuf<Chunk> chunks, const uint32_t *choff, uint2 *gchunks,
                                                      const unsigned long long *over)
 {
